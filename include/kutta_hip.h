@@ -1,0 +1,194 @@
+/* kutta_hip.h — C ABI of the MI355X (gfx950) HIP backend for Kutta's SIMPLEC
+ * pressure-correction path.
+ *
+ * This is the drop-in seam behind Kutta's abstract device runtime
+ * (src/kutta/runtime/device.hpp:107-240). A HipDevice : kutta::Device adapter
+ * forwards each virtual to one entry point below; buffers are plain device
+ * pointers and scalars travel by value, so no C++ or HIP type crosses the ABI.
+ *
+ * Conventions:
+ *   - Every function returning int returns KH_OK (0) on success. On failure it
+ *     returns KH_ERROR (or KH_NONFINITE) and kh_last_error() describes it.
+ *   - All work is ordered on one in-order stream per device, in call order
+ *     (device.hpp:7-9). Host-visible results (download, host reductions) are
+ *     complete on return; everything else is asynchronous.
+ */
+#ifndef KUTTA_HIP_H
+#define KUTTA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { KH_OK = 0, KH_ERROR = 1, KH_NONFINITE = 2 };
+
+typedef struct kh_device kh_device;
+
+/* Replaces CudaDevice construction / Device::create(Backend::Cuda)
+ * (runtime/device.cpp:32-49, runtime/cuda/cuda_device.cu:112-131). */
+int kh_create(int ordinal, kh_device** out);
+/* Replaces ~CudaDevice (cuda_device.cu:121-124). */
+void kh_destroy(kh_device* dev);
+/* Device::name() (device.hpp:117). */
+const char* kh_name(const kh_device* dev);
+/* Message for the most recent failure on this thread. */
+const char* kh_last_error(void);
+/* Number of visible devices (Device::enumerate, device.cpp:60-88). */
+int kh_device_count(void);
+
+/* Device::alloc_bytes / BufferHandle lifetime (device.hpp:234; cuda_device.cu:17-29).
+ * A zero-byte allocation returns a unique non-null pointer. */
+int kh_alloc(kh_device* dev, size_t bytes, void** out);
+int kh_free(kh_device* dev, void* ptr);
+/* Device::upload_bytes: returns once host memory is reusable (device.hpp:235). */
+int kh_upload(kh_device* dev, void* dst, const void* src, size_t bytes);
+/* Device::download_bytes: complete on return (device.hpp:236). */
+int kh_download(kh_device* dev, const void* src, void* dst, size_t bytes);
+/* Device::copy_bytes: ordered, asynchronous; src == dst is a no-op (device.hpp:237). */
+int kh_copy(kh_device* dev, void* dst, const void* src, size_t bytes);
+/* Device::fill_bytes with 32-bit words (device.hpp:238; cuda_device.cu:204-213).
+ * Implemented as a kernel, so it may be recorded into a graph. */
+int kh_fill32(kh_device* dev, void* dst, uint32_t word, size_t count);
+/* Device::synchronize (device.hpp:208). */
+int kh_synchronize(kh_device* dev);
+
+/* One kernel argument (KernelArg, device.hpp:73-102): a device buffer or a
+ * scalar of 4 bytes (float/int32/uint32) passed by value. */
+enum { KH_ARG_BUFFER = 0, KH_ARG_SCALAR = 1 };
+typedef struct kh_arg {
+  uint32_t kind;
+  uint32_t size;
+  union {
+    void* ptr;
+    float f32;
+    int32_t i32;
+    uint32_t u32;
+    uint64_t bits;
+  } value;
+} kh_arg;
+
+/* Kernel ids. The first block names the reference kernels on the path; their
+ * parameter order is the reference .kern.h signature order (kernelgen.py:150-160).
+ * The KH_FUSED_* block are MI355X-specific fused kernels with no reference
+ * counterpart; each is bitwise equal to the reference sequence it names. */
+typedef enum kh_kernel {
+  KH_KERNEL_NONE = 0,
+  /* kernels/linalg.kern.h */
+  KH_CSR_SPMV = 1,               /* :5-16   rows,cols,values,x,y */
+  KH_CSR_SPMV_GUARDED,           /* :111-114 rows,cols,values,x,y,status */
+  KH_CSR_RESIDUAL,               /* :47-50  rows,cols,values,x,b,r */
+  KH_CSR_RESIDUAL_GUARDED,       /* :115-118 rows,cols,values,x,b,r,status */
+  KH_KRYLOV_DIAGONAL,            /* :52-70  rows,diag,values,inverse,invalid,use_l1 */
+  KH_KRYLOV_MULTIPLY,            /* :72-75  r,inverse,z */
+  KH_KRYLOV_MULTIPLY_GUARDED,    /* :119-122 r,inverse,z,status */
+  KH_KRYLOV_SMOOTH,              /* :77-80  r,inverse,x,omega */
+  KH_KRYLOV_SHIFT,               /* :82     x,mean */
+  KH_KRYLOV_DIRECTION,           /* :84-86  z,p,beta */
+  KH_KRYLOV_STEP,                /* :88-92  x,r,p,ap,alpha */
+  KH_KRYLOV_BICG_DIRECTION,      /* :94-97  r,v,p,beta,omega */
+  KH_KRYLOV_DIFFERENCE,          /* :99-102 a,b,out,scale */
+  KH_KRYLOV_BICG_STEP,           /* :104-107 x,p,s,alpha,omega */
+  KH_AMG_COARSE_SOLVE,           /* :126-147 rows,cols,values,x,b,inverse,status,count,sweeps */
+  KH_AMG_SMOOTH,                 /* :150-156 rows,cols,values,x,b,inverse,next,status */
+  KH_AMG_SMOOTH_ZERO,            /* :162-170 rows,cols,values,x,b,inverse,next,status */
+  /* kernels/amg.kern.h */
+  KH_AMG_RESTRICT,               /* :5-16   offsets,members,fine,coarse */
+  KH_AMG_PROLONG,                /* :18-21  aggregate,coarse,fine */
+  KH_AMG_GALERKIN,               /* :27-38  offsets,entries,fine,coarse */
+  KH_AMG_M_MATRIX_DIAGONAL,      /* :44-58  offsets,members,fine_rows,fine_diag,fine,rows,diag,coarse */
+  KH_AMG_RESTRICT_GUARDED,       /* :84-87  offsets,members,fine,coarse,status */
+  KH_AMG_PROLONG_GUARDED,        /* :88-91  aggregate,coarse,fine,status */
+  /* kernels/blas.kern.h */
+  KH_AXPY,                       /* :5  y,x,a */
+  KH_XPBY,                       /* :8  y,x,b */
+  KH_AXPBYZ,                     /* :11 z,x,y,a,b */
+  KH_SCALE_COPY,                 /* :16 y,x,a */
+  KH_MUL,                        /* :19 z,x,y */
+  /* kernels/scalars.kern.h */
+  KH_SOLVER_LIMITS,              /* :51-55  scalars,tol_hi,tol_lo,div_hi,div_lo */
+  KH_SOLVER_RESET,               /* :56-62  scalars,status,iterations */
+  KH_SOLVER_PCG_ALPHA,           /* :63-71  scalars,status */
+  KH_SOLVER_PCG_BETA,            /* :72-80  scalars,status,flexible */
+  KH_SOLVER_BICG_BETA,           /* :81-89  scalars,status */
+  KH_SOLVER_BICG_ALPHA,          /* :90-97  scalars,status */
+  KH_SOLVER_BICG_S,              /* :98-103 scalars,status */
+  KH_SOLVER_BICG_OMEGA,          /* :104-112 scalars,status */
+  KH_SOLVER_CHECK,               /* :113-121 scalars,status,pcg */
+  KH_SOLVER_STEP,                /* :122-127 x,r,p,q,scalars,status */
+  KH_SOLVER_DIRECTION,           /* :128-131 z,p,scalars,status */
+  KH_SOLVER_BICG_DIRECTION,      /* :132-137 r,v,p,scalars,status */
+  KH_SOLVER_BICG_DIFFERENCE,     /* :138-141 r,q,s,scalars,status */
+  KH_SOLVER_BICG_STEP,           /* :142-149 x,r,z,zs,s,t,scalars,status */
+  KH_SOLVER_PROJECT,             /* :150-152 x,sum,count_hi,count_lo */
+
+  /* Fused MI355X kernels (no reference counterpart). */
+  KH_FUSED_SPMV_DOT_ALPHA = 200, /* csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha:
+                                    rows,cols,values,p,q,scalars,status */
+  KH_FUSED_STEP_DOT_CHECK,       /* solver_step + dot_into(r,r,18) + solver_check(pcg=1):
+                                    x,r,p,q,scalars,status */
+  KH_FUSED_RESIDUAL_RESTRICT,    /* csr_residual_guarded + amg_restrict_guarded (fine r unused):
+                                    rows,cols,values,x,b,offsets,members,coarse_b,status */
+  KH_FUSED_PROLONG_SMOOTH,       /* amg_prolong_guarded + amg_smooth (x + P c, then sweep):
+                                    rows,cols,values,x,b,inverse,next,aggregate,coarse,status */
+  KH_FUSED_SMOOTH_DOT_BETA,      /* amg_smooth + dot_into(r_outer,next,4) + solver_pcg_beta(0):
+                                    rows,cols,values,x,b,inverse,next,status,scalars */
+  KH_FUSED_SMOOTH_SUM,           /* amg_smooth + sum_into(next -> pair at sum[0..1]):
+                                    rows,cols,values,x,b,inverse,next,status,sum */
+  KH_FUSED_PROJECT_DOT_BETA,     /* solver_project(z) + dot_into(r,z,4) + solver_pcg_beta(0):
+                                    z,sum,count_hi,count_lo,r,scalars,status */
+  KH_FUSED_SMOOTH_FROM_ZERO,     /* fill(x,0) + amg_smooth_zero, without storing the zeros:
+                                    rows,cols,values,b,inverse,next,status */
+  KH_KERNEL_COUNT_MARK
+} kh_kernel;
+
+/* Device::launch (device.hpp:205; cuda_device.cu:132-140): runs kernel `kernel`
+ * for global indices [0, n). n == 0 is a no-op; n > INT32_MAX, an unknown id or
+ * arguments of the wrong number or kind are errors. */
+int kh_launch(kh_device* dev, uint32_t kernel, size_t n, const kh_arg* args, size_t nargs);
+/* Name of a kernel id, or NULL. */
+const char* kh_kernel_name(uint32_t kernel);
+
+/* Host reductions: Device::sum/dot/max_abs and host_reductions
+ * (device.hpp:157-184, 227-233). FP64 accumulation in a fixed order; max is
+ * NaN-propagating. All requests share one host wait. */
+enum { KH_REDUCE_SUM = 0, KH_REDUCE_DOT = 1, KH_REDUCE_MAX_ABS = 2 };
+typedef struct kh_reduction {
+  uint32_t kind;
+  uint32_t pad;
+  const float* x;
+  const float* y;
+  size_t n;
+} kh_reduction;
+int kh_reduce_host(kh_device* dev, const kh_reduction* requests, size_t count, double* out);
+
+/* Device::reduce_into (device.hpp:186-202, 211-225): asynchronous FP64 sum,
+ * dot or fused dot pair, stored as normalized (hi, lo) float pairs at
+ * dst[offset..offset+1] (and +2..3 for KH_INTO_DOT2). dst must not alias inputs. */
+enum { KH_INTO_SUM = 0, KH_INTO_DOT = 1, KH_INTO_DOT2 = 2 };
+int kh_reduce_into(kh_device* dev, int mode, const float* x, const float* y, const float* u,
+                   const float* v, size_t n, float* dst, size_t offset);
+
+/* Graph capture of a fixed launch sequence (no reference counterpart: the
+ * reference replays nothing on CUDA). Between begin and end, entry points that
+ * enqueue work are recorded instead of executed; blocking calls are errors. */
+typedef struct kh_graph kh_graph;
+int kh_graph_begin(kh_device* dev);
+int kh_graph_end(kh_device* dev, kh_graph** out);
+int kh_graph_launch(kh_device* dev, kh_graph* graph);
+void kh_graph_destroy(kh_graph* graph);
+
+/* Per-kernel timing with HIP events on the device stream (no reference
+ * counterpart): while enabled, every launch of `kernel` is bracketed by events.
+ * kh_profile_read synchronizes and returns the launch count and summed ms. */
+int kh_profile_enable(kh_device* dev, uint32_t kernel, int enable);
+int kh_profile_read(kh_device* dev, uint32_t kernel, long* launches, double* total_ms);
+int kh_profile_reset(kh_device* dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KUTTA_HIP_H */

@@ -1,0 +1,592 @@
+"""kutta_amd — MI355X (gfx950) HIP backend for Kutta's SIMPLEC pressure path.
+
+Python view of the two native libraries built in-tree by ``kutta_amd.build``:
+
+* ``libkutta_hip.so`` — the device ABI (include/kutta_hip.h): buffers, ordered
+  launches of the hand-written HIP kernels, FP64 reductions, graphs.
+* ``libkutta_amd.so`` — C++ host code (include/kutta_amd.h): aggregation AMG,
+  the device-resident Krylov solvers, mesh and pressure-operator setup.
+
+Names mirror the reference's C++ API (``Device``, ``make_amg_solver``,
+``SolveControl``, ``SolveStats``, ``AmgOptions`` …). There is no fallback: if
+the native libraries are missing or no HIP device is present, construction
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_hip = None
+_host = None
+
+
+class KuttaError(RuntimeError):
+    """A failure reported by the native libraries (kutta::Error)."""
+
+
+class NonfiniteError(KuttaError):
+    """A diverged / non-finite solve (kutta::NonfiniteError)."""
+
+
+def _load(name: str) -> C.CDLL:
+    path = _HERE / name
+    if not path.exists():
+        raise KuttaError(f"{path} is not built; run `python -m kutta_amd.build`")
+    return C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+
+
+def hip_lib() -> C.CDLL:
+    """The device ABI library, with prototypes declared."""
+    global _hip
+    if _hip is None:
+        lib = _load("libkutta_hip.so")
+        vp, i32, sz = C.c_void_p, C.c_int, C.c_size_t
+        lib.kh_last_error.restype = C.c_char_p
+        lib.kh_device_count.restype = i32
+        lib.kh_create.argtypes = [i32, C.POINTER(vp)]
+        lib.kh_destroy.argtypes = [vp]
+        lib.kh_name.argtypes = [vp]
+        lib.kh_name.restype = C.c_char_p
+        lib.kh_alloc.argtypes = [vp, sz, C.POINTER(vp)]
+        lib.kh_free.argtypes = [vp, vp]
+        lib.kh_upload.argtypes = [vp, vp, vp, sz]
+        lib.kh_download.argtypes = [vp, vp, vp, sz]
+        lib.kh_copy.argtypes = [vp, vp, vp, sz]
+        lib.kh_fill32.argtypes = [vp, vp, C.c_uint32, sz]
+        lib.kh_synchronize.argtypes = [vp]
+        lib.kh_launch.argtypes = [vp, C.c_uint32, sz, C.POINTER(KhArg), sz]
+        lib.kh_kernel_name.argtypes = [C.c_uint32]
+        lib.kh_kernel_name.restype = C.c_char_p
+        lib.kh_reduce_host.argtypes = [vp, C.POINTER(KhReduction), sz, C.POINTER(C.c_double)]
+        lib.kh_reduce_into.argtypes = [vp, i32, vp, vp, vp, vp, sz, vp, sz]
+        lib.kh_graph_begin.argtypes = [vp]
+        lib.kh_graph_end.argtypes = [vp, C.POINTER(vp)]
+        lib.kh_graph_launch.argtypes = [vp, vp]
+        lib.kh_graph_destroy.argtypes = [vp]
+        lib.kh_profile_enable.argtypes = [vp, C.c_uint32, i32]
+        lib.kh_profile_read.argtypes = [vp, C.c_uint32, C.POINTER(C.c_long), C.POINTER(C.c_double)]
+        lib.kh_profile_reset.argtypes = [vp]
+        _hip = lib
+    return _hip
+
+
+def host_lib() -> C.CDLL:
+    """The host library (solvers, mesh), with prototypes declared."""
+    global _host
+    if _host is None:
+        hip_lib()
+        lib = _load("libkutta_amd.so")
+        vp, i32, dbl = C.c_void_p, C.c_int, C.c_double
+        pi32 = C.POINTER(C.c_int32)
+        lib.ka_last_error.restype = C.c_char_p
+        lib.ka_device_create.argtypes = [i32, C.POINTER(vp)]
+        lib.ka_device_destroy.argtypes = [vp]
+        lib.ka_device_raw.argtypes = [vp]
+        lib.ka_device_raw.restype = vp
+        lib.ka_csr_create.argtypes = [i32, vp, vp, vp, vp, C.POINTER(vp)]
+        lib.ka_csr_destroy.argtypes = [vp]
+        lib.ka_csr_rows.argtypes = [vp]
+        lib.ka_csr_nnz.argtypes = [vp]
+        lib.ka_csr_copy.argtypes = [vp, vp, vp, vp, vp]
+        lib.ka_matrix_upload.argtypes = [vp, vp, C.POINTER(vp)]
+        lib.ka_matrix_destroy.argtypes = [vp]
+        for f in ("ka_matrix_values", "ka_matrix_row_ptr", "ka_matrix_col", "ka_matrix_diag"):
+            getattr(lib, f).argtypes = [vp]
+            getattr(lib, f).restype = vp
+        lib.ka_solver_create.argtypes = [vp, i32, vp, pi32, dbl, C.POINTER(vp)]
+        lib.ka_solver_destroy.argtypes = [vp]
+        lib.ka_solver_refresh.argtypes = [vp, vp]
+        lib.ka_solver_solve.argtypes = [vp, vp, vp, vp, dbl, dbl, i32, i32, C.POINTER(dbl)]
+        lib.ka_solver_describe.argtypes = [vp, C.c_char_p, i32]
+        lib.ka_solver_levels.argtypes = [vp, pi32, pi32, i32]
+        lib.ka_solver_transfer.argtypes = [vp, i32, i32, pi32]
+        lib.ka_solver_transfer.restype = C.c_long
+        lib.ka_solver_level_values.argtypes = [vp, i32, C.POINTER(C.c_float)]
+        lib.ka_solver_level_values.restype = C.c_long
+        lib.ka_solver_set_fused.argtypes = [vp, i32]
+        lib.ka_solver_host_waits.argtypes = [vp]
+        lib.ka_aggregate.argtypes = [vp, pi32, dbl, C.POINTER(vp)]
+        lib.ka_coarsening_array.argtypes = [vp, i32, pi32]
+        lib.ka_coarsening_array.restype = C.c_long
+        lib.ka_coarsening_values.argtypes = [vp, C.POINTER(C.c_double)]
+        lib.ka_coarsening_values.restype = C.c_long
+        lib.ka_coarsening_destroy.argtypes = [vp]
+        _host = lib
+    return _host
+
+
+def _check_hip(status: int) -> None:
+    if status != 0:
+        raise KuttaError(hip_lib().kh_last_error().decode())
+
+
+def _check_host(status: int) -> None:
+    if status == 2:
+        raise NonfiniteError(host_lib().ka_last_error().decode())
+    if status != 0:
+        raise KuttaError(host_lib().ka_last_error().decode())
+
+
+# ----------------------------------------------------------------- device ABI
+class KhArg(C.Structure):
+    class _V(C.Union):
+        _fields_ = [("ptr", C.c_void_p), ("f32", C.c_float), ("i32", C.c_int32),
+                    ("u32", C.c_uint32), ("bits", C.c_uint64)]
+    _fields_ = [("kind", C.c_uint32), ("size", C.c_uint32), ("value", _V)]
+
+
+class KhReduction(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("x", C.c_void_p),
+                ("y", C.c_void_p), ("n", C.c_size_t)]
+
+
+class Kernel(enum.IntEnum):
+    """Kernel ids of include/kutta_hip.h (reference kernel names)."""
+    csr_spmv = 1
+    csr_spmv_guarded = 2
+    csr_residual = 3
+    csr_residual_guarded = 4
+    krylov_diagonal = 5
+    krylov_multiply = 6
+    krylov_multiply_guarded = 7
+    krylov_smooth = 8
+    krylov_shift = 9
+    krylov_direction = 10
+    krylov_step = 11
+    krylov_bicg_direction = 12
+    krylov_difference = 13
+    krylov_bicg_step = 14
+    amg_coarse_solve = 15
+    amg_smooth = 16
+    amg_smooth_zero = 17
+    amg_restrict = 18
+    amg_prolong = 19
+    amg_galerkin = 20
+    amg_m_matrix_diagonal = 21
+    amg_restrict_guarded = 22
+    amg_prolong_guarded = 23
+    axpy = 24
+    xpby = 25
+    axpbyz = 26
+    scale_copy = 27
+    mul = 28
+    solver_limits = 29
+    solver_reset = 30
+    solver_pcg_alpha = 31
+    solver_pcg_beta = 32
+    solver_bicg_beta = 33
+    solver_bicg_alpha = 34
+    solver_bicg_s = 35
+    solver_bicg_omega = 36
+    solver_check = 37
+    solver_step = 38
+    solver_direction = 39
+    solver_bicg_direction = 40
+    solver_bicg_difference = 41
+    solver_bicg_step = 42
+    solver_project = 43
+    fused_spmv_dot_alpha = 200
+    fused_step_dot_check = 201
+    fused_residual_restrict = 202
+    fused_prolong_smooth = 203
+    fused_smooth_dot_beta = 204
+    fused_smooth_sum = 205
+    fused_project_dot_beta = 206
+    fused_smooth_from_zero = 207
+
+
+class Buffer:
+    """A device array (kutta::Buffer<T>): float32, int32 or uint32 (or other
+    fixed-size dtypes for backend-private storage)."""
+
+    def __init__(self, device: "Device", n: int, dtype=np.float32):
+        self.device = device
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        ptr = C.c_void_p()
+        _check_hip(hip_lib().kh_alloc(device.kh, self.n * self.dtype.itemsize, C.byref(ptr)))
+        self.ptr = ptr.value
+
+    @property
+    def nbytes(self) -> int:
+        return self.n * self.dtype.itemsize
+
+    def upload(self, host) -> "Buffer":
+        a = np.ascontiguousarray(host, dtype=self.dtype)
+        if a.size != self.n:
+            raise KuttaError("upload size mismatch")
+        _check_hip(hip_lib().kh_upload(self.device.kh, self.ptr, a.ctypes.data, self.nbytes))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        _check_hip(hip_lib().kh_download(self.device.kh, self.ptr, out.ctypes.data, self.nbytes))
+        return out
+
+    def fill(self, value) -> None:
+        word = np.array([value], dtype=self.dtype).view(np.uint32)[0] if self.dtype.itemsize == 4 else None
+        if word is None:
+            raise KuttaError("fill supports 32-bit element types only")
+        _check_hip(hip_lib().kh_fill32(self.device.kh, self.ptr, int(word), self.n))
+
+    def free(self) -> None:
+        if self.ptr:
+            hip_lib().kh_free(self.device.kh, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.device._alive:
+                self.free()
+        except Exception:
+            pass
+
+
+def _arg(a) -> KhArg:
+    k = KhArg()
+    if isinstance(a, Buffer):
+        k.kind, k.size, k.value.ptr = 0, 0, a.ptr
+    elif isinstance(a, (float, np.floating)):
+        k.kind, k.size, k.value.f32 = 1, 4, float(a)
+    elif isinstance(a, (int, np.integer)):
+        k.kind, k.size, k.value.i32 = 1, 4, int(a)
+    else:
+        raise TypeError(f"unsupported kernel argument {a!r}")
+    return k
+
+
+class Device:
+    """One MI355X device and its in-order stream (kutta::Device)."""
+
+    def __init__(self, ordinal: int = 0):
+        lib = host_lib()
+        h = C.c_void_p()
+        _check_host(lib.ka_device_create(ordinal, C.byref(h)))
+        self.handle = h.value
+        self.kh = lib.ka_device_raw(self.handle)
+        self._alive = True
+
+    @staticmethod
+    def count() -> int:
+        return hip_lib().kh_device_count()
+
+    @property
+    def name(self) -> str:
+        return hip_lib().kh_name(self.kh).decode()
+
+    def close(self) -> None:
+        if self._alive:
+            self._alive = False
+            host_lib().ka_device_destroy(self.handle)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # buffers
+    def alloc(self, n: int, dtype=np.float32) -> Buffer:
+        return Buffer(self, n, dtype)
+
+    def upload(self, host, dtype=None) -> Buffer:
+        a = np.asarray(host) if dtype is None else np.asarray(host, dtype=dtype)
+        return Buffer(self, a.size, a.dtype).upload(a)
+
+    def copy(self, dst: Buffer, src: Buffer) -> None:
+        if dst.nbytes != src.nbytes:
+            raise KuttaError("copy size mismatch")
+        _check_hip(hip_lib().kh_copy(self.kh, dst.ptr, src.ptr, src.nbytes))
+
+    def synchronize(self) -> None:
+        _check_hip(hip_lib().kh_synchronize(self.kh))
+
+    def launch(self, kernel, n: int, *args) -> None:
+        arr = (KhArg * max(len(args), 1))(*[_arg(a) for a in args])
+        _check_hip(hip_lib().kh_launch(self.kh, int(kernel), int(n), arr, len(args)))
+
+    # reductions (device.hpp:157-202)
+    def reduce_many(self, requests) -> list[float]:
+        reqs = (KhReduction * max(len(requests), 1))()
+        for i, (kind, x, y, n) in enumerate(requests):
+            reqs[i].kind = {"sum": 0, "dot": 1, "max_abs": 2}[kind]
+            reqs[i].x = x.ptr
+            reqs[i].y = (y or x).ptr
+            reqs[i].n = n
+        out = (C.c_double * max(len(requests), 1))()
+        _check_hip(hip_lib().kh_reduce_host(self.kh, reqs, len(requests), out))
+        return [out[i] for i in range(len(requests))]
+
+    def sum(self, x: Buffer, n: int | None = None) -> float:
+        return self.reduce_many([("sum", x, None, x.n if n is None else n)])[0]
+
+    def dot(self, x: Buffer, y: Buffer, n: int | None = None) -> float:
+        return self.reduce_many([("dot", x, y, x.n if n is None else n)])[0]
+
+    def max_abs(self, x: Buffer, n: int | None = None) -> float:
+        return self.reduce_many([("max_abs", x, None, x.n if n is None else n)])[0]
+
+    def reduce_into(self, mode: str, x, y, u, v, n, dst: Buffer, offset=0) -> None:
+        m = {"sum": 0, "dot": 1, "dot2": 2}[mode]
+        _check_hip(hip_lib().kh_reduce_into(self.kh, m, x.ptr, (y or x).ptr, (u or x).ptr,
+                                            (v or x).ptr, n, dst.ptr, offset))
+
+    def sum_into(self, x, n, dst, offset=0):
+        self.reduce_into("sum", x, x, x, x, n, dst, offset)
+
+    def dot_into(self, x, y, n, dst, offset=0):
+        self.reduce_into("dot", x, y, x, y, n, dst, offset)
+
+    def dot2_into(self, x, y, u, v, n, dst, offset=0):
+        self.reduce_into("dot2", x, y, u, v, n, dst, offset)
+
+    # profiling
+    def profile(self, kernel, enable: bool = True) -> None:
+        _check_hip(hip_lib().kh_profile_enable(self.kh, int(kernel), 1 if enable else 0))
+
+    def profile_read(self, kernel) -> tuple[int, float]:
+        n, ms = C.c_long(), C.c_double()
+        _check_hip(hip_lib().kh_profile_read(self.kh, int(kernel), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def profile_reset(self) -> None:
+        _check_hip(hip_lib().kh_profile_reset(self.kh))
+
+
+# ----------------------------------------------------------------- linalg API
+class SolverKind(enum.IntEnum):
+    """linalg/solver.hpp:237-243"""
+    Jacobi = 0
+    PcgJacobi = 1
+    BicgstabJacobi = 2
+    AmgPcg = 3
+    AmgBicgstab = 4
+
+
+class NullSpace(enum.IntEnum):
+    NONE = 0
+    Constant = 1
+
+
+class OperatorStructure(enum.IntEnum):
+    General = 0
+    SymmetricMMatrix = 1
+
+
+@dataclass
+class AmgOptions:
+    """linalg/amg.hpp:8-20 (V-cycle)."""
+    pre_sweeps: int = 5
+    post_sweeps: int = 5
+    coarse_sweeps: int = 32
+    coarse_rows: int = 24
+    max_levels: int = 24
+    aggregation_passes: int = 3
+    strength_threshold: float = 0.25
+    nullspace: NullSpace = NullSpace.NONE
+    structure: OperatorStructure = OperatorStructure.General
+
+    def packed(self):
+        return (C.c_int32 * 8)(self.pre_sweeps, self.post_sweeps, self.coarse_sweeps,
+                               self.coarse_rows, self.max_levels, self.aggregation_passes,
+                               int(self.nullspace), int(self.structure))
+
+
+@dataclass
+class SolveControl:
+    """linalg/solver.hpp:223-228"""
+    rel_tol: float = 1e-2
+    abs_tol: float = 1e-12
+    max_iter: int = 200
+    residual_check_interval: int = 8
+
+
+@dataclass
+class SolveStats:
+    """linalg/solver.hpp:230-235"""
+    iterations: int = 0
+    initial_residual: float = 0.0
+    final_residual: float = 0.0
+    converged: bool = False
+
+
+class CsrMatrixHost:
+    """Host CSR (linalg/csr.hpp:16-28)."""
+
+    def __init__(self, row_ptr, col, diag, val):
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.diag = np.ascontiguousarray(diag, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n_rows = len(self.row_ptr) - 1
+        h = C.c_void_p()
+        _check_host(host_lib().ka_csr_create(self.n_rows, self.row_ptr.ctypes.data,
+                                             self.col.ctypes.data, self.diag.ctypes.data,
+                                             self.val.ctypes.data, C.byref(h)))
+        self.handle = h.value
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1])
+
+    def __del__(self):
+        try:
+            host_lib().ka_csr_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class CsrMatrix:
+    """Device CSR (linalg/csr.hpp:31-38); `val` changes between solves."""
+
+    def __init__(self, device: Device, host: CsrMatrixHost):
+        h = C.c_void_p()
+        _check_host(host_lib().ka_matrix_upload(device.handle, host.handle, C.byref(h)))
+        self.device = device
+        self.handle = h.value
+        self.n_rows = host.n_rows
+        self.nnz = host.nnz
+
+    def _view(self, fn, n, dtype):
+        b = Buffer.__new__(Buffer)
+        b.device, b.dtype, b.n = self.device, np.dtype(dtype), n
+        b.ptr = getattr(host_lib(), fn)(self.handle)
+        b.free = lambda: None  # owned by the matrix
+        return b
+
+    @property
+    def val(self) -> Buffer:
+        return self._view("ka_matrix_values", self.nnz, np.float32)
+
+    @property
+    def row_ptr(self) -> Buffer:
+        return self._view("ka_matrix_row_ptr", self.n_rows + 1, np.int32)
+
+    @property
+    def col(self) -> Buffer:
+        return self._view("ka_matrix_col", self.nnz, np.int32)
+
+    @property
+    def diag(self) -> Buffer:
+        return self._view("ka_matrix_diag", self.n_rows, np.int32)
+
+    def __del__(self):
+        try:
+            if self.device._alive:
+                host_lib().ka_matrix_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def upload(device: Device, host: CsrMatrixHost) -> CsrMatrix:
+    """linalg/csr.hpp:40-41"""
+    return CsrMatrix(device, host)
+
+
+class LinearSolver:
+    """A device-resident Krylov solver (linalg/solver.hpp:279-286)."""
+
+    def __init__(self, device: Device, kind: SolverKind, a: CsrMatrixHost, options: AmgOptions):
+        h = C.c_void_p()
+        _check_host(host_lib().ka_solver_create(device.handle, int(kind), a.handle, options.packed(),
+                                                options.strength_threshold, C.byref(h)))
+        self.device = device
+        self.handle = h.value
+        self.n_rows = a.n_rows
+
+    def refresh(self, a: CsrMatrix) -> None:
+        _check_host(host_lib().ka_solver_refresh(self.handle, a.handle))
+
+    def solve(self, a: CsrMatrix, x: Buffer, b: Buffer, control: SolveControl | None = None) -> SolveStats:
+        c = control or SolveControl()
+        stats = (C.c_double * 4)()
+        _check_host(host_lib().ka_solver_solve(self.handle, a.handle, x.ptr, b.ptr, c.rel_tol, c.abs_tol,
+                                               c.max_iter, c.residual_check_interval, stats))
+        return SolveStats(int(stats[0]), stats[1], stats[2], bool(stats[3]))
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(512)
+        host_lib().ka_solver_describe(self.handle, buf, 512)
+        return buf.value.decode()
+
+    def levels(self) -> list[tuple[int, int]]:
+        rows, nnz = (C.c_int32 * 64)(), (C.c_int32 * 64)()
+        k = host_lib().ka_solver_levels(self.handle, rows, nnz, 64)
+        return [(rows[i], nnz[i]) for i in range(k)]
+
+    def transfer(self, level: int, which: str) -> np.ndarray | None:
+        w = ["aggregate", "member_offsets", "members", "entry_offsets", "entries"].index(which)
+        n = host_lib().ka_solver_transfer(self.handle, level, w, None)
+        if n < 0:
+            return None
+        out = np.empty(n, dtype=np.int32)
+        host_lib().ka_solver_transfer(self.handle, level, w, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def level_values(self, level: int) -> np.ndarray | None:
+        n = host_lib().ka_solver_level_values(self.handle, level, None)
+        if n < 0:
+            return None
+        out = np.empty(n, dtype=np.float32)
+        host_lib().ka_solver_level_values(self.handle, level, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def set_fused(self, fused: bool) -> None:
+        _check_host(host_lib().ka_solver_set_fused(self.handle, 1 if fused else 0))
+
+    @property
+    def host_waits(self) -> int:
+        return host_lib().ka_solver_host_waits(self.handle)
+
+    def __del__(self):
+        try:
+            if self.device._alive:
+                host_lib().ka_solver_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def aggregate(a: CsrMatrixHost, options: AmgOptions | None = None) -> dict:
+    """One level of host aggregation (linalg/amg.cpp:160-174): aggregate map,
+    member lists, Galerkin entry maps and the coarse CSR operator."""
+    o = options or AmgOptions()
+    h = C.c_void_p()
+    _check_host(host_lib().ka_aggregate(a.handle, o.packed(), o.strength_threshold, C.byref(h)))
+    lib = host_lib()
+    out = {}
+    try:
+        names = ["aggregate", "member_offsets", "members", "entry_offsets", "entries",
+                 "row_ptr", "col", "diag"]
+        for k, name in enumerate(names):
+            n = lib.ka_coarsening_array(h, k, None)
+            arr = np.empty(n, np.int32)
+            lib.ka_coarsening_array(h, k, arr.ctypes.data_as(C.POINTER(C.c_int32)))
+            out[name] = arr
+        n = lib.ka_coarsening_values(h, None)
+        val = np.empty(n, np.float64)
+        lib.ka_coarsening_values(h, val.ctypes.data_as(C.POINTER(C.c_double)))
+        out["val"] = val
+    finally:
+        lib.ka_coarsening_destroy(h)
+    return out
+
+
+def make_amg_solver(kind: SolverKind, device: Device, a: CsrMatrixHost,
+                    options: AmgOptions | None = None) -> LinearSolver:
+    """linalg/amg.hpp:22-27 (only AmgPcg / AmgBicgstab)."""
+    if kind not in (SolverKind.AmgPcg, SolverKind.AmgBicgstab):
+        raise KuttaError("AMG: expected an AMG solver kind")
+    return LinearSolver(device, kind, a, options or AmgOptions())
+
+
+def make_solver(kind: SolverKind, device: Device, a: CsrMatrixHost,
+                nullspace: NullSpace = NullSpace.NONE,
+                structure: OperatorStructure = OperatorStructure.General) -> LinearSolver:
+    """linalg/solver.hpp:290-293; AMG kinds use default AmgOptions."""
+    return LinearSolver(device, kind, a, AmgOptions(nullspace=nullspace, structure=structure))

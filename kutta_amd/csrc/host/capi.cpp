@@ -1,0 +1,197 @@
+// C ABI of the host library (include/kutta_amd.h).
+#include <cstring>
+#include <string>
+
+#include "kutta_amd.h"
+#include "krylov.hpp"
+
+using namespace kutta_amd;
+
+namespace {
+thread_local std::string g_error;
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const NonfiniteError& e) {
+    g_error = e.what();
+    return 2;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+struct Matrix {
+  HipDevice* dev;
+  OwnedCsr csr;
+};
+struct Solver {
+  HipDevice* dev;
+  KrylovSolver solver;
+};
+}  // namespace
+
+extern "C" {
+
+const char* ka_last_error(void) { return g_error.c_str(); }
+
+int ka_device_create(int ordinal, void** out) {
+  return guard([&] { *out = new HipDevice(ordinal); });
+}
+void ka_device_destroy(void* dev) { delete static_cast<HipDevice*>(dev); }
+kh_device* ka_device_raw(void* dev) { return static_cast<HipDevice*>(dev)->raw(); }
+
+int ka_csr_create(int n_rows, const int32_t* row_ptr, const int32_t* col, const int32_t* diag,
+                  const double* val, void** out) {
+  return guard([&] {
+    KA_REQUIRE(n_rows >= 0, "CSR: negative row count");
+    auto* a = new CsrHost;
+    a->n_rows = n_rows;
+    a->row_ptr.assign(row_ptr, row_ptr + n_rows + 1);
+    const int32_t nnz = row_ptr[n_rows];
+    a->col.assign(col, col + nnz);
+    a->diag.assign(diag, diag + n_rows);
+    a->val.assign(val, val + nnz);
+    *out = a;
+  });
+}
+void ka_csr_destroy(void* csr) { delete static_cast<CsrHost*>(csr); }
+int ka_csr_rows(void* csr) { return static_cast<CsrHost*>(csr)->n_rows; }
+int ka_csr_nnz(void* csr) { return static_cast<CsrHost*>(csr)->nnz(); }
+int ka_csr_copy(void* csr, int32_t* row_ptr, int32_t* col, int32_t* diag, double* val) {
+  return guard([&] {
+    const auto* a = static_cast<CsrHost*>(csr);
+    std::memcpy(row_ptr, a->row_ptr.data(), a->row_ptr.size() * sizeof(int32_t));
+    std::memcpy(col, a->col.data(), a->col.size() * sizeof(int32_t));
+    std::memcpy(diag, a->diag.data(), a->diag.size() * sizeof(int32_t));
+    std::memcpy(val, a->val.data(), a->val.size() * sizeof(double));
+  });
+}
+
+int ka_aggregate(void* csr, const int32_t* options, double threshold, void** out) {
+  return guard([&] {
+    AmgOptions o;
+    o.aggregation_passes = options[5];
+    o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
+    o.strength_threshold = threshold;
+    const auto& a = *static_cast<CsrHost*>(csr);
+    validate_host(a);
+    *out = new Coarsening(aggregate(a, o));
+  });
+}
+long ka_coarsening_array(void* c, int which, int32_t* out) {
+  const auto* g = static_cast<Coarsening*>(c);
+  const std::vector<Index>* v = nullptr;
+  switch (which) {
+    case 0: v = &g->aggregate; break;
+    case 1: v = &g->member_offsets; break;
+    case 2: v = &g->members; break;
+    case 3: v = &g->entry_offsets; break;
+    case 4: v = &g->entries; break;
+    case 5: v = &g->matrix.row_ptr; break;
+    case 6: v = &g->matrix.col; break;
+    case 7: v = &g->matrix.diag; break;
+    default: return -1;
+  }
+  if (out) std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
+  return long(v->size());
+}
+long ka_coarsening_values(void* c, double* out) {
+  const auto* g = static_cast<Coarsening*>(c);
+  if (out) std::memcpy(out, g->matrix.val.data(), g->matrix.val.size() * sizeof(double));
+  return long(g->matrix.val.size());
+}
+void ka_coarsening_destroy(void* c) { delete static_cast<Coarsening*>(c); }
+
+int ka_matrix_upload(void* dev, void* csr, void** out) {
+  return guard([&] {
+    auto* d = static_cast<HipDevice*>(dev);
+    const auto& a = *static_cast<CsrHost*>(csr);
+    validate_host(a);
+    *out = new Matrix{d, OwnedCsr(*d, a)};
+  });
+}
+void ka_matrix_destroy(void* m) { delete static_cast<Matrix*>(m); }
+float* ka_matrix_values(void* m) { return static_cast<Matrix*>(m)->csr.val.data(); }
+const int32_t* ka_matrix_row_ptr(void* m) { return static_cast<Matrix*>(m)->csr.row_ptr.data(); }
+const int32_t* ka_matrix_col(void* m) { return static_cast<Matrix*>(m)->csr.col.data(); }
+const int32_t* ka_matrix_diag(void* m) { return static_cast<Matrix*>(m)->csr.diag.data(); }
+
+int ka_solver_create(void* dev, int kind, void* csr, const int32_t* options, double threshold,
+                     void** out) {
+  return guard([&] {
+    KA_REQUIRE(kind >= 0 && kind <= 4, "solver: unknown SolverKind");
+    AmgOptions o;
+    o.pre_sweeps = options[0];
+    o.post_sweeps = options[1];
+    o.coarse_sweeps = options[2];
+    o.coarse_rows = options[3];
+    o.max_levels = options[4];
+    o.aggregation_passes = options[5];
+    o.nullspace = options[6] ? NullSpace::Constant : NullSpace::None;
+    o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
+    o.strength_threshold = threshold;
+    auto* d = static_cast<HipDevice*>(dev);
+    *out = new Solver{d, KrylovSolver(*d, static_cast<SolverKind>(kind), *static_cast<CsrHost*>(csr), o)};
+  });
+}
+void ka_solver_destroy(void* s) { delete static_cast<Solver*>(s); }
+int ka_solver_refresh(void* s, void* m) {
+  return guard([&] { static_cast<Solver*>(s)->solver.refresh(static_cast<Matrix*>(m)->csr.view()); });
+}
+int ka_solver_solve(void* s, void* m, float* x, const float* b, double rel_tol, double abs_tol,
+                    int max_iter, int interval, double* stats) {
+  return guard([&] {
+    SolveControl c{rel_tol, abs_tol, max_iter, interval};
+    const auto st = static_cast<Solver*>(s)->solver.solve(static_cast<Matrix*>(m)->csr.view(), x, b, c);
+    stats[0] = st.iterations;
+    stats[1] = st.initial_residual;
+    stats[2] = st.final_residual;
+    stats[3] = st.converged ? 1 : 0;
+  });
+}
+int ka_solver_describe(void* s, char* out, int capacity) {
+  const auto text = static_cast<Solver*>(s)->solver.describe();
+  if (capacity > 0) {
+    std::strncpy(out, text.c_str(), std::size_t(capacity));
+    out[capacity - 1] = 0;
+  }
+  return int(text.size());
+}
+int ka_solver_levels(void* s, int32_t* rows, int32_t* nnz, int capacity) {
+  const auto levels = static_cast<Solver*>(s)->solver.levels();
+  for (std::size_t i = 0; i < levels.size() && int(i) < capacity; ++i) {
+    rows[i] = levels[i].first;
+    nnz[i] = levels[i].second;
+  }
+  return int(levels.size());
+}
+long ka_solver_transfer(void* s, int level, int which, int32_t* out) {
+  try {
+    const auto v = static_cast<Solver*>(s)->solver.transfer(std::size_t(level), which);
+    if (v.empty()) return -1;
+    if (out) std::memcpy(out, v.data(), v.size() * sizeof(int32_t));
+    return long(v.size());
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+long ka_solver_level_values(void* s, int level, float* out) {
+  try {
+    const auto v = static_cast<Solver*>(s)->solver.level_values(std::size_t(level));
+    if (v.empty()) return -1;
+    if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
+    return long(v.size());
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+int ka_solver_set_fused(void* s, int fused) {
+  return guard([&] { static_cast<Solver*>(s)->solver.set_fused(fused != 0); });
+}
+int ka_solver_host_waits(void* s) { return static_cast<Solver*>(s)->solver.host_waits(); }
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+// Device-side arithmetic shared by the kernels. Everything is compiled with
+// -ffp-contract=off and without fast-math: a*b+c is a rounded product then a
+// rounded sum, division and sqrt are correctly rounded, exactly as the serial
+// reference evaluates them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kh_internal.hpp"
+
+namespace kh {
+
+// ------------------------------------------------------------ upward rounding
+// Upper bound of a float sum (kernels/prelude.h:82-100): TwoSum, then step up
+// one ulp only when round-to-nearest went down.
+__device__ __forceinline__ float add_up(float a, float b) {
+  float s = a + b;
+  float part = s - a;
+  float err = (a - (s - part)) + (b - part);
+  return err > 0.0f ? nextafterf(s, __builtin_inff()) : s;
+}
+struct UpperSum {
+  float sum, error;
+};
+__device__ __forceinline__ UpperSum upper_add(UpperSum a, float b) {
+  float s = a.sum + b;
+  float part = s - a.sum;
+  float err = (a.sum - (s - part)) + (b - part);
+  return {s, add_up(a.error, err)};
+}
+__device__ __forceinline__ float upper_value(UpperSum a) { return add_up(a.sum, a.error); }
+
+// ------------------------------------------------- compensated row operations
+// Dekker split product error (kernels/linalg.kern.h:21-30).
+__device__ __forceinline__ float product_error(float a, float b, float product) {
+  if (fabsf(a) > 1.0e30f || fabsf(b) > 1.0e30f) return 0.0f;
+  float ca = 4097.0f * a, cb = 4097.0f * b;
+  float ah = ca - (ca - a), bh = cb - (cb - b);
+  float al = a - ah, bl = b - bh;
+  return ((ah * bh - product) + ah * bl + al * bh) + al * bl;
+}
+
+// Neumaier accumulator for b - sum a*x with product errors
+// (kernels/linalg.kern.h:32-45). Callers feed entries in row order.
+struct RowResidual {
+  float s, e;
+  __device__ __forceinline__ explicit RowResidual(float b) : s(b), e(0.0f) {}
+  __device__ __forceinline__ void add(float a, float x) {
+    float prod = a * x;
+    float t = -prod;
+    float nx = s + t;
+    e += fabsf(s) >= fabsf(t) ? (s - nx) + t : (t - nx) + s;
+    e -= product_error(a, x, prod);
+    s = nx;
+  }
+  __device__ __forceinline__ float value() const { return s + e; }
+};
+
+// Kahan accumulator (kernels/linalg.kern.h:5-16, kernels/amg.kern.h:5-16).
+struct Kahan {
+  float s = 0.0f, e = 0.0f;
+  __device__ __forceinline__ void add(float term_in) {
+    float t = term_in - e;
+    float nx = s + t;
+    e = (nx - s) - t;
+    s = nx;
+  }
+};
+
+// Neumaier accumulator (kernels/amg.kern.h:27-38).
+struct Neumaier {
+  float s = 0.0f, e = 0.0f;
+  __device__ __forceinline__ void add(float t) {
+    float nx = s + t;
+    e += fabsf(s) >= fabsf(t) ? (s - nx) + t : (t - nx) + s;
+    s = nx;
+  }
+  __device__ __forceinline__ float value() const { return s + e; }
+};
+
+// ----------------------------------------------------------- scalar pairs
+// Normalized (hi, lo) float pairs (kernels/scalars.kern.h:3-42).
+struct Pair {
+  float hi, lo;
+};
+__device__ __forceinline__ Pair pload(const float* p, int off) { return {p[off], p[off + 1]}; }
+__device__ __forceinline__ void pstore(float* p, int off, Pair v) {
+  p[off] = v.hi;
+  p[off + 1] = v.lo;
+}
+__device__ __forceinline__ Pair padd(Pair a, Pair b) {
+  float s = a.hi + b.hi;
+  if (s - s != 0.0f) return {s, 0.0f};
+  float v = s - a.hi;
+  float e = (a.hi - (s - v)) + (b.hi - v);
+  e = e + a.lo + b.lo;
+  float h = s + e;
+  return {h, e - (h - s)};
+}
+__device__ __forceinline__ Pair pneg(Pair a) { return {-a.hi, -a.lo}; }
+__device__ __forceinline__ Pair pmul(Pair a, Pair b) {
+  float h = a.hi * b.hi;
+  float l = __builtin_fmaf(a.hi, b.hi, -h) + a.hi * b.lo + a.lo * b.hi;
+  return padd({h, 0.0f}, {l, 0.0f});
+}
+__device__ __forceinline__ Pair pdiv(Pair a, Pair b) {
+  float q = a.hi / b.hi;
+  Pair r = padd(a, pneg(pmul({q, 0.0f}, b)));
+  return padd({q, 0.0f}, {(r.hi + r.lo) / b.hi, 0.0f});
+}
+__device__ __forceinline__ float pfloat(Pair a) { return a.hi + a.lo; }
+__device__ __forceinline__ bool ppositive(Pair a) {
+  return a.hi > 0.0f || (a.hi == 0.0f && a.lo > 0.0f);
+}
+__device__ __forceinline__ bool pfinite(Pair a) {
+  return a.hi - a.hi == 0.0f && a.lo - a.lo == 0.0f;
+}
+__device__ __forceinline__ bool pzero(Pair a) { return a.hi == 0.0f && a.lo == 0.0f; }
+__device__ __forceinline__ bool ple(Pair a, Pair b) {
+  return a.hi < b.hi || (a.hi == b.hi && a.lo <= b.lo);
+}
+__device__ __forceinline__ bool pcoefficient(Pair a) {
+  float v = pfloat(a);
+  return pfinite(a) && v - v == 0.0f && (v != 0.0f || pzero(a));
+}
+// FP64 value to a normalized pair (runtime/cpu/cpu_device.cpp:83-86).
+__device__ __forceinline__ Pair to_pair(double value) {
+  float hi = static_cast<float>(value);
+  float lo = (hi - hi == 0.0f) ? static_cast<float>(value - double(hi)) : 0.0f;
+  return {hi, lo};
+}
+
+// ------------------------------------------------------- scalar state machine
+// Single-lane bodies of kernels/scalars.kern.h:63-121, shared by the plain
+// scalar kernels and by the fused kernels' last workgroup.
+__device__ __forceinline__ void pcg_alpha(float* s, int* status) {
+  if (status[0] != 0) return;
+  Pair rho = pload(s, 0), pq = pload(s, 2);
+  Pair alpha = pdiv(rho, pq);
+  if (!ppositive(rho) || !ppositive(pq) || !pcoefficient(alpha) || pzero(alpha)) status[0] = 2;
+  else pstore(s, 8, alpha);
+}
+__device__ __forceinline__ void pcg_beta(float* s, int* status, int flexible) {
+  if (status[0] != 0) return;
+  Pair next = pload(s, 4);
+  Pair numer = flexible != 0 ? pneg(pmul(pload(s, 8), pload(s, 6))) : next;
+  Pair beta = pdiv(numer, pload(s, 0));
+  if (!ppositive(next) || !pcoefficient(beta) || (pzero(beta) && !pzero(numer))) status[0] = 2;
+  else {
+    pstore(s, 10, beta);
+    pstore(s, 0, next);
+  }
+}
+__device__ __forceinline__ void solver_check_body(float* s, int* status, int pcg) {
+  if (status[0] != 0) return;
+  if (pcg != 0) ++status[1];
+  Pair rr = pload(s, 18);
+  if (!pfinite(rr) || ple(rr, pload(s, 20)) || !ple(rr, pload(s, 22)) || status[1] % 50 == 0)
+    status[0] = 1;
+  if (pcg == 0) pstore(s, 14, pload(s, 0));
+}
+
+// ---------------------------------------------------------- block reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// Deterministic sum over the workgroup; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds /* kBlock/64 */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double total = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) total += lds[w];
+  __syncthreads();
+  return total;
+}
+
+// Publishes this workgroup's N partial sums; returns true in the workgroup
+// that arrives last, with the grand totals (fixed summation order) in `total`
+// on thread 0. That workgroup also rearms the ticket for the next kernel.
+template <int N>
+__device__ __forceinline__ bool finish_grid(const double (&mine)[N], double* partials,
+                                            unsigned* ticket, double (&total)[N]) {
+  __shared__ unsigned is_last;
+  __shared__ double lds[N][kBlock / 64];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) partials[blockIdx.x * N + k] = mine[k];
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    is_last = (t == gridDim.x - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  const unsigned nb = gridDim.x;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double v = 0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
+      v += __builtin_nontemporal_load(&partials[b * N + k]);
+    total[k] = block_sum(v, lds[k]);
+  }
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+  return true;
+}
+
+// Maps workgroups so that each of the 8 XCDs (which receive workgroups
+// round-robin) processes one contiguous range of the index space, keeping
+// neighbour gathers inside one XCD's L2.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+}  // namespace kh

@@ -1,0 +1,776 @@
+// Linear-algebra kernels of the pressure solve for gfx950: CSR products and
+// compensated residuals, l1-Jacobi smoothing, aggregation transfers, Galerkin
+// refresh, Krylov vector updates and the device-scalar state machine, plus
+// fused MI355X variants that remove launches and HBM round trips.
+//
+// Per-element arithmetic follows the reference kernel bodies operation for
+// operation (cited per kernel), so results are bitwise equal to the serial
+// reference wherever no FP64 reduction order is involved.
+#include "kh_device_math.hpp"
+
+namespace kh {
+namespace {
+
+// Global index of this thread for an n-index launch, XCD-contiguous.
+__device__ __forceinline__ int row_index() { return int(xcd_block() * kBlock + threadIdx.x); }
+
+// ------------------------------------------------------------ CSR products
+// kernels/linalg.kern.h:5-16
+__device__ __forceinline__ float spmv_row(int i, const int* __restrict__ rows,
+                                          const int* __restrict__ cols,
+                                          const float* __restrict__ values,
+                                          const float* __restrict__ x) {
+  Kahan k;
+  const int end = rows[i + 1];
+  for (int j = rows[i]; j < end; ++j) k.add(values[j] * x[cols[j]]);
+  return k.s;
+}
+// kernels/linalg.kern.h:32-45
+__device__ __forceinline__ float residual_row(int i, const int* __restrict__ rows,
+                                              const int* __restrict__ cols,
+                                              const float* __restrict__ values,
+                                              const float* __restrict__ x,
+                                              const float* __restrict__ b) {
+  RowResidual acc(b[i]);
+  const int end = rows[i + 1];
+  for (int j = rows[i]; j < end; ++j) acc.add(values[j], x[cols[j]]);
+  return acc.value();
+}
+// The same residual evaluated at x + P c without storing x + P c: each gathered
+// x[j] is prolonged on the fly, the float sum the reference's in-place prolong
+// would have stored (kernels/amg.kern.h:18-21).
+__device__ __forceinline__ float prolonged(const float* __restrict__ x,
+                                           const int* __restrict__ aggregate,
+                                           const float* __restrict__ coarse, int j) {
+  const int g = aggregate[j];
+  return g >= 0 ? x[j] + coarse[g] : x[j];
+}
+
+__global__ void __launch_bounds__(kBlock) k_csr_spmv(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ x, float* __restrict__ y, const int* __restrict__ status) {
+  if (status && status[0] != 0) return;
+  const int i = row_index();
+  if (i < n) y[i] = spmv_row(i, rows, cols, values, x);
+}
+__global__ void __launch_bounds__(kBlock) k_csr_residual(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
+    const int* __restrict__ status) {
+  if (status && status[0] != 0) return;
+  const int i = row_index();
+  if (i < n) r[i] = residual_row(i, rows, cols, values, x, b);
+}
+// kernels/linalg.kern.h:52-70
+__global__ void __launch_bounds__(kBlock) k_krylov_diagonal(int n, const int* __restrict__ rows,
+    const int* __restrict__ diag, const float* __restrict__ values, float* __restrict__ inverse,
+    float* __restrict__ invalid, int use_l1) {
+  const int i = row_index();
+  if (i >= n) return;
+  const int di = diag[i];
+  float d = values[di];
+  float off = 0.0f, bad = 0.0f;
+  for (int j = rows[i]; j < rows[i + 1]; ++j) {
+    float v = values[j];
+    if (v - v != 0.0f) bad = 1.0f;
+    if (j != di) off += fabsf(v);
+  }
+  float inv = d != 0.0f ? 1.0f / d : 0.0f;
+  float l1 = fmaxf(fabsf(d), 0.5f * (fabsf(d) + off));
+  inverse[i] = use_l1 != 0 ? (l1 != 0.0f ? 1.0f / l1 : 0.0f) : inv;
+  invalid[i] = (bad != 0.0f || inv - inv != 0.0f || l1 - l1 != 0.0f)
+                   ? 2.0f
+                   : (use_l1 != 0 && d < 0.0f ? 3.0f : (d == 0.0f ? 1.0f : 0.0f));
+}
+// kernels/linalg.kern.h:72-107 — elementwise Krylov updates.
+__global__ void k_krylov_multiply(int n, const float* __restrict__ r,
+    const float* __restrict__ inverse, float* __restrict__ z, const int* __restrict__ status) {
+  if (status && status[0] != 0) return;
+  const int i = row_index();
+  if (i < n) z[i] = r[i] * inverse[i];
+}
+__global__ void k_krylov_smooth(int n, const float* __restrict__ r, const float* __restrict__ inv,
+                                float* __restrict__ x, float omega) {
+  const int i = row_index();
+  if (i < n) x[i] += omega * inv[i] * r[i];
+}
+__global__ void k_krylov_shift(int n, float* __restrict__ x, float mean) {
+  const int i = row_index();
+  if (i < n) x[i] -= mean;
+}
+__global__ void k_krylov_direction(int n, const float* __restrict__ z, float* __restrict__ p,
+                                   float beta) {
+  const int i = row_index();
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+__global__ void k_krylov_step(int n, float* __restrict__ x, float* __restrict__ r,
+    const float* __restrict__ p, const float* __restrict__ ap, float alpha) {
+  const int i = row_index();
+  if (i < n) {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * ap[i];
+  }
+}
+__global__ void k_krylov_bicg_direction(int n, const float* __restrict__ r,
+    const float* __restrict__ v, float* __restrict__ p, float beta, float omega) {
+  const int i = row_index();
+  if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+__global__ void k_krylov_difference(int n, const float* __restrict__ a, const float* __restrict__ b,
+                                    float* __restrict__ out, float scale) {
+  const int i = row_index();
+  if (i < n) out[i] = a[i] - scale * b[i];
+}
+__global__ void k_krylov_bicg_step(int n, float* __restrict__ x, const float* __restrict__ p,
+    const float* __restrict__ s, float alpha, float omega) {
+  const int i = row_index();
+  if (i < n) x[i] += alpha * p[i] + omega * s[i];
+}
+
+// ------------------------------------------------------------- AMG kernels
+// kernels/linalg.kern.h:126-147. The reference runs all sweeps in one thread;
+// here one lane owns each row and the lanes of one wave exchange the sweep's
+// iterate through LDS. Every row's arithmetic is the reference's, and all
+// rows read the previous sweep's iterate, so the result is identical.
+constexpr int kCoarseMax = 24;
+__global__ void __launch_bounds__(64) k_amg_coarse_solve(const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, float* __restrict__ x,
+    const float* __restrict__ b, const float* __restrict__ inverse,
+    const int* __restrict__ status, int count, int sweeps) {
+  if (status[0] != 0) return;
+  __shared__ float cur[kCoarseMax];
+  const int row = threadIdx.x;
+  if (row < count) cur[row] = 0.0f;
+  __syncthreads();
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    float r = 0.0f;
+    if (row < count) {
+      RowResidual acc(b[row]);
+      for (int j = rows[row]; j < rows[row + 1]; ++j) acc.add(values[j], cur[cols[j]]);
+      r = acc.value();
+    }
+    __syncthreads();
+    if (row < count) cur[row] += (2.0f / 3.0f) * inverse[row] * r;
+    __syncthreads();
+  }
+  if (row < count) x[row] = cur[row];
+}
+// kernels/linalg.kern.h:150-156
+__global__ void __launch_bounds__(kBlock) k_amg_smooth(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float r = residual_row(i, rows, cols, values, x, b);
+  next[i] = x[i] + (2.0f / 3.0f) * inverse[i] * r;
+}
+// kernels/linalg.kern.h:162-170
+__global__ void __launch_bounds__(kBlock) k_amg_smooth_zero(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float bi = b[i];
+  if (bi - bi != 0.0f) {
+    const float r = residual_row(i, rows, cols, values, x, b);
+    next[i] = x[i] + (2.0f / 3.0f) * inverse[i] * r;
+    return;
+  }
+  next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * (bi + 0.0f);
+}
+// kernels/amg.kern.h:5-16
+__global__ void k_amg_restrict(int n, const int* __restrict__ offsets,
+    const int* __restrict__ members, const float* __restrict__ fine, float* __restrict__ coarse,
+    const int* __restrict__ status) {
+  if (status && status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  Kahan k;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) k.add(fine[members[j]]);
+  coarse[i] = k.s;
+}
+// kernels/amg.kern.h:18-21
+__global__ void k_amg_prolong(int n, const int* __restrict__ aggregate,
+    const float* __restrict__ coarse, float* __restrict__ fine, const int* __restrict__ status) {
+  if (status && status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const int g = aggregate[i];
+  if (g >= 0) fine[i] += coarse[g];
+}
+// kernels/amg.kern.h:27-38
+__global__ void k_amg_galerkin(int n, const int* __restrict__ offsets,
+    const int* __restrict__ entries, const float* __restrict__ fine, float* __restrict__ coarse) {
+  const int i = row_index();
+  if (i >= n) return;
+  Neumaier acc;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) acc.add(fine[entries[j]]);
+  coarse[i] = acc.value();
+}
+// kernels/amg.kern.h:44-58
+__global__ void k_amg_m_matrix_diagonal(int n, const int* __restrict__ offsets,
+    const int* __restrict__ members, const int* __restrict__ fine_rows,
+    const int* __restrict__ fine_diag, const float* __restrict__ fine,
+    const int* __restrict__ rows, const int* __restrict__ diag, float* coarse) {
+  const int i = row_index();
+  if (i >= n) return;
+  UpperSum d{0.0f, 0.0f};
+  for (int m = offsets[i]; m < offsets[i + 1]; ++m) {
+    const int row = members[m];
+    const int fd = fine_diag[row];
+    UpperSum surplus{fine[fd], 0.0f};
+    for (int j = fine_rows[row]; j < fine_rows[row + 1]; ++j)
+      if (j != fd) surplus = upper_add(surplus, fine[j]);
+    d = upper_add(d, upper_value(surplus));
+  }
+  const int di = diag[i];
+  for (int j = rows[i]; j < rows[i + 1]; ++j)
+    if (j != di) d = upper_add(d, -coarse[j]);
+  coarse[di] = upper_value(d);
+}
+
+// ------------------------------------------------------------------- BLAS-1
+// kernels/blas.kern.h:5-19
+__global__ void k_axpy(int n, float* __restrict__ y, const float* __restrict__ x, float a) {
+  const int i = row_index();
+  if (i < n) y[i] = y[i] + a * x[i];
+}
+__global__ void k_xpby(int n, float* __restrict__ y, const float* __restrict__ x, float b) {
+  const int i = row_index();
+  if (i < n) y[i] = x[i] + b * y[i];
+}
+__global__ void k_axpbyz(int n, float* __restrict__ z, const float* __restrict__ x,
+                         const float* __restrict__ y, float a, float b) {
+  const int i = row_index();
+  if (i < n) z[i] = a * x[i] + b * y[i];
+}
+__global__ void k_scale_copy(int n, float* __restrict__ y, const float* __restrict__ x, float a) {
+  const int i = row_index();
+  if (i < n) y[i] = a * x[i];
+}
+__global__ void k_mul(int n, float* __restrict__ z, const float* __restrict__ x,
+                      const float* __restrict__ y) {
+  const int i = row_index();
+  if (i < n) z[i] = x[i] * y[i];
+}
+
+// ------------------------------------------------------- scalar kernels
+// kernels/scalars.kern.h:51-55
+__global__ void k_solver_limits(int n, float* s, float th, float tl, float dh, float dl) {
+  const int i = row_index();
+  if (i < n) s[i] = i == 20 ? th : (i == 21 ? tl : (i == 22 ? dh : (i == 23 ? dl : 0.0f)));
+}
+// kernels/scalars.kern.h:56-62
+__global__ void k_solver_reset(float* s, int* status, int iterations) {
+  if (threadIdx.x != 0) return;
+  status[0] = 0; status[1] = iterations; status[2] = 0;
+  s[8] = 1.0f; s[9] = 0.0f; s[12] = 1.0f; s[13] = 0.0f; s[14] = 1.0f; s[15] = 0.0f;
+}
+__global__ void k_solver_pcg_alpha(float* s, int* status) {
+  if (threadIdx.x == 0) pcg_alpha(s, status);
+}
+__global__ void k_solver_pcg_beta(float* s, int* status, int flexible) {
+  if (threadIdx.x == 0) pcg_beta(s, status, flexible);
+}
+// kernels/scalars.kern.h:81-89
+__global__ void k_solver_bicg_beta(float* s, int* status) {
+  if (threadIdx.x != 0 || status[0] != 0) return;
+  Pair rho = pload(s, 0), omega = pload(s, 12);
+  Pair beta = pmul(pdiv(rho, pload(s, 14)), pdiv(pload(s, 8), omega));
+  if (pzero(rho) || !pfinite(rho) || pzero(omega) || !pcoefficient(omega) ||
+      !pcoefficient(beta) || pzero(beta))
+    status[0] = 2;
+  else {
+    pstore(s, 10, beta);
+    status[2] = 0;
+  }
+}
+// kernels/scalars.kern.h:90-97
+__global__ void k_solver_bicg_alpha(float* s, int* status) {
+  if (threadIdx.x != 0 || status[0] != 0) return;
+  Pair rv = pload(s, 2);
+  Pair alpha = pdiv(pload(s, 0), rv);
+  if (pzero(rv) || !pfinite(rv) || !pcoefficient(alpha) || pzero(alpha)) status[0] = 2;
+  else pstore(s, 8, alpha);
+}
+// kernels/scalars.kern.h:98-103
+__global__ void k_solver_bicg_s(const float* s, int* status) {
+  if (threadIdx.x != 0 || status[0] != 0) return;
+  ++status[1];
+  status[2] = ple(pload(s, 16), pload(s, 20)) ? 1 : 0;
+}
+// kernels/scalars.kern.h:104-112
+__global__ void k_solver_bicg_omega(float* s, int* status) {
+  if (threadIdx.x != 0 || status[0] != 0) return;
+  if (status[2] != 0) {
+    pstore(s, 12, {0.0f, 0.0f});
+    return;
+  }
+  Pair tt = pload(s, 2), ts = pload(s, 4);
+  Pair omega = pdiv(ts, tt);
+  if (!ppositive(tt) || !pfinite(tt) || pzero(omega) || !pcoefficient(omega)) status[0] = 2;
+  else pstore(s, 12, omega);
+}
+__global__ void k_solver_check(float* s, int* status, int pcg) {
+  if (threadIdx.x == 0) solver_check_body(s, status, pcg);
+}
+// kernels/scalars.kern.h:122-127
+__global__ void k_solver_step(int n, float* __restrict__ x, float* __restrict__ r,
+    const float* __restrict__ p, const float* __restrict__ q, const float* __restrict__ s,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float alpha = pfloat(pload(s, 8));
+  x[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+}
+// kernels/scalars.kern.h:128-131
+__global__ void k_solver_direction(int n, const float* __restrict__ z, float* __restrict__ p,
+    const float* __restrict__ s, const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i < n) p[i] = z[i] + pfloat(pload(s, 10)) * p[i];
+}
+// kernels/scalars.kern.h:132-137
+__global__ void k_solver_bicg_direction(int n, const float* __restrict__ r,
+    const float* __restrict__ v, float* __restrict__ p, const float* __restrict__ s,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float beta = pfloat(pload(s, 10)), omega = pfloat(pload(s, 12));
+  p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+// kernels/scalars.kern.h:138-141
+__global__ void k_solver_bicg_difference(int n, const float* __restrict__ r,
+    const float* __restrict__ q, float* __restrict__ sv, const float* __restrict__ s,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i < n) sv[i] = r[i] - pfloat(pload(s, 8)) * q[i];
+}
+// kernels/scalars.kern.h:142-149
+__global__ void k_solver_bicg_step(int n, float* __restrict__ x, float* __restrict__ r,
+    const float* __restrict__ z, const float* __restrict__ zs, const float* __restrict__ sv,
+    const float* __restrict__ t, const float* __restrict__ s, const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float alpha = pfloat(pload(s, 8)), omega = pfloat(pload(s, 12));
+  if (status[2] != 0) {
+    x[i] += alpha * z[i];
+    r[i] = sv[i];
+  } else {
+    x[i] += alpha * z[i] + omega * zs[i];
+    r[i] = sv[i] - omega * t[i];
+  }
+}
+// kernels/scalars.kern.h:150-152
+__global__ void k_solver_project(int n, float* __restrict__ x, const float* __restrict__ sum,
+                                 float count_hi, float count_lo) {
+  const int i = row_index();
+  if (i < n) x[i] -= pfloat(pdiv(pload(sum, 0), {count_hi, count_lo}));
+}
+
+// ------------------------------------------------------------ fused kernels
+// q = A p (guarded spmv), then p.q reduced in FP64 and the alpha update, all in
+// one launch: csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha.
+__global__ void __launch_bounds__(kBlock) k_fused_spmv_dot_alpha(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ p, float* __restrict__ q, float* scalars, int* status,
+    double* partials, unsigned* ticket) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float qi = spmv_row(i, rows, cols, values, p);
+    q[i] = qi;
+    mine[0] = double(p[i]) * double(qi);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  double total[1];
+  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
+    pstore(scalars, 2, to_pair(total[0]));
+    pcg_alpha(scalars, status);
+  }
+}
+// x += alpha p, r -= alpha q, then r.r and the convergence check:
+// solver_step + dot_into(r,r,18) + solver_check(pcg=1).
+__global__ void __launch_bounds__(kBlock) k_fused_step_dot_check(int n, float* __restrict__ x,
+    float* __restrict__ r, const float* __restrict__ p, const float* __restrict__ q,
+    float* scalars, int* status, double* partials, unsigned* ticket) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  const float alpha = pfloat(pload(scalars, 8));
+  double mine[1] = {0.0};
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const float ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    mine[0] = double(ri) * double(ri);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  double total[1];
+  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
+    pstore(scalars, 18, to_pair(total[0]));
+    solver_check_body(scalars, status, 1);
+  }
+}
+// Restricted residual: each coarse row sums the compensated residuals of its
+// members, computed in place of the stored fine residual:
+// csr_residual_guarded + amg_restrict_guarded.
+__global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ x, const float* __restrict__ b, const int* __restrict__ offsets,
+    const int* __restrict__ members, float* __restrict__ coarse, const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int g = row_index();
+  if (g >= n) return;
+  Kahan k;
+  for (int m = offsets[g]; m < offsets[g + 1]; ++m)
+    k.add(residual_row(members[m], rows, cols, values, x, b));
+  coarse[g] = k.s;
+}
+// Prolongation followed by one smoothing sweep without storing x + P c:
+// amg_prolong_guarded + amg_smooth.
+__global__ void __launch_bounds__(kBlock) k_fused_prolong_smooth(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ x, const float* __restrict__ b, const float* __restrict__ inverse,
+    float* __restrict__ next, const int* __restrict__ aggregate, const float* __restrict__ coarse,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  RowResidual acc(b[i]);
+  for (int j = rows[i]; j < rows[i + 1]; ++j)
+    acc.add(values[j], prolonged(x, aggregate, coarse, cols[j]));
+  next[i] = prolonged(x, aggregate, coarse, i) + (2.0f / 3.0f) * inverse[i] * acc.value();
+}
+// Final smoothing sweep of the preconditioner, then r.z and the beta update:
+// amg_smooth + dot_into(r,z,4) + solver_pcg_beta(flexible=0). Here b is the
+// Krylov residual r that the V-cycle preconditions.
+__global__ void __launch_bounds__(kBlock) k_fused_smooth_dot_beta(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ x, const float* __restrict__ b, const float* __restrict__ inverse,
+    float* __restrict__ next, int* status, float* scalars, double* partials, unsigned* ticket) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float r = residual_row(i, rows, cols, values, x, b);
+    const float z = x[i] + (2.0f / 3.0f) * inverse[i] * r;
+    next[i] = z;
+    mine[0] = double(b[i]) * double(z);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  double total[1];
+  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
+    pstore(scalars, 4, to_pair(total[0]));
+    pcg_beta(scalars, status, 0);
+  }
+}
+// Smoothing sweep whose output is summed for the constant-nullspace
+// projection that follows: amg_smooth + sum_into(next -> sum[0..1]).
+__global__ void __launch_bounds__(kBlock) k_fused_smooth_sum(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+    const int* status, float* sum, double* partials, unsigned* ticket) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float r = residual_row(i, rows, cols, values, x, b);
+    const float z = x[i] + (2.0f / 3.0f) * inverse[i] * r;
+    next[i] = z;
+    mine[0] = double(z);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  double total[1];
+  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0)
+    pstore(sum, 0, to_pair(total[0]));
+}
+// Mean removal, r.z and beta: solver_project + dot_into(r,z,4) + solver_pcg_beta(0).
+__global__ void __launch_bounds__(kBlock) k_fused_project_dot_beta(int n, float* __restrict__ z,
+    const float* __restrict__ sum, float count_hi, float count_lo, const float* __restrict__ r,
+    float* scalars, int* status, double* partials, unsigned* ticket) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float zi = z[i] - pfloat(pdiv(pload(sum, 0), {count_hi, count_lo}));
+    z[i] = zi;
+    mine[0] = double(r[i]) * double(zi);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  double total[1];
+  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
+    pstore(scalars, 4, to_pair(total[0]));
+    pcg_beta(scalars, status, 0);
+  }
+}
+
+// fill(x, 0) followed by amg_smooth_zero, reading the zero iterate as the
+// constant +0.0f instead of from memory (kernels/linalg.kern.h:162-170).
+__global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float bi = b[i];
+  if (bi - bi != 0.0f) {
+    RowResidual acc(bi);
+    for (int j = rows[i]; j < rows[i + 1]; ++j) acc.add(values[j], 0.0f);
+    next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * acc.value();
+    return;
+  }
+  next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * (bi + 0.0f);
+}
+
+// ----------------------------------------------------------------- launchers
+#define KB(k) static_cast<const int*>(buf(a, k))
+#define KF(k) static_cast<float*>(buf(a, k))
+#define KCF(k) static_cast<const float*>(buf(a, k))
+#define GRID(n) dim3(blocks_for(n)), dim3(kBlock), 0, d->stream
+
+void l_csr_spmv(kh_device* d, int n, const kh_arg* a) {
+  k_csr_spmv<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), nullptr);
+}
+void l_csr_spmv_guarded(kh_device* d, int n, const kh_arg* a) {
+  k_csr_spmv<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KB(5));
+}
+void l_csr_residual(kh_device* d, int n, const kh_arg* a) {
+  k_csr_residual<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), nullptr);
+}
+void l_csr_residual_guarded(kh_device* d, int n, const kh_arg* a) {
+  k_csr_residual<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6));
+}
+void l_krylov_diagonal(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_diagonal<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KF(3), KF(4), i32(a, 5));
+}
+void l_krylov_multiply(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_multiply<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), nullptr);
+}
+void l_krylov_multiply_guarded(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_multiply<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), KB(3));
+}
+void l_krylov_smooth(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_smooth<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), f32(a, 3));
+}
+void l_krylov_shift(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_shift<<<GRID(n)>>>(n, KF(0), f32(a, 1));
+}
+void l_krylov_direction(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_direction<<<GRID(n)>>>(n, KCF(0), KF(1), f32(a, 2));
+}
+void l_krylov_step(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_step<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), f32(a, 4));
+}
+void l_krylov_bicg_direction(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_bicg_direction<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), f32(a, 3), f32(a, 4));
+}
+void l_krylov_difference(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_difference<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), f32(a, 3));
+}
+void l_krylov_bicg_step(kh_device* d, int n, const kh_arg* a) {
+  k_krylov_bicg_step<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2), f32(a, 3), f32(a, 4));
+}
+void l_amg_coarse_solve(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 7);
+  if (count < 0 || count > kCoarseMax) throw std::runtime_error("amg_coarse_solve: count must be in [0, 24]");
+  if (n < 1) return;
+  k_amg_coarse_solve<<<dim3(1), dim3(64), 0, d->stream>>>(KB(0), KB(1), KCF(2), KF(3), KCF(4),
+                                                           KCF(5), KB(6), count, i32(a, 8));
+}
+void l_amg_smooth(kh_device* d, int n, const kh_arg* a) {
+  k_amg_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+}
+void l_amg_smooth_zero(kh_device* d, int n, const kh_arg* a) {
+  k_amg_smooth_zero<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+}
+void l_amg_restrict(kh_device* d, int n, const kh_arg* a) {
+  k_amg_restrict<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KF(3), nullptr);
+}
+void l_amg_restrict_guarded(kh_device* d, int n, const kh_arg* a) {
+  k_amg_restrict<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KF(3), KB(4));
+}
+void l_amg_prolong(kh_device* d, int n, const kh_arg* a) {
+  k_amg_prolong<<<GRID(n)>>>(n, KB(0), KCF(1), KF(2), nullptr);
+}
+void l_amg_prolong_guarded(kh_device* d, int n, const kh_arg* a) {
+  k_amg_prolong<<<GRID(n)>>>(n, KB(0), KCF(1), KF(2), KB(3));
+}
+void l_amg_galerkin(kh_device* d, int n, const kh_arg* a) {
+  k_amg_galerkin<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KF(3));
+}
+void l_amg_m_matrix_diagonal(kh_device* d, int n, const kh_arg* a) {
+  k_amg_m_matrix_diagonal<<<GRID(n)>>>(n, KB(0), KB(1), KB(2), KB(3), KCF(4), KB(5), KB(6), KF(7));
+}
+void l_axpy(kh_device* d, int n, const kh_arg* a) { k_axpy<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2)); }
+void l_xpby(kh_device* d, int n, const kh_arg* a) { k_xpby<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2)); }
+void l_axpbyz(kh_device* d, int n, const kh_arg* a) {
+  k_axpbyz<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2), f32(a, 3), f32(a, 4));
+}
+void l_scale_copy(kh_device* d, int n, const kh_arg* a) {
+  k_scale_copy<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2));
+}
+void l_mul(kh_device* d, int n, const kh_arg* a) { k_mul<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2)); }
+#define SINGLE dim3(1), dim3(64), 0, d->stream
+void l_solver_limits(kh_device* d, int n, const kh_arg* a) {
+  k_solver_limits<<<GRID(n)>>>(n, KF(0), f32(a, 1), f32(a, 2), f32(a, 3), f32(a, 4));
+}
+void l_solver_reset(kh_device* d, int, const kh_arg* a) {
+  k_solver_reset<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)), i32(a, 2));
+}
+void l_solver_pcg_alpha(kh_device* d, int, const kh_arg* a) {
+  k_solver_pcg_alpha<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_solver_pcg_beta(kh_device* d, int, const kh_arg* a) {
+  k_solver_pcg_beta<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)), i32(a, 2));
+}
+void l_solver_bicg_beta(kh_device* d, int, const kh_arg* a) {
+  k_solver_bicg_beta<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_solver_bicg_alpha(kh_device* d, int, const kh_arg* a) {
+  k_solver_bicg_alpha<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_solver_bicg_s(kh_device* d, int, const kh_arg* a) {
+  k_solver_bicg_s<<<SINGLE>>>(KCF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_solver_bicg_omega(kh_device* d, int, const kh_arg* a) {
+  k_solver_bicg_omega<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_solver_check(kh_device* d, int, const kh_arg* a) {
+  k_solver_check<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)), i32(a, 2));
+}
+void l_solver_step(kh_device* d, int n, const kh_arg* a) {
+  k_solver_step<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), KCF(4), KB(5));
+}
+void l_solver_direction(kh_device* d, int n, const kh_arg* a) {
+  k_solver_direction<<<GRID(n)>>>(n, KCF(0), KF(1), KCF(2), KB(3));
+}
+void l_solver_bicg_direction(kh_device* d, int n, const kh_arg* a) {
+  k_solver_bicg_direction<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), KCF(3), KB(4));
+}
+void l_solver_bicg_difference(kh_device* d, int n, const kh_arg* a) {
+  k_solver_bicg_difference<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), KCF(3), KB(4));
+}
+void l_solver_bicg_step(kh_device* d, int n, const kh_arg* a) {
+  k_solver_bicg_step<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), KCF(4), KCF(5), KCF(6), KB(7));
+}
+void l_solver_project(kh_device* d, int n, const kh_arg* a) {
+  k_solver_project<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2), f32(a, 3));
+}
+// Fused reductions use one workgroup per 256 rows; the partials scratch bounds n.
+void check_reduce_grid(int n) {
+  if (blocks_for(std::size_t(n)) > unsigned(kMaxReduceBlocks))
+    throw std::runtime_error("fused reduction kernel: n exceeds 16,777,216 rows");
+}
+void l_fused_spmv_dot_alpha(kh_device* d, int n, const kh_arg* a) {
+  check_reduce_grid(n);
+  k_fused_spmv_dot_alpha<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KF(5),
+                                      static_cast<int*>(buf(a, 6)), d->partials, d->ticket);
+}
+void l_fused_step_dot_check(kh_device* d, int n, const kh_arg* a) {
+  check_reduce_grid(n);
+  k_fused_step_dot_check<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), KF(4),
+                                      static_cast<int*>(buf(a, 5)), d->partials, d->ticket);
+}
+void l_fused_residual_restrict(kh_device* d, int n, const kh_arg* a) {
+  k_fused_residual_restrict<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6),
+                                         KF(7), KB(8));
+}
+void l_fused_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
+  k_fused_prolong_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
+                                      KB(7), KCF(8), KB(9));
+}
+void l_fused_smooth_dot_beta(kh_device* d, int n, const kh_arg* a) {
+  check_reduce_grid(n);
+  k_fused_smooth_dot_beta<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
+                                       static_cast<int*>(buf(a, 7)), KF(8), d->partials, d->ticket);
+}
+void l_fused_smooth_sum(kh_device* d, int n, const kh_arg* a) {
+  check_reduce_grid(n);
+  k_fused_smooth_sum<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7),
+                                  KF(8), d->partials, d->ticket);
+}
+void l_fused_smooth_from_zero(kh_device* d, int n, const kh_arg* a) {
+  k_fused_smooth_from_zero<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6));
+}
+void l_fused_project_dot_beta(kh_device* d, int n, const kh_arg* a) {
+  check_reduce_grid(n);
+  k_fused_project_dot_beta<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2), f32(a, 3), KCF(4), KF(5),
+                                        static_cast<int*>(buf(a, 6)), d->partials, d->ticket);
+}
+
+}  // namespace
+
+const std::vector<KernelEntry>& linalg_kernels() {
+  static const std::vector<KernelEntry> table = {
+      {KH_CSR_SPMV, "csr_spmv", "bbbbb", l_csr_spmv},
+      {KH_CSR_SPMV_GUARDED, "csr_spmv_guarded", "bbbbbb", l_csr_spmv_guarded},
+      {KH_CSR_RESIDUAL, "csr_residual", "bbbbbb", l_csr_residual},
+      {KH_CSR_RESIDUAL_GUARDED, "csr_residual_guarded", "bbbbbbb", l_csr_residual_guarded},
+      {KH_KRYLOV_DIAGONAL, "krylov_diagonal", "bbbbbi", l_krylov_diagonal},
+      {KH_KRYLOV_MULTIPLY, "krylov_multiply", "bbb", l_krylov_multiply},
+      {KH_KRYLOV_MULTIPLY_GUARDED, "krylov_multiply_guarded", "bbbb", l_krylov_multiply_guarded},
+      {KH_KRYLOV_SMOOTH, "krylov_smooth", "bbbf", l_krylov_smooth},
+      {KH_KRYLOV_SHIFT, "krylov_shift", "bf", l_krylov_shift},
+      {KH_KRYLOV_DIRECTION, "krylov_direction", "bbf", l_krylov_direction},
+      {KH_KRYLOV_STEP, "krylov_step", "bbbbf", l_krylov_step},
+      {KH_KRYLOV_BICG_DIRECTION, "krylov_bicg_direction", "bbbff", l_krylov_bicg_direction},
+      {KH_KRYLOV_DIFFERENCE, "krylov_difference", "bbbf", l_krylov_difference},
+      {KH_KRYLOV_BICG_STEP, "krylov_bicg_step", "bbbff", l_krylov_bicg_step},
+      {KH_AMG_COARSE_SOLVE, "amg_coarse_solve", "bbbbbbbii", l_amg_coarse_solve},
+      {KH_AMG_SMOOTH, "amg_smooth", "bbbbbbbb", l_amg_smooth},
+      {KH_AMG_SMOOTH_ZERO, "amg_smooth_zero", "bbbbbbbb", l_amg_smooth_zero},
+      {KH_AMG_RESTRICT, "amg_restrict", "bbbb", l_amg_restrict},
+      {KH_AMG_PROLONG, "amg_prolong", "bbb", l_amg_prolong},
+      {KH_AMG_GALERKIN, "amg_galerkin", "bbbb", l_amg_galerkin},
+      {KH_AMG_M_MATRIX_DIAGONAL, "amg_m_matrix_diagonal", "bbbbbbbb", l_amg_m_matrix_diagonal},
+      {KH_AMG_RESTRICT_GUARDED, "amg_restrict_guarded", "bbbbb", l_amg_restrict_guarded},
+      {KH_AMG_PROLONG_GUARDED, "amg_prolong_guarded", "bbbb", l_amg_prolong_guarded},
+      {KH_AXPY, "axpy", "bbf", l_axpy},
+      {KH_XPBY, "xpby", "bbf", l_xpby},
+      {KH_AXPBYZ, "axpbyz", "bbbff", l_axpbyz},
+      {KH_SCALE_COPY, "scale_copy", "bbf", l_scale_copy},
+      {KH_MUL, "mul", "bbb", l_mul},
+      {KH_SOLVER_LIMITS, "solver_limits", "bffff", l_solver_limits},
+      {KH_SOLVER_RESET, "solver_reset", "bbi", l_solver_reset},
+      {KH_SOLVER_PCG_ALPHA, "solver_pcg_alpha", "bb", l_solver_pcg_alpha},
+      {KH_SOLVER_PCG_BETA, "solver_pcg_beta", "bbi", l_solver_pcg_beta},
+      {KH_SOLVER_BICG_BETA, "solver_bicg_beta", "bb", l_solver_bicg_beta},
+      {KH_SOLVER_BICG_ALPHA, "solver_bicg_alpha", "bb", l_solver_bicg_alpha},
+      {KH_SOLVER_BICG_S, "solver_bicg_s", "bb", l_solver_bicg_s},
+      {KH_SOLVER_BICG_OMEGA, "solver_bicg_omega", "bb", l_solver_bicg_omega},
+      {KH_SOLVER_CHECK, "solver_check", "bbi", l_solver_check},
+      {KH_SOLVER_STEP, "solver_step", "bbbbbb", l_solver_step},
+      {KH_SOLVER_DIRECTION, "solver_direction", "bbbb", l_solver_direction},
+      {KH_SOLVER_BICG_DIRECTION, "solver_bicg_direction", "bbbbb", l_solver_bicg_direction},
+      {KH_SOLVER_BICG_DIFFERENCE, "solver_bicg_difference", "bbbbb", l_solver_bicg_difference},
+      {KH_SOLVER_BICG_STEP, "solver_bicg_step", "bbbbbbbb", l_solver_bicg_step},
+      {KH_SOLVER_PROJECT, "solver_project", "bbff", l_solver_project},
+      {KH_FUSED_SPMV_DOT_ALPHA, "fused_spmv_dot_alpha", "bbbbbbb", l_fused_spmv_dot_alpha},
+      {KH_FUSED_STEP_DOT_CHECK, "fused_step_dot_check", "bbbbbb", l_fused_step_dot_check},
+      {KH_FUSED_RESIDUAL_RESTRICT, "fused_residual_restrict", "bbbbbbbbb", l_fused_residual_restrict},
+      {KH_FUSED_PROLONG_SMOOTH, "fused_prolong_smooth", "bbbbbbbbbb", l_fused_prolong_smooth},
+      {KH_FUSED_SMOOTH_DOT_BETA, "fused_smooth_dot_beta", "bbbbbbbbb", l_fused_smooth_dot_beta},
+      {KH_FUSED_SMOOTH_SUM, "fused_smooth_sum", "bbbbbbbbb", l_fused_smooth_sum},
+      {KH_FUSED_PROJECT_DOT_BETA, "fused_project_dot_beta", "bbffbbb", l_fused_project_dot_beta},
+      {KH_FUSED_SMOOTH_FROM_ZERO, "fused_smooth_from_zero", "bbbbbbb", l_fused_smooth_from_zero},
+  };
+  return table;
+}
+
+}  // namespace kh

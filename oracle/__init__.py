@@ -1,0 +1,270 @@
+"""ORACLE — test infrastructure only.
+
+ctypes view of ``oracle/liboracle.so`` (the serial CPU restatement of the
+reference path) and, when built, ``oracle/_ref`` (the reference's own kernel
+bodies and mesh code compiled from its sources). Only the tests, the smoke
+check and the benchmark's CPU-baseline leg load this module, and only to check
+or time against; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_lib = None
+_ref = None
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            raise OracleError(f"{path} not built; run `make -C oracle`")
+        L = C.CDLL(str(path))
+        vp, i32, dbl, lng = C.c_void_p, C.c_int, C.c_double, C.c_long
+        L.oracle_last_error.restype = C.c_char_p
+        for f in ("oracle_sum", "oracle_max_abs"):
+            getattr(L, f).argtypes = [vp, lng]
+            getattr(L, f).restype = dbl
+        L.oracle_dot.argtypes = [vp, vp, lng]
+        L.oracle_dot.restype = dbl
+        L.oracle_reduce_into.argtypes = [i32, vp, vp, vp, vp, lng, vp]
+        L.oracle_kadd_up.argtypes = [C.c_float, C.c_float]
+        L.oracle_kadd_up.restype = C.c_float
+        L.oracle_poisson.argtypes = [i32, i32, i32, dbl, i32, i32, dbl]
+        L.oracle_poisson.restype = vp
+        L.oracle_csr_from_arrays.argtypes = [i32, vp, vp, vp, vp]
+        L.oracle_csr_from_arrays.restype = vp
+        L.oracle_csr_free.argtypes = [vp]
+        L.oracle_csr_rows.argtypes = [vp]
+        L.oracle_csr_nnz.argtypes = [vp]
+        L.oracle_csr_copy.argtypes = [vp, vp, vp, vp, vp]
+        L.oracle_solver_create.argtypes = [i32, vp, vp, dbl, C.POINTER(vp)]
+        L.oracle_solver_free.argtypes = [vp]
+        L.oracle_solver_refresh.argtypes = [vp, vp]
+        L.oracle_solver_solve.argtypes = [vp, vp, vp, dbl, dbl, i32, i32, vp]
+        L.oracle_solver_describe.argtypes = [vp, C.c_char_p, i32]
+        L.oracle_solver_levels.argtypes = [vp, vp, vp, i32]
+        L.oracle_solver_transfer.argtypes = [vp, i32, i32, vp]
+        L.oracle_solver_transfer.restype = lng
+        L.oracle_solver_level_values.argtypes = [vp, i32, vp]
+        L.oracle_solver_level_values.restype = lng
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise OracleError(lib().oracle_last_error().decode())
+
+
+# ------------------------------------------------------------------ matrices
+class Csr:
+    """Host CSR with double values (linalg/csr.hpp:25-28)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        L = lib()
+        self.n_rows = L.oracle_csr_rows(handle)
+        self.nnz = L.oracle_csr_nnz(handle)
+        self.row_ptr = np.empty(self.n_rows + 1, np.int32)
+        self.col = np.empty(self.nnz, np.int32)
+        self.diag = np.empty(self.n_rows, np.int32)
+        self.val = np.empty(self.nnz, np.float64)
+        L.oracle_csr_copy(handle, _p(self.row_ptr), _p(self.col), _p(self.diag), _p(self.val))
+
+    @classmethod
+    def from_arrays(cls, row_ptr, col, diag, val):
+        rp, c, d = _i32(row_ptr), _i32(col), _i32(diag)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        return cls(lib().oracle_csr_from_arrays(len(rp) - 1, _p(rp), _p(c), _p(d), _p(v)))
+
+    def __del__(self):
+        try:
+            lib().oracle_csr_free(self.handle)
+        except Exception:
+            pass
+
+
+def poisson(nx, ny, nz=1, anisotropy=1.0, random=False, neumann=False, wind=0.0) -> Csr:
+    """tests/linalg/matrices.hpp:17-62"""
+    return Csr(lib().oracle_poisson(nx, ny, nz, anisotropy, int(random), int(neumann), wind))
+
+
+def hash32(x):
+    """tests/linalg/matrices.hpp:13-15, vectorised over uint32."""
+    x = np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    return (x ^ (x >> 16)).astype(np.uint32)
+
+
+def bench_rhs(n: int) -> np.ndarray:
+    """bench/bench_linalg.cpp:57-58: b_i = (hash(i+53) % 2001 - 1000) / 1024."""
+    h = hash32(np.arange(n, dtype=np.uint64) + 53)
+    return ((h % 2001).astype(np.int64) - 1000).astype(np.float32) / np.float32(1024.0)
+
+
+def aggregate(a: Csr, passes=3, m_matrix=False, threshold=0.25) -> dict:
+    """One level of host aggregation (linalg/amg.cpp:160-174)."""
+    L = lib()
+    L.oracle_aggregate.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_void_p)]
+    L.oracle_coarsening_array.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.oracle_coarsening_array.restype = C.c_long
+    L.oracle_coarsening_values.argtypes = [C.c_void_p, C.c_void_p]
+    L.oracle_coarsening_values.restype = C.c_long
+    L.oracle_coarsening_free.argtypes = [C.c_void_p]
+    opts = np.array([5, 5, 32, 24, 24, passes, 0, int(m_matrix)], dtype=np.int32)
+    h = C.c_void_p()
+    check(L.oracle_aggregate(a.handle, _p(opts), threshold, C.byref(h)))
+    out = {}
+    try:
+        for k, name in enumerate(["aggregate", "member_offsets", "members", "entry_offsets",
+                                  "entries", "row_ptr", "col", "diag"]):
+            arr = np.empty(L.oracle_coarsening_array(h, k, None), np.int32)
+            L.oracle_coarsening_array(h, k, _p(arr))
+            out[name] = arr
+        val = np.empty(L.oracle_coarsening_values(h, None), np.float64)
+        L.oracle_coarsening_values(h, _p(val))
+        out["val"] = val
+    finally:
+        L.oracle_coarsening_free(h)
+    return out
+
+
+# ------------------------------------------------------------------- solvers
+KINDS ={"Jacobi": 0, "PcgJacobi": 1, "BicgstabJacobi": 2, "AmgPcg": 3, "AmgBicgstab": 4}
+
+
+class Solver:
+    def __init__(self, kind: str, a: Csr, pre=5, post=5, coarse_sweeps=32, coarse_rows=24,
+                 max_levels=24, passes=3, nullspace=False, m_matrix=False, threshold=0.25):
+        opts = np.array([pre, post, coarse_sweeps, coarse_rows, max_levels, passes,
+                         int(nullspace), int(m_matrix)], dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().oracle_solver_create(KINDS[kind], a.handle, _p(opts), threshold, C.byref(h)))
+        self.handle = h.value
+        self.n = a.n_rows
+
+    def refresh(self, values) -> None:
+        v = _f32(values)
+        check(lib().oracle_solver_refresh(self.handle, _p(v)))
+
+    def solve(self, b, x0=None, rel_tol=1e-2, abs_tol=1e-12, max_iter=200, interval=8):
+        x = _f32(np.zeros(self.n) if x0 is None else x0).copy()
+        bb = _f32(b)
+        stats = np.zeros(4, np.float64)
+        check(lib().oracle_solver_solve(self.handle, _p(x), _p(bb), rel_tol, abs_tol, max_iter,
+                                        interval, _p(stats)))
+        return x, dict(iterations=int(stats[0]), initial_residual=stats[1],
+                       final_residual=stats[2], converged=bool(stats[3]))
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(512)
+        lib().oracle_solver_describe(self.handle, buf, 512)
+        return buf.value.decode()
+
+    def levels(self):
+        rows, nnz = np.zeros(64, np.int32), np.zeros(64, np.int32)
+        k = lib().oracle_solver_levels(self.handle, _p(rows), _p(nnz), 64)
+        return [(int(rows[i]), int(nnz[i])) for i in range(k)]
+
+    def transfer(self, level: int, which: str):
+        w = ["aggregate", "member_offsets", "members", "entry_offsets", "entries"].index(which)
+        n = lib().oracle_solver_transfer(self.handle, level, w, None)
+        if n < 0:
+            return None
+        out = np.empty(n, np.int32)
+        lib().oracle_solver_transfer(self.handle, level, w, _p(out))
+        return out
+
+    def level_values(self, level: int):
+        n = lib().oracle_solver_level_values(self.handle, level, None)
+        if n < 0:
+            return None
+        out = np.empty(n, np.float32)
+        lib().oracle_solver_level_values(self.handle, level, _p(out))
+        return out
+
+    def __del__(self):
+        try:
+            lib().oracle_solver_free(self.handle)
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------- kernels and sums
+def sum_(x):
+    x = _f32(x)
+    return lib().oracle_sum(_p(x), x.size)
+
+
+def dot(x, y):
+    x, y = _f32(x), _f32(y)
+    return lib().oracle_dot(_p(x), _p(y), x.size)
+
+
+def max_abs(x):
+    x = _f32(x)
+    return lib().oracle_max_abs(_p(x), x.size)
+
+
+def reduce_into(mode: str, x, y=None, u=None, v=None):
+    x = _f32(x)
+    y = _f32(x if y is None else y)
+    u = _f32(x if u is None else u)
+    v = _f32(x if v is None else v)
+    out = np.zeros(4, np.float32)
+    lib().oracle_reduce_into({"sum": 0, "dot": 1, "dot2": 2}[mode], _p(x), _p(y), _p(u), _p(v),
+                             x.size, _p(out))
+    return out if mode == "dot2" else out[:2]
+
+
+def kernel(name: str, *args):
+    """Calls oracle_<name> with numpy arrays (converted in place by dtype) and
+    scalars; arrays are passed by pointer so outputs land in them."""
+    fn = getattr(lib(), f"oracle_{name}")
+    cargs = []
+    for a in args:
+        if isinstance(a, np.ndarray):
+            if not a.flags["C_CONTIGUOUS"]:
+                raise OracleError("oracle kernels need contiguous arrays")
+            cargs.append(C.c_void_p(a.ctypes.data))
+        elif isinstance(a, float) or isinstance(a, np.float32):
+            cargs.append(C.c_float(a))
+        else:
+            cargs.append(C.c_int(int(a)))
+    fn(*cargs)
+
+
+def ref_kernels():
+    """The reference's own kernel bodies (oracle/_ref), or None when not built."""
+    global _ref
+    if _ref is None:
+        path = HERE / "_ref" / "libref_kernels.so"
+        if not path.exists():
+            return None
+        _ref = C.CDLL(str(path))
+    return _ref

@@ -1,0 +1,71 @@
+// ORACLE — test infrastructure only. Builds the reference's own kernel bodies
+// (src/kutta/kernels/*.kern.h, read in place under the reference tree) into
+// oracle/_ref/libref_kernels.so, with one extern "C" loop per kernel running
+// the body for every index in [0, n) as the reference's serial CPU backend
+// does (runtime/cpu/cpu_device.cpp:64-68). Used only to cross-check the
+// oracle's restatement.
+#include "kutta/kernels/prelude.h"
+
+// Kernel bodies live in their own namespace, as the reference's generated CPU
+// table arranges them (tools/kernelgen.py:176-186).
+namespace refk {
+#include "kutta/kernels/linalg.kern.h"
+#include "kutta/kernels/amg.kern.h"
+#include "kutta/kernels/blas.kern.h"
+#include "kutta/kernels/scalars.kern.h"
+}  // namespace refk
+
+#define LOOP(name, ...) for (int i = 0; i < n; ++i) refk::name(i, __VA_ARGS__)
+
+extern "C" {
+void ref_csr_spmv(int n, const int* r, const int* c, const float* v, const float* x, float* y) {
+  LOOP(csr_spmv, r, c, v, x, y);
+}
+void ref_csr_residual(int n, const int* r, const int* c, const float* v, const float* x,
+                      const float* b, float* out) {
+  LOOP(csr_residual, r, c, v, x, b, out);
+}
+void ref_krylov_diagonal(int n, const int* r, const int* d, const float* v, float* inv,
+                         float* invalid, int l1) {
+  LOOP(krylov_diagonal, r, d, v, inv, invalid, l1);
+}
+void ref_amg_smooth(int n, const int* r, const int* c, const float* v, const float* x,
+                    const float* b, const float* inv, float* next, const int* st) {
+  LOOP(amg_smooth, r, c, v, x, b, inv, next, st);
+}
+void ref_amg_smooth_zero(int n, const int* r, const int* c, const float* v, const float* x,
+                         const float* b, const float* inv, float* next, const int* st) {
+  LOOP(amg_smooth_zero, r, c, v, x, b, inv, next, st);
+}
+void ref_amg_coarse_solve(const int* r, const int* c, const float* v, float* x, const float* b,
+                          const float* inv, const int* st, int count, int sweeps) {
+  refk::amg_coarse_solve(0, r, c, v, x, b, inv, st, count, sweeps);
+}
+void ref_amg_restrict(int n, const int* off, const int* mem, const float* f, float* c) {
+  LOOP(amg_restrict, off, mem, f, c);
+}
+void ref_amg_prolong(int n, const int* agg, const float* c, float* f) { LOOP(amg_prolong, agg, c, f); }
+void ref_amg_galerkin(int n, const int* off, const int* ent, const float* f, float* c) {
+  LOOP(amg_galerkin, off, ent, f, c);
+}
+void ref_amg_m_matrix_diagonal(int n, const int* off, const int* mem, const int* fr, const int* fd,
+                               const float* f, const int* r, const int* d, float* c) {
+  LOOP(amg_m_matrix_diagonal, off, mem, fr, fd, f, r, d, c);
+}
+void ref_solver_scalar(int which, float* s, int* st, int flag) {
+  switch (which) {
+    case 0: refk::solver_reset(0, s, st, flag); break;
+    case 1: refk::solver_pcg_alpha(0, s, st); break;
+    case 2: refk::solver_pcg_beta(0, s, st, flag); break;
+    case 3: refk::solver_bicg_beta(0, s, st); break;
+    case 4: refk::solver_bicg_alpha(0, s, st); break;
+    case 5: refk::solver_bicg_s(0, s, st); break;
+    case 6: refk::solver_bicg_omega(0, s, st); break;
+    case 7: refk::solver_check(0, s, st, flag); break;
+  }
+}
+void ref_solver_project(int n, float* x, const float* sum, float hi, float lo) {
+  LOOP(solver_project, x, sum, hi, lo);
+}
+float ref_kadd_up(float a, float b) { return kadd_up(a, b); }
+}

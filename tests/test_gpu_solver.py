@@ -1,0 +1,167 @@
+"""The HIP-native Krylov/AMG solver end to end against the oracle and the
+reference's golden bench_linalg results (bench_linalg.log:1-12)."""
+import numpy as np
+import pytest
+
+import kutta_amd
+import oracle
+from kutta_amd import AmgOptions, NullSpace, OperatorStructure, SolveControl, SolverKind
+
+pytestmark = pytest.mark.gpu
+
+
+def g9(v):
+    return f"{v:.9g}"
+
+
+def product_setup(device, a, kind, options=None):
+    host = kutta_amd.CsrMatrixHost(a.row_ptr, a.col, a.diag, a.val)
+    matrix = kutta_amd.upload(device, host)
+    if kind in (SolverKind.AmgPcg, SolverKind.AmgBicgstab):
+        solver = kutta_amd.make_amg_solver(kind, device, host, options)
+    else:
+        solver = kutta_amd.make_solver(kind, device, host)
+    return host, matrix, solver
+
+
+def bench_flow(device, size, aniso, kind, fused=True):
+    """bench/bench_linalg.cpp:42-83 on the HIP backend."""
+    a = oracle.poisson(size, size, size, aniso)
+    host, matrix, solver = product_setup(device, a, kind)
+    solver.set_fused(fused)
+    vals = a.val.astype(np.float32)
+    matrix.val.upload(vals * np.float32(1.25))
+    solver.refresh(matrix)
+    matrix.val.upload(vals)
+    solver.refresh(matrix)
+    b = device.upload(oracle.bench_rhs(a.n_rows))
+    x = device.alloc(a.n_rows)
+    x.fill(np.float32(0))
+    st = solver.solve(matrix, x, b, SolveControl(1e-6, 1e-12, 2000))
+    return st, solver.describe(), x.download()
+
+
+GOLDEN = [
+    (64, 1, SolverKind.PcgJacobi, 278, "6.85768115e-07", "0.000197942264", True, "PCG, Jacobi"),
+    (64, 1, SolverKind.AmgPcg, 15, "7.91394974e-07", "0.000228430732", True, "6 levels, op complexity 1.140"),
+    (64, 1000, SolverKind.AmgPcg, 38, "1.51536251e-06", "0.000437398999", False, "6 levels, op complexity 1.134"),
+]
+
+
+@pytest.mark.parametrize("size,aniso,kind,its,rel,final,conv,desc", GOLDEN)
+def test_bench_linalg_golden_on_gpu(device, size, aniso, kind, its, rel, final, conv, desc):
+    print(f"[golden] {size}^3 aniso={aniso} {kind.name}", flush=True)
+    st, describe, _ = bench_flow(device, size, aniso, kind)
+    assert st.iterations == its
+    assert g9(st.final_residual / st.initial_residual) == rel
+    assert g9(st.initial_residual) == "288.643143"
+    assert g9(st.final_residual) == final
+    assert st.converged == conv
+    assert describe.endswith(desc)
+
+
+def test_fused_and_reference_sequences_agree(device):
+    st1, _, x1 = bench_flow(device, 40, 1, SolverKind.AmgPcg, fused=True)
+    st2, _, x2 = bench_flow(device, 40, 1, SolverKind.AmgPcg, fused=False)
+    assert st1.iterations == st2.iterations
+    assert st1.final_residual == st2.final_residual
+    assert np.array_equal(x1.view(np.uint32), x2.view(np.uint32))
+
+
+def oracle_solve(a, kind, b, control, **opts):
+    s = oracle.Solver(kind, a, **opts)
+    x, st = s.solve(b, rel_tol=control.rel_tol, abs_tol=control.abs_tol, max_iter=control.max_iter,
+                    interval=control.residual_check_interval)
+    return x, st, s
+
+
+CASES = [
+    ("poisson 3 passes", lambda: oracle.poisson(30, 28, 26), dict(), AmgOptions()),
+    ("random 2 passes V(2,2)", lambda: oracle.poisson(33, 31, 29, 1.0, random=True),
+     dict(pre=2, post=2, passes=2), AmgOptions(pre_sweeps=2, post_sweeps=2, aggregation_passes=2)),
+    ("neumann M-matrix, constant nullspace",
+     lambda: oracle.poisson(31, 29, 27, 1.0, random=True, neumann=True),
+     dict(passes=2, nullspace=True, m_matrix=True),
+     AmgOptions(aggregation_passes=2, nullspace=NullSpace.Constant,
+                structure=OperatorStructure.SymmetricMMatrix)),
+]
+
+
+@pytest.mark.parametrize("name,make,oopts,popts", CASES, ids=[c[0] for c in CASES])
+def test_amg_pcg_matches_oracle(device, name, make, oopts, popts):
+    print(f"[parity] {name}", flush=True)
+    a = make()
+    rng = np.random.default_rng(17)
+    b64 = rng.standard_normal(a.n_rows)
+    if popts.nullspace == NullSpace.Constant:
+        b64 -= b64.mean()
+    b = b64.astype(np.float32)
+    control = SolveControl(1e-6, 0.0, 300)
+    xo, sto, so = oracle_solve(a, "AmgPcg", b, control, **oopts)
+    host, matrix, solver = product_setup(device, a, SolverKind.AmgPcg, popts)
+    assert solver.describe() == so.describe()
+    assert solver.levels() == so.levels()
+    db = device.upload(b)
+    x = device.alloc(a.n_rows)
+    x.fill(np.float32(0))
+    st = solver.solve(matrix, x, db, control)
+    assert st.iterations == sto["iterations"]
+    assert st.converged == sto["converged"]
+    xp = x.download()
+    assert np.array_equal(xp.view(np.uint32), xo.view(np.uint32))
+    assert st.final_residual == pytest.approx(sto["final_residual"], rel=1e-12)
+
+
+def test_hierarchy_and_refresh_match_oracle(device):
+    """Aggregates, transfers and the device-refreshed coarse operators
+    (amg_galerkin + amg_m_matrix_diagonal) equal the oracle's bit for bit."""
+    a = oracle.poisson(26, 24, 22, 1.0, random=True, neumann=True)
+    popts = AmgOptions(aggregation_passes=2, nullspace=NullSpace.Constant,
+                       structure=OperatorStructure.SymmetricMMatrix)
+    host, matrix, solver = product_setup(device, a, SolverKind.AmgPcg, popts)
+    so = oracle.Solver("AmgPcg", a, passes=2, nullspace=True, m_matrix=True)
+    vals = a.val.astype(np.float32)
+    scaled = vals * np.float32(1.13)
+    matrix.val.upload(scaled)
+    solver.refresh(matrix)
+    so.refresh(scaled)
+    levels = solver.levels()
+    assert len(levels) >= 3
+    for lvl in range(len(levels) - 1):
+        for which in ("aggregate", "member_offsets", "members", "entry_offsets", "entries"):
+            assert np.array_equal(solver.transfer(lvl, which), so.transfer(lvl, which)), (lvl, which)
+    for lvl in range(1, len(levels)):
+        assert np.array_equal(solver.level_values(lvl).view(np.uint32),
+                              so.level_values(lvl).view(np.uint32)), lvl
+
+
+def test_bicgstab_matches_oracle(device):
+    a = oracle.poisson(24, 22, 1, 1.0, wind=2.0)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(a.n_rows).astype(np.float32)
+    control = SolveControl(1e-5, 0.0, 200)
+    for kind, ok in ((SolverKind.BicgstabJacobi, "BicgstabJacobi"), (SolverKind.AmgBicgstab, "AmgBicgstab")):
+        xo, sto, _ = oracle_solve(a, ok, b, control, passes=2, pre=2, post=2)
+        opts = AmgOptions(pre_sweeps=2, post_sweeps=2, aggregation_passes=2)
+        host, matrix, solver = product_setup(device, a, kind, opts)
+        x = device.alloc(a.n_rows)
+        x.fill(np.float32(0))
+        st = solver.solve(matrix, x, device.upload(b), control)
+        assert st.iterations == sto["iterations"], kind
+        assert np.array_equal(x.download().view(np.uint32), xo.view(np.uint32)), kind
+
+
+def test_failure_paths_are_host_forced(device):
+    """solver.cpp:97-176 failure handling without any GPU fault: a NaN RHS
+    raises NonfiniteError; max_iter=0 returns the initial residual."""
+    a = oracle.poisson(10, 10, 10)
+    host, matrix, solver = product_setup(device, a, SolverKind.AmgPcg)
+    x = device.alloc(a.n_rows)
+    x.fill(np.float32(0))
+    bad = np.ones(a.n_rows, np.float32)
+    bad[3] = np.nan
+    with pytest.raises(kutta_amd.NonfiniteError):
+        solver.solve(matrix, x, device.upload(bad), SolveControl(1e-6, 0, 10))
+    st = solver.solve(matrix, x, device.upload(np.ones(a.n_rows, np.float32)), SolveControl(1e-6, 0, 0))
+    assert st.iterations == 0 and not st.converged
+    assert st.final_residual == st.initial_residual
