@@ -52,6 +52,36 @@ const int32_t* ka_matrix_row_ptr(void* matrix);
 const int32_t* ka_matrix_col(void* matrix);
 const int32_t* ka_matrix_diag(void* matrix);
 
+/* Pressure-correction operator of a box mesh on [0,1]^3 as the SIMPLEC solver
+ * sets it up (fv/steady_solver.cpp:240-371): make_box_tet/hex
+ * (io/generators.cpp:98-103), an optional seeded point perturbation of
+ * relative amplitude `amplitude` (no reference counterpart), RCM renumbering,
+ * FP64 geometry, the FV stencil and fv_pressure_assemble on the device with the
+ * face diffusion coefficients and zero flux. The handle is also a matrix handle
+ * (its assembled device operator) for ka_solver_refresh / ka_solver_solve. */
+int ka_pressure_problem_create(void* dev, int nx, int ny, int nz, int tet, double amplitude,
+                               unsigned long long seed, int reference, void** out);
+void ka_pressure_problem_destroy(void* problem);
+/* Host CSR of the assembled operator, owned by the problem (for ka_solver_create). */
+void* ka_pressure_problem_csr(void* problem);
+/* Re-runs fv_pressure_assemble on the device. */
+int ka_pressure_problem_assemble(void* problem, int reference);
+/* reference: cell to pin (row/column eliminated), -1 none (pure Neumann), -2 the
+ * cell holding the origin, the SIMPLEC default (fv/steady_solver.cpp:247).
+ * info[7]: cells, faces, internal faces, nnz, cell-face entries, redraw rounds,
+ * pinned cell;
+ * seconds[6]: box, perturb, renumber, geometry, stencil, upload+assembly. */
+int ka_pressure_problem_info(void* problem, int32_t* info, double* seconds);
+/* Named host arrays (kind 0 int32, 1 float32, 2 float64); -1 if unknown. */
+long ka_pressure_problem_array(void* problem, const char* name, int kind, void* out);
+
+/* The host half of the same setup (mesh, renumbering, geometry, stencil),
+ * without a device; arrays by the same names as ka_pressure_problem_array. */
+int ka_mesh_setup_create(int nx, int ny, int nz, int tet, double amplitude, unsigned long long seed,
+                         void** out);
+void ka_mesh_setup_destroy(void* setup);
+long ka_mesh_setup_array(void* setup, const char* name, int kind, void* out);
+
 /* Solvers (linalg/solver.hpp:237-293, linalg/amg.hpp:8-27).
  * kind: 0 Jacobi, 1 PcgJacobi, 2 BicgstabJacobi, 3 AmgPcg, 4 AmgBicgstab.
  * options[8]: pre_sweeps, post_sweeps, coarse_sweeps, coarse_rows, max_levels,

@@ -124,6 +124,10 @@ typedef enum kh_kernel {
   KH_SOLVER_BICG_STEP,           /* :142-149 x,r,z,zs,s,t,scalars,status */
   KH_SOLVER_PROJECT,             /* :150-152 x,sum,count_hi,count_lo */
 
+  /* kernels/fv_flow.kern.h (finite-volume face loops and assembly) */
+  KH_FV_PRESSURE_ASSEMBLE = 100, /* :164-183 offsets,faces,sign,adjacent,entry,rows,diag,
+                                    coefficient,flux,values,rhs,reference */
+
   /* Fused MI355X kernels (no reference counterpart). */
   KH_FUSED_SPMV_DOT_ALPHA = 200, /* csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha:
                                     rows,cols,values,p,q,scalars,status */
@@ -184,6 +188,8 @@ void kh_graph_destroy(kh_graph* graph);
  * counterpart): while enabled, every launch of `kernel` is bracketed by events.
  * kh_profile_read synchronizes and returns the launch count and summed ms. */
 int kh_profile_enable(kh_device* dev, uint32_t kernel, int enable);
+/* Restricts timing of `kernel` to launches over exactly n indices (0: all). */
+int kh_profile_filter(kh_device* dev, uint32_t kernel, size_t n);
 int kh_profile_read(kh_device* dev, uint32_t kernel, long* launches, double* total_ms);
 int kh_profile_reset(kh_device* dev);
 

@@ -350,6 +350,11 @@ class Device:
     def profile(self, kernel, enable: bool = True) -> None:
         _check_hip(hip_lib().kh_profile_enable(self.kh, int(kernel), 1 if enable else 0))
 
+    def profile_filter(self, kernel, n: int) -> None:
+        lib = hip_lib()
+        lib.kh_profile_filter.argtypes = [C.c_void_p, C.c_uint32, C.c_size_t]
+        _check_hip(lib.kh_profile_filter(self.kh, int(kernel), int(n)))
+
     def profile_read(self, kernel) -> tuple[int, float]:
         n, ms = C.c_long(), C.c_double()
         _check_hip(hip_lib().kh_profile_read(self.kh, int(kernel), C.byref(n), C.byref(ms)))
@@ -550,6 +555,108 @@ class LinearSolver:
                 host_lib().ka_solver_destroy(self.handle)
         except Exception:
             pass
+
+
+_ARRAY_KINDS = {0: np.int32, 1: np.float32, 2: np.float64}
+
+
+def _named_array(fn, handle, name: str) -> np.ndarray:
+    fn.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p]
+    fn.restype = C.c_long
+    for kind, dtype in _ARRAY_KINDS.items():
+        n = fn(handle, name.encode(), kind, None)
+        if n >= 0:
+            out = np.empty(n, dtype)
+            fn(handle, name.encode(), kind, out.ctypes.data)
+            return out
+    raise KeyError(name)
+
+
+C2_SEED = 20261202
+
+
+class MeshSetup:
+    """Host mesh setup of a box on [0,1]^3: make_box_tet/hex, optional seeded
+    perturbation, RCM, FP64 geometry and FV stencil (no device needed)."""
+
+    def __init__(self, nx, ny, nz, tet=True, amplitude=0.2, seed=C2_SEED):
+        lib = host_lib()
+        lib.ka_mesh_setup_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.c_ulonglong, C.POINTER(C.c_void_p)]
+        lib.ka_mesh_setup_destroy.argtypes = [C.c_void_p]
+        h = C.c_void_p()
+        _check_host(lib.ka_mesh_setup_create(nx, ny, nz, int(tet), amplitude, seed, C.byref(h)))
+        self.handle = h.value
+
+    def __getitem__(self, name: str) -> np.ndarray:
+        return _named_array(host_lib().ka_mesh_setup_array, self.handle, name)
+
+    def __del__(self):
+        try:
+            host_lib().ka_mesh_setup_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class PressureProblem:
+    """The SIMPLEC pressure-correction operator of a box mesh, assembled on the
+    device by fv_pressure_assemble (zero flux, diffusion coefficients). Usable
+    as the matrix of a solver; ``host`` is its host CSR for solver setup."""
+
+    def __init__(self, device: Device, nx, ny, nz, tet=True, amplitude=0.2, seed=C2_SEED,
+                 reference=-2):
+        lib = host_lib()
+        lib.ka_pressure_problem_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                   C.c_double, C.c_ulonglong, C.c_int,
+                                                   C.POINTER(C.c_void_p)]
+        lib.ka_pressure_problem_destroy.argtypes = [C.c_void_p]
+        lib.ka_pressure_problem_csr.argtypes = [C.c_void_p]
+        lib.ka_pressure_problem_csr.restype = C.c_void_p
+        lib.ka_pressure_problem_assemble.argtypes = [C.c_void_p, C.c_int]
+        lib.ka_pressure_problem_info.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        h = C.c_void_p()
+        _check_host(lib.ka_pressure_problem_create(device.handle, nx, ny, nz, int(tet), amplitude,
+                                                   seed, reference, C.byref(h)))
+        self.device = device
+        self.handle = h.value
+        info = np.zeros(7, np.int32)
+        secs = np.zeros(6, np.float64)
+        _check_host(lib.ka_pressure_problem_info(self.handle, info.ctypes.data, secs.ctypes.data))
+        (self.n_cells, self.n_faces, self.n_internal_faces, self.nnz, self.n_entries,
+         self.rounds, self.reference) = (int(v) for v in info)
+        self.n_rows = self.n_cells
+        self.setup_seconds = dict(zip(["box", "perturb", "renumber", "geometry", "stencil",
+                                       "upload_assemble"], secs.tolist()))
+        self.host = _BorrowedCsr(lib.ka_pressure_problem_csr(self.handle), self.n_rows, self.nnz)
+
+    def __getitem__(self, name: str) -> np.ndarray:
+        return _named_array(host_lib().ka_pressure_problem_array, self.handle, name)
+
+    def assemble(self, reference: int = -2) -> None:
+        _check_host(host_lib().ka_pressure_problem_assemble(self.handle, reference))
+
+    @property
+    def val(self) -> Buffer:
+        b = Buffer.__new__(Buffer)
+        b.device, b.dtype, b.n = self.device, np.dtype(np.float32), self.nnz
+        b.ptr = host_lib().ka_matrix_values(self.handle)
+        b.free = lambda: None
+        return b
+
+    def __del__(self):
+        try:
+            if self.device._alive:
+                host_lib().ka_pressure_problem_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class _BorrowedCsr:
+    """A host CSR owned by another native object (same interface as CsrMatrixHost
+    for solver construction)."""
+
+    def __init__(self, handle, n_rows, nnz):
+        self.handle, self.n_rows, self.nnz = handle, n_rows, nnz
 
 
 def aggregate(a: CsrMatrixHost, options: AmgOptions | None = None) -> dict:

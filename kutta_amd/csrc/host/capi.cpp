@@ -2,39 +2,27 @@
 #include <cstring>
 #include <string>
 
+#include "capi_internal.hpp"
 #include "kutta_amd.h"
-#include "krylov.hpp"
 
 using namespace kutta_amd;
+using capi::guard;
+using capi::Matrix;
+using capi::MatrixHandle;
+using capi::Solver;
+
+std::string& kutta_amd::capi::last_error() {
+  thread_local std::string error;
+  return error;
+}
 
 namespace {
-thread_local std::string g_error;
-template <class F>
-int guard(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const NonfiniteError& e) {
-    g_error = e.what();
-    return 2;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return 1;
-  }
-}
-struct Matrix {
-  HipDevice* dev;
-  OwnedCsr csr;
-};
-struct Solver {
-  HipDevice* dev;
-  KrylovSolver solver;
-};
+DeviceCsr view_of(void* m) { return static_cast<MatrixHandle*>(m)->view(); }
 }  // namespace
 
 extern "C" {
 
-const char* ka_last_error(void) { return g_error.c_str(); }
+const char* ka_last_error(void) { return capi::last_error().c_str(); }
 
 int ka_device_create(int ordinal, void** out) {
   return guard([&] { *out = new HipDevice(ordinal); });
@@ -109,14 +97,14 @@ int ka_matrix_upload(void* dev, void* csr, void** out) {
     auto* d = static_cast<HipDevice*>(dev);
     const auto& a = *static_cast<CsrHost*>(csr);
     validate_host(a);
-    *out = new Matrix{d, OwnedCsr(*d, a)};
+    *out = static_cast<MatrixHandle*>(new Matrix(d, OwnedCsr(*d, a)));
   });
 }
-void ka_matrix_destroy(void* m) { delete static_cast<Matrix*>(m); }
-float* ka_matrix_values(void* m) { return static_cast<Matrix*>(m)->csr.val.data(); }
-const int32_t* ka_matrix_row_ptr(void* m) { return static_cast<Matrix*>(m)->csr.row_ptr.data(); }
-const int32_t* ka_matrix_col(void* m) { return static_cast<Matrix*>(m)->csr.col.data(); }
-const int32_t* ka_matrix_diag(void* m) { return static_cast<Matrix*>(m)->csr.diag.data(); }
+void ka_matrix_destroy(void* m) { delete static_cast<MatrixHandle*>(m); }
+float* ka_matrix_values(void* m) { return view_of(m).val; }
+const int32_t* ka_matrix_row_ptr(void* m) { return view_of(m).row_ptr; }
+const int32_t* ka_matrix_col(void* m) { return view_of(m).col; }
+const int32_t* ka_matrix_diag(void* m) { return view_of(m).diag; }
 
 int ka_solver_create(void* dev, int kind, void* csr, const int32_t* options, double threshold,
                      void** out) {
@@ -138,13 +126,13 @@ int ka_solver_create(void* dev, int kind, void* csr, const int32_t* options, dou
 }
 void ka_solver_destroy(void* s) { delete static_cast<Solver*>(s); }
 int ka_solver_refresh(void* s, void* m) {
-  return guard([&] { static_cast<Solver*>(s)->solver.refresh(static_cast<Matrix*>(m)->csr.view()); });
+  return guard([&] { static_cast<Solver*>(s)->solver.refresh(view_of(m)); });
 }
 int ka_solver_solve(void* s, void* m, float* x, const float* b, double rel_tol, double abs_tol,
                     int max_iter, int interval, double* stats) {
   return guard([&] {
     SolveControl c{rel_tol, abs_tol, max_iter, interval};
-    const auto st = static_cast<Solver*>(s)->solver.solve(static_cast<Matrix*>(m)->csr.view(), x, b, c);
+    const auto st = static_cast<Solver*>(s)->solver.solve(view_of(m), x, b, c);
     stats[0] = st.iterations;
     stats[1] = st.initial_residual;
     stats[2] = st.final_residual;
@@ -174,7 +162,7 @@ long ka_solver_transfer(void* s, int level, int which, int32_t* out) {
     if (out) std::memcpy(out, v.data(), v.size() * sizeof(int32_t));
     return long(v.size());
   } catch (const std::exception& e) {
-    g_error = e.what();
+    capi::last_error() = e.what();
     return -1;
   }
 }
@@ -185,7 +173,7 @@ long ka_solver_level_values(void* s, int level, float* out) {
     if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
     return long(v.size());
   } catch (const std::exception& e) {
-    g_error = e.what();
+    capi::last_error() = e.what();
     return -1;
   }
 }

@@ -23,6 +23,7 @@ constexpr int kMaxHostResults = 256;
 
 struct ProfileSlot {
   bool enabled = false;
+  std::size_t only_n = 0;  // time only launches over this many indices (0: all)
   long launches = 0;
   double total_ms = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -73,6 +74,7 @@ struct KernelEntry {
 };
 // Defined by each kernel translation unit.
 const std::vector<KernelEntry>& linalg_kernels();
+const std::vector<KernelEntry>& fv_kernels();
 
 // Reduction launchers (kh_reduce.hip).
 void launch_reduce_into(kh_device* dev, int mode, const float* x, const float* y,

@@ -160,6 +160,7 @@ const KernelEntry* find_kernel(uint32_t id) {
   static const std::unordered_map<uint32_t, const KernelEntry*> index = [] {
     std::unordered_map<uint32_t, const KernelEntry*> m;
     for (const auto& e : linalg_kernels()) m[e.id] = &e;
+    for (const auto& e : fv_kernels()) m[e.id] = &e;
     return m;
   }();
   const auto it = index.find(id);
@@ -339,7 +340,8 @@ int kh_launch(kh_device* d, uint32_t kernel, size_t n, const kh_arg* args, size_
     }
     if (n == 0) return;
     auto& slot = d->profile[kernel < d->profile.size() ? kernel : 0];
-    const bool timed = kernel < d->profile.size() && slot.enabled && !d->capturing;
+    const bool timed = kernel < d->profile.size() && slot.enabled && !d->capturing &&
+                       (slot.only_n == 0 || slot.only_n == n);
     hipEvent_t start = nullptr, stop = nullptr;
     if (timed) {
       start = take_event(d);
@@ -447,6 +449,14 @@ int kh_profile_enable(kh_device* d, uint32_t kernel, int enable) {
     require(d != nullptr, "kh_profile_enable: null device");
     require(kernel < d->profile.size() && find_kernel(kernel), "kh_profile_enable: bad kernel id");
     d->profile[kernel].enabled = enable != 0;
+  });
+}
+
+int kh_profile_filter(kh_device* d, uint32_t kernel, size_t n) {
+  return guard([&] {
+    require(d != nullptr, "kh_profile_filter: null device");
+    require(kernel < d->profile.size() && find_kernel(kernel), "kh_profile_filter: bad kernel id");
+    d->profile[kernel].only_n = n;
   });
 }
 

@@ -259,6 +259,89 @@ def kernel(name: str, *args):
     fn(*cargs)
 
 
+class Bundle:
+    """Named arrays from the oracle's mesh / pressure-operator pipeline."""
+
+    KINDS = {0: np.int32, 1: np.float32, 2: np.float64}
+
+    def __init__(self, handle):
+        if not handle:
+            L = lib()
+            L.oracle_mesh_last_error.restype = C.c_char_p
+            raise OracleError(L.oracle_mesh_last_error().decode())
+        self.handle = handle
+
+    def get(self, name: str) -> np.ndarray:
+        L = lib()
+        L.oracle_bundle_array.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p]
+        L.oracle_bundle_array.restype = C.c_long
+        for kind, dtype in self.KINDS.items():
+            n = L.oracle_bundle_array(self.handle, name.encode(), kind, None)
+            if n >= 0:
+                out = np.empty(n, dtype)
+                L.oracle_bundle_array(self.handle, name.encode(), kind, _p(out))
+                return out
+        raise KeyError(name)
+
+    def __getitem__(self, name):
+        return self.get(name)
+
+    def csr(self) -> Csr:
+        """The assembled pressure operator as a host CSR (float values as double)."""
+        return Csr.from_arrays(self["row_ptr"], self["col"], self["diag"],
+                               self["values"].astype(np.float64))
+
+    def __del__(self):
+        try:
+            lib().oracle_bundle_free.argtypes = [C.c_void_p]
+            lib().oracle_bundle_free(self.handle)
+        except Exception:
+            pass
+
+
+def pressure_problem(nx, ny, nz, tet=True, amplitude=0.2, seed=20261202, reference=-2,
+                     renumber=True) -> Bundle:
+    """Box mesh on [0,1]^3 (io/generators.cpp:98-103), seeded perturbation,
+    RCM, geometry, stencil and fv_pressure_assemble with the diffusion
+    coefficients and zero flux. reference: pinned cell, -1 for none (pure
+    Neumann), -2 for the cell holding the origin (fv/steady_solver.cpp:247)."""
+    L = lib()
+    L.oracle_pressure_problem.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                          C.c_ulonglong, C.c_int, C.c_int]
+    L.oracle_pressure_problem.restype = C.c_void_p
+    return Bundle(L.oracle_pressure_problem(nx, ny, nz, int(tet), amplitude, seed, reference,
+                                            int(renumber)))
+
+
+def neumann_rhs(n: int, seed: int = 20261202, reference: int = -1) -> np.ndarray:
+    """Zero-mean Gaussian right-hand side: numpy PCG64(seed) standard normals in
+    FP64, mean subtracted in FP64, rounded to FP32; zero at a pinned cell."""
+    b = np.random.default_rng(seed).standard_normal(n)
+    b -= b.mean()
+    b = b.astype(np.float32)
+    if reference >= 0:
+        b[reference] = 0
+    return b
+
+
+def ref_mesh():
+    """The reference's own mesh code (oracle/_ref/libref_mesh.so), or None."""
+    path = HERE / "_ref" / "libref_mesh.so"
+    if not path.exists():
+        return None
+    L = C.CDLL(str(path))
+    L.ref_box.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.ref_box.restype = C.c_void_p
+    L.ref_free.argtypes = [C.c_void_p]
+    L.ref_set_points.argtypes = [C.c_void_p, C.c_void_p]
+    L.ref_renumber.argtypes = [C.c_void_p]
+    L.ref_geometry.argtypes = [C.c_void_p]
+    L.ref_array.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+    L.ref_array.restype = C.c_long
+    L.ref_mesh_last_error.restype = C.c_char_p
+    return L
+
+
 def ref_kernels():
     """The reference's own kernel bodies (oracle/_ref), or None when not built."""
     global _ref

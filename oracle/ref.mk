@@ -7,7 +7,7 @@ REF_SRC := $(REFERENCE)/src
 REF_FLAGS := -O2 -std=c++20 -ffp-contract=off -fPIC -w -I$(REF_SRC)
 OUT := _ref
 
-all: $(OUT)/libref_kernels.so
+all: $(OUT)/libref_kernels.so $(OUT)/libref_mesh.so
 
 $(OUT):
 	mkdir -p $(OUT)

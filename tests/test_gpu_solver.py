@@ -80,7 +80,7 @@ CASES = [
     ("random 2 passes V(2,2)", lambda: oracle.poisson(33, 31, 29, 1.0, random=True),
      dict(pre=2, post=2, passes=2), AmgOptions(pre_sweeps=2, post_sweeps=2, aggregation_passes=2)),
     ("neumann M-matrix, constant nullspace",
-     lambda: oracle.poisson(31, 29, 27, 1.0, random=True, neumann=True),
+     lambda: oracle.pressure_problem(16, 15, 14).csr(),
      dict(passes=2, nullspace=True, m_matrix=True),
      AmgOptions(aggregation_passes=2, nullspace=NullSpace.Constant,
                 structure=OperatorStructure.SymmetricMMatrix)),
@@ -115,7 +115,7 @@ def test_amg_pcg_matches_oracle(device, name, make, oopts, popts):
 def test_hierarchy_and_refresh_match_oracle(device):
     """Aggregates, transfers and the device-refreshed coarse operators
     (amg_galerkin + amg_m_matrix_diagonal) equal the oracle's bit for bit."""
-    a = oracle.poisson(26, 24, 22, 1.0, random=True, neumann=True)
+    a = oracle.pressure_problem(13, 12, 11).csr()
     popts = AmgOptions(aggregation_passes=2, nullspace=NullSpace.Constant,
                        structure=OperatorStructure.SymmetricMMatrix)
     host, matrix, solver = product_setup(device, a, SolverKind.AmgPcg, popts)
