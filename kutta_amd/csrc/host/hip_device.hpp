@@ -124,8 +124,53 @@ class HipDevice {
     ok(kh_reduce_into(dev_, mode, x, y, u, v, n, dst, offset));
   }
 
+  // Records everything enqueued by `body` into a replayable graph.
+  template <class F>
+  kh_graph* capture(F&& body) {
+    ok(kh_graph_begin(dev_));
+    try {
+      body();
+    } catch (...) {
+      kh_graph* g = nullptr;
+      if (kh_graph_end(dev_, &g) == KH_OK) kh_graph_destroy(g);
+      throw;
+    }
+    kh_graph* g = nullptr;
+    ok(kh_graph_end(dev_, &g));
+    return g;
+  }
+  void replay(kh_graph* g) { ok(kh_graph_launch(dev_, g)); }
+
  private:
   kh_device* dev_ = nullptr;
+};
+
+// Owning handle of a captured graph.
+class Graph {
+ public:
+  Graph() = default;
+  explicit Graph(kh_graph* g) : g_(g) {}
+  Graph(Graph&& o) noexcept : g_(o.g_) { o.g_ = nullptr; }
+  Graph& operator=(Graph&& o) noexcept {
+    if (this != &o) {
+      reset();
+      g_ = o.g_;
+      o.g_ = nullptr;
+    }
+    return *this;
+  }
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+  ~Graph() { reset(); }
+  kh_graph* get() const { return g_; }
+  explicit operator bool() const { return g_ != nullptr; }
+  void reset() {
+    if (g_) kh_graph_destroy(g_);
+    g_ = nullptr;
+  }
+
+ private:
+  kh_graph* g_ = nullptr;
 };
 
 template <class T>

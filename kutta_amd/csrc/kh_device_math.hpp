@@ -179,34 +179,104 @@ __device__ __forceinline__ double block_sum(double v, double* lds /* kBlock/64 *
   return total;
 }
 
-// Publishes this workgroup's N partial sums; returns true in the workgroup
-// that arrives last, with the grand totals (fixed summation order) in `total`
-// on thread 0. That workgroup also rearms the ticket for the next kernel.
+// Grid reductions are two launches: each workgroup stores its partial sums at
+// a fixed slot (no fences, no atomics), and a one-workgroup finisher sums the
+// slots in a fixed order after the kernel boundary has made them visible.
+// A device-scope fence per workgroup would write back and invalidate the L2 of
+// every XCD and costs far more than the second launch.
 template <int N>
-__device__ __forceinline__ bool finish_grid(const double (&mine)[N], double* partials,
-                                            unsigned* ticket, double (&total)[N]) {
-  __shared__ unsigned is_last;
-  __shared__ double lds[N][kBlock / 64];
+__device__ __forceinline__ void store_partials(const double (&mine)[N], double* partials) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < N; ++k) partials[blockIdx.x * N + k] = mine[k];
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    is_last = (t == gridDim.x - 1) ? 1u : 0u;
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  const unsigned nb = gridDim.x;
+}
+// Finisher body: the totals of `count` partial slots, valid on thread 0.
+template <int N>
+__device__ __forceinline__ void sum_partials(const double* partials, unsigned count,
+                                             double (&total)[N]) {
+  __shared__ double lds[N][kBlock / 64];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     double v = 0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-      v += __builtin_nontemporal_load(&partials[b * N + k]);
+    for (unsigned b = threadIdx.x; b < count; b += blockDim.x) v += partials[b * N + k];
     total[k] = block_sum(v, lds[k]);
   }
-  if (threadIdx.x == 0) atomicExch(ticket, 0u);
-  return true;
+}
+
+// --------------------------------------------------- prefetched row products
+// Entries of one CSR row are processed in chunks of kChunk: all column ids and
+// values of a chunk are loaded first, then all gathered x values, then the
+// compensated sum runs over the registers in entry order. The arithmetic is
+// the sequential loop's, but memory latency is paid per chunk instead of per
+// entry.
+constexpr int kChunk = 8;
+
+// x[j] for a gather; Prolonged adds the coarse correction of j's aggregate,
+// the value the reference's in-place prolongation would have stored.
+struct Plain {
+  const float* __restrict__ x;
+  __device__ __forceinline__ float operator()(int j) const { return x[j]; }
+};
+struct Prolonged {
+  const float* __restrict__ x;
+  const int* __restrict__ aggregate;
+  const float* __restrict__ coarse;
+  __device__ __forceinline__ float operator()(int j) const {
+    const int g = aggregate[j];
+    return g >= 0 ? x[j] + coarse[g] : x[j];
+  }
+};
+
+// Kahan-compensated sum of a[j] * x[col[j]] over [begin, end)
+// (kernels/linalg.kern.h:5-16).
+template <class X>
+__device__ __forceinline__ float row_product(int begin, int end, const int* __restrict__ cols,
+                                             const float* __restrict__ values, X gather) {
+  Kahan k;
+  for (int j = begin; j < end; j += kChunk) {
+    int c[kChunk];
+    float v[kChunk], xv[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (j + u < end) {
+        c[u] = cols[j + u];
+        v[u] = values[j + u];
+      }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (j + u < end) xv[u] = gather(c[u]);
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (j + u < end) k.add(v[u] * xv[u]);
+  }
+  return k.s;
+}
+
+// Compensated residual b - sum a[j] * x[col[j]] over [begin, end)
+// (kernels/linalg.kern.h:32-45).
+template <class X>
+__device__ __forceinline__ float row_residual(float b, int begin, int end,
+                                              const int* __restrict__ cols,
+                                              const float* __restrict__ values, X gather) {
+  RowResidual acc(b);
+  for (int j = begin; j < end; j += kChunk) {
+    int c[kChunk];
+    float v[kChunk], xv[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (j + u < end) {
+        c[u] = cols[j + u];
+        v[u] = values[j + u];
+      }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (j + u < end) xv[u] = gather(c[u]);
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (j + u < end) acc.add(v[u], xv[u]);
+  }
+  return acc.value();
 }
 
 // Maps workgroups so that each of the 8 XCDs (which receive workgroups

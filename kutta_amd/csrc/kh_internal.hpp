@@ -35,10 +35,9 @@ struct kh_device {
   int ordinal = 0;
   std::string name;
   hipStream_t stream = nullptr;
-  // Reduction scratch: per-block partial sums (two doubles per block for the
-  // fused dot pair) and a ticket counter the last block resets to zero.
+  // Reduction scratch: per-workgroup partial sums (two doubles per workgroup
+  // for the fused dot pair), consumed by a finisher launch.
   double* partials = nullptr;
-  unsigned* ticket = nullptr;
   // Host reduction results: device staging and pinned host copy.
   double* results = nullptr;
   double* results_host = nullptr;

@@ -20,10 +20,7 @@ __device__ __forceinline__ float spmv_row(int i, const int* __restrict__ rows,
                                           const int* __restrict__ cols,
                                           const float* __restrict__ values,
                                           const float* __restrict__ x) {
-  Kahan k;
-  const int end = rows[i + 1];
-  for (int j = rows[i]; j < end; ++j) k.add(values[j] * x[cols[j]]);
-  return k.s;
+  return row_product(rows[i], rows[i + 1], cols, values, Plain{x});
 }
 // kernels/linalg.kern.h:32-45
 __device__ __forceinline__ float residual_row(int i, const int* __restrict__ rows,
@@ -31,19 +28,7 @@ __device__ __forceinline__ float residual_row(int i, const int* __restrict__ row
                                               const float* __restrict__ values,
                                               const float* __restrict__ x,
                                               const float* __restrict__ b) {
-  RowResidual acc(b[i]);
-  const int end = rows[i + 1];
-  for (int j = rows[i]; j < end; ++j) acc.add(values[j], x[cols[j]]);
-  return acc.value();
-}
-// The same residual evaluated at x + P c without storing x + P c: each gathered
-// x[j] is prolonged on the fly, the float sum the reference's in-place prolong
-// would have stored (kernels/amg.kern.h:18-21).
-__device__ __forceinline__ float prolonged(const float* __restrict__ x,
-                                           const int* __restrict__ aggregate,
-                                           const float* __restrict__ coarse, int j) {
-  const int g = aggregate[j];
-  return g >= 0 ? x[j] + coarse[g] : x[j];
+  return row_residual(b[i], rows[i], rows[i + 1], cols, values, Plain{x});
 }
 
 __global__ void __launch_bounds__(kBlock) k_csr_spmv(int n, const int* __restrict__ rows,
@@ -378,12 +363,14 @@ __global__ void k_solver_project(int n, float* __restrict__ x, const float* __re
 }
 
 // ------------------------------------------------------------ fused kernels
-// q = A p (guarded spmv), then p.q reduced in FP64 and the alpha update, all in
-// one launch: csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha.
-__global__ void __launch_bounds__(kBlock) k_fused_spmv_dot_alpha(int n,
-    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
-    const float* __restrict__ p, float* __restrict__ q, float* scalars, int* status,
-    double* partials, unsigned* ticket) {
+// A fused reduction is a row kernel that stores one FP64 partial per
+// workgroup (store_partials) and a one-workgroup finisher that sums the slots
+// in order and applies the scalar update of the reference sequence it replaces.
+
+// q = A p, partials of p.q: csr_spmv_guarded + dot_into(p,q,2).
+__global__ void __launch_bounds__(kBlock) k_fused_spmv_dot(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ p,
+    float* __restrict__ q, const int* __restrict__ status, double* __restrict__ partials) {
   if (status[0] != 0) return;
   const int i = row_index();
   double mine[1] = {0.0};
@@ -394,17 +381,12 @@ __global__ void __launch_bounds__(kBlock) k_fused_spmv_dot_alpha(int n,
   }
   __shared__ double lds[kBlock / 64];
   mine[0] = block_sum(mine[0], lds);
-  double total[1];
-  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
-    pstore(scalars, 2, to_pair(total[0]));
-    pcg_alpha(scalars, status);
-  }
+  store_partials<1>(mine, partials);
 }
-// x += alpha p, r -= alpha q, then r.r and the convergence check:
-// solver_step + dot_into(r,r,18) + solver_check(pcg=1).
-__global__ void __launch_bounds__(kBlock) k_fused_step_dot_check(int n, float* __restrict__ x,
+// x += alpha p, r -= alpha q, partials of r.r: solver_step + dot_into(r,r,18).
+__global__ void __launch_bounds__(kBlock) k_fused_step_dot(int n, float* __restrict__ x,
     float* __restrict__ r, const float* __restrict__ p, const float* __restrict__ q,
-    float* scalars, int* status, double* partials, unsigned* ticket) {
+    const float* __restrict__ scalars, const int* __restrict__ status, double* __restrict__ partials) {
   if (status[0] != 0) return;
   const int i = row_index();
   const float alpha = pfloat(pload(scalars, 8));
@@ -417,15 +399,94 @@ __global__ void __launch_bounds__(kBlock) k_fused_step_dot_check(int n, float* _
   }
   __shared__ double lds[kBlock / 64];
   mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+// Final smoothing sweep of the preconditioner, partials of r.z:
+// amg_smooth + dot_into(r,z,4). Here b is the Krylov residual r.
+__global__ void __launch_bounds__(kBlock) k_fused_smooth_dot(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+    const int* __restrict__ status, double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float bi = b[i];
+    const float r = row_residual(bi, rows[i], rows[i + 1], cols, values, Plain{x});
+    const float z = x[i] + (2.0f / 3.0f) * inverse[i] * r;
+    next[i] = z;
+    mine[0] = double(bi) * double(z);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+// Smoothing sweep whose output is summed for the constant-nullspace
+// projection that follows: amg_smooth + sum_into(next).
+__global__ void __launch_bounds__(kBlock) k_fused_smooth_partial_sum(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ x, const float* __restrict__ b, const float* __restrict__ inverse,
+    float* __restrict__ next, const int* __restrict__ status, double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float r = residual_row(i, rows, cols, values, x, b);
+    const float z = x[i] + (2.0f / 3.0f) * inverse[i] * r;
+    next[i] = z;
+    mine[0] = double(z);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+// Mean removal, partials of r.z: solver_project + dot_into(r,z,4).
+__global__ void __launch_bounds__(kBlock) k_fused_project_dot(int n, float* __restrict__ z,
+    const float* __restrict__ sum, float count_hi, float count_lo, const float* __restrict__ r,
+    const int* __restrict__ status, double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float zi = z[i] - pfloat(pdiv(pload(sum, 0), {count_hi, count_lo}));
+    z[i] = zi;
+    mine[0] = double(r[i]) * double(zi);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+
+// Finishers: the FP64 total of `count` partials as a normalized pair, then the
+// reference's scalar kernel on it.
+enum class Finish { Alpha, Check, Beta, Pair };
+template <Finish F>
+__global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ partials,
+                                                   unsigned count, float* scalars, int* status,
+                                                   float* dst) {
+  if (status[0] != 0) return;
   double total[1];
-  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
-    pstore(scalars, 18, to_pair(total[0]));
+  sum_partials<1>(partials, count, total);
+  if (threadIdx.x != 0) return;
+  const Pair value = to_pair(total[0]);
+  if constexpr (F == Finish::Alpha) {
+    pstore(scalars, 2, value);
+    pcg_alpha(scalars, status);
+  } else if constexpr (F == Finish::Check) {
+    pstore(scalars, 18, value);
     solver_check_body(scalars, status, 1);
+  } else if constexpr (F == Finish::Beta) {
+    pstore(scalars, 4, value);
+    pcg_beta(scalars, status, 0);
+  } else {
+    pstore(dst, 0, value);
   }
 }
+
 // Restricted residual: each coarse row sums the compensated residuals of its
 // members, computed in place of the stored fine residual:
-// csr_residual_guarded + amg_restrict_guarded.
+// csr_residual_guarded + amg_restrict_guarded. Member rows are fetched four
+// at a time so their loads overlap.
 __global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
     const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
     const float* __restrict__ x, const float* __restrict__ b, const int* __restrict__ offsets,
@@ -434,8 +495,24 @@ __global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
   const int g = row_index();
   if (g >= n) return;
   Kahan k;
-  for (int m = offsets[g]; m < offsets[g + 1]; ++m)
-    k.add(residual_row(members[m], rows, cols, values, x, b));
+  const int first = offsets[g], last = offsets[g + 1];
+  for (int base = first; base < last; base += 4) {
+    int row[4], rb[4], re[4];
+    float bb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (base + u < last) row[u] = members[base + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (base + u < last) {
+        rb[u] = rows[row[u]];
+        re[u] = rows[row[u] + 1];
+        bb[u] = b[row[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (base + u < last) k.add(row_residual(bb[u], rb[u], re[u], cols, values, Plain{x}));
+  }
   coarse[g] = k.s;
 }
 // Prolongation followed by one smoothing sweep without storing x + P c:
@@ -448,75 +525,9 @@ __global__ void __launch_bounds__(kBlock) k_fused_prolong_smooth(int n,
   if (status[0] != 0) return;
   const int i = row_index();
   if (i >= n) return;
-  RowResidual acc(b[i]);
-  for (int j = rows[i]; j < rows[i + 1]; ++j)
-    acc.add(values[j], prolonged(x, aggregate, coarse, cols[j]));
-  next[i] = prolonged(x, aggregate, coarse, i) + (2.0f / 3.0f) * inverse[i] * acc.value();
-}
-// Final smoothing sweep of the preconditioner, then r.z and the beta update:
-// amg_smooth + dot_into(r,z,4) + solver_pcg_beta(flexible=0). Here b is the
-// Krylov residual r that the V-cycle preconditions.
-__global__ void __launch_bounds__(kBlock) k_fused_smooth_dot_beta(int n,
-    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
-    const float* __restrict__ x, const float* __restrict__ b, const float* __restrict__ inverse,
-    float* __restrict__ next, int* status, float* scalars, double* partials, unsigned* ticket) {
-  if (status[0] != 0) return;
-  const int i = row_index();
-  double mine[1] = {0.0};
-  if (i < n) {
-    const float r = residual_row(i, rows, cols, values, x, b);
-    const float z = x[i] + (2.0f / 3.0f) * inverse[i] * r;
-    next[i] = z;
-    mine[0] = double(b[i]) * double(z);
-  }
-  __shared__ double lds[kBlock / 64];
-  mine[0] = block_sum(mine[0], lds);
-  double total[1];
-  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
-    pstore(scalars, 4, to_pair(total[0]));
-    pcg_beta(scalars, status, 0);
-  }
-}
-// Smoothing sweep whose output is summed for the constant-nullspace
-// projection that follows: amg_smooth + sum_into(next -> sum[0..1]).
-__global__ void __launch_bounds__(kBlock) k_fused_smooth_sum(int n, const int* __restrict__ rows,
-    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
-    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
-    const int* status, float* sum, double* partials, unsigned* ticket) {
-  if (status[0] != 0) return;
-  const int i = row_index();
-  double mine[1] = {0.0};
-  if (i < n) {
-    const float r = residual_row(i, rows, cols, values, x, b);
-    const float z = x[i] + (2.0f / 3.0f) * inverse[i] * r;
-    next[i] = z;
-    mine[0] = double(z);
-  }
-  __shared__ double lds[kBlock / 64];
-  mine[0] = block_sum(mine[0], lds);
-  double total[1];
-  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0)
-    pstore(sum, 0, to_pair(total[0]));
-}
-// Mean removal, r.z and beta: solver_project + dot_into(r,z,4) + solver_pcg_beta(0).
-__global__ void __launch_bounds__(kBlock) k_fused_project_dot_beta(int n, float* __restrict__ z,
-    const float* __restrict__ sum, float count_hi, float count_lo, const float* __restrict__ r,
-    float* scalars, int* status, double* partials, unsigned* ticket) {
-  if (status[0] != 0) return;
-  const int i = row_index();
-  double mine[1] = {0.0};
-  if (i < n) {
-    const float zi = z[i] - pfloat(pdiv(pload(sum, 0), {count_hi, count_lo}));
-    z[i] = zi;
-    mine[0] = double(r[i]) * double(zi);
-  }
-  __shared__ double lds[kBlock / 64];
-  mine[0] = block_sum(mine[0], lds);
-  double total[1];
-  if (finish_grid<1>(mine, partials, ticket, total) && threadIdx.x == 0) {
-    pstore(scalars, 4, to_pair(total[0]));
-    pcg_beta(scalars, status, 0);
-  }
+  const Prolonged gather{x, aggregate, coarse};
+  const float r = row_residual(b[i], rows[i], rows[i + 1], cols, values, gather);
+  next[i] = gather(i) + (2.0f / 3.0f) * inverse[i] * r;
 }
 
 // fill(x, 0) followed by amg_smooth_zero, reading the zero iterate as the
@@ -672,20 +683,25 @@ void l_solver_bicg_step(kh_device* d, int n, const kh_arg* a) {
 void l_solver_project(kh_device* d, int n, const kh_arg* a) {
   k_solver_project<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2), f32(a, 3));
 }
-// Fused reductions use one workgroup per 256 rows; the partials scratch bounds n.
-void check_reduce_grid(int n) {
-  if (blocks_for(std::size_t(n)) > unsigned(kMaxReduceBlocks))
+// Fused reductions use one partial slot per workgroup of 256 rows.
+unsigned reduce_grid(int n) {
+  const unsigned blocks = blocks_for(std::size_t(n));
+  if (blocks > unsigned(kMaxReduceBlocks))
     throw std::runtime_error("fused reduction kernel: n exceeds 16,777,216 rows");
+  return blocks;
 }
+#define FINISH dim3(1), dim3(kBlock), 0, d->stream
 void l_fused_spmv_dot_alpha(kh_device* d, int n, const kh_arg* a) {
-  check_reduce_grid(n);
-  k_fused_spmv_dot_alpha<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KF(5),
-                                      static_cast<int*>(buf(a, 6)), d->partials, d->ticket);
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 6));
+  k_fused_spmv_dot<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), st, d->partials);
+  k_finish<Finish::Alpha><<<FINISH>>>(d->partials, g, KF(5), st, nullptr);
 }
 void l_fused_step_dot_check(kh_device* d, int n, const kh_arg* a) {
-  check_reduce_grid(n);
-  k_fused_step_dot_check<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), KF(4),
-                                      static_cast<int*>(buf(a, 5)), d->partials, d->ticket);
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 5));
+  k_fused_step_dot<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), KCF(4), st, d->partials);
+  k_finish<Finish::Check><<<FINISH>>>(d->partials, g, KF(4), st, nullptr);
 }
 void l_fused_residual_restrict(kh_device* d, int n, const kh_arg* a) {
   k_fused_residual_restrict<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6),
@@ -696,22 +712,27 @@ void l_fused_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
                                       KB(7), KCF(8), KB(9));
 }
 void l_fused_smooth_dot_beta(kh_device* d, int n, const kh_arg* a) {
-  check_reduce_grid(n);
-  k_fused_smooth_dot_beta<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
-                                       static_cast<int*>(buf(a, 7)), KF(8), d->partials, d->ticket);
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 7));
+  k_fused_smooth_dot<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), st,
+                                  d->partials);
+  k_finish<Finish::Beta><<<FINISH>>>(d->partials, g, KF(8), st, nullptr);
 }
 void l_fused_smooth_sum(kh_device* d, int n, const kh_arg* a) {
-  check_reduce_grid(n);
-  k_fused_smooth_sum<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7),
-                                  KF(8), d->partials, d->ticket);
+  const unsigned g = reduce_grid(n);
+  int* st = const_cast<int*>(KB(7));
+  k_fused_smooth_partial_sum<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
+                                          st, d->partials);
+  k_finish<Finish::Pair><<<FINISH>>>(d->partials, g, nullptr, st, KF(8));
 }
 void l_fused_smooth_from_zero(kh_device* d, int n, const kh_arg* a) {
   k_fused_smooth_from_zero<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6));
 }
 void l_fused_project_dot_beta(kh_device* d, int n, const kh_arg* a) {
-  check_reduce_grid(n);
-  k_fused_project_dot_beta<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2), f32(a, 3), KCF(4), KF(5),
-                                        static_cast<int*>(buf(a, 6)), d->partials, d->ticket);
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 6));
+  k_fused_project_dot<<<GRID(n)>>>(n, KF(0), KCF(1), f32(a, 2), f32(a, 3), KCF(4), st, d->partials);
+  k_finish<Finish::Beta><<<FINISH>>>(d->partials, g, KF(5), st, nullptr);
 }
 
 }  // namespace

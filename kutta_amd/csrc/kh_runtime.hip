@@ -64,11 +64,11 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Writes pair(s) to dst (into mode) or a double to result (host mode).
+// Grid-stride FP64 partial sums (or NaN-propagating maxima) per workgroup.
 template <int M>
-__global__ void __launch_bounds__(kBlock) k_reduce(const float* __restrict__ x,
+__global__ void __launch_bounds__(kBlock) k_reduce_partials(const float* __restrict__ x,
     const float* __restrict__ y, const float* __restrict__ u, const float* __restrict__ v,
-    std::size_t n, float* dst, double* result, double* partials, unsigned* ticket) {
+    std::size_t n, double* __restrict__ partials) {
   double a = 0.0, b = 0.0;
   const std::size_t stride = std::size_t(gridDim.x) * kBlock;
   for (std::size_t i = std::size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
@@ -78,7 +78,6 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const float* __restrict__ x,
     if constexpr (M == kDot2) b += double(u[i]) * double(v[i]);
   }
   __shared__ double lds[2][kBlock / 64];
-  __shared__ unsigned is_last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if constexpr (M == kMaxAbs) {
     a = wave_max(a);
@@ -88,31 +87,37 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const float* __restrict__ x,
       double m = 0.0;
       for (int w = 0; w < kBlock / 64; ++w) m = nan_max(m, lds[0][w]);
       partials[blockIdx.x * 2] = m;
-      __threadfence();
-      is_last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double m = 0.0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock)
-      m = nan_max(m, __builtin_nontemporal_load(&partials[k * 2]));
-    m = wave_max(m);
-    if (lane == 0) lds[0][wave] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < kBlock / 64; ++w) t = nan_max(t, lds[0][w]);
-      *result = t;
-      atomicExch(ticket, 0u);
-    }
-    return;
   } else {
     double mine[2] = {a, b};
     mine[0] = block_sum(mine[0], lds[0]);
     if constexpr (M == kDot2) mine[1] = block_sum(mine[1], lds[1]);
+    store_partials<2>(mine, partials);
+  }
+}
+
+// One workgroup combines the partials in slot order; writes pair(s) to dst
+// (asynchronous reductions) or the double to result (host reductions).
+template <int M>
+__global__ void __launch_bounds__(kBlock) k_reduce_finish(const double* __restrict__ partials,
+    unsigned count, float* dst, double* result) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if constexpr (M == kMaxAbs) {
+    __shared__ double lds[kBlock / 64];
+    double m = 0.0;
+    for (unsigned k = threadIdx.x; k < count; k += kBlock) m = nan_max(m, partials[k * 2]);
+    m = wave_max(m);
+    if (lane == 0) lds[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) t = nan_max(t, lds[w]);
+      *result = t;
+    }
+  } else {
     double total[2];
-    if (!finish_grid<2>(mine, partials, ticket, total) || threadIdx.x != 0) return;
+    sum_partials<2>(partials, count, total);
+    if (threadIdx.x != 0) return;
     if (result) {
       *result = total[0];
     } else {
@@ -120,6 +125,14 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const float* __restrict__ x,
       if constexpr (M == kDot2) pstore(dst, 2, to_pair(total[1]));
     }
   }
+}
+
+template <int M>
+void reduce(kh_device* d, const float* x, const float* y, const float* u, const float* v,
+            std::size_t n, float* dst, double* result) {
+  const unsigned g = reduce_blocks(n);
+  k_reduce_partials<M><<<dim3(g), dim3(kBlock), 0, d->stream>>>(x, y, u, v, n, d->partials);
+  k_reduce_finish<M><<<dim3(1), dim3(kBlock), 0, d->stream>>>(d->partials, g, dst, result);
 }
 
 }  // namespace
@@ -134,24 +147,16 @@ void launch_fill32(kh_device* d, void* dst, uint32_t word, std::size_t count) {
 
 void launch_reduce_into(kh_device* d, int mode, const float* x, const float* y, const float* u,
                         const float* v, std::size_t n, float* dst) {
-  const dim3 grid(reduce_blocks(n)), block(kBlock);
-  if (mode == KH_INTO_SUM)
-    k_reduce<kSum><<<grid, block, 0, d->stream>>>(x, y, u, v, n, dst, nullptr, d->partials, d->ticket);
-  else if (mode == KH_INTO_DOT)
-    k_reduce<kDot><<<grid, block, 0, d->stream>>>(x, y, u, v, n, dst, nullptr, d->partials, d->ticket);
-  else
-    k_reduce<kDot2><<<grid, block, 0, d->stream>>>(x, y, u, v, n, dst, nullptr, d->partials, d->ticket);
+  if (mode == KH_INTO_SUM) reduce<kSum>(d, x, y, u, v, n, dst, nullptr);
+  else if (mode == KH_INTO_DOT) reduce<kDot>(d, x, y, u, v, n, dst, nullptr);
+  else reduce<kDot2>(d, x, y, u, v, n, dst, nullptr);
 }
 
 void launch_reduce_host(kh_device* d, int kind, const float* x, const float* y, std::size_t n,
                         double* result) {
-  const dim3 grid(reduce_blocks(n)), block(kBlock);
-  if (kind == KH_REDUCE_SUM)
-    k_reduce<kSum><<<grid, block, 0, d->stream>>>(x, y, x, y, n, nullptr, result, d->partials, d->ticket);
-  else if (kind == KH_REDUCE_DOT)
-    k_reduce<kDot><<<grid, block, 0, d->stream>>>(x, y, x, y, n, nullptr, result, d->partials, d->ticket);
-  else
-    k_reduce<kMaxAbs><<<grid, block, 0, d->stream>>>(x, y, x, y, n, nullptr, result, d->partials, d->ticket);
+  if (kind == KH_REDUCE_SUM) reduce<kSum>(d, x, y, x, y, n, nullptr, result);
+  else if (kind == KH_REDUCE_DOT) reduce<kDot>(d, x, y, x, y, n, nullptr, result);
+  else reduce<kMaxAbs>(d, x, y, x, y, n, nullptr, result);
 }
 
 namespace {
@@ -213,11 +218,11 @@ int kh_create(int ordinal, kh_device** out) {
     d->ordinal = ordinal;
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
-    d->name = prop.name;
+    d->name = prop.name[0] ? std::string(prop.name) : std::string();
+    if (d->name.empty() || d->name.find("gfx") == std::string::npos)
+      d->name += (d->name.empty() ? "" : " ") + std::string(prop.gcnArchName);
     check(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipMalloc(&d->partials, sizeof(double) * 2 * kMaxReduceBlocks), "hipMalloc partials");
-    check(hipMalloc(&d->ticket, sizeof(unsigned) * 64), "hipMalloc ticket");
-    check(hipMemset(d->ticket, 0, sizeof(unsigned) * 64), "hipMemset ticket");
     check(hipMalloc(&d->results, sizeof(double) * kMaxHostResults), "hipMalloc results");
     check(hipHostMalloc(&d->results_host, sizeof(double) * kMaxHostResults, hipHostMallocDefault),
           "hipHostMalloc results");
@@ -238,7 +243,6 @@ void kh_destroy(kh_device* d) {
     }
   for (auto e : d->event_pool) (void)hipEventDestroy(e);
   (void)hipFree(d->partials);
-  (void)hipFree(d->ticket);
   (void)hipFree(d->results);
   (void)hipHostFree(d->results_host);
   (void)hipStreamDestroy(d->stream);
