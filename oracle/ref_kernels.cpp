@@ -13,6 +13,7 @@ namespace refk {
 #include "kutta/kernels/amg.kern.h"
 #include "kutta/kernels/blas.kern.h"
 #include "kutta/kernels/scalars.kern.h"
+#include "kutta/kernels/fv_flow.kern.h"
 }  // namespace refk
 
 #define LOOP(name, ...) for (int i = 0; i < n; ++i) refk::name(i, __VA_ARGS__)
@@ -68,4 +69,11 @@ void ref_solver_project(int n, float* x, const float* sum, float hi, float lo) {
   LOOP(solver_project, x, sum, hi, lo);
 }
 float ref_kadd_up(float a, float b) { return kadd_up(a, b); }
+void ref_fv_pressure_assemble(int n, const int* offsets, const int* faces, const float* sign,
+                              const int* adjacent, const int* entry, const int* rows,
+                              const int* diag, const float* coefficient, const float* flux,
+                              float* values, float* rhs, int reference) {
+  LOOP(fv_pressure_assemble, offsets, faces, sign, adjacent, entry, rows, diag, coefficient, flux,
+       values, rhs, reference);
+}
 }

@@ -240,3 +240,25 @@ def test_pair_reduction_exact_under_cancellation():
     pair = oracle.reduce_into("sum", y)
     total = 1 + (n - 1) * 2.0 ** -24
     assert float(pair[0]) + float(pair[1]) == total
+
+
+@needs_ref
+@pytest.mark.parametrize("reference", [-1, -2, 5])
+@pytest.mark.parametrize("tet", [True, False])
+def test_pressure_assembly_bitwise_vs_reference(tet, reference):
+    """fv_pressure_assemble (kernels/fv_flow.kern.h:164-183) on a perturbed,
+    renumbered box with a random flux, graded coefficients and garbage initial
+    values: the oracle's values and right-hand side equal the reference body's."""
+    b = oracle.pressure_problem(7, 6, 5, tet=tet, amplitude=0.2)
+    n = b["row_ptr"].size - 1
+    pin = int(b["reference"][0]) if reference == -2 else reference
+    rng = np.random.default_rng(11)
+    coefficient = b["diffusion"] * np.exp(rng.uniform(-6, 6, b["diffusion"].size)).astype(np.float32)
+    flux = rng.standard_normal(b["diffusion"].size).astype(np.float32)
+    values = rng.standard_normal(b["col"].size).astype(np.float32)
+    rhs = np.full(n, np.nan, np.float32)
+    got, want = both("fv_pressure_assemble", n, b["cell_face_offsets"], b["cell_faces"],
+                     b["cell_face_sign"], b["adjacent"], b["matrix_entry"], b["row_ptr"], b["diag"],
+                     coefficient, flux, values, rhs, pin, outputs=[9, 10])
+    assert_bitwise(got, want)
+    assert np.all(np.isfinite(got[0])) and np.all(np.isfinite(got[1]))
