@@ -17,7 +17,8 @@ Prints one JSON line with the metric, the roofline figure of the fine-level
 l1-Jacobi smoothing kernel (the largest cost) and the serial CPU oracle timed
 on one host core beside it.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--sweeps 5] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--sweeps 5] [--tail T] [--no-cpu-baseline]
+    python bench.py --copy-probe BYTES [--steps K]   # HBM counter calibration
 """
 from __future__ import annotations
 
@@ -79,6 +80,31 @@ def cpu_baseline(problem, b, sweeps, gpu_stats):
     }, x
 
 
+def copy_probe(nbytes: int, repeats: int) -> int:
+    """Times `repeats` device-to-device copies of nbytes through the device ABI
+    (kh_copy) and prints one JSON line. A copy moves exactly nbytes in and
+    nbytes out of HBM, so profiler byte counters read on it calibrate the same
+    counters read on the solver's kernels."""
+    import numpy as np
+
+    import kutta_amd
+    dev = kutta_amd.Device(0)
+    words = nbytes // 4
+    src, dst = dev.alloc(words), dev.alloc(words)
+    src.fill(np.float32(1))
+    dev.copy(dst, src)
+    dev.synchronize()
+    t = time.perf_counter()
+    for _ in range(repeats):
+        dev.copy(dst, src)
+    dev.synchronize()
+    seconds = (time.perf_counter() - t) / repeats
+    print(json.dumps({"probe": "device copy", "bytes": words * 4, "repeats": repeats,
+                      "us_per_copy": round(seconds * 1e6, 3),
+                      "GB/s (read+write)": round(2 * words * 4 / seconds / 1e9, 1)}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,8 +112,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=55)
     ap.add_argument("--sweeps", type=int, default=5)
+    ap.add_argument("--tail", type=int, default=-1,
+                    help="first AMG level of the one-workgroup coarse-grid tail (-1 auto, 0 off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only time device copies of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a run)", file=sys.stderr)
@@ -109,12 +141,13 @@ def main() -> int:
     t = time.perf_counter()
     solver = kutta_amd.make_amg_solver(SolverKind.AmgPcg, dev, problem.host, options)
     setup_amg_s = time.perf_counter() - t
+    tail_level = solver.set_tail(args.tail)
     solver.refresh(problem)
     b_host = rhs_for(problem.n_cells, problem.reference)
     b = dev.upload(b_host)
     x = dev.alloc(problem.n_cells)
     control = SolveControl(rel_tol=1e-6, abs_tol=0.0, max_iter=2000)
-    print(f"[bench] {problem.n_cells} cells, nnz {problem.nnz}, {solver.describe()}", file=sys.stderr,
+    print(f"[bench] {problem.n_cells} cells, nnz {problem.nnz}, {solver.describe()}, tail level {tail_level}", file=sys.stderr,
           flush=True)
 
     for _ in range(args.warmup):
@@ -188,6 +221,7 @@ def main() -> int:
             "nnz": m,
             "hierarchy": solver.describe(),
             "levels": solver.levels(),
+            "coarse_tail_level": tail_level,
             "iterations": st.iterations,
             "relative_residual": st.final_residual / st.initial_residual,
             "converged": st.converged,

@@ -104,6 +104,12 @@ long ka_solver_level_values(void* solver, int level, float* out);
 /* 1: fused MI355X kernels (default); 0: the reference kernel sequence. */
 int ka_solver_set_fused(void* solver, int fused);
 int ka_solver_host_waits(void* solver);
+/* First AMG level of the one-workgroup coarse-grid tail (KH_FUSED_COARSE_TAIL)
+ * on the fused path: first < 0 picks it automatically (the default), 0 turns
+ * the tail off, k >= 1 requests level k (an error if levels k.. do not fit).
+ * *level receives the level in use, 0 for none. No reference counterpart: the
+ * iterates are those of the per-level launches either way. */
+int ka_solver_set_tail(void* solver, int first, int32_t* level);
 
 #ifdef __cplusplus
 }

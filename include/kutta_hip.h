@@ -124,9 +124,47 @@ typedef enum kh_kernel {
   KH_SOLVER_BICG_STEP,           /* :142-149 x,r,z,zs,s,t,scalars,status */
   KH_SOLVER_PROJECT,             /* :150-152 x,sum,count_hi,count_lo */
 
-  /* kernels/fv_flow.kern.h (finite-volume face loops and assembly) */
+  /* kernels/fv_flow.kern.h (finite-volume face loops and assembly); arguments
+     in the reference's parameter order. */
   KH_FV_PRESSURE_ASSEMBLE = 100, /* :164-183 offsets,faces,sign,adjacent,entry,rows,diag,
                                     coefficient,flux,values,rhs,reference */
+  KH_FV_VELOCITY_BOUNDARY,       /* :5-32   n = boundary faces */
+  KH_FV_PRESSURE_BOUNDARY,       /* :34-41  n = boundary faces */
+  KH_FV_PRESSURE_FACES,          /* :43-65  n = faces */
+  KH_FV_MOMENTUM_SOURCE,         /* :67-78  n = cells */
+  KH_FV_MOMENTUM_FACES,          /* :83-109 n = faces */
+  KH_FV_CONVECTIVE_OUTFLOW,      /* :111-116 n = cells */
+  KH_FV_PECLET_SIDE_VELOCITY,    /* :122-133 n = internal faces */
+  KH_FV_RHIE_CHOW,               /* :139-162 n = faces */
+  KH_FV_CORRECT_FLUX,            /* :185-190 n = faces */
+  KH_FV_CORRECT_VELOCITY,        /* :192-202 n = cells */
+  KH_FV_MONITOR_CELL,            /* :204-215 n = cells */
+  KH_FV_SPEED,                   /* :217-219 n = cells */
+  KH_FV_CONTINUITY,              /* :221-226 n = cells */
+  KH_FV_WALL_PRESSURE_FORCE,     /* :228-233 n = boundary faces */
+  KH_FV_WALL_VISCOUS_FORCE,      /* :235-257 n = boundary faces */
+  KH_FV_FLUENT_MOMENTUM_SCALE,   /* :261-266 n = cells */
+  KH_FV_PHYSICAL_TIME_TERMS,     /* :268-276 n = cells */
+  KH_FV_FACE_TIME_DEFECT,        /* :278-281 n = faces */
+  KH_FV_FACE_TIME_FLUX,          /* :283-295 n = faces */
+  KH_FV_CELL_MASS_ERROR,         /* :297-299 n = cells */
+  /* kernels/fv_gradient.kern.h (least-squares gradients and limiters) */
+  KH_FV_GRADIENT,                /* :6-23 */
+  KH_FV_PAIRED_GRADIENT,         /* :28-56 */
+  KH_FV_RANGE_SHIFT,             /* :72-74 */
+  KH_FV_LIMITER,                 /* :76-94 */
+  KH_FV_PAIRED_LIMITER,          /* :103-123 */
+  KH_FV_PAIRED_GRADIENT_LIMITER, /* :127-145 */
+  KH_FV_VECTOR_GRADIENT,         /* :147-166 */
+  KH_FV_SCALE_GRADIENT,          /* :168-171 */
+  KH_FV_LIMITER_SCALE,           /* :173-180 */
+  KH_FV_PAIRED_LIMITER_SCALE,    /* :182-189 */
+  KH_FV_PAIRED_GRADIENT_LIMITER_SCALE,/* :191-199 */
+  KH_FV_JUMP_GRADIENT,           /* :202-221 */
+  KH_FV_FACE_JUMP,               /* :227-240 */
+  KH_FV_GATHER_JUMP_GRADIENT,    /* :242-254 */
+  KH_FV_PAIRED_JUMP_GRADIENT,    /* :258-292 */
+  KH_FV_LSQ_RESTORE_INVERSE,     /* :296-299 */
 
   /* Fused MI355X kernels (no reference counterpart). */
   KH_FUSED_SPMV_DOT_ALPHA = 200, /* csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha:
@@ -145,8 +183,32 @@ typedef enum kh_kernel {
                                     z,sum,count_hi,count_lo,r,scalars,status */
   KH_FUSED_SMOOTH_FROM_ZERO,     /* fill(x,0) + amg_smooth_zero, without storing the zeros:
                                     rows,cols,values,b,inverse,next,status */
+  KH_FUSED_COARSE_TAIL,          /* the whole V-cycle (linalg/amg.cpp:283-310) from one level
+                                    down to the coarsest in one workgroup, iterates in LDS:
+                                    levels,count,pre,post,coarse_sweeps,nullspace,b,x,status
+                                    (n = 1; levels is a device array of kh_tail_level) */
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
+
+/* One level of KH_FUSED_COARSE_TAIL: the level's CSR operator and l1 inverse
+ * diagonal, and the aggregation transfer to the next level (ignored on the
+ * last). Levels run in LDS, so the tail accepts at most KH_TAIL_MAX_LEVELS
+ * levels of at most KH_TAIL_MAX_ROWS rows each, with 2 * (sum of rows) at most
+ * KH_TAIL_LDS_FLOATS; a table outside these limits makes the kernel a no-op. */
+#define KH_TAIL_MAX_LEVELS 16
+#define KH_TAIL_MAX_ROWS 8192
+#define KH_TAIL_LDS_FLOATS 24576
+typedef struct kh_tail_level {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const float* val;
+  const float* inverse;
+  const int32_t* member_offsets;
+  const int32_t* members;
+  const int32_t* aggregate;
+  int32_t rows;
+  int32_t coarse_rows;
+} kh_tail_level;
 
 /* Device::launch (device.hpp:205; cuda_device.cu:132-140): runs kernel `kernel`
  * for global indices [0, n). n == 0 is a no-op; n > INT32_MAX, an unknown id or

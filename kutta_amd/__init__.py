@@ -111,6 +111,7 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_level_values.restype = C.c_long
         lib.ka_solver_set_fused.argtypes = [vp, i32]
         lib.ka_solver_host_waits.argtypes = [vp]
+        lib.ka_solver_set_tail.argtypes = [vp, i32, pi32]
         lib.ka_aggregate.argtypes = [vp, pi32, dbl, C.POINTER(vp)]
         lib.ka_coarsening_array.argtypes = [vp, i32, pi32]
         lib.ka_coarsening_array.restype = C.c_long
@@ -191,6 +192,43 @@ class Kernel(enum.IntEnum):
     solver_bicg_difference = 41
     solver_bicg_step = 42
     solver_project = 43
+    fv_pressure_assemble = 100
+    fv_velocity_boundary = 101
+    fv_pressure_boundary = 102
+    fv_pressure_faces = 103
+    fv_momentum_source = 104
+    fv_momentum_faces = 105
+    fv_convective_outflow = 106
+    fv_peclet_side_velocity = 107
+    fv_rhie_chow = 108
+    fv_correct_flux = 109
+    fv_correct_velocity = 110
+    fv_monitor_cell = 111
+    fv_speed = 112
+    fv_continuity = 113
+    fv_wall_pressure_force = 114
+    fv_wall_viscous_force = 115
+    fv_fluent_momentum_scale = 116
+    fv_physical_time_terms = 117
+    fv_face_time_defect = 118
+    fv_face_time_flux = 119
+    fv_cell_mass_error = 120
+    fv_gradient = 121
+    fv_paired_gradient = 122
+    fv_range_shift = 123
+    fv_limiter = 124
+    fv_paired_limiter = 125
+    fv_paired_gradient_limiter = 126
+    fv_vector_gradient = 127
+    fv_scale_gradient = 128
+    fv_limiter_scale = 129
+    fv_paired_limiter_scale = 130
+    fv_paired_gradient_limiter_scale = 131
+    fv_jump_gradient = 132
+    fv_face_jump = 133
+    fv_gather_jump_gradient = 134
+    fv_paired_jump_gradient = 135
+    fv_lsq_restore_inverse = 136
     fused_spmv_dot_alpha = 200
     fused_step_dot_check = 201
     fused_residual_restrict = 202
@@ -199,6 +237,7 @@ class Kernel(enum.IntEnum):
     fused_smooth_sum = 205
     fused_project_dot_beta = 206
     fused_smooth_from_zero = 207
+    fused_coarse_tail = 208
 
 
 class Buffer:
@@ -549,6 +588,13 @@ class LinearSolver:
     def host_waits(self) -> int:
         return host_lib().ka_solver_host_waits(self.handle)
 
+    def set_tail(self, first: int = -1) -> int:
+        """First level of the one-workgroup coarse-grid tail: -1 automatic,
+        0 off, k >= 1 level k. Returns the level in use (0: none)."""
+        level = C.c_int32(0)
+        _check_host(host_lib().ka_solver_set_tail(self.handle, int(first), C.byref(level)))
+        return level.value
+
     def __del__(self):
         try:
             if self.device._alive:
@@ -573,6 +619,10 @@ def _named_array(fn, handle, name: str) -> np.ndarray:
 
 
 C2_SEED = 20261202
+# Limits of the one-workgroup coarse-grid tail (KH_TAIL_* in kutta_hip.h).
+TAIL_MAX_ROWS = 8192
+TAIL_MAX_LEVELS = 16
+TAIL_LDS_FLOATS = 24576
 
 
 class MeshSetup:

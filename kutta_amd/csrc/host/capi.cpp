@@ -181,5 +181,11 @@ int ka_solver_set_fused(void* s, int fused) {
   return guard([&] { static_cast<Solver*>(s)->solver.set_fused(fused != 0); });
 }
 int ka_solver_host_waits(void* s) { return static_cast<Solver*>(s)->solver.host_waits(); }
+int ka_solver_set_tail(void* s, int first, int* level) {
+  return guard([&] {
+    const int used = static_cast<Solver*>(s)->solver.set_tail(first);
+    if (level) *level = used;
+  });
+}
 
 }  // extern "C"

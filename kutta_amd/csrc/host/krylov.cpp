@@ -64,6 +64,11 @@ struct KrylovSolver::Impl {
   int previous_iterations = 0;
   bool verified_current = false;
   int waits = 0;
+  // Coarse-grid tail: levels [tail_first, end) run as one KH_FUSED_COARSE_TAIL
+  // launch (0: off). tail_request < 0 picks the level automatically.
+  int tail_request = -1;
+  int tail_first = 0;
+  DeviceArray<kh_tail_level> tail_table;
 
   Impl(HipDevice& d, SolverKind k, const CsrHost& a, const AmgOptions& o)
       : dev(d), kind(k), options(o), amg(k == SolverKind::AmgPcg || k == SolverKind::AmgBicgstab),
@@ -80,6 +85,7 @@ struct KrylovSolver::Impl {
                  "AMG-PCG requires equal pre/post sweeps");
       if (options.structure == Structure::SymmetricMMatrix) check_m_matrix(a);
       build_hierarchy(a);
+      plan_tail();
     } else {
       jacobi_inverse = to_device(dev, diagonal_values(a, false, false));
       jacobi_invalid = DeviceArray<float>(dev, n);
@@ -141,6 +147,58 @@ struct KrylovSolver::Impl {
       current = &owned;
     }
     complexity = a.nnz() ? double(total) / a.nnz() : 0;
+  }
+
+  // Whether levels [first, end) fit one KH_FUSED_COARSE_TAIL workgroup.
+  bool tail_fits(std::size_t first) const {
+    if (first < 1 || first >= levels.size() || levels.size() - first > KH_TAIL_MAX_LEVELS) return false;
+    std::size_t floats = 0;
+    for (std::size_t i = first; i < levels.size(); ++i) {
+      if (levels[i].rows > KH_TAIL_MAX_ROWS) return false;
+      floats += 2 * std::size_t(levels[i].rows);
+    }
+    return floats <= KH_TAIL_LDS_FLOATS;
+  }
+
+  // Picks the tail's first level: the requested one, or else the first level
+  // that fits and whose operator is small enough that one compute unit
+  // sweeps it faster than a grid launch completes.
+  void plan_tail() {
+    static constexpr Index kAutoNnz = 131072;
+    tail_first = 0;
+    tail_table = DeviceArray<kh_tail_level>();
+    if (!amg || tail_request == 0) return;
+    if (tail_request > 0) {
+      KA_REQUIRE(tail_fits(std::size_t(tail_request)),
+                 "AMG: coarse tail level " + std::to_string(tail_request) + " does not fit one workgroup");
+      tail_first = tail_request;
+    } else {
+      for (std::size_t i = 1; i < levels.size(); ++i)
+        if (levels[i].nnz <= kAutoNnz && tail_fits(i)) {
+          tail_first = int(i);
+          break;
+        }
+    }
+    if (tail_first == 0) return;
+    std::vector<kh_tail_level> table;
+    for (std::size_t i = std::size_t(tail_first); i < levels.size(); ++i) {
+      const Level& l = levels[i];
+      const DeviceCsr m = l.matrix.view();
+      kh_tail_level t{};
+      t.row_ptr = m.row_ptr;
+      t.col = m.col;
+      t.val = m.val;
+      t.inverse = l.l1.data();
+      if (l.has_transfer) {
+        t.member_offsets = l.member_offsets.data();
+        t.members = l.members.data();
+        t.aggregate = l.aggregate.data();
+        t.coarse_rows = l.coarse_rows;
+      }
+      t.rows = l.rows;
+      table.push_back(t);
+    }
+    tail_table = to_device(dev, table);
   }
 
   DeviceCsr level_operator(std::size_t i, const DeviceCsr& fine) const {
@@ -214,6 +272,12 @@ struct KrylovSolver::Impl {
     float* w = l.r.data();
     const int* st = status.data();
     CycleResult result;
+    if (fused && tail_first > 0 && int(index) == tail_first) {
+      dev.launch(KH_FUSED_COARSE_TAIL, 1,
+                 {tail_table, int(levels.size() - index), options.pre_sweeps, options.post_sweeps,
+                  options.coarse_sweeps, nullspace ? 1 : 0, b, x, st});
+      return result;
+    }
     // The reference clears x first; only the unfused sequence reads that zero.
     if (!fused) dev.fill(x, 0.0f, a.n_rows);
     const bool coarsest = index + 1 == levels.size();
@@ -539,6 +603,19 @@ SolveStats KrylovSolver::solve(const DeviceCsr& a, float* x, const float* b, con
   return impl_->solve(a, x, b, c);
 }
 void KrylovSolver::set_fused(bool fused) { impl_->fused = fused; }
+int KrylovSolver::set_tail(int first) {
+  const int previous = impl_->tail_request;
+  impl_->tail_request = first;
+  try {
+    impl_->plan_tail();
+  } catch (...) {
+    impl_->tail_request = previous;
+    impl_->plan_tail();
+    throw;
+  }
+  return impl_->tail_first;
+}
+int KrylovSolver::tail_level() const { return impl_->tail_first; }
 int KrylovSolver::host_waits() const { return impl_->waits; }
 
 // linalg/amg.cpp:263-269, linalg/solver.cpp:178-184

@@ -63,6 +63,10 @@ class KrylovSolver {
   std::vector<float> level_values(std::size_t level) const;
   // Fused-kernel use can be disabled to run the reference kernel sequence.
   void set_fused(bool fused);
+  // First level of the one-workgroup coarse-grid tail: < 0 automatic, 0 off.
+  // Returns the level in use (0: none).
+  int set_tail(int first);
+  int tail_level() const;
   int host_waits() const;
 
  private:

@@ -228,55 +228,77 @@ struct Prolonged {
   }
 };
 
-// Kahan-compensated sum of a[j] * x[col[j]] over [begin, end)
-// (kernels/linalg.kern.h:5-16).
-template <class X>
-__device__ __forceinline__ float row_product(int begin, int end, const int* __restrict__ cols,
-                                             const float* __restrict__ values, X gather) {
+// Kahan-compensated sum of a[j] * x[col[j]] over [begin, end), C entries per
+// chunk (kernels/linalg.kern.h:5-16).
+template <int C, class X>
+__device__ __forceinline__ float row_product_chunked(int begin, int end,
+                                                     const int* __restrict__ cols,
+                                                     const float* __restrict__ values, X gather) {
   Kahan k;
-  for (int j = begin; j < end; j += kChunk) {
-    int c[kChunk];
-    float v[kChunk], xv[kChunk];
+  for (int j = begin; j < end; j += C) {
+    int c[C];
+    float v[C], xv[C];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < C; ++u)
       if (j + u < end) {
         c[u] = cols[j + u];
         v[u] = values[j + u];
       }
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < C; ++u)
       if (j + u < end) xv[u] = gather(c[u]);
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < C; ++u)
       if (j + u < end) k.add(v[u] * xv[u]);
   }
   return k.s;
 }
 
-// Compensated residual b - sum a[j] * x[col[j]] over [begin, end)
-// (kernels/linalg.kern.h:32-45).
-template <class X>
-__device__ __forceinline__ float row_residual(float b, int begin, int end,
-                                              const int* __restrict__ cols,
-                                              const float* __restrict__ values, X gather) {
+// Compensated residual b - sum a[j] * x[col[j]] over [begin, end), C entries
+// per chunk (kernels/linalg.kern.h:32-45).
+template <int C, class X>
+__device__ __forceinline__ float row_residual_chunked(float b, int begin, int end,
+                                                      const int* __restrict__ cols,
+                                                      const float* __restrict__ values,
+                                                      X gather) {
   RowResidual acc(b);
-  for (int j = begin; j < end; j += kChunk) {
-    int c[kChunk];
-    float v[kChunk], xv[kChunk];
+  for (int j = begin; j < end; j += C) {
+    int c[C];
+    float v[C], xv[C];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < C; ++u)
       if (j + u < end) {
         c[u] = cols[j + u];
         v[u] = values[j + u];
       }
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < C; ++u)
       if (j + u < end) xv[u] = gather(c[u]);
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < C; ++u)
       if (j + u < end) acc.add(v[u], xv[u]);
   }
   return acc.value();
+}
+
+// Coarse AMG levels have 10-30 entries per row. When any row of the wave is
+// longer than one chunk, the wave takes chunks of twice the width, so such a
+// row costs one gather round trip instead of two. The choice is uniform per
+// wave and the entry order is unchanged.
+template <class X>
+__device__ __forceinline__ float row_product(int begin, int end, const int* __restrict__ cols,
+                                             const float* __restrict__ values, X gather) {
+  if (__any(end - begin > kChunk))
+    return row_product_chunked<2 * kChunk>(begin, end, cols, values, gather);
+  return row_product_chunked<kChunk>(begin, end, cols, values, gather);
+}
+template <class X>
+__device__ __forceinline__ float row_residual(float b, int begin, int end,
+                                              const int* __restrict__ cols,
+                                              const float* __restrict__ values, X gather) {
+  if (__any(end - begin > kChunk))
+    return row_residual_chunked<2 * kChunk>(b, begin, end, cols, values, gather);
+  return row_residual_chunked<kChunk>(b, begin, end, cols, values, gather);
 }
 
 // Maps workgroups so that each of the 8 XCDs (which receive workgroups

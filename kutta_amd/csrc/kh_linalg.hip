@@ -483,37 +483,42 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ pa
   }
 }
 
-// Restricted residual: each coarse row sums the compensated residuals of its
-// members, computed in place of the stored fine residual:
-// csr_residual_guarded + amg_restrict_guarded. Member rows are fetched four
-// at a time so their loads overlap.
+// Restricted residual without storing the fine residual:
+// csr_residual_guarded + amg_restrict_guarded. A workgroup owns kRestrictRows
+// consecutive coarse rows, whose members are consecutive in `members`. The
+// members are processed in tiles of one per lane: each lane computes one
+// member row's compensated residual into LDS, then the lane of each coarse row
+// adds its members' residuals in member order (Kahan), carrying the sum from
+// tile to tile. Every member row gets a lane of its own, as in a sweep.
+constexpr int kRestrictRows = 64;
 __global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
     const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
     const float* __restrict__ x, const float* __restrict__ b, const int* __restrict__ offsets,
     const int* __restrict__ members, float* __restrict__ coarse, const int* __restrict__ status) {
   if (status[0] != 0) return;
-  const int g = row_index();
-  if (g >= n) return;
+  __shared__ float residual[kBlock];
+  const int g0 = int(xcd_block()) * kRestrictRows;
+  if (g0 >= n) return;
+  const int g1 = min(g0 + kRestrictRows, n);
+  const int first = offsets[g0], last = offsets[g1];
+  const int g = g0 + int(threadIdx.x);
+  const bool owner = threadIdx.x < kRestrictRows && g < g1;
+  const int mine = owner ? offsets[g] : 0, mine_end = owner ? offsets[g + 1] : 0;
   Kahan k;
-  const int first = offsets[g], last = offsets[g + 1];
-  for (int base = first; base < last; base += 4) {
-    int row[4], rb[4], re[4];
-    float bb[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (base + u < last) row[u] = members[base + u];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (base + u < last) {
-        rb[u] = rows[row[u]];
-        re[u] = rows[row[u] + 1];
-        bb[u] = b[row[u]];
-      }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (base + u < last) k.add(row_residual(bb[u], rb[u], re[u], cols, values, Plain{x}));
+  for (int tile = first; tile < last; tile += kBlock) {
+    const int m = tile + int(threadIdx.x);
+    if (m < last) {
+      const int row = members[m];
+      residual[threadIdx.x] = row_residual(b[row], rows[row], rows[row + 1], cols, values, Plain{x});
+    }
+    __syncthreads();
+    if (owner) {
+      const int lo = max(mine, tile), hi = min(mine_end, tile + kBlock);
+      for (int j = lo; j < hi; ++j) k.add(residual[j - tile]);
+    }
+    __syncthreads();
   }
-  coarse[g] = k.s;
+  if (owner) coarse[g] = k.s;
 }
 // Prolongation followed by one smoothing sweep without storing x + P c:
 // amg_prolong_guarded + amg_smooth.
@@ -547,6 +552,165 @@ __global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
     return;
   }
   next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * (bi + 0.0f);
+}
+
+// -------------------------------------------------------- coarse-grid tail
+// Below a few thousand rows a level's launches are pure latency: each sweep
+// is a handful of dependent memory round trips plus a kernel boundary. The
+// tail runs every remaining level of the V-cycle (linalg/amg.cpp:283-310) in
+// one workgroup: each level's iterate and right-hand side live in LDS, the
+// operators stay in L2, and sweeps are separated by workgroup barriers. Every
+// row is computed by the same expression as the per-level kernels (l1-Jacobi,
+// kernels/linalg.kern.h:150-170; restriction, amg.kern.h:5-16; coarse solve,
+// linalg.kern.h:126-147), and Jacobi rows only read the previous sweep, so
+// the iterates are those of the launch sequence it replaces.
+constexpr int kTailThreads = 1024;
+constexpr int kTailRowsPerThread = KH_TAIL_MAX_ROWS / kTailThreads;
+static_assert(KH_TAIL_MAX_ROWS % kTailThreads == 0, "tail rows per thread");
+
+enum class TailSweep { Plain, FromZero, Prolonged };
+
+// One in-place sweep over a level: new values wait in registers until every
+// row has read the previous iterate.
+template <TailSweep S>
+__device__ __forceinline__ void tail_sweep(const kh_tail_level& L, float* x, const float* b,
+                                           const float* coarse) {
+  float next[kTailRowsPerThread];
+#pragma unroll
+  for (int u = 0; u < kTailRowsPerThread; ++u) {
+    const int i = int(threadIdx.x) + u * kTailThreads;
+    if (i >= L.rows) break;
+    const float bi = b[i];
+    const float w = (2.0f / 3.0f) * L.inverse[i];
+    if constexpr (S == TailSweep::FromZero) {
+      if (bi - bi != 0.0f) {
+        RowResidual acc(bi);
+        for (int j = L.row_ptr[i]; j < L.row_ptr[i + 1]; ++j) acc.add(L.val[j], 0.0f);
+        next[u] = 0.0f + w * acc.value();
+      } else {
+        next[u] = 0.0f + w * (bi + 0.0f);
+      }
+    } else if constexpr (S == TailSweep::Prolonged) {
+      const Prolonged gather{x, L.aggregate, coarse};
+      const float r =
+          row_residual_chunked<kChunk>(bi, L.row_ptr[i], L.row_ptr[i + 1], L.col, L.val, gather);
+      next[u] = gather(i) + w * r;
+    } else {
+      const float r =
+          row_residual_chunked<kChunk>(bi, L.row_ptr[i], L.row_ptr[i + 1], L.col, L.val, Plain{x});
+      next[u] = x[i] + w * r;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kTailRowsPerThread; ++u) {
+    const int i = int(threadIdx.x) + u * kTailThreads;
+    if (i >= L.rows) break;
+    x[i] = next[u];
+  }
+  __syncthreads();
+}
+
+// Restricted residual of level L into the next level's right-hand side.
+__device__ __forceinline__ void tail_restrict(const kh_tail_level& L, const float* x,
+                                              const float* b, float* coarse_b) {
+  for (int g = threadIdx.x; g < L.coarse_rows; g += kTailThreads) {
+    Kahan k;
+    for (int m = L.member_offsets[g]; m < L.member_offsets[g + 1]; ++m) {
+      const int row = L.members[m];
+      k.add(row_residual_chunked<kChunk>(b[row], L.row_ptr[row], L.row_ptr[row + 1], L.col, L.val,
+                                         Plain{x}));
+    }
+    coarse_b[g] = k.s;
+  }
+  __syncthreads();
+}
+
+// Constant-nullspace projection: sum_into + solver_project
+// (linalg/detail.hpp:13-19; kernels/scalars.kern.h:150-152).
+__device__ __forceinline__ void tail_project(float* x, int rows, double* red, float* pair) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < rows; i += kTailThreads) v += double(x[i]);
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) {
+    const Pair s = to_pair(v);
+    pair[0] = s.hi;
+    pair[1] = s.lo;
+  }
+  __syncthreads();
+  const float c = static_cast<float>(rows);
+  const float mean =
+      pfloat(pdiv({pair[0], pair[1]}, {c, static_cast<float>(double(rows) - double(c))}));
+  for (int i = threadIdx.x; i < rows; i += kTailThreads) x[i] -= mean;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
+    const kh_tail_level* __restrict__ levels, int count, int pre, int post, int coarse_sweeps,
+    int nullspace, const float* __restrict__ rhs, float* __restrict__ out,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  __shared__ float lds[KH_TAIL_LDS_FLOATS];
+  __shared__ double red[kTailThreads / 64];
+  __shared__ float pair[2];
+  // Level k keeps its iterate at lds[base[k]] and right-hand side right after.
+  __shared__ int base[KH_TAIL_MAX_LEVELS];
+  // The table is validated by every thread alike, so an invalid one returns
+  // uniformly before any barrier.
+  if (count < 1 || count > KH_TAIL_MAX_LEVELS) return;
+  int used = 0;
+  for (int k = 0; k < count; ++k) {
+    const int rows = levels[k].rows;
+    if (rows < 1 || rows > KH_TAIL_MAX_ROWS) return;
+    if (k + 1 < count && levels[k].coarse_rows != levels[k + 1].rows) return;
+    if (threadIdx.x == 0) base[k] = used;
+    used += 2 * rows;
+    if (used > KH_TAIL_LDS_FLOATS) return;
+  }
+  __syncthreads();
+  {
+    const int rows = levels[0].rows;
+    for (int i = threadIdx.x; i < rows; i += kTailThreads) lds[base[0] + rows + i] = rhs[i];
+  }
+  __syncthreads();
+  for (int k = 0; k + 1 < count; ++k) {
+    const kh_tail_level L = levels[k];
+    float* x = lds + base[k];
+    const float* b = x + L.rows;
+    for (int t = 1; t <= pre; ++t) {
+      if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
+      else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+    }
+    float* coarse_b = lds + base[k + 1] + L.coarse_rows;
+    tail_restrict(L, x, b, coarse_b);
+    if (nullspace) tail_project(coarse_b, L.coarse_rows, red, pair);
+  }
+  {
+    const kh_tail_level L = levels[count - 1];
+    float* x = lds + base[count - 1];
+    const float* b = x + L.rows;
+    if (L.rows <= kCoarseMax) {
+      // amg_coarse_solve: sweeps from the zero iterate, general residual form.
+      for (int i = threadIdx.x; i < L.rows; i += kTailThreads) x[i] = 0.0f;
+      __syncthreads();
+      for (int t = 0; t < coarse_sweeps; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+    } else {
+      for (int t = 1; t <= coarse_sweeps; ++t) {
+        if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
+        else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+      }
+    }
+    if (nullspace) tail_project(x, L.rows, red, pair);
+  }
+  for (int k = count - 2; k >= 0; --k) {
+    const kh_tail_level L = levels[k];
+    float* x = lds + base[k];
+    const float* b = x + L.rows;
+    tail_sweep<TailSweep::Prolonged>(L, x, b, lds + base[k + 1]);
+    for (int t = 2; t <= post; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+    if (nullspace) tail_project(x, L.rows, red, pair);
+  }
+  for (int i = threadIdx.x; i < levels[0].rows; i += kTailThreads) out[i] = lds[base[0] + i];
 }
 
 // ----------------------------------------------------------------- launchers
@@ -704,8 +868,9 @@ void l_fused_step_dot_check(kh_device* d, int n, const kh_arg* a) {
   k_finish<Finish::Check><<<FINISH>>>(d->partials, g, KF(4), st, nullptr);
 }
 void l_fused_residual_restrict(kh_device* d, int n, const kh_arg* a) {
-  k_fused_residual_restrict<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6),
-                                         KF(7), KB(8));
+  const unsigned groups = unsigned((n + kRestrictRows - 1) / kRestrictRows);
+  k_fused_residual_restrict<<<dim3(groups), dim3(kBlock), 0, d->stream>>>(
+      n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6), KF(7), KB(8));
 }
 void l_fused_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
   k_fused_prolong_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
@@ -727,6 +892,17 @@ void l_fused_smooth_sum(kh_device* d, int n, const kh_arg* a) {
 }
 void l_fused_smooth_from_zero(kh_device* d, int n, const kh_arg* a) {
   k_fused_smooth_from_zero<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6));
+}
+void l_fused_coarse_tail(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 1);
+  if (count < 1 || count > KH_TAIL_MAX_LEVELS)
+    throw std::runtime_error("fused_coarse_tail: level count must be in [1, 16]");
+  if (i32(a, 2) < 1 || i32(a, 3) < 1 || i32(a, 4) < 1)
+    throw std::runtime_error("fused_coarse_tail: sweep counts must be positive");
+  if (n < 1) return;
+  k_fused_coarse_tail<<<dim3(1), dim3(kTailThreads), 0, d->stream>>>(
+      static_cast<const kh_tail_level*>(buf(a, 0)), count, i32(a, 2), i32(a, 3), i32(a, 4),
+      i32(a, 5), KCF(6), KF(7), KB(8));
 }
 void l_fused_project_dot_beta(kh_device* d, int n, const kh_arg* a) {
   const unsigned g = reduce_grid(n);
@@ -790,6 +966,7 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_FUSED_SMOOTH_SUM, "fused_smooth_sum", "bbbbbbbbb", l_fused_smooth_sum},
       {KH_FUSED_PROJECT_DOT_BETA, "fused_project_dot_beta", "bbffbbb", l_fused_project_dot_beta},
       {KH_FUSED_SMOOTH_FROM_ZERO, "fused_smooth_from_zero", "bbbbbbb", l_fused_smooth_from_zero},
+      {KH_FUSED_COARSE_TAIL, "fused_coarse_tail", "biiiiibbb", l_fused_coarse_tail},
   };
   return table;
 }

@@ -351,3 +351,58 @@ def ref_kernels():
             return None
         _ref = C.CDLL(str(path))
     return _ref
+
+
+# ------------------------------------------------------ finite-volume kernels
+def _fv_pointers(args):
+    """args as the void* array oracle_fv_each / ref_fv_each / the device take:
+    numpy arrays by data pointer, Python ints and floats as 4-byte scalars."""
+    keep = []
+    ptrs = (C.c_void_p * len(args))()
+    for k, a in enumerate(args):
+        if isinstance(a, np.ndarray):
+            if not a.flags["C_CONTIGUOUS"]:
+                raise OracleError("fv kernels need contiguous arrays")
+            ptrs[k] = a.ctypes.data
+        else:
+            box = C.c_float(a) if isinstance(a, (float, np.floating)) else C.c_int32(int(a))
+            keep.append(box)
+            ptrs[k] = C.addressof(box)
+    return ptrs, keep
+
+
+def fv_each(name: str, n: int, *args) -> None:
+    """The oracle's restatement of finite-volume kernel `name` over [0, n);
+    output arrays are updated in place."""
+    L = lib()
+    L.oracle_fv_each.argtypes = [C.c_char_p, C.c_int, C.c_void_p]
+    ptrs, keep = _fv_pointers(args)
+    if L.oracle_fv_each(name.encode(), int(n), ptrs) != 0:
+        raise OracleError(f"unknown fv kernel {name}")
+
+
+def fv_signature(name: str) -> str:
+    L = lib()
+    L.oracle_fv_signature.argtypes = [C.c_char_p]
+    L.oracle_fv_signature.restype = C.c_char_p
+    s = L.oracle_fv_signature(name.encode())
+    if s is None:
+        raise OracleError(f"unknown fv kernel {name}")
+    return s.decode()
+
+
+def fv_names() -> list:
+    L = lib()
+    L.oracle_fv_name.restype = C.c_char_p
+    return [L.oracle_fv_name(k).decode() for k in range(L.oracle_fv_count())]
+
+
+def ref_fv_each(name: str, n: int, *args) -> None:
+    """The reference's own body of kernel `name` (oracle/_ref) over [0, n)."""
+    R = ref_kernels()
+    if R is None:
+        raise OracleError("oracle/_ref is not built")
+    R.ref_fv_each.argtypes = [C.c_char_p, C.c_int, C.c_void_p]
+    ptrs, keep = _fv_pointers(args)
+    if R.ref_fv_each(name.encode(), int(n), ptrs) != 0:
+        raise OracleError(f"reference build lacks fv kernel {name}")

@@ -1,0 +1,729 @@
+// ORACLE — test infrastructure only. Serial restatement of the reference's
+// finite-volume kernel bodies on the SIMPLEC path (src/kutta/kernels/fv_*.kern.h),
+// one function per kernel with the reference's parameter order, evaluated in
+// the reference's operation order with -ffp-contract=off. oracle_fv_each()
+// runs one of them over [0, n) as the serial CPU backend does
+// (runtime/cpu/cpu_device.cpp:64-68).
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <utility>
+
+namespace oracle::fv {
+namespace {
+
+// kernels/prelude.h:66-74 (CPU branch: std::fmax/fmin/sqrt/fabs/fma).
+inline float kmax(float a, float b) { return std::fmax(a, b); }
+inline float kmin(float a, float b) { return std::fmin(a, b); }
+inline float ksqrt(float a) { return std::sqrt(a); }
+inline float kabs(float a) { return std::fabs(a); }
+
+using I = const int*;
+using F = const float*;
+using W = float*;
+using WI = int*;
+
+// ---------------------------------------------------------- fv_flow.kern.h
+// :5-32. Boundary kinds: 0 wall, 1 inlet, 2 outlet with prescribed backflow,
+// 3 symmetry, 4 pressure outlet with normal backflow.
+void fv_velocity_boundary(int i, I owner, I kind, F sx, F sy, F sz, F u, F v, F w, F pu, F pv,
+                          F pw, F flux, W bu, W bv, W bw, WI scalar_kind, int internal, float rho) {
+  const int f = i + internal, c = owner[f], k = kind[i];
+  float vx = pu[i], vy = pv[i], vz = pw[i];
+  const float area2 = sx[f] * sx[f] + sy[f] * sy[f] + sz[f] * sz[f];
+  scalar_kind[i] = 0;
+  if ((k == 2 || k == 4) && flux[f] >= 0.0f) {
+    vx = u[c];
+    vy = v[c];
+    vz = w[c];
+    scalar_kind[i] = 2;
+  }
+  if (k == 4 && flux[f] < 0.0f) {
+    const float area = ksqrt(area2), speed = flux[f] / (rho * area);
+    vx = speed * (sx[f] / area);
+    vy = speed * (sy[f] / area);
+    vz = speed * (sz[f] / area);
+  }
+  if (k == 3) {
+    vx = u[c];
+    vy = v[c];
+    vz = w[c];
+    scalar_kind[i] = 2;
+  }
+  if (k == 0 || k == 3) {
+    const float un = (vx * sx[f] + vy * sy[f] + vz * sz[f]) / area2;
+    vx -= un * sx[f];
+    vy -= un * sy[f];
+    vz -= un * sz[f];
+  }
+  bu[i] = vx;
+  bv[i] = vy;
+  bw[i] = vz;
+}
+
+// :34-41
+void fv_pressure_boundary(int i, I owner, I kind, F rx, F ry, F rz, F p, F gx, F gy, F gz,
+                          F prescribed, W boundary, int internal) {
+  const int f = i + internal, c = owner[f];
+  const float dx = rx[f], dy = ry[f], dz = rz[f];
+  if (kind[i] == 2 || kind[i] == 4) boundary[i] = prescribed[i];
+  else boundary[i] = p[c] + gx[c] * dx + gy[c] * dy + gz[c] * dz;
+}
+
+// :43-65
+void fv_pressure_faces(int i, I owner, I neighbour, F px, F py, F pz, F nx, F ny, F nz, F p,
+                       F gx, F gy, F gz, F boundary, W face, W jump, int internal) {
+  const int c = owner[i];
+  const float side_p = p[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i];
+  if (i >= internal) {
+    face[i] = boundary[i - internal];
+    jump[i] = side_p - boundary[i - internal];
+    return;
+  }
+  const int n = neighbour[i];
+  const float side_n = p[n] + gx[n] * nx[i] + gy[n] * ny[i] + gz[n] * nz[i];
+  face[i] = 0.5f * (side_p + side_n);
+  const float dx = px[i] - nx[i], dy = py[i] - ny[i], dz = pz[i] - nz[i];
+  const float mx = 0.5f * (px[i] + nx[i]), my = 0.5f * (py[i] + ny[i]), mz = 0.5f * (pz[i] + nz[i]);
+  const float d2 = dx * dx + dy * dy + dz * dz, m2 = mx * mx + my * my + mz * mz;
+  const float trust = kmax(0.0f, kmin(1.0f, (4.0f * d2 - m2) / (3.0f * d2)));
+  const float link =
+      p[c] - p[n] + 0.5f * ((gx[c] + gx[n]) * dx + (gy[c] + gy[n]) * dy + (gz[c] + gz[n]) * dz);
+  jump[i] = trust == 1.0f ? side_p - side_n : link + trust * ((side_p - side_n) - link);
+}
+
+// :67-78
+void fv_momentum_source(int i, I offsets, I faces, F sign, F sx, F sy, F sz, F face_pressure,
+                        F volume, F body_u, F body_v, F body_w, W su, W sv, W sw) {
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    const float pf = sign[j] * face_pressure[f];
+    fx += pf * sx[f];
+    fy += pf * sy[f];
+    fz += pf * sz[f];
+  }
+  su[i] = body_u[i] - fx / volume[i];
+  sv[i] = body_v[i] - fy / volume[i];
+  sw[i] = body_w[i] - fz / volume[i];
+}
+
+// :83-109
+void fv_momentum_faces(int i, I owner, I neighbour, I kind, F px, F py, F pz, F nx, F ny, F nz,
+                       F u, F v, F w, F ux, F uy, F uz, F vx, F vy, F vz, F wx, F wy, F wz,
+                       F diagonal, F bu, F bv, F bw, W fu, W fv, W fw, int internal) {
+  if (i >= internal) {
+    const int b = i - internal, c = owner[i];
+    const bool outlet = kind[b] == 2 || kind[b] == 4;
+    fu[i] = outlet ? u[c] : bu[b];
+    fv[i] = outlet ? v[c] : bv[b];
+    fw[i] = outlet ? w[c] : bw[b];
+    return;
+  }
+  const int c = owner[i], n = neighbour[i];
+  const float a = diagonal[c] / (diagonal[c] + diagonal[n]);
+  const float rx = a * px[i] + (1.0f - a) * nx[i];
+  const float ry = a * py[i] + (1.0f - a) * ny[i];
+  const float rz = a * pz[i] + (1.0f - a) * nz[i];
+  fu[i] = a * u[c] + (1.0f - a) * u[n] + 0.5f * ((ux[c] + ux[n]) * rx + (uy[c] + uy[n]) * ry + (uz[c] + uz[n]) * rz);
+  fv[i] = a * v[c] + (1.0f - a) * v[n] + 0.5f * ((vx[c] + vx[n]) * rx + (vy[c] + vy[n]) * ry + (vz[c] + vz[n]) * rz);
+  fw[i] = a * w[c] + (1.0f - a) * w[n] + 0.5f * ((wx[c] + wx[n]) * rx + (wy[c] + wy[n]) * ry + (wz[c] + wz[n]) * rz);
+}
+
+// :111-116
+void fv_convective_outflow(int i, I offsets, I faces, F sign, F flux, W outflow) {
+  float total = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) total += kmax(0.0f, sign[j] * flux[faces[j]]);
+  outflow[i] = total;
+}
+
+// :122-133
+void fv_peclet_side_velocity(int i, I owner, I neighbour, F px, F py, F pz, F nx, F ny, F nz, F q,
+                             F gx, F gy, F gz, F weights, F diagonal, F outflow, W face) {
+  const int c = owner[i], n = neighbour[i];
+  const float a = weights[c] / (weights[c] + weights[n]);
+  const float beta = kmin(1.0f, (outflow[c] + outflow[n]) / kmax(diagonal[c] + diagonal[n], 1.0e-20f));
+  const float side = a * (q[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i]) +
+                     (1.0f - a) * (q[n] + gx[n] * nx[i] + gy[n] * ny[i] + gz[n] * nz[i]);
+  face[i] += beta * (side - face[i]);
+}
+
+// :139-162
+void fv_rhie_chow(int i, I owner, I neighbour, I kind, F sx, F sy, F sz, F diffusion, F volume,
+                  F diagonal, F inverse, F u, F v, F w, F pressure_jump, W flux, W coefficient,
+                  int internal, float rho) {
+  const int c = owner[i];
+  const float ux = u[i], uy = v[i], uz = w[i];
+  float rc = volume[c] / kmax(diagonal[c], 1.0e-20f), dc = volume[c] * inverse[c];
+  const float jump = pressure_jump[i];
+  if (i < internal) {
+    const int n = neighbour[i];
+    rc = 0.5f * (rc + volume[n] / kmax(diagonal[n], 1.0e-20f));
+    dc = 0.5f * (dc + volume[n] * inverse[n]);
+  } else {
+    const int k = kind[i - internal];
+    if (k == 0 || k == 3) {
+      flux[i] = 0.0f;
+      coefficient[i] = 0.0f;
+      return;
+    }
+    if (k == 1) {
+      flux[i] = rho * (ux * sx[i] + uy * sy[i] + uz * sz[i]);
+      coefficient[i] = 0.0f;
+      return;
+    }
+  }
+  flux[i] = rho * (ux * sx[i] + uy * sy[i] + uz * sz[i]) + rho * rc * diffusion[i] * jump;
+  coefficient[i] = rho * dc * diffusion[i];
+}
+
+// :185-190
+void fv_correct_flux(int i, I owner, I neighbour, F correction, F coefficient, W flux, int internal) {
+  const float pc = correction[owner[i]];
+  const float pn = i < internal ? correction[neighbour[i]] : 0.0f;
+  flux[i] += coefficient[i] * (pc - pn);
+}
+
+// :192-202
+void fv_correct_velocity(int i, F volume, F inverse, F diagonal, F gx, F gy, F gz, F correction,
+                         W u, W v, W w, W p, float alpha) {
+  const float d = volume[i] * inverse[i];
+  u[i] -= d * gx[i];
+  v[i] -= d * gy[i];
+  w[i] -= d * gz[i];
+  p[i] += kmin(alpha, 0.7f * diagonal[i] * inverse[i]) * correction[i];
+}
+
+// :204-215
+void fv_monitor_cell(int i, I offsets, I faces, F sign, F flux, F u, F v, F w, F diagonal,
+                     F residual_u, F residual_v, F residual_w, W continuity, W scale, W speed, W ru,
+                     W rv, W rw) {
+  float mass = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) mass += sign[j] * flux[faces[j]];
+  const float s = ksqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
+  continuity[i] = kabs(mass);
+  scale[i] = kabs(diagonal[i]) * s;
+  speed[i] = s;
+  ru[i] = kabs(residual_u[i]);
+  rv[i] = kabs(residual_v[i]);
+  rw[i] = kabs(residual_w[i]);
+}
+
+// :217-219
+void fv_speed(int i, F u, F v, F w, W speed) { speed[i] = ksqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]); }
+
+// :221-226
+void fv_continuity(int i, I offsets, I faces, F sign, F flux, W result) {
+  float mass = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) mass += sign[j] * flux[faces[j]];
+  result[i] = kabs(mass);
+}
+
+// :228-233
+void fv_wall_pressure_force(int i, F selected, F sx, F sy, F sz, F face_pressure, W x, W y, W z,
+                            int internal, float reference) {
+  const int f = i + internal;
+  const float pf = selected[i] * (face_pressure[f] - reference);
+  x[i] = pf * sx[f];
+  y[i] = pf * sy[f];
+  z[i] = pf * sz[f];
+}
+
+// :235-257
+void fv_wall_viscous_force(int i, I owner, F selected, F sx, F sy, F sz, F rx, F ry, F rz, F u, F v,
+                           F w, F ux, F uy, F uz, F vx, F vy, F vz, F wx, F wy, F wz, F bu, F bv,
+                           F bw, W x, W y, W z, int internal, float viscosity) {
+  const int f = i + internal, c = owner[f];
+  const float area = ksqrt(sx[f] * sx[f] + sy[f] * sy[f] + sz[f] * sz[f]);
+  const float ex = sx[f] / area, ey = sy[f] / area, ez = sz[f] / area;
+  const float dn = rx[f] * ex + ry[f] * ey + rz[f] * ez;
+  const float du = (bu[i] - u[c] - ux[c] * rx[f] - uy[c] * ry[f] - uz[c] * rz[f]) / dn;
+  const float dv = (bv[i] - v[c] - vx[c] * rx[f] - vy[c] * ry[f] - vz[c] * rz[f]) / dn;
+  const float dw = (bw[i] - w[c] - wx[c] * rx[f] - wy[c] * ry[f] - wz[c] * rz[f]) / dn;
+  // Wall-corrected velocity gradient rows.
+  const float g00 = ux[c] + du * ex, g01 = uy[c] + du * ey, g02 = uz[c] + du * ez;
+  const float g10 = vx[c] + dv * ex, g11 = vy[c] + dv * ey, g12 = vz[c] + dv * ez;
+  const float g20 = wx[c] + dw * ex, g21 = wy[c] + dw * ey, g22 = wz[c] + dw * ez;
+  const float s = -viscosity * selected[i];
+  x[i] = s * (2.0f * g00 * sx[f] + (g01 + g10) * sy[f] + (g02 + g20) * sz[f]);
+  y[i] = s * ((g10 + g01) * sx[f] + 2.0f * g11 * sy[f] + (g12 + g21) * sz[f]);
+  z[i] = s * ((g20 + g02) * sx[f] + (g21 + g12) * sy[f] + 2.0f * g22 * sz[f]);
+}
+
+// :261-266
+void fv_fluent_momentum_scale(int i, F ax, F ay, F az, F volume, F speed, W x, W y, W z, float rho,
+                              float dt) {
+  const float t = dt > 0.0f ? rho * volume[i] / dt : 0.0f;
+  x[i] = 2.0f * (ax[i] + t) * speed[i];
+  y[i] = 2.0f * (ay[i] + t) * speed[i];
+  z[i] = 2.0f * (az[i] + t) * speed[i];
+}
+
+// :268-276
+void fv_physical_time_terms(int i, F volume, F old1, F old2, W diagonal, W rhs, float rho_dt,
+                            float a0, float a1, float a2, int add) {
+  const float s = rho_dt * volume[i];
+  const float d0 = add ? diagonal[i] : 0.0f;
+  const float b0 = add ? rhs[i] : 0.0f;
+  diagonal[i] = d0 + s * a0;
+  rhs[i] = b0 + s * (a1 * old1[i] + a2 * old2[i]);
+}
+
+// :278-281
+void fv_face_time_defect(int i, F mass, F sx, F sy, F sz, F ux, F uy, F uz, W defect, float rho) {
+  defect[i] = mass[i] - rho * (sx[i] * ux[i] + sy[i] * uy[i] + sz[i] * uz[i]);
+}
+
+// :283-295
+void fv_face_time_flux(int i, I owner, I neighbour, I kind, F volume, F diagonal, F defect1,
+                       F defect2, W mass, int internal, float rho_dt, float a1, float a2) {
+  const int c = owner[i];
+  float d = volume[c] / kmax(diagonal[c], 1.0e-20f);
+  if (i < internal) {
+    const int n = neighbour[i];
+    d = 0.5f * (d + volume[n] / kmax(diagonal[n], 1.0e-20f));
+  } else {
+    const int k = kind[i - internal];
+    if (k == 0 || k == 1 || k == 3) return;
+  }
+  mass[i] += rho_dt * d * (a1 * defect1[i] + a2 * defect2[i]);
+}
+
+// :297-299
+void fv_cell_mass_error(int i, F imbalance, F volume, W error, float dt_over_rho) {
+  error[i] = dt_over_rho * kabs(imbalance[i]) / volume[i];
+}
+
+// ------------------------------------------------------ fv_gradient.kern.h
+// Pair helpers, kernels/prelude.h:102-133: TwoSum, FMA-carried product-add,
+// a gradient trace of a pair, and the difference of two pairs.
+struct Pair2 {
+  float hi, lo;
+};
+inline Pair2 two_sum(float a, float b) {
+  const float s = a + b, t = s - a;
+  return {s, (a - (s - t)) + (b - t)};
+}
+inline Pair2 product_add(Pair2 acc, float x, float y) {
+  const float p = x * y;
+  const Pair2 s = two_sum(acc.hi, p);
+  return two_sum(s.hi, (acc.lo + s.lo) + std::fma(x, y, -p));
+}
+inline Pair2 trace(float p, float low, float gx, float gy, float gz, float lx, float ly, float lz,
+                   float x, float y, float z) {
+  Pair2 v = two_sum(p, low);
+  v = product_add(v, gx, x);
+  v = product_add(v, lx, x);
+  v = product_add(v, gy, y);
+  v = product_add(v, ly, y);
+  v = product_add(v, gz, z);
+  v = product_add(v, lz, z);
+  return v;
+}
+inline float pair_delta(Pair2 a, Pair2 b) {
+  const Pair2 d = two_sum(a.hi, -b.hi);
+  return d.hi + (d.lo + (a.lo - b.lo));
+}
+// Adds w * (d.hi + d.lo) to a pair accumulator, high word first.
+inline Pair2 add_weighted(Pair2 acc, float w, Pair2 d) {
+  return product_add(product_add(acc, w, d.hi), w, d.lo);
+}
+
+// Neighbour value of entry j: a cell, or boundary face -(other+1).
+inline float across(int other, F cell, F boundary) { return other < 0 ? boundary[-other - 1] : cell[other]; }
+
+// :6-23. inverse holds the symmetric 3x3 as six planes xx xy xz yy yz zz.
+void fv_gradient(int i, I offsets, I adjacent, F dx, F dy, F dz, F fx, F fy, F fz, F inverse,
+                 F cell, F boundary, W gx, W gy, W gz, int n) {
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int o = adjacent[j];
+    const float d = across(o, cell, boundary) - cell[i];
+    const float wx = o < 0 ? fx[-o - 1] : dx[j];
+    const float wy = o < 0 ? fy[-o - 1] : dy[j];
+    const float wz = o < 0 ? fz[-o - 1] : dz[j];
+    bx += wx * d;
+    by += wy * d;
+    bz += wz * d;
+  }
+  gx[i] = inverse[i] * bx + inverse[i + n] * by + inverse[i + 2 * n] * bz;
+  gy[i] = inverse[i + n] * bx + inverse[i + 3 * n] * by + inverse[i + 4 * n] * bz;
+  gz[i] = inverse[i + 2 * n] * bx + inverse[i + 4 * n] * by + inverse[i + 5 * n] * bz;
+}
+
+// :28-56
+void fv_paired_gradient(int i, I offsets, I adjacent, F dx, F dy, F dz, F fx, F fy, F fz,
+                        F inverse, F cell, F low, F boundary, F boundary_low, W gx, W gy, W gz,
+                        W lx, W ly, W lz, int n) {
+  Pair2 bx{0.0f, 0.0f}, by{0.0f, 0.0f}, bz{0.0f, 0.0f};
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int o = adjacent[j];
+    const float hi = across(o, cell, boundary);
+    const float lo = o < 0 ? boundary_low[-o - 1] : low[o];
+    Pair2 d = two_sum(hi, -cell[i]);
+    d = product_add(d, lo, 1.0f);
+    d = product_add(d, -low[i], 1.0f);
+    const float wx = o < 0 ? fx[-o - 1] : dx[j];
+    const float wy = o < 0 ? fy[-o - 1] : dy[j];
+    const float wz = o < 0 ? fz[-o - 1] : dz[j];
+    bx = add_weighted(bx, wx, d);
+    by = add_weighted(by, wy, d);
+    bz = add_weighted(bz, wz, d);
+  }
+  const float xx = inverse[i], xy = inverse[i + n], xz = inverse[i + 2 * n];
+  const float yy = inverse[i + 3 * n], yz = inverse[i + 4 * n], zz = inverse[i + 5 * n];
+  Pair2 x{0.0f, 0.0f}, y{0.0f, 0.0f}, z{0.0f, 0.0f};
+  x = add_weighted(add_weighted(add_weighted(x, xx, bx), xy, by), xz, bz);
+  y = add_weighted(add_weighted(add_weighted(y, xy, bx), yy, by), yz, bz);
+  z = add_weighted(add_weighted(add_weighted(z, xz, bx), yz, by), zz, bz);
+  gx[i] = x.hi;
+  gy[i] = y.hi;
+  gz[i] = z.hi;
+  lx[i] = x.lo;
+  ly[i] = y.lo;
+  lz[i] = z.lo;
+}
+
+// :64-68. eps2 < 0: Barth-Jespersen; otherwise Venkatakrishnan's smooth form.
+inline float limit_face(float d, float delta, float eps2) {
+  if (eps2 < 0.0f) return delta / d;
+  const float a = delta * delta + eps2, b = d * delta;
+  return (a + 2.0f * b) / (a + 2.0f * d * d + b);
+}
+
+// :72-74
+void fv_range_shift(int i, F field, W above, W below, float bound) {
+  above[i] = field[i] + bound;
+  below[i] = bound - field[i];
+}
+
+// Increment grad . r to the face of entry j, r on the cell's own side.
+inline float face_increment(float gx, float gy, float gz, int j, I faces, F sign, F rx, F ry, F rz,
+                            F nx, F ny, F nz) {
+  const int f = faces[j];
+  const bool owner_side = sign[j] > 0.0f;
+  return gx * (owner_side ? rx[f] : nx[f]) + gy * (owner_side ? ry[f] : ny[f]) +
+         gz * (owner_side ? rz[f] : nz[f]);
+}
+// Smallest factor over the faces, clamped to [0, 1], given extremal rooms.
+inline float limit_cell(int i, float gx, float gy, float gz, float up, float down, I offsets,
+                        I faces, F sign, F rx, F ry, F rz, F nx, F ny, F nz, float eps2) {
+  float psi = 1.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const float d = face_increment(gx, gy, gz, j, faces, sign, rx, ry, rz, nx, ny, nz);
+    if (d > 0.0f) psi = kmin(psi, limit_face(d, up, eps2));
+    if (d < 0.0f) psi = kmin(psi, limit_face(d, down, eps2));
+  }
+  return kmax(0.0f, kmin(1.0f, psi));
+}
+
+// :76-94
+void fv_limiter(int i, I offsets, I adjacent, I faces, F sign, F rx, F ry, F rz, F nx, F ny, F nz,
+                F cell, F boundary, F gx, F gy, F gz, W limiter, float eps2) {
+  float lo = cell[i], hi = cell[i];
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const float v = across(adjacent[j], cell, boundary);
+    lo = kmin(lo, v);
+    hi = kmax(hi, v);
+  }
+  limiter[i] = limit_cell(i, gx[i], gy[i], gz[i], hi - cell[i], lo - cell[i], offsets, faces,
+                          sign, rx, ry, rz, nx, ny, nz, eps2);
+}
+
+// :103-123
+void fv_paired_limiter(int i, I offsets, I adjacent, I faces, F sign, F rx, F ry, F rz, F nx, F ny,
+                       F nz, F cell, F low, F boundary, I kind, F gx, F gy, F gz, W limiter,
+                       float eps2) {
+  float lo = 0.0f, hi = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int o = adjacent[j];
+    const float v = across(o, cell, boundary);
+    const float l = o < 0 ? (kind[-o - 1] == 2 ? low[i] : 0.0f) : low[o];
+    const float d = (v - cell[i]) + (l - low[i]);
+    lo = kmin(lo, d);
+    hi = kmax(hi, d);
+  }
+  limiter[i] = limit_cell(i, gx[i], gy[i], gz[i], hi, lo, offsets, faces, sign, rx, ry, rz, nx,
+                          ny, nz, eps2);
+}
+
+// :127-145
+void fv_paired_gradient_limiter(int i, I offsets, I adjacent, I faces, F sign, F rx, F ry, F rz,
+                                F nx, F ny, F nz, F cell, F low, F boundary, F boundary_low, F gx,
+                                F gy, F gz, F lx, F ly, F lz, W limiter, float eps2) {
+  float lo = 0.0f, hi = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int o = adjacent[j];
+    const float v = across(o, cell, boundary);
+    const float l = o < 0 ? boundary_low[-o - 1] : low[o];
+    const float d = pair_delta({v, l}, {cell[i], low[i]});
+    lo = kmin(lo, d);
+    hi = kmax(hi, d);
+  }
+  limiter[i] = limit_cell(i, gx[i] + lx[i], gy[i] + ly[i], gz[i] + lz[i], hi, lo, offsets, faces,
+                          sign, rx, ry, rz, nx, ny, nz, eps2);
+}
+
+// :147-166
+void fv_vector_gradient(int i, I offsets, I adjacent, F dx, F dy, F dz, F fx, F fy, F fz,
+                        F inverse, F u, F v, F w, F bu, F bv, F bw, W ux, W uy, W uz, W vx, W vy,
+                        W vz, W wx, W wy, W wz, int n) {
+  float au = 0.0f, bu_ = 0.0f, cu = 0.0f, av = 0.0f, bv_ = 0.0f, cv = 0.0f, aw = 0.0f, bw_ = 0.0f,
+        cw = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int o = adjacent[j];
+    const float du = across(o, u, bu) - u[i];
+    const float dv = across(o, v, bv) - v[i];
+    const float dw = across(o, w, bw) - w[i];
+    const float rx = o < 0 ? fx[-o - 1] : dx[j];
+    const float ry = o < 0 ? fy[-o - 1] : dy[j];
+    const float rz = o < 0 ? fz[-o - 1] : dz[j];
+    au += rx * du;
+    bu_ += ry * du;
+    cu += rz * du;
+    av += rx * dv;
+    bv_ += ry * dv;
+    cv += rz * dv;
+    aw += rx * dw;
+    bw_ += ry * dw;
+    cw += rz * dw;
+  }
+  const float xx = inverse[i], xy = inverse[i + n], xz = inverse[i + 2 * n];
+  const float yy = inverse[i + 3 * n], yz = inverse[i + 4 * n], zz = inverse[i + 5 * n];
+  ux[i] = xx * au + xy * bu_ + xz * cu;
+  uy[i] = xy * au + yy * bu_ + yz * cu;
+  uz[i] = xz * au + yz * bu_ + zz * cu;
+  vx[i] = xx * av + xy * bv_ + xz * cv;
+  vy[i] = xy * av + yy * bv_ + yz * cv;
+  vz[i] = xz * av + yz * bv_ + zz * cv;
+  wx[i] = xx * aw + xy * bw_ + xz * cw;
+  wy[i] = xy * aw + yy * bw_ + yz * cw;
+  wz[i] = xz * aw + yz * bw_ + zz * cw;
+}
+
+// :168-171
+void fv_scale_gradient(int i, F gx, F gy, F gz, F limiter, W x, W y, W z) {
+  x[i] = limiter[i] * gx[i];
+  y[i] = limiter[i] * gy[i];
+  z[i] = limiter[i] * gz[i];
+}
+
+// :173-180
+void fv_limiter_scale(int i, I offsets, I adjacent, I faces, F sign, F rx, F ry, F rz, F nx, F ny,
+                      F nz, F cell, F boundary, F gx, F gy, F gz, W limiter, W x, W y, W z,
+                      float eps2) {
+  fv_limiter(i, offsets, adjacent, faces, sign, rx, ry, rz, nx, ny, nz, cell, boundary, gx, gy, gz,
+             limiter, eps2);
+  fv_scale_gradient(i, gx, gy, gz, limiter, x, y, z);
+}
+
+// :182-189
+void fv_paired_limiter_scale(int i, I offsets, I adjacent, I faces, F sign, F rx, F ry, F rz, F nx,
+                             F ny, F nz, F cell, F low, F boundary, I kind, F gx, F gy, F gz,
+                             W limiter, W x, W y, W z, float eps2) {
+  fv_paired_limiter(i, offsets, adjacent, faces, sign, rx, ry, rz, nx, ny, nz, cell, low, boundary,
+                    kind, gx, gy, gz, limiter, eps2);
+  fv_scale_gradient(i, gx, gy, gz, limiter, x, y, z);
+}
+
+// :191-199
+void fv_paired_gradient_limiter_scale(int i, I offsets, I adjacent, I faces, F sign, F rx, F ry,
+                                      F rz, F nx, F ny, F nz, F cell, F low, F boundary,
+                                      F boundary_low, F gx, F gy, F gz, F lx, F ly, F lz,
+                                      W limiter, W x, W y, W z, float eps2) {
+  fv_paired_gradient_limiter(i, offsets, adjacent, faces, sign, rx, ry, rz, nx, ny, nz, cell, low,
+                             boundary, boundary_low, gx, gy, gz, lx, ly, lz, limiter, eps2);
+  fv_scale_gradient(i, gx, gy, gz, limiter, x, y, z);
+}
+
+// Half the inverse-weighted jump moments added to the gradient (:218-220).
+inline void add_half_jump(int i, int n, F inverse, F gx, F gy, F gz, float bx, float by, float bz,
+                          W x, W y, W z) {
+  x[i] = gx[i] + 0.5f * (inverse[i] * bx + inverse[i + n] * by + inverse[i + 2 * n] * bz);
+  y[i] = gy[i] + 0.5f * (inverse[i + n] * bx + inverse[i + 3 * n] * by + inverse[i + 4 * n] * bz);
+  z[i] = gz[i] + 0.5f * (inverse[i + 2 * n] * bx + inverse[i + 4 * n] * by + inverse[i + 5 * n] * bz);
+}
+
+// :202-221
+void fv_jump_gradient(int i, I offsets, I faces, I adjacent, F sign, F dx, F dy, F dz, F inverse,
+                      F px, F py, F pz, F nx, F ny, F nz, F cell, F boundary, F gx, F gy, F gz,
+                      W x, W y, W z, int n) {
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j], o = adjacent[j];
+    const bool owner_side = sign[j] > 0.0f;
+    const float own = cell[i] + gx[i] * (owner_side ? px[f] : nx[f]) +
+                      gy[i] * (owner_side ? py[f] : ny[f]) + gz[i] * (owner_side ? pz[f] : nz[f]);
+    float other = 0.0f;
+    if (o < 0) other = boundary[-o - 1];
+    else
+      other = cell[o] + gx[o] * (owner_side ? nx[f] : px[f]) + gy[o] * (owner_side ? ny[f] : py[f]) +
+              gz[o] * (owner_side ? nz[f] : pz[f]);
+    const float jump = other - own;
+    bx += dx[j] * jump;
+    by += dy[j] * jump;
+    bz += dz[j] * jump;
+  }
+  add_half_jump(i, n, inverse, gx, gy, gz, bx, by, bz, x, y, z);
+}
+
+// :227-240
+void fv_face_jump(int i, I owner, I neighbour, F px, F py, F pz, F nx, F ny, F nz, F cell,
+                  F boundary, F gx, F gy, F gz, W forward, W reverse, int internal) {
+  const int c = owner[i];
+  const float own = cell[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i];
+  float other = 0.0f;
+  if (i < internal) {
+    const int nb = neighbour[i];
+    other = cell[nb] + gx[nb] * nx[i] + gy[nb] * ny[i] + gz[nb] * nz[i];
+    reverse[i] = own - other;
+  } else {
+    other = boundary[i - internal];
+  }
+  forward[i] = other - own;
+}
+
+// :242-254
+void fv_gather_jump_gradient(int i, I offsets, I faces, F sign, F dx, F dy, F dz, F inverse,
+                             F forward, F reverse, F gx, F gy, F gz, W x, W y, W z, int n) {
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    const float jump = sign[j] > 0.0f ? forward[f] : reverse[f];
+    bx += dx[j] * jump;
+    by += dy[j] * jump;
+    bz += dz[j] * jump;
+  }
+  add_half_jump(i, n, inverse, gx, gy, gz, bx, by, bz, x, y, z);
+}
+
+// :258-292
+void fv_paired_jump_gradient(int i, I offsets, I faces, I adjacent, F sign, F dx, F dy, F dz,
+                             F inverse, F px, F py, F pz, F nx, F ny, F nz, F cell, F low,
+                             F boundary, F boundary_low, F gx, F gy, F gz, F lx, F ly, F lz, W x,
+                             W y, W z, int n) {
+  Pair2 bx{0.0f, 0.0f}, by{0.0f, 0.0f}, bz{0.0f, 0.0f};
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j], o = adjacent[j];
+    const bool owner_side = sign[j] > 0.0f;
+    const Pair2 own = trace(cell[i], low[i], gx[i], gy[i], gz[i], lx[i], ly[i], lz[i],
+                            owner_side ? px[f] : nx[f], owner_side ? py[f] : ny[f],
+                            owner_side ? pz[f] : nz[f]);
+    Pair2 other{0.0f, 0.0f};
+    if (o < 0) other = {boundary[-o - 1], boundary_low[-o - 1]};
+    else
+      other = trace(cell[o], low[o], gx[o], gy[o], gz[o], lx[o], ly[o], lz[o],
+                    owner_side ? nx[f] : px[f], owner_side ? ny[f] : py[f],
+                    owner_side ? nz[f] : pz[f]);
+    Pair2 jump = two_sum(other.hi, -own.hi);
+    jump = product_add(jump, other.lo, 1.0f);
+    jump = product_add(jump, -own.lo, 1.0f);
+    bx = add_weighted(bx, dx[j], jump);
+    by = add_weighted(by, dy[j], jump);
+    bz = add_weighted(bz, dz[j], jump);
+  }
+  const float xx = .5f * inverse[i], xy = .5f * inverse[i + n], xz = .5f * inverse[i + 2 * n];
+  const float yy = .5f * inverse[i + 3 * n], yz = .5f * inverse[i + 4 * n], zz = .5f * inverse[i + 5 * n];
+  Pair2 a = two_sum(gx[i], lx[i]), b = two_sum(gy[i], ly[i]), c = two_sum(gz[i], lz[i]);
+  a = add_weighted(add_weighted(add_weighted(a, xx, bx), xy, by), xz, bz);
+  b = add_weighted(add_weighted(add_weighted(b, xy, bx), yy, by), yz, bz);
+  c = add_weighted(add_weighted(add_weighted(c, xz, bx), yz, by), zz, bz);
+  x[i] = a.hi;
+  y[i] = b.hi;
+  z[i] = c.hi;
+}
+
+// :296-299. Scatters stored inverse words back to their cells.
+void fv_lsq_restore_inverse(int i, I cells, F values, W inverse, int count, int n) {
+  for (int k = 0; k < 6; ++k) inverse[cells[i] + k * n] = values[i + k * count];
+}
+
+// ---------------------------------------------------------------- dispatch
+template <class T>
+T from_arg(void* const* a, std::size_t k) {
+  if constexpr (std::is_pointer_v<T>) return static_cast<T>(a[k]);
+  else {
+    T v;
+    std::memcpy(&v, a[k], sizeof v);
+    return v;
+  }
+}
+template <class... P, std::size_t... K>
+void run(void (*body)(int, P...), int n, void* const* a, std::index_sequence<K...>) {
+  const std::tuple<P...> args{from_arg<P>(a, K)...};
+  for (int i = 0; i < n; ++i) std::apply([&](P... p) { body(i, p...); }, args);
+}
+template <class T>
+char code() {
+  if constexpr (std::is_pointer_v<T>) return 'b';
+  else if constexpr (std::is_same_v<T, float>) return 'f';
+  else if constexpr (std::is_same_v<T, unsigned>) return 'u';
+  else return 'i';
+}
+struct Entry {
+  void (*run)(int, void* const*);
+  std::string signature;
+};
+template <class... P>
+Entry entry(void (*)(int, P...), void (*runner)(int, void* const*)) {
+  return {runner, std::string{code<P>()...}};
+}
+#define KERNEL(name)                                                                     \
+  {                                                                                      \
+    #name, entry(&name, [](int n, void* const* a) {                                      \
+      run(&name, n, a, std::make_index_sequence<std::tuple_size_v<decltype(args_of(&name))>>{}); \
+    })                                                                                   \
+  }
+template <class... P>
+std::tuple<P...> args_of(void (*)(int, P...));
+
+const std::map<std::string, Entry>& table() {
+  static const std::map<std::string, Entry> t = {
+      KERNEL(fv_velocity_boundary), KERNEL(fv_pressure_boundary), KERNEL(fv_pressure_faces),
+      KERNEL(fv_momentum_source), KERNEL(fv_momentum_faces), KERNEL(fv_convective_outflow),
+      KERNEL(fv_peclet_side_velocity), KERNEL(fv_rhie_chow), KERNEL(fv_correct_flux),
+      KERNEL(fv_correct_velocity), KERNEL(fv_monitor_cell), KERNEL(fv_speed),
+      KERNEL(fv_continuity), KERNEL(fv_wall_pressure_force), KERNEL(fv_wall_viscous_force),
+      KERNEL(fv_fluent_momentum_scale), KERNEL(fv_physical_time_terms),
+      KERNEL(fv_face_time_defect), KERNEL(fv_face_time_flux), KERNEL(fv_cell_mass_error),
+      KERNEL(fv_gradient), KERNEL(fv_paired_gradient), KERNEL(fv_range_shift),
+      KERNEL(fv_limiter), KERNEL(fv_paired_limiter), KERNEL(fv_paired_gradient_limiter),
+      KERNEL(fv_vector_gradient), KERNEL(fv_scale_gradient), KERNEL(fv_limiter_scale),
+      KERNEL(fv_paired_limiter_scale), KERNEL(fv_paired_gradient_limiter_scale),
+      KERNEL(fv_jump_gradient), KERNEL(fv_face_jump), KERNEL(fv_gather_jump_gradient),
+      KERNEL(fv_paired_jump_gradient), KERNEL(fv_lsq_restore_inverse),
+  };
+  return t;
+}
+
+}  // namespace
+}  // namespace oracle::fv
+
+extern "C" {
+// Runs kernel `name` for i in [0, n); args[k] points at the k-th parameter
+// (a buffer, or a 4-byte scalar). Returns 0, or 1 for an unknown name.
+int oracle_fv_each(const char* name, int n, void* const* args) {
+  const auto& t = oracle::fv::table();
+  const auto it = t.find(name);
+  if (it == t.end()) return 1;
+  it->second.run(n, args);
+  return 0;
+}
+// Parameter kinds of kernel `name` ('b' buffer, 'f' float, 'i' int), or NULL.
+const char* oracle_fv_signature(const char* name) {
+  const auto& t = oracle::fv::table();
+  const auto it = t.find(name);
+  return it == t.end() ? nullptr : it->second.signature.c_str();
+}
+// Number of kernels and the name of the k-th, for enumeration.
+int oracle_fv_count() { return int(oracle::fv::table().size()); }
+const char* oracle_fv_name(int k) {
+  const auto& t = oracle::fv::table();
+  auto it = t.begin();
+  for (int j = 0; j < k && it != t.end(); ++j) ++it;
+  return it == t.end() ? nullptr : it->first.c_str();
+}
+}

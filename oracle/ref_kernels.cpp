@@ -4,6 +4,14 @@
 // the body for every index in [0, n) as the reference's serial CPU backend
 // does (runtime/cpu/cpu_device.cpp:64-68). Used only to cross-check the
 // oracle's restatement.
+#include <cstddef>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+
 #include "kutta/kernels/prelude.h"
 
 // Kernel bodies live in their own namespace, as the reference's generated CPU
@@ -13,12 +21,67 @@ namespace refk {
 #include "kutta/kernels/amg.kern.h"
 #include "kutta/kernels/blas.kern.h"
 #include "kutta/kernels/scalars.kern.h"
-#include "kutta/kernels/fv_flow.kern.h"
 }  // namespace refk
+// The finite-volume bodies use the prelude's pair type, which scalars.kern.h
+// shadows inside refk, so they get a namespace of their own.
+namespace reff {
+#include "kutta/kernels/fv_flow.kern.h"
+#include "kutta/kernels/fv_gradient.kern.h"
+}  // namespace reff
 
 #define LOOP(name, ...) for (int i = 0; i < n; ++i) refk::name(i, __VA_ARGS__)
 
+// Name-keyed runner over the reference's finite-volume bodies: args[k] points
+// at the k-th parameter (a buffer, or a 4-byte scalar), as for oracle_fv_each.
+namespace {
+template <class T>
+T from_arg(void* const* a, std::size_t k) {
+  if constexpr (std::is_pointer_v<T>) return static_cast<T>(a[k]);
+  else {
+    T v;
+    std::memcpy(&v, a[k], sizeof v);
+    return v;
+  }
+}
+template <class... P, std::size_t... K>
+void run_each(void (*body)(int, P...), int n, void* const* a, std::index_sequence<K...>) {
+  const std::tuple<P...> args{from_arg<P>(a, K)...};
+  for (int i = 0; i < n; ++i) std::apply([&](P... p) { body(i, p...); }, args);
+}
+template <class... P>
+void run_each(void (*body)(int, P...), int n, void* const* a) {
+  run_each(body, n, a, std::index_sequence_for<P...>{});
+}
+using Runner = void (*)(int, void* const*);
+#define REF_FV(name) {#name, [](int n, void* const* a) { run_each(&reff::name, n, a); }}
+const std::map<std::string, Runner>& fv_table() {
+  static const std::map<std::string, Runner> t = {
+      REF_FV(fv_velocity_boundary), REF_FV(fv_pressure_boundary), REF_FV(fv_pressure_faces),
+      REF_FV(fv_momentum_source), REF_FV(fv_momentum_faces), REF_FV(fv_convective_outflow),
+      REF_FV(fv_peclet_side_velocity), REF_FV(fv_rhie_chow), REF_FV(fv_correct_flux),
+      REF_FV(fv_correct_velocity), REF_FV(fv_monitor_cell), REF_FV(fv_speed),
+      REF_FV(fv_continuity), REF_FV(fv_wall_pressure_force), REF_FV(fv_wall_viscous_force),
+      REF_FV(fv_fluent_momentum_scale), REF_FV(fv_physical_time_terms),
+      REF_FV(fv_face_time_defect), REF_FV(fv_face_time_flux), REF_FV(fv_cell_mass_error),
+      REF_FV(fv_gradient), REF_FV(fv_paired_gradient), REF_FV(fv_range_shift),
+      REF_FV(fv_limiter), REF_FV(fv_paired_limiter), REF_FV(fv_paired_gradient_limiter),
+      REF_FV(fv_vector_gradient), REF_FV(fv_scale_gradient), REF_FV(fv_limiter_scale),
+      REF_FV(fv_paired_limiter_scale), REF_FV(fv_paired_gradient_limiter_scale),
+      REF_FV(fv_jump_gradient), REF_FV(fv_face_jump), REF_FV(fv_gather_jump_gradient),
+      REF_FV(fv_paired_jump_gradient), REF_FV(fv_lsq_restore_inverse),
+  };
+  return t;
+}
+}  // namespace
+
 extern "C" {
+int ref_fv_each(const char* name, int n, void* const* args) {
+  const auto& t = fv_table();
+  const auto it = t.find(name);
+  if (it == t.end()) return 1;
+  it->second(n, args);
+  return 0;
+}
 void ref_csr_spmv(int n, const int* r, const int* c, const float* v, const float* x, float* y) {
   LOOP(csr_spmv, r, c, v, x, y);
 }
@@ -73,7 +136,7 @@ void ref_fv_pressure_assemble(int n, const int* offsets, const int* faces, const
                               const int* adjacent, const int* entry, const int* rows,
                               const int* diag, const float* coefficient, const float* flux,
                               float* values, float* rhs, int reference) {
-  LOOP(fv_pressure_assemble, offsets, faces, sign, adjacent, entry, rows, diag, coefficient, flux,
+  for (int i = 0; i < n; ++i) reff::fv_pressure_assemble(i, offsets, faces, sign, adjacent, entry, rows, diag, coefficient, flux,
        values, rhs, reference);
 }
 }

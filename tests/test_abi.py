@@ -45,8 +45,11 @@ def test_kernel_enum_matches_header():
     for m in re.finditer(r"^\s*(KH_\w+)(?:\s*=\s*(\d+))?,", body, flags=re.M):
         value = int(m.group(2)) if m.group(2) else value + 1
         ids[m.group(1)] = value
+    ids.pop("KH_KERNEL_NONE")
+    ids.pop("KH_KERNEL_COUNT_MARK", None)
     for k in kutta_amd.Kernel:
         assert ids["KH_" + k.name.upper()] == int(k), k
+    assert sorted(ids) == sorted("KH_" + k.name.upper() for k in kutta_amd.Kernel)
 
 
 def test_last_error_is_a_string_without_a_device():
