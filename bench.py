@@ -80,6 +80,22 @@ def cpu_baseline(problem, b, sweeps, gpu_stats):
     }, x
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "amg_smooth_fine_c2_hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fine-level amg_smooth on the 55^3 operator,
+    from the committed FETCH_SIZE/WRITE_SIZE counter summary (scaled by a copy
+    kernel of known size), or None when that summary is absent."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(nbytes: int, repeats: int) -> int:
     """Times `repeats` device-to-device copies of nbytes through the device ABI
     (kh_copy) and prints one JSON line. A copy moves exactly nbytes in and
@@ -178,6 +194,7 @@ def main() -> int:
     # amg_smooth reads x, b, l1-inverse, row_ptr and writes next (5 x 4n) and
     # gathers col + val (8m) once: 20n + 8m bytes per launch.
     bytes_per_launch = 20 * n + 8 * m
+    traffic = measured_traffic() if args.size == 55 else None
     avg_us = kernel_ms * 1e3 / max(launches, 1)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if launches else None
     roofline = {
@@ -187,7 +204,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": None,
+        "traffic": traffic,
         "bytes_per_launch": bytes_per_launch,
         "avg_launch_us": round(avg_us, 3),
         "launches": launches,

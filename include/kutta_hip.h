@@ -165,6 +165,47 @@ typedef enum kh_kernel {
   KH_FV_GATHER_JUMP_GRADIENT,    /* :242-254 */
   KH_FV_PAIRED_JUMP_GRADIENT,    /* :258-292 */
   KH_FV_LSQ_RESTORE_INVERSE,     /* :296-299 */
+  /* kernels/fv_transport.kern.h (transport faces and matrix assembly) */
+  KH_FV_DIFFUSION_CORRECTION_BOUND,/* :3-10 */
+  KH_FV_TRANSPORT_FACES,         /* :12-35 */
+  KH_FV_TRANSPORT_ASSEMBLE,      /* :37-112 */
+  KH_FV_FIXED_ROWS,              /* :115-124 */
+  KH_FV_FLOOR,                   /* :126-128 */
+  KH_FV_SCALAR_STEADY_MEASURE,   /* :130-135 */
+  KH_FV_SCALAR_SET_FIXED,        /* :137-139 */
+  KH_FV_SCALAR_MONITOR,          /* :141-148 */
+  /* kernels/fv_paired.kern.h (float-pair SIMPLEC state) */
+  KH_FV_COMPENSATED_PRESSURE_JUMP,   /* :5-33 */
+  KH_FV_PAIRED_PRESSURE_JUMP,        /* :46-65 */
+  KH_FV_COMPENSATED_RHIE_CHOW,       /* :69-85 */
+  KH_FV_PAIRED_PRESSURE_BOUNDARY_LOW,/* :89-97 */
+  KH_FV_PAIRED_PRESSURE_FACES,       /* :98-128 */
+  KH_FV_PAIRED_MOMENTUM_RESIDUAL,    /* :140-152 */
+  KH_FV_COMPENSATED_MOMENTUM_RESIDUAL,/* :156-170 */
+  KH_FV_PAIRED_ACCUMULATE,           /* :172-174 */
+  KH_FV_COMPENSATED_PRESSURE_FORCE,  /* :179-196 */
+  KH_FV_PAIRED_VELOCITY_BOUNDARY,    /* :200-218 */
+  KH_FV_PAIRED_ADVECTION,            /* :220-236 */
+  KH_FV_PAIRED_FACE_RESIDUAL,        /* :238-247 */
+  KH_FV_PAIRED_FLUX,                 /* :259-290 */
+  KH_FV_PAIRED_FLUX_OFFSET,          /* :292-303 */
+  KH_FV_PAIRED_CONTINUITY,           /* :308-318 */
+  KH_FV_PAIRED_PRESSURE_RHS,         /* :321-323 */
+  KH_FV_PAIRED_CORRECT_FLUX,         /* :326-333 */
+  KH_FV_PAIRED_CORRECT_VELOCITY,     /* :336-347 */
+  KH_FV_PAIRED_VALUE,                /* :349-351 */
+  KH_FV_PAIRED_VECTOR_VALUES,        /* :352-355 */
+  KH_FV_COMPENSATED_CORRECT_FLUX,    /* :356-363 */
+  KH_FV_COMPENSATED_CORRECT_VELOCITY,/* :365-377 */
+  /* kernels/fv_rans.kern.h (turbulent stresses, momentum boundary terms) */
+  KH_FV_BACKFLOW_PRESSURE,           /* :4-10 */
+  KH_FV_EFFECTIVE_VISCOSITY,         /* :12-17 */
+  KH_FV_TRANSPOSE_STRESS,            /* :21-37 */
+  KH_FV_STRESS_SOURCE,               /* :38-46 */
+  KH_FV_BOUNDARY_MOMENTUM,           /* :47-72 */
+  KH_FV_MOMENTUM_DIAGONAL,           /* :75-78 */
+  KH_FV_SST_FORCE,                   /* :79-83 */
+  KH_FV_TURBULENCE_BOUNDARY,         /* :86-101 */
 
   /* Fused MI355X kernels (no reference counterpart). */
   KH_FUSED_SPMV_DOT_ALPHA = 200, /* csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha:
@@ -208,7 +249,15 @@ typedef struct kh_tail_level {
   const int32_t* aggregate;
   int32_t rows;
   int32_t coarse_rows;
+  int32_t nnz;
+  int32_t pad;
 } kh_tail_level;
+/* The tail's LDS pool in 32-bit words. The vectors (2 * sum of rows) come
+ * first; the rest holds one level's operator at a time while that level is
+ * swept: 3 * rows + 1 + 2 * nnz words (row starts, columns, values, l1 inverse,
+ * aggregate map). A level whose copy fits is swept from LDS, others from
+ * global memory. */
+#define KH_TAIL_POOL_WORDS 40448
 
 /* Device::launch (device.hpp:205; cuda_device.cu:132-140): runs kernel `kernel`
  * for global indices [0, n). n == 0 is a no-op; n > INT32_MAX, an unknown id or

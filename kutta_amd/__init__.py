@@ -229,6 +229,44 @@ class Kernel(enum.IntEnum):
     fv_gather_jump_gradient = 134
     fv_paired_jump_gradient = 135
     fv_lsq_restore_inverse = 136
+    fv_diffusion_correction_bound = 137
+    fv_transport_faces = 138
+    fv_transport_assemble = 139
+    fv_fixed_rows = 140
+    fv_floor = 141
+    fv_scalar_steady_measure = 142
+    fv_scalar_set_fixed = 143
+    fv_scalar_monitor = 144
+    fv_compensated_pressure_jump = 145
+    fv_paired_pressure_jump = 146
+    fv_compensated_rhie_chow = 147
+    fv_paired_pressure_boundary_low = 148
+    fv_paired_pressure_faces = 149
+    fv_paired_momentum_residual = 150
+    fv_compensated_momentum_residual = 151
+    fv_paired_accumulate = 152
+    fv_compensated_pressure_force = 153
+    fv_paired_velocity_boundary = 154
+    fv_paired_advection = 155
+    fv_paired_face_residual = 156
+    fv_paired_flux = 157
+    fv_paired_flux_offset = 158
+    fv_paired_continuity = 159
+    fv_paired_pressure_rhs = 160
+    fv_paired_correct_flux = 161
+    fv_paired_correct_velocity = 162
+    fv_paired_value = 163
+    fv_paired_vector_values = 164
+    fv_compensated_correct_flux = 165
+    fv_compensated_correct_velocity = 166
+    fv_backflow_pressure = 167
+    fv_effective_viscosity = 168
+    fv_transpose_stress = 169
+    fv_stress_source = 170
+    fv_boundary_momentum = 171
+    fv_momentum_diagonal = 172
+    fv_sst_force = 173
+    fv_turbulence_boundary = 174
     fused_spmv_dot_alpha = 200
     fused_step_dot_check = 201
     fused_residual_restrict = 202

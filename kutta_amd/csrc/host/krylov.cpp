@@ -160,11 +160,22 @@ struct KrylovSolver::Impl {
     return floats <= KH_TAIL_LDS_FLOATS;
   }
 
+  // Whether every level in [first, end) can be swept from an LDS copy of its
+  // operator, next to the tail's vectors.
+  bool tail_staged(std::size_t first) const {
+    std::size_t vectors = 0;
+    for (std::size_t i = first; i < levels.size(); ++i) vectors += 2 * std::size_t(levels[i].rows);
+    for (std::size_t i = first; i < levels.size(); ++i) {
+      const std::size_t words = 3 * std::size_t(levels[i].rows) + 1 + 2 * std::size_t(levels[i].nnz);
+      if (vectors + words > KH_TAIL_POOL_WORDS) return false;
+    }
+    return true;
+  }
+
   // Picks the tail's first level: the requested one, or else the first level
-  // that fits and whose operator is small enough that one compute unit
-  // sweeps it faster than a grid launch completes.
+  // from which every level fits one workgroup with its operator in LDS. One
+  // compute unit sweeping an operator out of L2 is slower than grid launches.
   void plan_tail() {
-    static constexpr Index kAutoNnz = 131072;
     tail_first = 0;
     tail_table = DeviceArray<kh_tail_level>();
     if (!amg || tail_request == 0) return;
@@ -174,7 +185,7 @@ struct KrylovSolver::Impl {
       tail_first = tail_request;
     } else {
       for (std::size_t i = 1; i < levels.size(); ++i)
-        if (levels[i].nnz <= kAutoNnz && tail_fits(i)) {
+        if (tail_fits(i) && tail_staged(i)) {
           tail_first = int(i);
           break;
         }
@@ -196,6 +207,7 @@ struct KrylovSolver::Impl {
         t.coarse_rows = l.coarse_rows;
       }
       t.rows = l.rows;
+      t.nnz = l.nnz;
       table.push_back(t);
     }
     tail_table = to_device(dev, table);

@@ -10,6 +10,20 @@
 
 namespace kh {
 
+// ----------------------------------------------------------- min and max
+// kmax/kmin of kernels/prelude.h as the reference's serial CPU build evaluates
+// them (x86-64 std::fmax/fmin): a NaN operand yields the other operand, and
+// otherwise the first operand is returned only when strictly larger (smaller),
+// so equal operands, +0 and -0 included, yield the second. The hardware
+// v_max_f32/v_min_f32 order -0 below +0 and would flip the sign of zero
+// results (limiters, clamps) against the reference.
+__device__ __forceinline__ float kmax(float a, float b) {
+  return a != a ? b : (b != b ? a : (a > b ? a : b));
+}
+__device__ __forceinline__ float kmin(float a, float b) {
+  return a != a ? b : (b != b ? a : (a < b ? a : b));
+}
+
 // ------------------------------------------------------------ upward rounding
 // Upper bound of a float sum (kernels/prelude.h:82-100): TwoSum, then step up
 // one ulp only when round-to-nearest went down.

@@ -118,7 +118,7 @@ struct FvPressureFaces {
     const float dx = ox - qx, dy = oy - qy, dz = oz - qz;
     const float mx = 0.5f * (ox + qx), my = 0.5f * (oy + qy), mz = 0.5f * (oz + qz);
     const float d2 = dx * dx + dy * dy + dz * dz, m2 = mx * mx + my * my + mz * mz;
-    const float trust = fmaxf(0.0f, fminf(1.0f, (4.0f * d2 - m2) / (3.0f * d2)));
+    const float trust = kmax(0.0f, kmin(1.0f, (4.0f * d2 - m2) / (3.0f * d2)));
     const float link = p[c] - p[n] + 0.5f * ((gx[c] + gx[n]) * dx + (gy[c] + gy[n]) * dy + (gz[c] + gz[n]) * dz);
     jump[i] = trust == 1.0f ? sp - sn : link + trust * ((sp - sn) - link);
   }
@@ -166,7 +166,7 @@ struct FvMomentumFaces {
 struct FvConvectiveOutflow {
   __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF flux, WF outflow) const {
     float total = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) total += fmaxf(0.0f, sign[j] * flux[faces[j]]);
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) total += kmax(0.0f, sign[j] * flux[faces[j]]);
     outflow[i] = total;
   }
 };
@@ -177,7 +177,7 @@ struct FvPecletSideVelocity {
                              CF outflow, WF face) const {
     const int c = owner[i], n = neighbour[i];
     const float a = weights[c] / (weights[c] + weights[n]);
-    const float beta = fminf(1.0f, (outflow[c] + outflow[n]) / fmaxf(diagonal[c] + diagonal[n], 1.0e-20f));
+    const float beta = kmin(1.0f, (outflow[c] + outflow[n]) / kmax(diagonal[c] + diagonal[n], 1.0e-20f));
     const float side = a * (q[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i]) +
                        (1.0f - a) * (q[n] + gx[n] * nx[i] + gy[n] * ny[i] + gz[n] * nz[i]);
     const float now = face[i];
@@ -191,10 +191,10 @@ struct FvRhieChow {
                              CF pressure_jump, WF flux, WF coefficient, int internal,
                              float rho) const {
     const int c = owner[i];
-    float rc = volume[c] / fmaxf(diagonal[c], 1.0e-20f), dc = volume[c] * inverse[c];
+    float rc = volume[c] / kmax(diagonal[c], 1.0e-20f), dc = volume[c] * inverse[c];
     if (i < internal) {
       const int n = neighbour[i];
-      rc = 0.5f * (rc + volume[n] / fmaxf(diagonal[n], 1.0e-20f));
+      rc = 0.5f * (rc + volume[n] / kmax(diagonal[n], 1.0e-20f));
       dc = 0.5f * (dc + volume[n] * inverse[n]);
     } else {
       const int k = kind[i - internal];
@@ -228,7 +228,7 @@ struct FvCorrectVelocity {
                              CF correction, WF u, WF v, WF w, WF p, float alpha) const {
     const float inv = inverse[i], d = volume[i] * inv;
     u[i] -= d * gx[i]; v[i] -= d * gy[i]; w[i] -= d * gz[i];
-    p[i] += fminf(alpha, 0.7f * diagonal[i] * inv) * correction[i];
+    p[i] += kmin(alpha, 0.7f * diagonal[i] * inv) * correction[i];
   }
 };
 // :204-215
@@ -322,10 +322,10 @@ struct FvFaceTimeFlux {
                              CF defect1, CF defect2, WF mass, int internal, float rho_dt,
                              float a1, float a2) const {
     const int c = owner[i];
-    float d = volume[c] / fmaxf(diagonal[c], 1.0e-20f);
+    float d = volume[c] / kmax(diagonal[c], 1.0e-20f);
     if (i < internal) {
       const int n = neighbour[i];
-      d = 0.5f * (d + volume[n] / fmaxf(diagonal[n], 1.0e-20f));
+      d = 0.5f * (d + volume[n] / kmax(diagonal[n], 1.0e-20f));
     } else {
       const int k = kind[i - internal];
       if (k == 0 || k == 1 || k == 3) return;
@@ -456,10 +456,10 @@ __device__ __forceinline__ float limit_cell(int i, float gx, float gy, float gz,
     const int f = faces[j];
     const bool own = sign[j] > 0.0f;
     const float d = gx * (own ? rx[f] : nx[f]) + gy * (own ? ry[f] : ny[f]) + gz * (own ? rz[f] : nz[f]);
-    if (d > 0.0f) psi = fminf(psi, limit_face(d, up, eps2));
-    if (d < 0.0f) psi = fminf(psi, limit_face(d, down, eps2));
+    if (d > 0.0f) psi = kmin(psi, limit_face(d, up, eps2));
+    if (d < 0.0f) psi = kmin(psi, limit_face(d, down, eps2));
   }
-  return fmaxf(0.0f, fminf(1.0f, psi));
+  return kmax(0.0f, kmin(1.0f, psi));
 }
 // :76-94
 struct FvLimiter {
@@ -470,7 +470,7 @@ struct FvLimiter {
     float lo = own, hi = own;
     for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
       const float v = across(adjacent[j], cell, boundary);
-      lo = fminf(lo, v); hi = fmaxf(hi, v);
+      lo = kmin(lo, v); hi = kmax(hi, v);
     }
     limiter[i] = limit_cell(i, gx[i], gy[i], gz[i], hi - own, lo - own, offsets, faces, sign, rx,
                             ry, rz, nx, ny, nz, eps2);
@@ -488,7 +488,7 @@ struct FvPairedLimiter {
       const float v = across(o, cell, boundary);
       const float l = o < 0 ? (kind[-o - 1] == 2 ? own_low : 0.0f) : low[o];
       const float d = (v - own) + (l - own_low);
-      lo = fminf(lo, d); hi = fmaxf(hi, d);
+      lo = kmin(lo, d); hi = kmax(hi, d);
     }
     limiter[i] = limit_cell(i, gx[i], gy[i], gz[i], hi, lo, offsets, faces, sign, rx, ry, rz, nx,
                             ny, nz, eps2);
@@ -506,7 +506,7 @@ struct FvPairedGradientLimiter {
       const int o = adjacent[j];
       const F2 v{across(o, cell, boundary), o < 0 ? boundary_low[-o - 1] : low[o]};
       const float d = pair_delta(v, own);
-      lo = fminf(lo, d); hi = fmaxf(hi, d);
+      lo = kmin(lo, d); hi = kmax(hi, d);
     }
     limiter[i] = limit_cell(i, gx[i] + lx[i], gy[i] + ly[i], gz[i] + lz[i], hi, lo, offsets,
                             faces, sign, rx, ry, rz, nx, ny, nz, eps2);
@@ -685,6 +685,798 @@ struct FvLsqRestoreInverse {
   }
 };
 
+// --------------------------------------------------- fv_transport.kern.h
+// :3-10
+struct FvDiffusionCorrectionBound {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF field, CF boundary,
+                             CF diffusion, WF bound, int internal) const {
+    const float own = field[owner[i]];
+    float other = own;
+    if (i < internal) other = field[neighbour[i]];
+    else if (kind[i - internal] == 0) other = boundary[i - internal];
+    bound[i] = diffusion[i] * fabsf(other - own);
+  }
+};
+// :12-35. Deferred correction of face i: the non-orthogonal diffusion part
+// from the face-weighted gradient, minus the limited second-order upwind part.
+struct FvTransportFaces {
+  __device__ void operator()(int i, CI owner, CI neighbour, CF weight, CF px, CF py, CF pz, CF nx,
+                             CF ny, CF nz, CF tx, CF ty, CF tz, CF gx, CF gy, CF gz, CF limiter,
+                             CF flux, CF mu, CI kind, WF correction, int internal,
+                             int second_order, int limited_diffusion) const {
+    const int c = owner[i];
+    const float m = flux[i];
+    float ax = gx[c], ay = gy[c], az = gz[c], convection = 0.0f;
+    if (i < internal) {
+      const int n = neighbour[i];
+      const float w = weight[i], v = 1.0f - w;
+      ax = w * ax + v * gx[n]; ay = w * ay + v * gy[n]; az = w * az + v * gz[n];
+      const bool forward = m >= 0.0f;
+      const int up = forward ? c : n;
+      const float rx = forward ? px[i] : nx[i], ry = forward ? py[i] : ny[i], rz = forward ? pz[i] : nz[i];
+      convection = limiter[up] * (gx[up] * rx + gy[up] * ry + gz[up] * rz) * m;
+    } else {
+      const int k = kind[i - internal];
+      if (k != 0) ax = ay = az = 0.0f;
+      if (m > 0.0f && k != 2) convection = limiter[c] * (gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i]) * m;
+    }
+    float diffusion = ax * tx[i] + ay * ty[i] + az * tz[i];
+    if (limited_diffusion) {
+      const float bound = correction[i];
+      diffusion = kmax(-bound, kmin(bound, diffusion));
+    }
+    correction[i] = mu[i] * diffusion - (second_order != 0 ? convection : 0.0f);
+  }
+};
+// :37-112. One transport-matrix row per lane, written into the row's own
+// entries only: off-diagonals -D + min(F, 0); the diagonal as the upward-
+// rounded sum of |off-diagonals|, net outflow, boundary anchors and sinks;
+// the right-hand side optionally carried as a (hi, lo) pair; relaxation with
+// the pseudo-time term; and the SIMPLEC consistent inverse.
+struct FvTransportAssemble {
+  __device__ void operator()(int i, CI offsets, CI faces, CF sign, CI adjacent, CI entry, CI rows,
+                             CI diag, CF diffusion, CF flux, CF mu, CI kind, CF boundary,
+                             CF correction, CF source, CF volume, CF old, CF sink,
+                             CF extra_diagonal, CF extra_rhs, WF values, WF rhs, WF rhs_low,
+                             WF diagonal, WF diagonal_low, WF consistent_inverse, int internal,
+                             float rho, float dt, float relax, int compensated) const {
+    const float vol = volume[i], sink_volume = sink[i] * vol;
+    UpperSum ap = upper_add({sink_volume, 0.0f}, extra_diagonal[i]);
+    float b = source[i] * vol + extra_rhs[i], carry = 0.0f;
+    if (compensated) {
+      const float product = source[i] * vol;
+      const F2 start = two_sum(product, extra_rhs[i]);
+      b = start.hi;
+      carry = start.lo + __builtin_fmaf(source[i], vol, -product);
+    }
+    const int begin = offsets[i], end = offsets[i + 1];
+    for (int j = begin; j < end; ++j)
+      if (adjacent[j] >= 0) values[entry[j]] = 0.0f;
+    for (int j = begin; j < end; ++j) {
+      const int f = faces[j], o = adjacent[j];
+      const float mass = sign[j] * flux[f], inflow = kmin(mass, 0.0f);
+      float d = mu[f] * diffusion[f];
+      ap = upper_add(ap, mass);
+      if (o >= 0) {
+        values[entry[j]] += -d + inflow;
+      } else {
+        const int bf = f - internal;
+        if (kind[bf] != 0) d = 0.0f;
+        ap = upper_add(ap, d);
+        ap = upper_add(ap, -inflow);
+        const float value = boundary[bf];
+        if (compensated) {
+          const F2 factor = two_sum(d, -inflow);
+          const float product = factor.hi * value;
+          const F2 sum = two_sum(b, product);
+          carry += sum.lo + __builtin_fmaf(factor.hi, value, -product) + factor.lo * value;
+          b = sum.hi;
+        } else {
+          b += (d - inflow) * value;
+        }
+      }
+      const float deferred = sign[j] * correction[f];
+      if (compensated) {
+        const F2 sum = two_sum(b, deferred);
+        carry += sum.lo;
+        b = sum.hi;
+      } else {
+        b += deferred;
+      }
+    }
+    const int di = diag[i], row_begin = rows[i], row_end = rows[i + 1];
+    for (int j = row_begin; j < row_end; ++j)
+      if (j != di) ap = upper_add(ap, -values[j]);
+    const float physical = upper_value(ap);
+    diagonal[i] = physical;
+    if (compensated) {
+      const F2 gap = two_sum(ap.sum, -physical);
+      diagonal_low[i] = gap.hi + (gap.lo + ap.error + __builtin_fmaf(sink[i], vol, -sink_volume));
+    }
+    const float pseudo_time = dt > 0.0f ? rho * vol / dt : 0.0f;
+    const float relaxed = add_up(physical, pseudo_time) / relax;
+    const float previous = old[i];
+    if (compensated) {
+      const F2 factor = two_sum(relaxed, -physical);
+      const float product = factor.hi * previous;
+      const F2 sum = two_sum(b, product);
+      carry += sum.lo + __builtin_fmaf(factor.hi, previous, -product) + factor.lo * previous;
+      b = sum.hi;
+    } else {
+      b += (relaxed - physical) * previous;
+    }
+    values[di] = relaxed;
+    rhs[i] = b;
+    if (compensated) rhs_low[i] = carry;
+    UpperSum denominator{relaxed, 0.0f};
+    for (int j = row_begin; j < row_end; ++j)
+      if (j != di) denominator = upper_add(denominator, values[j]);
+    const float consistent = upper_value(denominator);
+    consistent_inverse[i] = 1.0f / kmax(consistent > 0.0f ? consistent : relaxed, 1.0e-20f);
+  }
+};
+// :115-124
+struct FvFixedRows {
+  __device__ void operator()(int i, CI rows, CI diag, CI mask, CF fixed, WF values, WF rhs) const {
+    if (mask[i] == 0) return;
+    const int di = diag[i];
+    const float scale = values[di];
+    for (int j = rows[i]; j < rows[i + 1]; ++j) values[j] = 0.0f;
+    values[di] = scale;
+    rhs[i] = scale * fixed[i];
+  }
+};
+// :126-128
+struct FvFloor {
+  __device__ void operator()(int i, WF field, float floor) const { field[i] = kmax(field[i], floor); }
+};
+// :130-135
+struct FvScalarSteadyMeasure {
+  __device__ void operator()(int i, CF field, CF diagonal, CF residual, CI excluded, WF absolute,
+                             WF scale, int has_excluded) const {
+    const bool omit = has_excluded && excluded[i] != 0;
+    absolute[i] = omit ? 0.0f : fabsf(residual[i]);
+    scale[i] = omit ? 0.0f : fabsf(diagonal[i] * field[i]);
+  }
+};
+// :137-139
+struct FvScalarSetFixed {
+  __device__ void operator()(int i, CI fixed, CF value, WF field) const {
+    if (fixed[i]) field[i] = value[i];
+  }
+};
+// :141-148
+struct FvScalarMonitor {
+  __device__ void operator()(int i, CF field, CF diagonal, CF residual, CI excluded, CF volume,
+                             WF absolute, WF scale, WF fluent_scale, float rho, float dt) const {
+    const bool omit = excluded[i] != 0;
+    const float d = diagonal[i], v = field[i];
+    absolute[i] = omit ? 0.0f : fabsf(residual[i]);
+    scale[i] = omit ? 0.0f : fabsf(d * v);
+    const float pseudo_time = dt > 0.0f ? rho * volume[i] / dt : 0.0f;
+    fluent_scale[i] = omit ? 0.0f : fabsf((d + pseudo_time) * v);
+  }
+};
+
+// ------------------------------------------------------ fv_paired.kern.h
+// Pair additions with hysteresis (kernels/prelude.h:133-171): the high word
+// stays put while the low word absorbs increments within `slack`.
+__device__ __forceinline__ F2 pair_add(float hi, float lo, float d, float slack) {
+  const float low = lo + d;
+  if (fabsf(low) <= slack) return {hi, low};
+  const F2 s = two_sum(hi, d);
+  return two_sum(s.hi, s.lo + lo);
+}
+__device__ __forceinline__ F2 pair_sum3(float hi, float lo, float d) {
+  const F2 base = two_sum(hi, lo), increment = two_sum(base.lo, d);
+  const F2 value = two_sum(base.hi, increment.hi);
+  return two_sum(value.hi, value.lo + increment.lo);
+}
+__device__ __forceinline__ float held_offset(F2 value, float hi) {
+  const F2 difference = two_sum(value.hi, -hi);
+  return difference.hi + (difference.lo + value.lo);
+}
+__device__ __forceinline__ F2 pair_add_relative(float hi, float lo, float d, float relative_slack) {
+  const F2 base = two_sum(hi, lo);
+  const F2 value = pair_sum3(hi, lo, d);
+  const float slack = relative_slack * kmax(fabsf(base.hi), fabsf(value.hi));
+  const float offset = held_offset(value, hi);
+  if (fabsf(offset) <= slack) return {hi, offset};
+  return value;
+}
+__device__ __forceinline__ bool same_sign(float a, float b) {
+  return (a > 0.0f && b > 0.0f) || (a < 0.0f && b < 0.0f) || (a == 0.0f && b == 0.0f);
+}
+__device__ __forceinline__ F2 pair_add_flux(float hi, float lo, float d, float slack) {
+  const F2 value = pair_sum3(hi, lo, d);
+  const float offset = held_offset(value, hi);
+  if (same_sign(hi, value.hi) && fabsf(offset) <= slack) return {hi, offset};
+  return value;
+}
+__device__ __forceinline__ float link_trust(float dx, float dy, float dz, float mx, float my,
+                                            float mz) {
+  const float d2 = dx * dx + dy * dy + dz * dz, m2 = mx * mx + my * my + mz * mz;
+  return kmax(0.0f, kmin(1.0f, (4.0f * d2 - m2) / (3.0f * d2)));
+}
+__device__ __forceinline__ float trace_difference(F2 a, F2 b) {
+  F2 d = two_sum(a.hi, -b.hi);
+  d = product_add(d, a.lo, 1.0f);
+  d = product_add(d, -b.lo, 1.0f);
+  return d.hi + d.lo;
+}
+__device__ __forceinline__ float face_slack(float slack_per_area, float sx, float sy, float sz) {
+  return slack_per_area * sqrtf(sx * sx + sy * sy + sz * sz);
+}
+
+// :5-33
+struct FvCompensatedPressureJump {
+  __device__ void operator()(int i, CI owner, CI neighbour, CF px, CF py, CF pz, CF nx, CF ny,
+                             CF nz, CF p, CF gx, CF gy, CF gz, CF boundary, WF jump,
+                             int internal) const {
+    const int c = owner[i];
+    const float ox = px[i], oy = py[i], oz = pz[i];
+    const bool face = i < internal;
+    const int n = face ? neighbour[i] : 0;
+    F2 full = two_sum(p[c], face ? -p[n] : -boundary[i - internal]);
+    full = product_add(full, gx[c], ox);
+    full = product_add(full, gy[c], oy);
+    full = product_add(full, gz[c], oz);
+    if (!face) {
+      jump[i] = full.hi + full.lo;
+      return;
+    }
+    const float qx = nx[i], qy = ny[i], qz = nz[i];
+    full = product_add(full, -gx[n], qx);
+    full = product_add(full, -gy[n], qy);
+    full = product_add(full, -gz[n], qz);
+    const float side = full.hi + full.lo;
+    const float trust = link_trust(ox - qx, oy - qy, oz - qz, 0.5f * (ox + qx), 0.5f * (oy + qy), 0.5f * (oz + qz));
+    if (trust == 1.0f) {
+      jump[i] = side;
+      return;
+    }
+    F2 link = two_sum(p[c], -p[n]);
+    link = product_add(link, gx[c], 0.5f * ox); link = product_add(link, gx[n], 0.5f * ox);
+    link = product_add(link, gx[c], -0.5f * qx); link = product_add(link, gx[n], -0.5f * qx);
+    link = product_add(link, gy[c], 0.5f * oy); link = product_add(link, gy[n], 0.5f * oy);
+    link = product_add(link, gy[c], -0.5f * qy); link = product_add(link, gy[n], -0.5f * qy);
+    link = product_add(link, gz[c], 0.5f * oz); link = product_add(link, gz[n], 0.5f * oz);
+    link = product_add(link, gz[c], -0.5f * qz); link = product_add(link, gz[n], -0.5f * qz);
+    const float shared = link.hi + link.lo;
+    jump[i] = shared + trust * (side - shared);
+  }
+};
+// :46-65
+struct FvPairedPressureJump {
+  __device__ void operator()(int i, CI owner, CI neighbour, CF px, CF py, CF pz, CF nx, CF ny,
+                             CF nz, CF p, CF low, CF gx, CF gy, CF gz, CF lx, CF ly, CF lz,
+                             CF boundary, CF boundary_low, WF jump, int internal) const {
+    const int c = owner[i];
+    const float ox = px[i], oy = py[i], oz = pz[i];
+    const F2 own = pair_trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], ox, oy, oz);
+    if (i >= internal) {
+      jump[i] = trace_difference(own, {boundary[i - internal], boundary_low[i - internal]});
+      return;
+    }
+    const int n = neighbour[i];
+    const float qx = nx[i], qy = ny[i], qz = nz[i];
+    const F2 next = pair_trace(p[n], low[n], gx[n], gy[n], gz[n], lx[n], ly[n], lz[n], qx, qy, qz);
+    const float side = trace_difference(own, next);
+    const float dx = ox - qx, dy = oy - qy, dz = oz - qz;
+    const float trust = link_trust(dx, dy, dz, 0.5f * (ox + qx), 0.5f * (oy + qy), 0.5f * (oz + qz));
+    if (trust == 1.0f) {
+      jump[i] = side;
+      return;
+    }
+    const F2 a = pair_trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], .5f * dx, .5f * dy, .5f * dz);
+    const F2 b = pair_trace(p[n], low[n], gx[n], gy[n], gz[n], lx[n], ly[n], lz[n], -.5f * dx, -.5f * dy, -.5f * dz);
+    const float shared = trace_difference(a, b);
+    jump[i] = shared + trust * (side - shared);
+  }
+};
+// :69-85
+struct FvCompensatedRhieChow {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF sx, CF sy, CF sz,
+                             CF diffusion, CF volume, CF diagonal, CF u, CF v, CF w, CF lu, CF lv,
+                             CF lw, CF jump, WF flux, WF low, int internal, float rho) const {
+    const bool boundary = i >= internal;
+    const int k = boundary ? kind[i - internal] : -1;
+    if (k == 0 || k == 3) {
+      flux[i] = 0.0f; low[i] = 0.0f;
+      return;
+    }
+    const int c = owner[i];
+    float rc = volume[c] / kmax(diagonal[c], 1.0e-20f);
+    if (!boundary) {
+      const int n = neighbour[i];
+      rc = .5f * (rc + volume[n] / kmax(diagonal[n], 1.0e-20f));
+    }
+    const float ax = sx[i], ay = sy[i], az = sz[i];
+    F2 value{0.0f, 0.0f};
+    value = product_add(value, u[i], ax); value = product_add(value, lu[i], ax);
+    value = product_add(value, v[i], ay); value = product_add(value, lv[i], ay);
+    value = product_add(value, w[i], az); value = product_add(value, lw[i], az);
+    if (k != 1) value = product_add(value, rc * diffusion[i], jump[i]);
+    const F2 scaled = add_weighted({0.0f, 0.0f}, rho, value);
+    flux[i] = scaled.hi;
+    low[i] = scaled.lo;
+  }
+};
+// :89-97
+struct FvPairedPressureBoundaryLow {
+  __device__ void operator()(int i, CI owner, CI kind, CF px, CF py, CF pz, CF p, CF low, CF gx,
+                             CF gy, CF gz, CF lx, CF ly, CF lz, CF boundary, WF boundary_low,
+                             int internal) const {
+    const int k = kind[i];
+    if (k == 2 || k == 4) {
+      boundary_low[i] = 0.0f;
+      return;
+    }
+    const int f = i + internal, c = owner[f];
+    F2 value = pair_trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[f], py[f], pz[f]);
+    value = product_add(value, -boundary[i], 1.0f);
+    boundary_low[i] = value.hi + value.lo;
+  }
+};
+// :98-128
+struct FvPairedPressureFaces {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CI poor, CF px, CF py, CF pz,
+                             CF nx, CF ny, CF nz, CF p, CF low, CF gx, CF gy, CF gz, CF lx,
+                             CF ly, CF lz, CF boundary, CF boundary_low, CF diagonal, WF face,
+                             WF face_low, int internal, int standard) const {
+    const int c = owner[i];
+    F2 value = two_sum(p[c], low[c]);
+    if (i < internal) {
+      const int n = neighbour[i];
+      if (standard) {
+        const float wc = 1.0f / kmax(diagonal[c], 1.0e-20f), wn = 1.0f / kmax(diagonal[n], 1.0e-20f);
+        F2 num{0.0f, 0.0f};
+        num = product_add(num, wc, p[c]); num = product_add(num, wc, low[c]);
+        num = product_add(num, wn, p[n]); num = product_add(num, wn, low[n]);
+        const F2 den = two_sum(wc, wn);
+        const float q = num.hi / den.hi;
+        num = product_add(num, -q, den.hi);
+        num = product_add(num, -q, den.lo);
+        value = two_sum(q, (num.hi + num.lo) / den.hi);
+      } else {
+        const bool plain = poor[c] || poor[n];
+        const F2 own = plain ? two_sum(p[c], low[c])
+                             : pair_trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[i], py[i], pz[i]);
+        const F2 next = plain ? two_sum(p[n], low[n])
+                              : pair_trace(p[n], low[n], gx[n], gy[n], gz[n], lx[n], ly[n], lz[n], nx[i], ny[i], nz[i]);
+        value = two_sum(0.5f * own.hi, 0.5f * next.hi);
+        value = two_sum(value.hi, value.lo + 0.5f * own.lo + 0.5f * next.lo);
+      }
+    } else {
+      const int b = i - internal, k = kind[b];
+      if (k == 2 || k == 4) value = two_sum(boundary[b], boundary_low[b]);
+      else if (!standard && !poor[c])
+        value = pair_trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[i], py[i], pz[i]);
+    }
+    face[i] = value.hi;
+    face_low[i] = value.lo;
+  }
+};
+// :140-152
+struct FvPairedMomentumResidual {
+  __device__ void operator()(int i, CI rows, CI cols, CF values, CI diag, CF physical,
+                             CF physical_low, CF hi, CF lo, CF pressure_gradient_lo, CF volume,
+                             CF rhs, CF rhs_low, WF r) const {
+    float sum = rhs[i], error = rhs_low[i];
+    for (int j = rows[i]; j < rows[i + 1]; ++j) {
+      const int col = cols[j];
+      const float a = values[j], x = hi[col], product = a * x;
+      const F2 s = two_sum(sum, -product);
+      sum = s.hi;
+      error += s.lo - __builtin_fmaf(a, x, -product) - a * lo[col];
+    }
+    const float pl = physical_low[i], li = lo[i];
+    error += (values[diag[i]] - physical[i]) * li - volume[i] * pressure_gradient_lo[i];
+    error -= pl * hi[i] + pl * li;
+    r[i] = sum + error;
+  }
+};
+// :156-170
+struct FvCompensatedMomentumResidual {
+  __device__ void operator()(int i, CI rows, CI cols, CF values, CI diag, CF physical,
+                             CF physical_low, CF hi, CF lo, CF pressure_gradient_lo, CF volume,
+                             CF rhs, CF rhs_low, WF r) const {
+    F2 value = two_sum(rhs[i], rhs_low[i]);
+    const int di = diag[i];
+    for (int j = rows[i]; j < rows[i + 1]; ++j) {
+      const int col = cols[j];
+      value = product_add(value, -values[j], hi[col]);
+      const float low_coefficient = j == di ? physical[i] : values[j];
+      value = product_add(value, -low_coefficient, lo[col]);
+    }
+    value = product_add(value, -physical_low[i], hi[i]);
+    value = product_add(value, -physical_low[i], lo[i]);
+    value = product_add(value, -volume[i], pressure_gradient_lo[i]);
+    r[i] = value.hi + value.lo;
+  }
+};
+// :172-174
+struct FvPairedAccumulate {
+  __device__ void operator()(int i, CF delta, WF hi, WF lo, float slack) const {
+    const F2 x = pair_add(hi[i], lo[i], delta[i], slack);
+    hi[i] = x.hi; lo[i] = x.lo;
+  }
+};
+// :179-196
+struct FvCompensatedPressureForce {
+  __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF sx, CF sy, CF sz, CF volume,
+                             CF pressure, CF low, WF x, WF y, WF z) const {
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    F2 ax{0.0f, 0.0f}, ay{0.0f, 0.0f}, az{0.0f, 0.0f};
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+      const int f = faces[j];
+      const float s = sign[j], hp = s * pressure[f], lp = s * low[f];
+      const float fx = sx[f], fy = sy[f], fz = sz[f];
+      a += hp * fx; b += hp * fy; c += hp * fz;
+      ax = product_add(product_add(ax, hp, fx), lp, fx);
+      ay = product_add(product_add(ay, hp, fy), lp, fy);
+      az = product_add(product_add(az, hp, fz), lp, fz);
+    }
+    const float vol = volume[i];
+    ax = product_add(ax, -a / vol, vol);
+    ay = product_add(ay, -b / vol, vol);
+    az = product_add(az, -c / vol, vol);
+    x[i] = (ax.hi + ax.lo) / vol; y[i] = (ay.hi + ay.lo) / vol; z[i] = (az.hi + az.lo) / vol;
+  }
+};
+// :200-218
+struct FvPairedVelocityBoundary {
+  __device__ void operator()(int i, CI owner, CI kind, CF sx, CF sy, CF sz, CF u, CF v, CF w,
+                             CF flux, CF flux_low, WF bu, WF bv, WF bw, WI scalar_kind,
+                             int internal, float rho) const {
+    const int f = i + internal, c = owner[f], k = kind[i];
+    const float ax = sx[f], ay = sy[f], az = sz[f], m = flux[f];
+    const float area2 = ax * ax + ay * ay + az * az;
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    int sk = 0;
+    if (((k == 2 || k == 4) && m >= 0.0f) || k == 3) {
+      vx = u[c]; vy = v[c]; vz = w[c];
+      sk = 2;
+    }
+    if (k == 4 && m < 0.0f) {
+      const float factor = flux_low[f] / (rho * area2);
+      vx = factor * ax; vy = factor * ay; vz = factor * az;
+    }
+    if (k == 3) {
+      const float un = (vx * ax + vy * ay + vz * az) / area2;
+      vx -= un * ax; vy -= un * ay; vz -= un * az;
+    }
+    scalar_kind[i] = sk;
+    bu[i] = vx; bv[i] = vy; bw[i] = vz;
+  }
+};
+// :220-236
+struct FvPairedAdvection {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF px, CF py, CF pz, CF nx,
+                             CF ny, CF nz, CF field, CF boundary, CF gx, CF gy, CF gz, CF limiter,
+                             CF flux, CF flux_low, WF correction, int internal,
+                             int second_order) const {
+    const int c = owner[i];
+    const float m = flux[i];
+    int up = c;
+    float value = field[c], rx = px[i], ry = py[i], rz = pz[i];
+    bool reconstruct = second_order != 0;
+    if (i < internal) {
+      if (m < 0.0f) {
+        up = neighbour[i];
+        rx = nx[i]; ry = ny[i]; rz = nz[i];
+      }
+      value = field[up];
+    } else {
+      if (m < 0.0f) value = boundary[i - internal];
+      if (m <= 0.0f || kind[i - internal] == 2) reconstruct = false;
+    }
+    if (reconstruct) value += limiter[up] * (gx[up] * rx + gy[up] * ry + gz[up] * rz);
+    correction[i] = flux_low[i] * value;
+  }
+};
+// :238-247
+struct FvPairedFaceResidual {
+  __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF secondary, CF advection,
+                             CF stress, CF boundary_rhs, WF residual, int turbulent) const {
+    float sum = residual[i], error = boundary_rhs[i];
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+      const int f = faces[j];
+      const float value = sign[j] * (secondary[f] - advection[f] + (turbulent ? stress[f] : 0.0f));
+      const F2 s = two_sum(sum, value);
+      sum = s.hi;
+      error += s.lo;
+    }
+    residual[i] = sum + error;
+  }
+};
+// :259-290
+struct FvPairedFlux {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF sx, CF sy, CF sz, CF px,
+                             CF py, CF pz, CF nx, CF ny, CF nz, CF weights, CF diagonal,
+                             CF volume, CF diffusion, CF u, CF v, CF w, CF p, CF gx, CF gy, CF gz,
+                             CF previous, WF flux, WF flux_lo, int internal, float rho,
+                             float slack_per_area) const {
+    const int c = owner[i];
+    const float ax = sx[i], ay = sy[i], az = sz[i];
+    float rc = volume[c] / kmax(diagonal[c], 1.0e-20f), low = 0.0f;
+    if (i < internal) {
+      const int n = neighbour[i];
+      const float a = weights[c] / (weights[c] + weights[n]), b = 1.0f - a;
+      const float ux = a * u[c] + b * u[n], uy = a * v[c] + b * v[n], uz = a * w[c] + b * w[n];
+      rc = 0.5f * (rc + volume[n] / kmax(diagonal[n], 1.0e-20f));
+      const float dx = px[i] - nx[i], dy = py[i] - ny[i], dz = pz[i] - nz[i];
+      const float jump = p[c] - p[n] + 0.5f * ((gx[c] + gx[n]) * dx + (gy[c] + gy[n]) * dy + (gz[c] + gz[n]) * dz);
+      low = rho * (ux * ax + uy * ay + uz * az) + rho * rc * diffusion[i] * jump;
+    } else {
+      const int k = kind[i - internal];
+      if (k == 2 || k == 4) {
+        const float jump = p[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i];
+        low = rho * (u[c] * ax + v[c] * ay + w[c] * az) + rho * rc * diffusion[i] * jump;
+      }
+    }
+    const float slack = face_slack(slack_per_area, ax, ay, az);
+    const float held = previous[i];
+    const F2 x = two_sum(flux[i], -held);
+    const float offset = x.hi + (x.lo + low);
+    if (fabsf(offset) <= slack) {
+      flux[i] = held;
+      flux_lo[i] = offset;
+    } else {
+      flux_lo[i] = low;
+    }
+  }
+};
+// :292-303
+struct FvPairedFluxOffset {
+  __device__ void operator()(int i, CF sx, CF sy, CF sz, CF previous, WF flux, WF low,
+                             float slack_per_area) const {
+    const float slack = face_slack(slack_per_area, sx[i], sy[i], sz[i]);
+    const float held = previous[i];
+    const F2 total = two_sum(flux[i], low[i]);
+    const F2 x = two_sum(total.hi, -held);
+    const float offset = x.hi + (x.lo + total.lo);
+    if (same_sign(held, total.hi) && fabsf(offset) <= slack) {
+      flux[i] = held; low[i] = offset;
+    } else {
+      flux[i] = total.hi; low[i] = total.lo;
+    }
+  }
+};
+// :308-318
+struct FvPairedContinuity {
+  __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF flux, CF flux_lo,
+                             WF imbalance, WF magnitude) const {
+    float sum = 0.0f, error = 0.0f;
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+      const int f = faces[j];
+      const float s = sign[j];
+      const F2 t = two_sum(sum, s * flux[f]);
+      sum = t.hi;
+      error += t.lo + s * flux_lo[f];
+    }
+    const float mass = sum + error;
+    imbalance[i] = mass;
+    magnitude[i] = fabsf(mass);
+  }
+};
+// :321-323
+struct FvPairedPressureRhs {
+  __device__ void operator()(int i, CF imbalance, WF rhs, int reference) const {
+    rhs[i] = i == reference ? 0.0f : -imbalance[i];
+  }
+};
+// :326-333
+struct FvPairedCorrectFlux {
+  __device__ void operator()(int i, CI owner, CI neighbour, CF sx, CF sy, CF sz, CF correction,
+                             CF coefficient, WF flux, WF flux_lo, int internal,
+                             float slack_per_area) const {
+    const float pc = correction[owner[i]];
+    const float pn = i < internal ? correction[neighbour[i]] : 0.0f;
+    const float slack = face_slack(slack_per_area, sx[i], sy[i], sz[i]);
+    const F2 x = pair_add(flux[i], flux_lo[i], coefficient[i] * (pc - pn), slack);
+    flux[i] = x.hi; flux_lo[i] = x.lo;
+  }
+};
+// Velocity and pressure pair corrections of fv_paired_correct_velocity
+// (:336-347) and fv_compensated_correct_velocity (:365-377), which differ
+// only in the pressure pair's hysteresis rule.
+template <bool Relative>
+__device__ __forceinline__ void correct_velocity_pairs(int i, CF volume, CF inverse, CF diagonal,
+                                                       CF gx, CF gy, CF gz, CF correction, WF u,
+                                                       WF v, WF w, WF p, WF u_lo, WF v_lo,
+                                                       WF w_lo, WF p_lo, float alpha,
+                                                       float velocity_slack,
+                                                       float pressure_slack) {
+  const float inv = inverse[i], d = volume[i] * inv;
+  F2 x = pair_add(u[i], u_lo[i], -d * gx[i], velocity_slack);
+  u[i] = x.hi; u_lo[i] = x.lo;
+  x = pair_add(v[i], v_lo[i], -d * gy[i], velocity_slack);
+  v[i] = x.hi; v_lo[i] = x.lo;
+  x = pair_add(w[i], w_lo[i], -d * gz[i], velocity_slack);
+  w[i] = x.hi; w_lo[i] = x.lo;
+  const float step = kmin(alpha, 0.7f * diagonal[i] * inv) * correction[i];
+  x = Relative ? pair_add_relative(p[i], p_lo[i], step, pressure_slack)
+               : pair_add(p[i], p_lo[i], step, pressure_slack);
+  p[i] = x.hi; p_lo[i] = x.lo;
+}
+struct FvPairedCorrectVelocity {
+  __device__ void operator()(int i, CF volume, CF inverse, CF diagonal, CF gx, CF gy, CF gz,
+                             CF correction, WF u, WF v, WF w, WF p, WF u_lo, WF v_lo, WF w_lo,
+                             WF p_lo, float alpha, float velocity_slack,
+                             float pressure_slack) const {
+    correct_velocity_pairs<false>(i, volume, inverse, diagonal, gx, gy, gz, correction, u, v, w, p,
+                                  u_lo, v_lo, w_lo, p_lo, alpha, velocity_slack, pressure_slack);
+  }
+};
+// :349-351
+struct FvPairedValue {
+  __device__ void operator()(int i, CF hi, CF lo, WF value) const { value[i] = hi[i] + lo[i]; }
+};
+// :352-355
+struct FvPairedVectorValues {
+  __device__ void operator()(int i, CF x, CF y, CF z, CF lx, CF ly, CF lz, WF u, WF v,
+                             WF w) const {
+    u[i] = x[i] + lx[i]; v[i] = y[i] + ly[i]; w[i] = z[i] + lz[i];
+  }
+};
+// :356-363
+struct FvCompensatedCorrectFlux {
+  __device__ void operator()(int i, CI owner, CI neighbour, CF sx, CF sy, CF sz, CF correction,
+                             CF coefficient, WF flux, WF flux_lo, int internal,
+                             float slack_per_area) const {
+    const float pn = i < internal ? correction[neighbour[i]] : 0.0f;
+    const float slack = face_slack(slack_per_area, sx[i], sy[i], sz[i]);
+    const F2 x = pair_add_flux(flux[i], flux_lo[i], coefficient[i] * (correction[owner[i]] - pn), slack);
+    flux[i] = x.hi; flux_lo[i] = x.lo;
+  }
+};
+struct FvCompensatedCorrectVelocity {
+  __device__ void operator()(int i, CF volume, CF inverse, CF diagonal, CF gx, CF gy, CF gz,
+                             CF correction, WF u, WF v, WF w, WF p, WF u_lo, WF v_lo, WF w_lo,
+                             WF p_lo, float alpha, float velocity_slack,
+                             float pressure_slack) const {
+    correct_velocity_pairs<true>(i, volume, inverse, diagonal, gx, gy, gz, correction, u, v, w, p,
+                                 u_lo, v_lo, w_lo, p_lo, alpha, velocity_slack, pressure_slack);
+  }
+};
+
+// -------------------------------------------------------- fv_rans.kern.h
+// :4-10
+struct FvBackflowPressure {
+  __device__ void operator()(int i, CI owner, CI total, CF mass, CF u, CF v, CF w, WF boundary,
+                             int internal, float rho) const {
+    const int f = i + internal;
+    if (total[i] != 0 && mass[f] < 0.0f) {
+      const int c = owner[f];
+      boundary[i] -= 0.5f * rho * (u[c] * u[c] + v[c] * v[c] + w[c] * w[c]);
+    }
+  }
+};
+// :12-17
+struct FvEffectiveViscosity {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF weight, CF mut, WF mu,
+                             int internal, float molecular) const {
+    float turbulent = mut[owner[i]];
+    if (i < internal) {
+      const float t = weight[i];
+      turbulent = t * turbulent + (1.0f - t) * mut[neighbour[i]];
+    }
+    mu[i] = i >= internal && kind[i - internal] == 0 ? 0.0f : molecular + turbulent;
+  }
+};
+// :21-37
+struct FvTransposeStress {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF weight, CF mu, CF sx,
+                             CF sy, CF sz, CF ux, CF uy, CF uz, CF vx, CF vy, CF vz, CF wx, CF wy,
+                             CF wz, WF x, WF y, WF z, int internal) const {
+    const int c = owner[i];
+    float a = ux[c], b = uy[c], d = uz[c], e = vx[c], g = vy[c], h = vz[c], j = wx[c], k = wy[c], l = wz[c];
+    if (i < internal) {
+      const int n = neighbour[i];
+      const float t = weight[i], s = 1.0f - t;
+      a = t * a + s * ux[n]; b = t * b + s * uy[n]; d = t * d + s * uz[n];
+      e = t * e + s * vx[n]; g = t * g + s * vy[n]; h = t * h + s * vz[n];
+      j = t * j + s * wx[n]; k = t * k + s * wy[n]; l = t * l + s * wz[n];
+    }
+    const float m = i >= internal && kind[i - internal] == 3 ? 0.0f : mu[i];
+    const float ax = sx[i], ay = sy[i], az = sz[i];
+    x[i] = m * (a * ax + e * ay + j * az);
+    y[i] = m * (b * ax + g * ay + k * az);
+    z[i] = m * (d * ax + h * ay + l * az);
+  }
+};
+// :38-46
+struct FvStressSource {
+  __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF volume, CF x, CF y, CF z,
+                             WF su, WF sv, WF sw) const {
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+      const int f = faces[j];
+      const float s = sign[j];
+      a += s * x[f]; b += s * y[f]; c += s * z[f];
+    }
+    const float vol = volume[i];
+    su[i] += a / vol; sv[i] += b / vol; sw[i] += c / vol;
+  }
+};
+// :47-72
+struct FvBoundaryMomentum {
+  __device__ void operator()(int i, CI offsets, CI faces, CI kind, CF sx, CF sy, CF sz,
+                             CF coefficient, CF mu, CF diffusion, CF u, CF v, CF w, CF bu, CF bv,
+                             CF bw, WF diagonal, WF rhs, int internal, int component,
+                             int wall_function) const {
+    float ap = 0.0f, b = 0.0f;
+    const float u0 = u[i], v0 = v[i], w0 = w[i];
+    const float uc = component == 0 ? u0 : (component == 1 ? v0 : w0);
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+      const int f = faces[j];
+      if (f < internal) continue;
+      const int bf = f - internal, k = kind[bf];
+      if (k != 3 && !(k == 0 && wall_function)) continue;
+      const float ax = sx[f], ay = sy[f], az = sz[f];
+      const float area = sqrtf(ax * ax + ay * ay + az * az);
+      const float ex = ax / area, ey = ay / area, ez = az / area;
+      const float nc = component == 0 ? ex : (component == 1 ? ey : ez);
+      if (k == 3) {
+        const float md = mu[f] * diffusion[f];
+        ap += md * nc * nc;
+        b -= md * nc * (ex * u0 + ey * v0 + ez * w0 - nc * uc);
+        continue;
+      }
+      const float wx = bu[bf], wy = bv[bf], wz = bw[bf];
+      const float wc = component == 0 ? wx : (component == 1 ? wy : wz);
+      const float c = area * coefficient[bf];
+      ap += c * (1.0f - nc * nc);
+      b += c * (wc - nc * (ex * wx + ey * wy + ez * wz) + nc * (ex * u0 + ey * v0 + ez * w0 - nc * uc));
+    }
+    diagonal[i] = ap;
+    rhs[i] = b;
+  }
+};
+// :75-78
+struct FvMomentumDiagonal {
+  __device__ void operator()(int i, CF ax, CF ay, CF az, CF ix, CF iy, CF iz, WF physical,
+                             WF inverse) const {
+    physical[i] = (ax[i] + ay[i] + az[i]) / 3.0f;
+    inverse[i] = (ix[i] + iy[i] + iz[i]) / 3.0f;
+  }
+};
+// :79-83
+struct FvSstForce {
+  __device__ void operator()(int i, CF selected, CF sx, CF sy, CF sz, CF tx, CF ty, CF tz, WF x,
+                             WF y, WF z, int internal) const {
+    const int f = i + internal;
+    const float a = -selected[i] * sqrtf(sx[f] * sx[f] + sy[f] * sy[f] + sz[f] * sz[f]);
+    x[i] = a * tx[i]; y[i] = a * ty[i]; z[i] = a * tz[i];
+  }
+};
+// :86-101
+struct FvTurbulenceBoundary {
+  __device__ void operator()(int i, CI owner, CI kind, CI spec, CF a, CF b, CF mass, CF bu, CF bv,
+                             CF bw, CF k, CF omega, CF wall_omega, WF bk, WF bo,
+                             WI transport_kind, int internal, float rho, float mu) const {
+    const int f = i + internal, c = owner[f], type = kind[i];
+    float kinetic = k[c], dissipation = omega[c];
+    int tk = 1;
+    if (type == 1 || ((type == 2 || type == 4) && mass[f] < 0.0f)) {
+      const float ai = a[i], bi = b[i];
+      kinetic = ai;
+      dissipation = bi;
+      const int sp = spec[i];
+      if (sp != 0) {
+        const float x = bu[i], y = bv[i], z = bw[i];
+        kinetic = 1.5f * ai * ai * (x * x + y * y + z * z);
+        dissipation = sp == 1 ? sqrtf(kinetic) / (0.5477225575f * bi) : rho * kinetic / (mu * bi);
+      }
+      tk = 0;
+    }
+    transport_kind[i] = tk;
+    bk[i] = kinetic;
+    bo[i] = type == 0 ? wall_omega[i] : dissipation;
+  }
+};
+
 template <class Body, class... P>
 __global__ void __launch_bounds__(kBlock) k_each(int n, P... p) {
   const int i = cell_index();
@@ -779,6 +1571,44 @@ const std::vector<KernelEntry>& fv_kernels() {
       FV(KH_FV_GATHER_JUMP_GRADIENT, "fv_gather_jump_gradient", FvGatherJumpGradient),
       FV(KH_FV_PAIRED_JUMP_GRADIENT, "fv_paired_jump_gradient", FvPairedJumpGradient),
       FV(KH_FV_LSQ_RESTORE_INVERSE, "fv_lsq_restore_inverse", FvLsqRestoreInverse),
+      FV(KH_FV_DIFFUSION_CORRECTION_BOUND, "fv_diffusion_correction_bound", FvDiffusionCorrectionBound),
+      FV(KH_FV_TRANSPORT_FACES, "fv_transport_faces", FvTransportFaces),
+      FV(KH_FV_TRANSPORT_ASSEMBLE, "fv_transport_assemble", FvTransportAssemble),
+      FV(KH_FV_FIXED_ROWS, "fv_fixed_rows", FvFixedRows),
+      FV(KH_FV_FLOOR, "fv_floor", FvFloor),
+      FV(KH_FV_SCALAR_STEADY_MEASURE, "fv_scalar_steady_measure", FvScalarSteadyMeasure),
+      FV(KH_FV_SCALAR_SET_FIXED, "fv_scalar_set_fixed", FvScalarSetFixed),
+      FV(KH_FV_SCALAR_MONITOR, "fv_scalar_monitor", FvScalarMonitor),
+      FV(KH_FV_COMPENSATED_PRESSURE_JUMP, "fv_compensated_pressure_jump", FvCompensatedPressureJump),
+      FV(KH_FV_PAIRED_PRESSURE_JUMP, "fv_paired_pressure_jump", FvPairedPressureJump),
+      FV(KH_FV_COMPENSATED_RHIE_CHOW, "fv_compensated_rhie_chow", FvCompensatedRhieChow),
+      FV(KH_FV_PAIRED_PRESSURE_BOUNDARY_LOW, "fv_paired_pressure_boundary_low", FvPairedPressureBoundaryLow),
+      FV(KH_FV_PAIRED_PRESSURE_FACES, "fv_paired_pressure_faces", FvPairedPressureFaces),
+      FV(KH_FV_PAIRED_MOMENTUM_RESIDUAL, "fv_paired_momentum_residual", FvPairedMomentumResidual),
+      FV(KH_FV_COMPENSATED_MOMENTUM_RESIDUAL, "fv_compensated_momentum_residual", FvCompensatedMomentumResidual),
+      FV(KH_FV_PAIRED_ACCUMULATE, "fv_paired_accumulate", FvPairedAccumulate),
+      FV(KH_FV_COMPENSATED_PRESSURE_FORCE, "fv_compensated_pressure_force", FvCompensatedPressureForce),
+      FV(KH_FV_PAIRED_VELOCITY_BOUNDARY, "fv_paired_velocity_boundary", FvPairedVelocityBoundary),
+      FV(KH_FV_PAIRED_ADVECTION, "fv_paired_advection", FvPairedAdvection),
+      FV(KH_FV_PAIRED_FACE_RESIDUAL, "fv_paired_face_residual", FvPairedFaceResidual),
+      FV(KH_FV_PAIRED_FLUX, "fv_paired_flux", FvPairedFlux),
+      FV(KH_FV_PAIRED_FLUX_OFFSET, "fv_paired_flux_offset", FvPairedFluxOffset),
+      FV(KH_FV_PAIRED_CONTINUITY, "fv_paired_continuity", FvPairedContinuity),
+      FV(KH_FV_PAIRED_PRESSURE_RHS, "fv_paired_pressure_rhs", FvPairedPressureRhs),
+      FV(KH_FV_PAIRED_CORRECT_FLUX, "fv_paired_correct_flux", FvPairedCorrectFlux),
+      FV(KH_FV_PAIRED_CORRECT_VELOCITY, "fv_paired_correct_velocity", FvPairedCorrectVelocity),
+      FV(KH_FV_PAIRED_VALUE, "fv_paired_value", FvPairedValue),
+      FV(KH_FV_PAIRED_VECTOR_VALUES, "fv_paired_vector_values", FvPairedVectorValues),
+      FV(KH_FV_COMPENSATED_CORRECT_FLUX, "fv_compensated_correct_flux", FvCompensatedCorrectFlux),
+      FV(KH_FV_COMPENSATED_CORRECT_VELOCITY, "fv_compensated_correct_velocity", FvCompensatedCorrectVelocity),
+      FV(KH_FV_BACKFLOW_PRESSURE, "fv_backflow_pressure", FvBackflowPressure),
+      FV(KH_FV_EFFECTIVE_VISCOSITY, "fv_effective_viscosity", FvEffectiveViscosity),
+      FV(KH_FV_TRANSPOSE_STRESS, "fv_transpose_stress", FvTransposeStress),
+      FV(KH_FV_STRESS_SOURCE, "fv_stress_source", FvStressSource),
+      FV(KH_FV_BOUNDARY_MOMENTUM, "fv_boundary_momentum", FvBoundaryMomentum),
+      FV(KH_FV_MOMENTUM_DIAGONAL, "fv_momentum_diagonal", FvMomentumDiagonal),
+      FV(KH_FV_SST_FORCE, "fv_sst_force", FvSstForce),
+      FV(KH_FV_TURBULENCE_BOUNDARY, "fv_turbulence_boundary", FvTurbulenceBoundary),
 #undef FV
   };
   return table;

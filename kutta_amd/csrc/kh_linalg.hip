@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kBlock) k_krylov_diagonal(int n, const int* __
     if (j != di) off += fabsf(v);
   }
   float inv = d != 0.0f ? 1.0f / d : 0.0f;
-  float l1 = fmaxf(fabsf(d), 0.5f * (fabsf(d) + off));
+  float l1 = kmax(fabsf(d), 0.5f * (fabsf(d) + off));
   inverse[i] = use_l1 != 0 ? (l1 != 0.0f ? 1.0f / l1 : 0.0f) : inv;
   invalid[i] = (bad != 0.0f || inv - inv != 0.0f || l1 - l1 != 0.0f)
                    ? 2.0f
@@ -558,22 +558,48 @@ __global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
 // Below a few thousand rows a level's launches are pure latency: each sweep
 // is a handful of dependent memory round trips plus a kernel boundary. The
 // tail runs every remaining level of the V-cycle (linalg/amg.cpp:283-310) in
-// one workgroup: each level's iterate and right-hand side live in LDS, the
-// operators stay in L2, and sweeps are separated by workgroup barriers. Every
-// row is computed by the same expression as the per-level kernels (l1-Jacobi,
-// kernels/linalg.kern.h:150-170; restriction, amg.kern.h:5-16; coarse solve,
-// linalg.kern.h:126-147), and Jacobi rows only read the previous sweep, so
-// the iterates are those of the launch sequence it replaces.
+// one workgroup with sweeps separated by workgroup barriers. Each level's
+// iterate and right-hand side live in LDS for the whole cycle; on each visit
+// to a level its operator (row starts, columns, values, l1 inverse and
+// aggregate map) is copied into the rest of LDS when it fits, so its sweeps
+// touch no global memory. Every row is computed by the same expression as the
+// per-level kernels (l1-Jacobi, kernels/linalg.kern.h:150-170; restriction,
+// amg.kern.h:5-16; coarse solve, linalg.kern.h:126-147), and Jacobi rows only
+// read the previous sweep, so the iterates are those of the launch sequence
+// it replaces.
 constexpr int kTailThreads = 1024;
 constexpr int kTailRowsPerThread = KH_TAIL_MAX_ROWS / kTailThreads;
 static_assert(KH_TAIL_MAX_ROWS % kTailThreads == 0, "tail rows per thread");
+// LDS pool in 32-bit words (gfx950 allows 160 KiB per workgroup).
+constexpr int kTailPoolWords = KH_TAIL_POOL_WORDS;
+static_assert(KH_TAIL_LDS_FLOATS <= kTailPoolWords, "tail vector budget");
+
+// Words an operator copy of a level takes: row starts, columns, values, the
+// l1 inverse and the aggregate map.
+__host__ __device__ constexpr int tail_stage_words(int rows, int nnz) { return 3 * rows + 1 + 2 * nnz; }
+
+// Reads through pointers known to address global memory.
+template <class T>
+__device__ __forceinline__ const T* global_ptr(const T* p) {
+  return (const T*)((const __attribute__((address_space(1))) T*)p);
+}
+
+// A level's operator as the sweeps read it: in LDS or in global memory.
+struct TailOperator {
+  const int* row_ptr;
+  const int* col;
+  const float* val;
+  const float* inverse;
+  const int* aggregate;
+  int rows;
+};
 
 enum class TailSweep { Plain, FromZero, Prolonged };
 
 // One in-place sweep over a level: new values wait in registers until every
 // row has read the previous iterate.
 template <TailSweep S>
-__device__ __forceinline__ void tail_sweep(const kh_tail_level& L, float* x, const float* b,
+__device__ __forceinline__ void tail_sweep(const TailOperator& L, float* x, const float* b,
                                            const float* coarse) {
   float next[kTailRowsPerThread];
 #pragma unroll
@@ -612,12 +638,14 @@ __device__ __forceinline__ void tail_sweep(const kh_tail_level& L, float* x, con
 }
 
 // Restricted residual of level L into the next level's right-hand side.
-__device__ __forceinline__ void tail_restrict(const kh_tail_level& L, const float* x,
-                                              const float* b, float* coarse_b) {
-  for (int g = threadIdx.x; g < L.coarse_rows; g += kTailThreads) {
+__device__ __forceinline__ void tail_restrict(const TailOperator& L, const kh_tail_level& T,
+                                              const float* x, const float* b, float* coarse_b) {
+  const int* offsets = global_ptr(T.member_offsets);
+  const int* members = global_ptr(T.members);
+  for (int g = threadIdx.x; g < T.coarse_rows; g += kTailThreads) {
     Kahan k;
-    for (int m = L.member_offsets[g]; m < L.member_offsets[g + 1]; ++m) {
-      const int row = L.members[m];
+    for (int m = offsets[g]; m < offsets[g + 1]; ++m) {
+      const int row = members[m];
       k.add(row_residual_chunked<kChunk>(b[row], L.row_ptr[row], L.row_ptr[row + 1], L.col, L.val,
                                          Plain{x}));
     }
@@ -645,15 +673,51 @@ __device__ __forceinline__ void tail_project(float* x, int rows, double* red, fl
   __syncthreads();
 }
 
+// Copies words [0, count) of src into LDS at dst, all lanes, then waits.
+__device__ __forceinline__ void tail_copy(uint32_t* dst, const uint32_t* src, int count) {
+  const uint32_t* from = global_ptr(src);
+  for (int k = threadIdx.x; k < count; k += kTailThreads) dst[k] = from[k];
+}
+
+// Runs `work` on level T's operator: staged into LDS at `stage` when
+// `staged`, else read in place. `staged` is uniform across the workgroup.
+template <class Work>
+__device__ __forceinline__ void tail_visit(const kh_tail_level& T, bool has_aggregate,
+                                           bool staged, uint32_t* stage, Work work) {
+  if (staged) {
+    const int rows = T.rows, nnz = T.nnz;
+    uint32_t* row_ptr = stage;
+    uint32_t* col = row_ptr + rows + 1;
+    uint32_t* val = col + nnz;
+    uint32_t* inverse = val + nnz;
+    uint32_t* aggregate = inverse + rows;
+    tail_copy(row_ptr, reinterpret_cast<const uint32_t*>(T.row_ptr), rows + 1);
+    tail_copy(col, reinterpret_cast<const uint32_t*>(T.col), nnz);
+    tail_copy(val, reinterpret_cast<const uint32_t*>(T.val), nnz);
+    tail_copy(inverse, reinterpret_cast<const uint32_t*>(T.inverse), rows);
+    if (has_aggregate) tail_copy(aggregate, reinterpret_cast<const uint32_t*>(T.aggregate), rows);
+    __syncthreads();
+    work(TailOperator{reinterpret_cast<const int*>(row_ptr), reinterpret_cast<const int*>(col),
+                      reinterpret_cast<const float*>(val), reinterpret_cast<const float*>(inverse),
+                      reinterpret_cast<const int*>(aggregate), rows});
+    // The next visit overwrites the staged copy.
+    __syncthreads();
+  } else {
+    work(TailOperator{global_ptr(T.row_ptr), global_ptr(T.col), global_ptr(T.val),
+                      global_ptr(T.inverse), has_aggregate ? global_ptr(T.aggregate) : nullptr,
+                      T.rows});
+  }
+}
+
 __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
     const kh_tail_level* __restrict__ levels, int count, int pre, int post, int coarse_sweeps,
     int nullspace, const float* __restrict__ rhs, float* __restrict__ out,
     const int* __restrict__ status) {
   if (status[0] != 0) return;
-  __shared__ float lds[KH_TAIL_LDS_FLOATS];
+  __shared__ uint32_t pool[kTailPoolWords];
   __shared__ double red[kTailThreads / 64];
   __shared__ float pair[2];
-  // Level k keeps its iterate at lds[base[k]] and right-hand side right after.
+  // Level k keeps its iterate at pool[base[k]] and right-hand side right after.
   __shared__ int base[KH_TAIL_MAX_LEVELS];
   // The table is validated by every thread alike, so an invalid one returns
   // uniformly before any barrier.
@@ -661,56 +725,67 @@ __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
   int used = 0;
   for (int k = 0; k < count; ++k) {
     const int rows = levels[k].rows;
-    if (rows < 1 || rows > KH_TAIL_MAX_ROWS) return;
+    if (rows < 1 || rows > KH_TAIL_MAX_ROWS || levels[k].nnz < 0) return;
     if (k + 1 < count && levels[k].coarse_rows != levels[k + 1].rows) return;
     if (threadIdx.x == 0) base[k] = used;
     used += 2 * rows;
     if (used > KH_TAIL_LDS_FLOATS) return;
   }
+  uint32_t* stage = pool + used;
+  const int room = kTailPoolWords - used;
   __syncthreads();
+  float* vectors = reinterpret_cast<float*>(pool);
   {
     const int rows = levels[0].rows;
-    for (int i = threadIdx.x; i < rows; i += kTailThreads) lds[base[0] + rows + i] = rhs[i];
+    for (int i = threadIdx.x; i < rows; i += kTailThreads) vectors[base[0] + rows + i] = rhs[i];
   }
   __syncthreads();
+  auto fits = [&](const kh_tail_level& T) { return tail_stage_words(T.rows, T.nnz) <= room; };
   for (int k = 0; k + 1 < count; ++k) {
-    const kh_tail_level L = levels[k];
-    float* x = lds + base[k];
-    const float* b = x + L.rows;
-    for (int t = 1; t <= pre; ++t) {
-      if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
-      else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-    }
-    float* coarse_b = lds + base[k + 1] + L.coarse_rows;
-    tail_restrict(L, x, b, coarse_b);
-    if (nullspace) tail_project(coarse_b, L.coarse_rows, red, pair);
-  }
-  {
-    const kh_tail_level L = levels[count - 1];
-    float* x = lds + base[count - 1];
-    const float* b = x + L.rows;
-    if (L.rows <= kCoarseMax) {
-      // amg_coarse_solve: sweeps from the zero iterate, general residual form.
-      for (int i = threadIdx.x; i < L.rows; i += kTailThreads) x[i] = 0.0f;
-      __syncthreads();
-      for (int t = 0; t < coarse_sweeps; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-    } else {
-      for (int t = 1; t <= coarse_sweeps; ++t) {
+    const kh_tail_level T = levels[k];
+    float* x = vectors + base[k];
+    const float* b = x + T.rows;
+    float* coarse_b = vectors + base[k + 1] + T.coarse_rows;
+    tail_visit(T, true, fits(T), stage, [&](const TailOperator& L) {
+      for (int t = 1; t <= pre; ++t) {
         if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
         else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
       }
-    }
-    if (nullspace) tail_project(x, L.rows, red, pair);
+      tail_restrict(L, T, x, b, coarse_b);
+    });
+    if (nullspace) tail_project(coarse_b, T.coarse_rows, red, pair);
+  }
+  {
+    const kh_tail_level T = levels[count - 1];
+    float* x = vectors + base[count - 1];
+    const float* b = x + T.rows;
+    tail_visit(T, false, fits(T), stage, [&](const TailOperator& L) {
+      if (T.rows <= kCoarseMax) {
+        // amg_coarse_solve: sweeps from the zero iterate, general residual form.
+        for (int i = threadIdx.x; i < T.rows; i += kTailThreads) x[i] = 0.0f;
+        __syncthreads();
+        for (int t = 0; t < coarse_sweeps; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+      } else {
+        for (int t = 1; t <= coarse_sweeps; ++t) {
+          if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
+          else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+        }
+      }
+    });
+    if (nullspace) tail_project(x, T.rows, red, pair);
   }
   for (int k = count - 2; k >= 0; --k) {
-    const kh_tail_level L = levels[k];
-    float* x = lds + base[k];
-    const float* b = x + L.rows;
-    tail_sweep<TailSweep::Prolonged>(L, x, b, lds + base[k + 1]);
-    for (int t = 2; t <= post; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-    if (nullspace) tail_project(x, L.rows, red, pair);
+    const kh_tail_level T = levels[k];
+    float* x = vectors + base[k];
+    const float* b = x + T.rows;
+    const float* coarse = vectors + base[k + 1];
+    tail_visit(T, true, fits(T), stage, [&](const TailOperator& L) {
+      tail_sweep<TailSweep::Prolonged>(L, x, b, coarse);
+      for (int t = 2; t <= post; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
+    });
+    if (nullspace) tail_project(x, T.rows, red, pair);
   }
-  for (int i = threadIdx.x; i < levels[0].rows; i += kTailThreads) out[i] = lds[base[0] + i];
+  for (int i = threadIdx.x; i < levels[0].rows; i += kTailThreads) out[i] = vectors[base[0] + i];
 }
 
 // ----------------------------------------------------------------- launchers

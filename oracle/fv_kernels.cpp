@@ -12,6 +12,8 @@
 #include <tuple>
 #include <utility>
 
+#include "oracle.hpp"
+
 namespace oracle::fv {
 namespace {
 
@@ -641,6 +643,740 @@ void fv_lsq_restore_inverse(int i, I cells, F values, W inverse, int count, int 
   for (int k = 0; k < 6; ++k) inverse[cells[i] + k * n] = values[i + k * count];
 }
 
+// ----------------------------------------------------- fv_transport.kern.h
+// :3-10
+void fv_diffusion_correction_bound(int i, I owner, I neighbour, I kind, F field, F boundary,
+                                   F diffusion, W bound, int internal) {
+  const float own = field[owner[i]];
+  float other = own;
+  if (i < internal) other = field[neighbour[i]];
+  else if (kind[i - internal] == 0) other = boundary[i - internal];
+  bound[i] = diffusion[i] * kabs(other - own);
+}
+
+// :12-35
+void fv_transport_faces(int i, I owner, I neighbour, F weight, F px, F py, F pz, F nx, F ny, F nz,
+                        F tx, F ty, F tz, F gx, F gy, F gz, F limiter, F flux, F mu, I kind,
+                        W correction, int internal, int second_order, int limited_diffusion) {
+  const int c = owner[i];
+  float ax = gx[c], ay = gy[c], az = gz[c];
+  float convection = 0.0f;
+  const float m = flux[i];
+  if (i < internal) {
+    const int n = neighbour[i];
+    const float w = weight[i];
+    ax = w * ax + (1.0f - w) * gx[n];
+    ay = w * ay + (1.0f - w) * gy[n];
+    az = w * az + (1.0f - w) * gz[n];
+    const bool forward = m >= 0.0f;
+    const int up = forward ? c : n;
+    const float rx = forward ? px[i] : nx[i], ry = forward ? py[i] : ny[i], rz = forward ? pz[i] : nz[i];
+    convection = limiter[up] * (gx[up] * rx + gy[up] * ry + gz[up] * rz) * m;
+  } else {
+    const int k = kind[i - internal];
+    if (k != 0) ax = ay = az = 0.0f;
+    if (m > 0.0f && k != 2) convection = limiter[c] * (gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i]) * m;
+  }
+  float diffusion = ax * tx[i] + ay * ty[i] + az * tz[i];
+  if (limited_diffusion) diffusion = kmax(-correction[i], kmin(correction[i], diffusion));
+  correction[i] = mu[i] * diffusion - (second_order != 0 ? convection : 0.0f);
+}
+
+// :37-112. Row i of a transport matrix: off-diagonals -D + min(F, 0) into the
+// row's own entries, an upward-rounded diagonal, boundary and deferred
+// correction terms on the right-hand side (optionally carried as a float
+// pair), the relaxed/pseudo-time diagonal, and the SIMPLEC consistent inverse.
+void fv_transport_assemble(int i, I offsets, I faces, F sign, I adjacent, I entry, I rows, I diag,
+                           F diffusion, F flux, F mu, I kind, F boundary, F correction, F source,
+                           F volume, F old, F sink, F extra_diagonal, F extra_rhs, W values,
+                           W rhs, W rhs_low, W diagonal, W diagonal_low, W consistent_inverse,
+                           int internal, float rho, float dt, float relax, int compensated) {
+  const float sink_volume = sink[i] * volume[i];
+  UpperSum ap = kupper_add({sink_volume, 0.0f}, extra_diagonal[i]);
+  float b = source[i] * volume[i] + extra_rhs[i];
+  float carry = 0.0f;
+  if (compensated) {
+    const float product = source[i] * volume[i];
+    const Pair2 start = two_sum(product, extra_rhs[i]);
+    b = start.hi;
+    carry = start.lo + std::fma(source[i], volume[i], -product);
+  }
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j)
+    if (adjacent[j] >= 0) values[entry[j]] = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j], o = adjacent[j];
+    const float mass = sign[j] * flux[f];
+    float d = mu[f] * diffusion[f];
+    ap = kupper_add(ap, mass);
+    if (o >= 0) {
+      values[entry[j]] += -d + kmin(mass, 0.0f);
+    } else {
+      const int bf = f - internal;
+      if (kind[bf] != 0) d = 0.0f;
+      ap = kupper_add(ap, d);
+      ap = kupper_add(ap, -kmin(mass, 0.0f));
+      if (compensated) {
+        const Pair2 factor = two_sum(d, -kmin(mass, 0.0f));
+        const float product = factor.hi * boundary[bf];
+        const Pair2 sum = two_sum(b, product);
+        carry += sum.lo + std::fma(factor.hi, boundary[bf], -product) + factor.lo * boundary[bf];
+        b = sum.hi;
+      } else {
+        b += (d - kmin(mass, 0.0f)) * boundary[bf];
+      }
+    }
+    const float deferred = sign[j] * correction[f];
+    if (compensated) {
+      const Pair2 sum = two_sum(b, deferred);
+      carry += sum.lo;
+      b = sum.hi;
+    } else {
+      b += deferred;
+    }
+  }
+  for (int j = rows[i]; j < rows[i + 1]; ++j)
+    if (j != diag[i]) ap = kupper_add(ap, -values[j]);
+  const float physical = kupper_value(ap);
+  diagonal[i] = physical;
+  if (compensated) {
+    const Pair2 gap = two_sum(ap.sum, -physical);
+    diagonal_low[i] = gap.hi + (gap.lo + ap.error + std::fma(sink[i], volume[i], -sink_volume));
+  }
+  const float pseudo_time = dt > 0.0f ? rho * volume[i] / dt : 0.0f;
+  const float relaxed = kadd_up(physical, pseudo_time) / relax;
+  if (compensated) {
+    const Pair2 factor = two_sum(relaxed, -physical);
+    const float product = factor.hi * old[i];
+    const Pair2 sum = two_sum(b, product);
+    carry += sum.lo + std::fma(factor.hi, old[i], -product) + factor.lo * old[i];
+    b = sum.hi;
+  } else {
+    b += (relaxed - physical) * old[i];
+  }
+  values[diag[i]] = relaxed;
+  rhs[i] = b;
+  if (compensated) rhs_low[i] = carry;
+  UpperSum denominator{relaxed, 0.0f};
+  for (int j = rows[i]; j < rows[i + 1]; ++j)
+    if (j != diag[i]) denominator = kupper_add(denominator, values[j]);
+  const float consistent = kupper_value(denominator);
+  consistent_inverse[i] = 1.0f / kmax(consistent > 0.0f ? consistent : relaxed, 1.0e-20f);
+}
+
+// :115-124
+void fv_fixed_rows(int i, I rows, I diag, I mask, F fixed, W values, W rhs) {
+  if (mask[i] == 0) return;
+  const float scale = values[diag[i]];
+  for (int j = rows[i]; j < rows[i + 1]; ++j) values[j] = 0.0f;
+  values[diag[i]] = scale;
+  rhs[i] = scale * fixed[i];
+}
+
+// :126-128
+void fv_floor(int i, W field, float floor) { field[i] = kmax(field[i], floor); }
+
+// :130-135
+void fv_scalar_steady_measure(int i, F field, F diagonal, F residual, I excluded, W absolute,
+                              W scale, int has_excluded) {
+  const bool omit = has_excluded && excluded[i] != 0;
+  absolute[i] = omit ? 0.0f : kabs(residual[i]);
+  scale[i] = omit ? 0.0f : kabs(diagonal[i] * field[i]);
+}
+
+// :137-139
+void fv_scalar_set_fixed(int i, I fixed, F value, W field) {
+  if (fixed[i]) field[i] = value[i];
+}
+
+// :141-148
+void fv_scalar_monitor(int i, F field, F diagonal, F residual, I excluded, F volume, W absolute,
+                       W scale, W fluent_scale, float rho, float dt) {
+  const bool omit = excluded[i] != 0;
+  absolute[i] = omit ? 0.0f : kabs(residual[i]);
+  scale[i] = omit ? 0.0f : kabs(diagonal[i] * field[i]);
+  const float pseudo_time = dt > 0.0f ? rho * volume[i] / dt : 0.0f;
+  fluent_scale[i] = omit ? 0.0f : kabs((diagonal[i] + pseudo_time) * field[i]);
+}
+
+// -------------------------------------------------------- fv_paired.kern.h
+// Pair additions with hysteresis, kernels/prelude.h:133-171.
+inline Pair2 pair_add(float hi, float lo, float d, float slack) {
+  const float low = lo + d;
+  if (kabs(low) <= slack) return {hi, low};
+  const Pair2 s = two_sum(hi, d);
+  return two_sum(s.hi, s.lo + lo);
+}
+// Renormalised hi + lo + d, and its offset from the held high word.
+inline Pair2 pair_sum3(float hi, float lo, float d, float& offset) {
+  const Pair2 base = two_sum(hi, lo), increment = two_sum(base.lo, d);
+  Pair2 value = two_sum(base.hi, increment.hi);
+  value = two_sum(value.hi, value.lo + increment.lo);
+  const Pair2 difference = two_sum(value.hi, -hi);
+  offset = difference.hi + (difference.lo + value.lo);
+  return value;
+}
+inline Pair2 pair_add_relative(float hi, float lo, float d, float relative_slack) {
+  const Pair2 base = two_sum(hi, lo), increment = two_sum(base.lo, d);
+  Pair2 value = two_sum(base.hi, increment.hi);
+  value = two_sum(value.hi, value.lo + increment.lo);
+  const float slack = relative_slack * kmax(kabs(base.hi), kabs(value.hi));
+  const Pair2 difference = two_sum(value.hi, -hi);
+  const float offset = difference.hi + (difference.lo + value.lo);
+  if (kabs(offset) <= slack) return {hi, offset};
+  return value;
+}
+inline bool same_sign(float a, float b) {
+  return (a > 0.0f && b > 0.0f) || (a < 0.0f && b < 0.0f) || (a == 0.0f && b == 0.0f);
+}
+inline Pair2 pair_add_flux(float hi, float lo, float d, float slack) {
+  float offset;
+  const Pair2 value = pair_sum3(hi, lo, d, offset);
+  if (same_sign(hi, value.hi) && kabs(offset) <= slack) return {hi, offset};
+  return value;
+}
+// Centroid-link jump fade factor, as in fv_pressure_faces.
+inline float link_trust(float dx, float dy, float dz, float mx, float my, float mz) {
+  const float d2 = dx * dx + dy * dy + dz * dz, m2 = mx * mx + my * my + mz * mz;
+  return kmax(0.0f, kmin(1.0f, (4.0f * d2 - m2) / (3.0f * d2)));
+}
+
+// :5-33
+void fv_compensated_pressure_jump(int i, I owner, I neighbour, F px, F py, F pz, F nx, F ny, F nz,
+                                  F p, F gx, F gy, F gz, F boundary, W jump, int internal) {
+  const int c = owner[i];
+  Pair2 full = two_sum(p[c], i < internal ? -p[neighbour[i]] : -boundary[i - internal]);
+  full = product_add(full, gx[c], px[i]);
+  full = product_add(full, gy[c], py[i]);
+  full = product_add(full, gz[c], pz[i]);
+  if (i >= internal) {
+    jump[i] = full.hi + full.lo;
+    return;
+  }
+  const int n = neighbour[i];
+  full = product_add(full, -gx[n], nx[i]);
+  full = product_add(full, -gy[n], ny[i]);
+  full = product_add(full, -gz[n], nz[i]);
+  const float side = full.hi + full.lo;
+  const float trust = link_trust(px[i] - nx[i], py[i] - ny[i], pz[i] - nz[i], 0.5f * (px[i] + nx[i]),
+                                 0.5f * (py[i] + ny[i]), 0.5f * (pz[i] + nz[i]));
+  if (trust == 1.0f) {
+    jump[i] = side;
+    return;
+  }
+  Pair2 link = two_sum(p[c], -p[n]);
+  const float r[3][2] = {{px[i], nx[i]}, {py[i], ny[i]}, {pz[i], nz[i]}};
+  F g[3] = {gx, gy, gz};
+  for (int k = 0; k < 3; ++k) {
+    link = product_add(link, g[k][c], 0.5f * r[k][0]);
+    link = product_add(link, g[k][n], 0.5f * r[k][0]);
+    link = product_add(link, g[k][c], -0.5f * r[k][1]);
+    link = product_add(link, g[k][n], -0.5f * r[k][1]);
+  }
+  const float shared = link.hi + link.lo;
+  jump[i] = shared + trust * (side - shared);
+}
+
+// :39-44
+inline float trace_difference(Pair2 a, Pair2 b) {
+  Pair2 d = two_sum(a.hi, -b.hi);
+  d = product_add(d, a.lo, 1.0f);
+  d = product_add(d, -b.lo, 1.0f);
+  return d.hi + d.lo;
+}
+// :46-65
+void fv_paired_pressure_jump(int i, I owner, I neighbour, F px, F py, F pz, F nx, F ny, F nz, F p,
+                             F low, F gx, F gy, F gz, F lx, F ly, F lz, F boundary,
+                             F boundary_low, W jump, int internal) {
+  const int c = owner[i];
+  const Pair2 own = trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[i], py[i], pz[i]);
+  if (i >= internal) {
+    jump[i] = trace_difference(own, {boundary[i - internal], boundary_low[i - internal]});
+    return;
+  }
+  const int n = neighbour[i];
+  const Pair2 next = trace(p[n], low[n], gx[n], gy[n], gz[n], lx[n], ly[n], lz[n], nx[i], ny[i], nz[i]);
+  const float side = trace_difference(own, next);
+  const float dx = px[i] - nx[i], dy = py[i] - ny[i], dz = pz[i] - nz[i];
+  const float trust = link_trust(dx, dy, dz, 0.5f * (px[i] + nx[i]), 0.5f * (py[i] + ny[i]),
+                                 0.5f * (pz[i] + nz[i]));
+  if (trust == 1.0f) {
+    jump[i] = side;
+    return;
+  }
+  const Pair2 a = trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], .5f * dx, .5f * dy, .5f * dz);
+  const Pair2 b = trace(p[n], low[n], gx[n], gy[n], gz[n], lx[n], ly[n], lz[n], -.5f * dx, -.5f * dy, -.5f * dz);
+  const float shared = trace_difference(a, b);
+  jump[i] = shared + trust * (side - shared);
+}
+
+// :69-85
+void fv_compensated_rhie_chow(int i, I owner, I neighbour, I kind, F sx, F sy, F sz, F diffusion,
+                              F volume, F diagonal, F u, F v, F w, F lu, F lv, F lw, F jump,
+                              W flux, W low, int internal, float rho) {
+  const bool boundary = i >= internal;
+  if (boundary && (kind[i - internal] == 0 || kind[i - internal] == 3)) {
+    flux[i] = 0.0f;
+    low[i] = 0.0f;
+    return;
+  }
+  const int c = owner[i];
+  float rc = volume[c] / kmax(diagonal[c], 1.0e-20f);
+  if (!boundary) {
+    const int n = neighbour[i];
+    rc = .5f * (rc + volume[n] / kmax(diagonal[n], 1.0e-20f));
+  }
+  Pair2 value{0.0f, 0.0f};
+  value = product_add(value, u[i], sx[i]);
+  value = product_add(value, lu[i], sx[i]);
+  value = product_add(value, v[i], sy[i]);
+  value = product_add(value, lv[i], sy[i]);
+  value = product_add(value, w[i], sz[i]);
+  value = product_add(value, lw[i], sz[i]);
+  if (!boundary || kind[i - internal] != 1) value = product_add(value, rc * diffusion[i], jump[i]);
+  const Pair2 scaled = add_weighted({0.0f, 0.0f}, rho, value);
+  flux[i] = scaled.hi;
+  low[i] = scaled.lo;
+}
+
+// :89-97
+void fv_paired_pressure_boundary_low(int i, I owner, I kind, F px, F py, F pz, F p, F low, F gx,
+                                     F gy, F gz, F lx, F ly, F lz, F boundary, W boundary_low,
+                                     int internal) {
+  if (kind[i] == 2 || kind[i] == 4) {
+    boundary_low[i] = 0.0f;
+    return;
+  }
+  const int f = i + internal, c = owner[f];
+  Pair2 value = trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[f], py[f], pz[f]);
+  value = product_add(value, -boundary[i], 1.0f);
+  boundary_low[i] = value.hi + value.lo;
+}
+
+// :98-128
+void fv_paired_pressure_faces(int i, I owner, I neighbour, I kind, I poor, F px, F py, F pz, F nx,
+                              F ny, F nz, F p, F low, F gx, F gy, F gz, F lx, F ly, F lz,
+                              F boundary, F boundary_low, F diagonal, W face, W face_low,
+                              int internal, int standard) {
+  const int c = owner[i];
+  Pair2 value = two_sum(p[c], low[c]);
+  if (i < internal) {
+    const int n = neighbour[i];
+    if (standard) {
+      const float wc = 1.0f / kmax(diagonal[c], 1.0e-20f), wn = 1.0f / kmax(diagonal[n], 1.0e-20f);
+      Pair2 numerator{0.0f, 0.0f};
+      numerator = product_add(numerator, wc, p[c]);
+      numerator = product_add(numerator, wc, low[c]);
+      numerator = product_add(numerator, wn, p[n]);
+      numerator = product_add(numerator, wn, low[n]);
+      const Pair2 denominator = two_sum(wc, wn);
+      const float q = numerator.hi / denominator.hi;
+      numerator = product_add(numerator, -q, denominator.hi);
+      numerator = product_add(numerator, -q, denominator.lo);
+      value = two_sum(q, (numerator.hi + numerator.lo) / denominator.hi);
+    } else {
+      const bool plain = poor[c] || poor[n];
+      const Pair2 own = plain ? two_sum(p[c], low[c])
+                              : trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[i], py[i], pz[i]);
+      const Pair2 next = plain ? two_sum(p[n], low[n])
+                               : trace(p[n], low[n], gx[n], gy[n], gz[n], lx[n], ly[n], lz[n], nx[i], ny[i], nz[i]);
+      value = two_sum(0.5f * own.hi, 0.5f * next.hi);
+      value = two_sum(value.hi, value.lo + 0.5f * own.lo + 0.5f * next.lo);
+    }
+  } else {
+    const int b = i - internal;
+    if (kind[b] == 2 || kind[b] == 4) value = two_sum(boundary[b], boundary_low[b]);
+    else if (!standard && !poor[c])
+      value = trace(p[c], low[c], gx[c], gy[c], gz[c], lx[c], ly[c], lz[c], px[i], py[i], pz[i]);
+  }
+  face[i] = value.hi;
+  face_low[i] = value.lo;
+}
+
+// :140-152
+void fv_paired_momentum_residual(int i, I rows, I cols, F values, I diag, F physical,
+                                 F physical_low, F hi, F lo, F pressure_gradient_lo, F volume,
+                                 F rhs, F rhs_low, W r) {
+  float sum = rhs[i], error = rhs_low[i];
+  for (int j = rows[i]; j < rows[i + 1]; ++j) {
+    const float a = values[j], x = hi[cols[j]], product = a * x;
+    const Pair2 s = two_sum(sum, -product);
+    sum = s.hi;
+    error += s.lo - std::fma(a, x, -product) - a * lo[cols[j]];
+  }
+  error += (values[diag[i]] - physical[i]) * lo[i] - volume[i] * pressure_gradient_lo[i];
+  error -= physical_low[i] * hi[i] + physical_low[i] * lo[i];
+  r[i] = sum + error;
+}
+
+// :156-170
+void fv_compensated_momentum_residual(int i, I rows, I cols, F values, I diag, F physical,
+                                      F physical_low, F hi, F lo, F pressure_gradient_lo,
+                                      F volume, F rhs, F rhs_low, W r) {
+  Pair2 value = two_sum(rhs[i], rhs_low[i]);
+  for (int j = rows[i]; j < rows[i + 1]; ++j) {
+    value = product_add(value, -values[j], hi[cols[j]]);
+    const float low_coefficient = j == diag[i] ? physical[i] : values[j];
+    value = product_add(value, -low_coefficient, lo[cols[j]]);
+  }
+  value = product_add(value, -physical_low[i], hi[i]);
+  value = product_add(value, -physical_low[i], lo[i]);
+  value = product_add(value, -volume[i], pressure_gradient_lo[i]);
+  r[i] = value.hi + value.lo;
+}
+
+// :172-174
+void fv_paired_accumulate(int i, F delta, W hi, W lo, float slack) {
+  const Pair2 x = pair_add(hi[i], lo[i], delta[i], slack);
+  hi[i] = x.hi;
+  lo[i] = x.lo;
+}
+
+// :179-196
+void fv_compensated_pressure_force(int i, I offsets, I faces, F sign, F sx, F sy, F sz, F volume,
+                                   F pressure, F low, W x, W y, W z) {
+  float a = 0.0f, b = 0.0f, c = 0.0f;
+  Pair2 ax{0.0f, 0.0f}, ay{0.0f, 0.0f}, az{0.0f, 0.0f};
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    const float hp = sign[j] * pressure[f], lp = sign[j] * low[f];
+    a += hp * sx[f];
+    b += hp * sy[f];
+    c += hp * sz[f];
+    ax = product_add(product_add(ax, hp, sx[f]), lp, sx[f]);
+    ay = product_add(product_add(ay, hp, sy[f]), lp, sy[f]);
+    az = product_add(product_add(az, hp, sz[f]), lp, sz[f]);
+  }
+  const float vol = volume[i];
+  ax = product_add(ax, -a / vol, vol);
+  ay = product_add(ay, -b / vol, vol);
+  az = product_add(az, -c / vol, vol);
+  x[i] = (ax.hi + ax.lo) / vol;
+  y[i] = (ay.hi + ay.lo) / vol;
+  z[i] = (az.hi + az.lo) / vol;
+}
+
+// :200-218
+void fv_paired_velocity_boundary(int i, I owner, I kind, F sx, F sy, F sz, F u, F v, F w, F flux,
+                                 F flux_low, W bu, W bv, W bw, WI scalar_kind, int internal,
+                                 float rho) {
+  const int f = i + internal, c = owner[f], k = kind[i];
+  float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+  const float area2 = sx[f] * sx[f] + sy[f] * sy[f] + sz[f] * sz[f];
+  scalar_kind[i] = 0;
+  if (((k == 2 || k == 4) && flux[f] >= 0.0f) || k == 3) {
+    vx = u[c];
+    vy = v[c];
+    vz = w[c];
+    scalar_kind[i] = 2;
+  }
+  if (k == 4 && flux[f] < 0.0f) {
+    const float factor = flux_low[f] / (rho * area2);
+    vx = factor * sx[f];
+    vy = factor * sy[f];
+    vz = factor * sz[f];
+  }
+  if (k == 3) {
+    const float un = (vx * sx[f] + vy * sy[f] + vz * sz[f]) / area2;
+    vx -= un * sx[f];
+    vy -= un * sy[f];
+    vz -= un * sz[f];
+  }
+  bu[i] = vx;
+  bv[i] = vy;
+  bw[i] = vz;
+}
+
+// :220-236
+void fv_paired_advection(int i, I owner, I neighbour, I kind, F px, F py, F pz, F nx, F ny, F nz,
+                         F field, F boundary, F gx, F gy, F gz, F limiter, F flux, F flux_low,
+                         W correction, int internal, int second_order) {
+  const int c = owner[i];
+  int up = c;
+  float value = field[c], rx = px[i], ry = py[i], rz = pz[i];
+  bool reconstruct = second_order != 0;
+  if (i < internal) {
+    if (flux[i] < 0.0f) {
+      up = neighbour[i];
+      rx = nx[i];
+      ry = ny[i];
+      rz = nz[i];
+    }
+    value = field[up];
+  } else {
+    if (flux[i] < 0.0f) value = boundary[i - internal];
+    if (flux[i] <= 0.0f || kind[i - internal] == 2) reconstruct = false;
+  }
+  if (reconstruct) value += limiter[up] * (gx[up] * rx + gy[up] * ry + gz[up] * rz);
+  correction[i] = flux_low[i] * value;
+}
+
+// :238-247
+void fv_paired_face_residual(int i, I offsets, I faces, F sign, F secondary, F advection, F stress,
+                             F boundary_rhs, W residual, int turbulent) {
+  float sum = residual[i], error = boundary_rhs[i];
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    const float value = sign[j] * (secondary[f] - advection[f] + (turbulent ? stress[f] : 0.0f));
+    const Pair2 s = two_sum(sum, value);
+    sum = s.hi;
+    error += s.lo;
+  }
+  residual[i] = sum + error;
+}
+
+// :259-290
+void fv_paired_flux(int i, I owner, I neighbour, I kind, F sx, F sy, F sz, F px, F py, F pz, F nx,
+                    F ny, F nz, F weights, F diagonal, F volume, F diffusion, F u, F v, F w, F p,
+                    F gx, F gy, F gz, F previous, W flux, W flux_lo, int internal, float rho,
+                    float slack_per_area) {
+  const int c = owner[i];
+  float rc = volume[c] / kmax(diagonal[c], 1.0e-20f), low = 0.0f;
+  if (i < internal) {
+    const int n = neighbour[i];
+    const float a = weights[c] / (weights[c] + weights[n]);
+    const float ux = a * u[c] + (1.0f - a) * u[n];
+    const float uy = a * v[c] + (1.0f - a) * v[n];
+    const float uz = a * w[c] + (1.0f - a) * w[n];
+    rc = 0.5f * (rc + volume[n] / kmax(diagonal[n], 1.0e-20f));
+    const float dx = px[i] - nx[i], dy = py[i] - ny[i], dz = pz[i] - nz[i];
+    const float jump = p[c] - p[n] + 0.5f * ((gx[c] + gx[n]) * dx + (gy[c] + gy[n]) * dy + (gz[c] + gz[n]) * dz);
+    low = rho * (ux * sx[i] + uy * sy[i] + uz * sz[i]) + rho * rc * diffusion[i] * jump;
+  } else if (kind[i - internal] == 2 || kind[i - internal] == 4) {
+    const float jump = p[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i];
+    low = rho * (u[c] * sx[i] + v[c] * sy[i] + w[c] * sz[i]) + rho * rc * diffusion[i] * jump;
+  }
+  const float slack = slack_per_area * ksqrt(sx[i] * sx[i] + sy[i] * sy[i] + sz[i] * sz[i]);
+  const Pair2 x = two_sum(flux[i], -previous[i]);
+  const float offset = x.hi + (x.lo + low);
+  if (kabs(offset) <= slack) {
+    flux[i] = previous[i];
+    flux_lo[i] = offset;
+  } else {
+    flux_lo[i] = low;
+  }
+}
+
+// :292-303
+void fv_paired_flux_offset(int i, F sx, F sy, F sz, F previous, W flux, W low, float slack_per_area) {
+  const float slack = slack_per_area * ksqrt(sx[i] * sx[i] + sy[i] * sy[i] + sz[i] * sz[i]);
+  const Pair2 total = two_sum(flux[i], low[i]);
+  const Pair2 x = two_sum(total.hi, -previous[i]);
+  const float offset = x.hi + (x.lo + total.lo);
+  if (same_sign(previous[i], total.hi) && kabs(offset) <= slack) {
+    flux[i] = previous[i];
+    low[i] = offset;
+  } else {
+    flux[i] = total.hi;
+    low[i] = total.lo;
+  }
+}
+
+// :308-318
+void fv_paired_continuity(int i, I offsets, I faces, F sign, F flux, F flux_lo, W imbalance,
+                          W magnitude) {
+  float sum = 0.0f, error = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    const Pair2 s = two_sum(sum, sign[j] * flux[f]);
+    sum = s.hi;
+    error += s.lo + sign[j] * flux_lo[f];
+  }
+  const float mass = sum + error;
+  imbalance[i] = mass;
+  magnitude[i] = kabs(mass);
+}
+
+// :321-323
+void fv_paired_pressure_rhs(int i, F imbalance, W rhs, int reference) {
+  rhs[i] = i == reference ? 0.0f : -imbalance[i];
+}
+
+// :326-333
+void fv_paired_correct_flux(int i, I owner, I neighbour, F sx, F sy, F sz, F correction,
+                            F coefficient, W flux, W flux_lo, int internal, float slack_per_area) {
+  const float pc = correction[owner[i]];
+  const float pn = i < internal ? correction[neighbour[i]] : 0.0f;
+  const float slack = slack_per_area * ksqrt(sx[i] * sx[i] + sy[i] * sy[i] + sz[i] * sz[i]);
+  const Pair2 x = pair_add(flux[i], flux_lo[i], coefficient[i] * (pc - pn), slack);
+  flux[i] = x.hi;
+  flux_lo[i] = x.lo;
+}
+
+// :336-347
+void fv_paired_correct_velocity(int i, F volume, F inverse, F diagonal, F gx, F gy, F gz,
+                                F correction, W u, W v, W w, W p, W u_lo, W v_lo, W w_lo, W p_lo,
+                                float alpha, float velocity_slack, float pressure_slack) {
+  const float d = volume[i] * inverse[i];
+  Pair2 x = pair_add(u[i], u_lo[i], -d * gx[i], velocity_slack);
+  u[i] = x.hi;
+  u_lo[i] = x.lo;
+  x = pair_add(v[i], v_lo[i], -d * gy[i], velocity_slack);
+  v[i] = x.hi;
+  v_lo[i] = x.lo;
+  x = pair_add(w[i], w_lo[i], -d * gz[i], velocity_slack);
+  w[i] = x.hi;
+  w_lo[i] = x.lo;
+  const float pressure_alpha = kmin(alpha, 0.7f * diagonal[i] * inverse[i]);
+  x = pair_add(p[i], p_lo[i], pressure_alpha * correction[i], pressure_slack);
+  p[i] = x.hi;
+  p_lo[i] = x.lo;
+}
+
+// :349-351
+void fv_paired_value(int i, F hi, F lo, W value) { value[i] = hi[i] + lo[i]; }
+// :352-355
+void fv_paired_vector_values(int i, F x, F y, F z, F lx, F ly, F lz, W u, W v, W w) {
+  u[i] = x[i] + lx[i];
+  v[i] = y[i] + ly[i];
+  w[i] = z[i] + lz[i];
+}
+
+// :356-363
+void fv_compensated_correct_flux(int i, I owner, I neighbour, F sx, F sy, F sz, F correction,
+                                 F coefficient, W flux, W flux_lo, int internal,
+                                 float slack_per_area) {
+  const int c = owner[i];
+  const float pn = i < internal ? correction[neighbour[i]] : 0.0f;
+  const float slack = slack_per_area * ksqrt(sx[i] * sx[i] + sy[i] * sy[i] + sz[i] * sz[i]);
+  const Pair2 x = pair_add_flux(flux[i], flux_lo[i], coefficient[i] * (correction[c] - pn), slack);
+  flux[i] = x.hi;
+  flux_lo[i] = x.lo;
+}
+
+// :365-377
+void fv_compensated_correct_velocity(int i, F volume, F inverse, F diagonal, F gx, F gy, F gz,
+                                     F correction, W u, W v, W w, W p, W u_lo, W v_lo, W w_lo,
+                                     W p_lo, float alpha, float velocity_slack,
+                                     float pressure_slack) {
+  const float d = volume[i] * inverse[i];
+  Pair2 x = pair_add(u[i], u_lo[i], -d * gx[i], velocity_slack);
+  u[i] = x.hi;
+  u_lo[i] = x.lo;
+  x = pair_add(v[i], v_lo[i], -d * gy[i], velocity_slack);
+  v[i] = x.hi;
+  v_lo[i] = x.lo;
+  x = pair_add(w[i], w_lo[i], -d * gz[i], velocity_slack);
+  w[i] = x.hi;
+  w_lo[i] = x.lo;
+  const float pressure_alpha = kmin(alpha, 0.7f * diagonal[i] * inverse[i]);
+  x = pair_add_relative(p[i], p_lo[i], pressure_alpha * correction[i], pressure_slack);
+  p[i] = x.hi;
+  p_lo[i] = x.lo;
+}
+
+// ---------------------------------------------------------- fv_rans.kern.h
+// :4-10
+void fv_backflow_pressure(int i, I owner, I total, F mass, F u, F v, F w, W boundary, int internal,
+                          float rho) {
+  const int f = i + internal, c = owner[f];
+  if (total[i] != 0 && mass[f] < 0.0f) boundary[i] -= 0.5f * rho * (u[c] * u[c] + v[c] * v[c] + w[c] * w[c]);
+}
+
+// :12-17
+void fv_effective_viscosity(int i, I owner, I neighbour, I kind, F weight, F mut, W mu,
+                            int internal, float molecular) {
+  float turbulent = mut[owner[i]];
+  if (i < internal) turbulent = weight[i] * turbulent + (1.0f - weight[i]) * mut[neighbour[i]];
+  mu[i] = i >= internal && kind[i - internal] == 0 ? 0.0f : molecular + turbulent;
+}
+
+// :21-37
+void fv_transpose_stress(int i, I owner, I neighbour, I kind, F weight, F mu, F sx, F sy, F sz,
+                         F ux, F uy, F uz, F vx, F vy, F vz, F wx, F wy, F wz, W x, W y, W z,
+                         int internal) {
+  const int c = owner[i];
+  float g[9] = {ux[c], uy[c], uz[c], vx[c], vy[c], vz[c], wx[c], wy[c], wz[c]};
+  if (i < internal) {
+    const int n = neighbour[i];
+    const float t = weight[i], s = 1.0f - t;
+    const float h[9] = {ux[n], uy[n], uz[n], vx[n], vy[n], vz[n], wx[n], wy[n], wz[n]};
+    for (int k = 0; k < 9; ++k) g[k] = t * g[k] + s * h[k];
+  }
+  const float m = i >= internal && kind[i - internal] == 3 ? 0.0f : mu[i];
+  x[i] = m * (g[0] * sx[i] + g[3] * sy[i] + g[6] * sz[i]);
+  y[i] = m * (g[1] * sx[i] + g[4] * sy[i] + g[7] * sz[i]);
+  z[i] = m * (g[2] * sx[i] + g[5] * sy[i] + g[8] * sz[i]);
+}
+
+// :38-46
+void fv_stress_source(int i, I offsets, I faces, F sign, F volume, F x, F y, F z, W su, W sv, W sw) {
+  float a = 0.0f, b = 0.0f, c = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    a += sign[j] * x[f];
+    b += sign[j] * y[f];
+    c += sign[j] * z[f];
+  }
+  su[i] += a / volume[i];
+  sv[i] += b / volume[i];
+  sw[i] += c / volume[i];
+}
+
+// :47-72. Implicit symmetry and wall-function terms of one velocity component.
+void fv_boundary_momentum(int i, I offsets, I faces, I kind, F sx, F sy, F sz, F coefficient, F mu,
+                          F diffusion, F u, F v, F w, F bu, F bv, F bw, W diagonal, W rhs,
+                          int internal, int component, int wall_function) {
+  float ap = 0.0f, b = 0.0f;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int f = faces[j];
+    if (f < internal) continue;
+    const int bf = f - internal, k = kind[bf];
+    if (k != 3 && !(k == 0 && wall_function)) continue;
+    const float area = ksqrt(sx[f] * sx[f] + sy[f] * sy[f] + sz[f] * sz[f]);
+    const float ex = sx[f] / area, ey = sy[f] / area, ez = sz[f] / area;
+    const float nc = component == 0 ? ex : (component == 1 ? ey : ez);
+    const float uc = component == 0 ? u[i] : (component == 1 ? v[i] : w[i]);
+    const float wc = component == 0 ? bu[bf] : (component == 1 ? bv[bf] : bw[bf]);
+    if (k == 3) {
+      const float d = mu[f] * diffusion[f] * nc * nc;
+      ap += d;
+      b -= mu[f] * diffusion[f] * nc * (ex * u[i] + ey * v[i] + ez * w[i] - nc * uc);
+      continue;
+    }
+    const float c = area * coefficient[bf];
+    ap += c * (1.0f - nc * nc);
+    b += c * (wc - nc * (ex * bu[bf] + ey * bv[bf] + ez * bw[bf]) + nc * (ex * u[i] + ey * v[i] + ez * w[i] - nc * uc));
+  }
+  diagonal[i] = ap;
+  rhs[i] = b;
+}
+
+// :75-78
+void fv_momentum_diagonal(int i, F ax, F ay, F az, F ix, F iy, F iz, W physical, W inverse) {
+  physical[i] = (ax[i] + ay[i] + az[i]) / 3.0f;
+  inverse[i] = (ix[i] + iy[i] + iz[i]) / 3.0f;
+}
+
+// :79-83
+void fv_sst_force(int i, F selected, F sx, F sy, F sz, F tx, F ty, F tz, W x, W y, W z, int internal) {
+  const int f = i + internal;
+  const float a = -selected[i] * ksqrt(sx[f] * sx[f] + sy[f] * sy[f] + sz[f] * sz[f]);
+  x[i] = a * tx[i];
+  y[i] = a * ty[i];
+  z[i] = a * tz[i];
+}
+
+// :86-101. Turbulence boundary values: spec 0 direct k/omega, 1 intensity and
+// length scale, 2 intensity and viscosity ratio.
+void fv_turbulence_boundary(int i, I owner, I kind, I spec, F a, F b, F mass, F bu, F bv, F bw,
+                            F k, F omega, F wall_omega, W bk, W bo, WI transport_kind,
+                            int internal, float rho, float mu) {
+  const int f = i + internal, c = owner[f], type = kind[i];
+  float kinetic = k[c], dissipation = omega[c];
+  transport_kind[i] = 1;
+  if (type == 1 || ((type == 2 || type == 4) && mass[f] < 0.0f)) {
+    kinetic = a[i];
+    dissipation = b[i];
+    if (spec[i] != 0) {
+      kinetic = 1.5f * a[i] * a[i] * (bu[i] * bu[i] + bv[i] * bv[i] + bw[i] * bw[i]);
+      dissipation = spec[i] == 1 ? ksqrt(kinetic) / (0.5477225575f * b[i]) : rho * kinetic / (mu * b[i]);
+    }
+    transport_kind[i] = 0;
+  }
+  bk[i] = kinetic;
+  bo[i] = type == 0 ? wall_omega[i] : dissipation;
+}
+
 // ---------------------------------------------------------------- dispatch
 template <class T>
 T from_arg(void* const* a, std::size_t k) {
@@ -695,6 +1431,22 @@ const std::map<std::string, Entry>& table() {
       KERNEL(fv_paired_limiter_scale), KERNEL(fv_paired_gradient_limiter_scale),
       KERNEL(fv_jump_gradient), KERNEL(fv_face_jump), KERNEL(fv_gather_jump_gradient),
       KERNEL(fv_paired_jump_gradient), KERNEL(fv_lsq_restore_inverse),
+      KERNEL(fv_diffusion_correction_bound), KERNEL(fv_transport_faces),
+      KERNEL(fv_transport_assemble), KERNEL(fv_fixed_rows), KERNEL(fv_floor),
+      KERNEL(fv_scalar_steady_measure), KERNEL(fv_scalar_set_fixed), KERNEL(fv_scalar_monitor),
+      KERNEL(fv_compensated_pressure_jump), KERNEL(fv_paired_pressure_jump),
+      KERNEL(fv_compensated_rhie_chow), KERNEL(fv_paired_pressure_boundary_low),
+      KERNEL(fv_paired_pressure_faces), KERNEL(fv_paired_momentum_residual),
+      KERNEL(fv_compensated_momentum_residual), KERNEL(fv_paired_accumulate),
+      KERNEL(fv_compensated_pressure_force), KERNEL(fv_paired_velocity_boundary),
+      KERNEL(fv_paired_advection), KERNEL(fv_paired_face_residual), KERNEL(fv_paired_flux),
+      KERNEL(fv_paired_flux_offset), KERNEL(fv_paired_continuity), KERNEL(fv_paired_pressure_rhs),
+      KERNEL(fv_paired_correct_flux), KERNEL(fv_paired_correct_velocity), KERNEL(fv_paired_value),
+      KERNEL(fv_paired_vector_values), KERNEL(fv_compensated_correct_flux),
+      KERNEL(fv_compensated_correct_velocity),
+      KERNEL(fv_backflow_pressure), KERNEL(fv_effective_viscosity), KERNEL(fv_transpose_stress),
+      KERNEL(fv_stress_source), KERNEL(fv_boundary_momentum), KERNEL(fv_momentum_diagonal),
+      KERNEL(fv_sst_force), KERNEL(fv_turbulence_boundary),
   };
   return t;
 }

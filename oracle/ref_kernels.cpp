@@ -27,6 +27,9 @@ namespace refk {
 namespace reff {
 #include "kutta/kernels/fv_flow.kern.h"
 #include "kutta/kernels/fv_gradient.kern.h"
+#include "kutta/kernels/fv_transport.kern.h"
+#include "kutta/kernels/fv_paired.kern.h"
+#include "kutta/kernels/fv_rans.kern.h"
 }  // namespace reff
 
 #define LOOP(name, ...) for (int i = 0; i < n; ++i) refk::name(i, __VA_ARGS__)
@@ -69,6 +72,22 @@ const std::map<std::string, Runner>& fv_table() {
       REF_FV(fv_paired_limiter_scale), REF_FV(fv_paired_gradient_limiter_scale),
       REF_FV(fv_jump_gradient), REF_FV(fv_face_jump), REF_FV(fv_gather_jump_gradient),
       REF_FV(fv_paired_jump_gradient), REF_FV(fv_lsq_restore_inverse),
+      REF_FV(fv_diffusion_correction_bound), REF_FV(fv_transport_faces),
+      REF_FV(fv_transport_assemble), REF_FV(fv_fixed_rows), REF_FV(fv_floor),
+      REF_FV(fv_scalar_steady_measure), REF_FV(fv_scalar_set_fixed), REF_FV(fv_scalar_monitor),
+      REF_FV(fv_compensated_pressure_jump), REF_FV(fv_paired_pressure_jump),
+      REF_FV(fv_compensated_rhie_chow), REF_FV(fv_paired_pressure_boundary_low),
+      REF_FV(fv_paired_pressure_faces), REF_FV(fv_paired_momentum_residual),
+      REF_FV(fv_compensated_momentum_residual), REF_FV(fv_paired_accumulate),
+      REF_FV(fv_compensated_pressure_force), REF_FV(fv_paired_velocity_boundary),
+      REF_FV(fv_paired_advection), REF_FV(fv_paired_face_residual), REF_FV(fv_paired_flux),
+      REF_FV(fv_paired_flux_offset), REF_FV(fv_paired_continuity), REF_FV(fv_paired_pressure_rhs),
+      REF_FV(fv_paired_correct_flux), REF_FV(fv_paired_correct_velocity), REF_FV(fv_paired_value),
+      REF_FV(fv_paired_vector_values), REF_FV(fv_compensated_correct_flux),
+      REF_FV(fv_compensated_correct_velocity),
+      REF_FV(fv_backflow_pressure), REF_FV(fv_effective_viscosity), REF_FV(fv_transpose_stress),
+      REF_FV(fv_stress_source), REF_FV(fv_boundary_momentum), REF_FV(fv_momentum_diagonal),
+      REF_FV(fv_sst_force), REF_FV(fv_turbulence_boundary),
   };
   return t;
 }

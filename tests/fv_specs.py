@@ -21,6 +21,14 @@ Roles:
   f f+ one     float scalar (normal, uniform [0.5, 2]) / int scalar 1
   eps          limiter eps2: negative (Barth-Jespersen) for odd seeds, positive
                (Venkatakrishnan) for even seeds
+  entry rows cols diag   the pressure stencil's cell-face entry map and CSR arrays
+  ref          a pinned cell (int scalar): the middle cell
+  bmask spec   int per boundary face: 0 or 1 / a turbulence specification 0-2
+  component    int scalar: the velocity component, seed % 3
+  N oN         float per matrix nonzero (input / output)
+  mask         int 0 or 1 per cell
+  flag flag2   int switches: seed % 2 and (seed // 2) % 2, so consecutive seeds
+               cover both sides of each
 """
 from __future__ import annotations
 
@@ -73,6 +81,53 @@ SPECS = {
     "fv_paired_jump_gradient": ("cells", "offsets faces adjacent sign E E E C6 F F F F F F C C B B "
                                          "C C C C C C oC oC oC cells"),
     "fv_lsq_restore_inverse": ("subset", "subset S6 oC6 count cells"),
+    # kernels/fv_transport.kern.h
+    "fv_diffusion_correction_bound": ("faces", "owner neighbour kind C B F+ oF internal"),
+    "fv_transport_faces": ("faces", "owner neighbour F F F F F F F F F F C C C C F F+ kind oF "
+                                    "internal flag flag2"),
+    "fv_transport_assemble": ("cells", "offsets faces sign adjacent entry rows diag F+ F F+ kind B F "
+                                       "C C+ C C+ C C oN oC oC oC oC oC internal f+ f+ f+ flag"),
+    "fv_fixed_rows": ("cells", "rows diag mask C oN oC"),
+    "fv_floor": ("cells", "oC f"),
+    "fv_scalar_steady_measure": ("cells", "C C C mask oC oC flag"),
+    "fv_scalar_set_fixed": ("cells", "mask C oC"),
+    "fv_scalar_monitor": ("cells", "C C C mask C+ oC oC oC f+ f+"),
+    # kernels/fv_paired.kern.h
+    "fv_compensated_pressure_jump": ("faces", "owner neighbour F F F F F F C C C C B oF internal"),
+    "fv_paired_pressure_jump": ("faces", "owner neighbour F F F F F F C C C C C C C C B B oF internal"),
+    "fv_compensated_rhie_chow": ("faces", "owner neighbour kind F F F F+ C+ C+ F F F F F F F oF oF "
+                                          "internal f+"),
+    "fv_paired_pressure_boundary_low": ("boundary", "owner kind F F F C C C C C C C C B oB internal"),
+    "fv_paired_pressure_faces": ("faces", "owner neighbour kind mask F F F F F F C C C C C C C C B B C+ "
+                                          "oF oF internal flag"),
+    "fv_paired_momentum_residual": ("cells", "rows cols N diag C C C C C C+ C C oC"),
+    "fv_compensated_momentum_residual": ("cells", "rows cols N diag C C C C C C+ C C oC"),
+    "fv_paired_accumulate": ("cells", "C oC oC f+"),
+    "fv_compensated_pressure_force": ("cells", "offsets faces sign F F F C+ F F oC oC oC"),
+    "fv_paired_velocity_boundary": ("boundary", "owner kind F F F C C C F F oB oB oB oBi internal f+"),
+    "fv_paired_advection": ("faces", "owner neighbour kind F F F F F F C B C C C C F F oF internal flag"),
+    "fv_paired_face_residual": ("cells", "offsets faces sign F F F C oC flag"),
+    "fv_paired_flux": ("faces", "owner neighbour kind F F F F F F F F F C+ C+ C+ F+ C C C C C C C F "
+                                "oF oF internal f+ f+"),
+    "fv_paired_flux_offset": ("faces", "F F F F oF oF f+"),
+    "fv_paired_continuity": ("cells", "offsets faces sign F F oC oC"),
+    "fv_paired_pressure_rhs": ("cells", "C oC ref"),
+    "fv_paired_correct_flux": ("faces", "owner neighbour F F F C F oF oF internal f+"),
+    "fv_paired_correct_velocity": ("cells", "C+ C+ C+ C C C C oC oC oC oC oC oC oC oC f+ f+ f+"),
+    "fv_paired_value": ("cells", "C C oC"),
+    "fv_paired_vector_values": ("cells", "C C C C C C oC oC oC"),
+    "fv_compensated_correct_flux": ("faces", "owner neighbour F F F C F oF oF internal f+"),
+    "fv_compensated_correct_velocity": ("cells", "C+ C+ C+ C C C C oC oC oC oC oC oC oC oC f+ f+ f+"),
+    # kernels/fv_rans.kern.h
+    "fv_backflow_pressure": ("boundary", "owner bmask F C C C oB internal f+"),
+    "fv_effective_viscosity": ("faces", "owner neighbour kind F C+ oF internal f+"),
+    "fv_transpose_stress": ("faces", "owner neighbour kind F F+ F F F C C C C C C C C C oF oF oF internal"),
+    "fv_stress_source": ("cells", "offsets faces sign C+ F F F oC oC oC"),
+    "fv_boundary_momentum": ("cells", "offsets faces kind F F F B+ F+ F+ C C C B B B oC oC internal "
+                                      "component flag"),
+    "fv_momentum_diagonal": ("cells", "C C C C C C oC oC"),
+    "fv_sst_force": ("boundary", "B01 F F F B B B oB oB oB internal"),
+    "fv_turbulence_boundary": ("boundary", "owner kind spec B+ B+ F B B B C+ C+ B+ oB oB oBi internal f+ f+"),
 }
 
 
@@ -86,6 +141,11 @@ class FvMesh:
         self.faces = np.ascontiguousarray(bundle["cell_faces"], np.int32)
         self.sign = np.ascontiguousarray(bundle["cell_face_sign"], np.float32)
         self.adjacent = np.ascontiguousarray(bundle["adjacent"], np.int32)
+        self.entry = np.ascontiguousarray(bundle["matrix_entry"], np.int32)
+        self.rows = np.ascontiguousarray(bundle["row_ptr"], np.int32)
+        self.diag = np.ascontiguousarray(bundle["diag"], np.int32)
+        self.cols = np.ascontiguousarray(bundle["col"], np.int32)
+        self.nnz = int(self.rows[-1])
         self.cells = self.offsets.size - 1
         self.entries = self.faces.size
         self.subset = self.cells // 3
@@ -102,7 +162,7 @@ def build_args(mesh: FvMesh, spec: str, seed: int):
     """Arguments for one kernel call and the indices of its output arrays."""
     rng = np.random.default_rng(seed)
     size = {"F": mesh.n_faces, "C": mesh.cells, "B": mesh.boundary, "E": mesh.entries,
-            "C6": 6 * mesh.cells, "S6": 6 * mesh.subset}
+            "C6": 6 * mesh.cells, "S6": 6 * mesh.subset, "N": mesh.nnz}
     args, outputs = [], []
     for role in spec.split():
         if role == "owner":
@@ -119,15 +179,29 @@ def build_args(mesh: FvMesh, spec: str, seed: int):
             args.append(mesh.sign.copy())
         elif role == "adjacent":
             args.append(mesh.adjacent.copy())
+        elif role == "entry":
+            args.append(mesh.entry.copy())
+        elif role == "rows":
+            args.append(mesh.rows.copy())
+        elif role == "diag":
+            args.append(mesh.diag.copy())
+        elif role == "cols":
+            args.append(mesh.cols.copy())
+        elif role == "mask":
+            args.append(rng.integers(0, 2, mesh.cells).astype(np.int32))
+        elif role == "bmask":
+            args.append(rng.integers(0, 2, mesh.boundary).astype(np.int32))
+        elif role == "spec":
+            args.append(rng.integers(0, 3, mesh.boundary).astype(np.int32))
         elif role == "subset":
             args.append(rng.permutation(mesh.cells)[:mesh.subset].astype(np.int32))
-        elif role in ("F", "C", "B", "E", "C6", "S6"):
+        elif role in ("F", "C", "B", "E", "C6", "S6", "N"):
             args.append(rng.standard_normal(size[role]).astype(np.float32))
         elif role in ("F+", "C+", "B+"):
             args.append(rng.uniform(0.5, 2.0, size[role[0]]).astype(np.float32))
         elif role == "B01":
             args.append(rng.integers(0, 2, mesh.boundary).astype(np.float32))
-        elif role in ("oF", "oC", "oB", "oC6"):
+        elif role in ("oF", "oC", "oB", "oC6", "oN"):
             outputs.append(len(args))
             args.append(rng.standard_normal(size[role[1:]]).astype(np.float32))
         elif role == "oBi":
@@ -139,6 +213,10 @@ def build_args(mesh: FvMesh, spec: str, seed: int):
             args.append(int(mesh.cells))
         elif role == "count":
             args.append(int(mesh.subset))
+        elif role == "ref":
+            args.append(int(mesh.cells // 2))
+        elif role == "component":
+            args.append(seed % 3)
         elif role == "eps":
             args.append(-1.0 if seed % 2 else float(np.float32(rng.uniform(0.01, 0.5))))
         elif role == "f":
@@ -147,6 +225,10 @@ def build_args(mesh: FvMesh, spec: str, seed: int):
             args.append(float(np.float32(rng.uniform(0.5, 2.0))))
         elif role == "one":
             args.append(1)
+        elif role == "flag":
+            args.append(seed % 2)
+        elif role == "flag2":
+            args.append((seed // 2) % 2)
         else:
             raise ValueError(f"unknown role {role}")
     return args, outputs
@@ -156,7 +238,7 @@ def signature_of(spec: str) -> str:
     """The parameter kinds a spec implies ('b' buffer, 'f' float, 'i' int)."""
     out = []
     for role in spec.split():
-        if role in ("internal", "one", "cells", "count"):
+        if role in ("internal", "one", "cells", "count", "flag", "flag2", "ref", "component"):
             out.append("i")
         elif role in ("f", "f+", "eps"):
             out.append("f")

@@ -24,16 +24,20 @@ def mesh(request):
 def test_fv_kernel_bitwise_vs_oracle(device, mesh, name):
     domain, spec = SPECS[name]
     n = mesh.domain(domain)
-    print(f"[fv] {name} n={n}", flush=True)
-    args, outputs = build_args(mesh, spec, seed=3)
-    on_device = [device.upload(a) if isinstance(a, np.ndarray) else a for a in args]
-    expected = [a.copy() if isinstance(a, np.ndarray) else a for a in args]
-    device.launch(kutta_amd.Kernel[name], n, *on_device)
-    oracle.fv_each(name, n, *expected)
-    for k in outputs:
-        got = on_device[k].download()
-        assert same_bits(got, expected[k]), (name, k, int(np.sum(got.view(np.uint32) != expected[k].view(np.uint32))))
-    # Inputs are untouched.
-    for k, a in enumerate(args):
-        if isinstance(a, np.ndarray) and k not in outputs:
-            assert np.array_equal(on_device[k].download().view(np.uint8), a.view(np.uint8)), (name, k)
+    for seed in (3, 4):
+        print(f"[fv] {name} n={n} seed={seed}", flush=True)
+        args, outputs = build_args(mesh, spec, seed=seed)
+        on_device = [device.upload(a) if isinstance(a, np.ndarray) else a for a in args]
+        expected = [a.copy() if isinstance(a, np.ndarray) else a for a in args]
+        device.launch(kutta_amd.Kernel[name], n, *on_device)
+        oracle.fv_each(name, n, *expected)
+        for k in outputs:
+            got = on_device[k].download()
+            if not same_bits(got, expected[k]):
+                bad = np.flatnonzero(got.view(np.uint32) != expected[k].view(np.uint32))
+                raise AssertionError(f"{name} seed {seed} output {k}: {bad.size} differ, first at "
+                                     f"{bad[0]}: {got[bad[0]]!r} vs {expected[k][bad[0]]!r}")
+        # Inputs are untouched.
+        for k, a in enumerate(args):
+            if isinstance(a, np.ndarray) and k not in outputs:
+                assert np.array_equal(on_device[k].download().view(np.uint8), a.view(np.uint8)), (name, k)
