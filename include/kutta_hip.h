@@ -231,11 +231,13 @@ typedef enum kh_kernel {
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 
-/* One level of KH_FUSED_COARSE_TAIL: the level's CSR operator and l1 inverse
- * diagonal, and the aggregation transfer to the next level (ignored on the
- * last). Levels run in LDS, so the tail accepts at most KH_TAIL_MAX_LEVELS
- * levels of at most KH_TAIL_MAX_ROWS rows each, with 2 * (sum of rows) at most
- * KH_TAIL_LDS_FLOATS; a table outside these limits makes the kernel a no-op. */
+/* One level of KH_FUSED_COARSE_TAIL: the level's CSR operator (rows, nnz) and
+ * l1 inverse diagonal, and the aggregation transfer to the next level (ignored
+ * on the last). Levels run in LDS, so the tail accepts at most
+ * KH_TAIL_MAX_LEVELS levels of at most KH_TAIL_MAX_ROWS rows each, with
+ * 2 * (sum of rows) at most KH_TAIL_LDS_FLOATS and
+ * 2 * (sum of rows) + max(rows) + max(3 * rows + 1 + 2 * nnz) at most
+ * KH_TAIL_POOL_WORDS; a table outside these limits makes the kernel a no-op. */
 #define KH_TAIL_MAX_LEVELS 16
 #define KH_TAIL_MAX_ROWS 8192
 #define KH_TAIL_LDS_FLOATS 24576
@@ -252,11 +254,10 @@ typedef struct kh_tail_level {
   int32_t nnz;
   int32_t pad;
 } kh_tail_level;
-/* The tail's LDS pool in 32-bit words. The vectors (2 * sum of rows) come
- * first; the rest holds one level's operator at a time while that level is
- * swept: 3 * rows + 1 + 2 * nnz words (row starts, columns, values, l1 inverse,
- * aggregate map). A level whose copy fits is swept from LDS, others from
- * global memory. */
+/* The tail's LDS pool in 32-bit words: the iterates and right-hand sides of
+ * all its levels, one sweep's new values, and the operator of the level being
+ * swept (row starts, columns, values, l1 inverse, aggregate map), copied in on
+ * each visit. */
 #define KH_TAIL_POOL_WORDS 40448
 
 /* Device::launch (device.hpp:205; cuda_device.cu:132-140): runs kernel `kernel`

@@ -149,32 +149,24 @@ struct KrylovSolver::Impl {
     complexity = a.nnz() ? double(total) / a.nnz() : 0;
   }
 
-  // Whether levels [first, end) fit one KH_FUSED_COARSE_TAIL workgroup.
+  // Whether levels [first, end) fit one KH_FUSED_COARSE_TAIL workgroup: all
+  // iterates and right-hand sides, one sweep's new values and the largest
+  // level's operator copy in LDS at once.
   bool tail_fits(std::size_t first) const {
     if (first < 1 || first >= levels.size() || levels.size() - first > KH_TAIL_MAX_LEVELS) return false;
-    std::size_t floats = 0;
+    std::size_t vectors = 0, widest = 0, largest = 0;
     for (std::size_t i = first; i < levels.size(); ++i) {
-      if (levels[i].rows > KH_TAIL_MAX_ROWS) return false;
-      floats += 2 * std::size_t(levels[i].rows);
+      const std::size_t rows = std::size_t(levels[i].rows), nnz = std::size_t(levels[i].nnz);
+      if (rows > KH_TAIL_MAX_ROWS) return false;
+      vectors += 2 * rows;
+      widest = std::max(widest, rows);
+      largest = std::max(largest, 3 * rows + 1 + 2 * nnz);
     }
-    return floats <= KH_TAIL_LDS_FLOATS;
-  }
-
-  // Whether every level in [first, end) can be swept from an LDS copy of its
-  // operator, next to the tail's vectors.
-  bool tail_staged(std::size_t first) const {
-    std::size_t vectors = 0;
-    for (std::size_t i = first; i < levels.size(); ++i) vectors += 2 * std::size_t(levels[i].rows);
-    for (std::size_t i = first; i < levels.size(); ++i) {
-      const std::size_t words = 3 * std::size_t(levels[i].rows) + 1 + 2 * std::size_t(levels[i].nnz);
-      if (vectors + words > KH_TAIL_POOL_WORDS) return false;
-    }
-    return true;
+    return vectors <= KH_TAIL_LDS_FLOATS && vectors + widest + largest <= KH_TAIL_POOL_WORDS;
   }
 
   // Picks the tail's first level: the requested one, or else the first level
-  // from which every level fits one workgroup with its operator in LDS. One
-  // compute unit sweeping an operator out of L2 is slower than grid launches.
+  // from which the rest of the hierarchy fits one workgroup.
   void plan_tail() {
     tail_first = 0;
     tail_table = DeviceArray<kh_tail_level>();
@@ -185,7 +177,7 @@ struct KrylovSolver::Impl {
       tail_first = tail_request;
     } else {
       for (std::size_t i = 1; i < levels.size(); ++i)
-        if (tail_fits(i) && tail_staged(i)) {
+        if (tail_fits(i)) {
           tail_first = int(i);
           break;
         }

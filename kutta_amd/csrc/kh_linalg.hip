@@ -558,33 +558,24 @@ __global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
 // Below a few thousand rows a level's launches are pure latency: each sweep
 // is a handful of dependent memory round trips plus a kernel boundary. The
 // tail runs every remaining level of the V-cycle (linalg/amg.cpp:283-310) in
-// one workgroup with sweeps separated by workgroup barriers. Each level's
-// iterate and right-hand side live in LDS for the whole cycle; on each visit
-// to a level its operator (row starts, columns, values, l1 inverse and
-// aggregate map) is copied into the rest of LDS when it fits, so its sweeps
-// touch no global memory. Every row is computed by the same expression as the
+// one workgroup, with sweeps separated by workgroup barriers and everything
+// in LDS: each level's iterate and right-hand side for the whole cycle, and,
+// on each visit to a level, a copy of its operator (row starts, columns,
+// values, l1 inverse, aggregate map). The code is kept small on purpose: one
+// workgroup re-executes it for every sweep from one compute unit's
+// instruction cache. Every row is computed by the same expression as the
 // per-level kernels (l1-Jacobi, kernels/linalg.kern.h:150-170; restriction,
 // amg.kern.h:5-16; coarse solve, linalg.kern.h:126-147), and Jacobi rows only
 // read the previous sweep, so the iterates are those of the launch sequence
 // it replaces.
 constexpr int kTailThreads = 1024;
-constexpr int kTailRowsPerThread = KH_TAIL_MAX_ROWS / kTailThreads;
-static_assert(KH_TAIL_MAX_ROWS % kTailThreads == 0, "tail rows per thread");
-// LDS pool in 32-bit words (gfx950 allows 160 KiB per workgroup).
 constexpr int kTailPoolWords = KH_TAIL_POOL_WORDS;
-static_assert(KH_TAIL_LDS_FLOATS <= kTailPoolWords, "tail vector budget");
 
 // Words an operator copy of a level takes: row starts, columns, values, the
 // l1 inverse and the aggregate map.
-__host__ __device__ constexpr int tail_stage_words(int rows, int nnz) { return 3 * rows + 1 + 2 * nnz; }
+__device__ __forceinline__ int tail_stage_words(int rows, int nnz) { return 3 * rows + 1 + 2 * nnz; }
 
-// Reads through pointers known to address global memory.
-template <class T>
-__device__ __forceinline__ const T* global_ptr(const T* p) {
-  return (const T*)((const __attribute__((address_space(1))) T*)p);
-}
-
-// A level's operator as the sweeps read it: in LDS or in global memory.
+// A level's operator, copied into LDS.
 struct TailOperator {
   const int* row_ptr;
   const int* col;
@@ -594,60 +585,49 @@ struct TailOperator {
   int rows;
 };
 
+// Compensated residual b - sum a[j] x[col[j]] of row i, entries in order.
+template <class X>
+__device__ __forceinline__ float tail_residual(const TailOperator& L, float b, int i, X gather) {
+  RowResidual acc(b);
+  for (int j = L.row_ptr[i]; j < L.row_ptr[i + 1]; ++j) acc.add(L.val[j], gather(L.col[j]));
+  return acc.value();
+}
+
 enum class TailSweep { Plain, FromZero, Prolonged };
 
-// One in-place sweep over a level: new values wait in registers until every
-// row has read the previous iterate.
-template <TailSweep S>
-__device__ __forceinline__ void tail_sweep(const TailOperator& L, float* x, const float* b,
-                                           const float* coarse) {
-  float next[kTailRowsPerThread];
-#pragma unroll
-  for (int u = 0; u < kTailRowsPerThread; ++u) {
-    const int i = int(threadIdx.x) + u * kTailThreads;
-    if (i >= L.rows) break;
+// One sweep over a level: every row writes its new value to `next`, then the
+// iterate is replaced once every row has read the previous one.
+__device__ void tail_sweep(TailSweep kind, const TailOperator& L, float* x, const float* b,
+                           const float* coarse, float* next) {
+  for (int i = threadIdx.x; i < L.rows; i += kTailThreads) {
     const float bi = b[i];
     const float w = (2.0f / 3.0f) * L.inverse[i];
-    if constexpr (S == TailSweep::FromZero) {
-      if (bi - bi != 0.0f) {
-        RowResidual acc(bi);
-        for (int j = L.row_ptr[i]; j < L.row_ptr[i + 1]; ++j) acc.add(L.val[j], 0.0f);
-        next[u] = 0.0f + w * acc.value();
-      } else {
-        next[u] = 0.0f + w * (bi + 0.0f);
-      }
-    } else if constexpr (S == TailSweep::Prolonged) {
+    float value;
+    if (kind == TailSweep::FromZero) {
+      value = bi - bi != 0.0f ? 0.0f + w * tail_residual(L, bi, i, [](int) { return 0.0f; })
+                              : 0.0f + w * (bi + 0.0f);
+    } else if (kind == TailSweep::Prolonged) {
       const Prolonged gather{x, L.aggregate, coarse};
-      const float r =
-          row_residual_chunked<kChunk>(bi, L.row_ptr[i], L.row_ptr[i + 1], L.col, L.val, gather);
-      next[u] = gather(i) + w * r;
+      value = gather(i) + w * tail_residual(L, bi, i, gather);
     } else {
-      const float r =
-          row_residual_chunked<kChunk>(bi, L.row_ptr[i], L.row_ptr[i + 1], L.col, L.val, Plain{x});
-      next[u] = x[i] + w * r;
+      value = x[i] + w * tail_residual(L, bi, i, Plain{x});
     }
+    next[i] = value;
   }
   __syncthreads();
-#pragma unroll
-  for (int u = 0; u < kTailRowsPerThread; ++u) {
-    const int i = int(threadIdx.x) + u * kTailThreads;
-    if (i >= L.rows) break;
-    x[i] = next[u];
-  }
+  for (int i = threadIdx.x; i < L.rows; i += kTailThreads) x[i] = next[i];
   __syncthreads();
 }
 
 // Restricted residual of level L into the next level's right-hand side.
-__device__ __forceinline__ void tail_restrict(const TailOperator& L, const kh_tail_level& T,
-                                              const float* x, const float* b, float* coarse_b) {
-  const int* offsets = global_ptr(T.member_offsets);
-  const int* members = global_ptr(T.members);
-  for (int g = threadIdx.x; g < T.coarse_rows; g += kTailThreads) {
+__device__ void tail_restrict(const TailOperator& L, const int* __restrict__ offsets,
+                              const int* __restrict__ members, int coarse_rows, const float* x,
+                              const float* b, float* coarse_b) {
+  for (int g = threadIdx.x; g < coarse_rows; g += kTailThreads) {
     Kahan k;
     for (int m = offsets[g]; m < offsets[g + 1]; ++m) {
       const int row = members[m];
-      k.add(row_residual_chunked<kChunk>(b[row], L.row_ptr[row], L.row_ptr[row + 1], L.col, L.val,
-                                         Plain{x}));
+      k.add(tail_residual(L, b[row], row, Plain{x}));
     }
     coarse_b[g] = k.s;
   }
@@ -656,7 +636,7 @@ __device__ __forceinline__ void tail_restrict(const TailOperator& L, const kh_ta
 
 // Constant-nullspace projection: sum_into + solver_project
 // (linalg/detail.hpp:13-19; kernels/scalars.kern.h:150-152).
-__device__ __forceinline__ void tail_project(float* x, int rows, double* red, float* pair) {
+__device__ void tail_project(float* x, int rows, double* red, float* pair) {
   double v = 0.0;
   for (int i = threadIdx.x; i < rows; i += kTailThreads) v += double(x[i]);
   v = block_sum(v, red);
@@ -673,40 +653,38 @@ __device__ __forceinline__ void tail_project(float* x, int rows, double* red, fl
   __syncthreads();
 }
 
-// Copies words [0, count) of src into LDS at dst, all lanes, then waits.
-__device__ __forceinline__ void tail_copy(uint32_t* dst, const uint32_t* src, int count) {
-  const uint32_t* from = global_ptr(src);
-  for (int k = threadIdx.x; k < count; k += kTailThreads) dst[k] = from[k];
+// Copies `count` words from global memory into LDS, eight loads in flight
+// per lane.
+__device__ void tail_copy(uint32_t* dst, const uint32_t* __restrict__ src, int count) {
+  constexpr int kUnroll = 8;
+  int k = threadIdx.x;
+  for (; k + (kUnroll - 1) * kTailThreads < count; k += kUnroll * kTailThreads) {
+    uint32_t v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) v[u] = src[k + u * kTailThreads];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) dst[k + u * kTailThreads] = v[u];
+  }
+  for (; k < count; k += kTailThreads) dst[k] = src[k];
 }
 
-// Runs `work` on level T's operator: staged into LDS at `stage` when
-// `staged`, else read in place. `staged` is uniform across the workgroup.
-template <class Work>
-__device__ __forceinline__ void tail_visit(const kh_tail_level& T, bool has_aggregate,
-                                           bool staged, uint32_t* stage, Work work) {
-  if (staged) {
-    const int rows = T.rows, nnz = T.nnz;
-    uint32_t* row_ptr = stage;
-    uint32_t* col = row_ptr + rows + 1;
-    uint32_t* val = col + nnz;
-    uint32_t* inverse = val + nnz;
-    uint32_t* aggregate = inverse + rows;
-    tail_copy(row_ptr, reinterpret_cast<const uint32_t*>(T.row_ptr), rows + 1);
-    tail_copy(col, reinterpret_cast<const uint32_t*>(T.col), nnz);
-    tail_copy(val, reinterpret_cast<const uint32_t*>(T.val), nnz);
-    tail_copy(inverse, reinterpret_cast<const uint32_t*>(T.inverse), rows);
-    if (has_aggregate) tail_copy(aggregate, reinterpret_cast<const uint32_t*>(T.aggregate), rows);
-    __syncthreads();
-    work(TailOperator{reinterpret_cast<const int*>(row_ptr), reinterpret_cast<const int*>(col),
-                      reinterpret_cast<const float*>(val), reinterpret_cast<const float*>(inverse),
-                      reinterpret_cast<const int*>(aggregate), rows});
-    // The next visit overwrites the staged copy.
-    __syncthreads();
-  } else {
-    work(TailOperator{global_ptr(T.row_ptr), global_ptr(T.col), global_ptr(T.val),
-                      global_ptr(T.inverse), has_aggregate ? global_ptr(T.aggregate) : nullptr,
-                      T.rows});
-  }
+// Copies level T's operator into LDS at `stage` and describes it.
+__device__ TailOperator tail_stage(const kh_tail_level& T, bool has_aggregate, uint32_t* stage) {
+  const int rows = T.rows, nnz = T.nnz;
+  uint32_t* row_ptr = stage;
+  uint32_t* col = row_ptr + rows + 1;
+  uint32_t* val = col + nnz;
+  uint32_t* inverse = val + nnz;
+  uint32_t* aggregate = inverse + rows;
+  tail_copy(row_ptr, reinterpret_cast<const uint32_t*>(T.row_ptr), rows + 1);
+  tail_copy(col, reinterpret_cast<const uint32_t*>(T.col), nnz);
+  tail_copy(val, reinterpret_cast<const uint32_t*>(T.val), nnz);
+  tail_copy(inverse, reinterpret_cast<const uint32_t*>(T.inverse), rows);
+  if (has_aggregate) tail_copy(aggregate, reinterpret_cast<const uint32_t*>(T.aggregate), rows);
+  __syncthreads();
+  return {reinterpret_cast<const int*>(row_ptr), reinterpret_cast<const int*>(col),
+          reinterpret_cast<const float*>(val), reinterpret_cast<const float*>(inverse),
+          reinterpret_cast<const int*>(aggregate), rows};
 }
 
 __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
@@ -717,72 +695,66 @@ __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
   __shared__ uint32_t pool[kTailPoolWords];
   __shared__ double red[kTailThreads / 64];
   __shared__ float pair[2];
-  // Level k keeps its iterate at pool[base[k]] and right-hand side right after.
+  // Level k keeps its iterate at vectors[base[k]] and right-hand side right
+  // after; `next` holds a sweep's new values; `stage` the visited operator.
   __shared__ int base[KH_TAIL_MAX_LEVELS];
   // The table is validated by every thread alike, so an invalid one returns
   // uniformly before any barrier.
   if (count < 1 || count > KH_TAIL_MAX_LEVELS) return;
-  int used = 0;
+  int used = 0, widest = 0, largest = 0;
   for (int k = 0; k < count; ++k) {
-    const int rows = levels[k].rows;
-    if (rows < 1 || rows > KH_TAIL_MAX_ROWS || levels[k].nnz < 0) return;
+    const int rows = levels[k].rows, nnz = levels[k].nnz;
+    if (rows < 1 || rows > KH_TAIL_MAX_ROWS || nnz < 0 || nnz > kTailPoolWords) return;
     if (k + 1 < count && levels[k].coarse_rows != levels[k + 1].rows) return;
     if (threadIdx.x == 0) base[k] = used;
     used += 2 * rows;
-    if (used > KH_TAIL_LDS_FLOATS) return;
+    widest = max(widest, rows);
+    largest = max(largest, tail_stage_words(rows, nnz));
   }
-  uint32_t* stage = pool + used;
-  const int room = kTailPoolWords - used;
+  if (used + widest + largest > kTailPoolWords) return;
   __syncthreads();
   float* vectors = reinterpret_cast<float*>(pool);
+  float* next = vectors + used;
+  uint32_t* stage = pool + used + widest;
   {
     const int rows = levels[0].rows;
     for (int i = threadIdx.x; i < rows; i += kTailThreads) vectors[base[0] + rows + i] = rhs[i];
   }
   __syncthreads();
-  auto fits = [&](const kh_tail_level& T) { return tail_stage_words(T.rows, T.nnz) <= room; };
   for (int k = 0; k + 1 < count; ++k) {
     const kh_tail_level T = levels[k];
     float* x = vectors + base[k];
     const float* b = x + T.rows;
     float* coarse_b = vectors + base[k + 1] + T.coarse_rows;
-    tail_visit(T, true, fits(T), stage, [&](const TailOperator& L) {
-      for (int t = 1; t <= pre; ++t) {
-        if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
-        else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-      }
-      tail_restrict(L, T, x, b, coarse_b);
-    });
+    const TailOperator L = tail_stage(T, true, stage);
+    for (int t = 1; t <= pre; ++t)
+      tail_sweep(t == 1 ? TailSweep::FromZero : TailSweep::Plain, L, x, b, nullptr, next);
+    tail_restrict(L, T.member_offsets, T.members, T.coarse_rows, x, b, coarse_b);
     if (nullspace) tail_project(coarse_b, T.coarse_rows, red, pair);
   }
   {
     const kh_tail_level T = levels[count - 1];
     float* x = vectors + base[count - 1];
     const float* b = x + T.rows;
-    tail_visit(T, false, fits(T), stage, [&](const TailOperator& L) {
-      if (T.rows <= kCoarseMax) {
-        // amg_coarse_solve: sweeps from the zero iterate, general residual form.
-        for (int i = threadIdx.x; i < T.rows; i += kTailThreads) x[i] = 0.0f;
-        __syncthreads();
-        for (int t = 0; t < coarse_sweeps; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-      } else {
-        for (int t = 1; t <= coarse_sweeps; ++t) {
-          if (t == 1) tail_sweep<TailSweep::FromZero>(L, x, b, nullptr);
-          else tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-        }
-      }
-    });
+    const TailOperator L = tail_stage(T, false, stage);
+    if (T.rows <= kCoarseMax) {
+      // amg_coarse_solve: sweeps from the zero iterate, general residual form.
+      for (int i = threadIdx.x; i < T.rows; i += kTailThreads) x[i] = 0.0f;
+      __syncthreads();
+      for (int t = 0; t < coarse_sweeps; ++t) tail_sweep(TailSweep::Plain, L, x, b, nullptr, next);
+    } else {
+      for (int t = 1; t <= coarse_sweeps; ++t)
+        tail_sweep(t == 1 ? TailSweep::FromZero : TailSweep::Plain, L, x, b, nullptr, next);
+    }
     if (nullspace) tail_project(x, T.rows, red, pair);
   }
   for (int k = count - 2; k >= 0; --k) {
     const kh_tail_level T = levels[k];
     float* x = vectors + base[k];
     const float* b = x + T.rows;
-    const float* coarse = vectors + base[k + 1];
-    tail_visit(T, true, fits(T), stage, [&](const TailOperator& L) {
-      tail_sweep<TailSweep::Prolonged>(L, x, b, coarse);
-      for (int t = 2; t <= post; ++t) tail_sweep<TailSweep::Plain>(L, x, b, nullptr);
-    });
+    const TailOperator L = tail_stage(T, true, stage);
+    tail_sweep(TailSweep::Prolonged, L, x, b, vectors + base[k + 1], next);
+    for (int t = 2; t <= post; ++t) tail_sweep(TailSweep::Plain, L, x, b, nullptr, next);
     if (nullspace) tail_project(x, T.rows, red, pair);
   }
   for (int i = threadIdx.x; i < levels[0].rows; i += kTailThreads) out[i] = vectors[base[0] + i];

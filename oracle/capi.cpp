@@ -1,6 +1,8 @@
 // ORACLE — test infrastructure only (see oracle.hpp). A plain C interface to
 // the oracle for the Python tests (ctypes). Every entry point returns 0 on
 // success; on failure it returns nonzero and oracle_last_error() names why.
+#include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 
@@ -33,6 +35,48 @@ struct SolverHandle {
 extern "C" {
 
 const char* oracle_last_error() { return g_error.c_str(); }
+
+// Checks the window in which the device's one-FMA product error replaces the
+// reference's Dekker split (kernels/linalg.kern.h:21-30): over `count` seeded
+// pairs (half with uniformly random bits, half with exponents in [-60, 60)),
+// returns how many inside the window 2^-100 <= |a*b| <= 2^100, |a|,|b| <= 1e30
+// give a Dekker result whose bits differ from fma(a, b, -a*b); *inside receives
+// how many pairs fell in the window.
+long oracle_product_error_window(long count, unsigned long long seed, long* inside) {
+  std::uint64_t state = seed;
+  auto next = [&] {  // splitmix64
+    std::uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  long differ = 0, in = 0;
+  for (long k = 0; k < count; ++k) {
+    std::uint64_t r = next();
+    std::uint32_t ua = std::uint32_t(r), ub = std::uint32_t(r >> 32);
+    float a, b;
+    std::memcpy(&a, &ua, 4);
+    std::memcpy(&b, &ub, 4);
+    if (k & 1) {
+      int ea = 0, eb = 0;
+      const std::uint64_t e = next();
+      a = std::ldexp(std::frexp(a, &ea), int(e % 120) - 60);
+      b = std::ldexp(std::frexp(b, &eb), int((e >> 32) % 120) - 60);
+    }
+    if (!std::isfinite(a) || !std::isfinite(b)) continue;
+    const float p = a * b;
+    const float m = std::fabs(p);
+    if (!(m >= 0x1p-100f && m <= 0x1p100f && std::fabs(a) <= 1.0e30f && std::fabs(b) <= 1.0e30f)) continue;
+    ++in;
+    const float d = kern::csr_product_error(a, b, p), f = std::fma(a, b, -p);
+    std::uint32_t bd, bf;
+    std::memcpy(&bd, &d, 4);
+    std::memcpy(&bf, &f, 4);
+    if (bd != bf) ++differ;
+  }
+  *inside = in;
+  return differ;
+}
 
 // ---- reductions (runtime/cpu/cpu_device.cpp:37-89)
 double oracle_sum(const float* x, long n) { return sum(x, std::size_t(n)); }

@@ -262,3 +262,18 @@ def test_pressure_assembly_bitwise_vs_reference(tet, reference):
                      coefficient, flux, values, rhs, pin, outputs=[9, 10])
     assert_bitwise(got, want)
     assert np.all(np.isfinite(got[0])) and np.all(np.isfinite(got[1]))
+
+
+def test_fma_product_error_window_equals_dekker():
+    """The device computes the compensated residual's product error with one
+    FMA where 2^-100 <= |a*b| <= 2^100 and |a|, |b| <= 1e30, and with the
+    reference's Dekker split elsewhere (kernels/linalg.kern.h:21-30). Inside
+    that window Dekker's split is exact, so both give the exact rounding error;
+    40 million seeded pairs agree bit for bit."""
+    L = oracle.lib()
+    L.oracle_product_error_window.argtypes = [C.c_long, C.c_ulonglong, C.POINTER(C.c_long)]
+    L.oracle_product_error_window.restype = C.c_long
+    inside = C.c_long(0)
+    differ = L.oracle_product_error_window(40_000_000, 20261202, C.byref(inside))
+    assert inside.value > 25_000_000
+    assert differ == 0
