@@ -19,6 +19,7 @@ on one host core beside it.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--sweeps 5] [--tail T] [--no-cpu-baseline]
     python bench.py --copy-probe BYTES [--steps K]   # HBM counter calibration
+    python bench.py --fv-kernels [--steps K]         # each FV kernel at C2 size
 """
 from __future__ import annotations
 
@@ -121,6 +122,48 @@ def copy_probe(nbytes: int, repeats: int) -> int:
     return 0
 
 
+def fv_kernel_bench(size: int, repeats: int) -> int:
+    """Times every finite-volume kernel on the perturbed size^3 tetrahedral box
+    (998,250 cells at size 55) with seeded inputs (kutta_amd.fv_specs), HIP
+    events around each launch on the device stream, and prints one JSON line.
+    Algorithmic bytes count every array argument once at its full size (an
+    array updated in place counts once); the roofline is HBM bandwidth."""
+    import numpy as np
+
+    import kutta_amd
+    from kutta_amd.fv_specs import SPECS, FvMesh, build_args
+    dev = kutta_amd.Device(0)
+    mesh = FvMesh(kutta_amd.MeshSetup(size, size, size, tet=True, amplitude=0.2, seed=SEED))
+    rows = []
+    for name in sorted(SPECS):
+        domain, spec = SPECS[name]
+        n = mesh.domain(domain)
+        args, outputs = build_args(mesh, spec, seed=3)
+        on_device = [dev.upload(a) if isinstance(a, np.ndarray) else a for a in args]
+        kernel = kutta_amd.Kernel[name]
+        dev.launch(kernel, n, *on_device)
+        dev.synchronize()
+        dev.profile_reset()
+        dev.profile(kernel, True)
+        for _ in range(repeats):
+            dev.launch(kernel, n, *on_device)
+        dev.synchronize()
+        launches, ms = dev.profile_read(kernel)
+        dev.profile(kernel, False)
+        nbytes = sum(a.nbytes for a in args if isinstance(a, np.ndarray))
+        us = ms * 1e3 / max(launches, 1)
+        gbs = nbytes / (us * 1e-6) / 1e9
+        rows.append({"kernel": name, "n": n, "bytes": nbytes, "us": round(us, 3),
+                     "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        print(f"[fv] {name:36s} n={n:8d} {us:9.3f} us {gbs:8.1f} GB/s", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "finite-volume kernel HBM bandwidth", "unit": "GB/s",
+                      "peak": HBM_PEAK_GBS, "repeats": repeats, "data": "synthetic",
+                      "config": {"workload": f"perturbed {size}^3 tet box, RCM ordered",
+                                 "cells": mesh.cells, "faces": mesh.n_faces},
+                      "kernels": rows}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,11 +174,15 @@ def main() -> int:
     ap.add_argument("--tail", type=int, default=-1,
                     help="first AMG level of the one-workgroup coarse-grid tail (-1 auto, 0 off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fv-kernels", action="store_true",
+                    help="only time each finite-volume kernel on the size^3 mesh")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time device copies of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe, max(args.steps, 1))
+    if args.fv_kernels:
+        return fv_kernel_bench(args.size, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a run)", file=sys.stderr)

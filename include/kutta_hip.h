@@ -206,6 +206,17 @@ typedef enum kh_kernel {
   KH_FV_MOMENTUM_DIAGONAL,           /* :75-78 */
   KH_FV_SST_FORCE,                   /* :79-83 */
   KH_FV_TURBULENCE_BOUNDARY,         /* :86-101 */
+  /* kernels/sst.kern.h (k-omega SST; tanh/log/pow from the device library) */
+  KH_SST_RATES,                      /* :40-50 */
+  KH_SST_BLENDING,                   /* :52-70 */
+  KH_SST_VISCOSITY,                  /* :72-88 */
+  KH_SST_BLENDING_VISCOSITY,         /* :90-103 */
+  KH_SST_WALL,                       /* :105-151 */
+  KH_SST_WALL_OMEGA,                 /* :153-169 */
+  KH_SST_WALL_CELLS,                 /* :171-190 */
+  KH_SST_SOURCES,                    /* :192-236 */
+  KH_SST_FACE_DIFFUSION,             /* :238-253 */
+  KH_SST_SUSTAIN,                    /* :255-259 */
 
   /* Fused MI355X kernels (no reference counterpart). */
   KH_FUSED_SPMV_DOT_ALPHA = 200, /* csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha:

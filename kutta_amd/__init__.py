@@ -267,6 +267,16 @@ class Kernel(enum.IntEnum):
     fv_momentum_diagonal = 172
     fv_sst_force = 173
     fv_turbulence_boundary = 174
+    sst_rates = 175
+    sst_blending = 176
+    sst_viscosity = 177
+    sst_blending_viscosity = 178
+    sst_wall = 179
+    sst_wall_omega = 180
+    sst_wall_cells = 181
+    sst_sources = 182
+    sst_face_diffusion = 183
+    sst_sustain = 184
     fused_spmv_dot_alpha = 200
     fused_step_dot_check = 201
     fused_residual_restrict = 202

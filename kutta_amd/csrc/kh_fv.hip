@@ -1477,6 +1477,239 @@ struct FvTurbulenceBoundary {
   }
 };
 
+// ------------------------------------------------------------ sst.kern.h
+// k-omega SST coefficient blends and limiters (:5-38). tanh, log and pow come
+// from the device math library, so those results agree with the serial
+// reference to a few ulps rather than bit for bit.
+__device__ __forceinline__ float sst_inv_sigma_k(float f1) { return f1 / 1.176f + (1.0f - f1); }
+__device__ __forceinline__ float sst_inv_sigma_w(float f1) { return f1 / 2.0f + (1.0f - f1) / 1.168f; }
+__device__ __forceinline__ float sst_beta(float f1) { return 0.075f * f1 + 0.0828f * (1.0f - f1); }
+__device__ __forceinline__ float sst_gamma(float f1, int set_2003) {
+  const float g1 = set_2003 ? 5.0f / 9.0f : 0.075f / 0.09f - 0.41f * 0.41f / (2.0f * 0.3f);
+  const float g2 = set_2003 ? 0.44f : 0.0828f / 0.09f - 0.41f * 0.41f / (1.168f * 0.3f);
+  return f1 * g1 + (1.0f - f1) * g2;
+}
+__device__ __forceinline__ float sst_alpha_star(float ret, int low_re) {
+  return low_re ? 1.0f - 0.976f / (1.0f + ret / 6.0f) : 1.0f;
+}
+__device__ __forceinline__ float sst_beta_star(float ret, int low_re) {
+  if (!low_re) return 0.09f;
+  const float r = kmin(ret / 8.0f, 100.0f), r4 = r * r * r * r;
+  return 0.09f * ((4.0f / 15.0f) + r4) / (1.0f + r4);
+}
+__device__ __forceinline__ float sst_power_blend(float a, float b, float p) {
+  const float hi = kmax(a, b), lo = kmin(a, b);
+  return hi == 0.0f ? 0.0f : hi * powf(1.0f + powf(lo / hi, p), 1.0f / p);
+}
+__device__ __forceinline__ float sst_finite_ratio(float numerator, float denominator) {
+  return kmax(-1e38f, kmin(numerator / denominator, 1e38f));
+}
+// :40-50
+struct SstRates {
+  __device__ void operator()(int i, CF ux, CF uy, CF uz, CF vx, CF vy, CF vz, CF wx, CF wy, CF wz,
+                             WF strain, WF vorticity) const {
+    const float a = ux[i], b = uy[i], c = uz[i], d = vx[i], e = vy[i], f = vz[i], g = wx[i], h = wy[i], l = wz[i];
+    const float sxy = b + d, sxz = c + g, syz = f + h;
+    strain[i] = sqrtf(2.0f * (a * a + e * e + l * l) + sxy * sxy + sxz * sxz + syz * syz);
+    const float wxy = b - d, wxz = c - g, wyz = f - h;
+    vorticity[i] = sqrtf(wxy * wxy + wxz * wxz + wyz * wyz);
+  }
+};
+// :52-70
+struct SstBlending {
+  __device__ void operator()(int i, CF rho, CF mu, CF k, CF omega, CF distance, CF kx, CF ky, CF kz,
+                             CF ox, CF oy, CF oz, WF f1, WF f2, WF cdkw, float cd_floor,
+                             float k_floor, float omega_floor) const {
+    const float r = rho[i], ke = kmax(k[i], k_floor), w = kmax(omega[i], omega_floor), y = distance[i];
+    const float cd = sst_finite_ratio(2.0f * r * (kx[i] * ox[i] + ky[i] * oy[i] + kz[i] * oz[i]), 1.168f * w);
+    const float a = sqrtf(ke) / (0.09f * w * y);
+    const float b = 500.0f * (mu[i] / r) / (y * y * w);
+    const float c = 4.0f * r * ke / (1.168f * kmax(cd, cd_floor) * y * y);
+    const float p1 = kmin(kmin(kmax(a, b), c), 10.0f);
+    const float p2 = kmin(kmax(2.0f * a, b), 10.0f);
+    f1[i] = tanhf(p1 * p1 * p1 * p1);
+    f2[i] = tanhf(p2 * p2);
+    cdkw[i] = cd;
+  }
+};
+// :72-88
+struct SstViscosity {
+  __device__ void operator()(int i, CF rho, CF mu, CF k, CF omega, CF strain, CF vorticity, CF f1,
+                             CF f2, WF mut, WF gamma_k, WF gamma_w, int low_re, int use_vorticity,
+                             float max_ratio, float k_floor, float omega_floor) const {
+    const float r = rho[i], m = mu[i];
+    const float ke = kmax(k[i], k_floor), w = kmax(omega[i], omega_floor);
+    const float ret = (r / m) * (ke / w);
+    const float alpha = sst_alpha_star(ret, low_re);
+    const float rate = use_vorticity ? vorticity[i] : strain[i];
+    float mt = r * ke / kmax(w / alpha, rate * f2[i] / 0.31f);
+    mt = kmin(mt, max_ratio * m);
+    mut[i] = mt;
+    const float blend = f1[i];
+    gamma_k[i] = m + mt * sst_inv_sigma_k(blend);
+    gamma_w[i] = m + mt * sst_inv_sigma_w(blend);
+  }
+};
+// :90-103
+struct SstBlendingViscosity {
+  __device__ void operator()(int i, CF rho, CF mu, CF k, CF omega, CF distance, CF kx, CF ky, CF kz,
+                             CF ox, CF oy, CF oz, CF strain, CF vorticity, WF f1, WF f2, WF cdkw,
+                             WF mut, WF gamma_k, WF gamma_w, float cd_floor, float k_floor,
+                             float omega_floor, int low_re, int use_vorticity,
+                             float max_ratio) const {
+    SstBlending{}(i, rho, mu, k, omega, distance, kx, ky, kz, ox, oy, oz, f1, f2, cdkw, cd_floor,
+                  k_floor, omega_floor);
+    SstViscosity{}(i, rho, mu, k, omega, strain, vorticity, f1, f2, mut, gamma_k, gamma_w, low_re,
+                   use_vorticity, max_ratio, k_floor, omega_floor);
+  }
+};
+// :105-151
+struct SstWall {
+  __device__ void operator()(int i, CI owner, CI wall, int n_internal, CF ax, CF ay, CF az, CF ux,
+                             CF uy, CF uz, CF wall_ux, CF wall_uy, CF wall_uz, CF dy, CF rho,
+                             CF mu, CF k, CF omega, WF tau, WF shear_x, WF shear_y, WF shear_z,
+                             WF u_star, WF u_tau, WF y_plus, WF y_plus_utau, WF production,
+                             WF shear_coefficient, float k_floor, float omega_floor) const {
+    float t_tau = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, t_us = 0.0f, t_ut = 0.0f, t_yp = 0.0f;
+    float t_ypu = 0.0f, t_prod = 0.0f, t_coef = 0.0f;
+    if (wall[i]) {
+      const int f = n_internal + i, c = owner[f];
+      const float r = rho[c], m = mu[c], d = dy[i], ke = kmax(k[c], k_floor);
+      const float w = kmax(omega[c], omega_floor);
+      const float fx = ax[f], fy = ay[f], fz = az[f];
+      const float inv = 1.0f / sqrtf(fx * fx + fy * fy + fz * fz);
+      const float nx = inv * fx, ny = inv * fy, nz = inv * fz;
+      const float rx = ux[c] - wall_ux[i], ry = uy[c] - wall_uy[i], rz = uz[c] - wall_uz[i];
+      const float along = rx * nx + ry * ny + rz * nz;
+      const float tx = rx - along * nx, ty = ry - along * ny, tz = rz - along * nz;
+      const float speed = sqrtf(tx * tx + ty * ty + tz * tz);
+      const float us = sqrtf((m / r) * speed / d + 0.3f * ke);
+      const float yp = (r / m) * us * d;
+      const float log_up = logf(9.793f * kmax(yp, 0.2f)) / 0.4187f;
+      const float ratio = kmin(yp, log_up) / kmax(yp, log_up);
+      const float blend = powf(1.0f + ratio * ratio * ratio * ratio, 0.25f);
+      const float coefficient = (yp <= log_up ? m / d : r * us / log_up) * blend;
+      const float stress = coefficient * speed;
+      const float ut = sqrtf(stress / r);
+      t_coef = coefficient; t_tau = stress; t_us = us; t_ut = ut; t_yp = yp;
+      t_ypu = (r / m) * ut * d;
+      if (speed > 0.0f) {
+        sx = (-coefficient) * tx; sy = (-coefficient) * ty; sz = (-coefficient) * tz;
+      }
+      const float p_lam = (r * ke / w) * (stress / m) * (stress / m);
+      const float p_log = yp > 0.0f ? (stress * stress / m) / (0.4187f * yp) : 0.0f;
+      const float hi = kmax(p_lam, p_log), lo = kmin(p_lam, p_log);
+      t_prod = hi > 0.0f ? lo / (1.0f + lo / hi) : 0.0f;
+    }
+    tau[i] = t_tau; shear_x[i] = sx; shear_y[i] = sy; shear_z[i] = sz;
+    u_star[i] = t_us; u_tau[i] = t_ut; y_plus[i] = t_yp; y_plus_utau[i] = t_ypu;
+    production[i] = t_prod; shear_coefficient[i] = t_coef;
+  }
+};
+// :153-169
+struct SstWallOmega {
+  __device__ void operator()(int i, CI owner, CI wall, int n_internal, CF dy, CF rho, CF mu,
+                             CF u_star, WF fixed_omega, WF face_omega) const {
+    float fixed = 0.0f, face = 0.0f;
+    if (wall[i]) {
+      const int c = owner[n_internal + i];
+      const float m = mu[c], r = rho[c], d = dy[i];
+      const float w_lam = 2.0f * (m / r) / (0.072f * d * d);
+      const float w_log = u_star[i] / (0.3f * 0.4187f * d);
+      fixed = sst_power_blend(w_lam, w_log, 1.3f);
+      face = sst_power_blend(3.0f * w_lam, 2.0f * w_log, 1.3f);
+    }
+    fixed_omega[i] = fixed;
+    face_omega[i] = face;
+  }
+};
+// :171-190
+struct SstWallCells {
+  __device__ void operator()(int i, CI offsets, CI faces, CI wall, CF dy, CF face_omega,
+                             CF face_production, int n_internal, WI fixed_mask, WI selected_face,
+                             WF fixed_omega, WF production) const {
+    int selected = -1;
+    float best = 0.0f;
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+      const int b = faces[j] - n_internal;
+      if (b >= 0 && wall[b]) {
+        const float d = dy[b];
+        if (selected < 0 || d < best || (d == best && b < selected)) {
+          selected = b;
+          best = d;
+        }
+      }
+    }
+    selected_face[i] = selected;
+    fixed_mask[i] = selected >= 0 ? 1 : 0;
+    fixed_omega[i] = selected >= 0 ? face_omega[selected] : 0.0f;
+    production[i] = selected >= 0 ? face_production[selected] : 0.0f;
+  }
+};
+// :192-236
+struct SstSources {
+  __device__ void operator()(int i, CF rho, CF mu, CF k, CF omega, CF strain, CF vorticity, CF f1,
+                             CF cdkw, CF mut, CI wall_mask, CF wall_pk, WF k_source, WF k_sink,
+                             WF omega_source, WF omega_sink, WF production_k, int low_re,
+                             int gamma_2003, int production_mode, int limiter, int limit_omega,
+                             int wall_override, float clip, int limit_cross, float cross_clip,
+                             float k_floor, float omega_floor, float omega_molecular_ratio) const {
+    const float r = rho[i], m = mu[i], ke = kmax(k[i], k_floor), w = kmax(omega[i], omega_floor);
+    const float ret = (r / m) * (ke / w);
+    const float beta_star = sst_beta_star(ret, low_re);
+    const float s = strain[i], v = vorticity[i];
+    const float rate2 = production_mode == 1 ? s * v : (production_mode == 2 ? v * v : s * s);
+    const float mt = mut[i], blend = f1[i];
+    float raw_pk = mt * rate2;
+    if (wall_override && wall_mask[i]) raw_pk = wall_pk[i];
+    const float cap = clip * r * beta_star * ke * w;
+    const float pk = limiter ? kmin(raw_pk, cap) : raw_pk;
+    const float pw_k = limiter && limit_omega ? pk : raw_pk;
+    float gamma = sst_gamma(blend, gamma_2003);
+    if (low_re) gamma *= 1.0f - (8.0f / 9.0f) / (1.0f + ret / 2.95f);
+    float pw = mt > 0.0f ? gamma * r * (pw_k / mt) : gamma * r * rate2;
+    if (omega_molecular_ratio > 0.0f) pw = gamma * r * (pw_k / (mt + omega_molecular_ratio * m));
+    float cross = (1.0f - blend) * cdkw[i];
+    if (limit_cross) {
+      const float bound = (cross_clip * r * sst_beta(blend) * w) * w;
+      cross = kmax(-bound, kmin(cross, bound));
+    }
+    k_source[i] = pk;
+    k_sink[i] = r * beta_star * w;
+    const float destruction = r * sst_beta(blend) * w;
+    omega_source[i] = pw + kmax(cross, 0.0f) + destruction * w;
+    omega_sink[i] = 2.0f * destruction + sst_finite_ratio(kmax(-cross, 0.0f), w);
+    production_k[i] = pk;
+  }
+};
+// :238-253
+struct SstFaceDiffusion {
+  __device__ void operator()(int i, CI owner, CI neighbour, CF weight, CI wall, CF mu,
+                             CF gamma_k, CF gamma_w, int n_internal, WF face_k,
+                             WF face_w) const {
+    const int c = owner[i];
+    if (i < n_internal) {
+      const int n = neighbour[i];
+      const float a = weight[i], b = 1.0f - a;
+      face_k[i] = a * gamma_k[c] + b * gamma_k[n];
+      face_w[i] = a * gamma_w[c] + b * gamma_w[n];
+    } else {
+      const bool on_wall = wall[i - n_internal] != 0;
+      face_k[i] = on_wall ? 0.0f : gamma_k[c];
+      face_w[i] = on_wall ? mu[c] : gamma_w[c];
+    }
+  }
+};
+// :255-259
+struct SstSustain {
+  __device__ void operator()(int i, CF rho, CF f1, WF k_source, WF omega_source, float ambient_k,
+                             float ambient_omega) const {
+    const float r = rho[i];
+    k_source[i] += 0.09f * r * ambient_omega * ambient_k;
+    omega_source[i] += sst_beta(f1[i]) * r * ambient_omega * ambient_omega;
+  }
+};
+
 template <class Body, class... P>
 __global__ void __launch_bounds__(kBlock) k_each(int n, P... p) {
   const int i = cell_index();
@@ -1609,6 +1842,16 @@ const std::vector<KernelEntry>& fv_kernels() {
       FV(KH_FV_MOMENTUM_DIAGONAL, "fv_momentum_diagonal", FvMomentumDiagonal),
       FV(KH_FV_SST_FORCE, "fv_sst_force", FvSstForce),
       FV(KH_FV_TURBULENCE_BOUNDARY, "fv_turbulence_boundary", FvTurbulenceBoundary),
+      FV(KH_SST_RATES, "sst_rates", SstRates),
+      FV(KH_SST_BLENDING, "sst_blending", SstBlending),
+      FV(KH_SST_VISCOSITY, "sst_viscosity", SstViscosity),
+      FV(KH_SST_BLENDING_VISCOSITY, "sst_blending_viscosity", SstBlendingViscosity),
+      FV(KH_SST_WALL, "sst_wall", SstWall),
+      FV(KH_SST_WALL_OMEGA, "sst_wall_omega", SstWallOmega),
+      FV(KH_SST_WALL_CELLS, "sst_wall_cells", SstWallCells),
+      FV(KH_SST_SOURCES, "sst_sources", SstSources),
+      FV(KH_SST_FACE_DIFFUSION, "sst_face_diffusion", SstFaceDiffusion),
+      FV(KH_SST_SUSTAIN, "sst_sustain", SstSustain),
 #undef FV
   };
   return table;

@@ -1,10 +1,11 @@
-"""Input specifications of the finite-volume kernels for the parity tests.
+"""Parameter roles of the finite-volume kernels, and synthetic inputs built
+from them, for the parity tests and the per-kernel benchmark.
 
 Each kernel is listed with its iteration domain and one role per parameter, in
-the reference's parameter order (src/kutta/kernels/fv_*.kern.h). The harness
-builds every argument from a real box mesh (connectivity) and seeded random
-values (everything else), so the oracle, the reference body and the device
-kernel all see identical bytes.
+the reference's parameter order (src/kutta/kernels/fv_*.kern.h, sst.kern.h).
+build_args() makes every argument from a real box mesh (connectivity) and
+seeded random values (everything else), so the oracle, the reference body and
+the device kernel all see identical bytes.
 
 Roles:
   owner neighbour kind offsets faces sign   mesh connectivity / boundary kinds 0-4
@@ -24,7 +25,9 @@ Roles:
   entry rows cols diag   the pressure stencil's cell-face entry map and CSR arrays
   ref          a pinned cell (int scalar): the middle cell
   bmask spec   int per boundary face: 0 or 1 / a turbulence specification 0-2
-  component    int scalar: the velocity component, seed % 3
+  component    int scalar: the velocity component (or another 0-2 mode), seed % 3
+  Cu           float per cell, uniform in [0, 1] (blending factors)
+  oCi          int output per cell
   N oN         float per matrix nonzero (input / output)
   mask         int 0 or 1 per cell
   flag flag2   int switches: seed % 2 and (seed // 2) % 2, so consecutive seeds
@@ -128,11 +131,33 @@ SPECS = {
     "fv_momentum_diagonal": ("cells", "C C C C C C oC oC"),
     "fv_sst_force": ("boundary", "B01 F F F B B B oB oB oB internal"),
     "fv_turbulence_boundary": ("boundary", "owner kind spec B+ B+ F B B B C+ C+ B+ oB oB oBi internal f+ f+"),
+    # kernels/sst.kern.h
+    "sst_rates": ("cells", "C C C C C C C C C oC oC"),
+    "sst_blending": ("cells", "C+ C+ C+ C+ C+ C C C C C C oC oC oC f+ f+ f+"),
+    "sst_viscosity": ("cells", "C+ C+ C+ C+ C+ C+ Cu Cu oC oC oC flag flag2 f+ f+ f+"),
+    "sst_blending_viscosity": ("cells", "C+ C+ C+ C+ C+ C C C C C C C+ C+ oC oC oC oC oC oC f+ f+ f+ "
+                                        "flag flag2 f+"),
+    "sst_wall": ("boundary", "owner bmask internal F F F C C C B B B B+ C+ C+ C+ C+ "
+                             "oB oB oB oB oB oB oB oB oB oB f+ f+"),
+    "sst_wall_omega": ("boundary", "owner bmask internal B+ C+ C+ B+ oB oB"),
+    "sst_wall_cells": ("cells", "offsets faces bmask B+ B B internal oCi oCi oC oC"),
+    "sst_sources": ("cells", "C+ C+ C+ C+ C+ C+ Cu C C+ mask C+ oC oC oC oC oC flag flag2 component "
+                             "flag flag2 flag f+ flag2 f+ f+ f+ f"),
+    "sst_face_diffusion": ("faces", "owner neighbour F bmask C+ C+ C+ internal oF oF"),
+    "sst_sustain": ("cells", "C+ Cu oC oC f+ f+"),
 }
+
+# Kernels whose bodies call transcendental functions (tanh, log, pow): the
+# device's math library and the host's round differently in the last bits,
+# so the device comparison for these uses a relative tolerance.
+TRANSCENDENTAL = {"sst_blending", "sst_blending_viscosity", "sst_wall", "sst_wall_omega"}
 
 
 class FvMesh:
-    """Connectivity of a renumbered, perturbed box mesh from the oracle."""
+    """Connectivity of a renumbered, perturbed box mesh: any mapping with the
+    arrays owner, neighbour, cell_face_offsets, cell_faces, cell_face_sign,
+    adjacent, matrix_entry, row_ptr, diag and col (kutta_amd.MeshSetup or a
+    pressure-problem bundle)."""
 
     def __init__(self, bundle):
         self.owner = np.ascontiguousarray(bundle["owner"], np.int32)
@@ -197,6 +222,11 @@ def build_args(mesh: FvMesh, spec: str, seed: int):
             args.append(rng.permutation(mesh.cells)[:mesh.subset].astype(np.int32))
         elif role in ("F", "C", "B", "E", "C6", "S6", "N"):
             args.append(rng.standard_normal(size[role]).astype(np.float32))
+        elif role == "Cu":
+            args.append(rng.uniform(0.0, 1.0, mesh.cells).astype(np.float32))
+        elif role == "oCi":
+            outputs.append(len(args))
+            args.append(np.full(mesh.cells, -7, np.int32))
         elif role in ("F+", "C+", "B+"):
             args.append(rng.uniform(0.5, 2.0, size[role[0]]).astype(np.float32))
         elif role == "B01":

@@ -1377,6 +1377,212 @@ void fv_turbulence_boundary(int i, I owner, I kind, I spec, F a, F b, F mass, F 
   bo[i] = type == 0 ? wall_omega[i] : dissipation;
 }
 
+// -------------------------------------------------------------- sst.kern.h
+// Coefficient blends and limiters, :5-38. Transcendentals are the C++
+// library's float overloads, as in the reference's CPU build (prelude.h:59-64).
+inline float sst_inv_sigma_k(float f1) { return f1 / 1.176f + (1.0f - f1); }
+inline float sst_inv_sigma_w(float f1) { return f1 / 2.0f + (1.0f - f1) / 1.168f; }
+inline float sst_beta(float f1) { return 0.075f * f1 + 0.0828f * (1.0f - f1); }
+inline float sst_gamma(float f1, int set_2003) {
+  const float g1 = set_2003 ? 5.0f / 9.0f : 0.075f / 0.09f - 0.41f * 0.41f / (2.0f * 0.3f);
+  const float g2 = set_2003 ? 0.44f : 0.0828f / 0.09f - 0.41f * 0.41f / (1.168f * 0.3f);
+  return f1 * g1 + (1.0f - f1) * g2;
+}
+inline float sst_alpha_star(float ret, int low_re) { return low_re ? 1.0f - 0.976f / (1.0f + ret / 6.0f) : 1.0f; }
+inline float sst_beta_star(float ret, int low_re) {
+  if (!low_re) return 0.09f;
+  const float r = kmin(ret / 8.0f, 100.0f), r4 = r * r * r * r;
+  return 0.09f * ((4.0f / 15.0f) + r4) / (1.0f + r4);
+}
+inline float sst_power_blend(float a, float b, float p) {
+  const float hi = kmax(a, b), lo = kmin(a, b);
+  return hi == 0.0f ? 0.0f : hi * std::pow(1.0f + std::pow(lo / hi, p), 1.0f / p);
+}
+inline float sst_finite_ratio(float numerator, float denominator) {
+  return kmax(-1e38f, kmin(numerator / denominator, 1e38f));
+}
+
+// :40-50
+void sst_rates(int i, F ux, F uy, F uz, F vx, F vy, F vz, F wx, F wy, F wz, W strain, W vorticity) {
+  float xy = uy[i] + vx[i], xz = uz[i] + wx[i], yz = vz[i] + wy[i];
+  strain[i] = ksqrt(2.0f * (ux[i] * ux[i] + vy[i] * vy[i] + wz[i] * wz[i]) + xy * xy + xz * xz + yz * yz);
+  xy = uy[i] - vx[i];
+  xz = uz[i] - wx[i];
+  yz = vz[i] - wy[i];
+  vorticity[i] = ksqrt(xy * xy + xz * xz + yz * yz);
+}
+
+// :52-70
+void sst_blending(int i, F rho, F mu, F k, F omega, F distance, F kx, F ky, F kz, F ox, F oy, F oz,
+                  W f1, W f2, W cdkw, float cd_floor, float k_floor, float omega_floor) {
+  const float ke = kmax(k[i], k_floor), w = kmax(omega[i], omega_floor), y = distance[i];
+  const float cd = sst_finite_ratio(2.0f * rho[i] * (kx[i] * ox[i] + ky[i] * oy[i] + kz[i] * oz[i]), 1.168f * w);
+  const float a = ksqrt(ke) / (0.09f * w * y);
+  const float b = 500.0f * (mu[i] / rho[i]) / (y * y * w);
+  const float c = 4.0f * rho[i] * ke / (1.168f * kmax(cd, cd_floor) * y * y);
+  const float p1 = kmin(kmin(kmax(a, b), c), 10.0f);
+  const float p2 = kmin(kmax(2.0f * a, b), 10.0f);
+  f1[i] = std::tanh(p1 * p1 * p1 * p1);
+  f2[i] = std::tanh(p2 * p2);
+  cdkw[i] = cd;
+}
+
+// :72-88
+void sst_viscosity(int i, F rho, F mu, F k, F omega, F strain, F vorticity, F f1, F f2, W mut,
+                   W gamma_k, W gamma_w, int low_re, int use_vorticity, float max_ratio,
+                   float k_floor, float omega_floor) {
+  const float ke = kmax(k[i], k_floor), w = kmax(omega[i], omega_floor);
+  const float ret = (rho[i] / mu[i]) * (ke / w);
+  const float alpha = sst_alpha_star(ret, low_re);
+  const float rate = use_vorticity ? vorticity[i] : strain[i];
+  float mt = rho[i] * ke / kmax(w / alpha, rate * f2[i] / 0.31f);
+  mt = kmin(mt, max_ratio * mu[i]);
+  mut[i] = mt;
+  gamma_k[i] = mu[i] + mt * sst_inv_sigma_k(f1[i]);
+  gamma_w[i] = mu[i] + mt * sst_inv_sigma_w(f1[i]);
+}
+
+// :90-103
+void sst_blending_viscosity(int i, F rho, F mu, F k, F omega, F distance, F kx, F ky, F kz, F ox,
+                            F oy, F oz, F strain, F vorticity, W f1, W f2, W cdkw, W mut,
+                            W gamma_k, W gamma_w, float cd_floor, float k_floor, float omega_floor,
+                            int low_re, int use_vorticity, float max_ratio) {
+  sst_blending(i, rho, mu, k, omega, distance, kx, ky, kz, ox, oy, oz, f1, f2, cdkw, cd_floor, k_floor,
+               omega_floor);
+  sst_viscosity(i, rho, mu, k, omega, strain, vorticity, f1, f2, mut, gamma_k, gamma_w, low_re,
+                use_vorticity, max_ratio, k_floor, omega_floor);
+}
+
+// :105-151. Blended wall law on wall face i (boundary index).
+void sst_wall(int i, I owner, I wall, int n_internal, F ax, F ay, F az, F ux, F uy, F uz, F wall_ux,
+              F wall_uy, F wall_uz, F dy, F rho, F mu, F k, F omega, W tau, W shear_x, W shear_y,
+              W shear_z, W u_star, W u_tau, W y_plus, W y_plus_utau, W production,
+              W shear_coefficient, float k_floor, float omega_floor) {
+  tau[i] = shear_x[i] = shear_y[i] = shear_z[i] = 0.0f;
+  u_star[i] = u_tau[i] = y_plus[i] = y_plus_utau[i] = 0.0f;
+  production[i] = 0.0f;
+  shear_coefficient[i] = 0.0f;
+  if (!wall[i]) return;
+  const int f = n_internal + i, c = owner[f];
+  const float r = rho[c], m = mu[c], d = dy[i], ke = kmax(k[c], k_floor);
+  const float w = kmax(omega[c], omega_floor);
+  const float length = ksqrt(ax[f] * ax[f] + ay[f] * ay[f] + az[f] * az[f]);
+  const float inv = 1.0f / length;
+  const float nx = inv * ax[f], ny = inv * ay[f], nz = inv * az[f];
+  const float rx = ux[c] - wall_ux[i], ry = uy[c] - wall_uy[i], rz = uz[c] - wall_uz[i];
+  const float along = rx * nx + ry * ny + rz * nz;
+  const float tx = rx - along * nx, ty = ry - along * ny, tz = rz - along * nz;
+  const float speed = ksqrt(tx * tx + ty * ty + tz * tz);
+  const float us = ksqrt((m / r) * speed / d + 0.3f * ke);
+  const float yp = (r / m) * us * d;
+  const float log_up = std::log(9.793f * kmax(yp, 0.2f)) / 0.4187f;
+  const float ratio = kmin(yp, log_up) / kmax(yp, log_up);
+  const float blend = std::pow(1.0f + ratio * ratio * ratio * ratio, 0.25f);
+  const float coefficient = (yp <= log_up ? m / d : r * us / log_up) * blend;
+  const float stress = coefficient * speed;
+  shear_coefficient[i] = coefficient;
+  const float ut = ksqrt(stress / r);
+  tau[i] = stress;
+  u_star[i] = us;
+  u_tau[i] = ut;
+  y_plus[i] = yp;
+  y_plus_utau[i] = (r / m) * ut * d;
+  if (speed > 0.0f) {
+    shear_x[i] = (-coefficient) * tx;
+    shear_y[i] = (-coefficient) * ty;
+    shear_z[i] = (-coefficient) * tz;
+  }
+  const float p_lam = (r * ke / w) * (stress / m) * (stress / m);
+  const float p_log = yp > 0.0f ? (stress * stress / m) / (0.4187f * yp) : 0.0f;
+  const float hi = kmax(p_lam, p_log), lo = kmin(p_lam, p_log);
+  production[i] = hi > 0.0f ? lo / (1.0f + lo / hi) : 0.0f;
+}
+
+// :153-169
+void sst_wall_omega(int i, I owner, I wall, int n_internal, F dy, F rho, F mu, F u_star,
+                    W fixed_omega, W face_omega) {
+  fixed_omega[i] = 0.0f;
+  face_omega[i] = 0.0f;
+  if (!wall[i]) return;
+  const int c = owner[n_internal + i];
+  const float m = mu[c], r = rho[c], d = dy[i], us = u_star[i];
+  const float w_lam = 2.0f * (m / r) / (0.072f * d * d);
+  const float w_log = us / (0.3f * 0.4187f * d);
+  fixed_omega[i] = sst_power_blend(w_lam, w_log, 1.3f);
+  face_omega[i] = sst_power_blend(3.0f * w_lam, 2.0f * w_log, 1.3f);
+}
+
+// :171-190
+void sst_wall_cells(int i, I offsets, I faces, I wall, F dy, F face_omega, F face_production,
+                    int n_internal, WI fixed_mask, WI selected_face, W fixed_omega, W production) {
+  int selected = -1;
+  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+    const int b = faces[j] - n_internal;
+    if (b >= 0 && wall[b] &&
+        (selected < 0 || dy[b] < dy[selected] || (dy[b] == dy[selected] && b < selected)))
+      selected = b;
+  }
+  selected_face[i] = selected;
+  fixed_mask[i] = selected >= 0 ? 1 : 0;
+  fixed_omega[i] = selected >= 0 ? face_omega[selected] : 0.0f;
+  production[i] = selected >= 0 ? face_production[selected] : 0.0f;
+}
+
+// :192-236
+void sst_sources(int i, F rho, F mu, F k, F omega, F strain, F vorticity, F f1, F cdkw, F mut,
+                 I wall_mask, F wall_pk, W k_source, W k_sink, W omega_source, W omega_sink,
+                 W production_k, int low_re, int gamma_2003, int production_mode, int limiter,
+                 int limit_omega, int wall_override, float clip, int limit_cross, float cross_clip,
+                 float k_floor, float omega_floor, float omega_molecular_ratio) {
+  const float r = rho[i], ke = kmax(k[i], k_floor), w = kmax(omega[i], omega_floor);
+  const float ret = (r / mu[i]) * (ke / w);
+  const float beta_star = sst_beta_star(ret, low_re);
+  const float rate2 = production_mode == 1 ? strain[i] * vorticity[i]
+                      : (production_mode == 2 ? vorticity[i] * vorticity[i] : strain[i] * strain[i]);
+  float raw_pk = mut[i] * rate2;
+  if (wall_override && wall_mask[i]) raw_pk = wall_pk[i];
+  const float cap = clip * r * beta_star * ke * w;
+  const float pk = limiter ? kmin(raw_pk, cap) : raw_pk;
+  const float pw_k = limiter && limit_omega ? pk : raw_pk;
+  float gamma = sst_gamma(f1[i], gamma_2003);
+  if (low_re) gamma *= 1.0f - (8.0f / 9.0f) / (1.0f + ret / 2.95f);
+  float pw = mut[i] > 0.0f ? gamma * r * (pw_k / mut[i]) : gamma * r * rate2;
+  if (omega_molecular_ratio > 0.0f) pw = gamma * r * (pw_k / (mut[i] + omega_molecular_ratio * mu[i]));
+  float cross = (1.0f - f1[i]) * cdkw[i];
+  if (limit_cross) {
+    const float bound = (cross_clip * r * sst_beta(f1[i]) * w) * w;
+    cross = kmax(-bound, kmin(cross, bound));
+  }
+  k_source[i] = pk;
+  k_sink[i] = r * beta_star * w;
+  const float destruction = r * sst_beta(f1[i]) * w;
+  omega_source[i] = pw + kmax(cross, 0.0f) + destruction * w;
+  omega_sink[i] = 2.0f * destruction + sst_finite_ratio(kmax(-cross, 0.0f), w);
+  production_k[i] = pk;
+}
+
+// :238-253
+void sst_face_diffusion(int i, I owner, I neighbour, F weight, I wall, F mu, F gamma_k, F gamma_w,
+                        int n_internal, W face_k, W face_w) {
+  const int c = owner[i];
+  if (i < n_internal) {
+    const int n = neighbour[i];
+    const float a = weight[i];
+    face_k[i] = a * gamma_k[c] + (1.0f - a) * gamma_k[n];
+    face_w[i] = a * gamma_w[c] + (1.0f - a) * gamma_w[n];
+  } else {
+    const bool on_wall = wall[i - n_internal] != 0;
+    face_k[i] = on_wall ? 0.0f : gamma_k[c];
+    face_w[i] = on_wall ? mu[c] : gamma_w[c];
+  }
+}
+
+// :255-259
+void sst_sustain(int i, F rho, F f1, W k_source, W omega_source, float ambient_k, float ambient_omega) {
+  k_source[i] += 0.09f * rho[i] * ambient_omega * ambient_k;
+  omega_source[i] += sst_beta(f1[i]) * rho[i] * ambient_omega * ambient_omega;
+}
+
 // ---------------------------------------------------------------- dispatch
 template <class T>
 T from_arg(void* const* a, std::size_t k) {
@@ -1447,6 +1653,9 @@ const std::map<std::string, Entry>& table() {
       KERNEL(fv_backflow_pressure), KERNEL(fv_effective_viscosity), KERNEL(fv_transpose_stress),
       KERNEL(fv_stress_source), KERNEL(fv_boundary_momentum), KERNEL(fv_momentum_diagonal),
       KERNEL(fv_sst_force), KERNEL(fv_turbulence_boundary),
+      KERNEL(sst_rates), KERNEL(sst_blending), KERNEL(sst_viscosity), KERNEL(sst_blending_viscosity),
+      KERNEL(sst_wall), KERNEL(sst_wall_omega), KERNEL(sst_wall_cells), KERNEL(sst_sources),
+      KERNEL(sst_face_diffusion), KERNEL(sst_sustain),
   };
   return t;
 }

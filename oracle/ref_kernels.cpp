@@ -30,6 +30,7 @@ namespace reff {
 #include "kutta/kernels/fv_transport.kern.h"
 #include "kutta/kernels/fv_paired.kern.h"
 #include "kutta/kernels/fv_rans.kern.h"
+#include "kutta/kernels/sst.kern.h"
 }  // namespace reff
 
 #define LOOP(name, ...) for (int i = 0; i < n; ++i) refk::name(i, __VA_ARGS__)
@@ -88,6 +89,9 @@ const std::map<std::string, Runner>& fv_table() {
       REF_FV(fv_backflow_pressure), REF_FV(fv_effective_viscosity), REF_FV(fv_transpose_stress),
       REF_FV(fv_stress_source), REF_FV(fv_boundary_momentum), REF_FV(fv_momentum_diagonal),
       REF_FV(fv_sst_force), REF_FV(fv_turbulence_boundary),
+      REF_FV(sst_rates), REF_FV(sst_blending), REF_FV(sst_viscosity), REF_FV(sst_blending_viscosity),
+      REF_FV(sst_wall), REF_FV(sst_wall_omega), REF_FV(sst_wall_cells), REF_FV(sst_sources),
+      REF_FV(sst_face_diffusion), REF_FV(sst_sustain),
   };
   return t;
 }

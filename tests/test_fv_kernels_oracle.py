@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fv_specs import SPECS, FvMesh, build_args, same_bits, signature_of
+from kutta_amd.fv_specs import SPECS, FvMesh, build_args, same_bits, signature_of
 
 needs_ref = pytest.mark.skipif(oracle.ref_kernels() is None,
                                reason="oracle/_ref not built (reference tree absent)")

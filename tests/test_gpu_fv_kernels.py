@@ -1,13 +1,26 @@
-"""Each finite-volume kernel of the HIP backend (kernels/fv_*.kern.h bodies on
-gfx950) against the oracle's restatement on the same seeded inputs over real
-box meshes, up to the 998,250-cell benchmark mesh: bitwise equal outputs (two
-NaNs count as equal; their bits depend on the hardware)."""
+"""Each finite-volume kernel of the HIP backend (kernels/fv_*.kern.h and
+sst.kern.h bodies on gfx950) against the oracle's restatement on the same
+seeded inputs over real box meshes, up to the 998,250-cell benchmark mesh:
+bitwise equal outputs (two NaNs count as equal; their bits depend on the
+hardware), except within a stated relative tolerance for the kernels that call
+tanh, log or pow."""
 import numpy as np
 import pytest
 
 import kutta_amd
 import oracle
-from fv_specs import SPECS, FvMesh, build_args, same_bits
+from kutta_amd.fv_specs import SPECS, TRANSCENDENTAL, FvMesh, build_args, same_bits
+
+# Relative tolerance for kernels that call tanh/log/pow: the device library's
+# results differ from the host's by a few ulps (about 1e-7 relative) and pass
+# through a few products and quotients.
+TRANSCENDENTAL_RTOL = 1e-5
+
+
+def close(got, want):
+    if got.dtype != np.float32:
+        return bool(np.array_equal(got, want))
+    return bool(np.allclose(got, want, rtol=TRANSCENDENTAL_RTOL, atol=0.0, equal_nan=True))
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +46,8 @@ def test_fv_kernel_bitwise_vs_oracle(device, mesh, name):
         oracle.fv_each(name, n, *expected)
         for k in outputs:
             got = on_device[k].download()
-            if not same_bits(got, expected[k]):
+            ok = close(got, expected[k]) if name in TRANSCENDENTAL else same_bits(got, expected[k])
+            if not ok:
                 bad = np.flatnonzero(got.view(np.uint32) != expected[k].view(np.uint32))
                 raise AssertionError(f"{name} seed {seed} output {k}: {bad.size} differ, first at "
                                      f"{bad[0]}: {got[bad[0]]!r} vs {expected[k][bad[0]]!r}")
