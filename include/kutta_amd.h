@@ -111,6 +111,34 @@ int ka_solver_host_waits(void* solver);
  * iterates are those of the per-level launches either way. */
 int ka_solver_set_tail(void* solver, int first, int32_t* level);
 
+/* Steady laminar SIMPLEC (fv/steady_solver.hpp:286-360: SteadySolver
+ * constructor, iterate, monitor, report, field access) on a box mesh
+ * make_box_hex(nx, ny, nz) on [0,1]^3 (io/generators.cpp:98-103), closed
+ * domain. kinds[6]/wall[18]: one condition per patch in the generator's order
+ * xmin, xmax, ymin, ymax, zmin, zmax; kind 0 Wall{velocity wall[3p..3p+2]},
+ * 3 Symmetry. settings[11]: density, viscosity, momentum_relaxation,
+ * pressure_relaxation, pseudo_time_scale, monitor_interval, paired_state,
+ * paired_state_slack_ulps, reference_point x, y, z (SolverSettings,
+ * fv/steady_solver.hpp:193-271; the members not listed keep their defaults). */
+int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, const double* wall,
+                      const double* settings, void** out);
+void ka_simplec_destroy(void* solver);
+/* SteadySolver::iterate(count): count SIMPLEC outer iterations. */
+int ka_simplec_iterate(void* solver, int count);
+/* The IterationReport (fv/steady_solver.hpp:276-284), after running monitor()
+ * first when run_monitor != 0. out[27]: iteration; u, v, w, continuity as
+ * (raw, scaled, fluent_scaled, fluent_normalization); corrected mass
+ * imbalance; maximum velocity; pseudo time step; momentum iterations x3;
+ * pressure iterations; pressure final residual; pressure converged. */
+int ka_simplec_report(void* solver, int run_monitor, double* out);
+/* Downloads device field `name` (u v w p mass u_lo v_lo w_lo p_lo mass_lo ...)
+ * into out when non-null; returns its length, or -1 if unknown. */
+long ka_simplec_field(void* solver, const char* name, float* out);
+/* info[4]: cells, pinned reference cell, linear-solver host waits so far,
+ * iterations. centroids: 3 doubles per cell of the renumbered mesh. */
+int ka_simplec_info(void* solver, int32_t* info);
+int ka_simplec_centroids(void* solver, double* out);
+
 #ifdef __cplusplus
 }
 #endif

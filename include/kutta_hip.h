@@ -217,6 +217,13 @@ typedef enum kh_kernel {
   KH_SST_SOURCES,                    /* :192-236 */
   KH_SST_FACE_DIFFUSION,             /* :238-253 */
   KH_SST_SUSTAIN,                    /* :255-259 */
+  /* kernels/fv_fluent_schemes.kern.h */
+  KH_FV_SIMPLE_INVERSE,              /* :10-12 */
+  KH_FV_STANDARD_PRESSURE_FACES,     /* :28-39 */
+  KH_FV_CELL_PRESSURE_BOUNDARY,      /* :47-50 */
+  KH_FV_FORCE_TO_SHEAR,              /* :54-59 */
+  KH_FV_POOR_CELL_PRESSURE_FACES,    /* :70-84 */
+  KH_FV_POOR_PRESSURE_INTERFACES,    /* :93-97 */
 
   /* Fused MI355X kernels (no reference counterpart). */
   KH_FUSED_SPMV_DOT_ALPHA = 200, /* csr_spmv_guarded + dot_into(p,q,2) + solver_pcg_alpha:

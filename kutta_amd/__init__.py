@@ -118,6 +118,15 @@ def host_lib() -> C.CDLL:
         lib.ka_coarsening_values.argtypes = [vp, C.POINTER(C.c_double)]
         lib.ka_coarsening_values.restype = C.c_long
         lib.ka_coarsening_destroy.argtypes = [vp]
+        lib.ka_simplec_create.argtypes = [vp, i32, i32, i32, pi32, C.POINTER(dbl), C.POINTER(dbl),
+                                          C.POINTER(vp)]
+        lib.ka_simplec_destroy.argtypes = [vp]
+        lib.ka_simplec_iterate.argtypes = [vp, i32]
+        lib.ka_simplec_report.argtypes = [vp, i32, C.POINTER(dbl)]
+        lib.ka_simplec_field.argtypes = [vp, C.c_char_p, vp]
+        lib.ka_simplec_field.restype = C.c_long
+        lib.ka_simplec_info.argtypes = [vp, pi32]
+        lib.ka_simplec_centroids.argtypes = [vp, vp]
         _host = lib
     return _host
 
@@ -277,6 +286,12 @@ class Kernel(enum.IntEnum):
     sst_sources = 182
     sst_face_diffusion = 183
     sst_sustain = 184
+    fv_simple_inverse = 185
+    fv_standard_pressure_faces = 186
+    fv_cell_pressure_boundary = 187
+    fv_force_to_shear = 188
+    fv_poor_cell_pressure_faces = 189
+    fv_poor_pressure_interfaces = 190
     fused_spmv_dot_alpha = 200
     fused_step_dot_check = 201
     fused_residual_restrict = 202
@@ -795,3 +810,126 @@ def make_solver(kind: SolverKind, device: Device, a: CsrMatrixHost,
                 structure: OperatorStructure = OperatorStructure.General) -> LinearSolver:
     """linalg/solver.hpp:290-293; AMG kinds use default AmgOptions."""
     return LinearSolver(device, kind, a, AmgOptions(nullspace=nullspace, structure=structure))
+
+
+# ---------------------------------------------------------------- SIMPLEC
+PATCHES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+
+
+@dataclass
+class Wall:
+    """fv/steady_solver.hpp:23-27 (a moving or stationary no-slip wall)."""
+    velocity: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class Symmetry:
+    """fv/steady_solver.hpp:65."""
+
+
+@dataclass
+class SolverSettings:
+    """fv/steady_solver.hpp:193-271: the members the MI355X SIMPLEC honours
+    (the others keep the reference defaults)."""
+    density: float = 1.0
+    viscosity: float = 0.01
+    momentum_relaxation: float = 0.7
+    pressure_relaxation: float = 1.0
+    pseudo_time_scale: float = 1.0
+    monitor_interval: int = 20
+    paired_state: bool = True
+    paired_state_slack_ulps: float = 1.0
+    reference_point: tuple = (0.0, 0.0, 0.0)
+
+    def packed(self):
+        return (C.c_double * 11)(self.density, self.viscosity, self.momentum_relaxation,
+                                 self.pressure_relaxation, self.pseudo_time_scale, self.monitor_interval,
+                                 1.0 if self.paired_state else 0.0, self.paired_state_slack_ulps,
+                                 *self.reference_point)
+
+
+def pack_conditions(conditions: dict):
+    """One condition per box patch in generator order (xmin ... zmax)."""
+    kinds, wall = (C.c_int32 * 6)(), (C.c_double * 18)()
+    for p, name in enumerate(PATCHES):
+        c = conditions[name]
+        if isinstance(c, Wall):
+            kinds[p] = 0
+            wall[3 * p:3 * p + 3] = [float(v) for v in c.velocity]
+        elif isinstance(c, Symmetry):
+            kinds[p] = 3
+        else:
+            raise KuttaError(f"SIMPLEC: unsupported condition for {name}")
+    return kinds, wall
+
+
+def cavity_conditions(lid: float = 1.0) -> dict:
+    """validation/cases/cavity.hpp:113-118: lid (ymax) moving in x, walls at
+    rest, symmetry front and back (zmin, zmax)."""
+    c = {name: Wall() for name in PATCHES}
+    c["ymax"] = Wall((lid, 0.0, 0.0))
+    c["zmin"] = c["zmax"] = Symmetry()
+    return c
+
+
+REPORT_FIELDS = ("iteration", "u_raw", "u_scaled", "u_fluent_scaled", "u_fluent_normalization",
+                 "v_raw", "v_scaled", "v_fluent_scaled", "v_fluent_normalization",
+                 "w_raw", "w_scaled", "w_fluent_scaled", "w_fluent_normalization",
+                 "continuity_raw", "continuity_scaled", "continuity_fluent_scaled",
+                 "continuity_fluent_normalization", "corrected_mass_imbalance", "maximum_velocity",
+                 "pseudo_time_step", "momentum_u_iterations", "momentum_v_iterations",
+                 "momentum_w_iterations", "pressure_iterations", "pressure_final_residual",
+                 "pressure_converged")
+
+
+class SteadySolver:
+    """Steady laminar SIMPLEC on the MI355X (fv/steady_solver.hpp:286-360) over
+    make_box_hex(nx, ny, nz) on [0,1]^3 with the given patch conditions."""
+
+    def __init__(self, device: Device, nx: int, ny: int, nz: int, conditions: dict,
+                 settings: SolverSettings | None = None):
+        s = settings or SolverSettings()
+        kinds, wall = pack_conditions(conditions)
+        h = C.c_void_p()
+        _check_host(host_lib().ka_simplec_create(device.handle, nx, ny, nz, kinds, wall, s.packed(),
+                                                 C.byref(h)))
+        self.device, self.handle, self.settings = device, h.value, s
+
+    def close(self) -> None:
+        if self.handle:
+            host_lib().ka_simplec_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def iterate(self, count: int = 1) -> None:
+        _check_host(host_lib().ka_simplec_iterate(self.handle, count))
+
+    def report(self, monitor: bool = False) -> dict:
+        out = (C.c_double * len(REPORT_FIELDS))()
+        _check_host(host_lib().ka_simplec_report(self.handle, 1 if monitor else 0, out))
+        return dict(zip(REPORT_FIELDS, list(out)))
+
+    def field(self, name: str) -> np.ndarray:
+        n = host_lib().ka_simplec_field(self.handle, name.encode(), None)
+        if n < 0:
+            raise KuttaError(f"SIMPLEC: unknown field {name}")
+        out = np.empty(n, np.float32)
+        if host_lib().ka_simplec_field(self.handle, name.encode(), out.ctypes.data) < 0:
+            raise KuttaError(host_lib().ka_last_error().decode())
+        return out
+
+    def info(self) -> dict:
+        out = (C.c_int32 * 4)()
+        _check_host(host_lib().ka_simplec_info(self.handle, out))
+        return {"cells": out[0], "reference": out[1], "host_waits": out[2], "iteration": out[3]}
+
+    def centroids(self) -> np.ndarray:
+        n = self.info()["cells"]
+        out = np.empty((n, 3), np.float64)
+        _check_host(host_lib().ka_simplec_centroids(self.handle, out.ctypes.data))
+        return out

@@ -19,7 +19,8 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["kh_runtime.hip", "kh_linalg.hip", "kh_fv.hip"]
-HOST_SOURCES = ["amg_setup.cpp", "krylov.cpp", "mesh.cpp", "pressure.cpp", "capi.cpp", "capi_mesh.cpp"]
+HOST_SOURCES = ["amg_setup.cpp", "krylov.cpp", "mesh.cpp", "pressure.cpp", "simplec.cpp", "capi.cpp",
+                "capi_mesh.cpp", "capi_simplec.cpp"]
 
 # Device arithmetic contract: no contraction of a*b+c, IEEE division and sqrt,
 # denormals kept, packed FP32 instructions disabled.

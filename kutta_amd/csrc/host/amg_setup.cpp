@@ -10,26 +10,22 @@
 
 namespace kutta_amd {
 
-namespace {
-
-// Upper-bound float sums (kernels/prelude.h:82-100).
 float add_up(float a, float b) {
   float s = a + b;
   float part = s - a;
   float err = (a - (s - part)) + (b - part);
   return err > 0.0f ? std::nextafter(s, std::numeric_limits<float>::infinity()) : s;
 }
-struct UpperSum {
-  float sum, error;
-  void add(float b) {
-    float s = sum + b;
-    float part = s - sum;
-    float err = (sum - (s - part)) + (b - part);
-    sum = s;
-    error = add_up(error, err);
-  }
-  float value() const { return add_up(sum, error); }
-};
+void UpperSum::add(float b) {
+  float s = sum + b;
+  float part = s - sum;
+  float err = (sum - (s - part)) + (b - part);
+  sum = s;
+  error = add_up(error, err);
+}
+float UpperSum::value() const { return add_up(sum, error); }
+
+namespace {
 
 // Incoming edges of every row, in increasing source order, with the position
 // of the stored entry (source -> row).

@@ -19,6 +19,15 @@ struct CsrHost {
 };
 void validate_host(const CsrHost& a);
 
+// Upper-bound float sums (kernels/prelude.h:82-100): add_up rounds a + b up
+// when the rounding dropped a positive error; UpperSum carries the error term.
+float add_up(float a, float b);
+struct UpperSum {
+  float sum = 0, error = 0;
+  void add(float b);
+  float value() const;
+};
+
 enum class NullSpace : std::uint8_t { None, Constant };
 enum class Structure : std::uint8_t { General, SymmetricMMatrix };
 

@@ -1,0 +1,102 @@
+// C ABI of the steady SIMPLEC solver (include/kutta_amd.h).
+#include <cstring>
+#include <memory>
+
+#include "capi_internal.hpp"
+#include "kutta_amd.h"
+#include "simplec.hpp"
+
+using namespace kutta_amd;
+using capi::guard;
+
+namespace {
+struct Simplec {
+  HipDevice* dev;
+  std::unique_ptr<SteadySolver> solver;
+};
+Simplec& handle(void* h) { return *static_cast<Simplec*>(h); }
+}  // namespace
+
+extern "C" {
+
+int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, const double* wall,
+                      const double* settings, void** out) {
+  return guard([&] {
+    KA_REQUIRE(nx > 0 && ny > 0 && nz > 0, "box dimensions must be positive");
+    auto mesh = make_box(nx, ny, nz, {0, 0, 0}, {1, 1, 1}, false);
+    std::vector<PatchCondition> conditions;
+    for (std::size_t p = 0; p < mesh.patches.size(); ++p)
+      conditions.push_back({mesh.patches[p].name, kinds[p], {wall[3 * p], wall[3 * p + 1], wall[3 * p + 2]}});
+    SolverSettings s;
+    s.density = settings[0];
+    s.viscosity = settings[1];
+    s.momentum_relaxation = settings[2];
+    s.pressure_relaxation = settings[3];
+    s.pseudo_time_scale = settings[4];
+    s.monitor_interval = int(settings[5]);
+    s.paired_state = settings[6] != 0;
+    s.paired_state_slack_ulps = settings[7];
+    s.reference_point = {settings[8], settings[9], settings[10]};
+    auto* h = new Simplec{static_cast<HipDevice*>(dev), nullptr};
+    try {
+      h->solver = std::make_unique<SteadySolver>(*h->dev, std::move(mesh), std::move(conditions), s);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+void ka_simplec_destroy(void* solver) { delete static_cast<Simplec*>(solver); }
+int ka_simplec_iterate(void* solver, int count) {
+  return guard([&] { handle(solver).solver->iterate(count); });
+}
+int ka_simplec_report(void* solver, int run_monitor, double* out) {
+  return guard([&] {
+    auto& s = *handle(solver).solver;
+    const IterationReport r = run_monitor ? s.monitor() : s.report();
+    int k = 0;
+    out[k++] = r.iteration;
+    for (const auto* v : {&r.u, &r.v, &r.w, &r.continuity}) {
+      out[k++] = v->raw;
+      out[k++] = v->scaled;
+      out[k++] = v->fluent_scaled;
+      out[k++] = v->fluent_normalization;
+    }
+    out[k++] = r.corrected_mass_imbalance;
+    out[k++] = r.maximum_velocity;
+    out[k++] = r.pseudo_time_step;
+    for (const auto& m : r.momentum) out[k++] = m.iterations;
+    out[k++] = r.pressure.iterations;
+    out[k++] = r.pressure.final_residual;
+    out[k++] = r.pressure.converged;
+  });
+}
+long ka_simplec_field(void* solver, const char* name, float* out) {
+  std::size_t size = 0;
+  const float* p = nullptr;
+  const int rc = guard([&] { p = handle(solver).solver->field(name, &size); });
+  if (rc != 0 || !p) return -1;
+  if (out && guard([&] { ok(kh_download(handle(solver).dev->raw(), p, out, size * sizeof(float))); }) != 0) return -1;
+  return long(size);
+}
+int ka_simplec_info(void* solver, int32_t* info) {
+  return guard([&] {
+    auto& s = *handle(solver).solver;
+    info[0] = s.cells();
+    info[1] = s.reference_cell();
+    info[2] = s.host_waits();
+    info[3] = s.report().iteration;
+  });
+}
+int ka_simplec_centroids(void* solver, double* out) {
+  return guard([&] {
+    const auto& g = handle(solver).solver->geometry();
+    for (std::size_t c = 0; c < g.cell_centroid.size(); ++c) {
+      out[3 * c] = g.cell_centroid[c].x;
+      out[3 * c + 1] = g.cell_centroid[c].y;
+      out[3 * c + 2] = g.cell_centroid[c].z;
+    }
+  });
+}
+}

@@ -1,0 +1,838 @@
+// Steady SIMPLEC on the MI355X backend (fv/steady_solver.cpp:203-597 setup,
+// :1303-1369 step, :1417-1477 monitor), laminar closed-domain subset. The
+// order of launches and host reductions is the reference's, so every field
+// equals the serial CPU backend's bit for bit; the layout is the device's.
+#include "simplec.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <limits>
+#include <map>
+#include <numeric>
+
+namespace kutta_amd {
+namespace {
+
+using F = DeviceArray<float>;
+using I = DeviceArray<Index>;
+
+struct Vec3F {
+  F x, y, z;
+  Vec3F() = default;
+  Vec3F(HipDevice& dev, std::size_t n) : x(dev, n), y(dev, n), z(dev, n) {}
+  F& operator[](int k) { return k == 0 ? x : (k == 1 ? y : z); }
+  const F& operator[](int k) const { return k == 0 ? x : (k == 1 ? y : z); }
+};
+
+double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Vec3 sub(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+Vec3 scale(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double length_of(Vec3 a) { return std::sqrt(dot(a, a)); }
+
+// fv/operators.hpp:52 BoundaryObservation.
+enum class Observation { Omitted, FaceValue, NormalFootValue };
+
+// One cell's least-squares fit (fv/operators.cpp:97-137): weighted rows
+// d/|d|^2 appended in cell-face order (zero for omitted boundary faces) and
+// the inverse of the normal matrix, eigen-decomposed by cyclic Jacobi so that
+// rank-deficient directions get unit eigenvalues.
+void cell_fit(const PolyMesh& m, const MeshGeometry& g, const CellFaces& cf,
+              const std::vector<Observation>& obs, Index c, std::vector<Vec3>& delta, double inv[3][3]) {
+  double a[3][3]{}, q[3][3]{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const Index ni = m.n_internal_faces();
+  for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+    const Index f = cf.faces[j];
+    const bool internal = f < ni;
+    if (!internal && obs[f - ni] == Observation::Omitted) {
+      delta.push_back({});
+      continue;
+    }
+    const Index other = internal ? (m.owner[f] == c ? m.neighbour[f] : m.owner[f]) : -1;
+    Vec3 d = sub(internal ? g.cell_centroid[other] : g.face_centroid[f], g.cell_centroid[c]);
+    if (!internal && obs[f - ni] == Observation::NormalFootValue) {
+      const Vec3 s = g.face_area[f];
+      d = scale(s, dot(d, s) / dot(s, s));
+    }
+    const double w = 1 / dot(d, d), x[]{d.x, d.y, d.z};
+    delta.push_back(scale(d, w));
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) a[r][k] += w * x[r] * x[k];
+  }
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    int p = 0, r = 1;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i + 1; j < 3; ++j)
+        if (std::abs(a[i][j]) > std::abs(a[p][r])) p = i, r = j;
+    if (std::abs(a[p][r]) < 1e-14) break;
+    const double angle = .5 * std::atan2(2 * a[p][r], a[r][r] - a[p][p]);
+    const double cs = std::cos(angle), sn = std::sin(angle);
+    const double app = a[p][p], arr = a[r][r], apr = a[p][r];
+    a[p][p] = cs * cs * app - 2 * cs * sn * apr + sn * sn * arr;
+    a[r][r] = sn * sn * app + 2 * cs * sn * apr + cs * cs * arr;
+    a[p][r] = a[r][p] = 0;
+    for (int i = 0; i < 3; ++i) {
+      if (i != p && i != r) {
+        const double x = a[i][p], y = a[i][r];
+        a[i][p] = a[p][i] = cs * x - sn * y;
+        a[i][r] = a[r][i] = sn * x + cs * y;
+      }
+      const double x = q[i][p], y = q[i][r];
+      q[i][p] = cs * x - sn * y;
+      q[i][r] = sn * x + cs * y;
+    }
+  }
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 3; ++j) inv[r][j] = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double e = a[k][k] > 1e-10 ? 1 / a[k][k] : 1;
+    for (int r = 0; r < 3; ++r)
+      for (int j = 0; j < 3; ++j) inv[r][j] += q[r][k] * e * q[j][k];
+  }
+}
+
+// The LSQ gradient's boundary-face rows (one per boundary face, FP32) and the
+// six-plane FP32 inverse; interior rows are the stencil's (fv/operators.cpp:166-211).
+struct LsqFit {
+  F bx, by, bz, inverse;
+};
+LsqFit least_squares(HipDevice& dev, const PolyMesh& m, const MeshGeometry& g, const CellFaces& cf,
+                     const std::vector<Observation>& obs) {
+  const std::size_t n = m.n_cells, ni = m.n_internal_faces(), nb = m.n_faces() - ni;
+  std::vector<Vec3> delta;
+  delta.reserve(cf.faces.size());
+  std::vector<float> inverse(6 * n);
+  for (Index c = 0; c < m.n_cells; ++c) {
+    double inv[3][3];
+    cell_fit(m, g, cf, obs, c, delta, inv);
+    const double v[]{inv[0][0], inv[0][1], inv[0][2], inv[1][1], inv[1][2], inv[2][2]};
+    for (int k = 0; k < 6; ++k) inverse[c + k * n] = float(v[k]);
+  }
+  std::vector<float> bx(nb, 0.0f), by(nb, 0.0f), bz(nb, 0.0f);
+  for (std::size_t j = 0; j < cf.faces.size(); ++j)
+    if (std::size_t(cf.faces[j]) >= ni) {
+      const auto b = cf.faces[j] - ni;
+      bx[b] = float(delta[j].x);
+      by[b] = float(delta[j].y);
+      bz[b] = float(delta[j].z);
+    }
+  return {to_device(dev, bx), to_device(dev, by), to_device(dev, bz), to_device(dev, inverse)};
+}
+
+// fv/operators.cpp:139-164: how far a fit's extrapolation to each boundary
+// face centroid can amplify its cell's neighbour differences.
+std::vector<double> extrapolation_gain(const PolyMesh& m, const MeshGeometry& g, const CellFaces& cf,
+                                       const std::vector<Observation>& obs) {
+  const Index ni = m.n_internal_faces();
+  std::vector<double> gain(m.n_faces() - ni, 0.0);
+  std::vector<Vec3> delta;
+  for (Index c = 0; c < m.n_cells; ++c) {
+    double inv[3][3];
+    delta.clear();
+    cell_fit(m, g, cf, obs, c, delta, inv);
+    for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+      const Index f = cf.faces[j];
+      if (f < ni) continue;
+      const Vec3 r = sub(g.face_centroid[f], g.cell_centroid[c]);
+      const Vec3 w{r.x * inv[0][0] + r.y * inv[1][0] + r.z * inv[2][0],
+                   r.x * inv[0][1] + r.y * inv[1][1] + r.z * inv[2][1],
+                   r.x * inv[0][2] + r.y * inv[1][2] + r.z * inv[2][2]};
+      for (const auto& d : delta) gain[f - ni] += std::abs(dot(w, d));
+    }
+  }
+  return gain;
+}
+
+// mesh/mesh.cpp:172-187
+std::vector<double> orthogonal_quality(const PolyMesh& m, const MeshGeometry& g) {
+  std::vector<double> q(m.n_cells, 1);
+  const auto cosine = [](Vec3 a, Vec3 b) { return dot(a, b) / (length_of(a) * length_of(b)); };
+  for (Index f = 0; f < m.n_faces(); ++f) {
+    const Index o = m.owner[f];
+    q[o] = std::min(q[o], cosine(g.face_area[f], sub(g.face_centroid[f], g.cell_centroid[o])));
+    if (f >= m.n_internal_faces()) continue;
+    const Index nb = m.neighbour[f];
+    const Vec3 link = sub(g.cell_centroid[nb], g.cell_centroid[o]);
+    q[o] = std::min(q[o], cosine(g.face_area[f], link));
+    q[nb] = std::min({q[nb], cosine(g.face_area[f], sub(g.cell_centroid[nb], g.face_centroid[f])),
+                      cosine(g.face_area[f], link)});
+  }
+  for (auto& v : q) v = std::max(v, 0.);
+  return q;
+}
+
+void upload3(HipDevice& dev, Vec3F& out, std::size_t n, const std::function<Vec3(std::size_t)>& at) {
+  std::vector<float> x(n), y(n), z(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    const Vec3 v = at(i);
+    x[i] = float(v.x);
+    y[i] = float(v.y);
+    z[i] = float(v.z);
+  }
+  out.x = to_device(dev, x);
+  out.y = to_device(dev, y);
+  out.z = to_device(dev, z);
+}
+
+}  // namespace
+
+// fv/transport.hpp:50-106: one momentum component's equation. The operator's
+// values and the face-correction scratch are shared by all three (and by the
+// pressure equation), as in the reference.
+struct Equation {
+  F rhs, diagonal, consistent_inverse, limiter;
+  Vec3F grad;
+  std::unique_ptr<KrylovSolver> solver;
+};
+
+struct SteadySolver::Impl {
+  HipDevice& dev;
+  SolverSettings settings;
+  PolyMesh host;
+  MeshGeometry geom;
+  CellFaces cf;
+  Stencil stencil;
+  Index n = 0, nf = 0, ni = 0, nb = 0;
+  Index reference = -1;
+  bool paired = false;
+
+  // fv/device_mesh.hpp:36-67 (wall distance is a turbulence-model input).
+  I owner, neighbour, offsets, faces;
+  F sign, weight, volume, wall_normal_distance;
+  Vec3F area, owner_to_face, neighbour_to_face, owner_to_neighbour;
+  // fv/operators.hpp:16-25
+  I adjacent, matrix_entry;
+  F lsq_x, lsq_y, lsq_z, diffusion, nonorth_x, nonorth_y, nonorth_z;
+  LsqFit velocity_lsq, pressure_lsq;
+  OwnedCsr matrix;
+  std::unique_ptr<KrylovSolver> pressure_solver;
+  Equation momentum[3];
+  F face_correction, zero;
+
+  Vec3F velocity, face_velocity, body, source, boundary_velocity, prescribed_velocity;
+  Vec3F limited_gradient[3], grad_p, grad_correction, absolute_residual, fluent_scale;
+  F pressure, mass, viscosity, boundary_pressure, prescribed_pressure;
+  I boundary_kind, scalar_kind, total_pressure;
+  F pressure_rhs, correction, correction_boundary, coefficient, face_pressure, pressure_jump;
+  F residual_u, residual_v, residual_w, continuity, provisional_continuity, scale_cells, speed;
+  F wall_diagonal, wall_rhs, physical_diagonal, consistent_inverse;
+  Vec3F velocity_lo, pressure_gradient_lo;
+  F pressure_lo, mass_lo, previous_mass, pressure_boundary_lo, imbalance, momentum_delta;
+
+  float limiter_epsilon2[3]{-1, -1, -1};
+  double paired_speed = 0, paired_pressure_magnitude = 0;
+  double length = 0, bc_speed = 0, pressure_speed = 0, continuity_scale = 0;
+  double first_five_continuity[5]{};
+  float time_step = 0;
+  double velocity_scale = 1;
+  IterationReport report;
+
+  Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition> conditions, SolverSettings s);
+
+  F zeros(std::size_t count) {
+    F a(dev, count);
+    dev.fill(a.data(), 0.0f, count);
+    return a;
+  }
+  I zeros_int(std::size_t count) {
+    I a(dev, count);
+    dev.fill_int(a.data(), 0, count);
+    return a;
+  }
+  Vec3F zeros3(std::size_t count) {
+    Vec3F v(dev, count);
+    for (int k = 0; k < 3; ++k) dev.fill(v[k].data(), 0.0f, count);
+    return v;
+  }
+  void zero3(Vec3F& v, std::size_t count) {
+    for (int k = 0; k < 3; ++k) dev.fill(v[k].data(), 0.0f, count);
+  }
+  double max_abs(const F& x, std::size_t count) { return dev.reduce_many({HipDevice::max_abs(x.data(), count)})[0]; }
+
+  // :843-851 (segregated solvers: the maximum cell speed).
+  void update_time(double maximum_speed) {
+    velocity_scale = std::max({.1 * bc_speed, maximum_speed, pressure_speed, 1e-10});
+    time_step = float(.3 * length * settings.pseudo_time_scale / velocity_scale);
+  }
+  // :857-873
+  void refresh_time_step() {
+    dev.launch(KH_FV_SPEED, n, {velocity.x, velocity.y, velocity.z, speed});
+    update_time(max_abs(speed, n));
+  }
+  // :874-885
+  void update_boundaries() {
+    dev.launch(KH_FV_VELOCITY_BOUNDARY, nb,
+               {owner, boundary_kind, area.x, area.y, area.z, velocity.x, velocity.y, velocity.z,
+                prescribed_velocity.x, prescribed_velocity.y, prescribed_velocity.z, mass, boundary_velocity.x,
+                boundary_velocity.y, boundary_velocity.z, scalar_kind, ni, float(settings.density)});
+  }
+  // :599-606
+  void apply_backflow_pressure() {
+    dev.launch(KH_FV_BACKFLOW_PRESSURE, nb,
+               {owner, total_pressure, mass, velocity.x, velocity.y, velocity.z, boundary_pressure, ni,
+                float(settings.density)});
+  }
+  // fv/operators.cpp:296-302
+  void gradient(const LsqFit& fit, const F& cell, const F& bc, Vec3F& out) {
+    dev.launch(KH_FV_GRADIENT, n,
+               {offsets, adjacent, lsq_x, lsq_y, lsq_z, fit.bx, fit.by, fit.bz, fit.inverse, cell, bc, out.x,
+                out.y, out.z, n});
+  }
+  void pressure_boundary(const F& p, const Vec3F& g, const F& prescribed, F& out) {
+    dev.launch(KH_FV_PRESSURE_BOUNDARY, nb,
+               {owner, boundary_kind, owner_to_face.x, owner_to_face.y, owner_to_face.z, p, g.x, g.y, g.z,
+                prescribed, out, ni});
+  }
+  // :886-889 with fv/operators.cpp:311-318
+  void reconstruct_velocity() {
+    const auto& fit = velocity_lsq;
+    auto &g0 = momentum[0].grad, &g1 = momentum[1].grad, &g2 = momentum[2].grad;
+    dev.launch(KH_FV_VECTOR_GRADIENT, n,
+               {offsets, adjacent, lsq_x, lsq_y, lsq_z, fit.bx, fit.by, fit.bz, fit.inverse, velocity.x,
+                velocity.y, velocity.z, boundary_velocity.x, boundary_velocity.y, boundary_velocity.z, g0.x,
+                g0.y, g0.z, g1.x, g1.y, g1.z, g2.x, g2.y, g2.z, n});
+  }
+  // :896-925 (second-order pressure faces, no poor cells).
+  void update_pressure() {
+    pressure_boundary(pressure, grad_p, prescribed_pressure, boundary_pressure);
+    dev.launch(KH_FV_CELL_PRESSURE_BOUNDARY, nb, {owner, boundary_kind, pressure, boundary_pressure, ni});
+    apply_backflow_pressure();
+    gradient(pressure_lsq, pressure, boundary_pressure, grad_p);
+    pressure_boundary(pressure, grad_p, prescribed_pressure, boundary_pressure);
+    apply_backflow_pressure();
+    dev.launch(KH_FV_PRESSURE_FACES, nf,
+               {owner, neighbour, owner_to_face.x, owner_to_face.y, owner_to_face.z, neighbour_to_face.x,
+                neighbour_to_face.y, neighbour_to_face.z, pressure, grad_p.x, grad_p.y, grad_p.z,
+                boundary_pressure, face_pressure, pressure_jump, ni});
+  }
+  // :955-961
+  struct Slack {
+    float velocity, pressure, flux_per_area;
+  };
+  Slack paired_slack() const {
+    const double ulps = settings.paired_state_slack_ulps * double(std::numeric_limits<float>::epsilon());
+    const double s = std::max(bc_speed, paired_speed);
+    const double p = std::max(settings.density * s * s, paired_pressure_magnitude);
+    return {float(ulps * s), float(ulps * p), float(ulps * settings.density * s)};
+  }
+  // :1139-1248 (Barth-Jespersen limiter, momentum-weighted faces).
+  void update_flux() {
+    if (paired) dev.copy(previous_mass.data(), mass.data(), std::size_t(nf) * 4);
+    reconstruct_velocity();
+    for (int k = 0; k < 3; ++k) {
+      auto& eq = momentum[k];
+      auto& g = limited_gradient[k];
+      if (paired)
+        dev.launch(KH_FV_PAIRED_LIMITER_SCALE, n,
+                   {offsets, adjacent, faces, sign, owner_to_face.x, owner_to_face.y, owner_to_face.z,
+                    neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z, velocity[k], velocity_lo[k],
+                    boundary_velocity[k], scalar_kind, eq.grad.x, eq.grad.y, eq.grad.z, eq.limiter, g.x, g.y,
+                    g.z, limiter_epsilon2[k]});
+      else
+        dev.launch(KH_FV_LIMITER_SCALE, n,
+                   {offsets, adjacent, faces, sign, owner_to_face.x, owner_to_face.y, owner_to_face.z,
+                    neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z, velocity[k],
+                    boundary_velocity[k], eq.grad.x, eq.grad.y, eq.grad.z, eq.limiter, g.x, g.y, g.z,
+                    limiter_epsilon2[k]});
+    }
+    const auto &gu = limited_gradient[0], &gv = limited_gradient[1], &gw = limited_gradient[2];
+    dev.launch(KH_FV_MOMENTUM_FACES, nf,
+               {owner, neighbour, boundary_kind, owner_to_face.x, owner_to_face.y, owner_to_face.z,
+                neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z, velocity.x, velocity.y,
+                velocity.z, gu.x, gu.y, gu.z, gv.x, gv.y, gv.z, gw.x, gw.y, gw.z, physical_diagonal,
+                boundary_velocity.x, boundary_velocity.y, boundary_velocity.z, face_velocity.x, face_velocity.y,
+                face_velocity.z, ni});
+    dev.launch(KH_FV_RHIE_CHOW, nf,
+               {owner, neighbour, boundary_kind, area.x, area.y, area.z, diffusion, volume, physical_diagonal,
+                consistent_inverse, face_velocity.x, face_velocity.y, face_velocity.z, pressure_jump, mass,
+                coefficient, ni, float(settings.density)});
+    if (paired)
+      dev.launch(KH_FV_PAIRED_FLUX, nf,
+                 {owner, neighbour, boundary_kind, area.x, area.y, area.z, owner_to_face.x, owner_to_face.y,
+                  owner_to_face.z, neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z,
+                  physical_diagonal, physical_diagonal, volume, diffusion, velocity_lo.x, velocity_lo.y,
+                  velocity_lo.z, pressure_lo, pressure_gradient_lo.x, pressure_gradient_lo.y,
+                  pressure_gradient_lo.z, previous_mass, mass, mass_lo, ni, float(settings.density),
+                  paired_slack().flux_per_area});
+  }
+  // :765-776 (laminar: the coefficient argument is an unused placeholder).
+  void wall_terms(int k) {
+    dev.launch(KH_FV_BOUNDARY_MOMENTUM, n,
+               {offsets, faces, boundary_kind, area.x, area.y, area.z, prescribed_pressure, viscosity, diffusion,
+                velocity.x, velocity.y, velocity.z, boundary_velocity.x, boundary_velocity.y,
+                boundary_velocity.z, wall_diagonal, wall_rhs, ni, k, 0});
+  }
+  // fv/transport.cpp:83-121 with the paired limiter of with_low (:963-976).
+  void assemble(int k) {
+    auto& eq = momentum[k];
+    const F& field = velocity[k];
+    const F& bc = boundary_velocity[k];
+    if (paired)
+      dev.launch(KH_FV_PAIRED_LIMITER, n,
+                 {offsets, adjacent, faces, sign, owner_to_face.x, owner_to_face.y, owner_to_face.z,
+                  neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z, field, velocity_lo[k], bc,
+                  scalar_kind, eq.grad.x, eq.grad.y, eq.grad.z, eq.limiter, limiter_epsilon2[k]});
+    else
+      dev.launch(KH_FV_LIMITER, n,
+                 {offsets, adjacent, faces, sign, owner_to_face.x, owner_to_face.y, owner_to_face.z,
+                  neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z, field, bc, eq.grad.x,
+                  eq.grad.y, eq.grad.z, eq.limiter, limiter_epsilon2[k]});
+    dev.launch(KH_FV_TRANSPORT_FACES, nf,
+               {owner, neighbour, weight, owner_to_face.x, owner_to_face.y, owner_to_face.z, neighbour_to_face.x,
+                neighbour_to_face.y, neighbour_to_face.z, nonorth_x, nonorth_y, nonorth_z, eq.grad.x, eq.grad.y,
+                eq.grad.z, eq.limiter, mass, viscosity, scalar_kind, face_correction, ni, 1, 0});
+    dev.launch(KH_FV_TRANSPORT_ASSEMBLE, n,
+               {offsets, faces, sign, adjacent, matrix_entry, matrix.row_ptr, matrix.diag, diffusion, mass,
+                viscosity, scalar_kind, bc, face_correction, source[k], volume, field, zero, wall_diagonal,
+                wall_rhs, matrix.val, eq.rhs, zero, eq.diagonal, zero, eq.consistent_inverse, ni,
+                float(settings.density), time_step, float(settings.momentum_relaxation), 0});
+  }
+  // :1045-1060 (paired momentum without compensated assembly).
+  void paired_momentum_residual(int k, F& r) {
+    auto& eq = momentum[k];
+    dev.launch(KH_FV_PAIRED_MOMENTUM_RESIDUAL, n,
+               {matrix.row_ptr, matrix.col, matrix.val, matrix.diag, eq.diagonal, zero, velocity[k],
+                velocity_lo[k], pressure_gradient_lo[k], volume, eq.rhs, zero, r});
+  }
+  SolveStats solve_momentum(int k, F& field) {
+    auto& eq = momentum[k];
+    eq.solver->refresh(matrix.view());
+    return eq.solver->solve(matrix.view(), field.data(), eq.rhs.data(), settings.momentum_linear);
+  }
+  // :607-614
+  void update_momentum_diagonal() {
+    dev.launch(KH_FV_MOMENTUM_DIAGONAL, n,
+               {momentum[0].diagonal, momentum[1].diagonal, momentum[2].diagonal, momentum[0].consistent_inverse,
+                momentum[1].consistent_inverse, momentum[2].consistent_inverse, physical_diagonal,
+                consistent_inverse});
+  }
+  // :1026-1034
+  void update_pressure_gradient_lo() {
+    zero3(pressure_gradient_lo, n);
+    pressure_boundary(pressure_lo, pressure_gradient_lo, correction_boundary, pressure_boundary_lo);
+    gradient(pressure_lsq, pressure_lo, pressure_boundary_lo, pressure_gradient_lo);
+  }
+  // :943-949
+  void refresh_paired_scales() {
+    if (!paired) return;
+    dev.launch(KH_FV_SPEED, n, {velocity.x, velocity.y, velocity.z, speed});
+    const auto m = dev.reduce_many({HipDevice::max_abs(speed.data(), n), HipDevice::max_abs(pressure.data(), n)});
+    paired_speed = m[0];
+    paired_pressure_magnitude = m[1];
+  }
+  void momentum_source() {
+    dev.launch(KH_FV_MOMENTUM_SOURCE, n,
+               {offsets, faces, sign, area.x, area.y, area.z, face_pressure, volume, body.x, body.y, body.z,
+                source.x, source.y, source.z});
+  }
+  void step();
+  IterationReport monitor();
+};
+
+SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition> conditions, SolverSettings s)
+    : dev(d), settings(s), host(std::move(mesh)) {
+  KA_REQUIRE(s.density > 0 && std::isfinite(s.density) && s.viscosity > 0 && std::isfinite(s.viscosity) &&
+                 s.momentum_relaxation > 0 && s.momentum_relaxation <= 1 && s.pressure_relaxation > 0 &&
+                 s.pressure_relaxation <= 1 && s.pseudo_time_scale > 0 && s.monitor_interval > 0 &&
+                 std::isfinite(s.paired_state_slack_ulps) && s.paired_state_slack_ulps > 0,
+             "FV invalid steady solver settings");
+  std::map<std::string, PatchCondition> by_name;
+  for (auto& c : conditions) {
+    KA_REQUIRE(c.kind == PatchCondition::Wall || c.kind == PatchCondition::Symmetry,
+               "SIMPLEC: only wall and symmetry patches are supported (closed domains)");
+    KA_REQUIRE(std::isfinite(c.velocity.x) && std::isfinite(c.velocity.y) && std::isfinite(c.velocity.z),
+               "wall: nonfinite velocity");
+    KA_REQUIRE(by_name.emplace(c.patch, c).second, "FV duplicate patch condition");
+  }
+  KA_REQUIRE(by_name.size() == host.patches.size(), "FV supply exactly one condition per patch");
+  std::vector<PatchCondition> patch_condition;
+  for (const auto& p : host.patches) {
+    const auto it = by_name.find(p.name);
+    KA_REQUIRE(it != by_name.end(), "FV missing boundary: " + p.name);
+    patch_condition.push_back(it->second);
+  }
+  renumber_cells_rcm(host);
+  geom = compute_geometry(host);
+  KA_REQUIRE(host.n_cells > 0, "FV steady solver needs a nonempty mesh");
+  reference = locate_cell(host, geom, s.reference_point);
+  {
+    const auto cn = build_cell_neighbours(host);
+    std::vector<char> seen(host.n_cells, 0);
+    std::vector<Index> stack{0};
+    seen[0] = 1;
+    for (std::size_t k = 0; k < stack.size(); ++k)
+      for (Index j = cn.offsets[stack[k]]; j < cn.offsets[stack[k] + 1]; ++j)
+        if (!seen[cn.cells[j]]) seen[cn.cells[j]] = 1, stack.push_back(cn.cells[j]);
+    KA_REQUIRE(stack.size() == std::size_t(host.n_cells), "FV disconnected domains require separate solvers");
+  }
+  validate(host, geom);
+  n = host.n_cells;
+  nf = host.n_faces();
+  ni = host.n_internal_faces();
+  nb = nf - ni;
+  cf = build_cell_faces(host);
+  stencil = make_stencil(host, geom);
+
+  // upload_mesh (fv/device_mesh.cpp:208-256).
+  owner = to_device(dev, host.owner);
+  neighbour = to_device(dev, host.neighbour);
+  upload3(dev, area, nf, [&](std::size_t f) { return geom.face_area[f]; });
+  const auto rp = [&](std::size_t f) { return sub(geom.face_centroid[f], geom.cell_centroid[host.owner[f]]); };
+  const auto rn = [&](std::size_t f) { return sub(geom.face_centroid[f], geom.cell_centroid[host.neighbour[f]]); };
+  {
+    std::vector<float> w(ni);
+    for (Index f = 0; f < ni; ++f) {
+      const double on = dot(rp(f), geom.face_area[f]), nn = -dot(rn(f), geom.face_area[f]);
+      w[f] = float(nn / (on + nn));
+    }
+    weight = to_device(dev, w);
+  }
+  upload3(dev, owner_to_face, nf, rp);
+  upload3(dev, neighbour_to_face, ni, rn);
+  upload3(dev, owner_to_neighbour, nf, [&](std::size_t f) {
+    return f < std::size_t(ni) ? sub(geom.cell_centroid[host.neighbour[f]], geom.cell_centroid[host.owner[f]])
+                               : rp(f);
+  });
+  offsets = to_device(dev, cf.offsets);
+  faces = to_device(dev, cf.faces);
+  sign = to_device(dev, std::vector<float>(cf.signs.begin(), cf.signs.end()));
+  {
+    std::vector<float> v(n);
+    for (Index c = 0; c < n; ++c) {
+      KA_REQUIRE(std::isfinite(geom.cell_volume[c]) && geom.cell_volume[c] > 0,
+                 "FV requires finite positive cell volumes");
+      v[c] = float(geom.cell_volume[c]);
+    }
+    volume = to_device(dev, v);
+    std::vector<float> wd(nb, 0.0f);
+    for (std::size_t p = 0; p < host.patches.size(); ++p)
+      if (patch_condition[p].kind == PatchCondition::Wall)
+        for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f)
+          wd[f - ni] = float(std::abs(dot(rp(f), geom.face_area[f])) / length_of(geom.face_area[f]));
+    wall_normal_distance = to_device(dev, wd);
+  }
+  adjacent = to_device(dev, stencil.adjacent);
+  matrix_entry = to_device(dev, stencil.matrix_entry);
+  lsq_x = to_device(dev, stencil.lsq_x);
+  lsq_y = to_device(dev, stencil.lsq_y);
+  lsq_z = to_device(dev, stencil.lsq_z);
+  diffusion = to_device(dev, stencil.diffusion);
+  nonorth_x = to_device(dev, stencil.nonorth_x);
+  nonorth_y = to_device(dev, stencil.nonorth_y);
+  nonorth_z = to_device(dev, stencil.nonorth_z);
+
+  // Fields (:262-296).
+  velocity = zeros3(n);
+  face_velocity = zeros3(nf);
+  body = zeros3(n);
+  source = zeros3(n);
+  boundary_velocity = zeros3(nb);
+  for (auto& g : limited_gradient) g = zeros3(n);
+  grad_p = zeros3(n);
+  grad_correction = zeros3(n);
+  absolute_residual = zeros3(n);
+  fluent_scale = zeros3(n);
+  pressure = zeros(n);
+  mass = zeros(nf);
+  viscosity = F(dev, nf);
+  dev.fill(viscosity.data(), float(s.viscosity), nf);
+  boundary_pressure = zeros(nb);
+  scalar_kind = zeros_int(nb);
+  pressure_rhs = zeros(n);
+  correction = zeros(n);
+  correction_boundary = zeros(nb);
+  coefficient = zeros(nf);
+  face_pressure = zeros(nf);
+  pressure_jump = zeros(nf);
+  for (F* v : {&residual_u, &residual_v, &residual_w, &continuity, &provisional_continuity, &scale_cells, &speed,
+               &wall_diagonal, &wall_rhs})
+    *v = zeros(n);
+  paired = s.paired_state;
+  if (paired) {
+    velocity_lo = zeros3(n);
+    pressure_gradient_lo = zeros3(n);
+    pressure_lo = zeros(n);
+    mass_lo = zeros(nf);
+    previous_mass = zeros(nf);
+    pressure_boundary_lo = zeros(nb);
+    imbalance = zeros(n);
+    momentum_delta = zeros(n);
+  }
+  total_pressure = zeros_int(nb);
+  physical_diagonal = F(dev, n);
+  dev.fill(physical_diagonal.data(), 1.0f, n);
+  consistent_inverse = zeros(n);
+
+  // Boundary kinds, prescribed values and LSQ observations (:314-400).
+  std::vector<Index> kinds(nb, 0);
+  std::vector<float> pu(nb, 0.0f), pv(nb, 0.0f), pw(nb, 0.0f);
+  for (std::size_t p = 0; p < host.patches.size(); ++p)
+    for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+      const Index b = f - ni;
+      Vec3 v{};
+      if (patch_condition[p].kind == PatchCondition::Wall) {
+        v = patch_condition[p].velocity;
+        kinds[b] = 0;
+      } else {
+        kinds[b] = 3;
+      }
+      pu[b] = float(v.x);
+      pv[b] = float(v.y);
+      pw[b] = float(v.z);
+      bc_speed = std::max(bc_speed, length_of(v));
+    }
+  pressure_speed = 0;  // no pressure boundary
+  std::vector<Observation> observe_velocity(nb), observe_pressure(nb, Observation::Omitted);
+  for (Index b = 0; b < nb; ++b)
+    observe_velocity[b] = kinds[b] == 3 ? Observation::NormalFootValue : Observation::FaceValue;
+  boundary_kind = to_device(dev, kinds);
+  prescribed_pressure = zeros(nb);
+  prescribed_velocity.x = to_device(dev, pu);
+  prescribed_velocity.y = to_device(dev, pv);
+  prescribed_velocity.z = to_device(dev, pw);
+
+  CsrHost pattern;
+  pattern.n_rows = n;
+  pattern.row_ptr = stencil.row_ptr;
+  pattern.col = stencil.col;
+  pattern.diag = stencil.diag;
+  pattern.val.assign(stencil.col.size(), 0.0);
+  matrix = OwnedCsr(dev, pattern);
+
+  // The projected face reconstruction (fv/face_projection.cpp:136-150) corrects
+  // faces whose cell-to-face vectors leave the face normal; none may here.
+  for (Index f = 0; f < nf; ++f) {
+    if (f >= ni && observe_velocity[f - ni] != Observation::FaceValue) continue;
+    const Index o = host.owner[f];
+    const Vec3 unit = scale(geom.face_area[f], 1 / length_of(geom.face_area[f]));
+    const Vec3 a = sub(geom.face_centroid[f], geom.cell_centroid[o]);
+    const Vec3 b = f < ni ? sub(geom.cell_centroid[host.neighbour[f]], geom.face_centroid[f]) : Vec3{};
+    const double dp = dot(a, unit), dn = dot(b, unit);
+    const double roundoff =
+        256 * std::numeric_limits<double>::epsilon() *
+        std::max({1., length_of(geom.face_centroid[f]), length_of(geom.cell_centroid[o]),
+                  f < ni ? length_of(geom.cell_centroid[host.neighbour[f]]) : 0.});
+    KA_REQUIRE(length_of(sub(a, scale(unit, dp))) <= roundoff && length_of(sub(b, scale(unit, dn))) <= roundoff,
+               "SIMPLEC: meshes that need the projected face reconstruction are not supported");
+  }
+  face_correction = zeros(nf);
+  zero = zeros(n);
+  CsrHost identity = pattern;
+  for (auto j : identity.diag) identity.val[j] = 1;
+  for (auto& eq : momentum) {
+    eq.rhs = zeros(n);
+    eq.diagonal = zeros(n);
+    eq.consistent_inverse = zeros(n);
+    eq.limiter = zeros(n);
+    eq.grad = zeros3(n);
+    eq.solver = std::make_unique<KrylovSolver>(dev, SolverKind::BicgstabJacobi, identity, AmgOptions{});
+  }
+  velocity_lsq = least_squares(dev, host, geom, cf, observe_velocity);
+  {
+    const auto gain = extrapolation_gain(host, geom, cf, observe_pressure);
+    for (Index b = 0; b < nb; ++b)
+      if (observe_pressure[b] == Observation::Omitted && gain[b] > 1) observe_pressure[b] = Observation::FaceValue;
+  }
+  pressure_lsq = least_squares(dev, host, geom, cf, observe_pressure);
+  {
+    const auto q = orthogonal_quality(host, geom);
+    KA_REQUIRE(std::all_of(q.begin(), q.end(), [](double v) { return v > .05; }),
+               "SIMPLEC: cells at or below the poor-quality threshold are not supported");
+  }
+  // The initial pressure operator (:553-572): stored-float M-matrix values.
+  CsrHost initial = pattern;
+  for (Index c = 0; c < n; ++c) {
+    UpperSum ap{0, 0};
+    for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+      const Index f = cf.faces[j];
+      const Index other = f < ni ? (host.owner[f] == c ? host.neighbour[f] : host.owner[f]) : -1;
+      if (other < 0 && kinds[f - ni] != 2 && kinds[f - ni] != 4) continue;
+      const Vec3 dvec = sub(other >= 0 ? geom.cell_centroid[other] : geom.face_centroid[f], geom.cell_centroid[c]);
+      const float a = float(dot(geom.face_area[f], geom.face_area[f]) / std::abs(dot(geom.face_area[f], dvec)));
+      if (other < 0 || other == reference) ap.add(a);
+      if (other >= 0 && other != reference && c != reference) {
+        const auto first = initial.col.begin() + initial.row_ptr[c], last = initial.col.begin() + initial.row_ptr[c + 1];
+        const auto at = std::lower_bound(first, last, other) - initial.col.begin();
+        initial.val[at] = float(float(initial.val[at]) - a);
+      }
+    }
+    for (Index j = initial.row_ptr[c]; j < initial.row_ptr[c + 1]; ++j)
+      if (j != initial.diag[c]) ap.add(-float(initial.val[j]));
+    initial.val[initial.diag[c]] = c == reference ? 1 : ap.value();
+  }
+  AmgOptions amg;
+  amg.structure = Structure::SymmetricMMatrix;
+  pressure_solver = std::make_unique<KrylovSolver>(dev, SolverKind::AmgPcg, initial, amg);
+  const double total = std::accumulate(geom.cell_volume.begin(), geom.cell_volume.end(), 0.);
+  Vec3 lo = host.points.front(), hi = lo;
+  for (auto p : host.points) {
+    lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+    hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+  }
+  length = std::min(std::cbrt(total), std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z}));
+  update_time(bc_speed);
+  dev.fill(momentum[0].diagonal.data(), 1.0f, n);
+  dev.fill(momentum[0].consistent_inverse.data(), 0.0f, n);
+  update_boundaries();
+  update_pressure();
+  update_flux();
+}
+
+// :1303-1369
+void SteadySolver::Impl::step() {
+  refresh_time_step();
+  update_boundaries();
+  update_pressure();
+  momentum_source();
+  reconstruct_velocity();
+  for (int k = 0; k < 3; ++k) {
+    wall_terms(k);
+    assemble(k);
+    if (!paired) {
+      report.momentum[k] = solve_momentum(k, velocity[k]);
+      continue;
+    }
+    // Delta form: the same Krylov iterates, with the pair's residual.
+    paired_momentum_residual(k, residual_u);
+    dev.copy(momentum[k].rhs.data(), residual_u.data(), std::size_t(n) * 4);
+    dev.fill(momentum_delta.data(), 0.0f, n);
+    report.momentum[k] = solve_momentum(k, momentum_delta);
+    dev.launch(KH_FV_PAIRED_ACCUMULATE, n, {momentum_delta, velocity[k], velocity_lo[k], paired_slack().velocity});
+  }
+  update_momentum_diagonal();
+  update_boundaries();
+  update_flux();
+  if (paired)
+    dev.launch(KH_FV_PAIRED_CONTINUITY, n, {offsets, faces, sign, mass, mass_lo, imbalance, provisional_continuity});
+  else
+    dev.launch(KH_FV_CONTINUITY, n, {offsets, faces, sign, mass, provisional_continuity});
+  dev.launch(KH_FV_PRESSURE_ASSEMBLE, n,
+             {offsets, faces, sign, adjacent, matrix_entry, matrix.row_ptr, matrix.diag, coefficient, mass,
+              matrix.val, pressure_rhs, reference});
+  if (paired) dev.launch(KH_FV_PAIRED_PRESSURE_RHS, n, {imbalance, pressure_rhs, reference});
+  pressure_solver->refresh(matrix.view());
+  dev.fill(correction.data(), 0.0f, n);
+  report.pressure = pressure_solver->solve(matrix.view(), correction.data(), pressure_rhs.data(), settings.pressure_linear);
+  if (paired)
+    dev.launch(KH_FV_PAIRED_CORRECT_FLUX, nf,
+               {owner, neighbour, area.x, area.y, area.z, correction, coefficient, mass, mass_lo, ni,
+                paired_slack().flux_per_area});
+  else
+    dev.launch(KH_FV_CORRECT_FLUX, nf, {owner, neighbour, correction, coefficient, mass, ni});
+  // The correction has homogeneous Neumann data on fixed-velocity boundaries.
+  zero3(grad_correction, n);
+  pressure_boundary(correction, grad_correction, correction_boundary, boundary_pressure);
+  gradient(pressure_lsq, correction, boundary_pressure, grad_correction);
+  if (paired) {
+    const auto slack = paired_slack();
+    dev.launch(KH_FV_PAIRED_CORRECT_VELOCITY, n,
+               {volume, consistent_inverse, physical_diagonal, grad_correction.x, grad_correction.y,
+                grad_correction.z, correction, velocity.x, velocity.y, velocity.z, pressure, velocity_lo.x,
+                velocity_lo.y, velocity_lo.z, pressure_lo, float(settings.pressure_relaxation), slack.velocity,
+                slack.pressure});
+    update_pressure_gradient_lo();
+  } else {
+    dev.launch(KH_FV_CORRECT_VELOCITY, n,
+               {volume, consistent_inverse, physical_diagonal, grad_correction.x, grad_correction.y,
+                grad_correction.z, correction, velocity.x, velocity.y, velocity.z, pressure,
+                float(settings.pressure_relaxation)});
+  }
+  // finish_step (:950-954)
+  ++report.iteration;
+  if (report.iteration <= 5 || report.iteration % 20 == 0) refresh_paired_scales();
+  if (report.iteration <= 5 || report.iteration % settings.monitor_interval == 0) monitor();
+}
+
+// :1417-1477
+IterationReport SteadySolver::Impl::monitor() {
+  refresh_time_step();
+  update_boundaries();
+  update_pressure();
+  momentum_source();
+  reconstruct_velocity();
+  F* residuals[]{&residual_u, &residual_v, &residual_w};
+  for (int k = 0; k < 3; ++k) {
+    wall_terms(k);
+    assemble(k);
+    if (paired)
+      paired_momentum_residual(k, *residuals[k]);
+    else
+      dev.launch(KH_CSR_RESIDUAL, n, {matrix.row_ptr, matrix.col, matrix.val, velocity[k], momentum[k].rhs, *residuals[k]});
+  }
+  update_momentum_diagonal();
+  dev.launch(KH_FV_MONITOR_CELL, n,
+             {offsets, faces, sign, mass, velocity.x, velocity.y, velocity.z, physical_diagonal, residual_u,
+              residual_v, residual_w, continuity, scale_cells, speed, absolute_residual.x, absolute_residual.y,
+              absolute_residual.z});
+  if (paired) dev.launch(KH_FV_PAIRED_CONTINUITY, n, {offsets, faces, sign, mass, mass_lo, imbalance, continuity});
+  dev.launch(KH_FV_FLUENT_MOMENTUM_SCALE, n,
+             {momentum[0].diagonal, momentum[1].diagonal, momentum[2].diagonal, volume, speed, fluent_scale.x,
+              fluent_scale.y, fluent_scale.z, float(settings.density), time_step});
+  // All of the monitor's sums behind one wait (each equals its single call).
+  const auto r = dev.reduce_many(
+      {HipDevice::sum(scale_cells.data(), n), HipDevice::sum(provisional_continuity.data(), n),
+       HipDevice::sum(continuity.data(), n), HipDevice::sum(absolute_residual.x.data(), n),
+       HipDevice::sum(absolute_residual.y.data(), n), HipDevice::sum(absolute_residual.z.data(), n),
+       HipDevice::sum(fluent_scale.x.data(), n), HipDevice::sum(fluent_scale.y.data(), n),
+       HipDevice::sum(fluent_scale.z.data(), n), HipDevice::max_abs(speed.data(), n)});
+  const double denominator = r[0], raw = r[1];
+  report.corrected_mass_imbalance = r[2];
+  if (report.iteration > 0 && report.iteration <= 5) first_five_continuity[report.iteration - 1] = raw;
+  continuity_scale = *std::max_element(std::begin(first_five_continuity), std::end(first_five_continuity));
+  const auto scaled = [](double v, double d) {
+    return d > 0 ? v / d : (v == 0 ? 0 : std::numeric_limits<double>::infinity());
+  };
+  ResidualValue* values[]{&report.u, &report.v, &report.w};
+  for (int k = 0; k < 3; ++k) {
+    values[k]->raw = r[3 + k];
+    values[k]->scaled = scaled(values[k]->raw, denominator);
+    values[k]->fluent_normalization = r[6 + k];
+    values[k]->fluent_scaled = scaled(values[k]->raw, values[k]->fluent_normalization);
+  }
+  report.continuity = {raw, scaled(raw, continuity_scale), scaled(raw, continuity_scale), continuity_scale};
+  report.maximum_velocity = r[9];
+  report.pseudo_time_step = time_step;
+  return report;
+}
+
+SteadySolver::SteadySolver(HipDevice& dev, PolyMesh mesh, std::vector<PatchCondition> conditions,
+                           SolverSettings settings)
+    : impl_(std::make_unique<Impl>(dev, std::move(mesh), std::move(conditions), settings)) {}
+SteadySolver::~SteadySolver() = default;
+void SteadySolver::iterate(int count) {
+  KA_REQUIRE(count >= 0, "FV negative iteration count");
+  while (count-- > 0) impl_->step();
+}
+IterationReport SteadySolver::monitor() { return impl_->monitor(); }
+const IterationReport& SteadySolver::report() const { return impl_->report; }
+Index SteadySolver::reference_cell() const { return impl_->reference; }
+const MeshGeometry& SteadySolver::geometry() const { return impl_->geom; }
+Index SteadySolver::cells() const { return impl_->n; }
+int SteadySolver::host_waits() const {
+  int waits = impl_->pressure_solver->host_waits();
+  for (const auto& eq : impl_->momentum) waits += eq.solver->host_waits();
+  return waits;
+}
+const float* SteadySolver::field(const std::string& name, std::size_t* size) const {
+  auto& s = *impl_;
+  const std::map<std::string, const F*> table = {
+      {"u", &s.velocity.x}, {"v", &s.velocity.y}, {"w", &s.velocity.z}, {"p", &s.pressure},
+      {"mass", &s.mass}, {"u_lo", &s.velocity_lo.x}, {"v_lo", &s.velocity_lo.y}, {"w_lo", &s.velocity_lo.z},
+      {"p_lo", &s.pressure_lo}, {"mass_lo", &s.mass_lo}, {"correction", &s.correction},
+      {"coefficient", &s.coefficient}, {"face_pressure", &s.face_pressure},
+      {"boundary_u", &s.boundary_velocity.x}, {"boundary_pressure", &s.boundary_pressure},
+      {"physical_diagonal", &s.physical_diagonal}, {"consistent_inverse", &s.consistent_inverse},
+      {"grad_px", &s.grad_p.x}, {"face_u", &s.face_velocity.x}, {"matrix_values", &s.matrix.val},
+      {"pressure_rhs", &s.pressure_rhs}, {"volume", &s.volume}, {"weight", &s.weight},
+      {"owner_to_face_x", &s.owner_to_face.x}, {"neighbour_to_face_y", &s.neighbour_to_face.y},
+      {"owner_to_neighbour_z", &s.owner_to_neighbour.z}, {"area_x", &s.area.x},
+      {"wall_normal_distance", &s.wall_normal_distance}, {"velocity_lsq_inverse", &s.velocity_lsq.inverse},
+      {"pressure_lsq_inverse", &s.pressure_lsq.inverse}, {"velocity_lsq_boundary_x", &s.velocity_lsq.bx},
+      {"pressure_lsq_boundary_y", &s.pressure_lsq.by}};
+  const auto it = table.find(name);
+  if (it == table.end() || it->second->empty()) return nullptr;
+  if (size) *size = it->second->size();
+  return it->second->data();
+}
+
+}  // namespace kutta_amd

@@ -1,0 +1,76 @@
+// Steady SIMPLEC solver on the MI355X backend: the HIP-native counterpart of
+// kutta::fv::SteadySolver (fv/steady_solver.hpp:286-360) for laminar flow in a
+// closed domain (wall and symmetry patches, the reference cell pinned), with
+// the reference's default numerics: paired (hi+lo) or float state,
+// second-order upwind momentum limited by Barth-Jespersen, least-squares
+// gradients, momentum-weighted Rhie-Chow faces, the automatic pseudo time
+// step and the AMG-PCG pressure correction. Every field lives in HBM; the host
+// reads one reduction per iteration (the time step's speed scale) plus the
+// linear solvers' convergence polls, as the reference does.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "krylov.hpp"
+#include "mesh.hpp"
+
+namespace kutta_amd {
+
+// fv/steady_solver.hpp:23-27, 65 (the supported conditions).
+struct PatchCondition {
+  enum Kind : int { Wall = 0, Symmetry = 3 };
+  std::string patch;
+  int kind = Wall;
+  Vec3 velocity{};  // wall velocity
+};
+
+// fv/steady_solver.hpp:193-271, the members this solver honours; the others
+// keep their reference defaults, which are the only ones accepted.
+struct SolverSettings {
+  double density = 1, viscosity = .01;
+  double momentum_relaxation = .7, pressure_relaxation = 1;
+  double pseudo_time_scale = 1;
+  int monitor_interval = 20;
+  bool paired_state = true;
+  double paired_state_slack_ulps = 1;
+  Vec3 reference_point{};
+  SolveControl momentum_linear{1e-2, 1e-20, 100, 8};
+  SolveControl pressure_linear{1e-3, 1e-20, 300, 8};
+};
+
+// fv/steady_solver.hpp:273-284
+struct ResidualValue {
+  double raw = 0, scaled = 0, fluent_scaled = 0, fluent_normalization = 0;
+};
+struct IterationReport {
+  int iteration = 0;
+  ResidualValue u, v, w, continuity;
+  double corrected_mass_imbalance = 0, maximum_velocity = 0, pseudo_time_step = 0;
+  SolveStats momentum[3], pressure;
+};
+
+class SteadySolver {
+ public:
+  SteadySolver(HipDevice& dev, PolyMesh mesh, std::vector<PatchCondition> conditions,
+               SolverSettings settings = {});
+  ~SteadySolver();
+  void iterate(int count);
+  IterationReport monitor();
+  const IterationReport& report() const;
+  // Device array by name (u, v, w, p, mass, ...; the names of the oracle's
+  // restatement) and its length; nullptr when unknown or not allocated.
+  const float* field(const std::string& name, std::size_t* size) const;
+  Index reference_cell() const;
+  const MeshGeometry& geometry() const;
+  Index cells() const;
+  // Host waits of the linear solvers so far (pressure, momentum).
+  int host_waits() const;
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace kutta_amd

@@ -5,6 +5,8 @@
 #include <type_traits>
 #include <utility>
 
+#include <hip/hip_ext.h>
+
 #include "kh_device_math.hpp"
 
 namespace kh {
@@ -1710,6 +1712,79 @@ struct SstSustain {
   }
 };
 
+// ------------------------------------------------------------------------
+// kernels/fv_fluent_schemes.kern.h
+// :10-12. SIMPLE's correction coefficient, the inverse relaxed diagonal.
+struct FvSimpleInverse {
+  __device__ void operator()(int i, CI diag, CF values, WF inverse) const {
+    inverse[i] = 1.0f / kmax(values[diag[i]], 1.0e-20f);
+  }
+};
+// :28-39. Inverse-diagonal weighted face pressure; unprescribed boundary
+// faces (kinds other than 2 and 4) hold the owner's pressure.
+struct FvStandardPressureFaces {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CF p, CF diagonal, WF face,
+                             WF boundary, int internal) const {
+    const int c = owner[i];
+    if (i < internal) {
+      const int nb = neighbour[i];
+      const float wc = 1.0f / kmax(diagonal[c], 1.0e-20f);
+      const float wn = 1.0f / kmax(diagonal[nb], 1.0e-20f);
+      face[i] = (p[c] * wc + p[nb] * wn) / (wc + wn);
+      return;
+    }
+    const int b = i - internal;
+    if (kind[b] == 2 || kind[b] == 4) return;
+    face[i] = p[c];
+    boundary[b] = p[c];
+  }
+};
+// :47-50
+struct FvCellPressureBoundary {
+  __device__ void operator()(int i, CI owner, CI kind, CF p, WF boundary, int internal) const {
+    if (kind[i] != 2 && kind[i] != 4) boundary[i] = p[owner[i + internal]];
+  }
+};
+// :54-59
+struct FvForceToShear {
+  __device__ void operator()(int i, CF mask, CF ax, CF ay, CF az, CF fx, CF fy, CF fz, WF sx, WF sy,
+                             WF sz, int internal) const {
+    const int f = i + internal;
+    const float a = sqrtf(ax[f] * ax[f] + ay[f] * ay[f] + az[f] * az[f]);
+    const float scale = mask[i] > 0.0f ? -1.0f / a : 0.0f;
+    sx[i] = scale * fx[i];
+    sy[i] = scale * fy[i];
+    sz[i] = scale * fz[i];
+  }
+};
+// :70-84
+struct FvPoorCellPressureFaces {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI kind, CI poor, CF px, CF py, CF pz,
+                             CF nx, CF ny, CF nz, CF p, CF gx, CF gy, CF gz, WF face, WF boundary,
+                             int internal) const {
+    const int c = owner[i];
+    if (i >= internal) {
+      const int b = i - internal;
+      if (poor[c] == 0 || kind[b] == 2 || kind[b] == 4) return;
+      face[i] = p[c];
+      boundary[b] = p[c];
+      return;
+    }
+    const int nb = neighbour[i];
+    if (poor[c] == 0 && poor[nb] == 0) return;
+    const float pp = poor[c] != 0 ? p[c] : p[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i];
+    const float pn = poor[nb] != 0 ? p[nb] : p[nb] + gx[nb] * nx[i] + gy[nb] * ny[i] + gz[nb] * nz[i];
+    face[i] = 0.5f * (pp + pn);
+  }
+};
+// :93-97
+struct FvPoorPressureInterfaces {
+  __device__ void operator()(int i, CI owner, CI neighbour, CI poor, CF p, WF face) const {
+    const int c = owner[i], nb = neighbour[i];
+    if (poor[c] != poor[nb]) face[i] = 0.5f * (p[c] + p[nb]);
+  }
+};
+
 template <class Body, class... P>
 __global__ void __launch_bounds__(kBlock) k_each(int n, P... p) {
   const int i = cell_index();
@@ -1732,7 +1807,12 @@ constexpr char kind_code() {
 template <class Body, class... P, std::size_t... K>
 void launch_impl(kh_device* d, int n, const kh_arg* a, void (Body::*)(int, P...) const,
                  std::index_sequence<K...>) {
-  k_each<Body, P...><<<dim3(blocks_for(n)), dim3(kBlock), 0, d->stream>>>(n, from_arg<P>(a, K)...);
+  if (d->packet_start) {
+    hipExtLaunchKernelGGL(k_each<Body, P...>, dim3(blocks_for(n)), dim3(kBlock), 0, d->stream,
+                          d->packet_start, d->packet_stop, 0, n, from_arg<P>(a, K)...);
+  } else {
+    k_each<Body, P...><<<dim3(blocks_for(n)), dim3(kBlock), 0, d->stream>>>(n, from_arg<P>(a, K)...);
+  }
 }
 template <class Body, class... P>
 constexpr std::size_t arity(void (Body::*)(int, P...) const) { return sizeof...(P); }
@@ -1766,7 +1846,7 @@ void l_fv_pressure_assemble(kh_device* d, int n, const kh_arg* a) {
 const std::vector<KernelEntry>& fv_kernels() {
   static const std::vector<KernelEntry> table = {
       {KH_FV_PRESSURE_ASSEMBLE, "fv_pressure_assemble", "bbbbbbbbbbbi", l_fv_pressure_assemble},
-#define FV(id, name, Body) {id, name, signature_of<Body>(), launch_body<Body>}
+#define FV(id, name, Body) {id, name, signature_of<Body>(), launch_body<Body>, true}
       FV(KH_FV_VELOCITY_BOUNDARY, "fv_velocity_boundary", FvVelocityBoundary),
       FV(KH_FV_PRESSURE_BOUNDARY, "fv_pressure_boundary", FvPressureBoundary),
       FV(KH_FV_PRESSURE_FACES, "fv_pressure_faces", FvPressureFaces),
@@ -1852,6 +1932,12 @@ const std::vector<KernelEntry>& fv_kernels() {
       FV(KH_SST_SOURCES, "sst_sources", SstSources),
       FV(KH_SST_FACE_DIFFUSION, "sst_face_diffusion", SstFaceDiffusion),
       FV(KH_SST_SUSTAIN, "sst_sustain", SstSustain),
+      FV(KH_FV_SIMPLE_INVERSE, "fv_simple_inverse", FvSimpleInverse),
+      FV(KH_FV_STANDARD_PRESSURE_FACES, "fv_standard_pressure_faces", FvStandardPressureFaces),
+      FV(KH_FV_CELL_PRESSURE_BOUNDARY, "fv_cell_pressure_boundary", FvCellPressureBoundary),
+      FV(KH_FV_FORCE_TO_SHEAR, "fv_force_to_shear", FvForceToShear),
+      FV(KH_FV_POOR_CELL_PRESSURE_FACES, "fv_poor_cell_pressure_faces", FvPoorCellPressureFaces),
+      FV(KH_FV_POOR_PRESSURE_INTERFACES, "fv_poor_pressure_interfaces", FvPoorPressureInterfaces),
 #undef FV
   };
   return table;

@@ -43,6 +43,9 @@ struct kh_device {
   double* results_host = nullptr;
   bool capturing = false;
   std::vector<kh::ProfileSlot> profile;
+  // Events for the next packet-timed launch (null when not profiling it).
+  hipEvent_t packet_start = nullptr;
+  hipEvent_t packet_stop = nullptr;
   std::vector<hipEvent_t> event_pool;
 };
 
@@ -70,6 +73,9 @@ struct KernelEntry {
   const char* name;
   const char* signature;
   Launcher launch;
+  // The launcher passes kh_device::packet_start/stop to hipExtLaunchKernelGGL,
+  // so a profiled launch is timed by its own dispatch packet.
+  bool packet_timed = false;
 };
 // Defined by each kernel translation unit.
 const std::vector<KernelEntry>& linalg_kernels();

@@ -6,6 +6,8 @@
 // Per-element arithmetic follows the reference kernel bodies operation for
 // operation (cited per kernel), so results are bitwise equal to the serial
 // reference wherever no FP64 reduction order is involved.
+#include <hip/hip_ext.h>
+
 #include "kh_device_math.hpp"
 
 namespace kh {
@@ -816,7 +818,13 @@ void l_amg_coarse_solve(kh_device* d, int n, const kh_arg* a) {
                                                            KCF(5), KB(6), count, i32(a, 8));
 }
 void l_amg_smooth(kh_device* d, int n, const kh_arg* a) {
-  k_amg_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+  if (d->packet_start) {
+    hipExtLaunchKernelGGL(k_amg_smooth, dim3(blocks_for(n)), dim3(kBlock), 0, d->stream,
+                          d->packet_start, d->packet_stop, 0, n, KB(0), KB(1), KCF(2), KCF(3),
+                          KCF(4), KCF(5), KF(6), KB(7));
+  } else {
+    k_amg_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+  }
 }
 void l_amg_smooth_zero(kh_device* d, int n, const kh_arg* a) {
   k_amg_smooth_zero<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
@@ -977,7 +985,7 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_KRYLOV_DIFFERENCE, "krylov_difference", "bbbf", l_krylov_difference},
       {KH_KRYLOV_BICG_STEP, "krylov_bicg_step", "bbbff", l_krylov_bicg_step},
       {KH_AMG_COARSE_SOLVE, "amg_coarse_solve", "bbbbbbbii", l_amg_coarse_solve},
-      {KH_AMG_SMOOTH, "amg_smooth", "bbbbbbbb", l_amg_smooth},
+      {KH_AMG_SMOOTH, "amg_smooth", "bbbbbbbb", l_amg_smooth, true},
       {KH_AMG_SMOOTH_ZERO, "amg_smooth_zero", "bbbbbbbb", l_amg_smooth_zero},
       {KH_AMG_RESTRICT, "amg_restrict", "bbbb", l_amg_restrict},
       {KH_AMG_PROLONG, "amg_prolong", "bbb", l_amg_prolong},

@@ -350,12 +350,18 @@ int kh_launch(kh_device* d, uint32_t kernel, size_t n, const kh_arg* args, size_
     if (timed) {
       start = take_event(d);
       stop = take_event(d);
-      check(hipEventRecord(start, d->stream), "hipEventRecord");
+      if (e->packet_timed) {
+        d->packet_start = start;
+        d->packet_stop = stop;
+      } else {
+        check(hipEventRecord(start, d->stream), "hipEventRecord");
+      }
     }
     e->launch(d, int(n), args);
+    d->packet_start = d->packet_stop = nullptr;
     check(hipGetLastError(), e->name);
     if (timed) {
-      check(hipEventRecord(stop, d->stream), "hipEventRecord");
+      if (!e->packet_timed) check(hipEventRecord(stop, d->stream), "hipEventRecord");
       slot.pending.emplace_back(start, stop);
     }
   });

@@ -145,6 +145,13 @@ SPECS = {
                              "flag flag2 flag f+ flag2 f+ f+ f+ f"),
     "sst_face_diffusion": ("faces", "owner neighbour F bmask C+ C+ C+ internal oF oF"),
     "sst_sustain": ("cells", "C+ Cu oC oC f+ f+"),
+    # kernels/fv_fluent_schemes.kern.h
+    "fv_simple_inverse": ("cells", "diag N oC"),
+    "fv_standard_pressure_faces": ("faces", "owner neighbour kind C C+ oF oB internal"),
+    "fv_cell_pressure_boundary": ("boundary", "owner kind C oB internal"),
+    "fv_force_to_shear": ("boundary", "B01 F+ F+ F+ B B B oB oB oB internal"),
+    "fv_poor_cell_pressure_faces": ("faces", "owner neighbour kind mask F F F F F F C C C C oF oB internal"),
+    "fv_poor_pressure_interfaces": ("internal", "owner neighbour mask C oF"),
 }
 
 # Kernels whose bodies call transcendental functions (tanh, log, pow): the

@@ -406,3 +406,68 @@ def ref_fv_each(name: str, n: int, *args) -> None:
     ptrs, keep = _fv_pointers(args)
     if R.ref_fv_each(name.encode(), int(n), ptrs) != 0:
         raise OracleError(f"reference build lacks fv kernel {name}")
+
+
+class SteadySolver:
+    """The serial restatement of the laminar closed-domain SIMPLEC solver
+    (fv/steady_solver.cpp; oracle/simplec.cpp) on make_box_hex(nx, ny, nz).
+    kinds/wall/settings are packed as for the product (kutta_amd)."""
+
+    REPORT_SIZE = 26
+
+    def __init__(self, nx, ny, nz, kinds, wall, settings):
+        L = lib()
+        vp, i32 = C.c_void_p, C.c_int
+        L.oracle_simplec_create.restype = vp
+        L.oracle_simplec_create.argtypes = [i32, i32, i32, vp, vp, vp]
+        L.oracle_simplec_last_error.restype = C.c_char_p
+        L.oracle_simplec_free.argtypes = [vp]
+        L.oracle_simplec_iterate.argtypes = [vp, i32]
+        L.oracle_simplec_report.argtypes = [vp, i32, vp]
+        L.oracle_simplec_array.argtypes = [vp, C.c_char_p, vp]
+        L.oracle_simplec_array.restype = C.c_long
+        L.oracle_simplec_reference.argtypes = [vp]
+        L.oracle_simplec_centroids.argtypes = [vp, vp]
+        self._keep = (kinds, wall, settings)
+        h = L.oracle_simplec_create(nx, ny, nz, C.cast(kinds, vp), C.cast(wall, vp), C.cast(settings, vp))
+        if not h:
+            raise OracleError(L.oracle_simplec_last_error().decode())
+        self.handle = h
+        self.cells = nx * ny * nz
+
+    def close(self):
+        if self.handle:
+            lib().oracle_simplec_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def iterate(self, count=1):
+        if lib().oracle_simplec_iterate(self.handle, count) != 0:
+            raise OracleError(lib().oracle_simplec_last_error().decode())
+
+    def report(self, monitor=False):
+        out = (C.c_double * self.REPORT_SIZE)()
+        if lib().oracle_simplec_report(self.handle, 1 if monitor else 0, out) != 0:
+            raise OracleError(lib().oracle_simplec_last_error().decode())
+        return list(out)
+
+    def field(self, name):
+        n = lib().oracle_simplec_array(self.handle, name.encode(), None)
+        if n < 0:
+            raise OracleError(f"unknown SIMPLEC array {name}")
+        out = np.empty(n, np.float32)
+        lib().oracle_simplec_array(self.handle, name.encode(), out.ctypes.data)
+        return out
+
+    def reference(self):
+        return lib().oracle_simplec_reference(self.handle)
+
+    def centroids(self):
+        out = np.empty((self.cells, 3), np.float64)
+        lib().oracle_simplec_centroids(self.handle, out.ctypes.data)
+        return out

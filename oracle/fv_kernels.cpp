@@ -1583,6 +1583,71 @@ void sst_sustain(int i, F rho, F f1, W k_source, W omega_source, float ambient_k
   omega_source[i] += sst_beta(f1[i]) * rho[i] * ambient_omega * ambient_omega;
 }
 
+// ------------------------------------------------- fv_fluent_schemes.kern.h
+// :10-12. SIMPLE's correction coefficient: the inverse relaxed diagonal.
+void fv_simple_inverse(int i, I diag, F values, W inverse) {
+  inverse[i] = 1.0f / kmax(values[diag[i]], 1.0e-20f);
+}
+
+// :28-39. Interior faces weigh the cell pressures by inverse momentum
+// diagonals; unprescribed boundary faces (not kinds 2, 4) take the cell value.
+void fv_standard_pressure_faces(int i, I owner, I neighbour, I kind, F p, F diagonal, W face,
+                                W boundary, int internal) {
+  const int c = owner[i];
+  if (i < internal) {
+    const int nb = neighbour[i];
+    const float wc = 1.0f / kmax(diagonal[c], 1.0e-20f);
+    const float wn = 1.0f / kmax(diagonal[nb], 1.0e-20f);
+    face[i] = (p[c] * wc + p[nb] * wn) / (wc + wn);
+    return;
+  }
+  const int b = i - internal;
+  if (kind[b] == 2 || kind[b] == 4) return;
+  face[i] = p[c];
+  boundary[b] = p[c];
+}
+
+// :47-50. Unprescribed boundary values hold the owner's pressure.
+void fv_cell_pressure_boundary(int i, I owner, I kind, F p, W boundary, int internal) {
+  if (kind[i] != 2 && kind[i] != 4) boundary[i] = p[owner[i + internal]];
+}
+
+// :54-59. Wall force per boundary face to the fluid's wall stress.
+void fv_force_to_shear(int i, F mask, F ax, F ay, F az, F fx, F fy, F fz, W sx, W sy, W sz,
+                       int internal) {
+  const int f = i + internal;
+  const float a = ksqrt(ax[f] * ax[f] + ay[f] * ay[f] + az[f] * az[f]);
+  const float scale = mask[i] > 0.0f ? -1.0f / a : 0.0f;
+  sx[i] = scale * fx[i];
+  sy[i] = scale * fy[i];
+  sz[i] = scale * fz[i];
+}
+
+// :70-84. A flagged cell's side of a face pressure is its cell value.
+void fv_poor_cell_pressure_faces(int i, I owner, I neighbour, I kind, I poor, F px, F py, F pz,
+                                 F nx, F ny, F nz, F p, F gx, F gy, F gz, W face, W boundary,
+                                 int internal) {
+  const int c = owner[i];
+  if (i >= internal) {
+    const int b = i - internal;
+    if (poor[c] == 0 || kind[b] == 2 || kind[b] == 4) return;
+    face[i] = p[c];
+    boundary[b] = p[c];
+    return;
+  }
+  const int nb = neighbour[i];
+  if (poor[c] == 0 && poor[nb] == 0) return;
+  const float pp = poor[c] != 0 ? p[c] : p[c] + gx[c] * px[i] + gy[c] * py[i] + gz[c] * pz[i];
+  const float pn = poor[nb] != 0 ? p[nb] : p[nb] + gx[nb] * nx[i] + gy[nb] * ny[i] + gz[nb] * nz[i];
+  face[i] = 0.5f * (pp + pn);
+}
+
+// :93-97. Mixed-order interior faces use both cell pressures.
+void fv_poor_pressure_interfaces(int i, I owner, I neighbour, I poor, F p, W face) {
+  const int c = owner[i], nb = neighbour[i];
+  if (poor[c] != poor[nb]) face[i] = 0.5f * (p[c] + p[nb]);
+}
+
 // ---------------------------------------------------------------- dispatch
 template <class T>
 T from_arg(void* const* a, std::size_t k) {
@@ -1656,6 +1721,9 @@ const std::map<std::string, Entry>& table() {
       KERNEL(sst_rates), KERNEL(sst_blending), KERNEL(sst_viscosity), KERNEL(sst_blending_viscosity),
       KERNEL(sst_wall), KERNEL(sst_wall_omega), KERNEL(sst_wall_cells), KERNEL(sst_sources),
       KERNEL(sst_face_diffusion), KERNEL(sst_sustain),
+      KERNEL(fv_simple_inverse), KERNEL(fv_standard_pressure_faces),
+      KERNEL(fv_cell_pressure_boundary), KERNEL(fv_force_to_shear),
+      KERNEL(fv_poor_cell_pressure_faces), KERNEL(fv_poor_pressure_interfaces),
   };
   return t;
 }

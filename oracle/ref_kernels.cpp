@@ -31,6 +31,7 @@ namespace reff {
 #include "kutta/kernels/fv_paired.kern.h"
 #include "kutta/kernels/fv_rans.kern.h"
 #include "kutta/kernels/sst.kern.h"
+#include "kutta/kernels/fv_fluent_schemes.kern.h"
 }  // namespace reff
 
 #define LOOP(name, ...) for (int i = 0; i < n; ++i) refk::name(i, __VA_ARGS__)
@@ -92,6 +93,9 @@ const std::map<std::string, Runner>& fv_table() {
       REF_FV(sst_rates), REF_FV(sst_blending), REF_FV(sst_viscosity), REF_FV(sst_blending_viscosity),
       REF_FV(sst_wall), REF_FV(sst_wall_omega), REF_FV(sst_wall_cells), REF_FV(sst_sources),
       REF_FV(sst_face_diffusion), REF_FV(sst_sustain),
+      REF_FV(fv_simple_inverse), REF_FV(fv_standard_pressure_faces),
+      REF_FV(fv_cell_pressure_boundary), REF_FV(fv_force_to_shear),
+      REF_FV(fv_poor_cell_pressure_faces), REF_FV(fv_poor_pressure_interfaces),
   };
   return t;
 }

@@ -1,0 +1,1076 @@
+// ORACLE — test infrastructure only. Serial restatement of the reference's
+// steady SIMPLEC solver (src/kutta/fv/steady_solver.cpp) for the laminar
+// closed-domain configuration: wall and symmetry patches, paired (hi+lo) or
+// float state, second-order upwind momentum with the Barth-Jespersen limiter,
+// least-squares gradients, momentum-weighted Rhie-Chow faces, the automatic
+// pseudo time step and the AMG-PCG pressure correction. It also restates the
+// host setup the step consumes: upload_mesh's FP32 face and cell arrays
+// (fv/device_mesh.cpp:208-256), boundary_least_squares and its extrapolation
+// gain (fv/operators.cpp:92-211), orthogonal_quality (mesh/mesh.cpp:172-187)
+// and the initial pressure operator (steady_solver.cpp:553-572).
+//
+// Kernels run through oracle_fv_each (the serial restatements in
+// fv_kernels.cpp), reductions are the serial FP64 loops of cpu_device.cpp,
+// linear solves are oracle::Solver. Every configuration outside the subset
+// above raises an error instead of approximating it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "oracle.hpp"
+#include "oracle_mesh.hpp"
+
+extern "C" int oracle_fv_each(const char* name, int n, void* const* args);
+
+namespace oracle::simplec {
+namespace {
+
+using V = std::vector<float>;
+using VI = std::vector<int>;
+
+struct Vec3f {
+  V x, y, z;
+  void assign(std::size_t n, float value = 0.0f) {
+    x.assign(n, value);
+    y.assign(n, value);
+    z.assign(n, value);
+  }
+  V& operator[](int k) { return k == 0 ? x : (k == 1 ? y : z); }
+};
+
+double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Vec3 operator-(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+Vec3 operator*(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double norm(Vec3 a) { return std::sqrt(dot(a, a)); }
+
+// One serial kernel launch in oracle_fv_each's convention: args[k] is the
+// buffer pointer itself, or points at a 4-byte scalar held in slot[k].
+union Slot {
+  float f;
+  int i;
+};
+inline void* put(Slot&, float* p) { return p; }
+inline void* put(Slot&, const float* p) { return const_cast<float*>(p); }
+inline void* put(Slot&, int* p) { return p; }
+inline void* put(Slot&, const int* p) { return const_cast<int*>(p); }
+inline void* put(Slot& s, float v) { s.f = v; return &s; }
+inline void* put(Slot& s, int v) { s.i = v; return &s; }
+template <class... T>
+void fv(const char* name, int n, T... a) {
+  Slot slot[sizeof...(T)];
+  void* args[sizeof...(T)];
+  int k = 0;
+  ((args[k] = put(slot[k], a), ++k), ...);
+  if (n > 0 && oracle_fv_each(name, n, args) != 0)
+    throw Error(std::string("oracle SIMPLEC: unknown kernel ") + name);
+}
+
+enum class Observation { Omitted, FaceValue, NormalFootValue };
+
+// fv/device_mesh.cpp:208-256 (wall distance, used only by turbulence models,
+// is not part of this restatement).
+struct DeviceMesh {
+  int n_cells = 0, n_faces = 0, n_internal = 0;
+  int n_boundary() const { return n_faces - n_internal; }
+  VI owner, neighbour, offsets, faces;
+  Vec3f area, owner_to_face, neighbour_to_face, owner_to_neighbour;
+  V weight, volume, sign, wall_normal_distance;
+};
+
+// fv/operators.hpp:33-50: rows of the fit (interior rows are the stencil's),
+// boundary-face rows and the six-plane inverse.
+struct LeastSquares {
+  Vec3f boundary_delta;
+  V inverse;
+};
+
+}  // namespace
+
+struct Condition {
+  int kind = 0;  // 0 wall, 3 symmetry (the boundary_kind codes of fv_flow.kern.h)
+  Vec3 velocity{};
+};
+
+struct Settings {
+  double density = 1, viscosity = .01;
+  double momentum_relaxation = .7, pressure_relaxation = 1;
+  double pseudo_time_scale = 1;
+  int monitor_interval = 20;
+  bool paired_state = true;
+  double paired_state_slack_ulps = 1;
+  Vec3 reference_point{};
+  SolveControl momentum_linear{1e-2, 1e-20, 100, 8};
+  SolveControl pressure_linear{1e-3, 1e-20, 300, 8};
+};
+
+struct ResidualValue {
+  double raw = 0, scaled = 0, fluent_scaled = 0, fluent_normalization = 0;
+};
+// fv/steady_solver.hpp:276-284
+struct Report {
+  int iteration = 0;
+  ResidualValue u, v, w, continuity;
+  double corrected_mass_imbalance = 0, maximum_velocity = 0, pseudo_time_step = 0;
+  SolveStats momentum[3], pressure;
+};
+
+namespace {
+
+// fv/operators.cpp:97-137
+void cell_fit(const PolyMesh& m, const Geometry& g, const CellFaces& cf,
+              const std::vector<Observation>& obs, Index c, std::vector<Vec3>& delta,
+              double inv[3][3]) {
+  double a[3][3]{}, q[3][3]{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+    const auto f = cf.faces[j];
+    const bool internal = f < m.n_internal_faces();
+    if (!internal && obs[f - m.n_internal_faces()] == Observation::Omitted) {
+      delta.push_back({});
+      continue;
+    }
+    const auto n = internal ? (m.owner[f] == c ? m.neighbour[f] : m.owner[f]) : -1;
+    auto d = (internal ? g.cell_centroid[n] : g.face_centroid[f]) - g.cell_centroid[c];
+    if (!internal && obs[f - m.n_internal_faces()] == Observation::NormalFootValue) {
+      const auto area = g.face_area[f];
+      d = area * (dot(d, area) / dot(area, area));
+    }
+    const double weight = 1 / dot(d, d), x[]{d.x, d.y, d.z};
+    delta.push_back(d * weight);
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) a[r][k] += weight * x[r] * x[k];
+  }
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    int p = 0, r = 1;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i + 1; j < 3; ++j)
+        if (std::abs(a[i][j]) > std::abs(a[p][r])) {
+          p = i;
+          r = j;
+        }
+    if (std::abs(a[p][r]) < 1e-14) break;
+    const double angle = .5 * std::atan2(2 * a[p][r], a[r][r] - a[p][p]);
+    const double cs = std::cos(angle), sn = std::sin(angle);
+    const double app = a[p][p], arr = a[r][r], apr = a[p][r];
+    a[p][p] = cs * cs * app - 2 * cs * sn * apr + sn * sn * arr;
+    a[r][r] = sn * sn * app + 2 * cs * sn * apr + cs * cs * arr;
+    a[p][r] = a[r][p] = 0;
+    for (int i = 0; i < 3; ++i) {
+      if (i != p && i != r) {
+        const double x = a[i][p], y = a[i][r];
+        a[i][p] = a[p][i] = cs * x - sn * y;
+        a[i][r] = a[r][i] = sn * x + cs * y;
+      }
+      const double x = q[i][p], y = q[i][r];
+      q[i][p] = cs * x - sn * y;
+      q[i][r] = sn * x + cs * y;
+    }
+  }
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 3; ++j) inv[r][j] = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double eigen = a[k][k] > 1e-10 ? 1 / a[k][k] : 1;
+    for (int r = 0; r < 3; ++r)
+      for (int j = 0; j < 3; ++j) inv[r][j] += q[r][k] * eigen * q[j][k];
+  }
+}
+
+void to_float(Vec3f& out, const std::vector<Vec3>& v) {
+  out.assign(v.size());
+  for (std::size_t i = 0; i < v.size(); ++i) {
+    out.x[i] = float(v[i].x);
+    out.y[i] = float(v[i].y);
+    out.z[i] = float(v[i].z);
+  }
+}
+
+// fv/operators.cpp:29-34
+std::vector<Vec3> boundary_rows(const PolyMesh& m, const CellFaces& cf, const std::vector<Vec3>& rows) {
+  std::vector<Vec3> result(m.n_faces() - m.n_internal_faces());
+  for (Index j = 0; j < Index(cf.faces.size()); ++j)
+    if (cf.faces[j] >= m.n_internal_faces()) result[cf.faces[j] - m.n_internal_faces()] = rows[j];
+  return result;
+}
+
+}  // namespace
+
+// fv/operators.cpp:139-164
+std::vector<double> boundary_extrapolation_gain(const PolyMesh& m, const Geometry& g,
+                                                const std::vector<Observation>& obs) {
+  const auto cf = build_cell_faces(m);
+  const auto ni = m.n_internal_faces();
+  std::vector<double> gain(m.n_faces() - ni);
+  std::vector<Vec3> delta;
+  for (Index c = 0; c < m.n_cells; ++c) {
+    double inv[3][3];
+    delta.clear();
+    cell_fit(m, g, cf, obs, c, delta, inv);
+    for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+      const auto f = cf.faces[j];
+      if (f < ni) continue;
+      const auto r = g.face_centroid[f] - g.cell_centroid[c];
+      const Vec3 w{r.x * inv[0][0] + r.y * inv[1][0] + r.z * inv[2][0],
+                   r.x * inv[0][1] + r.y * inv[1][1] + r.z * inv[2][1],
+                   r.x * inv[0][2] + r.y * inv[1][2] + r.z * inv[2][2]};
+      for (const auto& d : delta) gain[f - ni] += std::abs(dot(w, d));
+    }
+  }
+  return gain;
+}
+
+// fv/operators.cpp:166-211 with a stencil: interior rows are the stencil's
+// rows, so only the boundary-face rows and the inverse are stored.
+LeastSquares boundary_least_squares(const PolyMesh& m, const Geometry& g,
+                                    const std::vector<Observation>& obs) {
+  const auto cf = build_cell_faces(m);
+  std::vector<Vec3> delta;
+  LeastSquares fit;
+  fit.inverse.assign(6 * std::size_t(m.n_cells), 0.0f);
+  for (Index c = 0; c < m.n_cells; ++c) {
+    double inv[3][3];
+    cell_fit(m, g, cf, obs, c, delta, inv);
+    const double v[]{inv[0][0], inv[0][1], inv[0][2], inv[1][1], inv[1][2], inv[2][2]};
+    for (int k = 0; k < 6; ++k) fit.inverse[c + std::size_t(k) * m.n_cells] = float(v[k]);
+  }
+  to_float(fit.boundary_delta, boundary_rows(m, cf, delta));
+  return fit;
+}
+
+// mesh/mesh.cpp:172-187
+std::vector<double> orthogonal_quality(const PolyMesh& m, const Geometry& g) {
+  std::vector<double> quality(m.n_cells, 1);
+  const auto cosine = [](Vec3 a, Vec3 b) { return dot(a, b) / (norm(a) * norm(b)); };
+  for (Index f = 0; f < m.n_faces(); ++f) {
+    const auto o = m.owner[f];
+    quality[o] = std::min(quality[o], cosine(g.face_area[f], g.face_centroid[f] - g.cell_centroid[o]));
+    if (f >= m.n_internal_faces()) continue;
+    const auto n = m.neighbour[f];
+    const auto link = g.cell_centroid[n] - g.cell_centroid[o];
+    quality[o] = std::min({quality[o], cosine(g.face_area[f], link)});
+    quality[n] = std::min({quality[n], cosine(g.face_area[f], g.cell_centroid[n] - g.face_centroid[f]),
+                           cosine(g.face_area[f], link)});
+  }
+  for (auto& q : quality) q = std::max(q, 0.);
+  return quality;
+}
+
+// fv/device_mesh.cpp:208-256 (patch_is_wall marks the wall patches).
+DeviceMesh upload_mesh(const PolyMesh& mesh, const Geometry& g, const std::vector<bool>& patch_is_wall) {
+  validate(mesh);
+  DeviceMesh m;
+  m.n_cells = mesh.n_cells;
+  m.n_faces = mesh.n_faces();
+  m.n_internal = mesh.n_internal_faces();
+  m.owner = mesh.owner;
+  m.neighbour = mesh.neighbour;
+  const auto faces = std::size_t(m.n_faces), internal = std::size_t(m.n_internal);
+  to_float(m.area, g.face_area);
+  const auto rp = [&](std::size_t f) { return g.face_centroid[f] - g.cell_centroid[mesh.owner[f]]; };
+  const auto rn = [&](std::size_t f) { return g.face_centroid[f] - g.cell_centroid[mesh.neighbour[f]]; };
+  m.weight.resize(internal);
+  for (std::size_t f = 0; f < internal; ++f) {
+    const double owner_normal = dot(rp(f), g.face_area[f]);
+    const double neighbour_normal = -dot(rn(f), g.face_area[f]);
+    m.weight[f] = float(neighbour_normal / (owner_normal + neighbour_normal));
+  }
+  std::vector<Vec3> a(faces), b(internal), c(faces);
+  for (std::size_t f = 0; f < faces; ++f) {
+    a[f] = rp(f);
+    c[f] = f < internal ? g.cell_centroid[mesh.neighbour[f]] - g.cell_centroid[mesh.owner[f]] : rp(f);
+  }
+  for (std::size_t f = 0; f < internal; ++f) b[f] = rn(f);
+  to_float(m.owner_to_face, a);
+  to_float(m.neighbour_to_face, b);
+  to_float(m.owner_to_neighbour, c);
+  const auto cf = build_cell_faces(mesh);
+  m.offsets = cf.offsets;
+  m.faces = cf.faces;
+  m.sign.assign(cf.signs.begin(), cf.signs.end());
+  m.volume.resize(m.n_cells);
+  for (Index i = 0; i < m.n_cells; ++i) {
+    ORACLE_REQUIRE(std::isfinite(g.cell_volume[i]) && g.cell_volume[i] > 0,
+                   "FV requires finite positive cell volumes");
+    m.volume[i] = float(g.cell_volume[i]);
+  }
+  m.wall_normal_distance.assign(m.n_boundary(), 0.0f);
+  for (std::size_t p = 0; p < mesh.patches.size(); ++p)
+    if (patch_is_wall[p])
+      for (Index f = mesh.patches[p].start; f < mesh.patches[p].start + mesh.patches[p].size; ++f)
+        m.wall_normal_distance[f - m.n_internal] =
+            float(std::abs(dot(rp(std::size_t(f)), g.face_area[f])) / norm(g.face_area[f]));
+  return m;
+}
+
+namespace {
+
+// fv/transport.hpp:9-25
+struct TransportSettings {
+  bool second_order = true, limit = true;
+  float density = 1, pseudo_time_step = 0, relaxation = 1;
+  const V* limiter_low = nullptr;
+  float limiter_epsilon2 = -1;
+};
+
+// fv/transport.cpp:12-160: one equation sharing the pressure operator's
+// values, the face-correction scratch and the zero array; BiCGStab-Jacobi.
+struct TransportEquation {
+  CsrDevice* matrix = nullptr;
+  V rhs, diagonal, consistent_inverse, limiter;
+  Vec3f grad;
+  V* face_correction = nullptr;
+  V* zero = nullptr;
+  std::unique_ptr<Solver> solver;
+};
+
+}  // namespace
+
+class Steady {
+ public:
+  Steady(PolyMesh input, const std::vector<Condition>& conditions, const Settings& s);
+  void step();
+  Report monitor();
+  void iterate(int count) {
+    ORACLE_REQUIRE(count >= 0, "FV negative iteration count");
+    while (count-- > 0) step();
+  }
+  const Report& report() const { return report_; }
+  // Named state arrays, for parity checks.
+  const V* array(const std::string& name) const;
+  Index reference() const { return reference_; }
+  int cells() const { return mesh.n_cells; }
+  const Geometry& geometry() const { return geom; }
+
+ private:
+  void update_time(double maximum_speed);
+  void refresh_time_step();
+  void update_boundaries();
+  void apply_backflow_pressure();
+  void gradient(const LeastSquares& fit, const V& cell, const V& bc, Vec3f& out);
+  void reconstruct_velocity();
+  void update_pressure();
+  void update_flux();
+  void assemble(int k, const TransportSettings& t);
+  void wall_terms(int k);
+  void paired_momentum_residual(int k, V& r);
+  SolveStats solve_transport(int k, V& field, const SolveControl& control);
+  void update_momentum_diagonal();
+  void update_pressure_gradient_lo();
+  void refresh_paired_scales();
+  void finish_step();
+  TransportSettings transport() const;
+  struct Slack {
+    float velocity, pressure, flux_per_area;
+  };
+  Slack paired_slack() const;
+
+  Settings settings;
+  PolyMesh host;
+  Geometry geom;
+  DeviceMesh mesh;
+  Stencil stencil;
+  LeastSquares velocity_lsq, pressure_lsq;
+  CsrDevice matrix;
+  std::unique_ptr<Solver> pressure_solver;
+  TransportEquation momentum[3];
+  V face_correction_scratch, zero;
+  Index reference_ = -1;
+  bool paired = false;
+
+  Vec3f velocity, face_velocity, body, source, boundary_velocity, prescribed_velocity;
+  Vec3f limited_gradient[3], grad_p, grad_correction, absolute_residual, fluent_scale;
+  V pressure, mass, viscosity, boundary_pressure, prescribed_pressure;
+  VI boundary_kind, scalar_kind, total_pressure;
+  V pressure_rhs, correction, correction_boundary, coefficient, face_pressure, pressure_jump;
+  V residual_u, residual_v, residual_w, continuity, provisional_continuity, scale, speed;
+  V wall_diagonal, wall_rhs, physical_diagonal, consistent_inverse;
+  Vec3f velocity_lo, pressure_gradient_lo;
+  V pressure_lo, mass_lo, previous_mass, pressure_boundary_lo, imbalance, momentum_delta;
+
+  float limiter_epsilon2[3]{-1, -1, -1};
+  double paired_speed = 0, paired_pressure_magnitude = 0;
+  double length = 0, bc_speed = 0, pressure_speed = 0, continuity_scale = 0;
+  double first_five_continuity[5]{};
+  float time_step = 0;
+  double velocity_scale = 1;
+  Report report_;
+};
+
+// fv/steady_solver.cpp:203-597, the laminar closed-domain subset.
+Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const Settings& s)
+    : settings(s), host(std::move(input)) {
+  ORACLE_REQUIRE(s.density > 0 && std::isfinite(s.density) && s.viscosity > 0 &&
+                     std::isfinite(s.viscosity) && s.momentum_relaxation > 0 &&
+                     s.momentum_relaxation <= 1 && s.pressure_relaxation > 0 &&
+                     s.pressure_relaxation <= 1 && s.pseudo_time_scale > 0 &&
+                     s.monitor_interval > 0 && std::isfinite(s.paired_state_slack_ulps) &&
+                     s.paired_state_slack_ulps > 0,
+                 "FV invalid steady solver settings");
+  ORACLE_REQUIRE(conditions.size() == host.patches.size(), "FV supply exactly one condition per patch");
+  std::vector<bool> patch_is_wall;
+  for (const auto& c : conditions) {
+    ORACLE_REQUIRE(c.kind == 0 || c.kind == 3, "SIMPLEC oracle: only wall and symmetry patches");
+    patch_is_wall.push_back(c.kind == 0);
+  }
+  renumber_cells_rcm(host);
+  geom = compute_geometry(host);
+  ORACLE_REQUIRE(host.n_cells > 0, "FV steady solver needs a nonempty mesh");
+  reference_ = locate_cell(host, geom, s.reference_point);
+  {
+    const auto nb = build_cell_neighbours(host);
+    std::vector<bool> seen(host.n_cells);
+    std::vector<Index> stack{0};
+    seen[0] = true;
+    for (std::size_t k = 0; k < stack.size(); ++k)
+      for (Index j = nb.offsets[stack[k]]; j < nb.offsets[stack[k] + 1]; ++j)
+        if (!seen[nb.cells[j]]) {
+          seen[nb.cells[j]] = true;
+          stack.push_back(nb.cells[j]);
+        }
+    ORACLE_REQUIRE(stack.size() == std::size_t(host.n_cells), "FV disconnected domains require separate solvers");
+  }
+  mesh = upload_mesh(host, geom, patch_is_wall);
+  stencil = make_stencil(host, geom);
+  const auto n = std::size_t(mesh.n_cells), nf = std::size_t(mesh.n_faces), nb = std::size_t(mesh.n_boundary());
+  velocity.assign(n);
+  face_velocity.assign(nf);
+  body.assign(n);
+  source.assign(n);
+  boundary_velocity.assign(nb);
+  prescribed_velocity.assign(nb);
+  for (auto& g : limited_gradient) g.assign(n);
+  grad_p.assign(n);
+  grad_correction.assign(n);
+  absolute_residual.assign(n);
+  fluent_scale.assign(n);
+  pressure.assign(n, 0.0f);
+  mass.assign(nf, 0.0f);
+  viscosity.assign(nf, float(s.viscosity));
+  boundary_pressure.assign(nb, 0.0f);
+  prescribed_pressure.assign(nb, 0.0f);
+  boundary_kind.assign(nb, 0);
+  scalar_kind.assign(nb, 0);
+  pressure_rhs.assign(n, 0.0f);
+  correction.assign(n, 0.0f);
+  correction_boundary.assign(nb, 0.0f);
+  coefficient.assign(nf, 0.0f);
+  face_pressure.assign(nf, 0.0f);
+  pressure_jump.assign(nf, 0.0f);
+  for (V* v : {&residual_u, &residual_v, &residual_w, &continuity, &provisional_continuity, &scale, &speed,
+               &wall_diagonal, &wall_rhs})
+    v->assign(n, 0.0f);
+  paired = s.paired_state;
+  if (paired) {
+    velocity_lo.assign(n);
+    pressure_gradient_lo.assign(n);
+    pressure_lo.assign(n, 0.0f);
+    mass_lo.assign(nf, 0.0f);
+    previous_mass.assign(nf, 0.0f);
+    pressure_boundary_lo.assign(nb, 0.0f);
+    imbalance.assign(n, 0.0f);
+    momentum_delta.assign(n, 0.0f);
+  }
+  total_pressure.assign(nb, 0);
+  physical_diagonal.assign(n, 1.0f);
+  consistent_inverse.assign(n, 0.0f);
+
+  // Boundary kinds and prescribed values (:314-371).
+  std::vector<Index> kinds(nb);
+  std::vector<Vec3> velocities(nb);
+  for (std::size_t p = 0; p < host.patches.size(); ++p)
+    for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+      const auto b = f - mesh.n_internal;
+      if (conditions[p].kind == 0) {
+        kinds[b] = 0;
+        velocities[b] = conditions[p].velocity;
+      } else {
+        kinds[b] = 3;
+      }
+      bc_speed = std::max(bc_speed, norm(velocities[b]));
+    }
+  pressure_speed = 0;  // closed domain
+  std::vector<Observation> independent_velocity(nb), independent_pressure(nb);
+  for (std::size_t b = 0; b < nb; ++b) {
+    independent_velocity[b] = kinds[b] == 3 ? Observation::NormalFootValue
+                              : (kinds[b] == 0 || kinds[b] == 1) ? Observation::FaceValue
+                                                                 : Observation::Omitted;
+    independent_pressure[b] = (kinds[b] == 2 || kinds[b] == 4) ? Observation::FaceValue : Observation::Omitted;
+  }
+  boundary_kind.assign(kinds.begin(), kinds.end());
+  prescribed_velocity.assign(nb);
+  for (std::size_t b = 0; b < nb; ++b) {
+    prescribed_velocity.x[b] = float(velocities[b].x);
+    prescribed_velocity.y[b] = float(velocities[b].y);
+    prescribed_velocity.z[b] = float(velocities[b].z);
+  }
+  matrix = upload(stencil.pattern);
+  matrix.val.assign(matrix.col.size(), 0.0f);
+
+  // The projected face reconstruction corrects only faces whose cell-to-face
+  // vectors leave the face normal (face_projection.cpp:136-150); this
+  // restatement covers meshes where it corrects none.
+  {
+    const auto internal = mesh.n_internal;
+    for (Index face = 0; face < mesh.n_faces; ++face) {
+      if (face >= internal && independent_velocity[face - internal] != Observation::FaceValue) continue;
+      const auto owner = host.owner[face];
+      const auto area = geom.face_area[face], unit = area * (1 / norm(area));
+      const auto rp = geom.face_centroid[face] - geom.cell_centroid[owner];
+      const auto rn = face < internal ? geom.cell_centroid[host.neighbour[face]] - geom.face_centroid[face] : Vec3{};
+      const double dp = dot(rp, unit), dn = dot(rn, unit);
+      const double roundoff = 256 * std::numeric_limits<double>::epsilon() *
+                              std::max({1., norm(geom.face_centroid[face]), norm(geom.cell_centroid[owner]),
+                                        face < internal ? norm(geom.cell_centroid[host.neighbour[face]]) : 0.});
+      ORACLE_REQUIRE(norm(rp - unit * dp) <= roundoff && norm(rn - unit * dn) <= roundoff,
+                     "SIMPLEC oracle: meshes needing the projected face reconstruction are not covered");
+    }
+  }
+  face_correction_scratch.assign(nf, 0.0f);
+  zero.assign(n, 0.0f);
+  CsrHost identity = stencil.pattern;
+  identity.val.assign(identity.col.size(), 0.0);
+  for (auto j : identity.diag) identity.val[j] = 1;
+  for (auto& eq : momentum) {
+    eq.matrix = &matrix;
+    eq.rhs.assign(n, 0.0f);
+    eq.diagonal.assign(n, 0.0f);
+    eq.consistent_inverse.assign(n, 0.0f);
+    eq.limiter.assign(n, 0.0f);
+    eq.grad.assign(n);
+    eq.face_correction = &face_correction_scratch;
+    eq.zero = &zero;
+    eq.solver = std::make_unique<Solver>(SolverKind::BicgstabJacobi, identity, AmgOptions{});
+  }
+  velocity_lsq = boundary_least_squares(host, geom, independent_velocity);
+  {
+    const auto gain = boundary_extrapolation_gain(host, geom, independent_pressure);
+    for (std::size_t b = 0; b < nb; ++b)
+      if (independent_pressure[b] == Observation::Omitted && gain[b] > 1)
+        independent_pressure[b] = Observation::FaceValue;
+  }
+  pressure_lsq = boundary_least_squares(host, geom, independent_pressure);
+  {
+    const auto quality = orthogonal_quality(host, geom);
+    for (Index c = 0; c < mesh.n_cells; ++c)
+      ORACLE_REQUIRE(quality[c] > .05, "SIMPLEC oracle: poor-quality cells are not covered");
+  }
+  // The initial pressure operator (:553-572).
+  CsrHost initial = stencil.pattern;
+  initial.val.assign(initial.col.size(), 0.0);
+  {
+    const auto cf = build_cell_faces(host);
+    for (Index c = 0; c < mesh.n_cells; ++c) {
+      UpperSum ap{0, 0};
+      for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+        const auto f = cf.faces[j];
+        const auto other = f < mesh.n_internal ? (host.owner[f] == c ? host.neighbour[f] : host.owner[f]) : -1;
+        if (other < 0 && kinds[f - mesh.n_internal] != 2 && kinds[f - mesh.n_internal] != 4) continue;
+        const auto d = (other >= 0 ? geom.cell_centroid[other] : geom.face_centroid[f]) - geom.cell_centroid[c];
+        const float a = float(dot(geom.face_area[f], geom.face_area[f]) / std::abs(dot(geom.face_area[f], d)));
+        if (other < 0 || other == reference_) ap = kupper_add(ap, a);
+        if (other >= 0 && other != reference_ && c != reference_) {
+          const auto begin = initial.col.begin() + initial.row_ptr[c], end = initial.col.begin() + initial.row_ptr[c + 1];
+          const auto index = std::lower_bound(begin, end, other) - initial.col.begin();
+          initial.val[index] = float(float(initial.val[index]) - a);
+        }
+      }
+      for (Index j = initial.row_ptr[c]; j < initial.row_ptr[c + 1]; ++j)
+        if (j != initial.diag[c]) ap = kupper_add(ap, -float(initial.val[j]));
+      initial.val[initial.diag[c]] = c == reference_ ? 1 : kupper_value(ap);
+    }
+  }
+  AmgOptions amg;
+  amg.structure = Structure::SymmetricMMatrix;
+  pressure_solver = std::make_unique<Solver>(SolverKind::AmgPcg, initial, amg);
+  const double volume = std::accumulate(geom.cell_volume.begin(), geom.cell_volume.end(), 0.);
+  Vec3 lo = host.points.front(), hi = lo;
+  for (auto p : host.points) {
+    lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+    hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+  }
+  length = std::min(std::cbrt(volume), std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z}));
+  update_time(bc_speed);
+  std::fill(momentum[0].diagonal.begin(), momentum[0].diagonal.end(), 1.0f);
+  std::fill(momentum[0].consistent_inverse.begin(), momentum[0].consistent_inverse.end(), 0.0f);
+  update_boundaries();
+  update_pressure();
+  update_flux();
+}
+
+// :843-851 (segregated: the maximum cell speed is the velocity scale).
+void Steady::update_time(double maximum_speed) {
+  velocity_scale = std::max({.1 * bc_speed, maximum_speed, pressure_speed, 1e-10});
+  time_step = float(.3 * length * settings.pseudo_time_scale / velocity_scale);
+}
+
+// :857-873, automatic step, no ramp, Barth-Jespersen limiter.
+void Steady::refresh_time_step() {
+  fv("fv_speed", mesh.n_cells, velocity.x.data(), velocity.y.data(), velocity.z.data(), speed.data());
+  const auto maximum = max_abs(speed.data(), speed.size());
+  update_time(maximum);
+}
+
+// :874-885
+void Steady::update_boundaries() {
+  fv("fv_velocity_boundary", mesh.n_boundary(), mesh.owner.data(), boundary_kind.data(), mesh.area.x.data(),
+     mesh.area.y.data(), mesh.area.z.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(),
+     prescribed_velocity.x.data(), prescribed_velocity.y.data(), prescribed_velocity.z.data(), mass.data(),
+     boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(), scalar_kind.data(),
+     mesh.n_internal, float(settings.density));
+}
+
+// :599-606
+void Steady::apply_backflow_pressure() {
+  fv("fv_backflow_pressure", mesh.n_boundary(), mesh.owner.data(), total_pressure.data(), mass.data(),
+     velocity.x.data(), velocity.y.data(), velocity.z.data(), boundary_pressure.data(), mesh.n_internal,
+     float(settings.density));
+}
+
+// fv/operators.cpp:296-302
+void Steady::gradient(const LeastSquares& fit, const V& cell, const V& bc, Vec3f& out) {
+  fv("fv_gradient", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), stencil.lsq_x.data(),
+     stencil.lsq_y.data(), stencil.lsq_z.data(), fit.boundary_delta.x.data(), fit.boundary_delta.y.data(),
+     fit.boundary_delta.z.data(), fit.inverse.data(), cell.data(), bc.data(), out.x.data(), out.y.data(),
+     out.z.data(), mesh.n_cells);
+}
+
+// :886-889 with fv/operators.cpp:311-318
+void Steady::reconstruct_velocity() {
+  auto& g0 = momentum[0].grad;
+  auto& g1 = momentum[1].grad;
+  auto& g2 = momentum[2].grad;
+  const auto& fit = velocity_lsq;
+  fv("fv_vector_gradient", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), stencil.lsq_x.data(),
+     stencil.lsq_y.data(), stencil.lsq_z.data(), fit.boundary_delta.x.data(), fit.boundary_delta.y.data(),
+     fit.boundary_delta.z.data(), fit.inverse.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(),
+     boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(), g0.x.data(),
+     g0.y.data(), g0.z.data(), g1.x.data(), g1.y.data(), g1.z.data(), g2.x.data(), g2.y.data(), g2.z.data(),
+     mesh.n_cells);
+}
+
+// :896-925, second-order pressure interpolation with no poor cells.
+void Steady::update_pressure() {
+  const auto boundary = [&](const V& p, Vec3f& g, const V& prescribed, V& out) {
+    fv("fv_pressure_boundary", mesh.n_boundary(), mesh.owner.data(), boundary_kind.data(),
+       mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(), p.data(),
+       g.x.data(), g.y.data(), g.z.data(), prescribed.data(), out.data(), mesh.n_internal);
+  };
+  boundary(pressure, grad_p, prescribed_pressure, boundary_pressure);
+  fv("fv_cell_pressure_boundary", mesh.n_boundary(), mesh.owner.data(), boundary_kind.data(), pressure.data(),
+     boundary_pressure.data(), mesh.n_internal);
+  apply_backflow_pressure();
+  gradient(pressure_lsq, pressure, boundary_pressure, grad_p);
+  boundary(pressure, grad_p, prescribed_pressure, boundary_pressure);
+  apply_backflow_pressure();
+  fv("fv_pressure_faces", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), mesh.owner_to_face.x.data(),
+     mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(), mesh.neighbour_to_face.x.data(),
+     mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(), pressure.data(), grad_p.x.data(),
+     grad_p.y.data(), grad_p.z.data(), boundary_pressure.data(), face_pressure.data(), pressure_jump.data(),
+     mesh.n_internal);
+}
+
+// :943-961
+Steady::Slack Steady::paired_slack() const {
+  const double ulps = settings.paired_state_slack_ulps * double(std::numeric_limits<float>::epsilon());
+  const double speed_scale = std::max(bc_speed, paired_speed);
+  const double pressure_scale = std::max(settings.density * speed_scale * speed_scale, paired_pressure_magnitude);
+  return {float(ulps * speed_scale), float(ulps * pressure_scale), float(ulps * settings.density * speed_scale)};
+}
+
+// :1139-1248, limited, paired or float, momentum-weighted faces.
+void Steady::update_flux() {
+  if (paired) previous_mass = mass;
+  reconstruct_velocity();
+  for (int k = 0; k < 3; ++k) {
+    auto& eq = momentum[k];
+    auto& g = limited_gradient[k];
+    if (paired)
+      fv("fv_paired_limiter_scale", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), mesh.faces.data(),
+         mesh.sign.data(), mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+         mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+         velocity[k].data(), velocity_lo[k].data(), boundary_velocity[k].data(), scalar_kind.data(),
+         eq.grad.x.data(), eq.grad.y.data(), eq.grad.z.data(), eq.limiter.data(), g.x.data(), g.y.data(),
+         g.z.data(), limiter_epsilon2[k]);
+    else
+      fv("fv_limiter_scale", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), mesh.faces.data(),
+         mesh.sign.data(), mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+         mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+         velocity[k].data(), boundary_velocity[k].data(), eq.grad.x.data(), eq.grad.y.data(), eq.grad.z.data(),
+         eq.limiter.data(), g.x.data(), g.y.data(), g.z.data(), limiter_epsilon2[k]);
+  }
+  const auto& gu = limited_gradient[0];
+  const auto& gv = limited_gradient[1];
+  const auto& gw = limited_gradient[2];
+  fv("fv_momentum_faces", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), boundary_kind.data(),
+     mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+     mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+     velocity.x.data(), velocity.y.data(), velocity.z.data(), gu.x.data(), gu.y.data(), gu.z.data(),
+     gv.x.data(), gv.y.data(), gv.z.data(), gw.x.data(), gw.y.data(), gw.z.data(), physical_diagonal.data(),
+     boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(),
+     face_velocity.x.data(), face_velocity.y.data(), face_velocity.z.data(), mesh.n_internal);
+  fv("fv_rhie_chow", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), boundary_kind.data(),
+     mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), stencil.diffusion.data(), mesh.volume.data(),
+     physical_diagonal.data(), consistent_inverse.data(), face_velocity.x.data(), face_velocity.y.data(),
+     face_velocity.z.data(), pressure_jump.data(), mass.data(), coefficient.data(), mesh.n_internal,
+     float(settings.density));
+  if (paired)
+    fv("fv_paired_flux", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), boundary_kind.data(),
+       mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), mesh.owner_to_face.x.data(),
+       mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(), mesh.neighbour_to_face.x.data(),
+       mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(), physical_diagonal.data(),
+       physical_diagonal.data(), mesh.volume.data(), stencil.diffusion.data(), velocity_lo.x.data(),
+       velocity_lo.y.data(), velocity_lo.z.data(), pressure_lo.data(), pressure_gradient_lo.x.data(),
+       pressure_gradient_lo.y.data(), pressure_gradient_lo.z.data(), previous_mass.data(), mass.data(),
+       mass_lo.data(), mesh.n_internal, float(settings.density), paired_slack().flux_per_area);
+}
+
+// :1250-1256 (SIMPLEC: relaxation is the momentum relaxation).
+TransportSettings Steady::transport() const {
+  TransportSettings t;
+  t.density = float(settings.density);
+  t.pseudo_time_step = time_step;
+  t.relaxation = float(settings.momentum_relaxation);
+  return t;
+}
+
+// :765-776 (laminar: the coefficient argument is an unused placeholder).
+void Steady::wall_terms(int k) {
+  fv("fv_boundary_momentum", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), boundary_kind.data(),
+     mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), prescribed_pressure.data(), viscosity.data(),
+     stencil.diffusion.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(),
+     boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(), wall_diagonal.data(),
+     wall_rhs.data(), mesh.n_internal, k, 0);
+}
+
+// fv/transport.cpp:83-121 with limit_gradient (fv/operators.cpp:345-362) and
+// with_low (steady_solver.cpp:963-976): component k of momentum.
+void Steady::assemble(int k, const TransportSettings& t) {
+  auto& eq = momentum[k];
+  const V& field = velocity[k];
+  const V& bc = boundary_velocity[k];
+  const V& src = source[k];
+  if (t.limiter_low)
+    fv("fv_paired_limiter", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), mesh.faces.data(),
+       mesh.sign.data(), mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+       mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+       field.data(), t.limiter_low->data(), bc.data(), scalar_kind.data(), eq.grad.x.data(), eq.grad.y.data(),
+       eq.grad.z.data(), eq.limiter.data(), t.limiter_epsilon2);
+  else
+    fv("fv_limiter", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), mesh.faces.data(),
+       mesh.sign.data(), mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+       mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+       field.data(), bc.data(), eq.grad.x.data(), eq.grad.y.data(), eq.grad.z.data(), eq.limiter.data(),
+       t.limiter_epsilon2);
+  auto& correction_faces = *eq.face_correction;
+  fv("fv_transport_faces", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), mesh.weight.data(),
+     mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+     mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+     stencil.nonorth_x.data(), stencil.nonorth_y.data(), stencil.nonorth_z.data(), eq.grad.x.data(),
+     eq.grad.y.data(), eq.grad.z.data(), eq.limiter.data(), mass.data(), viscosity.data(), scalar_kind.data(),
+     correction_faces.data(), mesh.n_internal, t.second_order ? 1 : 0, 0);
+  fv("fv_transport_assemble", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+     stencil.adjacent.data(), stencil.matrix_entry.data(), matrix.row_ptr.data(), matrix.diag.data(),
+     stencil.diffusion.data(), mass.data(), viscosity.data(), scalar_kind.data(), bc.data(),
+     correction_faces.data(), src.data(), mesh.volume.data(), field.data(), eq.zero->data(),
+     wall_diagonal.data(), wall_rhs.data(), matrix.val.data(), eq.rhs.data(), eq.zero->data(),
+     eq.diagonal.data(), eq.zero->data(), eq.consistent_inverse.data(), mesh.n_internal, t.density,
+     t.pseudo_time_step, t.relaxation, 0);
+}
+
+// :1045-1060 (legacy paired mode: no compensated assembly).
+void Steady::paired_momentum_residual(int k, V& r) {
+  auto& eq = momentum[k];
+  fv("fv_paired_momentum_residual", mesh.n_cells, matrix.row_ptr.data(), matrix.col.data(), matrix.val.data(),
+     matrix.diag.data(), eq.diagonal.data(), eq.zero->data(), velocity[k].data(), velocity_lo[k].data(),
+     pressure_gradient_lo[k].data(), mesh.volume.data(), eq.rhs.data(), eq.zero->data(), r.data());
+}
+
+// fv/transport.cpp:123-159 (Jacobi preconditioner, no steady stopping).
+SolveStats Steady::solve_transport(int k, V& field, const SolveControl& control) {
+  auto& eq = momentum[k];
+  eq.solver->refresh(matrix);
+  return eq.solver->solve(matrix, field, eq.rhs, control);
+}
+
+// :607-614
+void Steady::update_momentum_diagonal() {
+  fv("fv_momentum_diagonal", mesh.n_cells, momentum[0].diagonal.data(), momentum[1].diagonal.data(),
+     momentum[2].diagonal.data(), momentum[0].consistent_inverse.data(), momentum[1].consistent_inverse.data(),
+     momentum[2].consistent_inverse.data(), physical_diagonal.data(), consistent_inverse.data());
+}
+
+// :1026-1034
+void Steady::update_pressure_gradient_lo() {
+  auto& g = pressure_gradient_lo;
+  g.assign(mesh.n_cells);
+  fv("fv_pressure_boundary", mesh.n_boundary(), mesh.owner.data(), boundary_kind.data(),
+     mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(), pressure_lo.data(),
+     g.x.data(), g.y.data(), g.z.data(), correction_boundary.data(), pressure_boundary_lo.data(),
+     mesh.n_internal);
+  gradient(pressure_lsq, pressure_lo, pressure_boundary_lo, g);
+}
+
+// :943-949
+void Steady::refresh_paired_scales() {
+  if (!paired) return;
+  fv("fv_speed", mesh.n_cells, velocity.x.data(), velocity.y.data(), velocity.z.data(), speed.data());
+  paired_speed = max_abs(speed.data(), speed.size());
+  paired_pressure_magnitude = max_abs(pressure.data(), pressure.size());
+}
+
+// :950-954
+void Steady::finish_step() {
+  ++report_.iteration;
+  if (report_.iteration <= 5 || report_.iteration % 20 == 0) refresh_paired_scales();
+  if (report_.iteration <= 5 || report_.iteration % settings.monitor_interval == 0) monitor();
+}
+
+// :1303-1369
+void Steady::step() {
+  refresh_time_step();
+  update_boundaries();
+  update_pressure();
+  fv("fv_momentum_source", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+     mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), face_pressure.data(), mesh.volume.data(),
+     body.x.data(), body.y.data(), body.z.data(), source.x.data(), source.y.data(), source.z.data());
+  reconstruct_velocity();
+  auto t = transport();
+  for (int k = 0; k < 3; ++k) {
+    t.limiter_epsilon2 = limiter_epsilon2[k];
+    t.limiter_low = paired ? &velocity_lo[k] : nullptr;
+    wall_terms(k);
+    assemble(k, t);
+    if (!paired) {
+      report_.momentum[k] = solve_transport(k, velocity[k], settings.momentum_linear);
+      continue;
+    }
+    paired_momentum_residual(k, residual_u);
+    momentum[k].rhs = residual_u;
+    std::fill(momentum_delta.begin(), momentum_delta.end(), 0.0f);
+    report_.momentum[k] = solve_transport(k, momentum_delta, settings.momentum_linear);
+    fv("fv_paired_accumulate", mesh.n_cells, momentum_delta.data(), velocity[k].data(), velocity_lo[k].data(),
+       paired_slack().velocity);
+  }
+  update_momentum_diagonal();
+  update_boundaries();
+  update_flux();
+  if (paired)
+    fv("fv_paired_continuity", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+       mass.data(), mass_lo.data(), imbalance.data(), provisional_continuity.data());
+  else
+    fv("fv_continuity", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(), mass.data(),
+       provisional_continuity.data());
+  fv_pressure_assemble(mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+                       stencil.adjacent.data(), stencil.matrix_entry.data(), matrix.row_ptr.data(),
+                       matrix.diag.data(), coefficient.data(), mass.data(), matrix.val.data(),
+                       pressure_rhs.data(), reference_);
+  if (paired) fv("fv_paired_pressure_rhs", mesh.n_cells, imbalance.data(), pressure_rhs.data(), reference_);
+  pressure_solver->refresh(matrix);
+  std::fill(correction.begin(), correction.end(), 0.0f);
+  report_.pressure = pressure_solver->solve(matrix, correction, pressure_rhs, settings.pressure_linear);
+  if (paired)
+    fv("fv_paired_correct_flux", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), mesh.area.x.data(),
+       mesh.area.y.data(), mesh.area.z.data(), correction.data(), coefficient.data(), mass.data(),
+       mass_lo.data(), mesh.n_internal, paired_slack().flux_per_area);
+  else
+    fv("fv_correct_flux", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), correction.data(),
+       coefficient.data(), mass.data(), mesh.n_internal);
+  grad_correction.assign(mesh.n_cells);
+  fv("fv_pressure_boundary", mesh.n_boundary(), mesh.owner.data(), boundary_kind.data(),
+     mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(), correction.data(),
+     grad_correction.x.data(), grad_correction.y.data(), grad_correction.z.data(), correction_boundary.data(),
+     boundary_pressure.data(), mesh.n_internal);
+  gradient(pressure_lsq, correction, boundary_pressure, grad_correction);
+  if (paired) {
+    const auto slack = paired_slack();
+    fv("fv_paired_correct_velocity", mesh.n_cells, mesh.volume.data(), consistent_inverse.data(),
+       physical_diagonal.data(), grad_correction.x.data(), grad_correction.y.data(), grad_correction.z.data(),
+       correction.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(), pressure.data(),
+       velocity_lo.x.data(), velocity_lo.y.data(), velocity_lo.z.data(), pressure_lo.data(),
+       float(settings.pressure_relaxation), slack.velocity, slack.pressure);
+    update_pressure_gradient_lo();
+  } else {
+    fv("fv_correct_velocity", mesh.n_cells, mesh.volume.data(), consistent_inverse.data(),
+       physical_diagonal.data(), grad_correction.x.data(), grad_correction.y.data(), grad_correction.z.data(),
+       correction.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(), pressure.data(),
+       float(settings.pressure_relaxation));
+  }
+  finish_step();
+}
+
+// :1417-1477
+Report Steady::monitor() {
+  refresh_time_step();
+  update_boundaries();
+  update_pressure();
+  fv("fv_momentum_source", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+     mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), face_pressure.data(), mesh.volume.data(),
+     body.x.data(), body.y.data(), body.z.data(), source.x.data(), source.y.data(), source.z.data());
+  reconstruct_velocity();
+  auto t = transport();
+  V* residuals[]{&residual_u, &residual_v, &residual_w};
+  for (int k = 0; k < 3; ++k) {
+    t.limiter_epsilon2 = limiter_epsilon2[k];
+    t.limiter_low = paired ? &velocity_lo[k] : nullptr;
+    wall_terms(k);
+    assemble(k, t);
+    if (paired)
+      paired_momentum_residual(k, *residuals[k]);
+    else
+      kern::csr_residual(mesh.n_cells, matrix.row_ptr.data(), matrix.col.data(), matrix.val.data(),
+                         velocity[k].data(), momentum[k].rhs.data(), residuals[k]->data());
+  }
+  update_momentum_diagonal();
+  fv("fv_monitor_cell", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(), mass.data(),
+     velocity.x.data(), velocity.y.data(), velocity.z.data(), physical_diagonal.data(), residual_u.data(),
+     residual_v.data(), residual_w.data(), continuity.data(), scale.data(), speed.data(),
+     absolute_residual.x.data(), absolute_residual.y.data(), absolute_residual.z.data());
+  if (paired)
+    fv("fv_paired_continuity", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+       mass.data(), mass_lo.data(), imbalance.data(), continuity.data());
+  fv("fv_fluent_momentum_scale", mesh.n_cells, momentum[0].diagonal.data(), momentum[1].diagonal.data(),
+     momentum[2].diagonal.data(), mesh.volume.data(), speed.data(), fluent_scale.x.data(), fluent_scale.y.data(),
+     fluent_scale.z.data(), float(settings.density), time_step);
+  const std::size_t n = mesh.n_cells;
+  const double denominator = sum(scale.data(), n);
+  const double raw = sum(provisional_continuity.data(), n);
+  report_.corrected_mass_imbalance = sum(continuity.data(), n);
+  if (report_.iteration > 0 && report_.iteration <= 5) first_five_continuity[report_.iteration - 1] = raw;
+  continuity_scale = *std::max_element(std::begin(first_five_continuity), std::end(first_five_continuity));
+  const auto scaled = [](double r, double d) {
+    return d > 0 ? r / d : (r == 0 ? 0 : std::numeric_limits<double>::infinity());
+  };
+  ResidualValue* values[]{&report_.u, &report_.v, &report_.w};
+  for (int k = 0; k < 3; ++k) {
+    values[k]->raw = sum(absolute_residual[k].data(), n);
+    values[k]->scaled = scaled(values[k]->raw, denominator);
+    values[k]->fluent_normalization = sum(fluent_scale[k].data(), n);
+    values[k]->fluent_scaled = scaled(values[k]->raw, values[k]->fluent_normalization);
+  }
+  report_.continuity = {raw, scaled(raw, continuity_scale), scaled(raw, continuity_scale), continuity_scale};
+  report_.maximum_velocity = max_abs(speed.data(), n);
+  report_.pseudo_time_step = time_step;
+  return report_;
+}
+
+const V* Steady::array(const std::string& name) const {
+  const std::map<std::string, const V*> table = {
+      {"u", &velocity.x},         {"v", &velocity.y},          {"w", &velocity.z},
+      {"p", &pressure},           {"mass", &mass},             {"u_lo", &velocity_lo.x},
+      {"v_lo", &velocity_lo.y},   {"w_lo", &velocity_lo.z},    {"p_lo", &pressure_lo},
+      {"mass_lo", &mass_lo},      {"correction", &correction}, {"coefficient", &coefficient},
+      {"face_pressure", &face_pressure}, {"boundary_u", &boundary_velocity.x},
+      {"boundary_pressure", &boundary_pressure}, {"physical_diagonal", &physical_diagonal},
+      {"consistent_inverse", &consistent_inverse}, {"grad_px", &grad_p.x},
+      {"face_u", &face_velocity.x}, {"matrix_values", &matrix.val}, {"pressure_rhs", &pressure_rhs},
+      {"volume", &mesh.volume}, {"weight", &mesh.weight}, {"owner_to_face_x", &mesh.owner_to_face.x},
+      {"neighbour_to_face_y", &mesh.neighbour_to_face.y}, {"owner_to_neighbour_z", &mesh.owner_to_neighbour.z},
+      {"area_x", &mesh.area.x}, {"wall_normal_distance", &mesh.wall_normal_distance},
+      {"velocity_lsq_inverse", &velocity_lsq.inverse}, {"pressure_lsq_inverse", &pressure_lsq.inverse},
+      {"velocity_lsq_boundary_x", &velocity_lsq.boundary_delta.x},
+      {"pressure_lsq_boundary_y", &pressure_lsq.boundary_delta.y}};
+  const auto it = table.find(name);
+  return it == table.end() ? nullptr : it->second;
+}
+
+}  // namespace oracle::simplec
+
+namespace {
+thread_local std::string g_simplec_error;
+}
+
+extern "C" {
+const char* oracle_simplec_last_error() { return g_simplec_error.c_str(); }
+
+// A box (nx, ny, nz; hex) on [0,1]^3 with one condition per patch in the
+// generator's order xmin, xmax, ymin, ymax, zmin, zmax: kind[p] 0 wall with
+// velocity wall[3p..3p+2], 3 symmetry. settings: density, viscosity,
+// momentum relaxation, pressure relaxation, pseudo-time scale, monitor
+// interval, paired state (0/1), slack ulps, reference point x, y, z.
+void* oracle_simplec_create(int nx, int ny, int nz, const int* kind, const double* wall,
+                            const double* settings) {
+  try {
+    using namespace oracle;
+    using namespace oracle::simplec;
+    auto mesh = make_box(nx, ny, nz, {0, 0, 0}, {1, 1, 1}, false);
+    std::vector<Condition> conditions(mesh.patches.size());
+    for (std::size_t p = 0; p < conditions.size(); ++p)
+      conditions[p] = {kind[p], {wall[3 * p], wall[3 * p + 1], wall[3 * p + 2]}};
+    Settings s;
+    s.density = settings[0];
+    s.viscosity = settings[1];
+    s.momentum_relaxation = settings[2];
+    s.pressure_relaxation = settings[3];
+    s.pseudo_time_scale = settings[4];
+    s.monitor_interval = int(settings[5]);
+    s.paired_state = settings[6] != 0;
+    s.paired_state_slack_ulps = settings[7];
+    s.reference_point = {settings[8], settings[9], settings[10]};
+    return new Steady(std::move(mesh), conditions, s);
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return nullptr;
+  }
+}
+void oracle_simplec_free(void* h) { delete static_cast<oracle::simplec::Steady*>(h); }
+int oracle_simplec_iterate(void* h, int count) {
+  try {
+    static_cast<oracle::simplec::Steady*>(h)->iterate(count);
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
+  }
+}
+// Report layout: iteration, u raw/scaled/fluent scaled/normalization, v (4),
+// w (4), continuity (4), corrected mass imbalance, maximum velocity, pseudo
+// time step, momentum iterations x3, pressure iterations, pressure final
+// residual, pressure converged.
+int oracle_simplec_report(void* h, int run_monitor, double* out) {
+  try {
+    auto* s = static_cast<oracle::simplec::Steady*>(h);
+    const auto r = run_monitor ? s->monitor() : s->report();
+    int k = 0;
+    out[k++] = r.iteration;
+    for (const auto* v : {&r.u, &r.v, &r.w, &r.continuity}) {
+      out[k++] = v->raw;
+      out[k++] = v->scaled;
+      out[k++] = v->fluent_scaled;
+      out[k++] = v->fluent_normalization;
+    }
+    out[k++] = r.corrected_mass_imbalance;
+    out[k++] = r.maximum_velocity;
+    out[k++] = r.pseudo_time_step;
+    for (const auto& m : r.momentum) out[k++] = m.iterations;
+    out[k++] = r.pressure.iterations;
+    out[k++] = r.pressure.final_residual;
+    out[k++] = r.pressure.converged;
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
+  }
+}
+// Copies array `name` into out (if non-null); returns its length, -1 if unknown.
+long oracle_simplec_array(void* h, const char* name, float* out) {
+  const auto* v = static_cast<oracle::simplec::Steady*>(h)->array(name);
+  if (!v) return -1;
+  if (out) std::memcpy(out, v->data(), v->size() * sizeof(float));
+  return long(v->size());
+}
+int oracle_simplec_reference(void* h) { return static_cast<oracle::simplec::Steady*>(h)->reference(); }
+// Cell centroids of the renumbered mesh, x y z per cell (3 * cells doubles).
+int oracle_simplec_centroids(void* h, double* out) {
+  const auto& g = static_cast<oracle::simplec::Steady*>(h)->geometry();
+  for (std::size_t c = 0; c < g.cell_centroid.size(); ++c) {
+    out[3 * c] = g.cell_centroid[c].x;
+    out[3 * c + 1] = g.cell_centroid[c].y;
+    out[3 * c + 2] = g.cell_centroid[c].z;
+  }
+  return int(g.cell_centroid.size());
+}
+}

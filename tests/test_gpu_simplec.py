@@ -1,0 +1,138 @@
+"""The MI355X steady SIMPLEC solver against the oracle's serial restatement
+(bitwise, every field, through the C ABI) and against the reference's own
+logged cavity results (fv/steady_solver.cpp; validation/cases/cavity.hpp)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import kutta_amd
+import oracle
+from cavity import (REFERENCE_129_RE100, REFERENCE_65_RE1000, centerline, interpolate, reference_error,
+                    run_to_steady)
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "w", "p", "mass", "face_pressure", "boundary_pressure", "coefficient",
+          "physical_diagonal", "consistent_inverse", "correction")
+PAIRED_FIELDS = ("u_lo", "v_lo", "w_lo", "p_lo", "mass_lo")
+# upload_mesh's FP32 arrays and the least-squares fits (host setup, uploaded).
+SETUP_FIELDS = ("volume", "weight", "area_x", "owner_to_face_x", "neighbour_to_face_y", "owner_to_neighbour_z",
+                "wall_normal_distance", "velocity_lsq_inverse", "pressure_lsq_inverse",
+                "velocity_lsq_boundary_x", "pressure_lsq_boundary_y")
+
+
+def make(device, nx, ny, nz, conditions, settings):
+    kinds, wall = kutta_amd.pack_conditions(conditions)
+    gpu = kutta_amd.SteadySolver(device, nx, ny, nz, conditions, settings)
+    cpu = oracle.SteadySolver(nx, ny, nz, kinds, wall, settings.packed())
+    return gpu, cpu
+
+
+def assert_same_state(gpu, cpu, paired):
+    for name in FIELDS + (PAIRED_FIELDS if paired else ()):
+        a, b = gpu.field(name), cpu.field(name)
+        assert a.shape == b.shape, name
+        bad = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32))
+        assert bad.size == 0, f"{name}: {bad.size} differ, first {bad[:4]} gpu {a[bad[:4]]} oracle {b[bad[:4]]}"
+
+
+def assert_same_report(gpu, cpu):
+    g, o = gpu.report(), cpu.report()
+    keys = kutta_amd.REPORT_FIELDS
+    for k, name in enumerate(keys):
+        if name.endswith("iterations") or name in ("iteration", "pressure_converged", "maximum_velocity",
+                                                   "pseudo_time_step"):
+            assert g[name] == o[k], name  # counts and max reductions are exact
+        else:  # FP64 sums in another order
+            assert g[name] == pytest.approx(o[k], rel=1e-9, abs=1e-300), name
+
+
+CASES = [
+    # nx, ny, nz, Re, paired, iterations
+    (16, 16, 1, 100, True, 25),
+    (16, 16, 1, 100, False, 25),
+    (24, 20, 1, 400, True, 21),
+    (7, 6, 5, 50, True, 12),   # a 3-D box: every velocity component moves
+]
+
+
+@pytest.mark.parametrize("nx,ny,nz,re,paired,iterations", CASES)
+def test_simplec_matches_oracle_bitwise(device, nx, ny, nz, re, paired, iterations):
+    print(f"[simplec] {nx}x{ny}x{nz} Re={re} paired={paired}", flush=True)
+    conditions = kutta_amd.cavity_conditions()
+    if nz > 1:  # closed 3-D box: the lid drags diagonally, symmetry replaced by walls
+        conditions = {name: kutta_amd.Wall() for name in kutta_amd.PATCHES}
+        conditions["ymax"] = kutta_amd.Wall((1.0, 0.0, 0.5))
+    settings = kutta_amd.SolverSettings(viscosity=1.0 / re, momentum_relaxation=0.7, paired_state=paired,
+                                        monitor_interval=7)
+    gpu, cpu = make(device, nx, ny, nz, conditions, settings)
+    assert gpu.info()["reference"] == cpu.reference()
+    for name in SETUP_FIELDS:
+        a, b = gpu.field(name), cpu.field(name)
+        assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+    assert_same_state(gpu, cpu, paired)  # after the constructor's flux
+    for block in (1, 4, iterations - 5):
+        gpu.iterate(block)
+        cpu.iterate(block)
+        assert_same_state(gpu, cpu, paired)
+        assert_same_report(gpu, cpu)
+    if nz > 1:
+        assert np.abs(gpu.field("w")).max() > 0
+
+
+def test_c1_cavity_first_iterations_match_oracle(device):
+    """The 128x128x1 Re=1000 cavity (momentum and pressure relaxation 1,
+    monitor every 100) for 6 iterations, bit for bit."""
+    print("[simplec] C1 128x128x1 Re=1000", flush=True)
+    settings = kutta_amd.SolverSettings(viscosity=1e-3, momentum_relaxation=1.0, pressure_relaxation=1.0,
+                                        monitor_interval=100)
+    gpu, cpu = make(device, 128, 128, 1, kutta_amd.cavity_conditions(), settings)
+    gpu.iterate(6)
+    cpu.iterate(6)
+    assert_same_state(gpu, cpu, True)
+    assert_same_report(gpu, cpu)
+    assert np.all(gpu.field("w") == 0)  # symmetry front and back keep the flow planar
+
+
+def steady_cavity(device, n, re):
+    settings = kutta_amd.SolverSettings(viscosity=1.0 / re, momentum_relaxation=1.0, pressure_relaxation=1.0,
+                                        monitor_interval=100)
+    solver = kutta_amd.SteadySolver(device, n, n, 1, kutta_amd.cavity_conditions(), settings)
+    tolerance = 2e-6 if re == 1000 else 2e-5
+    r, u, v, steady = run_to_steady(solver, solver.iterate, solver.report,
+                                    lambda: (solver.field("u"), solver.field("v")), tolerance)
+    cent = solver.centroids()
+    return r, centerline(cent, u, True), centerline(cent, v, False), steady
+
+
+def test_cavity_65_re1000_matches_reference_log(device):
+    """validation::cavity(65x65, Re=1000, update tolerance 2e-6) as in the
+    reference's refinement test; expected values from its CPU backend's log."""
+    print("[simplec] 65x65 Re=1000 to steady", flush=True)
+    r, U, V, steady = steady_cavity(device, 65, 1000)
+    ref = REFERENCE_65_RE1000
+    assert steady and r["iteration"] == ref["iterations"]
+    eu = reference_error(U, "erturk2005_u_vertical_centerline_Re1000.csv", 0, 1)
+    ev = reference_error(V, "erturk2005_v_horizontal_centerline_Re1000.csv", 0, 1)
+    print(f"[simplec] Erturk u {eu:.12f} v {ev:.12f} centre {interpolate(U, .5):.12f} {interpolate(V, .5):.12f}")
+    # The reference's converged state, to a few float ulps of the velocity.
+    assert eu == pytest.approx(ref["u_error"], abs=3e-7)
+    assert ev == pytest.approx(ref["v_error"], abs=3e-7)
+    assert interpolate(U, .5) == pytest.approx(ref["u_center"], abs=3e-7)
+    assert interpolate(V, .5) == pytest.approx(ref["v_center"], abs=3e-7)
+
+
+def test_cavity_129_re100_matches_reference_log(device):
+    """validation::cavity(129x129, Re=100) against Ghia's tables; the
+    reference logs max deviations 0.005044 (u) and 0.008972 (v)."""
+    print("[simplec] 129x129 Re=100 to steady", flush=True)
+    r, U, V, steady = steady_cavity(device, 129, 100)
+    ref = REFERENCE_129_RE100
+    assert steady
+    eu = reference_error(U, "ghia1982_u_vertical_centerline.csv", 2, 3)
+    ev = reference_error(V, "ghia1982_v_horizontal_centerline.csv", 2, 3)
+    print(f"[simplec] iterations {r['iteration']} Ghia u {eu:.6f} v {ev:.6f}")
+    # Converged only to the harness's 2e-5 update tolerance: a few 1e-6 apart.
+    assert eu == pytest.approx(ref["ghia_u"], abs=5e-6)
+    assert ev == pytest.approx(ref["ghia_v"], abs=5e-6)
