@@ -20,6 +20,11 @@ on one host core beside it.
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--sweeps 5] [--tail T] [--no-cpu-baseline]
     python bench.py --copy-probe BYTES [--steps K]   # HBM counter calibration
     python bench.py --fv-kernels [--steps K]         # each FV kernel at C2 size
+    python bench.py --simplec-box N [--steps K]       # SIMPLEC on the N^3 lid-driven box
+
+The line also carries "simplec": ms per steady SIMPLEC iteration on the
+128x128x1 lid-driven cavity at Re=1000 (500 iterations, iterations 6-500
+timed), with the oracle timed on one core beside it.
 """
 from __future__ import annotations
 
@@ -165,6 +170,66 @@ def fv_kernel_bench(size: int, repeats: int) -> int:
     return 0
 
 
+def simplec_bench(dev, size: int, nz: int, reynolds: float, relaxation: float, monitor: int,
+                  warmup: int, iterations: int, cpu_iterations: int) -> dict:
+    """ms per steady SIMPLEC outer iteration on the lid-driven cavity
+    make_box_hex(size, size, nz): lid u=1 on ymax, walls at rest, symmetry on
+    zmin/zmax, viscosity 1/Re, paired state and the reference's default
+    numerics (validation/cases/cavity.hpp:111-121 for the 2-D cavity;
+    bench/bench_fv.cpp:128-143 for the 3-D box). `warmup` iterations run
+    untimed (setup, first monitors), then `iterations` are timed on the device
+    stream. The oracle's serial restatement is timed on one host core for
+    `cpu_iterations` iterations after the same warm-up."""
+    import kutta_amd
+    settings = kutta_amd.SolverSettings(viscosity=1.0 / reynolds, momentum_relaxation=relaxation,
+                                        pressure_relaxation=1.0, monitor_interval=monitor)
+    conditions = kutta_amd.cavity_conditions()
+    t = time.perf_counter()
+    solver = kutta_amd.SteadySolver(dev, size, size, nz, conditions, settings)
+    setup_s = time.perf_counter() - t
+    solver.iterate(warmup)
+    dev.synchronize()
+    waits0 = solver.info()["host_waits"]
+    t0 = time.perf_counter()
+    solver.iterate(iterations)
+    dev.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(iterations, 1)
+    info = solver.info()
+    report = solver.report()
+    out = {"workload": f"lid-driven cavity {size}x{size}x{nz} hex, Re={reynolds:g}, paired state, "
+                       f"SOU + Barth-Jespersen, automatic pseudo time step, AMG-PCG pressure",
+           "cells": info["cells"], "ms_per_iteration": round(ms, 4), "timed_iterations": iterations,
+           "first_timed_iteration": warmup + 1, "setup_s": round(setup_s, 3),
+           "linear_solver_host_waits_per_iteration": round((info["host_waits"] - waits0) / max(iterations, 1), 2),
+           "last_pressure_iterations": int(report["pressure_iterations"]),
+           "maximum_velocity": report["maximum_velocity"]}
+    solver.close()
+    if cpu_iterations > 0:
+        out["cpu_baseline"] = simplec_cpu_baseline(size, nz, settings, conditions, warmup, cpu_iterations, ms)
+    return out
+
+
+def simplec_cpu_baseline(size, nz, settings, conditions, warmup, iterations, gpu_ms) -> dict:
+    """The oracle (serial restatement of the reference's CPU path) on one host
+    core: `iterations` SIMPLEC iterations after the same warm-up."""
+    import kutta_amd
+    import oracle
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    kinds, wall = kutta_amd.pack_conditions(conditions)
+    cpu = oracle.SteadySolver(size, size, nz, kinds, wall, settings.packed())
+    cpu.iterate(warmup)
+    t0 = time.perf_counter()
+    cpu.iterate(iterations)
+    ms = (time.perf_counter() - t0) * 1e3 / iterations
+    cpu.close()
+    return {"value": round(ms, 3), "unit": "ms per SIMPLEC iteration", "cores": 1, "kind": "port",
+            "sample": f"{iterations} iterations after {warmup} warm-up iterations of the same cavity",
+            "gpu_speedup": round(ms / gpu_ms, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,11 +244,35 @@ def main() -> int:
                     help="only time each finite-volume kernel on the size^3 mesh")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time device copies of BYTES (counter calibration)")
+    ap.add_argument("--simplec-size", type=int, default=128,
+                    help="cells per side of the 2-D cavity timed beside the solve (0 skips it)")
+    ap.add_argument("--simplec-iterations", type=int, default=500)
+    ap.add_argument("--simplec-box", type=int, default=0, metavar="N",
+                    help="only time SIMPLEC on the N^3 lid-driven box (the reference's FV benchmark)")
+    ap.add_argument("--simplec-only", action="store_true",
+                    help="only time SIMPLEC on the 2-D cavity (--simplec-size, --simplec-iterations)")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe, max(args.steps, 1))
     if args.fv_kernels:
         return fv_kernel_bench(args.size, max(args.steps, 1))
+    if args.simplec_box or args.simplec_only:
+        import kutta_amd
+        dev = kutta_amd.Device(0)
+        if args.simplec_box:
+            # bench/bench_fv.cpp:128-143: default settings except momentum
+            # relaxation 1, iterate(5) untimed, then timed iterations.
+            n = args.simplec_box
+            line = simplec_bench(dev, n, n, 100.0, 1.0, 20, 5, max(args.steps, 1), 0)
+        else:
+            line = simplec_bench(dev, args.simplec_size, 1, 1000.0, 1.0, 100, 5,
+                                 max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20)
+        line = {"metric": "ms/SIMPLEC iteration", "value": line["ms_per_iteration"], "unit": "ms",
+                "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+                "config": line}
+        print(json.dumps(line), flush=True)
+        dev.close()
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a run)", file=sys.stderr)
@@ -260,6 +349,13 @@ def main() -> int:
         "launches": launches,
     }
 
+    simplec = None
+    if args.simplec_size:
+        # Configuration 1: 128x128x1 cavity, Re=1000, relaxation 1, monitors
+        # every 100, 500 iterations; ms averaged over iterations 6-500.
+        simplec = simplec_bench(dev, args.simplec_size, 1, 1000.0, 1.0, 100, 5,
+                                max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20)
+
     baseline = None
     x_gpu = x.download()
     if not args.no_cpu_baseline:
@@ -298,6 +394,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": baseline,
+        "simplec": simplec,
     }
     print(json.dumps(line), flush=True)
     dev.close()

@@ -342,6 +342,39 @@ def ref_mesh():
     return L
 
 
+def ref_devmesh():
+    """The reference's own upload_mesh (oracle/_ref/libref_devmesh.so), or None."""
+    path = HERE / "_ref" / "libref_devmesh.so"
+    if not path.exists():
+        return None
+    L = C.CDLL(str(path))
+    L.ref_upload_mesh.argtypes = [C.c_int] * 4 + [C.c_void_p, C.c_char_p, C.c_void_p]
+    L.ref_upload_mesh.restype = C.c_long
+    L.ref_devmesh_last_error.restype = C.c_char_p
+    return L
+
+
+def upload_mesh_array(nx, ny, nz, tet, wall, name, reference=False):
+    """One FP32 array of upload_mesh (fv/device_mesh.cpp:208-256) for a box,
+    from the restatement or (reference=True) from the reference's build."""
+    w = np.ascontiguousarray(wall, dtype=np.int32)
+    if reference:
+        L = ref_devmesh()
+        fn, err = L.ref_upload_mesh, L.ref_devmesh_last_error
+    else:
+        L = lib()
+        L.oracle_upload_mesh.argtypes = [C.c_int] * 4 + [C.c_void_p, C.c_char_p, C.c_void_p]
+        L.oracle_upload_mesh.restype = C.c_long
+        L.oracle_simplec_last_error.restype = C.c_char_p
+        fn, err = L.oracle_upload_mesh, L.oracle_simplec_last_error
+    n = fn(nx, ny, nz, int(tet), w.ctypes.data, name.encode(), None)
+    if n < 0:
+        raise OracleError(err().decode() or f"unknown array {name}")
+    out = np.empty(n, np.float32)
+    fn(nx, ny, nz, int(tet), w.ctypes.data, name.encode(), out.ctypes.data)
+    return out
+
+
 def ref_kernels():
     """The reference's own kernel bodies (oracle/_ref), or None when not built."""
     global _ref

@@ -1063,6 +1063,93 @@ long oracle_simplec_array(void* h, const char* name, float* out) {
   return long(v->size());
 }
 int oracle_simplec_reference(void* h) { return static_cast<oracle::simplec::Steady*>(h)->reference(); }
+// upload_mesh's float array `name` for make_box_hex/tet(nx, ny, nz) after RCM,
+// patches marked Wall where wall[p] != 0; copies it into out when non-null
+// and returns its length, -1 for an unknown name or an error.
+long oracle_upload_mesh(int nx, int ny, int nz, int tet, const int* wall, const char* name, float* out) {
+  try {
+    using namespace oracle;
+    using namespace oracle::simplec;
+    auto m = make_box(nx, ny, nz, {0, 0, 0}, {1, 1, 1}, tet != 0);
+    renumber_cells_rcm(m);
+    const auto g = compute_geometry(m);
+    std::vector<bool> is_wall(m.patches.size());
+    for (std::size_t p = 0; p < is_wall.size(); ++p) is_wall[p] = wall[p] != 0;
+    const auto d = upload_mesh(m, g, is_wall);
+    const std::map<std::string, const std::vector<float>*> arrays = {
+        {"area_x", &d.area.x}, {"area_y", &d.area.y}, {"area_z", &d.area.z},
+        {"owner_to_face_x", &d.owner_to_face.x}, {"owner_to_face_y", &d.owner_to_face.y},
+        {"owner_to_face_z", &d.owner_to_face.z}, {"neighbour_to_face_x", &d.neighbour_to_face.x},
+        {"neighbour_to_face_y", &d.neighbour_to_face.y}, {"neighbour_to_face_z", &d.neighbour_to_face.z},
+        {"owner_to_neighbour_x", &d.owner_to_neighbour.x}, {"owner_to_neighbour_y", &d.owner_to_neighbour.y},
+        {"owner_to_neighbour_z", &d.owner_to_neighbour.z}, {"weight", &d.weight}, {"volume", &d.volume},
+        {"sign", &d.sign}, {"wall_normal_distance", &d.wall_normal_distance}};
+    const auto it = arrays.find(name);
+    if (it == arrays.end()) return -1;
+    if (out) std::memcpy(out, it->second->data(), it->second->size() * sizeof(float));
+    return long(it->second->size());
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return -1;
+  }
+}
+
+// The least-squares gradient (fv_gradient with boundary_least_squares' fit,
+// fv/operators.cpp:166-211, 296-302) of the linear field
+// phi = c[0] + c[1] x + c[2] y + c[3] z on a box (hex or tet, optionally
+// perturbed, RCM-renumbered), each patch observed as obs[p] (0 omitted,
+// 1 face value, 2 normal-foot value). Cell values are phi at the centroids;
+// boundary values are phi where the fit places them (the face centroid, or
+// the owner centroid's normal foot on the face plane). Writes the gradient
+// (3 * cells floats, x y z per cell) and the centroids (3 * cells doubles).
+int oracle_lsq_gradient(int nx, int ny, int nz, int tet, double amplitude, unsigned long long seed,
+                        const int* obs, const double* c, float* gradient, double* centroid) {
+  try {
+    using namespace oracle;
+    using namespace oracle::simplec;
+    auto m = make_box(nx, ny, nz, {0, 0, 0}, {1, 1, 1}, tet != 0);
+    if (amplitude > 0) perturb_box(m, nx, ny, nz, {0, 0, 0}, {1, 1, 1}, amplitude, seed);
+    renumber_cells_rcm(m);
+    const auto g = compute_geometry(m);
+    const auto st = make_stencil(m, g);
+    const auto cf = build_cell_faces(m);
+    const Index ni = m.n_internal_faces(), nb = m.n_faces() - ni;
+    std::vector<Observation> observations(nb);
+    for (std::size_t p = 0; p < m.patches.size(); ++p)
+      for (Index f = m.patches[p].start; f < m.patches[p].start + m.patches[p].size; ++f)
+        observations[f - ni] = Observation(obs[p]);
+    const auto fit = boundary_least_squares(m, g, observations);
+    const auto phi = [&](Vec3 x) { return c[0] + c[1] * x.x + c[2] * x.y + c[3] * x.z; };
+    std::vector<float> cell(m.n_cells), bc(nb, 0.0f), gx(m.n_cells), gy(m.n_cells), gz(m.n_cells);
+    for (Index i = 0; i < m.n_cells; ++i) cell[i] = float(phi(g.cell_centroid[i]));
+    for (Index f = ni; f < m.n_faces(); ++f) {
+      const auto o = m.owner[f];
+      Vec3 at = g.face_centroid[f];
+      if (observations[f - ni] == Observation::NormalFootValue) {
+        const auto s = g.face_area[f];
+        const auto d = g.face_centroid[f] - g.cell_centroid[o];
+        const auto foot = s * (dot(d, s) / dot(s, s));
+        at = {g.cell_centroid[o].x + foot.x, g.cell_centroid[o].y + foot.y, g.cell_centroid[o].z + foot.z};
+      }
+      bc[f - ni] = float(phi(at));
+    }
+    fv("fv_gradient", m.n_cells, cf.offsets.data(), st.adjacent.data(), st.lsq_x.data(), st.lsq_y.data(),
+       st.lsq_z.data(), fit.boundary_delta.x.data(), fit.boundary_delta.y.data(), fit.boundary_delta.z.data(),
+       fit.inverse.data(), cell.data(), bc.data(), gx.data(), gy.data(), gz.data(), m.n_cells);
+    for (Index i = 0; i < m.n_cells; ++i) {
+      gradient[3 * i] = gx[i];
+      gradient[3 * i + 1] = gy[i];
+      gradient[3 * i + 2] = gz[i];
+      centroid[3 * i] = g.cell_centroid[i].x;
+      centroid[3 * i + 1] = g.cell_centroid[i].y;
+      centroid[3 * i + 2] = g.cell_centroid[i].z;
+    }
+    return m.n_cells;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return -1;
+  }
+}
 // Cell centroids of the renumbered mesh, x y z per cell (3 * cells doubles).
 int oracle_simplec_centroids(void* h, double* out) {
   const auto& g = static_cast<oracle::simplec::Steady*>(h)->geometry();
