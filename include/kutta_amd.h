@@ -134,6 +134,16 @@ int ka_simplec_report(void* solver, int run_monitor, double* out);
 /* Downloads device field `name` (u v w p mass u_lo v_lo w_lo p_lo mass_lo ...)
  * into out when non-null; returns its length, or -1 if unknown. */
 long ka_simplec_field(void* solver, const char* name, float* out);
+/* SteadySolver::forces (fv/steady_solver.cpp:1711-1736): the fluid's pressure
+ * and viscous force on the wall patches named in `patches` (comma separated),
+ * FP64 sums of the per-face forces. out[10]: pressure xyz, viscous xyz, area
+ * vector xyz, reference pressure (the ForceReport). */
+int ka_simplec_forces(void* solver, const char* patches, double reference_pressure, double* out);
+/* ForceReport::coefficient (fv/steady_solver.cpp:1780-1784) of a report as
+ * laid out above, along `direction`, with reference density, velocity, area
+ * and pressure. */
+int ka_simplec_force_coefficient(const double* report, const double* direction, double density,
+                                 double velocity, double area, double pressure, double* out);
 /* info[4]: cells, pinned reference cell, linear-solver host waits so far,
  * iterations. centroids: 3 doubles per cell of the renumbered mesh. */
 int ka_simplec_info(void* solver, int32_t* info);

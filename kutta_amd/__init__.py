@@ -126,6 +126,9 @@ def host_lib() -> C.CDLL:
         lib.ka_simplec_field.argtypes = [vp, C.c_char_p, vp]
         lib.ka_simplec_field.restype = C.c_long
         lib.ka_simplec_info.argtypes = [vp, pi32]
+        lib.ka_simplec_forces.argtypes = [vp, C.c_char_p, dbl, C.POINTER(dbl)]
+        lib.ka_simplec_force_coefficient.argtypes = [C.POINTER(dbl), C.POINTER(dbl), dbl, dbl, dbl, dbl,
+                                                     C.POINTER(dbl)]
         lib.ka_simplec_centroids.argtypes = [vp, vp]
         _host = lib
     return _host
@@ -922,6 +925,27 @@ class SteadySolver:
         if host_lib().ka_simplec_field(self.handle, name.encode(), out.ctypes.data) < 0:
             raise KuttaError(host_lib().ka_last_error().decode())
         return out
+
+    def forces(self, patches, reference_pressure: float = 0.0) -> dict:
+        """Pressure and viscous force on the named wall patches
+        (fv/steady_solver.cpp:1711-1736), FP64."""
+        out = (C.c_double * 10)()
+        _check_host(host_lib().ka_simplec_forces(self.handle, ",".join(patches).encode(),
+                                                 float(reference_pressure), out))
+        v = list(out)
+        return {"pressure": v[0:3], "viscous": v[3:6], "area_vector": v[6:9], "reference_pressure": v[9]}
+
+    @staticmethod
+    def force_coefficient(report: dict, direction, density=1.0, velocity=1.0, area=1.0,
+                          pressure=0.0) -> float:
+        """ForceReport::coefficient (fv/steady_solver.cpp:1780-1784)."""
+        packed = (C.c_double * 10)(*report["pressure"], *report["viscous"], *report["area_vector"],
+                                   report["reference_pressure"])
+        d = (C.c_double * 3)(*direction)
+        out = C.c_double()
+        _check_host(host_lib().ka_simplec_force_coefficient(packed, d, density, velocity, area, pressure,
+                                                            C.byref(out)))
+        return out.value
 
     def info(self) -> dict:
         out = (C.c_int32 * 4)()

@@ -80,6 +80,36 @@ long ka_simplec_field(void* solver, const char* name, float* out) {
   if (out && guard([&] { ok(kh_download(handle(solver).dev->raw(), p, out, size * sizeof(float))); }) != 0) return -1;
   return long(size);
 }
+int ka_simplec_forces(void* solver, const char* patches, double reference_pressure, double* out) {
+  return guard([&] {
+    std::vector<std::string> names;
+    std::string item;
+    for (const char* c = patches;; ++c) {
+      if (*c == ',' || *c == 0) {
+        if (!item.empty()) names.push_back(item);
+        item.clear();
+        if (*c == 0) break;
+      } else {
+        item += *c;
+      }
+    }
+    const auto r = handle(solver).solver->forces(names, reference_pressure);
+    const double v[]{r.pressure.x, r.pressure.y, r.pressure.z, r.viscous.x, r.viscous.y, r.viscous.z,
+                     r.area_vector.x, r.area_vector.y, r.area_vector.z, r.reference_pressure};
+    std::memcpy(out, v, sizeof v);
+  });
+}
+int ka_simplec_force_coefficient(const double* report, const double* direction, double density,
+                                 double velocity, double area, double pressure, double* out) {
+  return guard([&] {
+    ForceReport r;
+    r.pressure = {report[0], report[1], report[2]};
+    r.viscous = {report[3], report[4], report[5]};
+    r.area_vector = {report[6], report[7], report[8]};
+    r.reference_pressure = report[9];
+    *out = r.coefficient({direction[0], direction[1], direction[2]}, density, velocity, area, pressure);
+  });
+}
 int ka_simplec_info(void* solver, int32_t* info) {
   return guard([&] {
     auto& s = *handle(solver).solver;

@@ -188,6 +188,7 @@ struct Equation {
 struct SteadySolver::Impl {
   HipDevice& dev;
   SolverSettings settings;
+  std::vector<PatchCondition> patch_condition;
   PolyMesh host;
   MeshGeometry geom;
   CellFaces cf;
@@ -427,6 +428,7 @@ struct SteadySolver::Impl {
   }
   void step();
   IterationReport monitor();
+  ForceReport forces(const std::vector<std::string>& patches, double reference_pressure);
 };
 
 SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition> conditions, SolverSettings s)
@@ -445,7 +447,6 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
     KA_REQUIRE(by_name.emplace(c.patch, c).second, "FV duplicate patch condition");
   }
   KA_REQUIRE(by_name.size() == host.patches.size(), "FV supply exactly one condition per patch");
-  std::vector<PatchCondition> patch_condition;
   for (const auto& p : host.patches) {
     const auto it = by_name.find(p.name);
     KA_REQUIRE(it != by_name.end(), "FV missing boundary: " + p.name);
@@ -795,6 +796,55 @@ IterationReport SteadySolver::Impl::monitor() {
   return report;
 }
 
+// :1711-1736 with viscous_face_forces (:1373-1416) for laminar walls.
+ForceReport SteadySolver::Impl::forces(const std::vector<std::string>& patches, double reference_pressure) {
+  KA_REQUIRE(std::isfinite(reference_pressure), "FV nonfinite force reference pressure");
+  std::vector<std::string> names(patches);
+  std::sort(names.begin(), names.end());
+  KA_REQUIRE(std::adjacent_find(names.begin(), names.end()) == names.end(), "FV duplicate force patch");
+  std::vector<float> selected(nb, 0.0f);
+  ForceReport out;
+  for (const auto& name : names) {
+    std::size_t p = 0;
+    while (p < host.patches.size() && host.patches[p].name != name) ++p;
+    KA_REQUIRE(p < host.patches.size() && patch_condition[p].kind == PatchCondition::Wall,
+               "FV force patch must name a wall: " + name);
+    for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+      selected[f - ni] = 1.0f;
+      out.area_vector = {out.area_vector.x + geom.face_area[f].x, out.area_vector.y + geom.face_area[f].y,
+                         out.area_vector.z + geom.face_area[f].z};
+    }
+  }
+  update_boundaries();
+  update_pressure();
+  reconstruct_velocity();
+  const F mask = to_device(dev, selected);
+  Vec3F pf = zeros3(nb), vf = zeros3(nb);
+  dev.launch(KH_FV_WALL_PRESSURE_FORCE, nb,
+             {mask, area.x, area.y, area.z, face_pressure, pf.x, pf.y, pf.z, ni, float(reference_pressure)});
+  const auto &gu = momentum[0].grad, &gv = momentum[1].grad, &gw = momentum[2].grad;
+  dev.launch(KH_FV_WALL_VISCOUS_FORCE, nb,
+             {owner, mask, area.x, area.y, area.z, owner_to_face.x, owner_to_face.y, owner_to_face.z, velocity.x,
+              velocity.y, velocity.z, gu.x, gu.y, gu.z, gv.x, gv.y, gv.z, gw.x, gw.y, gw.z, boundary_velocity.x,
+              boundary_velocity.y, boundary_velocity.z, vf.x, vf.y, vf.z, ni, float(settings.viscosity)});
+  const auto r = dev.reduce_many({HipDevice::sum(pf.x.data(), nb), HipDevice::sum(pf.y.data(), nb),
+                                  HipDevice::sum(pf.z.data(), nb), HipDevice::sum(vf.x.data(), nb),
+                                  HipDevice::sum(vf.y.data(), nb), HipDevice::sum(vf.z.data(), nb)});
+  out.pressure = {r[0], r[1], r[2]};
+  out.viscous = {r[3], r[4], r[5]};
+  out.reference_pressure = reference_pressure;
+  return out;
+}
+
+double ForceReport::coefficient(Vec3 direction, double density, double velocity, double area,
+                                double pressure) const {
+  KA_REQUIRE(density > 0 && velocity > 0 && area > 0, "FV positive force references required");
+  const Vec3 f = force();
+  const double shift = reference_pressure - pressure;
+  const Vec3 adjusted{f.x + area_vector.x * shift, f.y + area_vector.y * shift, f.z + area_vector.z * shift};
+  return dot(adjusted, direction) / (.5 * density * velocity * velocity * area);
+}
+
 SteadySolver::SteadySolver(HipDevice& dev, PolyMesh mesh, std::vector<PatchCondition> conditions,
                            SolverSettings settings)
     : impl_(std::make_unique<Impl>(dev, std::move(mesh), std::move(conditions), settings)) {}
@@ -804,6 +854,9 @@ void SteadySolver::iterate(int count) {
   while (count-- > 0) impl_->step();
 }
 IterationReport SteadySolver::monitor() { return impl_->monitor(); }
+ForceReport SteadySolver::forces(const std::vector<std::string>& patches, double reference_pressure) {
+  return impl_->forces(patches, reference_pressure);
+}
 const IterationReport& SteadySolver::report() const { return impl_->report; }
 Index SteadySolver::reference_cell() const { return impl_->reference; }
 const MeshGeometry& SteadySolver::geometry() const { return impl_->geom; }

@@ -51,6 +51,15 @@ struct IterationReport {
   SolveStats momentum[3], pressure;
 };
 
+// fv/steady_solver.hpp:286-292 (FP64 sums of the per-face wall forces).
+struct ForceReport {
+  Vec3 pressure, viscous, area_vector;
+  double reference_pressure = 0;
+  Vec3 force() const { return {pressure.x + viscous.x, pressure.y + viscous.y, pressure.z + viscous.z}; }
+  // fv/steady_solver.cpp:1780-1784
+  double coefficient(Vec3 direction, double density, double velocity, double area, double pressure) const;
+};
+
 class SteadySolver {
  public:
   SteadySolver(HipDevice& dev, PolyMesh mesh, std::vector<PatchCondition> conditions,
@@ -59,6 +68,9 @@ class SteadySolver {
   void iterate(int count);
   IterationReport monitor();
   const IterationReport& report() const;
+  // Pressure and viscous force of the fluid on the named wall patches
+  // (fv/steady_solver.cpp:1711-1736).
+  ForceReport forces(const std::vector<std::string>& patches, double reference_pressure = 0);
   // Device array by name (u, v, w, p, mass, ...; the names of the oracle's
   // restatement) and its length; nullptr when unknown or not allocated.
   const float* field(const std::string& name, std::size_t* size) const;

@@ -500,6 +500,16 @@ class SteadySolver:
     def reference(self):
         return lib().oracle_simplec_reference(self.handle)
 
+    def forces(self, patches, reference_pressure=0.0):
+        L = lib()
+        L.oracle_simplec_forces.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_void_p]
+        out = np.zeros(10, np.float64)
+        if L.oracle_simplec_forces(self.handle, ",".join(patches).encode(), reference_pressure,
+                                   out.ctypes.data) != 0:
+            raise OracleError(L.oracle_simplec_last_error().decode())
+        return {"pressure": list(out[0:3]), "viscous": list(out[3:6]), "area_vector": list(out[6:9]),
+                "reference_pressure": out[9]}
+
     def centroids(self):
         out = np.empty((self.cells, 3), np.float64)
         lib().oracle_simplec_centroids(self.handle, out.ctypes.data)

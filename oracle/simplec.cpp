@@ -112,6 +112,11 @@ struct Settings {
 struct ResidualValue {
   double raw = 0, scaled = 0, fluent_scaled = 0, fluent_normalization = 0;
 };
+// fv/steady_solver.hpp:286-292
+struct ForceReport {
+  Vec3 pressure, viscous, area_vector;
+  double reference_pressure = 0;
+};
 // fv/steady_solver.hpp:276-284
 struct Report {
   int iteration = 0;
@@ -339,6 +344,7 @@ class Steady {
     while (count-- > 0) step();
   }
   const Report& report() const { return report_; }
+  ForceReport forces(const std::vector<std::string>& patches, double reference_pressure);
   // Named state arrays, for parity checks.
   const V* array(const std::string& name) const;
   Index reference() const { return reference_; }
@@ -369,6 +375,7 @@ class Steady {
   Slack paired_slack() const;
 
   Settings settings;
+  std::vector<Condition> conditions_;
   PolyMesh host;
   Geometry geom;
   DeviceMesh mesh;
@@ -402,7 +409,7 @@ class Steady {
 
 // fv/steady_solver.cpp:203-597, the laminar closed-domain subset.
 Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const Settings& s)
-    : settings(s), host(std::move(input)) {
+    : settings(s), conditions_(conditions), host(std::move(input)) {
   ORACLE_REQUIRE(s.density > 0 && std::isfinite(s.density) && s.viscosity > 0 &&
                      std::isfinite(s.viscosity) && s.momentum_relaxation > 0 &&
                      s.momentum_relaxation <= 1 && s.pressure_relaxation > 0 &&
@@ -957,6 +964,51 @@ Report Steady::monitor() {
   return report_;
 }
 
+// :1711-1736 with viscous_face_forces (:1373-1416), laminar walls without
+// the projected face reconstruction: FP64 sums of the per-face forces.
+ForceReport Steady::forces(const std::vector<std::string>& patches, double reference_pressure) {
+  ORACLE_REQUIRE(std::isfinite(reference_pressure), "FV nonfinite force reference pressure");
+  const std::size_t nb = mesh.n_boundary();
+  V selected(nb, 0.0f);
+  ForceReport out;
+  std::vector<std::string> seen;
+  for (const auto& name : patches) {
+    ORACLE_REQUIRE(std::find(seen.begin(), seen.end(), name) == seen.end(), "FV duplicate force patch");
+    seen.push_back(name);
+  }
+  std::sort(seen.begin(), seen.end());
+  for (const auto& name : seen) {
+    std::size_t p = 0;
+    while (p < host.patches.size() && host.patches[p].name != name) ++p;
+    ORACLE_REQUIRE(p < host.patches.size() && conditions_[p].kind == 0, "FV force patch must name a wall: " + name);
+    for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+      selected[f - mesh.n_internal] = 1.0f;
+      out.area_vector = {out.area_vector.x + geom.face_area[f].x, out.area_vector.y + geom.face_area[f].y,
+                         out.area_vector.z + geom.face_area[f].z};
+    }
+  }
+  update_boundaries();
+  update_pressure();
+  reconstruct_velocity();
+  Vec3f pf, vf;
+  pf.assign(nb);
+  vf.assign(nb);
+  fv("fv_wall_pressure_force", int(nb), selected.data(), mesh.area.x.data(), mesh.area.y.data(),
+     mesh.area.z.data(), face_pressure.data(), pf.x.data(), pf.y.data(), pf.z.data(), mesh.n_internal,
+     float(reference_pressure));
+  const auto &gu = momentum[0].grad, &gv = momentum[1].grad, &gw = momentum[2].grad;
+  fv("fv_wall_viscous_force", int(nb), mesh.owner.data(), selected.data(), mesh.area.x.data(),
+     mesh.area.y.data(), mesh.area.z.data(), mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(),
+     mesh.owner_to_face.z.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(), gu.x.data(),
+     gu.y.data(), gu.z.data(), gv.x.data(), gv.y.data(), gv.z.data(), gw.x.data(), gw.y.data(), gw.z.data(),
+     boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(), vf.x.data(),
+     vf.y.data(), vf.z.data(), mesh.n_internal, float(settings.viscosity));
+  out.pressure = {sum(pf.x.data(), nb), sum(pf.y.data(), nb), sum(pf.z.data(), nb)};
+  out.viscous = {sum(vf.x.data(), nb), sum(vf.y.data(), nb), sum(vf.z.data(), nb)};
+  out.reference_pressure = reference_pressure;
+  return out;
+}
+
 const V* Steady::array(const std::string& name) const {
   const std::map<std::string, const V*> table = {
       {"u", &velocity.x},         {"v", &velocity.y},          {"w", &velocity.z},
@@ -1061,6 +1113,30 @@ long oracle_simplec_array(void* h, const char* name, float* out) {
   if (!v) return -1;
   if (out) std::memcpy(out, v->data(), v->size() * sizeof(float));
   return long(v->size());
+}
+// forces() over the wall patches named in `names` (comma separated):
+// out[10] = pressure xyz, viscous xyz, area vector xyz, reference pressure.
+int oracle_simplec_forces(void* h, const char* names, double reference_pressure, double* out) {
+  try {
+    std::vector<std::string> patches;
+    std::string all(names), item;
+    for (char ch : all + ",") {
+      if (ch == ',') {
+        if (!item.empty()) patches.push_back(item);
+        item.clear();
+      } else {
+        item += ch;
+      }
+    }
+    const auto r = static_cast<oracle::simplec::Steady*>(h)->forces(patches, reference_pressure);
+    const double v[]{r.pressure.x, r.pressure.y, r.pressure.z, r.viscous.x, r.viscous.y, r.viscous.z,
+                     r.area_vector.x, r.area_vector.y, r.area_vector.z, r.reference_pressure};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
+  }
 }
 int oracle_simplec_reference(void* h) { return static_cast<oracle::simplec::Steady*>(h)->reference(); }
 // upload_mesh's float array `name` for make_box_hex/tet(nx, ny, nz) after RCM,

@@ -136,3 +136,25 @@ def test_cavity_129_re100_matches_reference_log(device):
     # Converged only to the harness's 2e-5 update tolerance: a few 1e-6 apart.
     assert eu == pytest.approx(ref["ghia_u"], abs=5e-6)
     assert ev == pytest.approx(ref["ghia_v"], abs=5e-6)
+
+
+def test_forces_match_oracle(device):
+    """forces() (fv/steady_solver.cpp:1711-1736) on the GPU against the
+    oracle: per-face forces are bitwise equal, so the FP64 sums agree to
+    summation order; ForceReport::coefficient follows its closed form."""
+    print("[simplec] forces 20x16x1 Re=200", flush=True)
+    settings = kutta_amd.SolverSettings(viscosity=1.0 / 200, momentum_relaxation=0.8)
+    gpu, cpu = make(device, 20, 16, 1, kutta_amd.cavity_conditions(), settings)
+    gpu.iterate(40)
+    cpu.iterate(40)
+    for patches, pref in ((["ymax"], 0.0), (["xmin", "xmax", "ymin", "ymax"], 0.25), (["ymin", "xmin"], -1.0)):
+        g, o = gpu.forces(patches, pref), cpu.forces(patches, pref)
+        for key in ("pressure", "viscous", "area_vector"):
+            assert g[key] == pytest.approx(o[key], rel=1e-9, abs=1e-14), (patches, key)
+        force = np.add(g["pressure"], g["viscous"])
+        adjusted = force + np.array(g["area_vector"]) * (pref - 0.1)
+        expected = adjusted @ np.array([1.0, 0.5, 0.0]) / (0.5 * 1.2 * 2.0 ** 2 * 0.3)
+        got = kutta_amd.SteadySolver.force_coefficient(g, (1.0, 0.5, 0.0), 1.2, 2.0, 0.3, 0.1)
+        assert got == pytest.approx(expected, rel=1e-12)
+    with pytest.raises(kutta_amd.KuttaError):
+        gpu.forces(["zmin"])  # a symmetry plane is not a wall

@@ -162,3 +162,23 @@ def test_upload_mesh_matches_reference_build(nx, ny, nz, tet):
         a = oracle.upload_mesh_array(nx, ny, nz, tet, wall, name)
         b = oracle.upload_mesh_array(nx, ny, nz, tet, wall, name, reference=True)
         assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+
+
+def test_steady_cavity_walls_balance_the_lid():
+    """At steady state the fluid is in equilibrium, so the x forces it exerts
+    on the four walls (pressure plus viscous, fv/steady_solver.cpp:1711-1736)
+    cancel: the lid's drag (the fluid resists the lid, so along -x) returns
+    through the other walls, up to the discretization error of the wall
+    tractions (0.1% on this 16x16 grid). The area vector of a closed set of
+    walls in a one-cell-deep box is the symmetry planes' missing share: zero
+    in x and y."""
+    s = cavity(16, 16, 100)
+    s.iterate(400)
+    lid = s.forces(["ymax"])
+    total = s.forces(["xmin", "xmax", "ymin", "ymax"])
+    fx_lid = lid["pressure"][0] + lid["viscous"][0]
+    fx = total["pressure"][0] + total["viscous"][0]
+    assert fx_lid < -0.1
+    assert abs(fx) < 3e-3 * abs(fx_lid)
+    assert total["area_vector"][:2] == pytest.approx([0.0, 0.0], abs=1e-15)
+    assert lid["area_vector"] == pytest.approx([0.0, 1.0, 0.0], rel=1e-12)  # the unit lid
