@@ -246,6 +246,19 @@ typedef enum kh_kernel {
                                     down to the coarsest in one workgroup, iterates in LDS:
                                     levels,count,pre,post,coarse_sweeps,nullspace,b,x,status
                                     (n = 1; levels is a device array of kh_tail_level) */
+  /* BiCGStab-Jacobi (linalg/solver.cpp:314-361), one iteration in five launches: */
+  KH_FUSED_DOT_BICG_BETA,        /* dot_into(shadow,r,0) + solver_bicg_beta: shadow,r,scalars,status */
+  KH_FUSED_BICG_DIRECTION,       /* solver_bicg_direction + krylov_multiply_guarded(p -> z):
+                                    r,q,p,inverse,z,scalars,status */
+  KH_FUSED_SPMV_SHADOW_ALPHA,    /* csr_spmv_guarded(z -> q) + dot_into(shadow,q,2) + solver_bicg_alpha:
+                                    rows,cols,values,z,q,shadow,scalars,status */
+  KH_FUSED_BICG_DIFFERENCE_S,    /* solver_bicg_difference + dot_into(s,s,16) + solver_bicg_s +
+                                    krylov_multiply_guarded(s -> zs): r,q,s,inverse,zs,scalars,status */
+  KH_FUSED_SPMV_DOT2_OMEGA,      /* csr_spmv_guarded(zs -> t) + dot2_into(t,t,t,s,2) + solver_bicg_omega:
+                                    rows,cols,values,zs,t,s,scalars,status */
+  KH_FUSED_BICG_STEP_CHECK_BETA, /* solver_bicg_step + dot_into(r,r,18) + solver_check(pcg=0), then the
+                                    next iteration's dot_into(shadow,r,0) + solver_bicg_beta:
+                                    x,r,z,zs,s,t,shadow,scalars,status */
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 

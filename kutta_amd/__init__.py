@@ -304,6 +304,12 @@ class Kernel(enum.IntEnum):
     fused_project_dot_beta = 206
     fused_smooth_from_zero = 207
     fused_coarse_tail = 208
+    fused_dot_bicg_beta = 209
+    fused_bicg_direction = 210
+    fused_spmv_shadow_alpha = 211
+    fused_bicg_difference_s = 212
+    fused_spmv_dot2_omega = 213
+    fused_bicg_step_check_beta = 214
 
 
 class Buffer:

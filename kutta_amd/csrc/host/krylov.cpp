@@ -471,14 +471,18 @@ struct KrylovSolver::Impl {
     }
   }
 
-  // linalg/solver.cpp:314-361
+  // linalg/solver.cpp:314-361. With the Jacobi preconditioner and fused
+  // kernels an iteration is five launches (each with its reduction and scalar
+  // update); the first rho and beta after a restart are one more.
   void bicgstab(const DeviceCsr& a, float* x, const float* b, const SolveControl& c, double target,
                 SolveStats& st) {
+    const bool jacobi_fused = fused && !amg;
     auto restart = [&] {
       reset(st.iterations);
       dev.copy(shadow.data(), r.data(), sizeof(float) * n);
       dev.fill(p.data(), 0.0f, n);
       dev.fill(q.data(), 0.0f, n);
+      if (jacobi_fused) dev.launch(KH_FUSED_DOT_BICG_BETA, n, {shadow, r, scalars, status});
     };
     restart();
     int restarts = 0;
@@ -488,6 +492,14 @@ struct KrylovSolver::Impl {
       const int batch = std::min(interval, c.max_iter - st.iterations);
       verified_current = false;
       for (int j = 0; j < batch; ++j) {
+        if (jacobi_fused) {
+          dev.launch(KH_FUSED_BICG_DIRECTION, n, {r, q, p, jacobi_inverse, z, scalars, status});
+          dev.launch(KH_FUSED_SPMV_SHADOW_ALPHA, n, {a.row_ptr, a.col, a.val, z, q, shadow, scalars, status});
+          dev.launch(KH_FUSED_BICG_DIFFERENCE_S, n, {r, q, s, jacobi_inverse, zs, scalars, status});
+          dev.launch(KH_FUSED_SPMV_DOT2_OMEGA, n, {a.row_ptr, a.col, a.val, zs, t, s, scalars, status});
+          dev.launch(KH_FUSED_BICG_STEP_CHECK_BETA, n, {x, r, z, zs, s, t, shadow, scalars, status});
+          continue;
+        }
         dot_into(shadow.data(), r.data(), 0);
         dev.launch(KH_SOLVER_BICG_BETA, 1, {scalars, status});
         dev.launch(KH_SOLVER_BICG_DIRECTION, n, {r, q, p, scalars, status});

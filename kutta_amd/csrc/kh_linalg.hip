@@ -264,8 +264,8 @@ __global__ void k_solver_pcg_beta(float* s, int* status, int flexible) {
   if (threadIdx.x == 0) pcg_beta(s, status, flexible);
 }
 // kernels/scalars.kern.h:81-89
-__global__ void k_solver_bicg_beta(float* s, int* status) {
-  if (threadIdx.x != 0 || status[0] != 0) return;
+__device__ void bicg_beta(float* s, int* status) {
+  if (status[0] != 0) return;
   Pair rho = pload(s, 0), omega = pload(s, 12);
   Pair beta = pmul(pdiv(rho, pload(s, 14)), pdiv(pload(s, 8), omega));
   if (pzero(rho) || !pfinite(rho) || pzero(omega) || !pcoefficient(omega) ||
@@ -277,22 +277,22 @@ __global__ void k_solver_bicg_beta(float* s, int* status) {
   }
 }
 // kernels/scalars.kern.h:90-97
-__global__ void k_solver_bicg_alpha(float* s, int* status) {
-  if (threadIdx.x != 0 || status[0] != 0) return;
+__device__ void bicg_alpha(float* s, int* status) {
+  if (status[0] != 0) return;
   Pair rv = pload(s, 2);
   Pair alpha = pdiv(pload(s, 0), rv);
   if (pzero(rv) || !pfinite(rv) || !pcoefficient(alpha) || pzero(alpha)) status[0] = 2;
   else pstore(s, 8, alpha);
 }
 // kernels/scalars.kern.h:98-103
-__global__ void k_solver_bicg_s(const float* s, int* status) {
-  if (threadIdx.x != 0 || status[0] != 0) return;
+__device__ void bicg_s(const float* s, int* status) {
+  if (status[0] != 0) return;
   ++status[1];
   status[2] = ple(pload(s, 16), pload(s, 20)) ? 1 : 0;
 }
 // kernels/scalars.kern.h:104-112
-__global__ void k_solver_bicg_omega(float* s, int* status) {
-  if (threadIdx.x != 0 || status[0] != 0) return;
+__device__ void bicg_omega(float* s, int* status) {
+  if (status[0] != 0) return;
   if (status[2] != 0) {
     pstore(s, 12, {0.0f, 0.0f});
     return;
@@ -301,6 +301,18 @@ __global__ void k_solver_bicg_omega(float* s, int* status) {
   Pair omega = pdiv(ts, tt);
   if (!ppositive(tt) || !pfinite(tt) || pzero(omega) || !pcoefficient(omega)) status[0] = 2;
   else pstore(s, 12, omega);
+}
+__global__ void k_solver_bicg_beta(float* s, int* status) {
+  if (threadIdx.x == 0) bicg_beta(s, status);
+}
+__global__ void k_solver_bicg_alpha(float* s, int* status) {
+  if (threadIdx.x == 0) bicg_alpha(s, status);
+}
+__global__ void k_solver_bicg_s(const float* s, int* status) {
+  if (threadIdx.x == 0) bicg_s(s, status);
+}
+__global__ void k_solver_bicg_omega(float* s, int* status) {
+  if (threadIdx.x == 0) bicg_omega(s, status);
 }
 __global__ void k_solver_check(float* s, int* status, int pcg) {
   if (threadIdx.x == 0) solver_check_body(s, status, pcg);
@@ -459,9 +471,123 @@ __global__ void __launch_bounds__(kBlock) k_fused_project_dot(int n, float* __re
   store_partials<1>(mine, partials);
 }
 
+// BiCGStab with the Jacobi preconditioner (linalg/solver.cpp:314-361). Each
+// vector kernel of an iteration also applies the diagonal inverse to its
+// output or produces the partials of the reduction that follows it, so an
+// iteration is direction, shadow-dot SpMV, difference, two-dot SpMV and
+// step, each reduction with its one-workgroup finisher.
+
+// Partials of x.y: dot_into(x, y) ahead of a scalar update.
+__global__ void __launch_bounds__(kBlock) k_fused_dot(int n, const float* __restrict__ x,
+    const float* __restrict__ y, const int* __restrict__ status, double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {i < n ? double(x[i]) * double(y[i]) : 0.0};
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+// p = r + beta (p - omega q), z = p * inverse: solver_bicg_direction +
+// krylov_multiply_guarded.
+__global__ void __launch_bounds__(kBlock) k_fused_bicg_direction(int n, const float* __restrict__ r,
+    const float* __restrict__ q, float* __restrict__ p, const float* __restrict__ inverse,
+    float* __restrict__ z, const float* __restrict__ scalars, const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i >= n) return;
+  const float beta = pfloat(pload(scalars, 10)), omega = pfloat(pload(scalars, 12));
+  const float pi = r[i] + beta * (p[i] - omega * q[i]);
+  p[i] = pi;
+  z[i] = pi * inverse[i];
+}
+// q = A z, partials of shadow.q: csr_spmv_guarded + dot_into(shadow, q, 2).
+__global__ void __launch_bounds__(kBlock) k_fused_spmv_shadow_dot(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ z,
+    float* __restrict__ q, const float* __restrict__ shadow, const int* __restrict__ status,
+    double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float qi = spmv_row(i, rows, cols, values, z);
+    q[i] = qi;
+    mine[0] = double(shadow[i]) * double(qi);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+// s = r - alpha q, zs = s * inverse, partials of s.s: solver_bicg_difference +
+// dot_into(s, s, 16) + krylov_multiply_guarded.
+__global__ void __launch_bounds__(kBlock) k_fused_bicg_difference(int n, const float* __restrict__ r,
+    const float* __restrict__ q, float* __restrict__ sv, const float* __restrict__ inverse,
+    float* __restrict__ zs, const float* __restrict__ scalars, const int* __restrict__ status,
+    double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    const float si = r[i] - pfloat(pload(scalars, 8)) * q[i];
+    sv[i] = si;
+    zs[i] = si * inverse[i];
+    mine[0] = double(si) * double(si);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+// t = A zs, partials of t.t and t.s: csr_spmv_guarded + dot2_into(t,t,t,s,2).
+__global__ void __launch_bounds__(kBlock) k_fused_spmv_dot2(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ zs,
+    float* __restrict__ t, const float* __restrict__ sv, const int* __restrict__ status,
+    double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[2] = {0.0, 0.0};
+  if (i < n) {
+    const float ti = spmv_row(i, rows, cols, values, zs);
+    t[i] = ti;
+    mine[0] = double(ti) * double(ti);
+    mine[1] = double(ti) * double(sv[i]);
+  }
+  __shared__ double lds[2][kBlock / 64];
+  mine[0] = block_sum(mine[0], lds[0]);
+  mine[1] = block_sum(mine[1], lds[1]);
+  store_partials<2>(mine, partials);
+}
+// x += alpha z + omega zs, r = s - omega t (or the s exit), partials of r.r
+// and shadow.r: solver_bicg_step + dot_into(r, r, 18) + the next
+// iteration's dot_into(shadow, r, 0).
+__global__ void __launch_bounds__(kBlock) k_fused_bicg_step(int n, float* __restrict__ x,
+    float* __restrict__ r, const float* __restrict__ z, const float* __restrict__ zs,
+    const float* __restrict__ sv, const float* __restrict__ t, const float* __restrict__ shadow,
+    const float* __restrict__ scalars, const int* __restrict__ status, double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[2] = {0.0, 0.0};
+  if (i < n) {
+    const float alpha = pfloat(pload(scalars, 8)), omega = pfloat(pload(scalars, 12));
+    float ri;
+    if (status[2] != 0) {
+      x[i] += alpha * z[i];
+      ri = sv[i];
+    } else {
+      x[i] += alpha * z[i] + omega * zs[i];
+      ri = sv[i] - omega * t[i];
+    }
+    r[i] = ri;
+    mine[0] = double(ri) * double(ri);
+    mine[1] = double(shadow[i]) * double(ri);
+  }
+  __shared__ double lds[2][kBlock / 64];
+  mine[0] = block_sum(mine[0], lds[0]);
+  mine[1] = block_sum(mine[1], lds[1]);
+  store_partials<2>(mine, partials);
+}
+
 // Finishers: the FP64 total of `count` partials as a normalized pair, then the
 // reference's scalar kernel on it.
-enum class Finish { Alpha, Check, Beta, Pair };
+enum class Finish { Alpha, Check, Beta, Pair, BicgBeta, BicgAlpha, BicgS };
 template <Finish F>
 __global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ partials,
                                                    unsigned count, float* scalars, int* status,
@@ -480,8 +606,41 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ pa
   } else if constexpr (F == Finish::Beta) {
     pstore(scalars, 4, value);
     pcg_beta(scalars, status, 0);
+  } else if constexpr (F == Finish::BicgBeta) {
+    pstore(scalars, 0, value);
+    bicg_beta(scalars, status);
+  } else if constexpr (F == Finish::BicgAlpha) {
+    pstore(scalars, 2, value);
+    bicg_alpha(scalars, status);
+  } else if constexpr (F == Finish::BicgS) {
+    pstore(scalars, 16, value);
+    bicg_s(scalars, status);
   } else {
     pstore(dst, 0, value);
+  }
+}
+// Two-slot finishers. Omega: t.t, t.s into slots 2 and 4, then bicg_omega.
+// CheckBeta: r.r into slot 18 and the check (which keeps slot 0 as the
+// previous rho in slot 14), then, if the iteration goes on, shadow.r into
+// slot 0 and the next iteration's bicg_beta.
+enum class Finish2 { Omega, CheckBeta };
+template <Finish2 F>
+__global__ void __launch_bounds__(kBlock) k_finish2(const double* __restrict__ partials,
+                                                    unsigned count, float* scalars, int* status) {
+  if (status[0] != 0) return;
+  double total[2];
+  sum_partials<2>(partials, count, total);
+  if (threadIdx.x != 0) return;
+  if constexpr (F == Finish2::Omega) {
+    pstore(scalars, 2, to_pair(total[0]));
+    pstore(scalars, 4, to_pair(total[1]));
+    bicg_omega(scalars, status);
+  } else {
+    pstore(scalars, 18, to_pair(total[0]));
+    solver_check_body(scalars, status, 0);
+    if (status[0] != 0) return;
+    pstore(scalars, 0, to_pair(total[1]));
+    bicg_beta(scalars, status);
   }
 }
 
@@ -959,6 +1118,40 @@ void l_fused_coarse_tail(kh_device* d, int n, const kh_arg* a) {
       static_cast<const kh_tail_level*>(buf(a, 0)), count, i32(a, 2), i32(a, 3), i32(a, 4),
       i32(a, 5), KCF(6), KF(7), KB(8));
 }
+void l_fused_dot_bicg_beta(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 3));
+  k_fused_dot<<<GRID(n)>>>(n, KCF(0), KCF(1), st, d->partials);
+  k_finish<Finish::BicgBeta><<<FINISH>>>(d->partials, g, KF(2), st, nullptr);
+}
+void l_fused_bicg_direction(kh_device* d, int n, const kh_arg* a) {
+  k_fused_bicg_direction<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), KCF(3), KF(4), KCF(5), KB(6));
+}
+void l_fused_spmv_shadow_alpha(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 7));
+  k_fused_spmv_shadow_dot<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KCF(5), st, d->partials);
+  k_finish<Finish::BicgAlpha><<<FINISH>>>(d->partials, g, KF(6), st, nullptr);
+}
+void l_fused_bicg_difference_s(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 6));
+  k_fused_bicg_difference<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), KCF(3), KF(4), KCF(5), st, d->partials);
+  k_finish<Finish::BicgS><<<FINISH>>>(d->partials, g, KF(5), st, nullptr);
+}
+void l_fused_spmv_dot2_omega(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 7));
+  k_fused_spmv_dot2<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KCF(5), st, d->partials);
+  k_finish2<Finish2::Omega><<<FINISH>>>(d->partials, g, KF(6), st);
+}
+void l_fused_bicg_step_check_beta(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 8));
+  k_fused_bicg_step<<<GRID(n)>>>(n, KF(0), KF(1), KCF(2), KCF(3), KCF(4), KCF(5), KCF(6), KCF(7), st,
+                                 d->partials);
+  k_finish2<Finish2::CheckBeta><<<FINISH>>>(d->partials, g, KF(7), st);
+}
 void l_fused_project_dot_beta(kh_device* d, int n, const kh_arg* a) {
   const unsigned g = reduce_grid(n);
   int* st = static_cast<int*>(buf(a, 6));
@@ -1014,6 +1207,12 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_SOLVER_BICG_STEP, "solver_bicg_step", "bbbbbbbb", l_solver_bicg_step},
       {KH_SOLVER_PROJECT, "solver_project", "bbff", l_solver_project},
       {KH_FUSED_SPMV_DOT_ALPHA, "fused_spmv_dot_alpha", "bbbbbbb", l_fused_spmv_dot_alpha},
+      {KH_FUSED_DOT_BICG_BETA, "fused_dot_bicg_beta", "bbbb", l_fused_dot_bicg_beta},
+      {KH_FUSED_BICG_DIRECTION, "fused_bicg_direction", "bbbbbbb", l_fused_bicg_direction},
+      {KH_FUSED_SPMV_SHADOW_ALPHA, "fused_spmv_shadow_alpha", "bbbbbbbb", l_fused_spmv_shadow_alpha},
+      {KH_FUSED_BICG_DIFFERENCE_S, "fused_bicg_difference_s", "bbbbbbb", l_fused_bicg_difference_s},
+      {KH_FUSED_SPMV_DOT2_OMEGA, "fused_spmv_dot2_omega", "bbbbbbbb", l_fused_spmv_dot2_omega},
+      {KH_FUSED_BICG_STEP_CHECK_BETA, "fused_bicg_step_check_beta", "bbbbbbbbb", l_fused_bicg_step_check_beta},
       {KH_FUSED_STEP_DOT_CHECK, "fused_step_dot_check", "bbbbbb", l_fused_step_dot_check},
       {KH_FUSED_RESIDUAL_RESTRICT, "fused_residual_restrict", "bbbbbbbbb", l_fused_residual_restrict},
       {KH_FUSED_PROLONG_SMOOTH, "fused_prolong_smooth", "bbbbbbbbbb", l_fused_prolong_smooth},

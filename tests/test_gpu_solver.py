@@ -165,3 +165,33 @@ def test_failure_paths_are_host_forced(device):
     st = solver.solve(matrix, x, device.upload(np.ones(a.n_rows, np.float32)), SolveControl(1e-6, 0, 0))
     assert st.iterations == 0 and not st.converged
     assert st.final_residual == st.initial_residual
+
+
+@pytest.mark.parametrize("nx,ny,nz,wind,rel,interval,max_iter", [
+    (24, 22, 1, 2.0, 1e-5, 8, 200),
+    (40, 30, 1, 5.0, 1e-7, 8, 400),    # long enough for the 50-update replacement
+    (12, 10, 8, 1.0, 1e-6, 1, 200),
+    (30, 30, 1, 3.0, 1e-2, 3, 100),    # the momentum tolerance of the SIMPLEC step
+])
+def test_bicgstab_jacobi_fused_equals_reference_sequence(device, nx, ny, nz, wind, rel, interval, max_iter):
+    """The five-launch fused BiCGStab-Jacobi iteration gives the same
+    iterations, final residual and x, bit for bit, as the reference kernel
+    sequence (linalg/solver.cpp:314-361) and as the oracle."""
+    print(f"[bicgstab] {nx}x{ny}x{nz} wind {wind} rel {rel} interval {interval}", flush=True)
+    a = oracle.poisson(nx, ny, nz, 1.0, wind=wind)
+    b = np.random.default_rng(nx * ny + nz).standard_normal(a.n_rows).astype(np.float32)
+    control = SolveControl(rel, 0.0, max_iter, interval)
+    xo, sto, _ = oracle_solve(a, "BicgstabJacobi", b, control)
+    results = []
+    for fused in (True, False):
+        host, matrix, solver = product_setup(device, a, SolverKind.BicgstabJacobi, AmgOptions())
+        solver.set_fused(fused)
+        x = device.alloc(a.n_rows)
+        x.fill(np.float32(0))
+        st = solver.solve(matrix, x, device.upload(b), control)
+        results.append((st, x.download()))
+    for st, x in results:
+        assert st.iterations == sto["iterations"]
+        assert st.final_residual == pytest.approx(sto["final_residual"], rel=1e-12)
+        assert np.array_equal(x.view(np.uint32), xo.view(np.uint32))
+    assert results[0][0].final_residual == results[1][0].final_residual
