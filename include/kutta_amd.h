@@ -83,7 +83,9 @@ void ka_mesh_setup_destroy(void* setup);
 long ka_mesh_setup_array(void* setup, const char* name, int kind, void* out);
 
 /* Solvers (linalg/solver.hpp:237-293, linalg/amg.hpp:8-27).
- * kind: 0 Jacobi, 1 PcgJacobi, 2 BicgstabJacobi, 3 AmgPcg, 4 AmgBicgstab.
+ * kind: 0 Jacobi, 1 PcgJacobi, 2 BicgstabJacobi, 3 AmgPcg, 4 AmgBicgstab;
+ * additive, no reference counterpart: 5 FgmresJacobi, 6 AmgFgmres (flexible
+ * GMRES(m), right preconditioned, restart length 30 unless set).
  * options[8]: pre_sweeps, post_sweeps, coarse_sweeps, coarse_rows, max_levels,
  *             aggregation_passes, nullspace (0 none, 1 constant),
  *             structure (0 general, 1 symmetric M-matrix). */
@@ -110,6 +112,8 @@ int ka_solver_host_waits(void* solver);
  * *level receives the level in use, 0 for none. No reference counterpart: the
  * iterates are those of the per-level launches either way. */
 int ka_solver_set_tail(void* solver, int first, int32_t* level);
+/* Restart length m (1..63) of the FGMRES kinds. No reference counterpart. */
+int ka_solver_set_restart(void* solver, int m);
 
 /* Steady laminar SIMPLEC (fv/steady_solver.hpp:286-360: SteadySolver
  * constructor, iterate, monitor, report, field access) on a box mesh

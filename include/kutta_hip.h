@@ -259,6 +259,14 @@ typedef enum kh_kernel {
   KH_FUSED_BICG_STEP_CHECK_BETA, /* solver_bicg_step + dot_into(r,r,18) + solver_check(pcg=0), then the
                                     next iteration's dot_into(shadow,r,0) + solver_bicg_beta:
                                     x,r,z,zs,s,t,shadow,scalars,status */
+  /* Flexible GMRES(m) (no reference counterpart; Saad 1993): basis vectors
+     FP32 (column-major, n per vector), Hessenberg, rotations and residual
+     recurrence FP64 in a device block of (m+1)m + 5m + 3 doubles; partials
+     hold 64 doubles per 256-row workgroup. */
+  KH_GMRES_START,                /* beta = ||r||, g = beta e1, V0 = r/beta: r,V,block,m,status,partials */
+  KH_GMRES_ARNOLDI,              /* CGS2 of w against V_0..V_j, rotation, estimate check, V_{j+1}:
+                                    V,w,block,m,j,status,partials */
+  KH_GMRES_UPDATE,               /* y = H^-1 g over status[2] columns, x += Z y: Z,x,block,m,status */
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 

@@ -112,6 +112,7 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_set_fused.argtypes = [vp, i32]
         lib.ka_solver_host_waits.argtypes = [vp]
         lib.ka_solver_set_tail.argtypes = [vp, i32, pi32]
+        lib.ka_solver_set_restart.argtypes = [vp, i32]
         lib.ka_aggregate.argtypes = [vp, pi32, dbl, C.POINTER(vp)]
         lib.ka_coarsening_array.argtypes = [vp, i32, pi32]
         lib.ka_coarsening_array.restype = C.c_long
@@ -310,6 +311,9 @@ class Kernel(enum.IntEnum):
     fused_bicg_difference_s = 212
     fused_spmv_dot2_omega = 213
     fused_bicg_step_check_beta = 214
+    gmres_start = 215
+    gmres_arnoldi = 216
+    gmres_update = 217
 
 
 class Buffer:
@@ -483,6 +487,8 @@ class SolverKind(enum.IntEnum):
     BicgstabJacobi = 2
     AmgPcg = 3
     AmgBicgstab = 4
+    FgmresJacobi = 5   # flexible GMRES(m): no reference counterpart
+    AmgFgmres = 6
 
 
 class NullSpace(enum.IntEnum):
@@ -625,6 +631,10 @@ class LinearSolver:
         _check_host(host_lib().ka_solver_solve(self.handle, a.handle, x.ptr, b.ptr, c.rel_tol, c.abs_tol,
                                                c.max_iter, c.residual_check_interval, stats))
         return SolveStats(int(stats[0]), stats[1], stats[2], bool(stats[3]))
+
+    def set_restart(self, m: int) -> None:
+        """Restart length of the FGMRES kinds (1..63; default 30)."""
+        _check_host(host_lib().ka_solver_set_restart(self.handle, int(m)))
 
     def describe(self) -> str:
         buf = C.create_string_buffer(512)
@@ -809,7 +819,7 @@ def aggregate(a: CsrMatrixHost, options: AmgOptions | None = None) -> dict:
 def make_amg_solver(kind: SolverKind, device: Device, a: CsrMatrixHost,
                     options: AmgOptions | None = None) -> LinearSolver:
     """linalg/amg.hpp:22-27 (only AmgPcg / AmgBicgstab)."""
-    if kind not in (SolverKind.AmgPcg, SolverKind.AmgBicgstab):
+    if kind not in (SolverKind.AmgPcg, SolverKind.AmgBicgstab, SolverKind.AmgFgmres):
         raise KuttaError("AMG: expected an AMG solver kind")
     return LinearSolver(device, kind, a, options or AmgOptions())
 

@@ -109,7 +109,7 @@ const int32_t* ka_matrix_diag(void* m) { return view_of(m).diag; }
 int ka_solver_create(void* dev, int kind, void* csr, const int32_t* options, double threshold,
                      void** out) {
   return guard([&] {
-    KA_REQUIRE(kind >= 0 && kind <= 4, "solver: unknown SolverKind");
+    KA_REQUIRE(kind >= 0 && kind <= 6, "solver: unknown SolverKind");
     AmgOptions o;
     o.pre_sweeps = options[0];
     o.post_sweeps = options[1];
@@ -181,6 +181,9 @@ int ka_solver_set_fused(void* s, int fused) {
   return guard([&] { static_cast<Solver*>(s)->solver.set_fused(fused != 0); });
 }
 int ka_solver_host_waits(void* s) { return static_cast<Solver*>(s)->solver.host_waits(); }
+int ka_solver_set_restart(void* solver, int m) {
+  return guard([&] { static_cast<Solver*>(solver)->solver.set_restart(m); });
+}
 int ka_solver_set_tail(void* s, int first, int* level) {
   return guard([&] {
     const int used = static_cast<Solver*>(s)->solver.set_tail(first);

@@ -71,7 +71,8 @@ struct KrylovSolver::Impl {
   DeviceArray<kh_tail_level> tail_table;
 
   Impl(HipDevice& d, SolverKind k, const CsrHost& a, const AmgOptions& o)
-      : dev(d), kind(k), options(o), amg(k == SolverKind::AmgPcg || k == SolverKind::AmgBicgstab),
+      : dev(d), kind(k), options(o),
+        amg(k == SolverKind::AmgPcg || k == SolverKind::AmgBicgstab || k == SolverKind::AmgFgmres),
         nullspace(o.nullspace == NullSpace::Constant), n(a.n_rows), nnz(a.nnz()) {
     validate_host(a);
     if (amg) {
@@ -83,6 +84,8 @@ struct KrylovSolver::Impl {
                  "AMG: invalid options");
       KA_REQUIRE(k != SolverKind::AmgPcg || options.pre_sweeps == options.post_sweeps,
                  "AMG-PCG requires equal pre/post sweeps");
+      KA_REQUIRE(k != SolverKind::AmgFgmres || options.nullspace == NullSpace::None,
+                 "FGMRES: constant-nullspace operators are not supported");
       if (options.structure == Structure::SymmetricMMatrix) check_m_matrix(a);
       build_hierarchy(a);
       plan_tail();
@@ -108,6 +111,63 @@ struct KrylovSolver::Impl {
     dev.fill_int(status.data(), 0, 3);
   }
   bool bicg() const { return kind == SolverKind::BicgstabJacobi || kind == SolverKind::AmgBicgstab; }
+  bool gmres() const { return kind == SolverKind::FgmresJacobi || kind == SolverKind::AmgFgmres; }
+
+  // Flexible GMRES(m) (Saad 1993), right preconditioned by the V-cycle or the
+  // Jacobi inverse: the cycle's Arnoldi steps, rotations and convergence test
+  // run on the device (kh_gmres.hip); the host polls every
+  // residual_check_interval steps and verifies the true residual after each
+  // cycle, restarting from it.
+  int restart_length = 30;
+  DeviceArray<float> basis_v, basis_z, gmres_w;
+  DeviceArray<double> gmres_block, gmres_partials;
+  std::size_t gmres_target_slot() const {
+    const std::size_t m = std::size_t(restart_length);
+    return (m + 1) * m + 5 * m + 2;
+  }
+  void gmres_buffers() {
+    const std::size_t m = std::size_t(restart_length);
+    if (basis_v.size() != (m + 1) * std::size_t(n)) {
+      basis_v = DeviceArray<float>(dev, (m + 1) * std::size_t(n));
+      basis_z = DeviceArray<float>(dev, m * std::size_t(n));
+    }
+    if (gmres_w.size() != std::size_t(n)) gmres_w = DeviceArray<float>(dev, n);
+    if (gmres_block.size() != gmres_target_slot() + 1) gmres_block = DeviceArray<double>(dev, gmres_target_slot() + 1);
+    const std::size_t blocks = (std::size_t(n) + 255) / 256;
+    if (gmres_partials.size() != blocks * 64) gmres_partials = DeviceArray<double>(dev, std::max<std::size_t>(blocks * 64, 1));
+  }
+  void fgmres(const DeviceCsr& a, float* x, const float* b, const SolveControl& c, double target,
+              SolveStats& st) {
+    const int m = restart_length;
+    gmres_buffers();
+    {
+      std::vector<double> host(gmres_block.size(), 0.0);
+      host[gmres_target_slot()] = target;
+      gmres_block.upload(host);
+    }
+    int restarts = 0;
+    double last = st.initial_residual;
+    while (st.iterations < c.max_iter) {
+      reset(st.iterations);
+      dev.launch(KH_GMRES_START, n, {r, basis_v, gmres_block, m, status, gmres_partials});
+      const int steps = std::min(m, c.max_iter - st.iterations);
+      verified_current = false;
+      for (int j = 0; j < steps; ++j) {
+        float* vj = basis_v.data() + std::size_t(j) * n;
+        float* zj = basis_z.data() + std::size_t(j) * n;
+        apply(a, vj, zj, Tail::None);
+        dev.launch(KH_CSR_SPMV_GUARDED, n, {a.row_ptr, a.col, a.val, zj, gmres_w, status});
+        dev.launch(KH_GMRES_ARNOLDI, n, {basis_v, gmres_w, gmres_block, m, j, status, gmres_partials});
+        if ((j + 1) % c.residual_check_interval == 0 && j + 1 < steps && poll()[0] != 0) break;
+      }
+      dev.launch(KH_GMRES_UPDATE, n, {basis_z, x, gmres_block, m, status});
+      st.iterations = poll()[1];
+      if (verified(a, x, b, target, st) || diverged(st)) break;
+      restarts = st.final_residual >= 0.9 * last ? restarts + 1 : 0;
+      last = st.final_residual;
+      if (restarts >= 12) break;
+    }
+  }
 
   // linalg/amg.cpp:200-226
   void build_hierarchy(const CsrHost& a) {
@@ -166,7 +226,10 @@ struct KrylovSolver::Impl {
   }
 
   // Picks the tail's first level: the requested one, or else the first level
-  // from which the rest of the hierarchy fits one workgroup.
+  // of at most kTailAutoRows rows from which the rest of the hierarchy fits
+  // one workgroup. Above that size one CU's compensated sweeps cost more than
+  // a level's own launches, whose sweeps spread over the whole chip.
+  static constexpr Index kTailAutoRows = 2048;
   void plan_tail() {
     tail_first = 0;
     tail_table = DeviceArray<kh_tail_level>();
@@ -177,7 +240,7 @@ struct KrylovSolver::Impl {
       tail_first = tail_request;
     } else {
       for (std::size_t i = 1; i < levels.size(); ++i)
-        if (tail_fits(i)) {
+        if (levels[i].rows <= kTailAutoRows && tail_fits(i)) {
           tail_first = int(i);
           break;
         }
@@ -572,6 +635,7 @@ struct KrylovSolver::Impl {
     reset(0);
     if (kind == SolverKind::PcgJacobi || kind == SolverKind::AmgPcg) pcg(a, x, b, c, target, st);
     else if (bicg()) bicgstab(a, x, b, c, target, st);
+    else if (gmres()) fgmres(a, x, b, c, target, st);
     else jacobi(a, x, b, c, target, st);
     if (!verified_current || nullspace) verified(a, x, b, target, st);
     if (!std::isfinite(st.final_residual) || st.final_residual > best_residual) {
@@ -633,12 +697,20 @@ int KrylovSolver::set_tail(int first) {
 }
 int KrylovSolver::tail_level() const { return impl_->tail_first; }
 int KrylovSolver::host_waits() const { return impl_->waits; }
+void KrylovSolver::set_restart(int m) {
+  KA_REQUIRE(m >= 1 && m <= 63, "FGMRES: restart length must be in [1, 63]");
+  impl_->restart_length = m;
+}
 
 // linalg/amg.cpp:263-269, linalg/solver.cpp:178-184
 std::string KrylovSolver::describe() const {
   std::ostringstream out;
   const bool pcg = impl_->kind == SolverKind::PcgJacobi || impl_->kind == SolverKind::AmgPcg;
-  out << (impl_->kind == SolverKind::Jacobi ? "damped Jacobi" : pcg ? "PCG" : "BiCGStab") << ", ";
+  out << (impl_->kind == SolverKind::Jacobi ? "damped Jacobi"
+          : pcg                             ? "PCG"
+          : impl_->gmres()                  ? "FGMRES(" + std::to_string(impl_->restart_length) + ")"
+                                            : "BiCGStab")
+      << ", ";
   if (impl_->amg)
     out << "aggregation AMG V-cycle, " << impl_->levels.size() << " levels, op complexity "
         << std::fixed << std::setprecision(3) << impl_->complexity;

@@ -27,7 +27,9 @@ struct SolveStats {
   bool converged = false;
 };
 // linalg/solver.hpp:237-243
-enum class SolverKind : std::uint8_t { Jacobi, PcgJacobi, BicgstabJacobi, AmgPcg, AmgBicgstab };
+enum class SolverKind : std::uint8_t { Jacobi, PcgJacobi, BicgstabJacobi, AmgPcg, AmgBicgstab,
+                                       // Flexible GMRES(m) (no reference counterpart).
+                                       FgmresJacobi, AmgFgmres };
 
 // Non-owning device CSR view (linalg/csr.hpp:31-38).
 struct DeviceCsr {
@@ -67,6 +69,8 @@ class KrylovSolver {
   // Returns the level in use (0: none).
   int set_tail(int first);
   int tail_level() const;
+  // Restart length of the FGMRES kinds (default 30, at most 63).
+  void set_restart(int m);
   int host_waits() const;
 
  private:

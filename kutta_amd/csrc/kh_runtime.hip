@@ -166,6 +166,7 @@ const KernelEntry* find_kernel(uint32_t id) {
     std::unordered_map<uint32_t, const KernelEntry*> m;
     for (const auto& e : linalg_kernels()) m[e.id] = &e;
     for (const auto& e : fv_kernels()) m[e.id] = &e;
+    for (const auto& e : gmres_kernels()) m[e.id] = &e;
     return m;
   }();
   const auto it = index.find(id);
