@@ -349,6 +349,42 @@ def main() -> int:
         "launches": launches,
     }
 
+    # The same solve with the additive solver variants (no reference
+    # counterpart): the mixed-precision V-cycle (first coarse level of at most
+    # 2048 rows solved by an FP64 dense inverse) under PCG, and flexible
+    # GMRES(30) with the plain and the mixed V-cycle. x is compared with the
+    # plain FP32 solution, both within the 1e-6 relative residual.
+    variants = {}
+    x_plain = x.download()
+    for label, kind, direct in (("pcg_mixed", SolverKind.AmgPcg, 2048),
+                                ("fgmres_plain", SolverKind.AmgFgmres, 0),
+                                ("fgmres_mixed", SolverKind.AmgFgmres, 2048)):
+        other = kutta_amd.make_amg_solver(kind, dev, problem.host, options)
+        level = other.set_direct(direct) if direct else 0
+        t = time.perf_counter()
+        other.refresh(problem)
+        dev.synchronize()
+        refresh_ms = (time.perf_counter() - t) * 1e3
+        for _ in range(max(args.warmup, 1)):
+            x.fill(np.float32(0))
+            other.solve(problem, x, b, control)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            x.fill(np.float32(0))
+            vst = other.solve(problem, x, b, control)
+        dev.synchronize()
+        vms = (time.perf_counter() - t0) * 1e3 / args.steps
+        xv = x.download()
+        variants[label] = {"ms": round(vms, 4), "iterations": vst.iterations,
+                           "relative_residual": vst.final_residual / vst.initial_residual,
+                           "converged": vst.converged, "speedup_vs_plain_pcg": round(ms / vms, 3),
+                           "refresh_ms": round(refresh_ms, 2),
+                           "direct_level": level,
+                           "max_abs_diff_vs_plain_over_max_abs_x":
+                               float(np.abs(xv.astype(np.float64) - x_plain).max() / np.abs(x_plain).max())}
+        print(f"[bench] {label}: {vms:.3f} ms, {vst.iterations} iterations", file=sys.stderr, flush=True)
+
     simplec = None
     if args.simplec_size:
         # Configuration 1: 128x128x1 cavity, Re=1000, relaxation 1, monitors
@@ -357,7 +393,7 @@ def main() -> int:
                                 max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20)
 
     baseline = None
-    x_gpu = x.download()
+    x_gpu = x_plain
     if not args.no_cpu_baseline:
         baseline, x_cpu = cpu_baseline(problem, b_host, args.sweeps, st)
         baseline["x_bitwise_equal_gpu"] = bool(np.array_equal(x_cpu.view(np.uint32), x_gpu.view(np.uint32)))
@@ -394,6 +430,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": baseline,
+        "solver_variants": variants,
         "simplec": simplec,
     }
     print(json.dumps(line), flush=True)

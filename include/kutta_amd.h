@@ -114,6 +114,12 @@ int ka_solver_host_waits(void* solver);
 int ka_solver_set_tail(void* solver, int first, int32_t* level);
 /* Restart length m (1..63) of the FGMRES kinds. No reference counterpart. */
 int ka_solver_set_restart(void* solver, int m);
+/* Mixed-precision AMG (no reference counterpart): the first coarse level of
+ * at most `rows` rows (<= 8192) is solved by an FP64 dense inverse computed at
+ * the next ka_solver_refresh; coarser levels and the coarse tail go unused.
+ * rows = 0 turns it off. *level receives the level (0: none). Symmetric
+ * M-matrix operators without nullspace only. */
+int ka_solver_set_direct(void* solver, int rows, int32_t* level);
 
 /* Steady laminar SIMPLEC (fv/steady_solver.hpp:286-360: SteadySolver
  * constructor, iterate, monitor, report, field access) on a box mesh

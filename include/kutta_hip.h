@@ -267,6 +267,11 @@ typedef enum kh_kernel {
   KH_GMRES_ARNOLDI,              /* CGS2 of w against V_0..V_j, rotation, estimate check, V_{j+1}:
                                     V,w,block,m,j,status,partials */
   KH_GMRES_UPDATE,               /* y = H^-1 g over status[2] columns, x += Z y: Z,x,block,m,status */
+  /* FP64 dense coarsest-level solve of the mixed-precision AMG path (no
+     reference counterpart); the operator must be symmetric positive definite. */
+  KH_DENSE_INVERT,               /* Gauss-Jordan inverse of an n-row CSR operator, FP64, no pivoting:
+                                    rows,cols,values,inverse,scratch (n*n doubles each; n <= 8192) */
+  KH_DENSE_APPLY,                /* x = inverse b (FP64 accumulation): inverse,b,x,status */
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 

@@ -113,6 +113,7 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_host_waits.argtypes = [vp]
         lib.ka_solver_set_tail.argtypes = [vp, i32, pi32]
         lib.ka_solver_set_restart.argtypes = [vp, i32]
+        lib.ka_solver_set_direct.argtypes = [vp, i32, pi32]
         lib.ka_aggregate.argtypes = [vp, pi32, dbl, C.POINTER(vp)]
         lib.ka_coarsening_array.argtypes = [vp, i32, pi32]
         lib.ka_coarsening_array.restype = C.c_long
@@ -314,6 +315,8 @@ class Kernel(enum.IntEnum):
     gmres_start = 215
     gmres_arnoldi = 216
     gmres_update = 217
+    dense_invert = 218
+    dense_apply = 219
 
 
 class Buffer:
@@ -631,6 +634,14 @@ class LinearSolver:
         _check_host(host_lib().ka_solver_solve(self.handle, a.handle, x.ptr, b.ptr, c.rel_tol, c.abs_tol,
                                                c.max_iter, c.residual_check_interval, stats))
         return SolveStats(int(stats[0]), stats[1], stats[2], bool(stats[3]))
+
+    def set_direct(self, rows: int) -> int:
+        """Mixed-precision AMG: solve the first coarse level of at most `rows`
+        rows by an FP64 dense inverse (computed at the next refresh); 0 off.
+        Returns that level (0: none)."""
+        level = C.c_int32()
+        _check_host(host_lib().ka_solver_set_direct(self.handle, int(rows), C.byref(level)))
+        return level.value
 
     def set_restart(self, m: int) -> None:
         """Restart length of the FGMRES kinds (1..63; default 30)."""

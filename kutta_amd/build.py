@@ -18,7 +18,7 @@ PKG = ROOT / "kutta_amd"
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["kh_runtime.hip", "kh_linalg.hip", "kh_fv.hip", "kh_gmres.hip"]
+HIP_SOURCES = ["kh_runtime.hip", "kh_linalg.hip", "kh_fv.hip", "kh_gmres.hip", "kh_dense.hip"]
 HOST_SOURCES = ["amg_setup.cpp", "krylov.cpp", "mesh.cpp", "pressure.cpp", "simplec.cpp", "capi.cpp",
                 "capi_mesh.cpp", "capi_simplec.cpp"]
 

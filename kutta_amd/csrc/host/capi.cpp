@@ -181,6 +181,12 @@ int ka_solver_set_fused(void* s, int fused) {
   return guard([&] { static_cast<Solver*>(s)->solver.set_fused(fused != 0); });
 }
 int ka_solver_host_waits(void* s) { return static_cast<Solver*>(s)->solver.host_waits(); }
+int ka_solver_set_direct(void* solver, int rows, int32_t* level) {
+  return guard([&] {
+    const int l = static_cast<Solver*>(solver)->solver.set_direct(rows);
+    if (level) *level = l;
+  });
+}
 int ka_solver_set_restart(void* solver, int m) {
   return guard([&] { static_cast<Solver*>(solver)->solver.set_restart(m); });
 }

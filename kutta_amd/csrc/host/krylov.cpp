@@ -118,6 +118,36 @@ struct KrylovSolver::Impl {
   // run on the device (kh_gmres.hip); the host polls every
   // residual_check_interval steps and verifies the true residual after each
   // cycle, restarting from it.
+  // Mixed-precision AMG (no reference counterpart): level `direct_level` is
+  // solved by an FP64 dense inverse (Gauss-Jordan at refresh, one GEMV per
+  // V-cycle) instead of coarser levels and sweeps; 0 is off.
+  int direct_level = 0;
+  bool direct_ready = false;
+  DeviceArray<double> dense_inverse, dense_scratch;
+  int set_direct(Index rows) {
+    direct_level = 0;
+    direct_ready = false;
+    dense_inverse = DeviceArray<double>();
+    dense_scratch = DeviceArray<double>();
+    if (rows <= 0) {
+      plan_tail();
+      return 0;
+    }
+    KA_REQUIRE(amg && options.structure == Structure::SymmetricMMatrix && !nullspace,
+               "AMG: the dense coarse solve needs a symmetric M-matrix operator without nullspace");
+    KA_REQUIRE(rows <= 8192, "AMG: the dense coarse solve takes at most 8192 rows");
+    for (std::size_t i = 1; i < levels.size(); ++i)
+      if (levels[i].rows <= rows) {
+        direct_level = int(i);
+        break;
+      }
+    KA_REQUIRE(direct_level > 0, "AMG: no coarse level has at most " + std::to_string(rows) + " rows");
+    const std::size_t nc = std::size_t(levels[direct_level].rows);
+    dense_inverse = DeviceArray<double>(dev, nc * nc);
+    dense_scratch = DeviceArray<double>(dev, nc * nc);
+    plan_tail();
+    return direct_level;
+  }
   int restart_length = 30;
   DeviceArray<float> basis_v, basis_z, gmres_w;
   DeviceArray<double> gmres_block, gmres_partials;
@@ -233,7 +263,7 @@ struct KrylovSolver::Impl {
   void plan_tail() {
     tail_first = 0;
     tail_table = DeviceArray<kh_tail_level>();
-    if (!amg || tail_request == 0) return;
+    if (!amg || tail_request == 0 || direct_level > 0) return;
     if (tail_request > 0) {
       KA_REQUIRE(tail_fits(std::size_t(tail_request)),
                  "AMG: coarse tail level " + std::to_string(tail_request) + " does not fit one workgroup");
@@ -296,6 +326,11 @@ struct KrylovSolver::Impl {
       KA_REQUIRE(invalid[i] != 3, "AMG: requires positive diagonals");
       KA_REQUIRE(invalid[i] <= allowed[i], "AMG: invalid values or zero diagonal");
     }
+    if (direct_level > 0) {
+      const DeviceCsr c = levels[direct_level].matrix.view();
+      dev.launch(KH_DENSE_INVERT, c.n_rows, {c.row_ptr, c.col, c.val, dense_inverse, dense_scratch});
+      direct_ready = true;
+    }
   }
 
   // Constant-nullspace projection (linalg/detail.hpp:13-19).
@@ -343,6 +378,11 @@ struct KrylovSolver::Impl {
       dev.launch(KH_FUSED_COARSE_TAIL, 1,
                  {tail_table, int(levels.size() - index), options.pre_sweeps, options.post_sweeps,
                   options.coarse_sweeps, nullspace ? 1 : 0, b, x, st});
+      return result;
+    }
+    if (direct_level > 0 && int(index) == direct_level) {
+      KA_REQUIRE(direct_ready, "AMG: refresh before solving with the dense coarse solve");
+      dev.launch(KH_DENSE_APPLY, a.n_rows, {dense_inverse, b, x, st});
       return result;
     }
     // The reference clears x first; only the unfused sequence reads that zero.
@@ -697,6 +737,7 @@ int KrylovSolver::set_tail(int first) {
 }
 int KrylovSolver::tail_level() const { return impl_->tail_first; }
 int KrylovSolver::host_waits() const { return impl_->waits; }
+int KrylovSolver::set_direct(Index rows) { return impl_->set_direct(rows); }
 void KrylovSolver::set_restart(int m) {
   KA_REQUIRE(m >= 1 && m <= 63, "FGMRES: restart length must be in [1, 63]");
   impl_->restart_length = m;
@@ -716,6 +757,9 @@ std::string KrylovSolver::describe() const {
         << std::fixed << std::setprecision(3) << impl_->complexity;
   else out << "Jacobi";
   if (impl_->nullspace) out << ", constant-nullspace projection";
+  if (impl_->direct_level > 0)
+    out << ", FP64 dense solve of level " << impl_->direct_level << " ("
+        << impl_->levels[impl_->direct_level].rows << " rows)";
   return out.str();
 }
 std::vector<std::pair<Index, Index>> KrylovSolver::levels() const {

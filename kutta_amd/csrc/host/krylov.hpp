@@ -71,6 +71,10 @@ class KrylovSolver {
   int tail_level() const;
   // Restart length of the FGMRES kinds (default 30, at most 63).
   void set_restart(int m);
+  // Mixed-precision AMG: the first coarse level of at most `rows` rows is
+  // solved by an FP64 dense inverse, computed at the next refresh(); 0 turns
+  // it off. Returns that level (0: none). No reference counterpart.
+  int set_direct(Index rows);
   int host_waits() const;
 
  private:

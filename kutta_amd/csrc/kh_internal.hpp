@@ -80,6 +80,7 @@ struct KernelEntry {
 // Defined by each kernel translation unit.
 const std::vector<KernelEntry>& linalg_kernels();
 const std::vector<KernelEntry>& gmres_kernels();
+const std::vector<KernelEntry>& dense_kernels();
 const std::vector<KernelEntry>& fv_kernels();
 
 // Reduction launchers (kh_reduce.hip).
