@@ -43,11 +43,11 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
-def rhs_for(n: int, pinned: int):
+def rhs_for(n: int, pinned: int, seed: int = SEED):
     """Zero-mean Gaussian RHS: numpy PCG64(seed) normals, mean removed in FP64,
     rounded to FP32, zero at the pinned cell."""
     import numpy as np
-    b = np.random.default_rng(SEED).standard_normal(n)
+    b = np.random.default_rng(seed).standard_normal(n)
     b -= b.mean()
     b = b.astype(np.float32)
     if pinned >= 0:
@@ -249,6 +249,18 @@ def main() -> int:
     ap.add_argument("--simplec-iterations", type=int, default=500)
     ap.add_argument("--simplec-box", type=int, default=0, metavar="N",
                     help="only time SIMPLEC on the N^3 lid-driven box (the reference's FV benchmark)")
+    ap.add_argument("--variants", default="all",
+                    help="comma-separated solver variants to time beside the plain solve "
+                         "(pcg_mixed, fgmres_plain, fgmres_mixed, fgmres_learned, fgmres_mixed_learned), "
+                         "'all' or 'none'")
+    ap.add_argument("--learned-train", type=int, default=1000,
+                    help="Adam steps of the learned coarse correction (0 skips the learned variants)")
+    ap.add_argument("--learned-harvest", type=int, default=8,
+                    help="right-hand sides whose plain FGMRES steps are harvested for training")
+    ap.add_argument("--learned-level", type=int, default=2, help="AMG level the network works on")
+    ap.add_argument("--learned-max-unexplained", type=float, default=0.5,
+                    help="largest unexplained fraction of the training targets for which the "
+                         "network is applied (larger: apply it regardless)")
     ap.add_argument("--simplec-only", action="store_true",
                     help="only time SIMPLEC on the 2-D cavity (--simplec-size, --simplec-iterations)")
     args = ap.parse_args()
@@ -356,9 +368,13 @@ def main() -> int:
     # plain FP32 solution, both within the 1e-6 relative residual.
     variants = {}
     x_plain = x.download()
+    wanted = (None if args.variants == "all" else
+              set() if args.variants == "none" else set(args.variants.split(",")))
     for label, kind, direct in (("pcg_mixed", SolverKind.AmgPcg, 2048),
                                 ("fgmres_plain", SolverKind.AmgFgmres, 0),
                                 ("fgmres_mixed", SolverKind.AmgFgmres, 2048)):
+        if wanted is not None and label not in wanted:
+            continue
         other = kutta_amd.make_amg_solver(kind, dev, problem.host, options)
         level = other.set_direct(direct) if direct else 0
         t = time.perf_counter()
@@ -384,6 +400,70 @@ def main() -> int:
                            "max_abs_diff_vs_plain_over_max_abs_x":
                                float(np.abs(xv.astype(np.float64) - x_plain).max() / np.abs(x_plain).max())}
         print(f"[bench] {label}: {vms:.3f} ms, {vst.iterations} iterations", file=sys.stderr, flush=True)
+
+    # The learned coarse correction inside FGMRES(30), over the plain and the
+    # mixed V-cycle: the solver harvests restricted residuals from plain
+    # V-cycle steps on --learned-harvest other right-hand sides, their exact
+    # level corrections are computed by AMG-PCG on that level, the network is
+    # trained on the device (--learned-train Adam steps), and the timed solves
+    # of b then apply it (dropped within a solve whose residual falls by less
+    # than half per step). Harvest, targets and training are setup, reported
+    # beside the timed ms.
+    if args.learned_train > 0:
+        for label, direct in (("fgmres_learned", 0), ("fgmres_mixed_learned", 2048)):
+            if wanted is not None and label not in wanted:
+                continue
+            other = kutta_amd.make_amg_solver(SolverKind.AmgFgmres, dev, problem.host, options)
+            level = other.set_direct(direct) if direct else 0
+            other.refresh(problem)
+            other.set_learned(level=args.learned_level, layers=3, seed=0, capacity=256)
+            other.set_learned_mode(1)
+            t = time.perf_counter()
+            for k in range(args.learned_harvest):
+                x.fill(np.float32(0))
+                other.solve(problem, x, dev.upload(rhs_for(n, problem.reference, SEED + 1 + k)), control)
+            dev.synchronize()
+            harvest_s = time.perf_counter() - t
+            t = time.perf_counter()
+            pairs = other.learned_targets(1e-6)
+            dev.synchronize()
+            targets_s = time.perf_counter() - t
+            t = time.perf_counter()
+            losses, unexplained = other.learned_train(args.learned_train, 1e-3)
+            train_s = time.perf_counter() - t
+            other.set_learned_mode(2, 0.5, args.learned_max_unexplained)
+            for _ in range(max(args.warmup, 1)):
+                x.fill(np.float32(0))
+                other.solve(problem, x, b, control)
+            dev.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                x.fill(np.float32(0))
+                vst = other.solve(problem, x, b, control)
+            dev.synchronize()
+            vms = (time.perf_counter() - t0) * 1e3 / args.steps
+            xv = x.download()
+            rep = other.learned_report()
+            k = max(1, len(losses) // 20)
+            variants[label] = {"ms": round(vms, 4), "iterations": vst.iterations,
+                               "relative_residual": vst.final_residual / vst.initial_residual,
+                               "converged": vst.converged, "speedup_vs_plain_pcg": round(ms / vms, 3),
+                               "direct_level": level, "learned_level": rep["level"],
+                               "learned_rows": rep["rows"], "pairs": pairs,
+                               "train_steps": len(losses),
+                               "loss_first": float(np.mean(losses[:k])), "loss_last": float(np.mean(losses[-k:])),
+                               "unexplained_first": float(np.mean(unexplained[:k])),
+                               "unexplained_last_epoch": rep["relative"],
+                               "harvest_s": round(harvest_s, 3), "targets_s": round(targets_s, 3),
+                               "train_s": round(train_s, 3),
+                               "solves_in_learned_mode": rep["solves"], "gated": rep["gated"],
+                               "fallbacks": rep["fallbacks"],
+                               "max_abs_diff_vs_plain_over_max_abs_x":
+                                   float(np.abs(xv.astype(np.float64) - x_plain).max() / np.abs(x_plain).max())}
+            print(f"[bench] {label}: {vms:.3f} ms, {vst.iterations} iterations, {pairs} pairs, "
+                  f"unexplained {variants[label]['unexplained_first']:.3f} -> {rep['relative']:.3f}, "
+                  f"gated {rep['gated']}/{rep['solves']}, fallbacks {rep['fallbacks']}/{rep['solves']}",
+                  file=sys.stderr, flush=True)
 
     simplec = None
     if args.simplec_size:

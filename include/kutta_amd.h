@@ -97,6 +97,12 @@ int ka_solver_refresh(void* solver, void* matrix);
  * initial residual, final residual, converged. */
 int ka_solver_solve(void* solver, void* matrix, float* x, const float* b, double rel_tol,
                     double abs_tol, int max_iter, int interval, double* stats);
+/* Up to KH_BATCH_MAX_LANES independent systems with fused BiCGStab-Jacobi
+ * solvers (kind 2) at once, one launch per step for all (no reference
+ * counterpart): each system's x and stats[4k..4k+3] are bitwise those of its
+ * own ka_solver_solve. */
+int ka_solver_solve_batched(void* const* solvers, void* const* matrices, float* const* x, const float* const* b,
+                            int count, double rel_tol, double abs_tol, int max_iter, int interval, double* stats);
 int ka_solver_describe(void* solver, char* out, int capacity);
 int ka_solver_levels(void* solver, int32_t* rows, int32_t* nnz, int capacity);
 /* which: 0 aggregate, 1 member_offsets, 2 members, 3 entry_offsets, 4 entries.
@@ -114,6 +120,28 @@ int ka_solver_host_waits(void* solver);
 int ka_solver_set_tail(void* solver, int first, int32_t* level);
 /* Restart length m (1..63) of the FGMRES kinds. No reference counterpart. */
 int ka_solver_set_restart(void* solver, int m);
+/* Learned coarse correction stage of the AMG FGMRES kind (no reference
+ * counterpart). set_learned: a network of `layers` (1..4) message-passing
+ * layers on coarse level `level`, the composite aggregation from the fine
+ * level, and room for `capacity` harvested pairs. mode: 0 off; 1 harvest
+ * (each plain V-cycle step of FGMRES records the restricted residual
+ * R (v - A z)); 2 apply (z += P net(R (v - A z)) after each V-cycle; not applied
+ * when the last training epoch left more than `max_relative` of the targets'
+ * mean square unexplained, and dropped for the rest of a solve when the
+ * residual estimate falls by less than `reduction` per step on average; 0.5
+ * and 0.5 by default). targets: exact level corrections of the harvested
+ * residuals by AMG-PCG to rel_tol. train: Adam steps over the pairs,
+ * losses[steps] and the unexplained fractions relative[steps] (null skipped).
+ * report out[11]: level, rows, mode, harvested, targets, applications,
+ * fallbacks, solves in mode 2, last loss, last epoch's unexplained fraction
+ * (-1 before training), solves not applied by that gate.
+ * parameters: get (set = 0) or set (set = 1) the network's parameters. */
+int ka_solver_set_learned(void* solver, int level, int layers, uint64_t seed, int capacity);
+int ka_solver_learned_mode(void* solver, int mode, double reduction, double max_relative);
+int ka_solver_learned_targets(void* solver, double rel_tol, int32_t* pairs);
+int ka_solver_learned_train(void* solver, int steps, float lr, double* losses, double* relative);
+int ka_solver_learned_report(void* solver, double* out);
+int ka_solver_learned_parameters(void* solver, float* theta, int64_t count, int set);
 /* Mixed-precision AMG (no reference counterpart): the first coarse level of
  * at most `rows` rows (<= 8192) is solved by an FP64 dense inverse computed at
  * the next ka_solver_refresh; coarser levels and the coarse tail go unused.
@@ -157,7 +185,42 @@ int ka_simplec_force_coefficient(const double* report, const double* direction, 
 /* info[4]: cells, pinned reference cell, linear-solver host waits so far,
  * iterations. centroids: 3 doubles per cell of the renumbered mesh. */
 int ka_simplec_info(void* solver, int32_t* info);
+/* Batched momentum solve (no reference counterpart; on by default): the
+ * three components are assembled first and solved by one batched
+ * BiCGStab-Jacobi, which gives bitwise the reference's sequential fields when
+ * every symmetry face is axis-aligned; elsewhere it stays off. *used
+ * receives whether it is in use. */
+int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used);
 int ka_simplec_centroids(void* solver, double* out);
+
+/* ---- Learned coarse correction (no reference counterpart) ----------------
+ * A message-passing network (hidden width 64, `layers` layers, 4 input
+ * features) over a coarse operator's graph; bf16 MFMA layers with FP32
+ * accumulation, FP32 master weights and Adam. Parameter layout: W0 (4x64),
+ * b0 (64), per layer Ws (64x64), Wn (64x64), b (64), then wo (64), bo (1).
+ * Vector arguments r, e, target are device pointers of the graph's node
+ * count; grad, outputs and losses are host arrays. */
+int64_t ka_learned_parameter_count(int layers); /* -1 outside 1..4 */
+int ka_learned_create(void* dev, int layers, uint64_t seed, void** out);
+void ka_learned_destroy(void* net);
+/* Node features and neighbour weights of a symmetric operator with positive
+ * diagonal (a ka_matrix handle); again after its values change. */
+int ka_learned_set_graph(void* net, void* matrix);
+int ka_learned_set_parameters(void* net, const float* theta, int64_t count);
+int ka_learned_parameters(void* net, float* theta, int64_t count);
+/* e = (network output) * rms(r) / mean diagonal. */
+int ka_learned_infer(void* net, const float* r, float* e);
+/* loss[2]: the mean squared error of the normalised output against
+ * target / scale and the mean squared normalised target; grad (count floats)
+ * and the normalised outputs (nodes floats); null outputs are skipped. */
+int ka_learned_gradient(void* net, const float* r, const float* target, float* grad, float* outputs,
+                        double* loss);
+/* `steps` Adam steps cycling over `pairs` (r, target) pairs stored
+ * contiguously; losses[steps] receives the loss before each step and
+ * relative[steps] that loss over the mean squared normalised target (null
+ * arrays are skipped). */
+int ka_learned_train(void* net, const float* r, const float* target, int pairs, int steps, float lr,
+                     double* losses, double* relative);
 
 #ifdef __cplusplus
 }

@@ -272,8 +272,70 @@ typedef enum kh_kernel {
   KH_DENSE_INVERT,               /* Gauss-Jordan inverse of an n-row CSR operator, FP64, no pivoting:
                                     rows,cols,values,inverse,scratch (n*n doubles each; n <= 8192) */
   KH_DENSE_APPLY,                /* x = inverse b (FP64 accumulation): inverse,b,x,status */
+  /* Learned coarse correction (no reference counterpart): a message-passing
+     network of hidden width KH_LEARNED_HIDDEN over a coarse level's graph,
+     bf16 MFMA layers with FP32 accumulation. Gradient kernels store FP32
+     partials per slot (KH_LEARNED_SLOTS rows of `width` floats, one column
+     per parameter plus the loss and the target energy) that
+     KH_LEARNED_UPDATE sums in FP64. n is the node count unless stated. */
+  KH_LEARNED_GRAPH,              /* node features and neighbour weights of an operator:
+                                    row_ptr,col,val,features(2n),ws(nnz),wt(nnz),gstats(1),scratch */
+  KH_LEARNED_NORM,               /* slot sums of r^2: r,scratch */
+  KH_LEARNED_INPUT,              /* features and input layer: r,scratch,features,gstats,theta,x(4n),h0,stats(2) */
+  KH_LEARNED_LAYER,              /* h = relu(h_prev Ws + (S h_prev) Wn + b):
+                                    row_ptr,col,ws,h_prev,wf,bias,h,m */
+  KH_LEARNED_HEAD,               /* e = (h wo + bo) * scale: h,head,stats,e */
+  KH_LEARNED_HEAD_LOSS,          /* y, dy, wo/bo/loss/energy partials: h,head,stats,target,y,dy,partials,width */
+  KH_LEARNED_BACK,               /* one layer's backward: last,dy,head,p_in,dm_in,row_ptr,col,wt,h,wb,
+                                    dz,p_out,dm_out,partials,width */
+  KH_LEARNED_WEIGHT_GRAD,        /* [Ws;Wn] partials: h_prev,m,dz,partials,width */
+  KH_LEARNED_BACK_INPUT,         /* W0/b0 partials: p_in,dm_in,row_ptr,col,wt,h0,x,partials,width */
+  KH_LEARNED_UPDATE,             /* n = parameters + 2: mode (0 pack, 1 gradient, 2 Adam step),
+                                    partials,theta,adam_m,adam_v,grad,loss(2: mean squared error,
+                                    mean squared normalised target),layers,step,lr,nodes,wf,wb */
+  KH_LEARNED_RESTRICT,           /* n = coarse nodes: r = R (b - A z) through composite aggregates:
+                                    row_ptr,col,val,b,z,member_offsets,members,r,scratch (fine rows),
+                                    fine rows,group (lanes per coarse node: a power of two <= 64) */
+  KH_LEARNED_PROLONG,            /* n = fine rows: z += e[map] (map < 0: none): map,e,z */
+  /* Batched BiCGStab-Jacobi (no reference counterpart): the five launches of
+     a fused iteration for up to KH_BATCH_MAX_LANES independent systems of n
+     rows at once (the three momentum components). Arguments: lanes (device
+     array of kh_bicg_lane), lane count, mask of the lanes that step. Each
+     lane's iterates are bitwise those of its own KH_FUSED_* launches. */
+  KH_BATCH_BICG_DIRECTION,
+  KH_BATCH_SPMV_SHADOW_ALPHA,
+  KH_BATCH_BICG_DIFFERENCE_S,
+  KH_BATCH_SPMV_DOT2_OMEGA,
+  KH_BATCH_BICG_STEP_CHECK_BETA,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
+
+/* One system of the batched BiCGStab-Jacobi launches: its operator, its
+ * vectors (x, r, q, p, z, s, t, zs, shadow; n floats each), its Jacobi
+ * inverse, its 24 pair scalars and 3-int status. */
+#define KH_BATCH_MAX_LANES 4
+typedef struct kh_bicg_lane {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const float* val;
+  float* x;
+  float* r;
+  float* q;
+  float* p;
+  float* z;
+  float* s;
+  float* t;
+  float* zs;
+  const float* shadow;
+  const float* inverse;
+  float* scalars;
+  int32_t* status;
+} kh_bicg_lane;
+
+/* Learned coarse correction: hidden width, gradient partial slots, layers. */
+#define KH_LEARNED_HIDDEN 64
+#define KH_LEARNED_SLOTS 256
+#define KH_LEARNED_MAX_LAYERS 4
 
 /* One level of KH_FUSED_COARSE_TAIL: the level's CSR operator (rows, nnz) and
  * l1 inverse diagonal, and the aggregation transfer to the next level (ignored

@@ -104,6 +104,7 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_refresh.argtypes = [vp, vp]
         lib.ka_solver_solve.argtypes = [vp, vp, vp, vp, dbl, dbl, i32, i32, C.POINTER(dbl)]
         lib.ka_solver_describe.argtypes = [vp, C.c_char_p, i32]
+        lib.ka_solver_solve_batched.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, i32, i32, C.POINTER(dbl)]
         lib.ka_solver_levels.argtypes = [vp, pi32, pi32, i32]
         lib.ka_solver_transfer.argtypes = [vp, i32, i32, pi32]
         lib.ka_solver_transfer.restype = C.c_long
@@ -114,6 +115,12 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_set_tail.argtypes = [vp, i32, pi32]
         lib.ka_solver_set_restart.argtypes = [vp, i32]
         lib.ka_solver_set_direct.argtypes = [vp, i32, pi32]
+        lib.ka_solver_set_learned.argtypes = [vp, i32, i32, C.c_uint64, i32]
+        lib.ka_solver_learned_mode.argtypes = [vp, i32, dbl, dbl]
+        lib.ka_solver_learned_targets.argtypes = [vp, dbl, pi32]
+        lib.ka_solver_learned_train.argtypes = [vp, i32, C.c_float, vp, vp]
+        lib.ka_solver_learned_report.argtypes = [vp, C.POINTER(dbl)]
+        lib.ka_solver_learned_parameters.argtypes = [vp, vp, C.c_int64, i32]
         lib.ka_aggregate.argtypes = [vp, pi32, dbl, C.POINTER(vp)]
         lib.ka_coarsening_array.argtypes = [vp, i32, pi32]
         lib.ka_coarsening_array.restype = C.c_long
@@ -132,6 +139,17 @@ def host_lib() -> C.CDLL:
         lib.ka_simplec_force_coefficient.argtypes = [C.POINTER(dbl), C.POINTER(dbl), dbl, dbl, dbl, dbl,
                                                      C.POINTER(dbl)]
         lib.ka_simplec_centroids.argtypes = [vp, vp]
+        lib.ka_simplec_set_batched_momentum.argtypes = [vp, i32, pi32]
+        lib.ka_learned_parameter_count.argtypes = [i32]
+        lib.ka_learned_parameter_count.restype = C.c_int64
+        lib.ka_learned_create.argtypes = [vp, i32, C.c_uint64, C.POINTER(vp)]
+        lib.ka_learned_destroy.argtypes = [vp]
+        lib.ka_learned_set_graph.argtypes = [vp, vp]
+        lib.ka_learned_set_parameters.argtypes = [vp, vp, C.c_int64]
+        lib.ka_learned_parameters.argtypes = [vp, vp, C.c_int64]
+        lib.ka_learned_infer.argtypes = [vp, vp, vp]
+        lib.ka_learned_gradient.argtypes = [vp, vp, vp, vp, vp, C.POINTER(dbl)]
+        lib.ka_learned_train.argtypes = [vp, vp, vp, i32, i32, C.c_float, vp, vp]
         _host = lib
     return _host
 
@@ -317,6 +335,23 @@ class Kernel(enum.IntEnum):
     gmres_update = 217
     dense_invert = 218
     dense_apply = 219
+    learned_graph = 220
+    learned_norm = 221
+    learned_input = 222
+    learned_layer = 223
+    learned_head = 224
+    learned_head_loss = 225
+    learned_back = 226
+    learned_weight_grad = 227
+    learned_back_input = 228
+    learned_update = 229
+    learned_restrict = 230
+    learned_prolong = 231
+    batch_bicg_direction = 232
+    batch_spmv_shadow_alpha = 233
+    batch_bicg_difference_s = 234
+    batch_spmv_dot2_omega = 235
+    batch_bicg_step_check_beta = 236
 
 
 class Buffer:
@@ -643,6 +678,52 @@ class LinearSolver:
         _check_host(host_lib().ka_solver_set_direct(self.handle, int(rows), C.byref(level)))
         return level.value
 
+    def set_learned(self, level: int = 2, layers: int = 3, seed: int = 0, capacity: int = 256) -> None:
+        """Learned coarse correction stage (AMG FGMRES only): a network on
+        coarse level `level` with room for `capacity` harvested pairs."""
+        _check_host(host_lib().ka_solver_set_learned(self.handle, int(level), int(layers), int(seed),
+                                                     int(capacity)))
+
+    def set_learned_mode(self, mode: int, reduction: float = 0.5, max_relative: float = 0.5) -> None:
+        """0 off, 1 harvest, 2 apply: not applied when the last training epoch
+        left more than `max_relative` of the targets' mean square unexplained,
+        dropped within a solve when the residual falls by less than
+        `reduction` per step."""
+        _check_host(host_lib().ka_solver_learned_mode(self.handle, int(mode), float(reduction),
+                                                      float(max_relative)))
+
+    def learned_targets(self, rel_tol: float = 1e-6) -> int:
+        k = C.c_int32()
+        _check_host(host_lib().ka_solver_learned_targets(self.handle, float(rel_tol), C.byref(k)))
+        return k.value
+
+    def learned_train(self, steps: int, lr: float = 1e-3):
+        """(loss, unexplained fraction) before each Adam step."""
+        out = np.empty(max(int(steps), 1), np.float64)
+        rel = np.empty(max(int(steps), 1), np.float64)
+        _check_host(host_lib().ka_solver_learned_train(self.handle, int(steps), float(lr), out.ctypes.data,
+                                                       rel.ctypes.data))
+        return out[:int(steps)], rel[:int(steps)]
+
+    def learned_report(self) -> dict:
+        out = (C.c_double * 11)()
+        _check_host(host_lib().ka_solver_learned_report(self.handle, out))
+        keys = ["level", "rows", "mode", "harvested", "targets", "applications", "fallbacks", "solves"]
+        rep = {k: int(out[i]) for i, k in enumerate(keys)}
+        rep["last_loss"] = out[8]
+        rep["relative"] = out[9]
+        rep["gated"] = int(out[10])
+        return rep
+
+    def learned_parameters(self, layers: int = 3) -> np.ndarray:
+        out = np.empty(LearnedNetwork.parameter_count(layers), np.float32)
+        _check_host(host_lib().ka_solver_learned_parameters(self.handle, out.ctypes.data, out.size, 0))
+        return out
+
+    def set_learned_parameters(self, theta) -> None:
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        _check_host(host_lib().ka_solver_learned_parameters(self.handle, t.ctypes.data, t.size, 1))
+
     def set_restart(self, m: int) -> None:
         """Restart length of the FGMRES kinds (1..63; default 30)."""
         _check_host(host_lib().ka_solver_set_restart(self.handle, int(m)))
@@ -827,6 +908,21 @@ def aggregate(a: CsrMatrixHost, options: AmgOptions | None = None) -> dict:
     return out
 
 
+def solve_batched(solvers, matrices, xs, bs, control: "SolveControl | None" = None) -> list:
+    """Several fused BiCGStab-Jacobi solves at once (no reference
+    counterpart): each x and SolveStats is bitwise its own solve's."""
+    c = control or SolveControl()
+    k = len(solvers)
+    arr = lambda items: (C.c_void_p * k)(*items)
+    stats = (C.c_double * (4 * k))()
+    _check_host(host_lib().ka_solver_solve_batched(arr([s.handle for s in solvers]), arr([m.handle for m in matrices]),
+                                                   arr([x.ptr for x in xs]), arr([b.ptr for b in bs]), k,
+                                                   c.rel_tol, c.abs_tol, c.max_iter, c.residual_check_interval,
+                                                   stats))
+    return [SolveStats(int(stats[4 * i]), stats[4 * i + 1], stats[4 * i + 2], bool(stats[4 * i + 3]))
+            for i in range(k)]
+
+
 def make_amg_solver(kind: SolverKind, device: Device, a: CsrMatrixHost,
                     options: AmgOptions | None = None) -> LinearSolver:
     """linalg/amg.hpp:22-27 (only AmgPcg / AmgBicgstab)."""
@@ -979,8 +1075,82 @@ class SteadySolver:
         _check_host(host_lib().ka_simplec_info(self.handle, out))
         return {"cells": out[0], "reference": out[1], "host_waits": out[2], "iteration": out[3]}
 
+    def set_batched_momentum(self, on: bool) -> bool:
+        """Solve u, v, w in one batched BiCGStab (on by default where it is
+        bitwise the sequential order); returns whether it is in use."""
+        used = C.c_int32()
+        _check_host(host_lib().ka_simplec_set_batched_momentum(self.handle, int(bool(on)), C.byref(used)))
+        return bool(used.value)
+
     def centroids(self) -> np.ndarray:
         n = self.info()["cells"]
         out = np.empty((n, 3), np.float64)
         _check_host(host_lib().ka_simplec_centroids(self.handle, out.ctypes.data))
         return out
+
+
+# ------------------------------------------------- learned coarse correction
+class LearnedNetwork:
+    """Message-passing network over a coarse operator's graph (no reference
+    counterpart): hidden width 64, `layers` layers, bf16 MFMA layers with FP32
+    accumulation, trained with Adam on the device. Vectors are device Buffers
+    of the graph's node count."""
+
+    HIDDEN = 64
+
+    def __init__(self, device: Device, layers: int = 3, seed: int = 0):
+        h = C.c_void_p()
+        _check_host(host_lib().ka_learned_create(device.handle, int(layers), int(seed), C.byref(h)))
+        self.device = device
+        self.handle = h.value
+        self.layers = int(layers)
+        self.count = int(host_lib().ka_learned_parameter_count(int(layers)))
+
+    @staticmethod
+    def parameter_count(layers: int) -> int:
+        return int(host_lib().ka_learned_parameter_count(int(layers)))
+
+    def set_graph(self, a: CsrMatrix) -> None:
+        _check_host(host_lib().ka_learned_set_graph(self.handle, a.handle))
+
+    def set_parameters(self, theta) -> None:
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        _check_host(host_lib().ka_learned_set_parameters(self.handle, t.ctypes.data, t.size))
+
+    def parameters(self) -> np.ndarray:
+        out = np.empty(self.count, np.float32)
+        _check_host(host_lib().ka_learned_parameters(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def infer(self, r: Buffer, e: Buffer) -> None:
+        _check_host(host_lib().ka_learned_infer(self.handle, r.ptr, e.ptr))
+
+    def gradient(self, r: Buffer, target: Buffer, nodes: int):
+        """(loss, gradient, normalised outputs y) at the current parameters."""
+        grad = np.empty(self.count, np.float32)
+        y = np.empty(nodes, np.float32)
+        loss = (C.c_double * 2)()
+        _check_host(host_lib().ka_learned_gradient(self.handle, r.ptr, target.ptr, grad.ctypes.data,
+                                                   y.ctypes.data, loss))
+        return loss[0], grad, y
+
+    def train(self, r: Buffer, target: Buffer, pairs: int, steps: int, lr: float = 1e-3,
+              relative: bool = False):
+        """Losses before each Adam step (and, with relative=True, the
+        unexplained fractions as a second array)."""
+        losses = np.empty(max(int(steps), 1), np.float64)
+        rel = np.empty(max(int(steps), 1), np.float64)
+        _check_host(host_lib().ka_learned_train(self.handle, r.ptr, target.ptr, int(pairs), int(steps),
+                                                float(lr), losses.ctypes.data, rel.ctypes.data))
+        return (losses[:int(steps)], rel[:int(steps)]) if relative else losses[:int(steps)]
+
+    def close(self) -> None:
+        if self.handle:
+            host_lib().ka_learned_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -18,9 +18,10 @@ PKG = ROOT / "kutta_amd"
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["kh_runtime.hip", "kh_linalg.hip", "kh_fv.hip", "kh_gmres.hip", "kh_dense.hip"]
+HIP_SOURCES = ["kh_runtime.hip", "kh_linalg.hip", "kh_fv.hip", "kh_gmres.hip", "kh_dense.hip",
+               "kh_learned.hip"]
 HOST_SOURCES = ["amg_setup.cpp", "krylov.cpp", "mesh.cpp", "pressure.cpp", "simplec.cpp", "capi.cpp",
-                "capi_mesh.cpp", "capi_simplec.cpp"]
+                "capi_mesh.cpp", "capi_simplec.cpp", "learned.cpp", "capi_learned.cpp"]
 
 # Device arithmetic contract: no contraction of a*b+c, IEEE division and sqrt,
 # denormals kept, packed FP32 instructions disabled.

@@ -1,8 +1,11 @@
 // C ABI of the host library (include/kutta_amd.h).
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "capi_internal.hpp"
+#include "learned.hpp"
 #include "kutta_amd.h"
 
 using namespace kutta_amd;
@@ -139,6 +142,30 @@ int ka_solver_solve(void* s, void* m, float* x, const float* b, double rel_tol, 
     stats[3] = st.converged ? 1 : 0;
   });
 }
+int ka_solver_solve_batched(void* const* solvers, void* const* matrices, float* const* x, const float* const* b,
+                            int count, double rel_tol, double abs_tol, int max_iter, int interval, double* stats) {
+  return guard([&] {
+    KA_REQUIRE(count >= 1 && count <= KH_BATCH_MAX_LANES, "batched solve: bad system count");
+    SolveControl c{rel_tol, abs_tol, max_iter, interval};
+    std::vector<KrylovSolver*> ss;
+    std::vector<DeviceCsr> as;
+    std::vector<float*> xs;
+    std::vector<const float*> bs;
+    for (int k = 0; k < count; ++k) {
+      ss.push_back(&static_cast<Solver*>(solvers[k])->solver);
+      as.push_back(view_of(matrices[k]));
+      xs.push_back(x[k]);
+      bs.push_back(b[k]);
+    }
+    const auto st = KrylovSolver::solve_batched(ss, as, xs, bs, c);
+    for (int k = 0; k < count; ++k) {
+      stats[4 * k] = st[k].iterations;
+      stats[4 * k + 1] = st[k].initial_residual;
+      stats[4 * k + 2] = st[k].final_residual;
+      stats[4 * k + 3] = st[k].converged ? 1 : 0;
+    }
+  });
+}
 int ka_solver_describe(void* s, char* out, int capacity) {
   const auto text = static_cast<Solver*>(s)->solver.describe();
   if (capacity > 0) {
@@ -185,6 +212,46 @@ int ka_solver_set_direct(void* solver, int rows, int32_t* level) {
   return guard([&] {
     const int l = static_cast<Solver*>(solver)->solver.set_direct(rows);
     if (level) *level = l;
+  });
+}
+int ka_solver_set_learned(void* solver, int level, int layers, uint64_t seed, int capacity) {
+  return guard([&] { static_cast<Solver*>(solver)->solver.set_learned(level, layers, seed, capacity); });
+}
+int ka_solver_learned_mode(void* solver, int mode, double reduction, double max_relative) {
+  return guard([&] { static_cast<Solver*>(solver)->solver.set_learned_mode(mode, reduction, max_relative); });
+}
+int ka_solver_learned_targets(void* solver, double rel_tol, int32_t* pairs) {
+  return guard([&] {
+    const int k = static_cast<Solver*>(solver)->solver.learned_targets(rel_tol);
+    if (pairs) *pairs = k;
+  });
+}
+int ka_solver_learned_train(void* solver, int steps, float lr, double* losses, double* relative) {
+  return guard([&] {
+    const auto l = static_cast<Solver*>(solver)->solver.learned_train(steps, lr);
+    if (losses) std::copy(l.loss.begin(), l.loss.end(), losses);
+    if (relative) std::copy(l.relative.begin(), l.relative.end(), relative);
+  });
+}
+int ka_solver_learned_report(void* solver, double* out) {
+  return guard([&] {
+    const auto r = static_cast<Solver*>(solver)->solver.learned_report();
+    const double v[11] = {double(r.level), double(r.rows), double(r.mode), double(r.harvested),
+                          double(r.targets), double(r.applications), double(r.fallbacks),
+                          double(r.solves), r.last_loss, r.relative, double(r.gated)};
+    std::copy(v, v + 11, out);
+  });
+}
+int ka_solver_learned_parameters(void* solver, float* theta, int64_t count, int set) {
+  return guard([&] {
+    auto& s = static_cast<Solver*>(solver)->solver;
+    if (set) {
+      s.set_learned_parameters(std::vector<float>(theta, theta + count));
+    } else {
+      const auto v = s.learned_parameters();
+      KA_REQUIRE(int64_t(v.size()) == count, "learned: parameter count mismatch");
+      std::copy(v.begin(), v.end(), theta);
+    }
   });
 }
 int ka_solver_set_restart(void* solver, int m) {

@@ -10,10 +10,14 @@
 #include "krylov.hpp"
 
 #include <algorithm>
+#include <cstdint>
+#include <cstring>
 #include <array>
 #include <cmath>
 #include <iomanip>
 #include <sstream>
+
+#include "learned.hpp"
 
 namespace kutta_amd {
 
@@ -36,6 +40,7 @@ std::array<float, 2> pair_of(double value) {
 struct Level {
   Index rows = 0, nnz = 0;
   OwnedCsr matrix;  // coarse levels only; level 0 uses the caller's operator
+  CsrHost host;     // coarse levels: the operator as built (pattern and setup values)
   DeviceArray<float> l1, x, b, r, projection;
   bool has_transfer = false;
   DeviceArray<Index> aggregate, member_offsets, members, entry_offsets, entries;
@@ -166,6 +171,133 @@ struct KrylovSolver::Impl {
     const std::size_t blocks = (std::size_t(n) + 255) / 256;
     if (gmres_partials.size() != blocks * 64) gmres_partials = DeviceArray<double>(dev, std::max<std::size_t>(blocks * 64, 1));
   }
+  // Learned coarse correction stage (no reference counterpart): in FGMRES,
+  // after the V-cycle z = M v, the network maps the residual v - A z,
+  // restricted to level `learned_level` through the composite aggregates, to
+  // a correction that is prolonged back and added to z. Mode 1 harvests the
+  // restricted residuals of plain V-cycle steps instead; their exact
+  // corrections (learned_targets) are level solves by AMG-PCG. Plain AMG
+  // stays the fallback twice over: a network whose last training epoch left
+  // more than learned_max_relative of its targets' mean square unexplained
+  // is not applied at all, and within a solve the stage is dropped for the
+  // remaining steps when the FGMRES residual estimate falls by less than
+  // learned_reduction per step on average since the cycle began (checked at
+  // each poll).
+  int learned_level = 0, learned_mode = 0;
+  double learned_reduction = 0.5, learned_max_relative = 0.5;
+  double learned_relative = -1;  // unexplained fraction after training; < 0: not trained here
+  std::unique_ptr<LearnedNetwork> learned;
+  std::unique_ptr<KrylovSolver> level_solver;
+  DeviceArray<Index> learned_map, learned_offsets, learned_members;
+  DeviceArray<float> learned_r, learned_e, learned_fine, harvest, targets;
+  int learned_group = 1;  // lanes per coarse node in the restriction gather
+  int harvest_capacity = 0, harvested = 0, targeted = 0;
+  long learned_applications = 0, learned_fallbacks = 0, learned_solves = 0, learned_gated = 0;
+  double learned_loss = 0;
+  bool learned_active = false;
+
+  void set_learned(int level, int layers, std::uint64_t seed, int capacity) {
+    KA_REQUIRE(amg && gmres(), "learned stage: needs the AMG FGMRES solver kind");
+    KA_REQUIRE(level >= 1 && std::size_t(level) < levels.size(),
+               "learned stage: level must be a coarse level of the hierarchy");
+    KA_REQUIRE(capacity >= 0, "learned stage: negative harvest capacity");
+    const Level& target_level = levels[level];
+    const Index rows = target_level.rows;
+    // Composite aggregate of every fine row (-1: not aggregated).
+    std::vector<Index> map(static_cast<std::size_t>(n));
+    for (Index i = 0; i < n; ++i) map[i] = i;
+    for (int k = 0; k < level; ++k) {
+      const auto agg = levels[k].aggregate.download();
+      for (auto& v : map) v = v >= 0 ? agg[std::size_t(v)] : -1;
+    }
+    std::vector<Index> offsets(std::size_t(rows) + 1, 0), members;
+    for (Index v : map)
+      if (v >= 0) ++offsets[std::size_t(v) + 1];
+    for (Index a = 0; a < rows; ++a) offsets[a + 1] += offsets[a];
+    members.resize(std::size_t(offsets.back()));
+    std::vector<Index> cursor(offsets.begin(), offsets.end() - 1);
+    for (Index i = 0; i < n; ++i)
+      if (map[i] >= 0) members[std::size_t(cursor[map[i]]++)] = i;
+    learned_level = level;
+    learned_map = to_device(dev, map);
+    learned_offsets = to_device(dev, offsets);
+    learned_members = to_device(dev, members.empty() ? std::vector<Index>{0} : members);
+    learned_r = DeviceArray<float>(dev, rows);
+    learned_e = DeviceArray<float>(dev, rows);
+    learned_fine = DeviceArray<float>(dev, n);
+    learned_group = 1;
+    while (learned_group < 64 && std::size_t(learned_group) * std::size_t(rows) < members.size()) learned_group *= 2;
+    learned = std::make_unique<LearnedNetwork>(dev, layers, seed);
+    learned->set_graph(target_level.matrix.view());
+    AmgOptions sub;
+    sub.structure = options.structure;
+    level_solver = std::make_unique<KrylovSolver>(dev, SolverKind::AmgPcg, target_level.host, sub);
+    harvest_capacity = capacity;
+    harvest = capacity ? DeviceArray<float>(dev, std::size_t(capacity) * rows) : DeviceArray<float>();
+    targets = capacity ? DeviceArray<float>(dev, std::size_t(capacity) * rows) : DeviceArray<float>();
+    harvested = targeted = 0;
+    learned_applications = learned_fallbacks = learned_solves = learned_gated = 0;
+    learned_relative = -1;
+    learned_mode = 0;
+  }
+  // Exact corrections of the harvested residuals: A_l e = r by AMG-PCG on the
+  // level operator (current values) to `rel_tol`.
+  int learned_targets(double rel_tol) {
+    KA_REQUIRE(learned != nullptr, "learned stage: not configured");
+    const DeviceCsr op = levels[learned_level].matrix.view();
+    level_solver->refresh(op);
+    const std::size_t rows = std::size_t(op.n_rows);
+    SolveControl control{rel_tol, 1e-30, 500, 8};
+    for (int k = targeted; k < harvested; ++k) {
+      float* e = targets.data() + std::size_t(k) * rows;
+      dev.fill(e, 0.0f, rows);
+      level_solver->solve(op, e, harvest.data() + std::size_t(k) * rows, control);
+    }
+    targeted = harvested;
+    return targeted;
+  }
+  TrainResult learned_train(int steps, float lr) {
+    KA_REQUIRE(learned != nullptr, "learned stage: not configured");
+    KA_REQUIRE(targeted > 0, "learned stage: no harvested pairs with targets");
+    auto result = learned->train(harvest.data(), targets.data(), targeted, steps, lr);
+    if (!result.loss.empty()) {
+      learned_loss = result.loss.back();
+      // The last epoch: one step per pair (or every step, if fewer).
+      const std::size_t k = std::min<std::size_t>(std::size_t(targeted), result.relative.size());
+      double sum = 0;
+      for (std::size_t i = result.relative.size() - k; i < result.relative.size(); ++i) sum += result.relative[i];
+      learned_relative = sum / double(k);
+    }
+    return result;
+  }
+  void learned_stage(const DeviceCsr& a, const float* v, float* zj) {
+    const Index rows = levels[learned_level].rows;
+    dev.launch(KH_LEARNED_RESTRICT, rows,
+               {a.row_ptr, a.col, a.val, v, zj, learned_offsets, learned_members, learned_r, learned_fine,
+                int(n), learned_group});
+    learned->infer(learned_r.data(), learned_e.data());
+    dev.launch(KH_LEARNED_PROLONG, n, {learned_map, learned_e, zj});
+    ++learned_applications;
+  }
+  void learned_harvest(const DeviceCsr& a, const float* v, const float* zj) {
+    const Index rows = levels[learned_level].rows;
+    dev.launch(KH_LEARNED_RESTRICT, rows,
+               {a.row_ptr, a.col, a.val, v, zj, learned_offsets, learned_members,
+                harvest.data() + std::size_t(harvested) * rows, learned_fine, int(n), learned_group});
+    ++harvested;
+  }
+  // Whether the residual estimate has fallen by at least learned_reduction
+  // per step on average over the first `steps` steps of the cycle.
+  bool learned_keeps_pace(int steps) {
+    const std::size_t m = std::size_t(restart_length), g = (m + 1) * m;
+    ++waits;
+    const auto block = gmres_block.download();
+    const double start = std::fabs(block[g]), now = std::fabs(block[g + std::size_t(steps)]);
+    if (!(start > 0) || !std::isfinite(now)) return false;
+    const double factor = std::pow(now / start, 1.0 / steps);
+    return 1.0 - factor >= learned_reduction;
+  }
+
   void fgmres(const DeviceCsr& a, float* x, const float* b, const SolveControl& c, double target,
               SolveStats& st) {
     const int m = restart_length;
@@ -175,23 +307,43 @@ struct KrylovSolver::Impl {
       host[gmres_target_slot()] = target;
       gmres_block.upload(host);
     }
+    learned_active = learned && learned_mode == 2;
+    if (learned_active) {
+      ++learned_solves;
+      if (learned_relative > learned_max_relative) {
+        learned_active = false;
+        ++learned_gated;
+      }
+    }
     int restarts = 0;
     double last = st.initial_residual;
     while (st.iterations < c.max_iter) {
       reset(st.iterations);
       dev.launch(KH_GMRES_START, n, {r, basis_v, gmres_block, m, status, gmres_partials});
       const int steps = std::min(m, c.max_iter - st.iterations);
+      const int harvest_start = harvested;
       verified_current = false;
       for (int j = 0; j < steps; ++j) {
         float* vj = basis_v.data() + std::size_t(j) * n;
         float* zj = basis_z.data() + std::size_t(j) * n;
         apply(a, vj, zj, Tail::None);
+        if (learned_active) learned_stage(a, vj, zj);
+        else if (learned && learned_mode == 1 && harvested < harvest_capacity) learned_harvest(a, vj, zj);
         dev.launch(KH_CSR_SPMV_GUARDED, n, {a.row_ptr, a.col, a.val, zj, gmres_w, status});
         dev.launch(KH_GMRES_ARNOLDI, n, {basis_v, gmres_w, gmres_block, m, j, status, gmres_partials});
-        if ((j + 1) % c.residual_check_interval == 0 && j + 1 < steps && poll()[0] != 0) break;
+        if ((j + 1) % c.residual_check_interval == 0 && j + 1 < steps) {
+          if (poll()[0] != 0) break;
+          if (learned_active && !learned_keeps_pace(j + 1)) {
+            learned_active = false;
+            ++learned_fallbacks;
+          }
+        }
       }
       dev.launch(KH_GMRES_UPDATE, n, {basis_z, x, gmres_block, m, status});
-      st.iterations = poll()[1];
+      const auto state = poll();
+      st.iterations = state[1];
+      // Only the steps whose columns were built hold meaningful residuals.
+      harvested = std::min(harvested, harvest_start + int(state[2]));
       if (verified(a, x, b, target, st) || diverged(st)) break;
       restarts = st.final_residual >= 0.9 * last ? restarts + 1 : 0;
       last = st.final_residual;
@@ -232,6 +384,7 @@ struct KrylovSolver::Impl {
       coarse.r = DeviceArray<float>(dev, c.matrix.n_rows);
       if (nullspace) coarse.projection = DeviceArray<float>(dev, 2);
       coarse.matrix = OwnedCsr(dev, c.matrix);
+      coarse.host = c.matrix;
       owned = std::move(c.matrix);
       levels.push_back(std::move(coarse));
       current = &owned;
@@ -331,6 +484,7 @@ struct KrylovSolver::Impl {
       dev.launch(KH_DENSE_INVERT, c.n_rows, {c.row_ptr, c.col, c.val, dense_inverse, dense_scratch});
       direct_ready = true;
     }
+    if (learned) learned->set_graph(levels[learned_level].matrix.view());
   }
 
   // Constant-nullspace projection (linalg/detail.hpp:13-19).
@@ -577,63 +731,131 @@ struct KrylovSolver::Impl {
   // linalg/solver.cpp:314-361. With the Jacobi preconditioner and fused
   // kernels an iteration is five launches (each with its reduction and scalar
   // update); the first rho and beta after a restart are one more.
+  // linalg/solver.cpp:314-361, as a resumable loop: the host logic between
+  // polls is shared by bicgstab() and the batched driver, which interleaves
+  // several systems' batches (KrylovSolver::solve_batched).
+  struct BicgRun {
+    const DeviceCsr* a = nullptr;
+    float* x = nullptr;
+    const float* b = nullptr;
+    SolveControl c;
+    double target = 0;
+    SolveStats* st = nullptr;
+    int interval = 1, restarts = 0;
+    double last = 0;
+    bool done = false;
+  };
+  bool jacobi_fused() const { return fused && !amg; }
+  void bicg_restart(const SolveStats& st) {
+    reset(st.iterations);
+    dev.copy(shadow.data(), r.data(), sizeof(float) * n);
+    dev.fill(p.data(), 0.0f, n);
+    dev.fill(q.data(), 0.0f, n);
+    if (jacobi_fused()) dev.launch(KH_FUSED_DOT_BICG_BETA, n, {shadow, r, scalars, status});
+  }
+  void bicg_begin(BicgRun& run) {
+    bicg_restart(*run.st);
+    run.restarts = 0;
+    run.last = run.st->initial_residual;
+    run.interval = batch_interval(run.c);
+    run.done = !(run.st->iterations < run.c.max_iter);
+  }
+  int bicg_next_batch(const BicgRun& run) {
+    verified_current = false;
+    return std::min(run.interval, run.c.max_iter - run.st->iterations);
+  }
+  void bicg_after_poll(BicgRun& run, const std::array<Index, 3>& state) {
+    SolveStats& st = *run.st;
+    st.iterations = state[1];
+    if (state[0] == 2) {
+      run.done = true;
+      return;
+    }
+    if (state[0] == 1) {
+      if (!replacement(*run.a, run.x, run.b, run.target, st, run.restarts, run.last) ||
+          st.iterations == run.c.max_iter) {
+        run.done = true;
+        return;
+      }
+      if (st.final_residual <= 10 * run.target) run.interval = 1;
+      bicg_restart(st);
+    }
+    if (!(st.iterations < run.c.max_iter)) run.done = true;
+  }
+  void bicg_fused_step(const DeviceCsr& a, float* x) {
+    dev.launch(KH_FUSED_BICG_DIRECTION, n, {r, q, p, jacobi_inverse, z, scalars, status});
+    dev.launch(KH_FUSED_SPMV_SHADOW_ALPHA, n, {a.row_ptr, a.col, a.val, z, q, shadow, scalars, status});
+    dev.launch(KH_FUSED_BICG_DIFFERENCE_S, n, {r, q, s, jacobi_inverse, zs, scalars, status});
+    dev.launch(KH_FUSED_SPMV_DOT2_OMEGA, n, {a.row_ptr, a.col, a.val, zs, t, s, scalars, status});
+    dev.launch(KH_FUSED_BICG_STEP_CHECK_BETA, n, {x, r, z, zs, s, t, shadow, scalars, status});
+  }
+  void bicg_reference_step(const DeviceCsr& a, float* x) {
+    dot_into(shadow.data(), r.data(), 0);
+    dev.launch(KH_SOLVER_BICG_BETA, 1, {scalars, status});
+    dev.launch(KH_SOLVER_BICG_DIRECTION, n, {r, q, p, scalars, status});
+    apply(a, p.data(), z.data(), Tail::None);
+    dev.launch(KH_CSR_SPMV_GUARDED, n, {a.row_ptr, a.col, a.val, z, q, status});
+    dot_into(shadow.data(), q.data(), 2);
+    dev.launch(KH_SOLVER_BICG_ALPHA, 1, {scalars, status});
+    dev.launch(KH_SOLVER_BICG_DIFFERENCE, n, {r, q, s, scalars, status});
+    dot_into(s.data(), s.data(), 16);
+    dev.launch(KH_SOLVER_BICG_S, 1, {scalars, status});
+    apply(a, s.data(), zs.data(), Tail::None);
+    dev.launch(KH_CSR_SPMV_GUARDED, n, {a.row_ptr, a.col, a.val, zs, t, status});
+    dev.reduce_into(KH_INTO_DOT2, t.data(), t.data(), t.data(), s.data(), n, scalars.data(), 2);
+    dev.launch(KH_SOLVER_BICG_OMEGA, 1, {scalars, status});
+    dev.launch(KH_SOLVER_BICG_STEP, n, {x, r, z, zs, s, t, scalars, status});
+    dot_into(r.data(), r.data(), 18);
+    dev.launch(KH_SOLVER_CHECK, 1, {scalars, status, 0});
+  }
   void bicgstab(const DeviceCsr& a, float* x, const float* b, const SolveControl& c, double target,
                 SolveStats& st) {
-    const bool jacobi_fused = fused && !amg;
-    auto restart = [&] {
-      reset(st.iterations);
-      dev.copy(shadow.data(), r.data(), sizeof(float) * n);
-      dev.fill(p.data(), 0.0f, n);
-      dev.fill(q.data(), 0.0f, n);
-      if (jacobi_fused) dev.launch(KH_FUSED_DOT_BICG_BETA, n, {shadow, r, scalars, status});
-    };
-    restart();
-    int restarts = 0;
-    double last = st.initial_residual;
-    int interval = batch_interval(c);
-    while (st.iterations < c.max_iter) {
-      const int batch = std::min(interval, c.max_iter - st.iterations);
-      verified_current = false;
+    BicgRun run;
+    run.a = &a;
+    run.x = x;
+    run.b = b;
+    run.c = c;
+    run.target = target;
+    run.st = &st;
+    bicg_begin(run);
+    while (!run.done) {
+      const int batch = bicg_next_batch(run);
       for (int j = 0; j < batch; ++j) {
-        if (jacobi_fused) {
-          dev.launch(KH_FUSED_BICG_DIRECTION, n, {r, q, p, jacobi_inverse, z, scalars, status});
-          dev.launch(KH_FUSED_SPMV_SHADOW_ALPHA, n, {a.row_ptr, a.col, a.val, z, q, shadow, scalars, status});
-          dev.launch(KH_FUSED_BICG_DIFFERENCE_S, n, {r, q, s, jacobi_inverse, zs, scalars, status});
-          dev.launch(KH_FUSED_SPMV_DOT2_OMEGA, n, {a.row_ptr, a.col, a.val, zs, t, s, scalars, status});
-          dev.launch(KH_FUSED_BICG_STEP_CHECK_BETA, n, {x, r, z, zs, s, t, shadow, scalars, status});
-          continue;
-        }
-        dot_into(shadow.data(), r.data(), 0);
-        dev.launch(KH_SOLVER_BICG_BETA, 1, {scalars, status});
-        dev.launch(KH_SOLVER_BICG_DIRECTION, n, {r, q, p, scalars, status});
-        apply(a, p.data(), z.data(), Tail::None);
-        dev.launch(KH_CSR_SPMV_GUARDED, n, {a.row_ptr, a.col, a.val, z, q, status});
-        dot_into(shadow.data(), q.data(), 2);
-        dev.launch(KH_SOLVER_BICG_ALPHA, 1, {scalars, status});
-        dev.launch(KH_SOLVER_BICG_DIFFERENCE, n, {r, q, s, scalars, status});
-        dot_into(s.data(), s.data(), 16);
-        dev.launch(KH_SOLVER_BICG_S, 1, {scalars, status});
-        apply(a, s.data(), zs.data(), Tail::None);
-        dev.launch(KH_CSR_SPMV_GUARDED, n, {a.row_ptr, a.col, a.val, zs, t, status});
-        dev.reduce_into(KH_INTO_DOT2, t.data(), t.data(), t.data(), s.data(), n, scalars.data(), 2);
-        dev.launch(KH_SOLVER_BICG_OMEGA, 1, {scalars, status});
-        dev.launch(KH_SOLVER_BICG_STEP, n, {x, r, z, zs, s, t, scalars, status});
-        dot_into(r.data(), r.data(), 18);
-        dev.launch(KH_SOLVER_CHECK, 1, {scalars, status, 0});
+        if (jacobi_fused()) bicg_fused_step(a, x);
+        else bicg_reference_step(a, x);
       }
-      const auto state = poll();
-      st.iterations = state[1];
-      if (state[0] == 2) break;
-      if (state[0] == 1) {
-        if (!replacement(a, x, b, target, st, restarts, last) || st.iterations == c.max_iter) break;
-        if (st.final_residual <= 10 * target) interval = 1;
-        restart();
-      }
+      bicg_after_poll(run, poll());
     }
   }
+  // The lane of this solver in a batched launch.
+  kh_bicg_lane lane(const DeviceCsr& a, float* x) const {
+    kh_bicg_lane l{};
+    l.row_ptr = a.row_ptr;
+    l.col = a.col;
+    l.val = a.val;
+    l.x = x;
+    l.r = r.data();
+    l.q = q.data();
+    l.p = p.data();
+    l.z = z.data();
+    l.s = s.data();
+    l.t = t.data();
+    l.zs = zs.data();
+    l.shadow = shadow.data();
+    l.inverse = jacobi_inverse.data();
+    l.scalars = scalars.data();
+    l.status = status.data();
+    return l;
+  }
+  DeviceArray<kh_bicg_lane> batch_table;
+  std::vector<kh_bicg_lane> batch_table_host;
+  DeviceArray<Index> batch_status;
 
-  // linalg/solver.cpp:97-176
-  SolveStats solve(const DeviceCsr& a, float* x, const float* b, const SolveControl& c) {
+  // linalg/solver.cpp:97-176, in three parts: the entry (norms, the early
+  // exits, limits), the method's loop, and the exit (best-x recovery).
+  // solve_begin returns false when no iteration is needed.
+  bool solve_begin(const DeviceCsr& a, float* x, const float* b, const SolveControl& c, SolveStats& st,
+                   double& target) {
     KA_REQUIRE(a.n_rows == n && a.nnz == nnz, "solver: device or matrix mismatch");
     KA_REQUIRE(x != nullptr && b != nullptr && x != b, "solve: vector size or alias mismatch");
     KA_REQUIRE(c.max_iter >= 0 && std::isfinite(c.rel_tol) && c.rel_tol >= 0 &&
@@ -646,7 +868,7 @@ struct KrylovSolver::Impl {
     const double bnorm = std::sqrt(entry[0]);
     if (!(std::isfinite(bnorm) && std::isfinite(std::sqrt(entry[1]))))
       throw NonfiniteError("solve: nonfinite input vector");
-    SolveStats st;
+    st = SolveStats();
     if (nullspace) {
       if (n != 0) {
         KA_REQUIRE(std::abs(entry[2]) <= 1e-6 * std::sqrt(double(n)) * bnorm,
@@ -658,11 +880,11 @@ struct KrylovSolver::Impl {
     } else st.initial_residual = std::sqrt(entry[2]);
     st.final_residual = st.initial_residual;
     verified_current = true;
-    const double target = std::max(c.abs_tol, c.rel_tol * st.initial_residual);
-    if (!std::isfinite(st.initial_residual)) return st;
+    target = std::max(c.abs_tol, c.rel_tol * st.initial_residual);
+    if (!std::isfinite(st.initial_residual)) return false;
     if (st.final_residual <= target) {
       st.converged = true;
-      return st;
+      return false;
     }
     if (nullspace) project(r.data(), n, projection.data());
     best_residual = st.initial_residual;
@@ -673,16 +895,25 @@ struct KrylovSolver::Impl {
       dev.launch(KH_SOLVER_LIMITS, 24, {scalars, tol[0], tol[1], div[0], div[1]});
     }
     reset(0);
-    if (kind == SolverKind::PcgJacobi || kind == SolverKind::AmgPcg) pcg(a, x, b, c, target, st);
-    else if (bicg()) bicgstab(a, x, b, c, target, st);
-    else if (gmres()) fgmres(a, x, b, c, target, st);
-    else jacobi(a, x, b, c, target, st);
+    return true;
+  }
+  void solve_end(const DeviceCsr& a, float* x, const float* b, double target, SolveStats& st) {
     if (!verified_current || nullspace) verified(a, x, b, target, st);
     if (!std::isfinite(st.final_residual) || st.final_residual > best_residual) {
       dev.copy(x, best.data(), sizeof(float) * n);
       verified(a, x, b, target, st);
     }
     previous_iterations = st.converged ? st.iterations : 0;
+  }
+  SolveStats solve(const DeviceCsr& a, float* x, const float* b, const SolveControl& c) {
+    SolveStats st;
+    double target = 0;
+    if (!solve_begin(a, x, b, c, st, target)) return st;
+    if (kind == SolverKind::PcgJacobi || kind == SolverKind::AmgPcg) pcg(a, x, b, c, target, st);
+    else if (bicg()) bicgstab(a, x, b, c, target, st);
+    else if (gmres()) fgmres(a, x, b, c, target, st);
+    else jacobi(a, x, b, c, target, st);
+    solve_end(a, x, b, target, st);
     return st;
   }
 
@@ -723,6 +954,91 @@ SolveStats KrylovSolver::solve(const DeviceCsr& a, float* x, const float* b, con
   return impl_->solve(a, x, b, c);
 }
 void KrylovSolver::set_fused(bool fused) { impl_->fused = fused; }
+
+// Several BiCGStab-Jacobi solves in lockstep: each lane keeps its own entry,
+// batches, polls, replacements and exit, and its iterations run as lanes of
+// the KH_BATCH_* launches, so its iterates are bitwise those of its own
+// solve(). A round launches the common part of the active lanes' batches;
+// the lanes whose batch ends are polled with one wait.
+std::vector<SolveStats> KrylovSolver::solve_batched(const std::vector<KrylovSolver*>& solvers,
+                                                    const std::vector<DeviceCsr>& a, const std::vector<float*>& x,
+                                                    const std::vector<const float*>& b, const SolveControl& c) {
+  const std::size_t lanes = solvers.size();
+  KA_REQUIRE(lanes >= 1 && lanes <= KH_BATCH_MAX_LANES && a.size() == lanes && x.size() == lanes &&
+                 b.size() == lanes,
+             "batched solve: 1 to " + std::to_string(KH_BATCH_MAX_LANES) + " systems with one operator, x and b each");
+  Impl& first = *solvers[0]->impl_;
+  for (std::size_t k = 0; k < lanes; ++k) {
+    const Impl& I = *solvers[k]->impl_;
+    KA_REQUIRE(I.kind == SolverKind::BicgstabJacobi && I.jacobi_fused(),
+               "batched solve: needs fused BiCGStab-Jacobi solvers");
+    KA_REQUIRE(&I.dev == &first.dev && I.n == first.n, "batched solve: solvers of one device and size");
+    for (std::size_t j = 0; j < k; ++j) KA_REQUIRE(solvers[j] != solvers[k], "batched solve: a solver twice");
+  }
+  HipDevice& dev = first.dev;
+  const Index n = first.n;
+  std::vector<SolveStats> stats(lanes);
+  std::vector<double> targets(lanes, 0.0);
+  std::vector<Impl::BicgRun> runs(lanes);
+  std::vector<bool> iterate(lanes, false);
+  for (std::size_t k = 0; k < lanes; ++k) {
+    Impl& I = *solvers[k]->impl_;
+    iterate[k] = I.solve_begin(a[k], x[k], b[k], c, stats[k], targets[k]);
+    if (!iterate[k]) continue;
+    auto& run = runs[k];
+    run.a = &a[k];
+    run.x = x[k];
+    run.b = b[k];
+    run.c = c;
+    run.target = targets[k];
+    run.st = &stats[k];
+    I.bicg_begin(run);
+  }
+  std::vector<kh_bicg_lane> table(lanes);
+  for (std::size_t k = 0; k < lanes; ++k) table[k] = solvers[k]->impl_->lane(a[k], x[k]);
+  const bool same = first.batch_table_host.size() == lanes &&
+                    std::memcmp(first.batch_table_host.data(), table.data(), lanes * sizeof(kh_bicg_lane)) == 0;
+  if (!same) {
+    first.batch_table = to_device(dev, table);
+    first.batch_table_host = table;
+  }
+  if (first.batch_status.size() != 3 * lanes) first.batch_status = DeviceArray<Index>(dev, 3 * lanes);
+  std::vector<int> remaining(lanes, 0);
+  while (true) {
+    int mask = 0, steps = 0;
+    for (std::size_t k = 0; k < lanes; ++k) {
+      if (!iterate[k] || runs[k].done) continue;
+      if (remaining[k] == 0) remaining[k] = solvers[k]->impl_->bicg_next_batch(runs[k]);
+      if (remaining[k] <= 0) continue;
+      mask |= 1 << k;
+      steps = steps == 0 ? remaining[k] : std::min(steps, remaining[k]);
+    }
+    if (mask == 0) break;
+    const int count = int(lanes);
+    for (int j = 0; j < steps; ++j) {
+      dev.launch(KH_BATCH_BICG_DIRECTION, n, {first.batch_table, count, mask});
+      dev.launch(KH_BATCH_SPMV_SHADOW_ALPHA, n, {first.batch_table, count, mask});
+      dev.launch(KH_BATCH_BICG_DIFFERENCE_S, n, {first.batch_table, count, mask});
+      dev.launch(KH_BATCH_SPMV_DOT2_OMEGA, n, {first.batch_table, count, mask});
+      dev.launch(KH_BATCH_BICG_STEP_CHECK_BETA, n, {first.batch_table, count, mask});
+    }
+    std::vector<std::size_t> polled;
+    for (std::size_t k = 0; k < lanes; ++k) {
+      if (((mask >> k) & 1) == 0) continue;
+      remaining[k] -= steps;
+      if (remaining[k] == 0) polled.push_back(k);
+    }
+    for (std::size_t k : polled)
+      dev.copy(first.batch_status.data() + 3 * k, solvers[k]->impl_->status.data(), 3 * sizeof(Index));
+    ++first.waits;
+    const auto host = first.batch_status.download();
+    for (std::size_t k : polled)
+      solvers[k]->impl_->bicg_after_poll(runs[k], {host[3 * k], host[3 * k + 1], host[3 * k + 2]});
+  }
+  for (std::size_t k = 0; k < lanes; ++k)
+    if (iterate[k]) solvers[k]->impl_->solve_end(a[k], x[k], b[k], targets[k], stats[k]);
+  return stats;
+}
 int KrylovSolver::set_tail(int first) {
   const int previous = impl_->tail_request;
   impl_->tail_request = first;
@@ -738,6 +1054,46 @@ int KrylovSolver::set_tail(int first) {
 int KrylovSolver::tail_level() const { return impl_->tail_first; }
 int KrylovSolver::host_waits() const { return impl_->waits; }
 int KrylovSolver::set_direct(Index rows) { return impl_->set_direct(rows); }
+void KrylovSolver::set_learned(int level, int layers, std::uint64_t seed, int capacity) {
+  impl_->set_learned(level, layers, seed, capacity);
+}
+void KrylovSolver::set_learned_mode(int mode, double reduction, double max_relative) {
+  KA_REQUIRE(mode >= 0 && mode <= 2, "learned stage: mode must be 0 (off), 1 (harvest) or 2 (apply)");
+  KA_REQUIRE(mode == 0 || impl_->learned != nullptr, "learned stage: not configured");
+  KA_REQUIRE(std::isfinite(reduction) && reduction >= 0 && reduction < 1,
+             "learned stage: reduction per step must be in [0, 1)");
+  KA_REQUIRE(!std::isnan(max_relative) && max_relative > 0,
+             "learned stage: the largest unexplained fraction must be positive");
+  impl_->learned_mode = mode;
+  impl_->learned_reduction = reduction;
+  impl_->learned_max_relative = max_relative;
+}
+int KrylovSolver::learned_targets(double rel_tol) { return impl_->learned_targets(rel_tol); }
+TrainResult KrylovSolver::learned_train(int steps, float lr) { return impl_->learned_train(steps, lr); }
+LearnedReport KrylovSolver::learned_report() const {
+  const auto& i = *impl_;
+  LearnedReport r;
+  r.level = i.learned_level;
+  r.rows = i.learned ? i.levels[i.learned_level].rows : 0;
+  r.mode = i.learned_mode;
+  r.harvested = i.harvested;
+  r.targets = i.targeted;
+  r.applications = i.learned_applications;
+  r.fallbacks = i.learned_fallbacks;
+  r.solves = i.learned_solves;
+  r.gated = i.learned_gated;
+  r.last_loss = i.learned_loss;
+  r.relative = i.learned_relative;
+  return r;
+}
+std::vector<float> KrylovSolver::learned_parameters() const {
+  KA_REQUIRE(impl_->learned != nullptr, "learned stage: not configured");
+  return impl_->learned->parameters();
+}
+void KrylovSolver::set_learned_parameters(const std::vector<float>& theta) {
+  KA_REQUIRE(impl_->learned != nullptr, "learned stage: not configured");
+  impl_->learned->set_parameters(theta);
+}
 void KrylovSolver::set_restart(int m) {
   KA_REQUIRE(m >= 1 && m <= 63, "FGMRES: restart length must be in [1, 63]");
   impl_->restart_length = m;
@@ -760,6 +1116,9 @@ std::string KrylovSolver::describe() const {
   if (impl_->direct_level > 0)
     out << ", FP64 dense solve of level " << impl_->direct_level << " ("
         << impl_->levels[impl_->direct_level].rows << " rows)";
+  if (impl_->learned && impl_->learned_mode == 2)
+    out << ", learned correction on level " << impl_->learned_level << " ("
+        << impl_->levels[impl_->learned_level].rows << " rows)";
   return out.str();
 }
 std::vector<std::pair<Index, Index>> KrylovSolver::levels() const {

@@ -49,6 +49,15 @@ struct OwnedCsr {
   DeviceCsr view() const { return {n_rows, nnz, row_ptr.data(), col.data(), diag.data(), val.data()}; }
 };
 
+// State of the learned coarse correction stage (no reference counterpart).
+struct LearnedReport {
+  int level = 0, rows = 0, mode = 0, harvested = 0, targets = 0;
+  long applications = 0, fallbacks = 0, solves = 0, gated = 0;
+  double last_loss = 0;
+  double relative = -1;  // unexplained fraction of the last epoch (< 0: not trained)
+};
+struct TrainResult;
+
 class KrylovSolver {
  public:
   KrylovSolver(HipDevice& dev, SolverKind kind, const CsrHost& a, const AmgOptions& options);
@@ -65,6 +74,13 @@ class KrylovSolver {
   std::vector<float> level_values(std::size_t level) const;
   // Fused-kernel use can be disabled to run the reference kernel sequence.
   void set_fused(bool fused);
+  // Solves up to KH_BATCH_MAX_LANES independent systems with fused
+  // BiCGStab-Jacobi solvers at once (one launch per step for all of them):
+  // each system's iterates, statistics and x are bitwise those of
+  // solvers[k]->solve(a[k], x[k], b[k], control). No reference counterpart.
+  static std::vector<SolveStats> solve_batched(const std::vector<KrylovSolver*>& solvers,
+                                               const std::vector<DeviceCsr>& a, const std::vector<float*>& x,
+                                               const std::vector<const float*>& b, const SolveControl& control);
   // First level of the one-workgroup coarse-grid tail: < 0 automatic, 0 off.
   // Returns the level in use (0: none).
   int set_tail(int first);
@@ -75,6 +91,23 @@ class KrylovSolver {
   // solved by an FP64 dense inverse, computed at the next refresh(); 0 turns
   // it off. Returns that level (0: none). No reference counterpart.
   int set_direct(Index rows);
+  // Learned coarse correction stage of the AMG FGMRES kind (no reference
+  // counterpart): a message-passing network of `layers` layers on coarse
+  // level `level`, with room for `capacity` harvested pairs.
+  void set_learned(int level, int layers, std::uint64_t seed, int capacity);
+  // 0 off, 1 harvest (plain V-cycle steps record restricted residuals),
+  // 2 apply; `reduction` is the least average fall of the residual estimate
+  // per step that keeps the stage on within a solve, and a network whose
+  // last training epoch left more than `max_relative` of its targets' mean
+  // square unexplained is not applied.
+  void set_learned_mode(int mode, double reduction = 0.5, double max_relative = 0.5);
+  // Exact level corrections of the harvested residuals; returns the pairs.
+  int learned_targets(double rel_tol);
+  // Adam steps over the pairs: the loss and unexplained fraction per step.
+  TrainResult learned_train(int steps, float lr);
+  LearnedReport learned_report() const;
+  std::vector<float> learned_parameters() const;
+  void set_learned_parameters(const std::vector<float>& theta);
   int host_waits() const;
 
  private:

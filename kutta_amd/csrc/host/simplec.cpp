@@ -364,8 +364,9 @@ struct SteadySolver::Impl {
                 boundary_velocity.z, wall_diagonal, wall_rhs, ni, k, 0});
   }
   // fv/transport.cpp:83-121 with the paired limiter of with_low (:963-976).
-  void assemble(int k) {
+  void assemble(int k, float* values = nullptr) {
     auto& eq = momentum[k];
+    if (!values) values = matrix.val.data();
     const F& field = velocity[k];
     const F& bc = boundary_velocity[k];
     if (paired)
@@ -385,15 +386,77 @@ struct SteadySolver::Impl {
     dev.launch(KH_FV_TRANSPORT_ASSEMBLE, n,
                {offsets, faces, sign, adjacent, matrix_entry, matrix.row_ptr, matrix.diag, diffusion, mass,
                 viscosity, scalar_kind, bc, face_correction, source[k], volume, field, zero, wall_diagonal,
-                wall_rhs, matrix.val, eq.rhs, zero, eq.diagonal, zero, eq.consistent_inverse, ni,
+                wall_rhs, values, eq.rhs, zero, eq.diagonal, zero, eq.consistent_inverse, ni,
                 float(settings.density), time_step, float(settings.momentum_relaxation), 0});
   }
   // :1045-1060 (paired momentum without compensated assembly).
-  void paired_momentum_residual(int k, F& r) {
+  void paired_momentum_residual(int k, F& r, float* values = nullptr) {
     auto& eq = momentum[k];
+    if (!values) values = matrix.val.data();
     dev.launch(KH_FV_PAIRED_MOMENTUM_RESIDUAL, n,
-               {matrix.row_ptr, matrix.col, matrix.val, matrix.diag, eq.diagonal, zero, velocity[k],
+               {matrix.row_ptr, matrix.col, values, matrix.diag, eq.diagonal, zero, velocity[k],
                 velocity_lo[k], pressure_gradient_lo[k], volume, eq.rhs, zero, r});
+  }
+  // The three momentum components in one batched BiCGStab solve (no
+  // reference counterpart). The reference assembles and solves u, v and w in
+  // turn (:1320-1328); component k's assembly reads only its own velocity,
+  // and the boundary momentum terms (fv_rans.kern.h:46-74) read the others
+  // only through products with the face normal's other components, which are
+  // exactly zero on axis-aligned symmetry faces (no wall functions in the
+  // laminar solver). On such meshes assembling all three first and solving
+  // them together gives bitwise the reference's fields; elsewhere the
+  // components stay sequential.
+  bool batch_momentum = false;
+  F momentum_values[3], momentum_deltas[3];
+  bool batch_eligible() {
+    const auto ax = area.x.download(), ay = area.y.download(), az = area.z.download();
+    const auto kinds = boundary_kind.download();
+    for (Index b = 0; b < nb; ++b) {
+      if (kinds[b] != 3) continue;
+      const Index f = ni + b;
+      const int nonzero = (ax[f] != 0.0f) + (ay[f] != 0.0f) + (az[f] != 0.0f);
+      if (nonzero > 1) return false;
+    }
+    return true;
+  }
+  bool set_batched(bool on) {
+    batch_momentum = on && batch_eligible();
+    for (int k = 0; k < 3; ++k) {
+      momentum_values[k] = batch_momentum ? F(dev, matrix.val.size()) : F();
+      momentum_deltas[k] = batch_momentum && paired ? zeros(n) : F();
+    }
+    return batch_momentum;
+  }
+  void solve_momentum_batched() {
+    std::vector<KrylovSolver*> solvers;
+    std::vector<DeviceCsr> ops;
+    std::vector<float*> xs;
+    std::vector<const float*> bs;
+    for (int k = 0; k < 3; ++k) {
+      wall_terms(k);
+      assemble(k, momentum_values[k].data());
+      DeviceCsr op = matrix.view();
+      op.val = momentum_values[k].data();
+      if (paired) {
+        paired_momentum_residual(k, residual_u, op.val);
+        dev.copy(momentum[k].rhs.data(), residual_u.data(), std::size_t(n) * 4);
+        dev.fill(momentum_deltas[k].data(), 0.0f, n);
+      }
+      momentum[k].solver->refresh(op);
+      solvers.push_back(momentum[k].solver.get());
+      ops.push_back(op);
+      xs.push_back(paired ? momentum_deltas[k].data() : velocity[k].data());
+      bs.push_back(momentum[k].rhs.data());
+    }
+    const auto stats = KrylovSolver::solve_batched(solvers, ops, xs, bs, settings.momentum_linear);
+    for (int k = 0; k < 3; ++k) {
+      report.momentum[k] = stats[k];
+      if (paired)
+        dev.launch(KH_FV_PAIRED_ACCUMULATE, n,
+                   {momentum_deltas[k], velocity[k], velocity_lo[k], paired_slack().velocity});
+    }
+    // The shared operator values end as the last component's, as in turn.
+    dev.copy(matrix.val.data(), momentum_values[2].data(), matrix.val.size() * sizeof(float));
   }
   SolveStats solve_momentum(int k, F& field) {
     auto& eq = momentum[k];
@@ -677,6 +740,7 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   update_boundaries();
   update_pressure();
   update_flux();
+  set_batched(s.batched_momentum);
 }
 
 // :1303-1369
@@ -686,7 +750,8 @@ void SteadySolver::Impl::step() {
   update_pressure();
   momentum_source();
   reconstruct_velocity();
-  for (int k = 0; k < 3; ++k) {
+  if (batch_momentum) solve_momentum_batched();
+  for (int k = 0; k < 3 && !batch_momentum; ++k) {
     wall_terms(k);
     assemble(k);
     if (!paired) {
@@ -861,6 +926,8 @@ const IterationReport& SteadySolver::report() const { return impl_->report; }
 Index SteadySolver::reference_cell() const { return impl_->reference; }
 const MeshGeometry& SteadySolver::geometry() const { return impl_->geom; }
 Index SteadySolver::cells() const { return impl_->n; }
+bool SteadySolver::set_batched_momentum(bool on) { return impl_->set_batched(on); }
+bool SteadySolver::batched_momentum() const { return impl_->batch_momentum; }
 int SteadySolver::host_waits() const {
   int waits = impl_->pressure_solver->host_waits();
   for (const auto& eq : impl_->momentum) waits += eq.solver->host_waits();

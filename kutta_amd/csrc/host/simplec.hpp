@@ -38,6 +38,9 @@ struct SolverSettings {
   Vec3 reference_point{};
   SolveControl momentum_linear{1e-2, 1e-20, 100, 8};
   SolveControl pressure_linear{1e-3, 1e-20, 300, 8};
+  // Solve the three momentum components in one batched launch sequence when
+  // that is bitwise the reference's order (no reference counterpart).
+  bool batched_momentum = true;
 };
 
 // fv/steady_solver.hpp:273-284
@@ -79,6 +82,11 @@ class SteadySolver {
   Index cells() const;
   // Host waits of the linear solvers so far (pressure, momentum).
   int host_waits() const;
+  // Batched momentum solve on or off; returns whether it is in use (it
+  // stays off where it would change the iterates: symmetry faces that are not
+  // axis-aligned).
+  bool set_batched_momentum(bool on);
+  bool batched_momentum() const;
 
  private:
   struct Impl;

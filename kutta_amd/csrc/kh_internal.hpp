@@ -81,6 +81,7 @@ struct KernelEntry {
 const std::vector<KernelEntry>& linalg_kernels();
 const std::vector<KernelEntry>& gmres_kernels();
 const std::vector<KernelEntry>& dense_kernels();
+const std::vector<KernelEntry>& learned_kernels();
 const std::vector<KernelEntry>& fv_kernels();
 
 // Reduction launchers (kh_reduce.hip).

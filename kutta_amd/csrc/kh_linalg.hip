@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(kBlock) k_fused_project_dot(int n, float* __re
 // step, each reduction with its one-workgroup finisher.
 
 // Partials of x.y: dot_into(x, y) ahead of a scalar update.
-__global__ void __launch_bounds__(kBlock) k_fused_dot(int n, const float* __restrict__ x,
+__device__ __forceinline__ void fused_dot_body(int n, const float* __restrict__ x,
     const float* __restrict__ y, const int* __restrict__ status, double* __restrict__ partials) {
   if (status[0] != 0) return;
   const int i = row_index();
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(kBlock) k_fused_dot(int n, const float* __rest
 }
 // p = r + beta (p - omega q), z = p * inverse: solver_bicg_direction +
 // krylov_multiply_guarded.
-__global__ void __launch_bounds__(kBlock) k_fused_bicg_direction(int n, const float* __restrict__ r,
+__device__ __forceinline__ void fused_bicg_direction_body(int n, const float* __restrict__ r,
     const float* __restrict__ q, float* __restrict__ p, const float* __restrict__ inverse,
     float* __restrict__ z, const float* __restrict__ scalars, const int* __restrict__ status) {
   if (status[0] != 0) return;
@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(kBlock) k_fused_bicg_direction(int n, const fl
   z[i] = pi * inverse[i];
 }
 // q = A z, partials of shadow.q: csr_spmv_guarded + dot_into(shadow, q, 2).
-__global__ void __launch_bounds__(kBlock) k_fused_spmv_shadow_dot(int n, const int* __restrict__ rows,
+__device__ __forceinline__ void fused_spmv_shadow_dot_body(int n, const int* __restrict__ rows,
     const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ z,
     float* __restrict__ q, const float* __restrict__ shadow, const int* __restrict__ status,
     double* __restrict__ partials) {
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) k_fused_spmv_shadow_dot(int n, const i
 }
 // s = r - alpha q, zs = s * inverse, partials of s.s: solver_bicg_difference +
 // dot_into(s, s, 16) + krylov_multiply_guarded.
-__global__ void __launch_bounds__(kBlock) k_fused_bicg_difference(int n, const float* __restrict__ r,
+__device__ __forceinline__ void fused_bicg_difference_body(int n, const float* __restrict__ r,
     const float* __restrict__ q, float* __restrict__ sv, const float* __restrict__ inverse,
     float* __restrict__ zs, const float* __restrict__ scalars, const int* __restrict__ status,
     double* __restrict__ partials) {
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(kBlock) k_fused_bicg_difference(int n, const f
   store_partials<1>(mine, partials);
 }
 // t = A zs, partials of t.t and t.s: csr_spmv_guarded + dot2_into(t,t,t,s,2).
-__global__ void __launch_bounds__(kBlock) k_fused_spmv_dot2(int n, const int* __restrict__ rows,
+__device__ __forceinline__ void fused_spmv_dot2_body(int n, const int* __restrict__ rows,
     const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ zs,
     float* __restrict__ t, const float* __restrict__ sv, const int* __restrict__ status,
     double* __restrict__ partials) {
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(kBlock) k_fused_spmv_dot2(int n, const int* __
 // x += alpha z + omega zs, r = s - omega t (or the s exit), partials of r.r
 // and shadow.r: solver_bicg_step + dot_into(r, r, 18) + the next
 // iteration's dot_into(shadow, r, 0).
-__global__ void __launch_bounds__(kBlock) k_fused_bicg_step(int n, float* __restrict__ x,
+__device__ __forceinline__ void fused_bicg_step_body(int n, float* __restrict__ x,
     float* __restrict__ r, const float* __restrict__ z, const float* __restrict__ zs,
     const float* __restrict__ sv, const float* __restrict__ t, const float* __restrict__ shadow,
     const float* __restrict__ scalars, const int* __restrict__ status, double* __restrict__ partials) {
@@ -585,13 +585,84 @@ __global__ void __launch_bounds__(kBlock) k_fused_bicg_step(int n, float* __rest
   store_partials<2>(mine, partials);
 }
 
+// Single-system kernels (one BiCGStab solve per launch).
+__global__ void __launch_bounds__(kBlock) k_fused_dot(int n, const float* __restrict__ x,
+    const float* __restrict__ y, const int* __restrict__ status, double* __restrict__ partials) {
+  fused_dot_body(n, x, y, status, partials);
+}
+__global__ void __launch_bounds__(kBlock) k_fused_bicg_direction(int n, const float* __restrict__ r,
+    const float* __restrict__ q, float* __restrict__ p, const float* __restrict__ inverse,
+    float* __restrict__ z, const float* __restrict__ scalars, const int* __restrict__ status) {
+  fused_bicg_direction_body(n, r, q, p, inverse, z, scalars, status);
+}
+__global__ void __launch_bounds__(kBlock) k_fused_spmv_shadow_dot(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ z,
+    float* __restrict__ q, const float* __restrict__ shadow, const int* __restrict__ status,
+    double* __restrict__ partials) {
+  fused_spmv_shadow_dot_body(n, rows, cols, values, z, q, shadow, status, partials);
+}
+__global__ void __launch_bounds__(kBlock) k_fused_bicg_difference(int n, const float* __restrict__ r,
+    const float* __restrict__ q, float* __restrict__ sv, const float* __restrict__ inverse,
+    float* __restrict__ zs, const float* __restrict__ scalars, const int* __restrict__ status,
+    double* __restrict__ partials) {
+  fused_bicg_difference_body(n, r, q, sv, inverse, zs, scalars, status, partials);
+}
+__global__ void __launch_bounds__(kBlock) k_fused_spmv_dot2(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ zs,
+    float* __restrict__ t, const float* __restrict__ sv, const int* __restrict__ status,
+    double* __restrict__ partials) {
+  fused_spmv_dot2_body(n, rows, cols, values, zs, t, sv, status, partials);
+}
+__global__ void __launch_bounds__(kBlock) k_fused_bicg_step(int n, float* __restrict__ x,
+    float* __restrict__ r, const float* __restrict__ z, const float* __restrict__ zs,
+    const float* __restrict__ sv, const float* __restrict__ t, const float* __restrict__ shadow,
+    const float* __restrict__ scalars, const int* __restrict__ status, double* __restrict__ partials) {
+  fused_bicg_step_body(n, x, r, z, zs, sv, t, shadow, scalars, status, partials);
+}
+
+// Batched kernels (no reference counterpart): lane blockIdx.y of a
+// kh_bicg_lane table runs the single-system body on its own vectors, values,
+// scalars, status and partials region (partials + lane * stride), for the
+// lanes set in `mask`. Each lane's arithmetic, grid and partial order are the
+// single launch's, so its iterates are bitwise those of a solve of its own.
+#define LANE_OR_RETURN                                  \
+  const int lane = int(blockIdx.y);                     \
+  if (((mask >> lane) & 1) == 0) return;                \
+  const kh_bicg_lane& L = lanes[lane];                  \
+  double* part = partials + std::size_t(lane) * stride;
+__global__ void __launch_bounds__(kBlock) k_batch_bicg_direction(int n, const kh_bicg_lane* __restrict__ lanes,
+                                                                 int mask, double* partials, unsigned stride) {
+  LANE_OR_RETURN
+  (void)part;
+  fused_bicg_direction_body(n, L.r, L.q, L.p, L.inverse, L.z, L.scalars, L.status);
+}
+__global__ void __launch_bounds__(kBlock) k_batch_spmv_shadow_dot(int n, const kh_bicg_lane* __restrict__ lanes,
+                                                                  int mask, double* partials, unsigned stride) {
+  LANE_OR_RETURN
+  fused_spmv_shadow_dot_body(n, L.row_ptr, L.col, L.val, L.z, L.q, L.shadow, L.status, part);
+}
+__global__ void __launch_bounds__(kBlock) k_batch_bicg_difference(int n, const kh_bicg_lane* __restrict__ lanes,
+                                                                  int mask, double* partials, unsigned stride) {
+  LANE_OR_RETURN
+  fused_bicg_difference_body(n, L.r, L.q, L.s, L.inverse, L.zs, L.scalars, L.status, part);
+}
+__global__ void __launch_bounds__(kBlock) k_batch_spmv_dot2(int n, const kh_bicg_lane* __restrict__ lanes,
+                                                            int mask, double* partials, unsigned stride) {
+  LANE_OR_RETURN
+  fused_spmv_dot2_body(n, L.row_ptr, L.col, L.val, L.zs, L.t, L.s, L.status, part);
+}
+__global__ void __launch_bounds__(kBlock) k_batch_bicg_step(int n, const kh_bicg_lane* __restrict__ lanes,
+                                                            int mask, double* partials, unsigned stride) {
+  LANE_OR_RETURN
+  fused_bicg_step_body(n, L.x, L.r, L.z, L.zs, L.s, L.t, L.shadow, L.scalars, L.status, part);
+}
+
 // Finishers: the FP64 total of `count` partials as a normalized pair, then the
 // reference's scalar kernel on it.
 enum class Finish { Alpha, Check, Beta, Pair, BicgBeta, BicgAlpha, BicgS };
 template <Finish F>
-__global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ partials,
-                                                   unsigned count, float* scalars, int* status,
-                                                   float* dst) {
+__device__ __forceinline__ void finish_body(const double* __restrict__ partials, unsigned count,
+                                            float* scalars, int* status, float* dst) {
   if (status[0] != 0) return;
   double total[1];
   sum_partials<1>(partials, count, total);
@@ -625,8 +696,8 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ pa
 // slot 0 and the next iteration's bicg_beta.
 enum class Finish2 { Omega, CheckBeta };
 template <Finish2 F>
-__global__ void __launch_bounds__(kBlock) k_finish2(const double* __restrict__ partials,
-                                                    unsigned count, float* scalars, int* status) {
+__device__ __forceinline__ void finish2_body(const double* __restrict__ partials, unsigned count,
+                                             float* scalars, int* status) {
   if (status[0] != 0) return;
   double total[2];
   sum_partials<2>(partials, count, total);
@@ -642,6 +713,30 @@ __global__ void __launch_bounds__(kBlock) k_finish2(const double* __restrict__ p
     pstore(scalars, 0, to_pair(total[1]));
     bicg_beta(scalars, status);
   }
+}
+template <Finish F>
+__global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ partials,
+                                                   unsigned count, float* scalars, int* status,
+                                                   float* dst) {
+  finish_body<F>(partials, count, scalars, status, dst);
+}
+template <Finish2 F>
+__global__ void __launch_bounds__(kBlock) k_finish2(const double* __restrict__ partials,
+                                                    unsigned count, float* scalars, int* status) {
+  finish2_body<F>(partials, count, scalars, status);
+}
+// Batched finishers: one workgroup per lane (blockIdx.y).
+template <Finish F>
+__global__ void __launch_bounds__(kBlock) k_batch_finish(const kh_bicg_lane* __restrict__ lanes, int mask,
+                                                         double* partials, unsigned stride, unsigned count) {
+  LANE_OR_RETURN
+  finish_body<F>(part, count, L.scalars, L.status, nullptr);
+}
+template <Finish2 F>
+__global__ void __launch_bounds__(kBlock) k_batch_finish2(const kh_bicg_lane* __restrict__ lanes, int mask,
+                                                          double* partials, unsigned stride, unsigned count) {
+  LANE_OR_RETURN
+  finish2_body<F>(part, count, L.scalars, L.status);
 }
 
 // Restricted residual without storing the fine residual:
@@ -1152,6 +1247,52 @@ void l_fused_bicg_step_check_beta(kh_device* d, int n, const kh_arg* a) {
                                  d->partials);
   k_finish2<Finish2::CheckBeta><<<FINISH>>>(d->partials, g, KF(7), st);
 }
+// Batched BiCGStab-Jacobi launches: n = rows of every lane; lanes (device
+// array of kh_bicg_lane), lane count (1..KH_BATCH_MAX_LANES), mask of the
+// lanes that take the step. Lane k's partials occupy reduce_grid(n) x 2
+// doubles from d->partials + k * that.
+unsigned batch_stride(int n, int count) {
+  const unsigned g = reduce_grid(n);
+  if (count < 1 || count > KH_BATCH_MAX_LANES || std::size_t(g) * 2 * count > std::size_t(kMaxReduceBlocks) * 2)
+    throw std::runtime_error("batched BiCGStab: lane count out of range for this row count");
+  return 2 * g;
+}
+#define BATCH_GRID(n, lanes) dim3(blocks_for(n), unsigned(lanes)), dim3(kBlock), 0, d->stream
+#define BATCH_FINISH(lanes) dim3(1, unsigned(lanes)), dim3(kBlock), 0, d->stream
+void l_batch_bicg_direction(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 1), mask = i32(a, 2);
+  const unsigned stride = batch_stride(n, count);
+  const auto* lanes = static_cast<const kh_bicg_lane*>(buf(a, 0));
+  k_batch_bicg_direction<<<BATCH_GRID(n, count)>>>(n, lanes, mask, d->partials, stride);
+}
+void l_batch_spmv_shadow_alpha(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 1), mask = i32(a, 2);
+  const unsigned stride = batch_stride(n, count), g = reduce_grid(n);
+  const auto* lanes = static_cast<const kh_bicg_lane*>(buf(a, 0));
+  k_batch_spmv_shadow_dot<<<BATCH_GRID(n, count)>>>(n, lanes, mask, d->partials, stride);
+  k_batch_finish<Finish::BicgAlpha><<<BATCH_FINISH(count)>>>(lanes, mask, d->partials, stride, g);
+}
+void l_batch_bicg_difference_s(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 1), mask = i32(a, 2);
+  const unsigned stride = batch_stride(n, count), g = reduce_grid(n);
+  const auto* lanes = static_cast<const kh_bicg_lane*>(buf(a, 0));
+  k_batch_bicg_difference<<<BATCH_GRID(n, count)>>>(n, lanes, mask, d->partials, stride);
+  k_batch_finish<Finish::BicgS><<<BATCH_FINISH(count)>>>(lanes, mask, d->partials, stride, g);
+}
+void l_batch_spmv_dot2_omega(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 1), mask = i32(a, 2);
+  const unsigned stride = batch_stride(n, count), g = reduce_grid(n);
+  const auto* lanes = static_cast<const kh_bicg_lane*>(buf(a, 0));
+  k_batch_spmv_dot2<<<BATCH_GRID(n, count)>>>(n, lanes, mask, d->partials, stride);
+  k_batch_finish2<Finish2::Omega><<<BATCH_FINISH(count)>>>(lanes, mask, d->partials, stride, g);
+}
+void l_batch_bicg_step_check_beta(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 1), mask = i32(a, 2);
+  const unsigned stride = batch_stride(n, count), g = reduce_grid(n);
+  const auto* lanes = static_cast<const kh_bicg_lane*>(buf(a, 0));
+  k_batch_bicg_step<<<BATCH_GRID(n, count)>>>(n, lanes, mask, d->partials, stride);
+  k_batch_finish2<Finish2::CheckBeta><<<BATCH_FINISH(count)>>>(lanes, mask, d->partials, stride, g);
+}
 void l_fused_project_dot_beta(kh_device* d, int n, const kh_arg* a) {
   const unsigned g = reduce_grid(n);
   int* st = static_cast<int*>(buf(a, 6));
@@ -1213,6 +1354,11 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_FUSED_BICG_DIFFERENCE_S, "fused_bicg_difference_s", "bbbbbbb", l_fused_bicg_difference_s},
       {KH_FUSED_SPMV_DOT2_OMEGA, "fused_spmv_dot2_omega", "bbbbbbbb", l_fused_spmv_dot2_omega},
       {KH_FUSED_BICG_STEP_CHECK_BETA, "fused_bicg_step_check_beta", "bbbbbbbbb", l_fused_bicg_step_check_beta},
+      {KH_BATCH_BICG_DIRECTION, "batch_bicg_direction", "bii", l_batch_bicg_direction},
+      {KH_BATCH_SPMV_SHADOW_ALPHA, "batch_spmv_shadow_alpha", "bii", l_batch_spmv_shadow_alpha},
+      {KH_BATCH_BICG_DIFFERENCE_S, "batch_bicg_difference_s", "bii", l_batch_bicg_difference_s},
+      {KH_BATCH_SPMV_DOT2_OMEGA, "batch_spmv_dot2_omega", "bii", l_batch_spmv_dot2_omega},
+      {KH_BATCH_BICG_STEP_CHECK_BETA, "batch_bicg_step_check_beta", "bii", l_batch_bicg_step_check_beta},
       {KH_FUSED_STEP_DOT_CHECK, "fused_step_dot_check", "bbbbbb", l_fused_step_dot_check},
       {KH_FUSED_RESIDUAL_RESTRICT, "fused_residual_restrict", "bbbbbbbbb", l_fused_residual_restrict},
       {KH_FUSED_PROLONG_SMOOTH, "fused_prolong_smooth", "bbbbbbbbbb", l_fused_prolong_smooth},

@@ -168,6 +168,7 @@ const KernelEntry* find_kernel(uint32_t id) {
     for (const auto& e : fv_kernels()) m[e.id] = &e;
     for (const auto& e : gmres_kernels()) m[e.id] = &e;
     for (const auto& e : dense_kernels()) m[e.id] = &e;
+    for (const auto& e : learned_kernels()) m[e.id] = &e;
     return m;
   }();
   const auto it = index.find(id);

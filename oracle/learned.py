@@ -15,6 +15,13 @@ Network (hidden width H, L message-passing layers):
 The predicted correction is y * RMS(r) / mean(diag). Loss: mean squared error
 of y against the exact correction in the same units.
 
+With bf16=True, forward and loss_and_gradient round to bfloat16 (round to
+nearest even, via float32) at the points where the device kernels store
+bf16: the hidden states h_l and aggregates m_l after they are computed, the
+layer weights Ws and Wn as multiplied, and the backpropagated dz_l as
+multiplied into the layer and weight gradients; everything else stays FP64.
+The device result then differs from it only by FP32 accumulation.
+
 Parameters are one flat vector, in order: W0 (F x H, row major), b0 (H), then
 for each layer Ws (H x H), Wn (H x H), b (H), then wo (H), bo (1).
 """
@@ -62,38 +69,63 @@ def neighbour_matrix(row_ptr, col, val):
     return S
 
 
+def neighbour_operator(row_ptr, col, val):
+    """S as a scipy.sparse CSR matrix (large graphs), same entries as
+    neighbour_matrix."""
+    import scipy.sparse as sp
+    row_ptr = np.asarray(row_ptr)
+    col = np.asarray(col)
+    val = np.asarray(val, np.float64)
+    n = len(row_ptr) - 1
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    diag = np.zeros(n)
+    on = col == rows
+    diag[rows[on]] = val[on]
+    off = ~on
+    return sp.csr_matrix((np.abs(val[off]) / diag[rows[off]], (rows[off], col[off])), shape=(n, n))
+
+
 def features(row_ptr, col, val, r):
     """x [n, 4] and the output scale RMS(r) / mean(diag)."""
     n = len(row_ptr) - 1
+    rows = np.repeat(np.arange(n), np.diff(np.asarray(row_ptr)))
+    col = np.asarray(col)
+    val = np.asarray(val, np.float64)
+    on = col == rows
     diag = np.zeros(n)
-    off = np.zeros(n)
-    for i in range(n):
-        for j in range(row_ptr[i], row_ptr[i + 1]):
-            if col[j] == i:
-                diag[i] = val[j]
-            else:
-                off[i] += abs(val[j])
+    diag[rows[on]] = val[on]
+    off = np.bincount(rows[~on], weights=np.abs(val[~on]), minlength=n)
     rms = np.sqrt(np.mean(np.asarray(r, np.float64) ** 2))
     mean_diag = diag.mean()
     x = np.stack([np.asarray(r, np.float64) / rms, diag / mean_diag, off / mean_diag, np.ones(n)], axis=1)
     return x, rms / mean_diag
 
 
-def forward(theta, S, x, hidden, layers):
+def bf16(a):
+    """Round to bfloat16 (nearest even) through float32; returned as float64."""
+    f = np.asarray(a, np.float64).astype(np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(theta, S, x, hidden, layers, bf16_storage=False):
     W0, b0, Ws, Wn, b, wo, bo = unpack(theta, hidden, layers)
-    h = [np.maximum(x @ W0 + b0, 0.0)]
+    q = bf16 if bf16_storage else (lambda v: v)
+    h = [q(np.maximum(x @ W0 + b0, 0.0))]
     m = []
     for l in range(layers):
-        m.append(S @ h[-1])
-        h.append(np.maximum(h[-1] @ Ws[l] + m[-1] @ Wn[l] + b[l], 0.0))
+        m.append(q(S @ h[-1]))
+        h.append(q(np.maximum(h[-1] @ q(Ws[l]) + m[-1] @ q(Wn[l]) + b[l], 0.0)))
     y = h[-1] @ wo + bo[0]
     return y, (h, m)
 
 
-def loss_and_gradient(theta, S, x, target, hidden, layers):
+def loss_and_gradient(theta, S, x, target, hidden, layers, bf16_storage=False):
     """Mean squared error and its gradient with respect to theta."""
     W0, b0, Ws, Wn, b, wo, bo = unpack(theta, hidden, layers)
-    y, (h, m) = forward(theta, S, x, hidden, layers)
+    q = bf16 if bf16_storage else (lambda v: v)
+    y, (h, m) = forward(theta, S, x, hidden, layers, bf16_storage)
     n = x.shape[0]
     diff = y - target
     loss = float(np.mean(diff ** 2))
@@ -104,11 +136,12 @@ def loss_and_gradient(theta, S, x, target, hidden, layers):
     g_Ws, g_Wn, g_b = [None] * layers, [None] * layers, [None] * layers
     for l in reversed(range(layers)):
         dz = dh * (h[l + 1] > 0)
+        g_b[l] = dz.sum(axis=0)
+        dz = q(dz)
         g_Ws[l] = h[l].T @ dz
         g_Wn[l] = m[l].T @ dz
-        g_b[l] = dz.sum(axis=0)
-        dm = dz @ Wn[l].T
-        dh = dz @ Ws[l].T + S.T @ dm
+        dm = dz @ q(Wn[l]).T
+        dh = dz @ q(Ws[l]).T + S.T @ dm
     dz0 = dh * (h[0] > 0)
     g_W0 = x.T @ dz0
     g_b0 = dz0.sum(axis=0)

@@ -195,3 +195,49 @@ def test_bicgstab_jacobi_fused_equals_reference_sequence(device, nx, ny, nz, win
         assert st.final_residual == pytest.approx(sto["final_residual"], rel=1e-12)
         assert np.array_equal(x.view(np.uint32), xo.view(np.uint32))
     assert results[0][0].final_residual == results[1][0].final_residual
+
+
+@pytest.mark.parametrize("interval", [8, 3])
+def test_batched_bicgstab_equals_individual_solves(device, interval):
+    """Four systems in one batched BiCGStab-Jacobi solve (KH_BATCH_* launches,
+    no reference counterpart): different operators (wind, anisotropy), right
+    -hand sides and iteration counts, one that needs no iteration (b = 0), and
+    a second solve of each that starts from the first's adaptive batch
+    interval. Every x and every SolveStats equals its own solve's, bit for
+    bit; the oracle pins the single solves."""
+    n = 26 * 24
+    ops = [oracle.poisson(26, 24, 1, 1.0, wind=2.0), oracle.poisson(26, 24, 1, 3.0, wind=5.0),
+           oracle.poisson(26, 24, 1, 1.0, random=True, wind=1.0), oracle.poisson(26, 24, 1, 1.0, wind=0.5)]
+    rng = np.random.default_rng(interval)
+    rhs = [rng.standard_normal(n).astype(np.float32) for _ in range(3)] + [np.zeros(n, np.float32)]
+    controls = SolveControl(1e-6, 0.0, 300, interval)
+    print(f"[batched bicgstab] 4 lanes, interval {interval}", flush=True)
+
+    def fresh():
+        out = []
+        for a in ops:
+            _, matrix, solver = product_setup(device, a, SolverKind.BicgstabJacobi, AmgOptions())
+            out.append((matrix, solver))
+        return out
+
+    single, batched = fresh(), fresh()
+    xs_single = [device.alloc(n) for _ in ops]
+    xs_batched = [device.alloc(n) for _ in ops]
+    bs = [device.upload(b) for b in rhs]
+    for round_ in range(2):
+        for x in xs_single + xs_batched:
+            x.fill(np.float32(0))
+        st_single = [solver.solve(matrix, x, b, controls) for (matrix, solver), x, b in zip(single, xs_single, bs)]
+        st_batched = kutta_amd.solve_batched([s for _, s in batched], [m for m, _ in batched], xs_batched, bs,
+                                             controls)
+        for k in range(len(ops)):
+            a, b = st_single[k], st_batched[k]
+            assert (a.iterations, a.initial_residual, a.final_residual, a.converged) == \
+                   (b.iterations, b.initial_residual, b.final_residual, b.converged), (round_, k)
+            assert np.array_equal(xs_single[k].download().view(np.uint32),
+                                  xs_batched[k].download().view(np.uint32)), (round_, k)
+        its = [s.iterations for s in st_single]
+        assert its[3] == 0 and len(set(its[:3])) > 1  # the lanes really differ
+    xo, sto, _ = oracle_solve(ops[0], "BicgstabJacobi", rhs[0], controls)
+    assert st_single[0].iterations == sto["iterations"]
+    assert np.array_equal(xs_batched[0].download().view(np.uint32), xo.view(np.uint32))

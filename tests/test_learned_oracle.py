@@ -35,6 +35,7 @@ def test_neighbour_matrix_and_features_closed_form():
     d = np.diag(A)
     expected = np.abs(A - np.diag(d)) / d[:, None]
     assert np.allclose(S, expected)
+    assert np.array_equal(learned.neighbour_operator(row_ptr, col, val).toarray(), S)
     r = np.random.default_rng(1).standard_normal(len(d))
     x, scale = learned.features(row_ptr, col, val, r)
     assert np.allclose(x[:, 0] * np.sqrt(np.mean(r ** 2)), r)
@@ -78,3 +79,46 @@ def test_forward_is_linear_in_the_output_layer():
     theta2[-7:] *= 2.0  # wo (6) and bo (1)
     y2, _ = learned.forward(theta2, S, x, 6, 2)
     assert np.allclose(y2, 2 * y1)
+
+
+def test_library_parameter_count_matches_oracle():
+    """The C ABI's parameter layout size (no GPU needed) is the oracle's."""
+    import kutta_amd
+    for layers in (1, 2, 3, 4):
+        assert kutta_amd.LearnedNetwork.parameter_count(layers) == learned.parameter_count(64, layers)
+    assert kutta_amd.LearnedNetwork.parameter_count(0) == -1
+    assert kutta_amd.LearnedNetwork.parameter_count(5) == -1
+
+
+def test_bf16_rounding_closed_forms():
+    """Round to nearest even at 8 significant bits, through float32."""
+    one = 1.0
+    ulp = 2.0 ** -7  # bf16 spacing just above 1
+    assert learned.bf16(one + ulp / 2) == one            # tie: to even (1)
+    assert learned.bf16(one + 3 * ulp / 2) == one + 2 * ulp  # tie: to even (1 + 2 ulp)
+    assert learned.bf16(one + 0.51 * ulp) == one + ulp
+    assert learned.bf16(-(one + 0.49 * ulp)) == -one
+    assert learned.bf16(3.0) == 3.0 and learned.bf16(0.0) == 0.0
+    x = np.random.default_rng(2).standard_normal(1000)
+    r = learned.bf16(x)
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2.0 ** -8)
+    assert np.array_equal(learned.bf16(r), r)  # idempotent
+
+
+def test_bf16_storage_mode_is_a_small_perturbation():
+    """The bf16-storage variant stays within bf16 error of the FP64 network
+    (whose gradient the finite-difference tests pin)."""
+    row_ptr, col, val, A = small_graph(40, seed=4)
+    H, L = 16, 2
+    S = learned.neighbour_matrix(row_ptr, col, val)
+    r = np.random.default_rng(5).standard_normal(len(row_ptr) - 1)
+    x, _ = learned.features(row_ptr, col, val, r)
+    theta = learned.initial_parameters(H, L, seed=6)
+    y64, _ = learned.forward(theta, S, x, H, L)
+    y16, _ = learned.forward(theta, S, x, H, L, bf16_storage=True)
+    assert np.linalg.norm(y16 - y64) / np.linalg.norm(y64) < 3e-2
+    t = np.random.default_rng(7).standard_normal(len(r))
+    l64, g64 = learned.loss_and_gradient(theta, S, x, t, H, L)
+    l16, g16 = learned.loss_and_gradient(theta, S, x, t, H, L, bf16_storage=True)
+    assert abs(l16 - l64) / l64 < 3e-2
+    assert np.linalg.norm(g16 - g64) / np.linalg.norm(g64) < 0.2
