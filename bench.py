@@ -238,7 +238,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=55)
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--tail", type=int, default=-1,
-                    help="first AMG level of the one-workgroup coarse-grid tail (-1 auto, 0 off)")
+                    help="first AMG level of the one-workgroup coarse-grid tail (-1: chosen by the "
+                         "autotuner, 0 off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fv-kernels", action="store_true",
                     help="only time each finite-volume kernel on the size^3 mesh")
@@ -249,9 +250,13 @@ def main() -> int:
     ap.add_argument("--simplec-iterations", type=int, default=500)
     ap.add_argument("--simplec-box", type=int, default=0, metavar="N",
                     help="only time SIMPLEC on the N^3 lid-driven box (the reference's FV benchmark)")
+    ap.add_argument("--direct-rows", type=int, default=8192,
+                    help="mixed variants: the first coarse level of at most this many rows is solved by "
+                         "its FP64 dense inverse")
     ap.add_argument("--variants", default="all",
                     help="comma-separated solver variants to time beside the plain solve "
-                         "(pcg_mixed, fgmres_plain, fgmres_mixed, fgmres_learned, fgmres_mixed_learned), "
+                         "(pcg_mixed, pcg_mixed_bf16, fgmres_plain, fgmres_mixed, fgmres_learned, "
+                         "fgmres_mixed_learned), "
                          "'all' or 'none'")
     ap.add_argument("--learned-train", type=int, default=1000,
                     help="Adam steps of the learned coarse correction (0 skips the learned variants)")
@@ -308,6 +313,12 @@ def main() -> int:
     setup_amg_s = time.perf_counter() - t
     tail_level = solver.set_tail(args.tail)
     solver.refresh(problem)
+    if args.tail < 0:  # automatic: the autotuner times each admissible tail level (identical iterates)
+        t = time.perf_counter()
+        tail_level = solver.autotune_tail(problem, 5)
+        autotune_s = time.perf_counter() - t
+    else:
+        autotune_s = 0.0
     b_host = rhs_for(problem.n_cells, problem.reference)
     b = dev.upload(b_host)
     x = dev.alloc(problem.n_cells)
@@ -320,13 +331,7 @@ def main() -> int:
         solver.solve(problem, x, b, control)
     dev.synchronize()
 
-    # Fine-level smoothing sweeps are timed on the device stream by HIP events
-    # attached to their own dispatch packets (hipExtLaunchKernelGGL), so the
-    # average is the kernel's execution time, as a kernel trace reports it.
     n, m = problem.n_cells, problem.nnz
-    dev.profile_reset()
-    dev.profile(Kernel.amg_smooth, True)
-    dev.profile_filter(Kernel.amg_smooth, n)
     stats = []
     dev.synchronize()
     t0 = time.perf_counter()
@@ -335,9 +340,24 @@ def main() -> int:
         stats.append(solver.solve(problem, x, b, control))
     dev.synchronize()
     elapsed = time.perf_counter() - t0
+    ms = elapsed * 1e3 / args.steps
+
+    # Fine-level smoothing sweeps are timed on the device stream by HIP events
+    # attached to their own dispatch packets (hipExtLaunchKernelGGL), so the
+    # average is the kernel's execution time, as a kernel trace reports it.
+    # Graph-replayed launches carry no events, so these solves launch kernel
+    # by kernel (the same kernels and iterates).
+    solver.set_graphs(False)
+    dev.profile_reset()
+    dev.profile(Kernel.amg_smooth, True)
+    dev.profile_filter(Kernel.amg_smooth, n)
+    for _ in range(max(args.steps // 2, 2)):
+        x.fill(np.float32(0))
+        solver.solve(problem, x, b, control)
+    dev.synchronize()
     launches, kernel_ms = dev.profile_read(Kernel.amg_smooth)
     dev.profile(Kernel.amg_smooth, False)
-    ms = elapsed * 1e3 / args.steps
+    solver.set_graphs(True)
     st = stats[-1]
     if not st.converged:
         print(f"[bench] warning: solve did not converge ({st})", file=sys.stderr)
@@ -363,20 +383,24 @@ def main() -> int:
 
     # The same solve with the additive solver variants (no reference
     # counterpart): the mixed-precision V-cycle (first coarse level of at most
-    # 2048 rows solved by an FP64 dense inverse) under PCG, and flexible
+    # --direct-rows rows solved by an FP64 dense inverse; with pcg_mixed_bf16
+    # also the coarse levels' values in bfloat16) under PCG, and flexible
     # GMRES(30) with the plain and the mixed V-cycle. x is compared with the
     # plain FP32 solution, both within the 1e-6 relative residual.
     variants = {}
     x_plain = x.download()
     wanted = (None if args.variants == "all" else
               set() if args.variants == "none" else set(args.variants.split(",")))
-    for label, kind, direct in (("pcg_mixed", SolverKind.AmgPcg, 2048),
-                                ("fgmres_plain", SolverKind.AmgFgmres, 0),
-                                ("fgmres_mixed", SolverKind.AmgFgmres, 2048)):
+    dr = args.direct_rows
+    for label, kind, direct, bf16 in (("pcg_mixed", SolverKind.AmgPcg, dr, False),
+                                      ("pcg_mixed_bf16", SolverKind.AmgPcg, dr, True),
+                                      ("fgmres_plain", SolverKind.AmgFgmres, 0, False),
+                                      ("fgmres_mixed", SolverKind.AmgFgmres, dr, False)):
         if wanted is not None and label not in wanted:
             continue
         other = kutta_amd.make_amg_solver(kind, dev, problem.host, options)
         level = other.set_direct(direct) if direct else 0
+        other.set_coarse_bf16(bf16)
         t = time.perf_counter()
         other.refresh(problem)
         dev.synchronize()
@@ -410,7 +434,7 @@ def main() -> int:
     # than half per step). Harvest, targets and training are setup, reported
     # beside the timed ms.
     if args.learned_train > 0:
-        for label, direct in (("fgmres_learned", 0), ("fgmres_mixed_learned", 2048)):
+        for label, direct in (("fgmres_learned", 0), ("fgmres_mixed_learned", args.direct_rows)):
             if wanted is not None and label not in wanted:
                 continue
             other = kutta_amd.make_amg_solver(SolverKind.AmgFgmres, dev, problem.host, options)
@@ -501,6 +525,7 @@ def main() -> int:
             "hierarchy": solver.describe(),
             "levels": solver.levels(),
             "coarse_tail_level": tail_level,
+            "tail_autotune_s": round(autotune_s, 3),
             "iterations": st.iterations,
             "relative_residual": st.final_residual / st.initial_residual,
             "converged": st.converged,

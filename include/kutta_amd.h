@@ -111,6 +111,10 @@ long ka_solver_transfer(void* solver, int level, int which, int32_t* out);
 long ka_solver_level_values(void* solver, int level, float* out);
 /* 1: fused MI355X kernels (default); 0: the reference kernel sequence. */
 int ka_solver_set_fused(void* solver, int fused);
+/* 1 (default): each fused PCG iteration is replayed from a HIP graph captured
+ * on first use for the operator and x; 0: launched kernel by kernel. The
+ * iterates are identical. No reference counterpart. */
+int ka_solver_set_graphs(void* solver, int on);
 int ka_solver_host_waits(void* solver);
 /* First AMG level of the one-workgroup coarse-grid tail (KH_FUSED_COARSE_TAIL)
  * on the fused path: first < 0 picks it automatically (the default), 0 turns
@@ -118,6 +122,10 @@ int ka_solver_host_waits(void* solver);
  * *level receives the level in use, 0 for none. No reference counterpart: the
  * iterates are those of the per-level launches either way. */
 int ka_solver_set_tail(void* solver, int first, int32_t* level);
+/* The tail autotuner: times `repeats` V-cycles on the refreshed operator for
+ * each admissible first level (and none) and keeps the fastest in *level.
+ * The iterates are identical whichever is chosen. No reference counterpart. */
+int ka_solver_autotune_tail(void* solver, void* matrix, int repeats, int32_t* level);
 /* Restart length m (1..63) of the FGMRES kinds. No reference counterpart. */
 int ka_solver_set_restart(void* solver, int m);
 /* Learned coarse correction stage of the AMG FGMRES kind (no reference
@@ -142,6 +150,11 @@ int ka_solver_learned_targets(void* solver, double rel_tol, int32_t* pairs);
 int ka_solver_learned_train(void* solver, int steps, float lr, double* losses, double* relative);
 int ka_solver_learned_report(void* solver, double* out);
 int ka_solver_learned_parameters(void* solver, float* theta, int64_t count, int set);
+/* Mixed-precision AMG (no reference counterpart): coarse levels (1 and
+ * below) smoothed, restricted and prolonged from bfloat16 copies of their
+ * values, FP32 arithmetic; the copies are made at the next ka_solver_refresh.
+ * The coarse tail and the dense level keep FP32. Not with the nullspace. */
+int ka_solver_set_coarse_bf16(void* solver, int on);
 /* Mixed-precision AMG (no reference counterpart): the first coarse level of
  * at most `rows` rows (<= 8192) is solved by an FP64 dense inverse computed at
  * the next ka_solver_refresh; coarser levels and the coarse tail go unused.
@@ -191,6 +204,9 @@ int ka_simplec_info(void* solver, int32_t* info);
  * every symmetry face is axis-aligned; elsewhere it stays off. *used
  * receives whether it is in use. */
 int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used);
+/* HIP graph replay of the pressure solve's PCG iterations (default on;
+ * identical iterates; ka_solver_set_graphs). */
+int ka_simplec_set_graphs(void* solver, int on);
 int ka_simplec_centroids(void* solver, double* out);
 
 /* ---- Learned coarse correction (no reference counterpart) ----------------

@@ -269,8 +269,10 @@ typedef enum kh_kernel {
   KH_GMRES_UPDATE,               /* y = H^-1 g over status[2] columns, x += Z y: Z,x,block,m,status */
   /* FP64 dense coarsest-level solve of the mixed-precision AMG path (no
      reference counterpart); the operator must be symmetric positive definite. */
-  KH_DENSE_INVERT,               /* Gauss-Jordan inverse of an n-row CSR operator, FP64, no pivoting:
-                                    rows,cols,values,inverse,scratch (n*n doubles each; n <= 8192) */
+  KH_DENSE_INVERT,               /* blocked Gauss-Jordan inverse of an n-row CSR operator, FP64 (the
+                                    panel and update products on the FP64 MFMA), no pivoting:
+                                    rows,cols,values,inverse (n*n doubles),scratch (4096 doubles);
+                                    n <= 8192 */
   KH_DENSE_APPLY,                /* x = inverse b (FP64 accumulation): inverse,b,x,status */
   /* Learned coarse correction (no reference counterpart): a message-passing
      network of hidden width KH_LEARNED_HIDDEN over a coarse level's graph,
@@ -307,6 +309,16 @@ typedef enum kh_kernel {
   KH_BATCH_BICG_DIFFERENCE_S,
   KH_BATCH_SPMV_DOT2_OMEGA,
   KH_BATCH_BICG_STEP_CHECK_BETA,
+  /* bfloat16 coarse-level values of the mixed-precision V-cycle (no reference
+     counterpart): KH_TO_BF16 rounds FP32 values to nearest even (n = entries:
+     src,dst); the four smoothing kernels take the same arguments as their
+     FP32 counterparts with `values` in bfloat16, widened to FP32 for the
+     row arithmetic. */
+  KH_TO_BF16,
+  KH_BF16_AMG_SMOOTH,            /* as KH_AMG_SMOOTH */
+  KH_BF16_SMOOTH_FROM_ZERO,      /* as KH_FUSED_SMOOTH_FROM_ZERO */
+  KH_BF16_RESIDUAL_RESTRICT,     /* as KH_FUSED_RESIDUAL_RESTRICT */
+  KH_BF16_PROLONG_SMOOTH,        /* as KH_FUSED_PROLONG_SMOOTH */
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 

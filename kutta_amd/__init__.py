@@ -115,6 +115,9 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_set_tail.argtypes = [vp, i32, pi32]
         lib.ka_solver_set_restart.argtypes = [vp, i32]
         lib.ka_solver_set_direct.argtypes = [vp, i32, pi32]
+        lib.ka_solver_set_coarse_bf16.argtypes = [vp, i32]
+        lib.ka_solver_set_graphs.argtypes = [vp, i32]
+        lib.ka_solver_autotune_tail.argtypes = [vp, vp, i32, pi32]
         lib.ka_solver_set_learned.argtypes = [vp, i32, i32, C.c_uint64, i32]
         lib.ka_solver_learned_mode.argtypes = [vp, i32, dbl, dbl]
         lib.ka_solver_learned_targets.argtypes = [vp, dbl, pi32]
@@ -140,6 +143,7 @@ def host_lib() -> C.CDLL:
                                                      C.POINTER(dbl)]
         lib.ka_simplec_centroids.argtypes = [vp, vp]
         lib.ka_simplec_set_batched_momentum.argtypes = [vp, i32, pi32]
+        lib.ka_simplec_set_graphs.argtypes = [vp, i32]
         lib.ka_learned_parameter_count.argtypes = [i32]
         lib.ka_learned_parameter_count.restype = C.c_int64
         lib.ka_learned_create.argtypes = [vp, i32, C.c_uint64, C.POINTER(vp)]
@@ -352,6 +356,11 @@ class Kernel(enum.IntEnum):
     batch_bicg_difference_s = 234
     batch_spmv_dot2_omega = 235
     batch_bicg_step_check_beta = 236
+    to_bf16 = 237
+    bf16_amg_smooth = 238
+    bf16_smooth_from_zero = 239
+    bf16_residual_restrict = 240
+    bf16_prolong_smooth = 241
 
 
 class Buffer:
@@ -724,6 +733,24 @@ class LinearSolver:
         t = np.ascontiguousarray(theta, dtype=np.float32)
         _check_host(host_lib().ka_solver_learned_parameters(self.handle, t.ctypes.data, t.size, 1))
 
+    def autotune_tail(self, a, repeats: int = 5) -> int:
+        """Time V-cycles for each admissible coarse-tail level (and none) on
+        the refreshed operator; keep and return the fastest (identical
+        iterates whichever)."""
+        level = C.c_int32()
+        _check_host(host_lib().ka_solver_autotune_tail(self.handle, a.handle, int(repeats), C.byref(level)))
+        return level.value
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay each fused PCG iteration from a captured HIP graph (default
+        on; identical iterates)."""
+        _check_host(host_lib().ka_solver_set_graphs(self.handle, int(bool(on))))
+
+    def set_coarse_bf16(self, on: bool) -> None:
+        """Mixed-precision AMG: bfloat16 copies of the coarse levels' values
+        (made at the next refresh) for their smoothing and transfers."""
+        _check_host(host_lib().ka_solver_set_coarse_bf16(self.handle, int(bool(on))))
+
     def set_restart(self, m: int) -> None:
         """Restart length of the FGMRES kinds (1..63; default 30)."""
         _check_host(host_lib().ka_solver_set_restart(self.handle, int(m)))
@@ -1074,6 +1101,10 @@ class SteadySolver:
         out = (C.c_int32 * 4)()
         _check_host(host_lib().ka_simplec_info(self.handle, out))
         return {"cells": out[0], "reference": out[1], "host_waits": out[2], "iteration": out[3]}
+
+    def set_graphs(self, on: bool) -> None:
+        """HIP graph replay of the pressure PCG iterations (default on)."""
+        _check_host(host_lib().ka_simplec_set_graphs(self.handle, int(bool(on))))
 
     def set_batched_momentum(self, on: bool) -> bool:
         """Solve u, v, w in one batched BiCGStab (on by default where it is

@@ -254,6 +254,18 @@ int ka_solver_learned_parameters(void* solver, float* theta, int64_t count, int 
     }
   });
 }
+int ka_solver_autotune_tail(void* solver, void* matrix, int repeats, int32_t* level) {
+  return guard([&] {
+    const int l = static_cast<Solver*>(solver)->solver.autotune_tail(view_of(matrix), repeats);
+    if (level) *level = l;
+  });
+}
+int ka_solver_set_graphs(void* solver, int on) {
+  return guard([&] { static_cast<Solver*>(solver)->solver.set_graphs(on != 0); });
+}
+int ka_solver_set_coarse_bf16(void* solver, int on) {
+  return guard([&] { static_cast<Solver*>(solver)->solver.set_coarse_bf16(on != 0); });
+}
 int ka_solver_set_restart(void* solver, int m) {
   return guard([&] { static_cast<Solver*>(solver)->solver.set_restart(m); });
 }

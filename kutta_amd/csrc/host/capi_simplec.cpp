@@ -48,6 +48,9 @@ int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, c
   });
 }
 void ka_simplec_destroy(void* solver) { delete static_cast<Simplec*>(solver); }
+int ka_simplec_set_graphs(void* solver, int on) {
+  return guard([&] { handle(solver).solver->set_graphs(on != 0); });
+}
 int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used) {
   return guard([&] {
     const bool u = handle(solver).solver->set_batched_momentum(on != 0);

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <iomanip>
 #include <sstream>
@@ -41,6 +42,7 @@ struct Level {
   Index rows = 0, nnz = 0;
   OwnedCsr matrix;  // coarse levels only; level 0 uses the caller's operator
   CsrHost host;     // coarse levels: the operator as built (pattern and setup values)
+  DeviceArray<std::uint16_t> val16;  // bfloat16 copy of the values (mixed path)
   DeviceArray<float> l1, x, b, r, projection;
   bool has_transfer = false;
   DeviceArray<Index> aggregate, member_offsets, members, entry_offsets, entries;
@@ -124,12 +126,30 @@ struct KrylovSolver::Impl {
   // residual_check_interval steps and verifies the true residual after each
   // cycle, restarting from it.
   // Mixed-precision AMG (no reference counterpart): level `direct_level` is
-  // solved by an FP64 dense inverse (Gauss-Jordan at refresh, one GEMV per
-  // V-cycle) instead of coarser levels and sweeps; 0 is off.
+  // solved by an FP64 dense inverse (blocked Gauss-Jordan on the FP64 MFMA
+  // at refresh, one GEMV per V-cycle) instead of coarser levels and sweeps;
+  // 0 is off.
   int direct_level = 0;
   bool direct_ready = false;
+  // Mixed-precision AMG: coarse levels (1 and below) smoothed, restricted and
+  // prolonged from bfloat16 copies of their values (FP32 arithmetic), made at
+  // each refresh. The coarse tail and the dense direct level keep FP32.
+  bool coarse_bf16 = false, bf16_ready = false;
+  bool bf16_level(const Level& l) const {
+    return coarse_bf16 && fused && &l != &levels[0] && !l.val16.empty();
+  }
+  void set_coarse_bf16(bool on) {
+    drop_graphs();
+    KA_REQUIRE(!on || amg, "AMG: bfloat16 coarse levels need an AMG solver");
+    KA_REQUIRE(!on || !nullspace, "AMG: bfloat16 coarse levels are not supported with the nullspace projection");
+    coarse_bf16 = on;
+    bf16_ready = false;
+    if (!on)
+      for (auto& l : levels) l.val16 = DeviceArray<std::uint16_t>();
+  }
   DeviceArray<double> dense_inverse, dense_scratch;
   int set_direct(Index rows) {
+    drop_graphs();
     direct_level = 0;
     direct_ready = false;
     dense_inverse = DeviceArray<double>();
@@ -149,7 +169,7 @@ struct KrylovSolver::Impl {
     KA_REQUIRE(direct_level > 0, "AMG: no coarse level has at most " + std::to_string(rows) + " rows");
     const std::size_t nc = std::size_t(levels[direct_level].rows);
     dense_inverse = DeviceArray<double>(dev, nc * nc);
-    dense_scratch = DeviceArray<double>(dev, nc * nc);
+    dense_scratch = DeviceArray<double>(dev, 64 * 64);  // the pivot block's inverse
     plan_tail();
     return direct_level;
   }
@@ -414,6 +434,7 @@ struct KrylovSolver::Impl {
   // a level's own launches, whose sweeps spread over the whole chip.
   static constexpr Index kTailAutoRows = 2048;
   void plan_tail() {
+    drop_graphs();
     tail_first = 0;
     tail_table = DeviceArray<kh_tail_level>();
     if (!amg || tail_request == 0 || direct_level > 0) return;
@@ -484,6 +505,16 @@ struct KrylovSolver::Impl {
       dev.launch(KH_DENSE_INVERT, c.n_rows, {c.row_ptr, c.col, c.val, dense_inverse, dense_scratch});
       direct_ready = true;
     }
+    if (coarse_bf16)
+      for (std::size_t i = 1; i < levels.size(); ++i) {
+        Level& l = levels[i];
+        if (l.val16.size() != std::size_t(std::max<Index>(l.nnz, 1))) {
+          drop_graphs();
+          l.val16 = DeviceArray<std::uint16_t>(dev, std::max<Index>(l.nnz, 1));
+        }
+        if (l.nnz > 0) dev.launch(KH_TO_BF16, l.nnz, {l.matrix.val, l.val16});
+      }
+    bf16_ready = coarse_bf16;
     if (learned) learned->set_graph(levels[learned_level].matrix.view());
   }
 
@@ -504,6 +535,13 @@ struct KrylovSolver::Impl {
   void sweep(Level& l, const DeviceCsr& a, const float* b, const float* in, float* out, bool first,
              float* zero_x) {
     const int* st = status.data();
+    if (bf16_level(l)) {
+      if (first)
+        dev.launch(KH_BF16_SMOOTH_FROM_ZERO, a.n_rows, {a.row_ptr, a.col, l.val16, b, l.l1, out, st});
+      else
+        dev.launch(KH_BF16_AMG_SMOOTH, a.n_rows, {a.row_ptr, a.col, l.val16, in, b, l.l1, out, st});
+      return;
+    }
     if (first) {
       if (fused) {
         dev.launch(KH_FUSED_SMOOTH_FROM_ZERO, a.n_rows, {a.row_ptr, a.col, a.val, b, l.l1, out, st});
@@ -534,6 +572,7 @@ struct KrylovSolver::Impl {
                   options.coarse_sweeps, nullspace ? 1 : 0, b, x, st});
       return result;
     }
+    KA_REQUIRE(!coarse_bf16 || bf16_ready, "AMG: refresh before solving with bfloat16 coarse levels");
     if (direct_level > 0 && int(index) == direct_level) {
       KA_REQUIRE(direct_ready, "AMG: refresh before solving with the dense coarse solve");
       dev.launch(KH_DENSE_APPLY, a.n_rows, {dense_inverse, b, x, st});
@@ -575,7 +614,10 @@ struct KrylovSolver::Impl {
     if (!coarsest) {
       Level& c = levels[index + 1];
       const DeviceCsr ca = c.matrix.view();
-      if (fused) {
+      if (bf16_level(l)) {
+        dev.launch(KH_BF16_RESIDUAL_RESTRICT, ca.n_rows,
+                   {a.row_ptr, a.col, l.val16, current, b, l.member_offsets, l.members, c.b, st});
+      } else if (fused) {
         dev.launch(KH_FUSED_RESIDUAL_RESTRICT, ca.n_rows,
                    {a.row_ptr, a.col, a.val, current, b, l.member_offsets, l.members, c.b, st});
       } else {
@@ -588,7 +630,10 @@ struct KrylovSolver::Impl {
       for (int t = pre + 1; t <= total; ++t) {
         float* out = target(t);
         if (t == pre + 1) {
-          if (fused) {
+          if (bf16_level(l)) {
+            dev.launch(KH_BF16_PROLONG_SMOOTH, a.n_rows,
+                       {a.row_ptr, a.col, l.val16, current, b, l.l1, out, l.aggregate, c.x, st});
+          } else if (fused) {
             dev.launch(KH_FUSED_PROLONG_SMOOTH, a.n_rows,
                        {a.row_ptr, a.col, a.val, current, b, l.l1, out, l.aggregate, c.x, st});
           } else {
@@ -685,6 +730,43 @@ struct KrylovSolver::Impl {
     dev.reduce_into(KH_INTO_DOT, x, y, x, y, n, scalars.data(), offset);
   }
 
+  // HIP graph of one fused PCG iteration (no reference counterpart): the
+  // iteration's launches (spmv + p.q + alpha, step + r.r + check, the V-cycle
+  // with the fused r.z and beta, direction) read only buffers that stay put
+  // from one solve to the next, so the sequence is captured once per operator
+  // and x and replayed. Every kernel in it is status-guarded, so a replay
+  // past convergence is a no-op, as the launches are. Configuration changes
+  // that alter the sequence (tail, dense level, fusion, bfloat16 copies)
+  // drop the graph.
+  bool graphs = true;
+  Graph pcg_graph;
+  const Index* graph_rows = nullptr;
+  const Index* graph_cols = nullptr;
+  const float* graph_vals = nullptr;
+  const float* graph_x = nullptr;
+  void drop_graphs() {
+    pcg_graph.reset();
+    graph_rows = graph_cols = nullptr;
+    graph_vals = graph_x = nullptr;
+  }
+  void pcg_iteration(const DeviceCsr& a, float* x) {
+    dev.launch(KH_FUSED_SPMV_DOT_ALPHA, n, {a.row_ptr, a.col, a.val, p, q, scalars, status});
+    dev.launch(KH_FUSED_STEP_DOT_CHECK, n, {x, r, p, q, scalars, status});
+    apply(a, r.data(), z.data(), Tail::DotBeta);
+    dev.launch(KH_SOLVER_DIRECTION, n, {z, p, scalars, status});
+  }
+  void pcg_iteration_replay(const DeviceCsr& a, float* x) {
+    if (!pcg_graph || graph_rows != a.row_ptr || graph_cols != a.col || graph_vals != a.val || graph_x != x) {
+      drop_graphs();
+      pcg_graph = Graph(dev.capture([&] { pcg_iteration(a, x); }));
+      graph_rows = a.row_ptr;
+      graph_cols = a.col;
+      graph_vals = a.val;
+      graph_x = x;
+    }
+    dev.replay(pcg_graph.get());
+  }
+
   // linalg/solver.cpp:271-313
   void pcg(const DeviceCsr& a, float* x, const float* b, const SolveControl& c, double target,
            SolveStats& st) {
@@ -702,6 +784,10 @@ struct KrylovSolver::Impl {
       const int batch = std::min(interval, c.max_iter - st.iterations);
       verified_current = false;
       for (int j = 0; j < batch; ++j) {
+        if (fused && graphs && st.iterations + j + 1 != c.max_iter) {
+          pcg_iteration_replay(a, x);
+          continue;
+        }
         if (fused) {
           dev.launch(KH_FUSED_SPMV_DOT_ALPHA, n, {a.row_ptr, a.col, a.val, p, q, scalars, status});
           dev.launch(KH_FUSED_STEP_DOT_CHECK, n, {x, r, p, q, scalars, status});
@@ -728,12 +814,12 @@ struct KrylovSolver::Impl {
     }
   }
 
-  // linalg/solver.cpp:314-361. With the Jacobi preconditioner and fused
-  // kernels an iteration is five launches (each with its reduction and scalar
-  // update); the first rho and beta after a restart are one more.
   // linalg/solver.cpp:314-361, as a resumable loop: the host logic between
   // polls is shared by bicgstab() and the batched driver, which interleaves
-  // several systems' batches (KrylovSolver::solve_batched).
+  // several systems' batches (KrylovSolver::solve_batched). With the Jacobi
+  // preconditioner and fused kernels an iteration is five launches (each with
+  // its reduction and scalar update); the first rho and beta after a restart
+  // are one more.
   struct BicgRun {
     const DeviceCsr* a = nullptr;
     float* x = nullptr;
@@ -953,7 +1039,14 @@ void KrylovSolver::refresh(const DeviceCsr& a) { impl_->refresh(a); }
 SolveStats KrylovSolver::solve(const DeviceCsr& a, float* x, const float* b, const SolveControl& c) {
   return impl_->solve(a, x, b, c);
 }
-void KrylovSolver::set_fused(bool fused) { impl_->fused = fused; }
+void KrylovSolver::set_fused(bool fused) {
+  impl_->drop_graphs();
+  impl_->fused = fused;
+}
+void KrylovSolver::set_graphs(bool on) {
+  impl_->drop_graphs();
+  impl_->graphs = on;
+}
 
 // Several BiCGStab-Jacobi solves in lockstep: each lane keeps its own entry,
 // batches, polls, replacements and exit, and its iterations run as lanes of
@@ -1052,8 +1145,50 @@ int KrylovSolver::set_tail(int first) {
   return impl_->tail_first;
 }
 int KrylovSolver::tail_level() const { return impl_->tail_first; }
+
+// The coarse-tail autotuner (no reference counterpart): times `repeats`
+// V-cycles for each admissible first level of the tail, and for none, keeps
+// the fastest and returns it. Every choice gives bitwise the same iterates
+// (the tail computes each row as the per-level launches do), so it picks
+// only the launch structure: per-level launches are latency-bound below a
+// few thousand rows, the one-workgroup tail is compute-bound on one CU, and
+// where they cross depends on the hierarchy.
+int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
+  Impl& I = *impl_;
+  KA_REQUIRE(I.amg && I.fused, "autotune: needs a fused AMG solver");
+  KA_REQUIRE(repeats >= 1, "autotune: repeats must be positive");
+  KA_REQUIRE(a.n_rows == I.n && a.nnz == I.nnz, "solver: device or matrix mismatch");
+  if (I.direct_level > 0) return 0;  // the dense level replaces the tail
+  std::vector<int> candidates{0};
+  for (std::size_t i = 1; i < I.levels.size(); ++i)
+    if (I.tail_fits(i)) candidates.push_back(int(i));
+  DeviceArray<float> v(I.dev, I.n), out(I.dev, I.n);
+  I.dev.fill(v.data(), 1.0f, I.n);
+  const int previous = I.tail_request;
+  int best = previous;
+  double best_s = 0;
+  for (int first : candidates) {
+    I.tail_request = first;
+    I.plan_tail();
+    I.reset(0);
+    I.apply(a, v.data(), out.data(), Tail::None);
+    I.dev.synchronize();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < repeats; ++k) I.apply(a, v.data(), out.data(), Tail::None);
+    I.dev.synchronize();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (best_s == 0 || secs < best_s) {
+      best_s = secs;
+      best = first;
+    }
+  }
+  I.tail_request = best;
+  I.plan_tail();
+  return I.tail_first;
+}
 int KrylovSolver::host_waits() const { return impl_->waits; }
 int KrylovSolver::set_direct(Index rows) { return impl_->set_direct(rows); }
+void KrylovSolver::set_coarse_bf16(bool on) { impl_->set_coarse_bf16(on); }
 void KrylovSolver::set_learned(int level, int layers, std::uint64_t seed, int capacity) {
   impl_->set_learned(level, layers, seed, capacity);
 }
@@ -1116,6 +1251,7 @@ std::string KrylovSolver::describe() const {
   if (impl_->direct_level > 0)
     out << ", FP64 dense solve of level " << impl_->direct_level << " ("
         << impl_->levels[impl_->direct_level].rows << " rows)";
+  if (impl_->coarse_bf16) out << ", bfloat16 coarse-level values";
   if (impl_->learned && impl_->learned_mode == 2)
     out << ", learned correction on level " << impl_->learned_level << " ("
         << impl_->levels[impl_->learned_level].rows << " rows)";

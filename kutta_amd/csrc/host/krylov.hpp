@@ -74,6 +74,9 @@ class KrylovSolver {
   std::vector<float> level_values(std::size_t level) const;
   // Fused-kernel use can be disabled to run the reference kernel sequence.
   void set_fused(bool fused);
+  // Replay each fused PCG iteration from a captured HIP graph (default on;
+  // identical iterates). No reference counterpart.
+  void set_graphs(bool on);
   // Solves up to KH_BATCH_MAX_LANES independent systems with fused
   // BiCGStab-Jacobi solvers at once (one launch per step for all of them):
   // each system's iterates, statistics and x are bitwise those of
@@ -85,12 +88,20 @@ class KrylovSolver {
   // Returns the level in use (0: none).
   int set_tail(int first);
   int tail_level() const;
+  // Times `repeats` V-cycles per admissible tail level (and none) on the
+  // refreshed operator and keeps the fastest; returns it. Identical iterates
+  // either way. No reference counterpart.
+  int autotune_tail(const DeviceCsr& a, int repeats);
   // Restart length of the FGMRES kinds (default 30, at most 63).
   void set_restart(int m);
   // Mixed-precision AMG: the first coarse level of at most `rows` rows is
   // solved by an FP64 dense inverse, computed at the next refresh(); 0 turns
   // it off. Returns that level (0: none). No reference counterpart.
   int set_direct(Index rows);
+  // Mixed-precision AMG: coarse levels smoothed, restricted and prolonged
+  // from bfloat16 copies of their values (FP32 arithmetic), made at the next
+  // refresh(). No reference counterpart.
+  void set_coarse_bf16(bool on);
   // Learned coarse correction stage of the AMG FGMRES kind (no reference
   // counterpart): a message-passing network of `layers` layers on coarse
   // level `level`, with room for `capacity` harvested pairs.

@@ -407,6 +407,7 @@ struct SteadySolver::Impl {
   // them together gives bitwise the reference's fields; elsewhere the
   // components stay sequential.
   bool batch_momentum = false;
+  bool pressure_tuned = false;
   F momentum_values[3], momentum_deltas[3];
   bool batch_eligible() {
     const auto ax = area.x.download(), ay = area.y.download(), az = area.z.download();
@@ -777,6 +778,10 @@ void SteadySolver::Impl::step() {
               matrix.val, pressure_rhs, reference});
   if (paired) dev.launch(KH_FV_PAIRED_PRESSURE_RHS, n, {imbalance, pressure_rhs, reference});
   pressure_solver->refresh(matrix.view());
+  if (!pressure_tuned) {  // the first assembled operator: pick the coarse tail by timing
+    pressure_solver->autotune_tail(matrix.view(), 3);
+    pressure_tuned = true;
+  }
   dev.fill(correction.data(), 0.0f, n);
   report.pressure = pressure_solver->solve(matrix.view(), correction.data(), pressure_rhs.data(), settings.pressure_linear);
   if (paired)
@@ -927,6 +932,7 @@ Index SteadySolver::reference_cell() const { return impl_->reference; }
 const MeshGeometry& SteadySolver::geometry() const { return impl_->geom; }
 Index SteadySolver::cells() const { return impl_->n; }
 bool SteadySolver::set_batched_momentum(bool on) { return impl_->set_batched(on); }
+void SteadySolver::set_graphs(bool on) { impl_->pressure_solver->set_graphs(on); }
 bool SteadySolver::batched_momentum() const { return impl_->batch_momentum; }
 int SteadySolver::host_waits() const {
   int waits = impl_->pressure_solver->host_waits();

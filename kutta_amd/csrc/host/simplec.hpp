@@ -87,6 +87,9 @@ class SteadySolver {
   // axis-aligned).
   bool set_batched_momentum(bool on);
   bool batched_momentum() const;
+  // HIP graph replay of the pressure PCG iterations (default on; identical
+  // iterates).
+  void set_graphs(bool on);
 
  private:
   struct Impl;

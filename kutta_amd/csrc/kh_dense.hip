@@ -1,14 +1,24 @@
 // FP64 dense coarsest-level solve for the mixed-precision AMG path (no
-// reference counterpart): the coarsest operator, stored FP32 in CSR, is
-// expanded into a dense FP64 matrix and inverted in place by Gauss-Jordan
-// elimination without pivoting (the operator is symmetric positive definite:
-// a pinned M-matrix and its Galerkin products), one elimination step per
-// launch over a ping-pong pair of buffers. Each V-cycle then applies the
-// inverse as an FP64 matrix-vector product, one workgroup per row.
+// reference counterpart): the coarse operator, stored FP32 in CSR, is
+// expanded into a dense FP64 matrix and inverted in place by blocked
+// Gauss-Jordan elimination without pivoting (the operator is symmetric
+// positive definite: a pinned M-matrix and its Galerkin products). Per block
+// of kGjBlock pivots K:
+//   P = A_KK^-1 (one workgroup, element-wise Gauss-Jordan in LDS), A_KK = P
+//   A_KJ = P A_KJ                (row panel, J outside K)
+//   A_IJ -= A_IK A_KJ            (trailing update, I and J outside K)
+//   A_IK = -A_IK P               (column panel)
+// The three products are 64 x 64 tiles on the FP64 MFMA
+// (v_mfma_f64_16x16x4_f64). Each V-cycle then applies the inverse as an FP64
+// matrix-vector product, one workgroup per row.
 #include "kh_device_math.hpp"
 
 namespace kh {
 namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kGjBlock = 64;          // pivots per block, and the tile edge
+constexpr int kGjStride = kGjBlock + 1;  // LDS row stride (doubles), padded
 
 __global__ void __launch_bounds__(kBlock) k_dense_clear(long count, double* __restrict__ dense) {
   const long i = long(blockIdx.x) * kBlock + threadIdx.x;
@@ -20,21 +30,118 @@ __global__ void __launch_bounds__(kBlock) k_dense_scatter(int n, const int* __re
   if (i >= n) return;
   for (int j = rows[i]; j < rows[i + 1]; ++j) dense[long(i) * n + cols[j]] += double(values[j]);
 }
-// One Gauss-Jordan step on pivot k: src holds the matrix after k steps, dst
-// receives it after k + 1 (after all n steps, the inverse).
-__global__ void __launch_bounds__(kBlock) k_gauss_jordan(int n, int k, const double* __restrict__ src,
-                                                         double* __restrict__ dst) {
-  const long e = long(blockIdx.x) * kBlock + threadIdx.x;
-  if (e >= long(n) * n) return;
-  const int i = int(e / n), j = int(e - long(i) * n);
-  const double p = src[long(k) * n + k];
-  double v;
-  if (i == k && j == k) v = 1.0 / p;
-  else if (i == k) v = src[long(k) * n + j] / p;
-  else if (j == k) v = -src[long(i) * n + k] / p;
-  else v = src[e] - src[long(i) * n + k] * src[long(k) * n + j] / p;
-  dst[e] = v;
+
+// P = A_KK^-1 for K = [k0, k0 + b): element-wise Gauss-Jordan on the block in
+// LDS, pivot by pivot (a_pp' = 1/p, a_pj' = a_pj/p, a_ip' = -a_ip/p,
+// a_ij' = a_ij - a_ip a_pj / p); written to P (kGjBlock x kGjBlock) and A_KK.
+__global__ void __launch_bounds__(kBlock) k_gj_pivot(int n, int k0, int b, double* __restrict__ a,
+                                                     double* __restrict__ p) {
+  __shared__ double t[kGjBlock * kGjStride];
+  __shared__ double row[kGjBlock], col[kGjBlock];
+  for (int e = threadIdx.x; e < b * b; e += kBlock) {
+    const int i = e / b, j = e % b;
+    t[i * kGjStride + j] = a[long(k0 + i) * n + k0 + j];
+  }
+  __syncthreads();
+  for (int q = 0; q < b; ++q) {
+    for (int e = threadIdx.x; e < b; e += kBlock) {
+      row[e] = t[q * kGjStride + e];
+      col[e] = t[e * kGjStride + q];
+    }
+    __syncthreads();
+    const double piv = row[q];
+    for (int e = threadIdx.x; e < b * b; e += kBlock) {
+      const int i = e / b, j = e % b;
+      double v;
+      if (i == q && j == q) v = 1.0 / piv;
+      else if (i == q) v = row[j] / piv;
+      else if (j == q) v = -col[i] / piv;
+      else v = t[i * kGjStride + j] - col[i] * row[j] / piv;
+      t[i * kGjStride + j] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += kBlock) {
+    const int i = e / kGjBlock, j = e % kGjBlock;
+    const double v = (i < b && j < b) ? t[i * kGjStride + j] : 0.0;
+    p[e] = v;
+    if (i < b && j < b) a[long(k0 + i) * n + k0 + j] = v;
+  }
 }
+
+// The three panel and update products of one block step, 64 x 64 output
+// tiles of C = L R with L, R staged in LDS (zero outside the matrix and the
+// block). Wave w computes the 16 x 16 sub-blocks (w, 0..3); lane l of a
+// v_mfma_f64_16x16x4_f64 supplies L[16 br + l%16][4 s + l/16] and
+// R[4 s + l/16][16 bc + l%16] and its accumulator register q holds row
+// 4 q + l/16, column l % 16 (the FP64 accumulator interleaves rows; checked
+// against a CPU product on the device).
+enum class GjTile { Rows, Update, Cols };
+template <GjTile M>
+__global__ void __launch_bounds__(kBlock) k_gj_tile(int n, int k0, int b, double* __restrict__ a,
+                                                    const double* __restrict__ p) {
+  __shared__ double L[kGjBlock * kGjStride];
+  __shared__ double R[kGjBlock * kGjStride];
+  // Tile origin: Rows tiles columns, Cols tiles rows, Update tiles both.
+  int i0 = 0, j0 = 0;
+  if constexpr (M == GjTile::Rows) j0 = int(blockIdx.x) * kGjBlock;
+  if constexpr (M == GjTile::Cols) i0 = int(blockIdx.x) * kGjBlock;
+  if constexpr (M == GjTile::Update) {
+    i0 = int(blockIdx.y) * kGjBlock;
+    j0 = int(blockIdx.x) * kGjBlock;
+    if (i0 == k0 || j0 == k0) return;  // whole tile inside the block's rows or columns
+  }
+  if constexpr (M == GjTile::Rows) {
+    if (j0 == k0) return;
+  }
+  if constexpr (M == GjTile::Cols) {
+    if (i0 == k0) return;
+  }
+  for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += kBlock) {
+    const int r = e / kGjBlock, c = e % kGjBlock;
+    double lv = 0.0, rv = 0.0;
+    if constexpr (M == GjTile::Rows) {
+      lv = p[e];  // zero outside b x b
+      if (r < b && j0 + c < n) rv = a[long(k0 + r) * n + j0 + c];
+    } else if constexpr (M == GjTile::Update) {
+      if (i0 + r < n && c < b) lv = a[long(i0 + r) * n + k0 + c];
+      if (r < b && j0 + c < n) rv = a[long(k0 + r) * n + j0 + c];
+    } else {
+      if (i0 + r < n && c < b) lv = a[long(i0 + r) * n + k0 + c];
+      rv = p[e];
+    }
+    L[r * kGjStride + c] = lv;
+    R[r * kGjStride + c] = rv;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  f64x4 acc[4];
+#pragma unroll
+  for (int bc = 0; bc < 4; ++bc) acc[bc] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < kGjBlock / 4; ++s) {
+    const double lv = L[(16 * wave + (lane & 15)) * kGjStride + 4 * s + (lane >> 4)];
+#pragma unroll
+    for (int bc = 0; bc < 4; ++bc) {
+      const double rv = R[(4 * s + (lane >> 4)) * kGjStride + 16 * bc + (lane & 15)];
+      acc[bc] = __builtin_amdgcn_mfma_f64_16x16x4f64(lv, rv, acc[bc], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int bc = 0; bc < 4; ++bc) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = 16 * wave + 4 * q + (lane >> 4), c = 16 * bc + (lane & 15);
+      if constexpr (M == GjTile::Rows) {
+        if (r < b && j0 + c < n) a[long(k0 + r) * n + j0 + c] = acc[bc][q];
+      } else if constexpr (M == GjTile::Update) {
+        if (i0 + r < n && j0 + c < n) a[long(i0 + r) * n + j0 + c] -= acc[bc][q];
+      } else {
+        if (i0 + r < n && c < b) a[long(i0 + r) * n + k0 + c] = -acc[bc][q];
+      }
+    }
+  }
+}
+
 // x = inverse b in FP64, rounded to FP32; one workgroup per row.
 __global__ void __launch_bounds__(kBlock) k_dense_apply(int n, const double* __restrict__ inverse,
     const float* __restrict__ b, float* __restrict__ x, const int* __restrict__ status) {
@@ -51,21 +158,24 @@ __global__ void __launch_bounds__(kBlock) k_dense_apply(int n, const double* __r
 #define KCF(k) static_cast<const float*>(buf(a, k))
 #define KD(k) static_cast<double*>(buf(a, k))
 
-// rows, cols, values, inverse, scratch (n = rows of the dense operator; both
-// FP64 buffers n*n): the dense inverse of the CSR operator, left in `inverse`.
+// rows, cols, values, inverse, scratch (n = rows of the dense operator;
+// inverse n*n doubles, scratch at least kGjBlock^2): the dense inverse of the
+// CSR operator, left in `inverse`.
 void l_dense_invert(kh_device* d, int n, const kh_arg* a) {
   if (n > 8192) throw std::runtime_error("dense_invert: at most 8192 rows");
+  if (n < 1) return;
   const long count = long(n) * n;
-  const dim3 grid(unsigned((count + kBlock - 1) / kBlock));
-  double* first = KD(3);
-  double* second = KD(4);
-  double* src = (n % 2 == 0) ? first : second;  // the last step writes `first`
-  k_dense_clear<<<grid, dim3(kBlock), 0, d->stream>>>(count, src);
-  k_dense_scatter<<<dim3(blocks_for(n)), dim3(kBlock), 0, d->stream>>>(n, KB(0), KB(1), KCF(2), src);
-  for (int k = 0; k < n; ++k) {
-    double* dst = src == first ? second : first;
-    k_gauss_jordan<<<grid, dim3(kBlock), 0, d->stream>>>(n, k, src, dst);
-    src = dst;
+  double* m = KD(3);
+  double* p = KD(4);
+  k_dense_clear<<<dim3(unsigned((count + kBlock - 1) / kBlock)), dim3(kBlock), 0, d->stream>>>(count, m);
+  k_dense_scatter<<<dim3(blocks_for(n)), dim3(kBlock), 0, d->stream>>>(n, KB(0), KB(1), KCF(2), m);
+  const unsigned tiles = unsigned((n + kGjBlock - 1) / kGjBlock);
+  for (int k0 = 0; k0 < n; k0 += kGjBlock) {
+    const int b = std::min(kGjBlock, n - k0);
+    k_gj_pivot<<<dim3(1), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
+    k_gj_tile<GjTile::Rows><<<dim3(tiles), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
+    k_gj_tile<GjTile::Update><<<dim3(tiles, tiles), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
+    k_gj_tile<GjTile::Cols><<<dim3(tiles), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
   }
 }
 // inverse, b, x, status

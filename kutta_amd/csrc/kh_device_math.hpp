@@ -234,6 +234,14 @@ __device__ __forceinline__ void sum_partials(const double* partials, unsigned co
 // entry.
 constexpr int kChunk = 8;
 
+// A matrix value stored as bfloat16 (the mixed path's coarse AMG levels);
+// value_of widens either storage to the FP32 the row arithmetic uses.
+struct bf16v {
+  unsigned short bits;
+};
+__device__ __forceinline__ float value_of(float v) { return v; }
+__device__ __forceinline__ float value_of(bf16v v) { return __uint_as_float(unsigned(v.bits) << 16); }
+
 // x[j] for a gather; Prolonged adds the coarse correction of j's aggregate,
 // the value the reference's in-place prolongation would have stored.
 struct Plain {
@@ -252,10 +260,10 @@ struct Prolonged {
 
 // Kahan-compensated sum of a[j] * x[col[j]] over [begin, end), C entries per
 // chunk (kernels/linalg.kern.h:5-16).
-template <int C, class X>
+template <int C, class X, class V>
 __device__ __forceinline__ float row_product_chunked(int begin, int end,
                                                      const int* __restrict__ cols,
-                                                     const float* __restrict__ values, X gather) {
+                                                     const V* __restrict__ values, X gather) {
   Kahan k;
   for (int j = begin; j < end; j += C) {
     int c[C];
@@ -264,7 +272,7 @@ __device__ __forceinline__ float row_product_chunked(int begin, int end,
     for (int u = 0; u < C; ++u)
       if (j + u < end) {
         c[u] = cols[j + u];
-        v[u] = values[j + u];
+        v[u] = value_of(values[j + u]);
       }
 #pragma unroll
     for (int u = 0; u < C; ++u)
@@ -278,10 +286,10 @@ __device__ __forceinline__ float row_product_chunked(int begin, int end,
 
 // Compensated residual b - sum a[j] * x[col[j]] over [begin, end), C entries
 // per chunk (kernels/linalg.kern.h:32-45).
-template <int C, class X>
+template <int C, class X, class V>
 __device__ __forceinline__ float row_residual_chunked(float b, int begin, int end,
                                                       const int* __restrict__ cols,
-                                                      const float* __restrict__ values,
+                                                      const V* __restrict__ values,
                                                       X gather) {
   RowResidual acc(b);
   for (int j = begin; j < end; j += C) {
@@ -291,7 +299,7 @@ __device__ __forceinline__ float row_residual_chunked(float b, int begin, int en
     for (int u = 0; u < C; ++u)
       if (j + u < end) {
         c[u] = cols[j + u];
-        v[u] = values[j + u];
+        v[u] = value_of(values[j + u]);
       }
 #pragma unroll
     for (int u = 0; u < C; ++u)
@@ -307,17 +315,17 @@ __device__ __forceinline__ float row_residual_chunked(float b, int begin, int en
 // longer than one chunk, the wave takes chunks of twice the width, so such a
 // row costs one gather round trip instead of two. The choice is uniform per
 // wave and the entry order is unchanged.
-template <class X>
+template <class X, class V>
 __device__ __forceinline__ float row_product(int begin, int end, const int* __restrict__ cols,
-                                             const float* __restrict__ values, X gather) {
+                                             const V* __restrict__ values, X gather) {
   if (__any(end - begin > kChunk))
     return row_product_chunked<2 * kChunk>(begin, end, cols, values, gather);
   return row_product_chunked<kChunk>(begin, end, cols, values, gather);
 }
-template <class X>
+template <class X, class V>
 __device__ __forceinline__ float row_residual(float b, int begin, int end,
                                               const int* __restrict__ cols,
-                                              const float* __restrict__ values, X gather) {
+                                              const V* __restrict__ values, X gather) {
   if (__any(end - begin > kChunk))
     return row_residual_chunked<2 * kChunk>(b, begin, end, cols, values, gather);
   return row_residual_chunked<kChunk>(b, begin, end, cols, values, gather);

@@ -25,9 +25,10 @@ __device__ __forceinline__ float spmv_row(int i, const int* __restrict__ rows,
   return row_product(rows[i], rows[i + 1], cols, values, Plain{x});
 }
 // kernels/linalg.kern.h:32-45
+template <class V>
 __device__ __forceinline__ float residual_row(int i, const int* __restrict__ rows,
                                               const int* __restrict__ cols,
-                                              const float* __restrict__ values,
+                                              const V* __restrict__ values,
                                               const float* __restrict__ x,
                                               const float* __restrict__ b) {
   return row_residual(b[i], rows[i], rows[i + 1], cols, values, Plain{x});
@@ -143,8 +144,9 @@ __global__ void __launch_bounds__(64) k_amg_coarse_solve(const int* __restrict__
   if (row < count) x[row] = cur[row];
 }
 // kernels/linalg.kern.h:150-156
+template <class V>
 __global__ void __launch_bounds__(kBlock) k_amg_smooth(int n, const int* __restrict__ rows,
-    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
+    const int* __restrict__ cols, const V* __restrict__ values, const float* __restrict__ x,
     const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
     const int* __restrict__ status) {
   if (status[0] != 0) return;
@@ -747,8 +749,9 @@ __global__ void __launch_bounds__(kBlock) k_batch_finish2(const kh_bicg_lane* __
 // adds its members' residuals in member order (Kahan), carrying the sum from
 // tile to tile. Every member row gets a lane of its own, as in a sweep.
 constexpr int kRestrictRows = 64;
+template <class V>
 __global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
-    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const int* __restrict__ rows, const int* __restrict__ cols, const V* __restrict__ values,
     const float* __restrict__ x, const float* __restrict__ b, const int* __restrict__ offsets,
     const int* __restrict__ members, float* __restrict__ coarse, const int* __restrict__ status) {
   if (status[0] != 0) return;
@@ -778,8 +781,9 @@ __global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
 }
 // Prolongation followed by one smoothing sweep without storing x + P c:
 // amg_prolong_guarded + amg_smooth.
+template <class V>
 __global__ void __launch_bounds__(kBlock) k_fused_prolong_smooth(int n,
-    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const int* __restrict__ rows, const int* __restrict__ cols, const V* __restrict__ values,
     const float* __restrict__ x, const float* __restrict__ b, const float* __restrict__ inverse,
     float* __restrict__ next, const int* __restrict__ aggregate, const float* __restrict__ coarse,
     const int* __restrict__ status) {
@@ -793,8 +797,9 @@ __global__ void __launch_bounds__(kBlock) k_fused_prolong_smooth(int n,
 
 // fill(x, 0) followed by amg_smooth_zero, reading the zero iterate as the
 // constant +0.0f instead of from memory (kernels/linalg.kern.h:162-170).
+template <class V>
 __global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
-    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const int* __restrict__ rows, const int* __restrict__ cols, const V* __restrict__ values,
     const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
     const int* __restrict__ status) {
   if (status[0] != 0) return;
@@ -803,11 +808,25 @@ __global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
   const float bi = b[i];
   if (bi - bi != 0.0f) {
     RowResidual acc(bi);
-    for (int j = rows[i]; j < rows[i + 1]; ++j) acc.add(values[j], 0.0f);
+    for (int j = rows[i]; j < rows[i + 1]; ++j) acc.add(value_of(values[j]), 0.0f);
     next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * acc.value();
     return;
   }
   next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * (bi + 0.0f);
+}
+
+// FP32 -> bfloat16, round to nearest even (NaN stays NaN): the mixed path's
+// coarse-level value copies.
+__global__ void __launch_bounds__(kBlock) k_to_bf16(int n, const float* __restrict__ src, bf16v* __restrict__ dst) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  unsigned u = __float_as_uint(src[i]);
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    dst[i].bits = (unsigned short)((u >> 16) | 0x40u);
+    return;
+  }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  dst[i].bits = (unsigned short)(u >> 16);
 }
 
 // -------------------------------------------------------- coarse-grid tail
@@ -841,20 +860,20 @@ struct TailOperator {
   int rows;
 };
 
-// Compensated residual b - sum a[j] x[col[j]] of row i, entries in order.
+// Compensated residual b - sum a[j] x[col[j]] of row i, entries in order,
+// columns, values and gathers loaded a chunk at a time.
 template <class X>
 __device__ __forceinline__ float tail_residual(const TailOperator& L, float b, int i, X gather) {
-  RowResidual acc(b);
-  for (int j = L.row_ptr[i]; j < L.row_ptr[i + 1]; ++j) acc.add(L.val[j], gather(L.col[j]));
-  return acc.value();
+  return row_residual_chunked<kChunk>(b, L.row_ptr[i], L.row_ptr[i + 1], L.col, L.val, gather);
 }
 
 enum class TailSweep { Plain, FromZero, Prolonged };
 
-// One sweep over a level: every row writes its new value to `next`, then the
-// iterate is replaced once every row has read the previous one.
-__device__ void tail_sweep(TailSweep kind, const TailOperator& L, float* x, const float* b,
-                           const float* coarse, float* next) {
+// One sweep over a level: every row's new value goes to `out`, computed from
+// `in`. Jacobi rows read only the previous iterate, so `in` and `out` differ
+// (FromZero reads no iterate); one barrier ends the sweep.
+__device__ void tail_sweep(TailSweep kind, const TailOperator& L, const float* in, const float* b,
+                           const float* coarse, float* out) {
   for (int i = threadIdx.x; i < L.rows; i += kTailThreads) {
     const float bi = b[i];
     const float w = (2.0f / 3.0f) * L.inverse[i];
@@ -863,16 +882,24 @@ __device__ void tail_sweep(TailSweep kind, const TailOperator& L, float* x, cons
       value = bi - bi != 0.0f ? 0.0f + w * tail_residual(L, bi, i, [](int) { return 0.0f; })
                               : 0.0f + w * (bi + 0.0f);
     } else if (kind == TailSweep::Prolonged) {
-      const Prolonged gather{x, L.aggregate, coarse};
+      const Prolonged gather{in, L.aggregate, coarse};
       value = gather(i) + w * tail_residual(L, bi, i, gather);
     } else {
-      value = x[i] + w * tail_residual(L, bi, i, Plain{x});
+      value = in[i] + w * tail_residual(L, bi, i, Plain{in});
     }
-    next[i] = value;
+    out[i] = value;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < L.rows; i += kTailThreads) x[i] = next[i];
-  __syncthreads();
+}
+// `count` sweeps from the zero iterate, alternating between x and `next` so
+// that the last writes x.
+__device__ void tail_smooth_from_zero(const TailOperator& L, float* x, const float* b, float* next, int count) {
+  const float* cur = nullptr;
+  for (int t = 1; t <= count; ++t) {
+    float* out = (count - t) % 2 == 0 ? x : next;
+    tail_sweep(t == 1 ? TailSweep::FromZero : TailSweep::Plain, L, cur, b, nullptr, out);
+    cur = out;
+  }
 }
 
 // Restricted residual of level L into the next level's right-hand side.
@@ -983,8 +1010,7 @@ __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
     const float* b = x + T.rows;
     float* coarse_b = vectors + base[k + 1] + T.coarse_rows;
     const TailOperator L = tail_stage(T, true, stage);
-    for (int t = 1; t <= pre; ++t)
-      tail_sweep(t == 1 ? TailSweep::FromZero : TailSweep::Plain, L, x, b, nullptr, next);
+    tail_smooth_from_zero(L, x, b, next, pre);
     tail_restrict(L, T.member_offsets, T.members, T.coarse_rows, x, b, coarse_b);
     if (nullspace) tail_project(coarse_b, T.coarse_rows, red, pair);
   }
@@ -994,13 +1020,16 @@ __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
     const float* b = x + T.rows;
     const TailOperator L = tail_stage(T, false, stage);
     if (T.rows <= kCoarseMax) {
-      // amg_coarse_solve: sweeps from the zero iterate, general residual form.
-      for (int i = threadIdx.x; i < T.rows; i += kTailThreads) x[i] = 0.0f;
+      // amg_coarse_solve: sweeps from the zero iterate, general residual form
+      // (both buffers start at zero, so the first sweep may read either).
+      for (int i = threadIdx.x; i < T.rows; i += kTailThreads) x[i] = next[i] = 0.0f;
       __syncthreads();
-      for (int t = 0; t < coarse_sweeps; ++t) tail_sweep(TailSweep::Plain, L, x, b, nullptr, next);
+      for (int t = 0; t < coarse_sweeps; ++t) {
+        float* out = (coarse_sweeps - 1 - t) % 2 == 0 ? x : next;
+        tail_sweep(TailSweep::Plain, L, out == x ? next : x, b, nullptr, out);
+      }
     } else {
-      for (int t = 1; t <= coarse_sweeps; ++t)
-        tail_sweep(t == 1 ? TailSweep::FromZero : TailSweep::Plain, L, x, b, nullptr, next);
+      tail_smooth_from_zero(L, x, b, next, coarse_sweeps);
     }
     if (nullspace) tail_project(x, T.rows, red, pair);
   }
@@ -1009,8 +1038,21 @@ __global__ void __launch_bounds__(kTailThreads) k_fused_coarse_tail(
     float* x = vectors + base[k];
     const float* b = x + T.rows;
     const TailOperator L = tail_stage(T, true, stage);
+    // The prolongation sweep reads x, so it writes `next`; with an odd sweep
+    // count the result returns to x by a copy, then the remaining sweeps
+    // alternate so that the last writes x.
     tail_sweep(TailSweep::Prolonged, L, x, b, vectors + base[k + 1], next);
-    for (int t = 2; t <= post; ++t) tail_sweep(TailSweep::Plain, L, x, b, nullptr, next);
+    const float* cur = next;
+    if (post % 2 == 1) {
+      for (int i = threadIdx.x; i < T.rows; i += kTailThreads) x[i] = next[i];
+      __syncthreads();
+      cur = x;
+    }
+    for (int t = 2; t <= post; ++t) {
+      float* out = (post - t) % 2 == 0 ? x : next;
+      tail_sweep(TailSweep::Plain, L, cur, b, nullptr, out);
+      cur = out;
+    }
     if (nullspace) tail_project(x, T.rows, red, pair);
   }
   for (int i = threadIdx.x; i < levels[0].rows; i += kTailThreads) out[i] = vectors[base[0] + i];
@@ -1073,11 +1115,11 @@ void l_amg_coarse_solve(kh_device* d, int n, const kh_arg* a) {
 }
 void l_amg_smooth(kh_device* d, int n, const kh_arg* a) {
   if (d->packet_start) {
-    hipExtLaunchKernelGGL(k_amg_smooth, dim3(blocks_for(n)), dim3(kBlock), 0, d->stream,
+    hipExtLaunchKernelGGL(k_amg_smooth<float>, dim3(blocks_for(n)), dim3(kBlock), 0, d->stream,
                           d->packet_start, d->packet_stop, 0, n, KB(0), KB(1), KCF(2), KCF(3),
                           KCF(4), KCF(5), KF(6), KB(7));
   } else {
-    k_amg_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+    k_amg_smooth<float><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
   }
 }
 void l_amg_smooth_zero(kh_device* d, int n, const kh_arg* a) {
@@ -1178,12 +1220,12 @@ void l_fused_step_dot_check(kh_device* d, int n, const kh_arg* a) {
 }
 void l_fused_residual_restrict(kh_device* d, int n, const kh_arg* a) {
   const unsigned groups = unsigned((n + kRestrictRows - 1) / kRestrictRows);
-  k_fused_residual_restrict<<<dim3(groups), dim3(kBlock), 0, d->stream>>>(
+  k_fused_residual_restrict<float><<<dim3(groups), dim3(kBlock), 0, d->stream>>>(
       n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6), KF(7), KB(8));
 }
 void l_fused_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
-  k_fused_prolong_smooth<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
-                                      KB(7), KCF(8), KB(9));
+  k_fused_prolong_smooth<float><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
+                                             KB(7), KCF(8), KB(9));
 }
 void l_fused_smooth_dot_beta(kh_device* d, int n, const kh_arg* a) {
   const unsigned g = reduce_grid(n);
@@ -1200,7 +1242,28 @@ void l_fused_smooth_sum(kh_device* d, int n, const kh_arg* a) {
   k_finish<Finish::Pair><<<FINISH>>>(d->partials, g, nullptr, st, KF(8));
 }
 void l_fused_smooth_from_zero(kh_device* d, int n, const kh_arg* a) {
-  k_fused_smooth_from_zero<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6));
+  k_fused_smooth_from_zero<float><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6));
+}
+// The bfloat16-valued variants (values argument: KH_TO_BF16's output).
+#define KBF(k) static_cast<const bf16v*>(buf(a, k))
+void l_bf16_amg_smooth(kh_device* d, int n, const kh_arg* a) {
+  k_amg_smooth<bf16v><<<GRID(n)>>>(n, KB(0), KB(1), KBF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+}
+void l_bf16_residual_restrict(kh_device* d, int n, const kh_arg* a) {
+  const unsigned groups = unsigned((n + kRestrictRows - 1) / kRestrictRows);
+  k_fused_residual_restrict<bf16v><<<dim3(groups), dim3(kBlock), 0, d->stream>>>(
+      n, KB(0), KB(1), KBF(2), KCF(3), KCF(4), KB(5), KB(6), KF(7), KB(8));
+}
+void l_bf16_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
+  k_fused_prolong_smooth<bf16v><<<GRID(n)>>>(n, KB(0), KB(1), KBF(2), KCF(3), KCF(4), KCF(5), KF(6),
+                                             KB(7), KCF(8), KB(9));
+}
+void l_bf16_smooth_from_zero(kh_device* d, int n, const kh_arg* a) {
+  k_fused_smooth_from_zero<bf16v><<<GRID(n)>>>(n, KB(0), KB(1), KBF(2), KCF(3), KCF(4), KF(5), KB(6));
+}
+// n = entries; src (FP32), dst (bf16)
+void l_to_bf16(kh_device* d, int n, const kh_arg* a) {
+  k_to_bf16<<<GRID(n)>>>(n, KCF(0), static_cast<bf16v*>(buf(a, 1)));
 }
 void l_fused_coarse_tail(kh_device* d, int n, const kh_arg* a) {
   const int count = i32(a, 1);
@@ -1355,6 +1418,11 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_FUSED_SPMV_DOT2_OMEGA, "fused_spmv_dot2_omega", "bbbbbbbb", l_fused_spmv_dot2_omega},
       {KH_FUSED_BICG_STEP_CHECK_BETA, "fused_bicg_step_check_beta", "bbbbbbbbb", l_fused_bicg_step_check_beta},
       {KH_BATCH_BICG_DIRECTION, "batch_bicg_direction", "bii", l_batch_bicg_direction},
+      {KH_TO_BF16, "to_bf16", "bb", l_to_bf16},
+      {KH_BF16_AMG_SMOOTH, "bf16_amg_smooth", "bbbbbbbb", l_bf16_amg_smooth},
+      {KH_BF16_SMOOTH_FROM_ZERO, "bf16_smooth_from_zero", "bbbbbbb", l_bf16_smooth_from_zero},
+      {KH_BF16_RESIDUAL_RESTRICT, "bf16_residual_restrict", "bbbbbbbbb", l_bf16_residual_restrict},
+      {KH_BF16_PROLONG_SMOOTH, "bf16_prolong_smooth", "bbbbbbbbbb", l_bf16_prolong_smooth},
       {KH_BATCH_SPMV_SHADOW_ALPHA, "batch_spmv_shadow_alpha", "bii", l_batch_spmv_shadow_alpha},
       {KH_BATCH_BICG_DIFFERENCE_S, "batch_bicg_difference_s", "bii", l_batch_bicg_difference_s},
       {KH_BATCH_SPMV_DOT2_OMEGA, "batch_spmv_dot2_omega", "bii", l_batch_spmv_dot2_omega},

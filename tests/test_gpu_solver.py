@@ -241,3 +241,42 @@ def test_batched_bicgstab_equals_individual_solves(device, interval):
     xo, sto, _ = oracle_solve(ops[0], "BicgstabJacobi", rhs[0], controls)
     assert st_single[0].iterations == sto["iterations"]
     assert np.array_equal(xs_batched[0].download().view(np.uint32), xo.view(np.uint32))
+
+
+def test_pcg_graph_replay_equals_launches(device):
+    """Fused PCG iterations replayed from a captured HIP graph (the default)
+    against the same solver launching kernel by kernel: bitwise x, iterations
+    and residuals over several solves, with the operator's values changed in
+    place and refreshed between them (the graph is reused), a different
+    coarse tail (the graph is recaptured), and a solve that stops at
+    max_iter (its last iteration runs outside the graph). The oracle pins the
+    first solve."""
+    a = oracle.pressure_problem(20, 19, 18).csr()
+    n = a.n_rows
+    opts = AmgOptions(aggregation_passes=2, structure=OperatorStructure.SymmetricMMatrix)
+    host = kutta_amd.CsrMatrixHost(a.row_ptr, a.col, a.diag, a.val)
+    runs = []
+    for graphs in (True, False):
+        matrix = kutta_amd.upload(device, host)
+        solver = kutta_amd.make_amg_solver(SolverKind.AmgPcg, device, host, opts)
+        solver.set_graphs(graphs)
+        solver.refresh(matrix)
+        out = []
+        rng = np.random.default_rng(9)
+        for k in range(5):
+            if k == 2:  # scale the values in place (same buffers) and refresh
+                matrix.val.upload((a.val * 1.5).astype(np.float32))
+                solver.refresh(matrix)
+            if k == 3:
+                solver.set_tail(0)
+            b = rng.standard_normal(n).astype(np.float32)
+            x = device.alloc(n)
+            x.fill(np.float32(0))
+            control = SolveControl(1e-6, 0.0, 7 if k == 4 else 300, 8)
+            st = solver.solve(matrix, x, device.upload(b), control)
+            out.append((st.iterations, st.final_residual, st.converged, x.download()))
+        runs.append(out)
+    for (i0, r0, c0, x0), (i1, r1, c1, x1) in zip(*runs):
+        assert (i0, r0, c0) == (i1, r1, c1)
+        assert np.array_equal(x0.view(np.uint32), x1.view(np.uint32))
+    assert runs[0][4][0] == 7 and not runs[0][4][2]
