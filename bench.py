@@ -171,7 +171,7 @@ def fv_kernel_bench(size: int, repeats: int) -> int:
 
 
 def simplec_bench(dev, size: int, nz: int, reynolds: float, relaxation: float, monitor: int,
-                  warmup: int, iterations: int, cpu_iterations: int) -> dict:
+                  warmup: int, iterations: int, cpu_iterations: int, graphs: bool = True) -> dict:
     """ms per steady SIMPLEC outer iteration on the lid-driven cavity
     make_box_hex(size, size, nz): lid u=1 on ymax, walls at rest, symmetry on
     zmin/zmax, viscosity 1/Re, paired state and the reference's default
@@ -187,6 +187,7 @@ def simplec_bench(dev, size: int, nz: int, reynolds: float, relaxation: float, m
     t = time.perf_counter()
     solver = kutta_amd.SteadySolver(dev, size, size, nz, conditions, settings)
     setup_s = time.perf_counter() - t
+    solver.set_graphs(graphs)
     solver.iterate(warmup)
     dev.synchronize()
     waits0 = solver.info()["host_waits"]
@@ -241,6 +242,9 @@ def main() -> int:
                     help="first AMG level of the one-workgroup coarse-grid tail (-1: chosen by the "
                          "autotuner, 0 off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every PCG iteration kernel by kernel instead of replaying a captured HIP "
+                         "graph (identical iterates; for kernel tracers)")
     ap.add_argument("--fv-kernels", action="store_true",
                     help="only time each finite-volume kernel on the size^3 mesh")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -280,10 +284,11 @@ def main() -> int:
             # bench/bench_fv.cpp:128-143: default settings except momentum
             # relaxation 1, iterate(5) untimed, then timed iterations.
             n = args.simplec_box
-            line = simplec_bench(dev, n, n, 100.0, 1.0, 20, 5, max(args.steps, 1), 0)
+            line = simplec_bench(dev, n, n, 100.0, 1.0, 20, 5, max(args.steps, 1), 0, not args.no_graphs)
         else:
             line = simplec_bench(dev, args.simplec_size, 1, 1000.0, 1.0, 100, 5,
-                                 max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20)
+                                 max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20,
+                                 not args.no_graphs)
         line = {"metric": "ms/SIMPLEC iteration", "value": line["ms_per_iteration"], "unit": "ms",
                 "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
                 "config": line}
@@ -312,6 +317,7 @@ def main() -> int:
     solver = kutta_amd.make_amg_solver(SolverKind.AmgPcg, dev, problem.host, options)
     setup_amg_s = time.perf_counter() - t
     tail_level = solver.set_tail(args.tail)
+    solver.set_graphs(not args.no_graphs)
     solver.refresh(problem)
     if args.tail < 0:  # automatic: the autotuner times each admissible tail level (identical iterates)
         t = time.perf_counter()
@@ -357,7 +363,7 @@ def main() -> int:
     dev.synchronize()
     launches, kernel_ms = dev.profile_read(Kernel.amg_smooth)
     dev.profile(Kernel.amg_smooth, False)
-    solver.set_graphs(True)
+    solver.set_graphs(not args.no_graphs)
     st = stats[-1]
     if not st.converged:
         print(f"[bench] warning: solve did not converge ({st})", file=sys.stderr)
@@ -401,6 +407,7 @@ def main() -> int:
         other = kutta_amd.make_amg_solver(kind, dev, problem.host, options)
         level = other.set_direct(direct) if direct else 0
         other.set_coarse_bf16(bf16)
+        other.set_graphs(not args.no_graphs)
         t = time.perf_counter()
         other.refresh(problem)
         dev.synchronize()
@@ -494,7 +501,8 @@ def main() -> int:
         # Configuration 1: 128x128x1 cavity, Re=1000, relaxation 1, monitors
         # every 100, 500 iterations; ms averaged over iterations 6-500.
         simplec = simplec_bench(dev, args.simplec_size, 1, 1000.0, 1.0, 100, 5,
-                                max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20)
+                                max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20,
+                                not args.no_graphs)
 
     baseline = None
     x_gpu = x_plain

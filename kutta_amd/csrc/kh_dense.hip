@@ -32,36 +32,39 @@ __global__ void __launch_bounds__(kBlock) k_dense_scatter(int n, const int* __re
 }
 
 // P = A_KK^-1 for K = [k0, k0 + b): element-wise Gauss-Jordan on the block in
-// LDS, pivot by pivot (a_pp' = 1/p, a_pj' = a_pj/p, a_ip' = -a_ip/p,
-// a_ij' = a_ij - a_ip a_pj / p); written to P (kGjBlock x kGjBlock) and A_KK.
-__global__ void __launch_bounds__(kBlock) k_gj_pivot(int n, int k0, int b, double* __restrict__ a,
-                                                     double* __restrict__ p) {
+// LDS, pivot by pivot with the pivot's reciprocal r = 1/p (a_pp' = r,
+// a_pj' = a_pj r, a_ip' = -a_ip r, a_ij' = a_ij - a_ip (a_pj r)); written to
+// P (kGjBlock x kGjBlock) and A_KK. One 1024-thread workgroup, four entries
+// per thread.
+constexpr int kGjPivotThreads = 1024;
+__global__ void __launch_bounds__(kGjPivotThreads) k_gj_pivot(int n, int k0, int b, double* __restrict__ a,
+                                                              double* __restrict__ p) {
   __shared__ double t[kGjBlock * kGjStride];
   __shared__ double row[kGjBlock], col[kGjBlock];
-  for (int e = threadIdx.x; e < b * b; e += kBlock) {
+  for (int e = threadIdx.x; e < b * b; e += kGjPivotThreads) {
     const int i = e / b, j = e % b;
     t[i * kGjStride + j] = a[long(k0 + i) * n + k0 + j];
   }
   __syncthreads();
   for (int q = 0; q < b; ++q) {
-    for (int e = threadIdx.x; e < b; e += kBlock) {
-      row[e] = t[q * kGjStride + e];
-      col[e] = t[e * kGjStride + q];
+    const double r = 1.0 / t[q * kGjStride + q];
+    if (threadIdx.x < b) {
+      row[threadIdx.x] = t[q * kGjStride + threadIdx.x] * r;  // the new pivot row (a_qj r)
+      col[threadIdx.x] = t[threadIdx.x * kGjStride + q];
     }
     __syncthreads();
-    const double piv = row[q];
-    for (int e = threadIdx.x; e < b * b; e += kBlock) {
+    for (int e = threadIdx.x; e < b * b; e += kGjPivotThreads) {
       const int i = e / b, j = e % b;
       double v;
-      if (i == q && j == q) v = 1.0 / piv;
-      else if (i == q) v = row[j] / piv;
-      else if (j == q) v = -col[i] / piv;
-      else v = t[i * kGjStride + j] - col[i] * row[j] / piv;
+      if (i == q && j == q) v = r;
+      else if (i == q) v = row[j];
+      else if (j == q) v = -col[i] * r;
+      else v = t[i * kGjStride + j] - col[i] * row[j];
       t[i * kGjStride + j] = v;
     }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += kBlock) {
+  for (int e = threadIdx.x; e < kGjBlock * kGjBlock; e += kGjPivotThreads) {
     const int i = e / kGjBlock, j = e % kGjBlock;
     const double v = (i < b && j < b) ? t[i * kGjStride + j] : 0.0;
     p[e] = v;
@@ -172,7 +175,7 @@ void l_dense_invert(kh_device* d, int n, const kh_arg* a) {
   const unsigned tiles = unsigned((n + kGjBlock - 1) / kGjBlock);
   for (int k0 = 0; k0 < n; k0 += kGjBlock) {
     const int b = std::min(kGjBlock, n - k0);
-    k_gj_pivot<<<dim3(1), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
+    k_gj_pivot<<<dim3(1), dim3(kGjPivotThreads), 0, d->stream>>>(n, k0, b, m, p);
     k_gj_tile<GjTile::Rows><<<dim3(tiles), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
     k_gj_tile<GjTile::Update><<<dim3(tiles, tiles), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
     k_gj_tile<GjTile::Cols><<<dim3(tiles), dim3(kBlock), 0, d->stream>>>(n, k0, b, m, p);
