@@ -173,6 +173,23 @@ int ka_solver_set_direct(void* solver, int rows, int32_t* level);
  * fv/steady_solver.hpp:193-271; the members not listed keep their defaults). */
 int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, const double* wall,
                       const double* settings, void** out);
+/* As ka_simplec_create on the box [0, extent[0]] x [0, extent[1]] x
+ * [0, extent[2]] (the unit cube when extent is NULL) with inlets and outlets
+ * (fv/steady_solver.hpp:38-51, set up as fv/steady_solver.cpp:314-345):
+ * kinds[p] 0 wall, 1 velocity inlet,
+ * 2 pressure outlet with a backflow velocity, 3 symmetry, 4 pressure outlet
+ * with normal backflow; velocity[3p..3p+2] the wall, inlet or backflow
+ * velocity; pressure[p] the outlet static pressure and total[p] != 0 for
+ * BackflowPressure::Total (pressure, total may be NULL for closed domains).
+ * A domain with an outlet pins no reference cell (:247). */
+int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
+                           const double* velocity, const double* pressure, const int32_t* total,
+                           const double* settings, void** out);
+/* SteadySolver::initialize (fv/steady_solver.cpp:1480-1491): u, v, w, p hold
+ * one value per cell in the solver's cell order (that of
+ * ka_simplec_centroids); a closed domain's pressure is shifted so that the
+ * reference cell holds zero. */
+int ka_simplec_initialize(void* solver, const double* u, const double* v, const double* w, const double* p);
 void ka_simplec_destroy(void* solver);
 /* SteadySolver::iterate(count): count SIMPLEC outer iterations. */
 int ka_simplec_iterate(void* solver, int count);

@@ -132,6 +132,9 @@ def host_lib() -> C.CDLL:
         lib.ka_coarsening_destroy.argtypes = [vp]
         lib.ka_simplec_create.argtypes = [vp, i32, i32, i32, pi32, C.POINTER(dbl), C.POINTER(dbl),
                                           C.POINTER(vp)]
+        lib.ka_simplec_create_open.argtypes = [vp, i32, i32, i32, C.POINTER(dbl), pi32, C.POINTER(dbl),
+                                               C.POINTER(dbl), pi32, C.POINTER(dbl), C.POINTER(vp)]
+        lib.ka_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
         lib.ka_simplec_destroy.argtypes = [vp]
         lib.ka_simplec_iterate.argtypes = [vp, i32]
         lib.ka_simplec_report.argtypes = [vp, i32, C.POINTER(dbl)]
@@ -981,6 +984,24 @@ class Symmetry:
 
 
 @dataclass
+class VelocityInlet:
+    """fv/steady_solver.hpp:38-42 with a uniform velocity (laminar: no
+    turbulence condition)."""
+    velocity: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class PressureOutlet:
+    """fv/steady_solver.hpp:43-51: static `pressure`; backflow along the face
+    normal when `backflow_velocity` is None (NormalBackflow, the default),
+    else that uniform velocity; `backflow_total` selects
+    BackflowPressure::Total."""
+    pressure: float = 0.0
+    backflow_velocity: tuple | None = None
+    backflow_total: bool = False
+
+
+@dataclass
 class SolverSettings:
     """fv/steady_solver.hpp:193-271: the members the MI355X SIMPLEC honours
     (the others keep the reference defaults)."""
@@ -1016,6 +1037,49 @@ def pack_conditions(conditions: dict):
     return kinds, wall
 
 
+def pack_open_conditions(conditions: dict):
+    """As pack_conditions with inlets and outlets: (kinds, velocity, pressure,
+    total) in the boundary_kind codes 0 wall, 1 velocity inlet, 2 outlet with
+    a backflow velocity, 3 symmetry, 4 outlet with normal backflow."""
+    kinds, velocity = (C.c_int32 * 6)(), (C.c_double * 18)()
+    pressure, total = (C.c_double * 6)(), (C.c_int32 * 6)()
+    for p, name in enumerate(PATCHES):
+        c = conditions[name]
+        if isinstance(c, (Wall, VelocityInlet)):
+            kinds[p] = 0 if isinstance(c, Wall) else 1
+            velocity[3 * p:3 * p + 3] = [float(v) for v in c.velocity]
+        elif isinstance(c, Symmetry):
+            kinds[p] = 3
+        elif isinstance(c, PressureOutlet):
+            kinds[p] = 4 if c.backflow_velocity is None else 2
+            if c.backflow_velocity is not None:
+                velocity[3 * p:3 * p + 3] = [float(v) for v in c.backflow_velocity]
+            pressure[p] = float(c.pressure)
+            total[p] = 1 if c.backflow_total else 0
+        else:
+            raise KuttaError(f"SIMPLEC: unsupported condition for {name}")
+    return kinds, velocity, pressure, total
+
+
+def initial_fields(centroids: np.ndarray, velocity, pressure):
+    """(u, v, w, p) per cell in float64 from constants, functions of a
+    centroid, or per-cell arrays."""
+    n = len(centroids)
+
+    def each(value, width):
+        if callable(value):
+            value = [value(tuple(x)) for x in centroids]
+        out = np.asarray(value, np.float64)
+        if out.ndim == (1 if width else 0):
+            out = np.broadcast_to(out, (n, width) if width else (n,))
+        if out.shape != ((n, width) if width else (n,)):
+            raise KuttaError(f"SIMPLEC: initialization needs one value per cell ({n})")
+        return out
+
+    u, p = each(velocity, 3), each(pressure, 0)
+    return tuple(np.ascontiguousarray(u[:, k]) for k in range(3)) + (np.ascontiguousarray(p),)
+
+
 def cavity_conditions(lid: float = 1.0) -> dict:
     """validation/cases/cavity.hpp:113-118: lid (ymax) moving in x, walls at
     rest, symmetry front and back (zmin, zmax)."""
@@ -1037,16 +1101,26 @@ REPORT_FIELDS = ("iteration", "u_raw", "u_scaled", "u_fluent_scaled", "u_fluent_
 
 class SteadySolver:
     """Steady laminar SIMPLEC on the MI355X (fv/steady_solver.hpp:286-360) over
-    make_box_hex(nx, ny, nz) on [0,1]^3 with the given patch conditions."""
+    make_box_hex(nx, ny, nz) on [0, extent] (the unit cube by default) with
+    the given patch conditions."""
 
     def __init__(self, device: Device, nx: int, ny: int, nz: int, conditions: dict,
-                 settings: SolverSettings | None = None):
+                 settings: SolverSettings | None = None, extent=(1.0, 1.0, 1.0)):
         s = settings or SolverSettings()
-        kinds, wall = pack_conditions(conditions)
+        kinds, velocity, pressure, total = pack_open_conditions(conditions)
         h = C.c_void_p()
-        _check_host(host_lib().ka_simplec_create(device.handle, nx, ny, nz, kinds, wall, s.packed(),
-                                                 C.byref(h)))
+        box = (C.c_double * 3)(*[float(e) for e in extent])
+        _check_host(host_lib().ka_simplec_create_open(device.handle, nx, ny, nz, box, kinds, velocity, pressure,
+                                                      total, s.packed(), C.byref(h)))
         self.device, self.handle, self.settings = device, h.value, s
+
+    def initialize(self, velocity=(0.0, 0.0, 0.0), pressure=0.0) -> None:
+        """fv/steady_solver.cpp:1480-1491. `velocity` and `pressure` are
+        constants, functions of the cell centroid (as the reference's), or
+        per-cell arrays in the solver's cell order (that of centroids())."""
+        u, v, w, p = initial_fields(self.centroids(), velocity, pressure)
+        _check_host(host_lib().ka_simplec_initialize(self.handle, u.ctypes.data, v.ctypes.data, w.ctypes.data,
+                                                     p.ctypes.data))
 
     def close(self) -> None:
         if self.handle:

@@ -1,4 +1,5 @@
 // C ABI of the steady SIMPLEC solver (include/kutta_amd.h).
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -19,14 +20,21 @@ Simplec& handle(void* h) { return *static_cast<Simplec*>(h); }
 
 extern "C" {
 
-int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, const double* wall,
-                      const double* settings, void** out) {
+namespace {
+int create(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds, const double* velocity,
+           const double* pressure, const int32_t* total, const double* settings, void** out) {
   return guard([&] {
     KA_REQUIRE(nx > 0 && ny > 0 && nz > 0, "box dimensions must be positive");
-    auto mesh = make_box(nx, ny, nz, {0, 0, 0}, {1, 1, 1}, false);
+    const Vec3 upper = extent ? Vec3{extent[0], extent[1], extent[2]} : Vec3{1, 1, 1};
+    KA_REQUIRE(upper.x > 0 && upper.y > 0 && upper.z > 0 && std::isfinite(upper.x) && std::isfinite(upper.y) &&
+                   std::isfinite(upper.z),
+               "box extent must be positive and finite");
+    auto mesh = make_box(nx, ny, nz, {0, 0, 0}, upper, false);
     std::vector<PatchCondition> conditions;
     for (std::size_t p = 0; p < mesh.patches.size(); ++p)
-      conditions.push_back({mesh.patches[p].name, kinds[p], {wall[3 * p], wall[3 * p + 1], wall[3 * p + 2]}});
+      conditions.push_back({mesh.patches[p].name, kinds[p],
+                            {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]},
+                            pressure ? pressure[p] : 0.0, total && total[p] != 0});
     SolverSettings s;
     s.density = settings[0];
     s.viscosity = settings[1];
@@ -45,6 +53,32 @@ int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, c
       throw;
     }
     *out = h;
+  });
+}
+}  // namespace
+
+int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, const double* wall,
+                      const double* settings, void** out) {
+  const int rc = guard([&] {
+    for (int p = 0; p < 6; ++p)
+      KA_REQUIRE(kinds[p] == PatchCondition::Wall || kinds[p] == PatchCondition::Symmetry,
+                 "ka_simplec_create: walls (0) and symmetry (3) only; ka_simplec_create_open takes the rest");
+  });
+  if (rc != 0) return rc;
+  return create(dev, nx, ny, nz, nullptr, kinds, wall, nullptr, nullptr, settings, out);
+}
+int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
+                           const double* velocity, const double* pressure, const int32_t* total,
+                           const double* settings, void** out) {
+  return create(dev, nx, ny, nz, extent, kinds, velocity, pressure, total, settings, out);
+}
+int ka_simplec_initialize(void* solver, const double* u, const double* v, const double* w, const double* p) {
+  return guard([&] {
+    auto& s = *handle(solver).solver;
+    const auto n = std::size_t(s.cells());
+    std::vector<Vec3> velocity(n);
+    for (std::size_t c = 0; c < n; ++c) velocity[c] = {u[c], v[c], w[c]};
+    s.initialize(velocity, std::vector<double>(p, p + n));
   });
 }
 void ka_simplec_destroy(void* solver) { delete static_cast<Simplec*>(solver); }

@@ -503,11 +503,14 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
                  std::isfinite(s.paired_state_slack_ulps) && s.paired_state_slack_ulps > 0,
              "FV invalid steady solver settings");
   std::map<std::string, PatchCondition> by_name;
+  bool outlet = false;
   for (auto& c : conditions) {
-    KA_REQUIRE(c.kind == PatchCondition::Wall || c.kind == PatchCondition::Symmetry,
-               "SIMPLEC: only wall and symmetry patches are supported (closed domains)");
+    KA_REQUIRE(c.kind >= PatchCondition::Wall && c.kind <= PatchCondition::NormalBackflowOutlet,
+               "SIMPLEC: walls, velocity inlets, pressure outlets and symmetry are supported");
     KA_REQUIRE(std::isfinite(c.velocity.x) && std::isfinite(c.velocity.y) && std::isfinite(c.velocity.z),
-               "wall: nonfinite velocity");
+               "FV nonfinite boundary velocity");
+    KA_REQUIRE(std::isfinite(c.pressure), "FV nonfinite outlet pressure");
+    outlet = outlet || c.kind == PatchCondition::PressureOutlet || c.kind == PatchCondition::NormalBackflowOutlet;
     KA_REQUIRE(by_name.emplace(c.patch, c).second, "FV duplicate patch condition");
   }
   KA_REQUIRE(by_name.size() == host.patches.size(), "FV supply exactly one condition per patch");
@@ -519,7 +522,9 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   renumber_cells_rcm(host);
   geom = compute_geometry(host);
   KA_REQUIRE(host.n_cells > 0, "FV steady solver needs a nonempty mesh");
-  reference = locate_cell(host, geom, s.reference_point);
+  // :229-247: a pressure boundary sets the pressure level; a closed domain
+  // pins the cell holding the reference point.
+  reference = outlet ? -1 : locate_cell(host, geom, s.reference_point);
   {
     const auto cn = build_cell_neighbours(host);
     std::vector<char> seen(host.n_cells, 0);
@@ -629,29 +634,42 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   consistent_inverse = zeros(n);
 
   // Boundary kinds, prescribed values and LSQ observations (:314-400).
-  std::vector<Index> kinds(nb, 0);
-  std::vector<float> pu(nb, 0.0f), pv(nb, 0.0f), pw(nb, 0.0f);
+  std::vector<Index> kinds(nb, 0), totals(nb, 0);
+  std::vector<float> pu(nb, 0.0f), pv(nb, 0.0f), pw(nb, 0.0f), pp(nb, 0.0f);
+  double pmin = std::numeric_limits<double>::infinity(), pmax = -pmin;
   for (std::size_t p = 0; p < host.patches.size(); ++p)
     for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
       const Index b = f - ni;
+      const auto& c = patch_condition[p];
+      kinds[b] = c.kind;
       Vec3 v{};
-      if (patch_condition[p].kind == PatchCondition::Wall) {
-        v = patch_condition[p].velocity;
-        kinds[b] = 0;
-      } else {
-        kinds[b] = 3;
+      if (c.kind == PatchCondition::Wall || c.kind == PatchCondition::VelocityInlet ||
+          c.kind == PatchCondition::PressureOutlet)
+        v = c.velocity;
+      if (c.kind == PatchCondition::PressureOutlet || c.kind == PatchCondition::NormalBackflowOutlet) {
+        totals[b] = c.total_backflow ? 1 : 0;
+        pp[b] = float(c.pressure);
+        pmin = std::min(pmin, c.pressure);
+        pmax = std::max(pmax, c.pressure);
       }
       pu[b] = float(v.x);
       pv[b] = float(v.y);
       pw[b] = float(v.z);
       bc_speed = std::max(bc_speed, length_of(v));
     }
-  pressure_speed = 0;  // no pressure boundary
-  std::vector<Observation> observe_velocity(nb), observe_pressure(nb, Observation::Omitted);
-  for (Index b = 0; b < nb; ++b)
-    observe_velocity[b] = kinds[b] == 3 ? Observation::NormalFootValue : Observation::FaceValue;
+  pressure_speed = outlet ? .1 * std::sqrt((pmax - pmin) / s.density) : 0;
+  total_pressure = to_device(dev, totals);
+  // :351-358: velocity observed at walls and inlets, its normal foot at
+  // symmetry planes; pressure observed at outlets.
+  std::vector<Observation> observe_velocity(nb), observe_pressure(nb);
+  for (Index b = 0; b < nb; ++b) {
+    observe_velocity[b] = kinds[b] == 3                  ? Observation::NormalFootValue
+                          : (kinds[b] == 0 || kinds[b] == 1) ? Observation::FaceValue
+                                                             : Observation::Omitted;
+    observe_pressure[b] = (kinds[b] == 2 || kinds[b] == 4) ? Observation::FaceValue : Observation::Omitted;
+  }
   boundary_kind = to_device(dev, kinds);
-  prescribed_pressure = zeros(nb);
+  prescribed_pressure = to_device(dev, pp);
   prescribed_velocity.x = to_device(dev, pu);
   prescribed_velocity.y = to_device(dev, pv);
   prescribed_velocity.z = to_device(dev, pw);
@@ -922,6 +940,34 @@ SteadySolver::~SteadySolver() = default;
 void SteadySolver::iterate(int count) {
   KA_REQUIRE(count >= 0, "FV negative iteration count");
   while (count-- > 0) impl_->step();
+}
+void SteadySolver::initialize(const std::vector<Vec3>& u, const std::vector<double>& p) {
+  auto& s = *impl_;
+  const auto n = std::size_t(s.n);
+  KA_REQUIRE(u.size() == n && p.size() == n, "FV initialization needs one velocity and pressure per cell");
+  const double ref = s.reference >= 0 ? p[std::size_t(s.reference)] : 0;
+  std::vector<float> ux(n), uy(n), uz(n), pf(n);
+  for (std::size_t c = 0; c < n; ++c) {
+    const double value = p[c] - ref;
+    KA_REQUIRE(std::isfinite(value), "FV nonfinite pressure initialization");
+    ux[c] = float(u[c].x);
+    uy[c] = float(u[c].y);
+    uz[c] = float(u[c].z);
+    pf[c] = float(value);
+  }
+  if (s.paired)  // reset_paired (:977-988)
+    for (auto* v : {&s.velocity_lo.x, &s.velocity_lo.y, &s.velocity_lo.z, &s.pressure_gradient_lo.x,
+                    &s.pressure_gradient_lo.y, &s.pressure_gradient_lo.z, &s.pressure_lo, &s.mass_lo})
+      s.dev.fill(v->data(), 0.0f, v->size());
+  s.velocity.x.upload(ux);
+  s.velocity.y.upload(uy);
+  s.velocity.z.upload(uz);
+  s.pressure.upload(pf);
+  s.update_boundaries();
+  s.update_pressure();
+  s.dev.fill(s.physical_diagonal.data(), 1.0f, n);
+  s.dev.fill(s.consistent_inverse.data(), 0.0f, n);
+  s.update_flux();
 }
 IterationReport SteadySolver::monitor() { return impl_->monitor(); }
 ForceReport SteadySolver::forces(const std::vector<std::string>& patches, double reference_pressure) {

@@ -18,12 +18,16 @@
 
 namespace kutta_amd {
 
-// fv/steady_solver.hpp:23-27, 65 (the supported conditions).
+// fv/steady_solver.hpp:23-51, 65 (the supported conditions), coded by the
+// boundary_kind of the face kernels: a pressure outlet with a backflow
+// velocity is kind 2, with normal backflow (the reference's default) kind 4.
 struct PatchCondition {
-  enum Kind : int { Wall = 0, Symmetry = 3 };
+  enum Kind : int { Wall = 0, VelocityInlet = 1, PressureOutlet = 2, Symmetry = 3, NormalBackflowOutlet = 4 };
   std::string patch;
   int kind = Wall;
-  Vec3 velocity{};  // wall velocity
+  Vec3 velocity{};        // wall, inlet or backflow velocity (uniform)
+  double pressure = 0;    // outlet static pressure
+  bool total_backflow = false;  // BackflowPressure::Total at an outlet
 };
 
 // fv/steady_solver.hpp:193-271, the members this solver honours; the others
@@ -69,6 +73,11 @@ class SteadySolver {
                SolverSettings settings = {});
   ~SteadySolver();
   void iterate(int count);
+  // SteadySolver::initialize (fv/steady_solver.cpp:1480-1491): the velocity
+  // and pressure of every cell in the solver's cell order (that of
+  // geometry().cell_centroid); a closed domain's pressure is shifted so the
+  // reference cell holds zero.
+  void initialize(const std::vector<Vec3>& velocity, const std::vector<double>& pressure);
   IterationReport monitor();
   const IterationReport& report() const;
   // Pressure and viscous force of the fluid on the named wall patches

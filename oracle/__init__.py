@@ -448,7 +448,10 @@ class SteadySolver:
 
     REPORT_SIZE = 26
 
-    def __init__(self, nx, ny, nz, kinds, wall, settings):
+    def __init__(self, nx, ny, nz, kinds, wall, settings, pressure=None, total=None, extent=None):
+        """With `pressure` and `total` (kutta_amd.pack_open_conditions) the
+        patches may also be velocity inlets and pressure outlets, and `extent`
+        sets the box [0, extent] (the unit cube by default)."""
         L = lib()
         vp, i32 = C.c_void_p, C.c_int
         L.oracle_simplec_create.restype = vp
@@ -461,8 +464,19 @@ class SteadySolver:
         L.oracle_simplec_array.restype = C.c_long
         L.oracle_simplec_reference.argtypes = [vp]
         L.oracle_simplec_centroids.argtypes = [vp, vp]
-        self._keep = (kinds, wall, settings)
-        h = L.oracle_simplec_create(nx, ny, nz, C.cast(kinds, vp), C.cast(wall, vp), C.cast(settings, vp))
+        L.oracle_simplec_create_open.restype = vp
+        L.oracle_simplec_create_open.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
+        L.oracle_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
+        box = (C.c_double * 3)(*[float(e) for e in (extent or (1.0, 1.0, 1.0))])
+        self._keep = (kinds, wall, settings, pressure, total, box)
+        if pressure is None and extent is None:
+            h = L.oracle_simplec_create(nx, ny, nz, C.cast(kinds, vp), C.cast(wall, vp), C.cast(settings, vp))
+        else:
+            if pressure is None:
+                pressure, total = (C.c_double * 6)(), (C.c_int32 * 6)()
+                self._keep += (pressure, total)
+            h = L.oracle_simplec_create_open(nx, ny, nz, C.cast(box, vp), C.cast(kinds, vp), C.cast(wall, vp),
+                                             C.cast(pressure, vp), C.cast(total, vp), C.cast(settings, vp))
         if not h:
             raise OracleError(L.oracle_simplec_last_error().decode())
         self.handle = h
@@ -478,6 +492,14 @@ class SteadySolver:
             self.close()
         except Exception:
             pass
+
+    def initialize(self, u, v, w, p):
+        """Per-cell float64 arrays in the solver's cell order."""
+        arrays = [np.ascontiguousarray(a, np.float64) for a in (u, v, w, p)]
+        if any(a.shape != (self.cells,) for a in arrays):
+            raise OracleError("initialize needs one value per cell")
+        if lib().oracle_simplec_initialize(self.handle, *[a.ctypes.data for a in arrays]) != 0:
+            raise OracleError(lib().oracle_simplec_last_error().decode())
 
     def iterate(self, count=1):
         if lib().oracle_simplec_iterate(self.handle, count) != 0:

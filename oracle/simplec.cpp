@@ -93,8 +93,13 @@ struct LeastSquares {
 }  // namespace
 
 struct Condition {
-  int kind = 0;  // 0 wall, 3 symmetry (the boundary_kind codes of fv_flow.kern.h)
+  // The boundary_kind codes of fv_flow.kern.h: 0 wall (velocity), 1 velocity
+  // inlet (velocity), 2 pressure outlet with a backflow velocity (velocity),
+  // 3 symmetry, 4 pressure outlet with normal backflow; outlets hold `pressure`.
+  int kind = 0;
   Vec3 velocity{};
+  double pressure = 0;
+  int total = 0;  // BackflowPressure::Total at an outlet
 };
 
 struct Settings {
@@ -343,6 +348,9 @@ class Steady {
     ORACLE_REQUIRE(count >= 0, "FV negative iteration count");
     while (count-- > 0) step();
   }
+  // SteadySolver::initialize (:1480-1491) from per-cell values in the
+  // solver's cell order.
+  void initialize(const std::vector<Vec3>& u, const std::vector<double>& p);
   const Report& report() const { return report_; }
   ForceReport forces(const std::vector<std::string>& patches, double reference_pressure);
   // Named state arrays, for parity checks.
@@ -419,14 +427,21 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
                  "FV invalid steady solver settings");
   ORACLE_REQUIRE(conditions.size() == host.patches.size(), "FV supply exactly one condition per patch");
   std::vector<bool> patch_is_wall;
+  bool outlet = false;
   for (const auto& c : conditions) {
-    ORACLE_REQUIRE(c.kind == 0 || c.kind == 3, "SIMPLEC oracle: only wall and symmetry patches");
+    ORACLE_REQUIRE(c.kind >= 0 && c.kind <= 4, "SIMPLEC oracle: walls, velocity inlets, pressure outlets, symmetry");
+    ORACLE_REQUIRE(std::isfinite(c.velocity.x) && std::isfinite(c.velocity.y) && std::isfinite(c.velocity.z),
+                   "FV nonfinite boundary velocity");
+    ORACLE_REQUIRE(std::isfinite(c.pressure), "FV nonfinite outlet pressure");
     patch_is_wall.push_back(c.kind == 0);
+    outlet = outlet || c.kind == 2 || c.kind == 4;
   }
   renumber_cells_rcm(host);
   geom = compute_geometry(host);
   ORACLE_REQUIRE(host.n_cells > 0, "FV steady solver needs a nonempty mesh");
-  reference_ = locate_cell(host, geom, s.reference_point);
+  // :229-247: a pressure boundary sets the pressure level; a closed domain
+  // pins the cell holding the reference point.
+  reference_ = outlet ? -1 : locate_cell(host, geom, s.reference_point);
   {
     const auto nb = build_cell_neighbours(host);
     std::vector<bool> seen(host.n_cells);
@@ -488,18 +503,22 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
   // Boundary kinds and prescribed values (:314-371).
   std::vector<Index> kinds(nb);
   std::vector<Vec3> velocities(nb);
+  double pmin = std::numeric_limits<double>::infinity(), pmax = -pmin;
   for (std::size_t p = 0; p < host.patches.size(); ++p)
     for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
       const auto b = f - mesh.n_internal;
-      if (conditions[p].kind == 0) {
-        kinds[b] = 0;
-        velocities[b] = conditions[p].velocity;
-      } else {
-        kinds[b] = 3;
+      const auto& c = conditions[p];
+      kinds[b] = c.kind;
+      if (c.kind == 0 || c.kind == 1 || c.kind == 2) velocities[b] = c.velocity;
+      if (c.kind == 2 || c.kind == 4) {
+        prescribed_pressure[b] = float(c.pressure);
+        total_pressure[b] = c.total;
+        pmin = std::min(pmin, c.pressure);
+        pmax = std::max(pmax, c.pressure);
       }
       bc_speed = std::max(bc_speed, norm(velocities[b]));
     }
-  pressure_speed = 0;  // closed domain
+  pressure_speed = outlet ? .1 * std::sqrt((pmax - pmin) / settings.density) : 0;
   std::vector<Observation> independent_velocity(nb), independent_pressure(nb);
   for (std::size_t b = 0; b < nb; ++b) {
     independent_velocity[b] = kinds[b] == 3 ? Observation::NormalFootValue
@@ -605,6 +624,30 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
   std::fill(momentum[0].consistent_inverse.begin(), momentum[0].consistent_inverse.end(), 0.0f);
   update_boundaries();
   update_pressure();
+  update_flux();
+}
+
+void Steady::initialize(const std::vector<Vec3>& u, const std::vector<double>& p) {
+  const auto n = std::size_t(mesh.n_cells);
+  ORACLE_REQUIRE(u.size() == n && p.size() == n, "FV initialization needs one value per cell");
+  if (paired) {  // reset_paired (:977-988)
+    for (auto* v : {&velocity_lo.x, &velocity_lo.y, &velocity_lo.z, &pressure_gradient_lo.x,
+                    &pressure_gradient_lo.y, &pressure_gradient_lo.z, &pressure_lo, &mass_lo})
+      std::fill(v->begin(), v->end(), 0.0f);
+  }
+  const double ref = reference_ >= 0 ? p[std::size_t(reference_)] : 0;
+  for (std::size_t c = 0; c < n; ++c) {
+    const double value = p[c] - ref;
+    ORACLE_REQUIRE(std::isfinite(value), "FV nonfinite pressure initialization");
+    velocity.x[c] = float(u[c].x);
+    velocity.y[c] = float(u[c].y);
+    velocity.z[c] = float(u[c].z);
+    pressure[c] = float(value);
+  }
+  update_boundaries();
+  update_pressure();
+  std::fill(physical_diagonal.begin(), physical_diagonal.end(), 1.0f);
+  std::fill(consistent_inverse.begin(), consistent_inverse.end(), 0.0f);
   update_flux();
 }
 
@@ -1066,6 +1109,49 @@ void* oracle_simplec_create(int nx, int ny, int nz, const int* kind, const doubl
   } catch (const std::exception& e) {
     g_simplec_error = e.what();
     return nullptr;
+  }
+}
+// As oracle_simplec_create on [0, extent] with inlets and outlets: kind[p] 0 wall, 1
+// velocity inlet, 2 pressure outlet with backflow velocity, 3 symmetry,
+// 4 pressure outlet with normal backflow; velocity[3p..3p+2] for kinds 0-2,
+// pressure[p] and total[p] (backflow total pressure) for outlets.
+void* oracle_simplec_create_open(int nx, int ny, int nz, const double* extent, const int* kind,
+                                 const double* velocity, const double* pressure, const int* total,
+                                 const double* settings) {
+  try {
+    using namespace oracle;
+    using namespace oracle::simplec;
+    auto mesh = make_box(nx, ny, nz, {0, 0, 0}, {extent[0], extent[1], extent[2]}, false);
+    std::vector<Condition> conditions(mesh.patches.size());
+    for (std::size_t p = 0; p < conditions.size(); ++p)
+      conditions[p] = {kind[p], {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]}, pressure[p], total[p]};
+    Settings s;
+    s.density = settings[0];
+    s.viscosity = settings[1];
+    s.momentum_relaxation = settings[2];
+    s.pressure_relaxation = settings[3];
+    s.pseudo_time_scale = settings[4];
+    s.monitor_interval = int(settings[5]);
+    s.paired_state = settings[6] != 0;
+    s.paired_state_slack_ulps = settings[7];
+    s.reference_point = {settings[8], settings[9], settings[10]};
+    return new Steady(std::move(mesh), conditions, s);
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return nullptr;
+  }
+}
+int oracle_simplec_initialize(void* h, const double* u, const double* v, const double* w, const double* p) {
+  try {
+    auto& s = *static_cast<oracle::simplec::Steady*>(h);
+    const auto n = std::size_t(s.cells());
+    std::vector<oracle::Vec3> velocity(n);
+    for (std::size_t c = 0; c < n; ++c) velocity[c] = {u[c], v[c], w[c]};
+    s.initialize(velocity, std::vector<double>(p, p + n));
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
   }
 }
 void oracle_simplec_free(void* h) { delete static_cast<oracle::simplec::Steady*>(h); }
