@@ -190,6 +190,20 @@ int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* exte
  * ka_simplec_centroids); a closed domain's pressure is shifted so that the
  * reference cell holds zero. */
 int ka_simplec_initialize(void* solver, const double* u, const double* v, const double* w, const double* p);
+/* SstModel::update (turbulence/sst.cpp:101-158; SstOptions and SstInputs,
+ * turbulence/sst.hpp:46-124; no intermittency coupling) on the solver's mesh
+ * with its wall patches as walls. options[19]: has_ambient, ambient k,
+ * ambient omega, production_limiter, production_clip,
+ * limit_omega_production, omega_production_molecular_ratio, low_re,
+ * gamma_set == Menter2003, viscosity_limiter == Vorticity, kato_launder,
+ * vorticity_production, wall_production, cdkw_floor, limit_cross_diffusion,
+ * cross_diffusion_clip, omega_floor, k_floor, max_viscosity_ratio.
+ * inputs[25]: device pointers to velocity x, y, z; grad u (d/dx, d/dy,
+ * d/dz), grad v, grad w; density; viscosity; k; omega; grad k (x, y, z);
+ * grad omega (x, y, z) per cell; wall velocity x, y, z per boundary face.
+ * The SstFields are then read with ka_simplec_field by name (mu_t, k_source,
+ * ..., wall_shear_x, fixed_omega_mask and selected_wall_face as int32 bits). */
+int ka_simplec_sst_update(void* solver, const double* options, const float* const* inputs);
 void ka_simplec_destroy(void* solver);
 /* SteadySolver::iterate(count): count SIMPLEC outer iterations. */
 int ka_simplec_iterate(void* solver, int count);

@@ -319,6 +319,11 @@ typedef enum kh_kernel {
   KH_BF16_SMOOTH_FROM_ZERO,      /* as KH_FUSED_SMOOTH_FROM_ZERO */
   KH_BF16_RESIDUAL_RESTRICT,     /* as KH_FUSED_RESIDUAL_RESTRICT */
   KH_BF16_PROLONG_SMOOTH,        /* as KH_FUSED_PROLONG_SMOOTH */
+  /* fv/device_mesh.cpp:18-64, 136-157, 232-245 (WallBvh::distance): n = cells;
+     cell centroids (3n doubles), wall triangles (9 doubles each: a, b, c),
+     triangle count, out (n floats) = float(sqrt(min squared FP64 distance)),
+     +inf without triangles. */
+  KH_WALL_DISTANCE,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 

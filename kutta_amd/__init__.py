@@ -135,6 +135,7 @@ def host_lib() -> C.CDLL:
         lib.ka_simplec_create_open.argtypes = [vp, i32, i32, i32, C.POINTER(dbl), pi32, C.POINTER(dbl),
                                                C.POINTER(dbl), pi32, C.POINTER(dbl), C.POINTER(vp)]
         lib.ka_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
+        lib.ka_simplec_sst_update.argtypes = [vp, C.POINTER(dbl), C.POINTER(vp)]
         lib.ka_simplec_destroy.argtypes = [vp]
         lib.ka_simplec_iterate.argtypes = [vp, i32]
         lib.ka_simplec_report.argtypes = [vp, i32, C.POINTER(dbl)]
@@ -1037,6 +1038,49 @@ def pack_conditions(conditions: dict):
     return kinds, wall
 
 
+@dataclass
+class SstOptions:
+    """turbulence/sst.hpp:46-78 (SstOptions; no intermittency coupling)."""
+    ambient: tuple | None = None  # (k, omega)
+    production_limiter: bool = True
+    production_clip: float = 10.0
+    limit_omega_production: bool = True
+    omega_production_molecular_ratio: float = 0.0
+    low_re: bool = False
+    gamma_menter2003: bool = False
+    vorticity_limiter: bool = False
+    kato_launder: bool = False
+    vorticity_production: bool = False
+    wall_production: bool = True
+    cdkw_floor: float = 1e-10
+    limit_cross_diffusion: bool = True
+    cross_diffusion_clip: float = 1e6
+    omega_floor: float = 1e-20
+    k_floor: float = 1e-14
+    max_viscosity_ratio: float = 1e5
+
+    def packed(self):
+        a = self.ambient
+        return (C.c_double * 19)(1.0 if a else 0.0, a[0] if a else 0.0, a[1] if a else 0.0,
+                                 float(self.production_limiter), self.production_clip,
+                                 float(self.limit_omega_production), self.omega_production_molecular_ratio,
+                                 float(self.low_re), float(self.gamma_menter2003), float(self.vorticity_limiter),
+                                 float(self.kato_launder), float(self.vorticity_production),
+                                 float(self.wall_production), self.cdkw_floor, float(self.limit_cross_diffusion),
+                                 self.cross_diffusion_clip, self.omega_floor, self.k_floor, self.max_viscosity_ratio)
+
+
+SST_INPUTS = ("u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz", "density",
+              "viscosity", "k", "omega", "dkdx", "dkdy", "dkdz", "domegadx", "domegady", "domegadz",
+              "wall_u", "wall_v", "wall_w")
+SST_FIELDS = ("mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity", "f1", "f2", "cdkw",
+              "diffusivity_k", "diffusivity_omega", "production_k", "fixed_omega", "wall_production",
+              "wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau", "wall_shear_x",
+              "wall_shear_y", "wall_shear_z", "wall_shear_coefficient", "wall_face_omega",
+              "face_diffusivity_k", "face_diffusivity_omega")
+SST_INT_FIELDS = ("fixed_omega_mask", "selected_wall_face")
+
+
 def pack_open_conditions(conditions: dict):
     """As pack_conditions with inlets and outlets: (kinds, velocity, pressure,
     total) in the boundary_kind codes 0 wall, 1 velocity inlet, 2 outlet with
@@ -1175,6 +1219,15 @@ class SteadySolver:
         out = (C.c_int32 * 4)()
         _check_host(host_lib().ka_simplec_info(self.handle, out))
         return {"cells": out[0], "reference": out[1], "host_waits": out[2], "iteration": out[3]}
+
+    def sst_update(self, options: SstOptions, inputs) -> None:
+        """SstModel::update (turbulence/sst.cpp:101-158) on this mesh; `inputs`
+        are 25 device Buffers in SST_INPUTS order; read the results with
+        field(name) for SST_FIELDS and SST_INT_FIELDS (int32 bits)."""
+        if len(inputs) != len(SST_INPUTS):
+            raise KuttaError(f"SST: {len(SST_INPUTS)} input arrays expected")
+        ptrs = (C.c_void_p * len(inputs))(*[b.ptr for b in inputs])
+        _check_host(host_lib().ka_simplec_sst_update(self.handle, options.packed(), ptrs))
 
     def set_graphs(self, on: bool) -> None:
         """HIP graph replay of the pressure PCG iterations (default on)."""

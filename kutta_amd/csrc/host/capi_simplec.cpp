@@ -72,6 +72,9 @@ int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* exte
                            const double* settings, void** out) {
   return create(dev, nx, ny, nz, extent, kinds, velocity, pressure, total, settings, out);
 }
+int ka_simplec_sst_update(void* solver, const double* options, const float* const* inputs) {
+  return guard([&] { handle(solver).solver->sst_update(options, inputs); });
+}
 int ka_simplec_initialize(void* solver, const double* u, const double* v, const double* w, const double* p) {
   return guard([&] {
     auto& s = *handle(solver).solver;

@@ -26,6 +26,7 @@ struct Vec3F {
 };
 
 double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Vec3 add(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 Vec3 sub(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 Vec3 scale(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 double length_of(Vec3 a) { return std::sqrt(dot(a, a)); }
@@ -199,7 +200,7 @@ struct SteadySolver::Impl {
 
   // fv/device_mesh.hpp:36-67 (wall distance is a turbulence-model input).
   I owner, neighbour, offsets, faces;
-  F sign, weight, volume, wall_normal_distance;
+  F sign, weight, volume, wall_normal_distance, wall_distance;
   Vec3F area, owner_to_face, neighbour_to_face, owner_to_neighbour;
   // fv/operators.hpp:16-25
   I adjacent, matrix_entry;
@@ -214,6 +215,8 @@ struct SteadySolver::Impl {
   Vec3F limited_gradient[3], grad_p, grad_correction, absolute_residual, fluent_scale;
   F pressure, mass, viscosity, boundary_pressure, prescribed_pressure;
   I boundary_kind, scalar_kind, total_pressure;
+  std::map<std::string, F> sst_fields;
+  I sst_wall;
   F pressure_rhs, correction, correction_boundary, coefficient, face_pressure, pressure_jump;
   F residual_u, residual_v, residual_w, continuity, provisional_continuity, scale_cells, speed;
   F wall_diagonal, wall_rhs, physical_diagonal, consistent_inverse;
@@ -580,6 +583,39 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
         for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f)
           wd[f - ni] = float(std::abs(dot(rp(f), geom.face_area[f])) / length_of(geom.face_area[f]));
     wall_normal_distance = to_device(dev, wd);
+    // fv/device_mesh.cpp:232-245: every cell's distance to the nearest wall
+    // triangle (each wall face fanned about its node average), on the device.
+    std::vector<double> tri, cent(3 * std::size_t(n));
+    for (std::size_t p = 0; p < host.patches.size(); ++p)
+      if (patch_condition[p].kind == PatchCondition::Wall)
+        for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+          const Index b0 = host.face_offsets[f], s = host.face_offsets[f + 1] - b0;
+          const Index* ns = host.face_nodes.data() + b0;
+          const Vec3 origin = host.points[ns[0]];
+          Vec3 d{};
+          for (Index k = 0; k < s; ++k) d = add(d, sub(host.points[ns[k]], origin));
+          const Vec3 center = add(origin, scale(d, 1 / double(s)));
+          for (Index k = 0; k < s; ++k)
+            for (const Vec3& v : {center, host.points[ns[k]], host.points[ns[(k + 1) % s]]}) {
+              tri.push_back(v.x);
+              tri.push_back(v.y);
+              tri.push_back(v.z);
+            }
+        }
+    for (Index c = 0; c < n; ++c) {
+      cent[3 * std::size_t(c)] = geom.cell_centroid[c].x;
+      cent[3 * std::size_t(c) + 1] = geom.cell_centroid[c].y;
+      cent[3 * std::size_t(c) + 2] = geom.cell_centroid[c].z;
+    }
+    const auto count = tri.size() / 9;
+    KA_REQUIRE(count < std::size_t(1) << 31, "FV: too many wall triangles");
+    if (tri.empty()) tri.assign(9, 0.0);
+    DeviceArray<double> tri_dev(dev, tri.size()), cent_dev(dev, cent.size());
+    tri_dev.upload(tri);
+    cent_dev.upload(cent);
+    wall_distance = F(dev, n);
+    dev.launch(KH_WALL_DISTANCE, n, {cent_dev, tri_dev, int(count), wall_distance});
+    dev.synchronize();
   }
   adjacent = to_device(dev, stencil.adjacent);
   matrix_entry = to_device(dev, stencil.matrix_entry);
@@ -969,6 +1005,67 @@ void SteadySolver::initialize(const std::vector<Vec3>& u, const std::vector<doub
   s.dev.fill(s.consistent_inverse.data(), 0.0f, n);
   s.update_flux();
 }
+void SteadySolver::sst_update(const double* o, const float* const* in) {
+  auto& s = *impl_;
+  KA_REQUIRE(std::isfinite(o[4]) && o[4] > 0 && o[13] > 0 && o[15] > 0 && o[16] > 0 && o[17] > 0 && o[18] > 0 &&
+                 std::isfinite(o[6]) && o[6] >= 0 && !(o[10] != 0 && o[11] != 0),
+             "SST: invalid options");
+  for (int k = 0; k < 25; ++k) KA_REQUIRE(in[k] != nullptr, "SST: missing input array");
+  const Index n = s.n, nf = Index(s.area.x.size()), nb = s.nb, ni = s.ni;
+  if (s.sst_fields.empty()) {  // SstModel constructor (:60-99)
+    std::vector<Index> wall(nb, 0);
+    for (std::size_t p = 0; p < s.host.patches.size(); ++p)
+      if (s.patch_condition[p].kind == PatchCondition::Wall)
+        for (Index f = s.host.patches[p].start; f < s.host.patches[p].start + s.host.patches[p].size; ++f)
+          wall[f - ni] = 1;
+    s.sst_wall = to_device(s.dev, wall);
+    for (const char* name : {"mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity",
+                             "f1", "f2", "cdkw", "diffusivity_k", "diffusivity_omega", "production_k",
+                             "fixed_omega", "wall_production", "fixed_omega_mask", "selected_wall_face"})
+      s.sst_fields[name] = F(s.dev, n);
+    for (const char* name : {"wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau",
+                             "wall_shear_x", "wall_shear_y", "wall_shear_z", "wall_shear_coefficient",
+                             "wall_face_omega", "wall_cell_omega", "face_production"})
+      s.sst_fields[name] = F(s.dev, nb);
+    s.sst_fields["face_diffusivity_k"] = F(s.dev, nf);
+    s.sst_fields["face_diffusivity_omega"] = F(s.dev, nf);
+  }
+  auto d = [&](const char* name) { return s.sst_fields.at(name).data(); };
+  const float* const* g = in + 3;
+  const float *rho = in[12], *mu = in[13], *k = in[14], *omega = in[15];
+  const float* const* gk = in + 16;
+  const float* const* go = in + 19;
+  const float* const* wu = in + 22;
+  const float k_floor = float(o[17]), omega_floor = float(o[16]);
+  auto& dev = s.dev;
+  dev.launch(KH_SST_WALL, nb,
+             {s.owner, s.sst_wall, ni, s.area.x, s.area.y, s.area.z, in[0], in[1], in[2], wu[0], wu[1], wu[2],
+              s.wall_normal_distance, rho, mu, k, omega, d("wall_tau"), d("wall_shear_x"), d("wall_shear_y"),
+              d("wall_shear_z"), d("wall_u_star"), d("wall_u_tau"), d("wall_y_plus"), d("wall_y_plus_utau"),
+              d("face_production"), d("wall_shear_coefficient"), k_floor, omega_floor});
+  dev.launch(KH_SST_WALL_OMEGA, nb,
+             {s.owner, s.sst_wall, ni, s.wall_normal_distance, rho, mu, d("wall_u_star"), d("wall_cell_omega"),
+              d("wall_face_omega")});
+  dev.launch(KH_SST_WALL_CELLS, n,
+             {s.offsets, s.faces, s.sst_wall, s.wall_normal_distance, d("wall_cell_omega"), d("face_production"), ni,
+              d("fixed_omega_mask"), d("selected_wall_face"), d("fixed_omega"), d("wall_production")});
+  dev.launch(KH_SST_RATES, n, {g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], d("strain"), d("vorticity")});
+  dev.launch(KH_SST_BLENDING_VISCOSITY, n,
+             {rho, mu, k, omega, s.wall_distance, gk[0], gk[1], gk[2], go[0], go[1], go[2], d("strain"),
+              d("vorticity"), d("f1"), d("f2"), d("cdkw"), d("mu_t"), d("diffusivity_k"), d("diffusivity_omega"),
+              float(o[13]), k_floor, omega_floor, int(o[7] != 0), int(o[9] != 0), float(o[18])});
+  const int production_mode = o[10] != 0 ? 1 : (o[11] != 0 ? 2 : 0);
+  dev.launch(KH_SST_SOURCES, n,
+             {rho, mu, k, omega, d("strain"), d("vorticity"), d("f1"), d("cdkw"), d("mu_t"), d("fixed_omega_mask"),
+              d("wall_production"), d("k_source"), d("k_sink"), d("omega_source"), d("omega_sink"),
+              d("production_k"), int(o[7] != 0), int(o[8] != 0), production_mode, int(o[3] != 0), int(o[5] != 0),
+              int(o[12] != 0), float(o[4]), int(o[14] != 0), float(o[15]), k_floor, omega_floor, float(o[6])});
+  if (o[0] != 0)
+    dev.launch(KH_SST_SUSTAIN, n, {rho, d("f1"), d("k_source"), d("omega_source"), float(o[1]), float(o[2])});
+  dev.launch(KH_SST_FACE_DIFFUSION, nf,
+             {s.owner, s.neighbour, s.weight, s.sst_wall, mu, d("diffusivity_k"), d("diffusivity_omega"), ni,
+              d("face_diffusivity_k"), d("face_diffusivity_omega")});
+}
 IterationReport SteadySolver::monitor() { return impl_->monitor(); }
 ForceReport SteadySolver::forces(const std::vector<std::string>& patches, double reference_pressure) {
   return impl_->forces(patches, reference_pressure);
@@ -998,13 +1095,19 @@ const float* SteadySolver::field(const std::string& name, std::size_t* size) con
       {"pressure_rhs", &s.pressure_rhs}, {"volume", &s.volume}, {"weight", &s.weight},
       {"owner_to_face_x", &s.owner_to_face.x}, {"neighbour_to_face_y", &s.neighbour_to_face.y},
       {"owner_to_neighbour_z", &s.owner_to_neighbour.z}, {"area_x", &s.area.x},
-      {"wall_normal_distance", &s.wall_normal_distance}, {"velocity_lsq_inverse", &s.velocity_lsq.inverse},
+      {"wall_normal_distance", &s.wall_normal_distance}, {"wall_distance", &s.wall_distance},
+      {"velocity_lsq_inverse", &s.velocity_lsq.inverse},
       {"pressure_lsq_inverse", &s.pressure_lsq.inverse}, {"velocity_lsq_boundary_x", &s.velocity_lsq.bx},
       {"pressure_lsq_boundary_y", &s.pressure_lsq.by}};
   const auto it = table.find(name);
-  if (it == table.end() || it->second->empty()) return nullptr;
-  if (size) *size = it->second->size();
-  return it->second->data();
+  const F* array = it != table.end() ? it->second : nullptr;
+  if (!array) {
+    const auto sst = s.sst_fields.find(name);
+    if (sst != s.sst_fields.end()) array = &sst->second;
+  }
+  if (!array || array->empty()) return nullptr;
+  if (size) *size = array->size();
+  return array->data();
 }
 
 }  // namespace kutta_amd

@@ -91,6 +91,19 @@ class SteadySolver {
   Index cells() const;
   // Host waits of the linear solvers so far (pressure, momentum).
   int host_waits() const;
+  // SstModel::update (turbulence/sst.cpp:101-158, no intermittency) on this
+  // solver's mesh, walls as its wall patches. options[19]: has_ambient,
+  // ambient k, ambient omega, production_limiter, production_clip,
+  // limit_omega_production, omega_production_molecular_ratio, low_re,
+  // gamma_set Menter2003, viscosity limiter Vorticity, kato_launder,
+  // vorticity_production, wall_production, cdkw_floor,
+  // limit_cross_diffusion, cross_diffusion_clip, omega_floor, k_floor,
+  // max_viscosity_ratio (SstOptions, turbulence/sst.hpp:46-78). inputs[25]:
+  // device arrays velocity x y z, grad u (x y z), grad v, grad w, density,
+  // viscosity, k, omega, grad k (x y z), grad omega (x y z) per cell, then
+  // wall velocity x y z per boundary face (SstInputs). The fields
+  // (SstFields) are read by field(name); the integer ones hold int32 bits.
+  void sst_update(const double* options, const float* const* inputs);
   // Batched momentum solve on or off; returns whether it is in use (it
   // stays off where it would change the iterates: symmetry faces that are not
   // axis-aligned).

@@ -1836,6 +1836,80 @@ const char* signature_of() {
 #define KCF(k) static_cast<const float*>(buf(a, k))
 #define GRID(n) dim3(blocks_for(n)), dim3(kBlock), 0, d->stream
 
+// Exact FP64 distance from each cell centroid to the nearest wall triangle
+// (fv/device_mesh.cpp:18-64, 136-157). The reference searches a BVH, which
+// only skips triangles that cannot be nearer, so the minimum over every
+// triangle is the same number. Here each thread owns one cell and the
+// workgroup streams the triangles through LDS in tiles; the arithmetic is the
+// reference's, operation for operation, without contraction.
+struct WallPoint {
+  double x, y, z;
+};
+__device__ inline WallPoint wp_sub(WallPoint a, WallPoint b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline WallPoint wp_add(WallPoint a, WallPoint b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline WallPoint wp_scale(WallPoint a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ inline double wp_dot(WallPoint a, WallPoint b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline WallPoint wp_cross(WallPoint a, WallPoint b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ inline double wp_segment2(WallPoint p, WallPoint a, WallPoint b) {
+  const WallPoint ab = wp_sub(b, a);
+  const double length2 = wp_dot(ab, ab);
+  double t = 0;
+  if (length2 != 0) {
+    t = wp_dot(wp_sub(p, a), ab) / length2;
+    t = t < 0.0 ? 0.0 : (1.0 < t ? 1.0 : t);
+  }
+  const WallPoint r = wp_sub(p, wp_add(a, wp_scale(ab, t)));
+  return wp_dot(r, r);
+}
+__device__ inline double wp_triangle2(WallPoint p, WallPoint a, WallPoint b, WallPoint c) {
+  const WallPoint ab = wp_sub(b, a), ac = wp_sub(c, a), ap = wp_sub(p, a);
+  const WallPoint nrm = wp_cross(ab, ac);
+  const double nn = wp_dot(nrm, nrm);
+  if (nn > 0) {
+    const double h = wp_dot(ap, nrm);
+    const WallPoint q = wp_sub(p, wp_scale(nrm, h / nn));
+    if (wp_dot(wp_cross(ab, wp_sub(q, a)), nrm) >= 0 && wp_dot(wp_cross(wp_sub(c, b), wp_sub(q, b)), nrm) >= 0 &&
+        wp_dot(wp_cross(wp_sub(a, c), wp_sub(q, c)), nrm) >= 0)
+      return h * h / nn;
+  }
+  double best = wp_segment2(p, a, b);
+  const double e1 = wp_segment2(p, b, c), e2 = wp_segment2(p, c, a);
+  if (e1 < best) best = e1;
+  if (e2 < best) best = e2;
+  return best;
+}
+constexpr int kWallTile = 128;  // triangles per LDS tile (9 KiB)
+__global__ void __launch_bounds__(kBlock) k_wall_distance(int n, const double* __restrict__ centroid,
+                                                          const double* __restrict__ triangle, int count,
+                                                          float* __restrict__ out) {
+  __shared__ double tile[kWallTile * 9];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  WallPoint p{0, 0, 0};
+  if (i < n) p = {centroid[3 * std::size_t(i)], centroid[3 * std::size_t(i) + 1], centroid[3 * std::size_t(i) + 2]};
+  double best = __builtin_inf();
+  for (int base = 0; base < count; base += kWallTile) {
+    const int m = min(kWallTile, count - base);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 9 * m; k += kBlock) tile[k] = triangle[9 * std::size_t(base) + k];
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+      const double* t = tile + 9 * j;
+      const double d2 = wp_triangle2(p, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]});
+      if (d2 < best) best = d2;
+    }
+  }
+  if (i < n) out[i] = float(sqrt(best));
+}
+// n = cells; centroids, triangles, triangle count, out
+void l_wall_distance(kh_device* d, int n, const kh_arg* a) {
+  const int count = i32(a, 2);
+  if (count < 0) throw std::runtime_error("wall_distance: negative triangle count");
+  k_wall_distance<<<GRID(n)>>>(n, static_cast<const double*>(buf(a, 0)), static_cast<const double*>(buf(a, 1)),
+                               count, static_cast<float*>(buf(a, 3)));
+}
+
 void l_fv_pressure_assemble(kh_device* d, int n, const kh_arg* a) {
   k_fv_pressure_assemble<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KB(3), KB(4), KB(5), KB(6), KCF(7),
                                       KCF(8), KF(9), KF(10), i32(a, 11));
@@ -1846,6 +1920,7 @@ void l_fv_pressure_assemble(kh_device* d, int n, const kh_arg* a) {
 const std::vector<KernelEntry>& fv_kernels() {
   static const std::vector<KernelEntry> table = {
       {KH_FV_PRESSURE_ASSEMBLE, "fv_pressure_assemble", "bbbbbbbbbbbi", l_fv_pressure_assemble},
+      {KH_WALL_DISTANCE, "wall_distance", "bbib", l_wall_distance},
 #define FV(id, name, Body) {id, name, signature_of<Body>(), launch_body<Body>, true}
       FV(KH_FV_VELOCITY_BOUNDARY, "fv_velocity_boundary", FvVelocityBoundary),
       FV(KH_FV_PRESSURE_BOUNDARY, "fv_pressure_boundary", FvPressureBoundary),

@@ -501,6 +501,16 @@ class SteadySolver:
         if lib().oracle_simplec_initialize(self.handle, *[a.ctypes.data for a in arrays]) != 0:
             raise OracleError(lib().oracle_simplec_last_error().decode())
 
+    def sst_update(self, options, arrays):
+        """SstModel::update on this mesh: options[19] packed as for the
+        product, arrays the 25 float32 inputs (SST_INPUTS order)."""
+        L = lib()
+        L.oracle_simplec_sst_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        keep = [np.ascontiguousarray(a, np.float32) for a in arrays]
+        ptrs = (C.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
+        if L.oracle_simplec_sst_update(self.handle, C.cast(options, C.c_void_p), ptrs) != 0:
+            raise OracleError(L.oracle_simplec_last_error().decode())
+
     def iterate(self, count=1):
         if lib().oracle_simplec_iterate(self.handle, count) != 0:
             raise OracleError(lib().oracle_simplec_last_error().decode())

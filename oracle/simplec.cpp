@@ -47,6 +47,8 @@ struct Vec3f {
 double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 Vec3 operator-(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 Vec3 operator*(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+Vec3 operator+(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+Vec3 cross(Vec3 a, Vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 double norm(Vec3 a) { return std::sqrt(dot(a, a)); }
 
 // One serial kernel launch in oracle_fv_each's convention: args[k] is the
@@ -73,14 +75,13 @@ void fv(const char* name, int n, T... a) {
 
 enum class Observation { Omitted, FaceValue, NormalFootValue };
 
-// fv/device_mesh.cpp:208-256 (wall distance, used only by turbulence models,
-// is not part of this restatement).
+// fv/device_mesh.cpp:208-256.
 struct DeviceMesh {
   int n_cells = 0, n_faces = 0, n_internal = 0;
   int n_boundary() const { return n_faces - n_internal; }
   VI owner, neighbour, offsets, faces;
   Vec3f area, owner_to_face, neighbour_to_face, owner_to_neighbour;
-  V weight, volume, sign, wall_normal_distance;
+  V weight, volume, sign, wall_normal_distance, wall_distance;
 };
 
 // fv/operators.hpp:33-50: rows of the fit (interior rows are the stencil's),
@@ -269,6 +270,31 @@ std::vector<double> orthogonal_quality(const PolyMesh& m, const Geometry& g) {
   return quality;
 }
 
+// fv/device_mesh.cpp:18-64: squared distance from p to a triangle (the
+// plane projection when it falls inside, else the nearest edge).
+double segment_distance2(Vec3 p, Vec3 a, Vec3 b) {
+  const Vec3 ab = b - a;
+  const double length2 = dot(ab, ab);
+  const double t = length2 == 0 ? 0 : std::clamp(dot(p - a, ab) / length2, 0., 1.);
+  const Vec3 r = p - (a + ab * t);
+  return dot(r, r);
+}
+struct WallTriangle {
+  Vec3 a, b, c;
+  double distance2(Vec3 p) const {
+    const Vec3 ab = b - a, ac = c - a, ap = p - a;
+    const Vec3 n = cross(ab, ac);
+    const double nn = dot(n, n);
+    if (nn > 0) {
+      const double h = dot(ap, n);
+      const Vec3 q = p - n * (h / nn);
+      if (dot(cross(ab, q - a), n) >= 0 && dot(cross(c - b, q - b), n) >= 0 && dot(cross(a - c, q - c), n) >= 0)
+        return h * h / nn;
+    }
+    return std::min({segment_distance2(p, a, b), segment_distance2(p, b, c), segment_distance2(p, c, a)});
+  }
+};
+
 // fv/device_mesh.cpp:208-256 (patch_is_wall marks the wall patches).
 DeviceMesh upload_mesh(const PolyMesh& mesh, const Geometry& g, const std::vector<bool>& patch_is_wall) {
   validate(mesh);
@@ -313,6 +339,27 @@ DeviceMesh upload_mesh(const PolyMesh& mesh, const Geometry& g, const std::vecto
       for (Index f = mesh.patches[p].start; f < mesh.patches[p].start + mesh.patches[p].size; ++f)
         m.wall_normal_distance[f - m.n_internal] =
             float(std::abs(dot(rp(std::size_t(f)), g.face_area[f])) / norm(g.face_area[f]));
+  // :232-245 with WallBvh (:82-157): the minimum over every wall triangle of
+  // the squared distance, which the BVH's pruning does not change.
+  std::vector<WallTriangle> triangles;
+  for (std::size_t p = 0; p < mesh.patches.size(); ++p)
+    if (patch_is_wall[p])
+      for (Index f = mesh.patches[p].start; f < mesh.patches[p].start + mesh.patches[p].size; ++f) {
+        const Index b = mesh.face_offsets[f], e = mesh.face_offsets[f + 1];
+        const Vec3 origin = mesh.points[mesh.face_nodes[b]];
+        Vec3 d{};
+        for (Index k = b; k < e; ++k) d = d + (mesh.points[mesh.face_nodes[k]] - origin);
+        const Vec3 center = origin + d * (1 / double(e - b));
+        for (Index k = b; k < e; ++k)
+          triangles.push_back({center, mesh.points[mesh.face_nodes[k]],
+                               mesh.points[mesh.face_nodes[k + 1 < e ? k + 1 : b]]});
+      }
+  m.wall_distance.resize(m.n_cells);
+  for (Index c = 0; c < m.n_cells; ++c) {
+    double best = std::numeric_limits<double>::infinity();
+    for (const auto& t : triangles) best = std::min(best, t.distance2(g.cell_centroid[c]));
+    m.wall_distance[c] = float(std::sqrt(best));
+  }
   return m;
 }
 
@@ -355,6 +402,11 @@ class Steady {
   ForceReport forces(const std::vector<std::string>& patches, double reference_pressure);
   // Named state arrays, for parity checks.
   const V* array(const std::string& name) const;
+  // SstModel::update (turbulence/sst.cpp:101-158, no intermittency) on this
+  // solver's mesh: options[19] as ka_simplec_sst_update, in[25] the input
+  // arrays. The fields are kept by name (integer fields bit-copied into
+  // float storage) for array().
+  void sst_update(const double* options, const float* const* in);
   Index reference() const { return reference_; }
   int cells() const { return mesh.n_cells; }
   const Geometry& geometry() const { return geom; }
@@ -394,6 +446,7 @@ class Steady {
   TransportEquation momentum[3];
   V face_correction_scratch, zero;
   Index reference_ = -1;
+  std::map<std::string, V> sst_fields_;
   bool paired = false;
 
   Vec3f velocity, face_velocity, body, source, boundary_velocity, prescribed_velocity;
@@ -1065,11 +1118,74 @@ const V* Steady::array(const std::string& name) const {
       {"volume", &mesh.volume}, {"weight", &mesh.weight}, {"owner_to_face_x", &mesh.owner_to_face.x},
       {"neighbour_to_face_y", &mesh.neighbour_to_face.y}, {"owner_to_neighbour_z", &mesh.owner_to_neighbour.z},
       {"area_x", &mesh.area.x}, {"wall_normal_distance", &mesh.wall_normal_distance},
+      {"wall_distance", &mesh.wall_distance},
       {"velocity_lsq_inverse", &velocity_lsq.inverse}, {"pressure_lsq_inverse", &pressure_lsq.inverse},
       {"velocity_lsq_boundary_x", &velocity_lsq.boundary_delta.x},
       {"pressure_lsq_boundary_y", &pressure_lsq.boundary_delta.y}};
   const auto it = table.find(name);
-  return it == table.end() ? nullptr : it->second;
+  if (it != table.end()) return it->second;
+  const auto sst = sst_fields_.find(name);
+  return sst == sst_fields_.end() ? nullptr : &sst->second;
+}
+
+void Steady::sst_update(const double* o, const float* const* in) {
+  const int n = mesh.n_cells, nf = mesh.n_faces, nb = mesh.n_boundary(), ni = mesh.n_internal;
+  ORACLE_REQUIRE(o[4] > 0 && o[13] > 0 && o[15] > 0 && o[16] > 0 && o[17] > 0 && o[18] > 0 && o[6] >= 0 &&
+                     !(o[10] != 0 && o[11] != 0),
+                 "SST: invalid options");
+  VI wall(nb, 0);
+  for (std::size_t p = 0; p < host.patches.size(); ++p)
+    if (conditions_[p].kind == 0)
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) wall[f - ni] = 1;
+  auto& F = sst_fields_;
+  for (const char* name : {"mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity", "f1",
+                           "f2", "cdkw", "diffusivity_k", "diffusivity_omega", "production_k", "fixed_omega",
+                           "wall_production"})
+    F[name].assign(n, 0.0f);
+  for (const char* name : {"wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau",
+                           "wall_shear_x", "wall_shear_y", "wall_shear_z", "wall_shear_coefficient",
+                           "wall_face_omega", "wall_cell_omega", "face_production"})
+    F[name].assign(nb, 0.0f);
+  F["face_diffusivity_k"].assign(nf, 0.0f);
+  F["face_diffusivity_omega"].assign(nf, 0.0f);
+  VI fixed_mask(n, 0), selected(n, 0);
+  const auto ux = in[0], uy = in[1], uz = in[2];
+  const float* const* g = in + 3;  // du/dx, du/dy, du/dz, dv/dx, ...
+  const float *rho = in[12], *mu = in[13], *k = in[14], *omega = in[15];
+  const float* const* gk = in + 16;
+  const float* const* go = in + 19;
+  const float* const* wu = in + 22;
+  const float k_floor = float(o[17]), omega_floor = float(o[16]);
+  auto d = [&](const char* name) { return F[name].data(); };
+  fv("sst_wall", nb, mesh.owner.data(), wall.data(), ni, mesh.area.x.data(), mesh.area.y.data(),
+     mesh.area.z.data(), ux, uy, uz, wu[0], wu[1], wu[2], mesh.wall_normal_distance.data(), rho, mu, k, omega,
+     d("wall_tau"), d("wall_shear_x"), d("wall_shear_y"), d("wall_shear_z"), d("wall_u_star"), d("wall_u_tau"),
+     d("wall_y_plus"), d("wall_y_plus_utau"), d("face_production"), d("wall_shear_coefficient"), k_floor,
+     omega_floor);
+  fv("sst_wall_omega", nb, mesh.owner.data(), wall.data(), ni, mesh.wall_normal_distance.data(), rho, mu,
+     d("wall_u_star"), d("wall_cell_omega"), d("wall_face_omega"));
+  fv("sst_wall_cells", n, mesh.offsets.data(), mesh.faces.data(), wall.data(), mesh.wall_normal_distance.data(),
+     d("wall_cell_omega"), d("face_production"), ni, fixed_mask.data(), selected.data(), d("fixed_omega"),
+     d("wall_production"));
+  fv("sst_rates", n, g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], d("strain"), d("vorticity"));
+  fv("sst_blending_viscosity", n, rho, mu, k, omega, mesh.wall_distance.data(), gk[0], gk[1], gk[2], go[0], go[1],
+     go[2], d("strain"), d("vorticity"), d("f1"), d("f2"), d("cdkw"), d("mu_t"), d("diffusivity_k"),
+     d("diffusivity_omega"), float(o[13]), k_floor, omega_floor, int(o[7] != 0), int(o[9] != 0), float(o[18]));
+  const int production_mode = o[10] != 0 ? 1 : (o[11] != 0 ? 2 : 0);
+  fv("sst_sources", n, rho, mu, k, omega, d("strain"), d("vorticity"), d("f1"), d("cdkw"), d("mu_t"),
+     fixed_mask.data(), d("wall_production"), d("k_source"), d("k_sink"), d("omega_source"), d("omega_sink"),
+     d("production_k"), int(o[7] != 0), int(o[8] != 0), production_mode, int(o[3] != 0), int(o[5] != 0),
+     int(o[12] != 0), float(o[4]), int(o[14] != 0), float(o[15]), k_floor, omega_floor, float(o[6]));
+  if (o[0] != 0)
+    fv("sst_sustain", n, rho, d("f1"), d("k_source"), d("omega_source"), float(o[1]), float(o[2]));
+  fv("sst_face_diffusion", nf, mesh.owner.data(), mesh.neighbour.data(), mesh.weight.data(), wall.data(), mu,
+     d("diffusivity_k"), d("diffusivity_omega"), ni, d("face_diffusivity_k"), d("face_diffusivity_omega"));
+  for (const auto& [name, v] : {std::pair<const char*, const VI*>{"fixed_omega_mask", &fixed_mask},
+                                {"selected_wall_face", &selected}}) {
+    auto& out = F[name];
+    out.resize(v->size());
+    std::memcpy(out.data(), v->data(), v->size() * sizeof(int));
+  }
 }
 
 }  // namespace oracle::simplec
@@ -1224,6 +1340,15 @@ int oracle_simplec_forces(void* h, const char* names, double reference_pressure,
     return 1;
   }
 }
+int oracle_simplec_sst_update(void* h, const double* options, const float* const* inputs) {
+  try {
+    static_cast<oracle::simplec::Steady*>(h)->sst_update(options, inputs);
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
+  }
+}
 int oracle_simplec_reference(void* h) { return static_cast<oracle::simplec::Steady*>(h)->reference(); }
 // upload_mesh's float array `name` for make_box_hex/tet(nx, ny, nz) after RCM,
 // patches marked Wall where wall[p] != 0; copies it into out when non-null
@@ -1245,7 +1370,8 @@ long oracle_upload_mesh(int nx, int ny, int nz, int tet, const int* wall, const 
         {"neighbour_to_face_y", &d.neighbour_to_face.y}, {"neighbour_to_face_z", &d.neighbour_to_face.z},
         {"owner_to_neighbour_x", &d.owner_to_neighbour.x}, {"owner_to_neighbour_y", &d.owner_to_neighbour.y},
         {"owner_to_neighbour_z", &d.owner_to_neighbour.z}, {"weight", &d.weight}, {"volume", &d.volume},
-        {"sign", &d.sign}, {"wall_normal_distance", &d.wall_normal_distance}};
+        {"sign", &d.sign}, {"wall_normal_distance", &d.wall_normal_distance},
+        {"wall_distance", &d.wall_distance}};
     const auto it = arrays.find(name);
     if (it == arrays.end()) return -1;
     if (out) std::memcpy(out, it->second->data(), it->second->size() * sizeof(float));

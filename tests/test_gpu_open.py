@@ -67,6 +67,7 @@ def test_uniform_through_flow_is_preserved_and_matches_the_oracle(device, throug
     print(f"[open] through-flow {through}", flush=True)
     gpu, cpu = make(device, (8, 4, 1), through_flow(through), SolverSettings(), (1.0, 1.0, 0.1))
     assert gpu.info()["reference"] == -1 and cpu.reference() == -1
+    assert np.all(np.isinf(gpu.field("wall_distance")))  # no wall patch
     initialize(gpu, cpu, (through, 0.0, 0.0), 0.0)
     for s in (gpu, cpu):
         s.iterate(5)
@@ -144,6 +145,9 @@ def test_developing_duct_matches_the_oracle(device, outlet):
         s.iterate(20)
     assert_same_state(gpu, cpu, True)
     assert_same_report(gpu, cpu)
+    wd = gpu.field("wall_distance")
+    assert np.array_equal(wd.view(np.uint32), cpu.field("wall_distance").view(np.uint32))
+    assert 0 < wd.min() and wd.max() <= 0.5  # the duct's half width
     u = gpu.field("u")
     # Equal cells across a unit section: their mean streamwise velocity is
     # the inlet's, while the core speeds up between the no-slip walls.

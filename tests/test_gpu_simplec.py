@@ -18,7 +18,7 @@ FIELDS = ("u", "v", "w", "p", "mass", "face_pressure", "boundary_pressure", "coe
 PAIRED_FIELDS = ("u_lo", "v_lo", "w_lo", "p_lo", "mass_lo")
 # upload_mesh's FP32 arrays and the least-squares fits (host setup, uploaded).
 SETUP_FIELDS = ("volume", "weight", "area_x", "owner_to_face_x", "neighbour_to_face_y", "owner_to_neighbour_z",
-                "wall_normal_distance", "velocity_lsq_inverse", "pressure_lsq_inverse",
+                "wall_normal_distance", "wall_distance", "velocity_lsq_inverse", "pressure_lsq_inverse",
                 "velocity_lsq_boundary_x", "pressure_lsq_boundary_y")
 
 

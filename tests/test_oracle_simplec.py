@@ -149,7 +149,7 @@ def test_cavity_65_re1000_reaches_the_reference_state():
 UPLOAD_ARRAYS = ("area_x", "area_y", "area_z", "owner_to_face_x", "owner_to_face_y", "owner_to_face_z",
                  "neighbour_to_face_x", "neighbour_to_face_y", "neighbour_to_face_z", "owner_to_neighbour_x",
                  "owner_to_neighbour_y", "owner_to_neighbour_z", "weight", "volume", "sign",
-                 "wall_normal_distance")
+                 "wall_normal_distance", "wall_distance")
 
 
 @pytest.mark.skipif(oracle.ref_devmesh() is None, reason="oracle/_ref not built (no reference tree)")
@@ -162,6 +162,34 @@ def test_upload_mesh_matches_reference_build(nx, ny, nz, tet):
         a = oracle.upload_mesh_array(nx, ny, nz, tet, wall, name)
         b = oracle.upload_mesh_array(nx, ny, nz, tet, wall, name, reference=True)
         assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_wall_distance_closed_form(tet):
+    """fv/device_mesh.cpp:232-245: in the unit box every wall patch is a
+    whole face of the cube, so a cell's distance to the walls is its
+    centroid's distance to the nearest wall plane; with no wall it is
+    infinite. A wrong triangle fan, a dropped edge case or a square root
+    left out fails this."""
+    nx, ny, nz = 5, 4, 3
+    wall = [1, 0, 0, 1, 1, 0]  # xmin, ymax, zmin
+    d = oracle.upload_mesh_array(nx, ny, nz, tet, wall, "wall_distance").astype(np.float64)
+    s = oracle.SteadySolver(nx, ny, nz, (C.c_int32 * 6)(0, 3, 3, 0, 0, 3), (C.c_double * 18)(),
+                            (C.c_double * 11)(1, .01, .7, 1, 1, 20, 1, 1, 0, 0, 0)) if not tet else None
+    if s is not None:  # the same distances inside the solver, in its cell order
+        assert np.array_equal(s.field("wall_distance"), d.astype(np.float32))
+    cells = nx * ny * nz * (6 if tet else 1)
+    assert d.shape == (cells,) and np.all(d > 0)
+    # The cell centroids come back in the solver's order from a symmetric setup.
+    t = oracle.SteadySolver(nx, ny, nz, (C.c_int32 * 6)(3, 3, 3, 3, 3, 3), (C.c_double * 18)(),
+                            (C.c_double * 11)(1, .01, .7, 1, 1, 20, 1, 1, 0, 0, 0)) if not tet else None
+    if t is not None:
+        x = t.centroids()
+        expect = np.minimum(np.minimum(x[:, 0], 1 - x[:, 1]), x[:, 2])
+        assert np.allclose(d, expect, rtol=1e-6, atol=0)
+        assert np.all(np.isinf(t.field("wall_distance")))
+    else:
+        assert d.max() < 1.0  # the reference build pins the tet values bit for bit
 
 
 def test_steady_cavity_walls_balance_the_lid():
