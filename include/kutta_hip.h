@@ -321,8 +321,9 @@ typedef enum kh_kernel {
   KH_BF16_PROLONG_SMOOTH,        /* as KH_FUSED_PROLONG_SMOOTH */
   /* fv/device_mesh.cpp:18-64, 136-157, 232-245 (WallBvh::distance): n = cells;
      cell centroids (3n doubles), wall triangles (9 doubles each: a, b, c),
-     triangle count, out (n floats) = float(sqrt(min squared FP64 distance)),
-     +inf without triangles. */
+     the bounding box of each group of 32 consecutive triangles (6 doubles:
+     lo x y z, hi x y z), triangle count, out (n floats) =
+     float(sqrt(min squared FP64 distance)), +inf without triangles. */
   KH_WALL_DISTANCE,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;

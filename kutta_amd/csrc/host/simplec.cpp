@@ -609,12 +609,27 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
     }
     const auto count = tri.size() / 9;
     KA_REQUIRE(count < std::size_t(1) << 31, "FV: too many wall triangles");
-    if (tri.empty()) tri.assign(9, 0.0);
-    DeviceArray<double> tri_dev(dev, tri.size()), cent_dev(dev, cent.size());
+    // Bounding box of each group of 32 consecutive triangles (the kernel's
+    // pruning unit; the reference's Bounds::extend over the vertices).
+    constexpr std::size_t kGroup = 32;
+    std::vector<double> box;
+    for (std::size_t g = 0; g < count; g += kGroup) {
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (std::size_t t = g; t < std::min(count, g + kGroup); ++t)
+        for (int v = 0; v < 3; ++v)
+          for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], tri[9 * t + 3 * v + c]);
+            hi[c] = std::max(hi[c], tri[9 * t + 3 * v + c]);
+          }
+      box.insert(box.end(), {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+    }
+    if (tri.empty()) tri.assign(9, 0.0), box.assign(6, 0.0);
+    DeviceArray<double> tri_dev(dev, tri.size()), box_dev(dev, box.size()), cent_dev(dev, cent.size());
     tri_dev.upload(tri);
+    box_dev.upload(box);
     cent_dev.upload(cent);
     wall_distance = F(dev, n);
-    dev.launch(KH_WALL_DISTANCE, n, {cent_dev, tri_dev, int(count), wall_distance});
+    dev.launch(KH_WALL_DISTANCE, n, {cent_dev, tri_dev, box_dev, int(count), wall_distance});
     dev.synchronize();
   }
   adjacent = to_device(dev, stencil.adjacent);

@@ -1837,11 +1837,16 @@ const char* signature_of() {
 #define GRID(n) dim3(blocks_for(n)), dim3(kBlock), 0, d->stream
 
 // Exact FP64 distance from each cell centroid to the nearest wall triangle
-// (fv/device_mesh.cpp:18-64, 136-157). The reference searches a BVH, which
-// only skips triangles that cannot be nearer, so the minimum over every
-// triangle is the same number. Here each thread owns one cell and the
-// workgroup streams the triangles through LDS in tiles; the arithmetic is the
-// reference's, operation for operation, without contraction.
+// (fv/device_mesh.cpp:18-157). The reference searches a BVH that skips a node
+// when the distance to its bounding box exceeds the best distance so far; the
+// minimum it finds is the minimum over every triangle. Here each thread owns
+// one cell. The triangles come in groups of 32 with a bounding box per group
+// (the same box test as the BVH's nodes). A thread first evaluates the group
+// whose box is nearest, to start from a short distance. The workgroup then
+// streams the triangles through LDS in tiles of four groups, skipping a tile
+// no thread needs and a group that no thread of a wavefront needs. The
+// arithmetic is the reference's, operation for operation, without
+// contraction.
 struct WallPoint {
   double x, y, z;
 };
@@ -1880,34 +1885,70 @@ __device__ inline double wp_triangle2(WallPoint p, WallPoint a, WallPoint b, Wal
   if (e2 < best) best = e2;
   return best;
 }
-constexpr int kWallTile = 128;  // triangles per LDS tile (9 KiB)
+__device__ inline double wp_box2(WallPoint p, const double* b) {
+  const double lx = b[0] - p.x, ly = b[1] - p.y, lz = b[2] - p.z;
+  const double hx = p.x - b[3], hy = p.y - b[4], hz = p.z - b[5];
+  const WallPoint d{fmax(fmax(lx, 0.0), hx), fmax(fmax(ly, 0.0), hy), fmax(fmax(lz, 0.0), hz)};
+  return wp_dot(d, d);
+}
+__device__ inline double wp_at(const double* t, WallPoint p) {
+  return wp_triangle2(p, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]});
+}
+constexpr int kWallGroup = 32;             // triangles per bounding box
+constexpr int kWallTile = 4 * kWallGroup;  // triangles per LDS tile (9 KiB)
 __global__ void __launch_bounds__(kBlock) k_wall_distance(int n, const double* __restrict__ centroid,
-                                                          const double* __restrict__ triangle, int count,
+                                                          const double* __restrict__ triangle,
+                                                          const double* __restrict__ bounds, int count,
                                                           float* __restrict__ out) {
   __shared__ double tile[kWallTile * 9];
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < n;
+  const int groups = (count + kWallGroup - 1) / kWallGroup;
   WallPoint p{0, 0, 0};
-  if (i < n) p = {centroid[3 * std::size_t(i)], centroid[3 * std::size_t(i) + 1], centroid[3 * std::size_t(i) + 2]};
+  if (live) p = {centroid[3 * std::size_t(i)], centroid[3 * std::size_t(i) + 1], centroid[3 * std::size_t(i) + 2]};
   double best = __builtin_inf();
-  for (int base = 0; base < count; base += kWallTile) {
-    const int m = min(kWallTile, count - base);
-    __syncthreads();
-    for (int k = threadIdx.x; k < 9 * m; k += kBlock) tile[k] = triangle[9 * std::size_t(base) + k];
-    __syncthreads();
-    for (int j = 0; j < m; ++j) {
-      const double* t = tile + 9 * j;
-      const double d2 = wp_triangle2(p, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]});
+  if (live && groups > 0) {
+    int nearest = 0;
+    double box = wp_box2(p, bounds);
+    for (int g = 1; g < groups; ++g) {
+      const double d2 = wp_box2(p, bounds + 6 * std::size_t(g));
+      if (d2 < box) box = d2, nearest = g;
+    }
+    const int end = min(count, (nearest + 1) * kWallGroup);
+    for (int j = nearest * kWallGroup; j < end; ++j) {
+      const double d2 = wp_at(triangle + 9 * std::size_t(j), p);
       if (d2 < best) best = d2;
     }
   }
-  if (i < n) out[i] = float(sqrt(best));
+  for (int base = 0; base < count; base += kWallTile) {
+    const int m = min(kWallTile, count - base);
+    const int g0 = base / kWallGroup, gm = (m + kWallGroup - 1) / kWallGroup;
+    int need = 0;
+    for (int g = 0; g < gm && live && !need; ++g) need = !(wp_box2(p, bounds + 6 * std::size_t(g0 + g)) > best);
+    if (!__syncthreads_or(need)) continue;  // no thread of the workgroup needs this tile
+    for (int k = threadIdx.x; k < 9 * m; k += kBlock) tile[k] = triangle[9 * std::size_t(base) + k];
+    __syncthreads();
+    for (int g = 0; g < gm; ++g) {
+      const bool want = live && !(wp_box2(p, bounds + 6 * std::size_t(g0 + g)) > best);
+      if (!__any(want)) continue;
+      if (want) {
+        const int end = min(m, (g + 1) * kWallGroup);
+        for (int j = g * kWallGroup; j < end; ++j) {
+          const double d2 = wp_at(tile + 9 * j, p);
+          if (d2 < best) best = d2;
+        }
+      }
+    }
+    __syncthreads();  // the tile is read before the next one is loaded
+  }
+  if (live) out[i] = float(sqrt(best));
 }
-// n = cells; centroids, triangles, triangle count, out
+// n = cells; centroids, triangles, group bounds, triangle count, out
 void l_wall_distance(kh_device* d, int n, const kh_arg* a) {
-  const int count = i32(a, 2);
+  const int count = i32(a, 3);
   if (count < 0) throw std::runtime_error("wall_distance: negative triangle count");
   k_wall_distance<<<GRID(n)>>>(n, static_cast<const double*>(buf(a, 0)), static_cast<const double*>(buf(a, 1)),
-                               count, static_cast<float*>(buf(a, 3)));
+                               static_cast<const double*>(buf(a, 2)), count, static_cast<float*>(buf(a, 4)));
 }
 
 void l_fv_pressure_assemble(kh_device* d, int n, const kh_arg* a) {
@@ -1920,7 +1961,7 @@ void l_fv_pressure_assemble(kh_device* d, int n, const kh_arg* a) {
 const std::vector<KernelEntry>& fv_kernels() {
   static const std::vector<KernelEntry> table = {
       {KH_FV_PRESSURE_ASSEMBLE, "fv_pressure_assemble", "bbbbbbbbbbbi", l_fv_pressure_assemble},
-      {KH_WALL_DISTANCE, "wall_distance", "bbib", l_wall_distance},
+      {KH_WALL_DISTANCE, "wall_distance", "bbbib", l_wall_distance},
 #define FV(id, name, Body) {id, name, signature_of<Body>(), launch_body<Body>, true}
       FV(KH_FV_VELOCITY_BOUNDARY, "fv_velocity_boundary", FvVelocityBoundary),
       FV(KH_FV_PRESSURE_BOUNDARY, "fv_pressure_boundary", FvPressureBoundary),
