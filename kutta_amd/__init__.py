@@ -365,6 +365,7 @@ class Kernel(enum.IntEnum):
     bf16_smooth_from_zero = 239
     bf16_residual_restrict = 240
     bf16_prolong_smooth = 241
+    wall_distance = 242
 
 
 class Buffer:
