@@ -185,6 +185,17 @@ int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, c
 int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
                            const double* velocity, const double* pressure, const int32_t* total,
                            const double* settings, void** out);
+/* As ka_simplec_create_open, with the velocity of each patch whose
+ * profiled[p] != 0 (a velocity inlet, or an outlet's backflow velocity) given
+ * by profile(p, face centroid[3], velocity[3], user) at every face of the
+ * patch (VelocityProfile and BackflowProfile as functions,
+ * fv/steady_solver.hpp:22, 44; evaluated as fv/steady_solver.cpp:26-29,
+ * 317, 337). */
+typedef void (*ka_boundary_profile)(int32_t patch, const double* point, double* velocity, void* user);
+int ka_simplec_create_profiled(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
+                               const double* velocity, const double* pressure, const int32_t* total,
+                               const int32_t* profiled, ka_boundary_profile profile, void* user,
+                               const double* settings, void** out);
 /* SteadySolver::initialize (fv/steady_solver.cpp:1480-1491): u, v, w, p hold
  * one value per cell in the solver's cell order (that of
  * ka_simplec_centroids); a closed domain's pressure is shifted so that the

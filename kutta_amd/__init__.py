@@ -135,6 +135,9 @@ def host_lib() -> C.CDLL:
         lib.ka_simplec_create_open.argtypes = [vp, i32, i32, i32, C.POINTER(dbl), pi32, C.POINTER(dbl),
                                                C.POINTER(dbl), pi32, C.POINTER(dbl), C.POINTER(vp)]
         lib.ka_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
+        lib.ka_simplec_create_profiled.argtypes = [vp, i32, i32, i32, C.POINTER(dbl), pi32, C.POINTER(dbl),
+                                                   C.POINTER(dbl), pi32, pi32, BOUNDARY_PROFILE, vp,
+                                                   C.POINTER(dbl), C.POINTER(vp)]
         lib.ka_simplec_sst_update.argtypes = [vp, C.POINTER(dbl), C.POINTER(vp)]
         lib.ka_simplec_destroy.argtypes = [vp]
         lib.ka_simplec_iterate.argtypes = [vp, i32]
@@ -987,19 +990,19 @@ class Symmetry:
 
 @dataclass
 class VelocityInlet:
-    """fv/steady_solver.hpp:38-42 with a uniform velocity (laminar: no
-    turbulence condition)."""
-    velocity: tuple = (0.0, 0.0, 0.0)
+    """fv/steady_solver.hpp:38-42 (laminar: no turbulence condition); the
+    velocity is a tuple or a function of the face centroid."""
+    velocity: object = (0.0, 0.0, 0.0)
 
 
 @dataclass
 class PressureOutlet:
     """fv/steady_solver.hpp:43-51: static `pressure`; backflow along the face
     normal when `backflow_velocity` is None (NormalBackflow, the default),
-    else that uniform velocity; `backflow_total` selects
-    BackflowPressure::Total."""
+    else that velocity (a tuple or a function of the face centroid);
+    `backflow_total` selects BackflowPressure::Total."""
     pressure: float = 0.0
-    backflow_velocity: tuple | None = None
+    backflow_velocity: object = None
     backflow_total: bool = False
 
 
@@ -1082,6 +1085,30 @@ SST_FIELDS = ("mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strai
 SST_INT_FIELDS = ("fixed_omega_mask", "selected_wall_face")
 
 
+BOUNDARY_PROFILE = C.CFUNCTYPE(None, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+
+def profile_callback(conditions: dict):
+    """(profiled[6], callback) for the patches whose inlet or backflow
+    velocity is a function of the face centroid, else (None, None)."""
+    functions = {}
+    for p, name in enumerate(PATCHES):
+        c = conditions[name]
+        f = c.velocity if isinstance(c, VelocityInlet) else (
+            c.backflow_velocity if isinstance(c, PressureOutlet) else None)
+        if callable(f):
+            functions[p] = f
+    if not functions:
+        return None, None
+    profiled = (C.c_int32 * 6)(*[1 if p in functions else 0 for p in range(6)])
+
+    def call(patch, point, velocity, _user):
+        v = functions[patch]((point[0], point[1], point[2]))
+        velocity[0], velocity[1], velocity[2] = (float(x) for x in v)
+
+    return profiled, BOUNDARY_PROFILE(call)
+
+
 def pack_open_conditions(conditions: dict):
     """As pack_conditions with inlets and outlets: (kinds, velocity, pressure,
     total) in the boundary_kind codes 0 wall, 1 velocity inlet, 2 outlet with
@@ -1092,12 +1119,13 @@ def pack_open_conditions(conditions: dict):
         c = conditions[name]
         if isinstance(c, (Wall, VelocityInlet)):
             kinds[p] = 0 if isinstance(c, Wall) else 1
-            velocity[3 * p:3 * p + 3] = [float(v) for v in c.velocity]
+            if not callable(c.velocity):
+                velocity[3 * p:3 * p + 3] = [float(v) for v in c.velocity]
         elif isinstance(c, Symmetry):
             kinds[p] = 3
         elif isinstance(c, PressureOutlet):
             kinds[p] = 4 if c.backflow_velocity is None else 2
-            if c.backflow_velocity is not None:
+            if c.backflow_velocity is not None and not callable(c.backflow_velocity):
                 velocity[3 * p:3 * p + 3] = [float(v) for v in c.backflow_velocity]
             pressure[p] = float(c.pressure)
             total[p] = 1 if c.backflow_total else 0
@@ -1155,8 +1183,14 @@ class SteadySolver:
         kinds, velocity, pressure, total = pack_open_conditions(conditions)
         h = C.c_void_p()
         box = (C.c_double * 3)(*[float(e) for e in extent])
-        _check_host(host_lib().ka_simplec_create_open(device.handle, nx, ny, nz, box, kinds, velocity, pressure,
-                                                      total, s.packed(), C.byref(h)))
+        profiled, callback = profile_callback(conditions)
+        if profiled is None:
+            _check_host(host_lib().ka_simplec_create_open(device.handle, nx, ny, nz, box, kinds, velocity, pressure,
+                                                          total, s.packed(), C.byref(h)))
+        else:  # the callback runs inside the constructor only
+            _check_host(host_lib().ka_simplec_create_profiled(device.handle, nx, ny, nz, box, kinds, velocity,
+                                                              pressure, total, profiled, callback, None,
+                                                              s.packed(), C.byref(h)))
         self.device, self.handle, self.settings = device, h.value, s
 
     def initialize(self, velocity=(0.0, 0.0, 0.0), pressure=0.0) -> None:

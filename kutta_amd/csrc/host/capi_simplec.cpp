@@ -22,7 +22,8 @@ extern "C" {
 
 namespace {
 int create(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds, const double* velocity,
-           const double* pressure, const int32_t* total, const double* settings, void** out) {
+           const double* pressure, const int32_t* total, const int32_t* profiled, ka_boundary_profile fn,
+           void* user, const double* settings, void** out) {
   return guard([&] {
     KA_REQUIRE(nx > 0 && ny > 0 && nz > 0, "box dimensions must be positive");
     const Vec3 upper = extent ? Vec3{extent[0], extent[1], extent[2]} : Vec3{1, 1, 1};
@@ -35,6 +36,18 @@ int create(void* dev, int nx, int ny, int nz, const double* extent, const int32_
       conditions.push_back({mesh.patches[p].name, kinds[p],
                             {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]},
                             pressure ? pressure[p] : 0.0, total && total[p] != 0});
+    for (std::size_t p = 0; p < mesh.patches.size(); ++p)
+      if (profiled && profiled[p]) {
+        KA_REQUIRE(fn, "ka_simplec_create_profiled: profiled patch without a profile function");
+        KA_REQUIRE(kinds[p] == PatchCondition::VelocityInlet || kinds[p] == PatchCondition::PressureOutlet,
+                   "FV: profiles apply to velocity inlets and backflow velocities");
+        conditions[p].profile = [fn, user, p](Vec3 x) {
+          const double point[3] = {x.x, x.y, x.z};
+          double v[3] = {0, 0, 0};
+          fn(int32_t(p), point, v, user);
+          return Vec3{v[0], v[1], v[2]};
+        };
+      }
     SolverSettings s;
     s.density = settings[0];
     s.viscosity = settings[1];
@@ -65,12 +78,18 @@ int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, c
                  "ka_simplec_create: walls (0) and symmetry (3) only; ka_simplec_create_open takes the rest");
   });
   if (rc != 0) return rc;
-  return create(dev, nx, ny, nz, nullptr, kinds, wall, nullptr, nullptr, settings, out);
+  return create(dev, nx, ny, nz, nullptr, kinds, wall, nullptr, nullptr, nullptr, nullptr, nullptr, settings, out);
 }
 int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
                            const double* velocity, const double* pressure, const int32_t* total,
                            const double* settings, void** out) {
-  return create(dev, nx, ny, nz, extent, kinds, velocity, pressure, total, settings, out);
+  return create(dev, nx, ny, nz, extent, kinds, velocity, pressure, total, nullptr, nullptr, nullptr, settings, out);
+}
+int ka_simplec_create_profiled(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
+                               const double* velocity, const double* pressure, const int32_t* total,
+                               const int32_t* profiled, ka_boundary_profile profile, void* user,
+                               const double* settings, void** out) {
+  return create(dev, nx, ny, nz, extent, kinds, velocity, pressure, total, profiled, profile, user, settings, out);
 }
 int ka_simplec_sst_update(void* solver, const double* options, const float* const* inputs) {
   return guard([&] { handle(solver).solver->sst_update(options, inputs); });

@@ -696,7 +696,8 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
       Vec3 v{};
       if (c.kind == PatchCondition::Wall || c.kind == PatchCondition::VelocityInlet ||
           c.kind == PatchCondition::PressureOutlet)
-        v = c.velocity;
+        v = c.profile && c.kind != PatchCondition::Wall ? c.profile(geom.face_centroid[f]) : c.velocity;
+      KA_REQUIRE(std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z), "FV nonfinite field value");
       if (c.kind == PatchCondition::PressureOutlet || c.kind == PatchCondition::NormalBackflowOutlet) {
         totals[b] = c.total_backflow ? 1 : 0;
         pp[b] = float(c.pressure);

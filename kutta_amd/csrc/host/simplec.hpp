@@ -9,6 +9,7 @@
 // linear solvers' convergence polls, as the reference does.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -28,6 +29,10 @@ struct PatchCondition {
   Vec3 velocity{};        // wall, inlet or backflow velocity (uniform)
   double pressure = 0;    // outlet static pressure
   bool total_backflow = false;  // BackflowPressure::Total at an outlet
+  // A velocity profile of the face centroid for an inlet or an outlet's
+  // backflow (VelocityProfile / BackflowProfile as a function); replaces
+  // `velocity` when set.
+  std::function<Vec3(Vec3)> profile;
 };
 
 // fv/steady_solver.hpp:193-271, the members this solver honours; the others

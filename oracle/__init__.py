@@ -448,7 +448,8 @@ class SteadySolver:
 
     REPORT_SIZE = 26
 
-    def __init__(self, nx, ny, nz, kinds, wall, settings, pressure=None, total=None, extent=None):
+    def __init__(self, nx, ny, nz, kinds, wall, settings, pressure=None, total=None, extent=None, profiled=None,
+                 profile=None):
         """With `pressure` and `total` (kutta_amd.pack_open_conditions) the
         patches may also be velocity inlets and pressure outlets, and `extent`
         sets the box [0, extent] (the unit cube by default)."""
@@ -468,8 +469,15 @@ class SteadySolver:
         L.oracle_simplec_create_open.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
         L.oracle_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
         box = (C.c_double * 3)(*[float(e) for e in (extent or (1.0, 1.0, 1.0))])
-        self._keep = (kinds, wall, settings, pressure, total, box)
-        if pressure is None and extent is None:
+        self._keep = (kinds, wall, settings, pressure, total, box, profiled, profile)
+        if profiled is not None:
+            L.oracle_simplec_create_profiled.restype = vp
+            L.oracle_simplec_create_profiled.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, type(profile), vp,
+                                                         vp]
+            h = L.oracle_simplec_create_profiled(nx, ny, nz, C.cast(box, vp), C.cast(kinds, vp), C.cast(wall, vp),
+                                                 C.cast(pressure, vp), C.cast(total, vp), C.cast(profiled, vp),
+                                                 profile, None, C.cast(settings, vp))
+        elif pressure is None and extent is None:
             h = L.oracle_simplec_create(nx, ny, nz, C.cast(kinds, vp), C.cast(wall, vp), C.cast(settings, vp))
         else:
             if pressure is None:

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -101,6 +102,7 @@ struct Condition {
   Vec3 velocity{};
   double pressure = 0;
   int total = 0;  // BackflowPressure::Total at an outlet
+  std::function<Vec3(Vec3)> profile;  // inlet or backflow velocity of the face centroid
 };
 
 struct Settings {
@@ -562,7 +564,11 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
       const auto b = f - mesh.n_internal;
       const auto& c = conditions[p];
       kinds[b] = c.kind;
-      if (c.kind == 0 || c.kind == 1 || c.kind == 2) velocities[b] = c.velocity;
+      if (c.kind == 0 || c.kind == 1 || c.kind == 2)
+        velocities[b] = c.profile && c.kind != 0 ? c.profile(geom.face_centroid[f]) : c.velocity;
+      ORACLE_REQUIRE(std::isfinite(velocities[b].x) && std::isfinite(velocities[b].y) &&
+                         std::isfinite(velocities[b].z),
+                     "FV nonfinite field value");
       if (c.kind == 2 || c.kind == 4) {
         prescribed_pressure[b] = float(c.pressure);
         total_pressure[b] = c.total;
@@ -1231,16 +1237,40 @@ void* oracle_simplec_create(int nx, int ny, int nz, const int* kind, const doubl
 // velocity inlet, 2 pressure outlet with backflow velocity, 3 symmetry,
 // 4 pressure outlet with normal backflow; velocity[3p..3p+2] for kinds 0-2,
 // pressure[p] and total[p] (backflow total pressure) for outlets.
+typedef void (*oracle_boundary_profile)(int patch, const double* point, double* velocity, void* user);
+void* oracle_simplec_create_profiled(int nx, int ny, int nz, const double* extent, const int* kind,
+                                     const double* velocity, const double* pressure, const int* total,
+                                     const int* profiled, oracle_boundary_profile fn, void* user,
+                                     const double* settings);
 void* oracle_simplec_create_open(int nx, int ny, int nz, const double* extent, const int* kind,
                                  const double* velocity, const double* pressure, const int* total,
                                  const double* settings) {
+  return oracle_simplec_create_profiled(nx, ny, nz, extent, kind, velocity, pressure, total, nullptr, nullptr,
+                                        nullptr, settings);
+}
+// As oracle_simplec_create_open with a profile function for the patches
+// whose profiled[p] != 0.
+void* oracle_simplec_create_profiled(int nx, int ny, int nz, const double* extent, const int* kind,
+                                     const double* velocity, const double* pressure, const int* total,
+                                     const int* profiled, oracle_boundary_profile fn, void* user,
+                                     const double* settings) {
   try {
     using namespace oracle;
     using namespace oracle::simplec;
     auto mesh = make_box(nx, ny, nz, {0, 0, 0}, {extent[0], extent[1], extent[2]}, false);
     std::vector<Condition> conditions(mesh.patches.size());
     for (std::size_t p = 0; p < conditions.size(); ++p)
-      conditions[p] = {kind[p], {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]}, pressure[p], total[p]};
+      conditions[p] = {kind[p], {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]}, pressure[p], total[p], {}};
+    for (std::size_t p = 0; p < conditions.size(); ++p)
+      if (profiled && profiled[p]) {
+        ORACLE_REQUIRE(fn && (kind[p] == 1 || kind[p] == 2), "FV: profiles apply to inlets and backflow");
+        conditions[p].profile = [fn, user, p](Vec3 x) {
+          const double point[3] = {x.x, x.y, x.z};
+          double v[3] = {0, 0, 0};
+          fn(int(p), point, v, user);
+          return Vec3{v[0], v[1], v[2]};
+        };
+      }
     Settings s;
     s.density = settings[0];
     s.viscosity = settings[1];

@@ -161,3 +161,29 @@ def test_open_conditions_are_validated(device):
         kutta_amd.SteadySolver(device, 4, 2, 1, c)
     with pytest.raises(kutta_amd.KuttaError):
         kutta_amd.SteadySolver(device, 4, 2, 1, through_flow(1), extent=(1.0, -1.0, 1.0))
+
+
+def test_profiled_inlets_couette_matches_the_oracle(device):
+    """Inlets with the linear profile u = y (a function of the face centroid,
+    called back through the C ABI) between a moving lid and a wall at rest
+    (tests/fv/test_steady.cpp:49-82): bitwise with the oracle after 100
+    iterations, lid forces equal, and the steady profile reached."""
+    print("[open] couette with profiled inlets", flush=True)
+    c = {name: Symmetry() for name in PATCHES}
+    c["ymax"] = Wall((1.0, 4.0, 0.0))
+    c["ymin"] = Wall()
+    c["xmin"] = c["xmax"] = VelocityInlet(lambda x: (x[1], 0.0, 0.0))
+    settings = SolverSettings(viscosity=1.0)
+    kinds, velocity, pressure, total = kutta_amd.pack_open_conditions(c)
+    profiled, callback = kutta_amd.profile_callback(c)
+    extent = (1.0, 1.0, 0.1)
+    gpu = kutta_amd.SteadySolver(device, 4, 12, 1, c, settings, extent)
+    cpu = oracle.SteadySolver(4, 12, 1, kinds, velocity, settings.packed(), pressure, total, extent, profiled,
+                              callback)
+    for s in (gpu, cpu):
+        s.iterate(100)
+    assert_same_state(gpu, cpu, True)
+    assert_same_report(gpu, cpu)
+    g, o = gpu.forces(["ymax"]), cpu.forces(["ymax"])
+    assert g["viscous"][0] == pytest.approx(o["viscous"][0], rel=1e-12)
+    assert np.abs(gpu.field("u") - gpu.centroids()[:, 1]).max() < 2e-4

@@ -158,3 +158,39 @@ def test_open_conditions_are_validated():
         make((4, 2, 1), c)
     with pytest.raises(kutta_amd.KuttaError):
         kutta_amd.pack_open_conditions({name: object() for name in PATCHES})
+
+
+def couette():
+    c = {name: Symmetry() for name in PATCHES}
+    c["ymax"] = Wall((1.0, 4.0, 0.0))
+    c["ymin"] = Wall()
+    c["xmin"] = c["xmax"] = VelocityInlet(lambda x: (x[1], 0.0, 0.0))
+    return c
+
+
+def make_profiled(n, conditions, settings, extent=(1.0, 1.0, 0.1)):
+    kinds, velocity, pressure, total = kutta_amd.pack_open_conditions(conditions)
+    profiled, callback = kutta_amd.profile_callback(conditions)
+    return oracle.SteadySolver(*n, kinds, velocity, settings.packed(), pressure, total, extent, profiled, callback)
+
+
+def test_planar_couette_flow_with_moving_and_slip_walls():
+    """tests/fv/test_steady.cpp:49-82: a lid moving at (1, 4, 0) over a wall
+    at rest, inlets at both ends carrying the linear profile u = y: the steady
+    flow is u = y to 2e-4 (the lid's normal component is discarded), v
+    vanishes, the lid's viscous drag is -nu dU/dy A = -0.1 and its drag
+    coefficient -2, with no pressure force along x; moving the reference
+    pressure to 2 changes the lid's y pressure force by -2 x 0.1."""
+    s = make_profiled((4, 12, 1), couette(), SolverSettings(viscosity=1.0))
+    s.iterate(400)
+    u, v = s.field("u"), s.field("v")
+    y = s.centroids()[:, 1]
+    print(f"[couette] max error {np.abs(u - y).max():.3g}")
+    assert np.abs(u - y).max() < 2e-4
+    assert np.abs(v).max() < 2e-4
+    force = s.forces(["ymax"])
+    assert force["viscous"][0] == pytest.approx(-0.1, rel=2e-4)
+    assert abs(force["pressure"][0]) < 1e-6
+    assert force["viscous"][0] / (0.5 * 1.0 * 1.0 * 0.1) == pytest.approx(-2.0, rel=2e-4)
+    shifted = s.forces(["ymax"], 2.0)
+    assert shifted["pressure"][1] - force["pressure"][1] == pytest.approx(-0.2, abs=1e-6)
