@@ -187,3 +187,15 @@ def test_profiled_inlets_couette_matches_the_oracle(device):
     g, o = gpu.forces(["ymax"]), cpu.forces(["ymax"])
     assert g["viscous"][0] == pytest.approx(o["viscous"][0], rel=1e-12)
     assert np.abs(gpu.field("u") - gpu.centroids()[:, 1]).max() < 2e-4
+
+
+def test_symmetry_box_comes_to_rest_like_the_oracle(device):
+    """tests/fv/test_steady.cpp:9-20 on the device: a one-cell symmetry box
+    started in motion is at rest after 100 iterations, bitwise the oracle."""
+    c = {name: Symmetry() for name in PATCHES}
+    gpu, cpu = make(device, (1, 1, 1), c, SolverSettings(viscosity=0.1), (1.0, 1.0, 1.0))
+    initialize(gpu, cpu, (1.0, 0.4, -0.2), 0.0)
+    for s in (gpu, cpu):
+        s.iterate(100)
+    assert_same_state(gpu, cpu, True)
+    assert gpu.report()["maximum_velocity"] < 1e-5

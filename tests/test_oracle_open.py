@@ -194,3 +194,13 @@ def test_planar_couette_flow_with_moving_and_slip_walls():
     assert force["viscous"][0] / (0.5 * 1.0 * 1.0 * 0.1) == pytest.approx(-2.0, rel=2e-4)
     shifted = s.forces(["ymax"], 2.0)
     assert shifted["pressure"][1] - force["pressure"][1] == pytest.approx(-0.2, abs=1e-6)
+
+
+def test_symmetry_constrains_normal_velocity_in_a_closed_box():
+    """tests/fv/test_steady.cpp:9-20: one cell bounded by symmetry planes,
+    started with u = (1, 0.4, -0.2): every velocity component is normal to a
+    plane, so after 100 iterations the fluid is at rest to 1e-5."""
+    s = make((1, 1, 1), {name: Symmetry() for name in PATCHES}, SolverSettings(viscosity=0.1), (1.0, 1.0, 1.0))
+    initialize(s, (1.0, 0.4, -0.2), 0.0)
+    s.iterate(100)
+    assert s.report()[18] < 1e-5  # maximum velocity
