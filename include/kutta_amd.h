@@ -185,6 +185,11 @@ int ka_simplec_create(void* dev, int nx, int ny, int nz, const int32_t* kinds, c
 int ka_simplec_create_open(void* dev, int nx, int ny, int nz, const double* extent, const int32_t* kinds,
                            const double* velocity, const double* pressure, const int32_t* total,
                            const double* settings, void** out);
+/* In ka_simplec_create_open and ka_simplec_create_profiled, kinds[p] == 5 is
+ * a MassFlowInlet (fv/steady_solver.hpp:60-65; velocities as
+ * fv/boundary_physics.cpp:14-26): pressure[p] holds its mass flow (kg/s into
+ * the domain over the whole patch) and velocity[3p..3p+2] its inward
+ * direction, zero for each face's inward normal. */
 /* As ka_simplec_create_open, with the velocity of each patch whose
  * profiled[p] != 0 (a velocity inlet, or an outlet's backflow velocity) given
  * by profile(p, face centroid[3], velocity[3], user) at every face of the

@@ -996,6 +996,14 @@ class VelocityInlet:
 
 
 @dataclass
+class MassFlowInlet:
+    """fv/steady_solver.hpp:60-65 (laminar): kg/s into the domain over the
+    whole patch, along `direction` or each face's inward normal (None)."""
+    mass_flow: float = 0.0
+    direction: tuple | None = None
+
+
+@dataclass
 class PressureOutlet:
     """fv/steady_solver.hpp:43-51: static `pressure`; backflow along the face
     normal when `backflow_velocity` is None (NormalBackflow, the default),
@@ -1123,6 +1131,11 @@ def pack_open_conditions(conditions: dict):
                 velocity[3 * p:3 * p + 3] = [float(v) for v in c.velocity]
         elif isinstance(c, Symmetry):
             kinds[p] = 3
+        elif isinstance(c, MassFlowInlet):
+            kinds[p] = 5
+            pressure[p] = float(c.mass_flow)
+            if c.direction is not None:
+                velocity[3 * p:3 * p + 3] = [float(v) for v in c.direction]
         elif isinstance(c, PressureOutlet):
             kinds[p] = 4 if c.backflow_velocity is None else 2
             if c.backflow_velocity is not None and not callable(c.backflow_velocity):

@@ -37,6 +37,13 @@ int create(void* dev, int nx, int ny, int nz, const double* extent, const int32_
                             {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]},
                             pressure ? pressure[p] : 0.0, total && total[p] != 0});
     for (std::size_t p = 0; p < mesh.patches.size(); ++p)
+      if (kinds[p] == PatchCondition::MassFlowInlet) {  // mass flow in pressure[p], direction in velocity
+        conditions[p].mass_flow = pressure ? pressure[p] : 0.0;
+        conditions[p].direction = conditions[p].velocity;
+        conditions[p].velocity = {};
+        conditions[p].pressure = 0;
+      }
+    for (std::size_t p = 0; p < mesh.patches.size(); ++p)
       if (profiled && profiled[p]) {
         KA_REQUIRE(fn, "ka_simplec_create_profiled: profiled patch without a profile function");
         KA_REQUIRE(kinds[p] == PatchCondition::VelocityInlet || kinds[p] == PatchCondition::PressureOutlet,

@@ -508,7 +508,7 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   std::map<std::string, PatchCondition> by_name;
   bool outlet = false;
   for (auto& c : conditions) {
-    KA_REQUIRE(c.kind >= PatchCondition::Wall && c.kind <= PatchCondition::NormalBackflowOutlet,
+    KA_REQUIRE(c.kind >= PatchCondition::Wall && c.kind <= PatchCondition::MassFlowInlet,
                "SIMPLEC: walls, velocity inlets, pressure outlets and symmetry are supported");
     KA_REQUIRE(std::isfinite(c.velocity.x) && std::isfinite(c.velocity.y) && std::isfinite(c.velocity.z),
                "FV nonfinite boundary velocity");
@@ -687,16 +687,39 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   // Boundary kinds, prescribed values and LSQ observations (:314-400).
   std::vector<Index> kinds(nb, 0), totals(nb, 0);
   std::vector<float> pu(nb, 0.0f), pv(nb, 0.0f), pw(nb, 0.0f), pp(nb, 0.0f);
+  // :305-311: a mass-flow inlet's total and projected areas.
+  std::vector<double> inlet_area(host.patches.size(), 0.0), inlet_projected(host.patches.size(), 0.0);
+  for (std::size_t p = 0; p < host.patches.size(); ++p)
+    if (patch_condition[p].kind == PatchCondition::MassFlowInlet) {
+      const Vec3 dir = patch_condition[p].direction;
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+        inlet_area[p] += length_of(geom.face_area[f]);
+        if (length_of(dir) > 0) inlet_projected[p] -= dot(scale(dir, 1 / length_of(dir)), geom.face_area[f]);
+      }
+    }
   double pmin = std::numeric_limits<double>::infinity(), pmax = -pmin;
   for (std::size_t p = 0; p < host.patches.size(); ++p)
     for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
       const Index b = f - ni;
       const auto& c = patch_condition[p];
-      kinds[b] = c.kind;
+      kinds[b] = c.kind == PatchCondition::MassFlowInlet ? PatchCondition::VelocityInlet : c.kind;
       Vec3 v{};
       if (c.kind == PatchCondition::Wall || c.kind == PatchCondition::VelocityInlet ||
           c.kind == PatchCondition::PressureOutlet)
         v = c.profile && c.kind != PatchCondition::Wall ? c.profile(geom.face_centroid[f]) : c.velocity;
+      if (c.kind == PatchCondition::MassFlowInlet) {  // fv/boundary_physics.cpp:14-26
+        const Vec3 a = geom.face_area[f];
+        KA_REQUIRE(std::isfinite(c.mass_flow) && c.mass_flow >= 0 && inlet_area[p] > 0 && s.density > 0,
+                   "mass-flow-inlet: nonnegative mass flow and positive area/density required");
+        if (length_of(c.direction) > 0) {
+          KA_REQUIRE(inlet_projected[p] > 0, "mass-flow-inlet: nonzero inward direction required");
+          const Vec3 d = scale(c.direction, 1 / length_of(c.direction));
+          KA_REQUIRE(dot(d, a) < 0, "mass-flow-inlet: direction must enter every face");
+          v = scale(d, c.mass_flow / (s.density * inlet_projected[p]));
+        } else {
+          v = scale(a, -c.mass_flow / (s.density * inlet_area[p] * length_of(a)));
+        }
+      }
       KA_REQUIRE(std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z), "FV nonfinite field value");
       if (c.kind == PatchCondition::PressureOutlet || c.kind == PatchCondition::NormalBackflowOutlet) {
         totals[b] = c.total_backflow ? 1 : 0;

@@ -23,7 +23,11 @@ namespace kutta_amd {
 // boundary_kind of the face kernels: a pressure outlet with a backflow
 // velocity is kind 2, with normal backflow (the reference's default) kind 4.
 struct PatchCondition {
-  enum Kind : int { Wall = 0, VelocityInlet = 1, PressureOutlet = 2, Symmetry = 3, NormalBackflowOutlet = 4 };
+  // MassFlowInlet (fv/steady_solver.hpp:60-65) is a velocity inlet (kind 1)
+  // on the faces; its code here only selects the set-up.
+  enum Kind : int {
+    Wall = 0, VelocityInlet = 1, PressureOutlet = 2, Symmetry = 3, NormalBackflowOutlet = 4, MassFlowInlet = 5
+  };
   std::string patch;
   int kind = Wall;
   Vec3 velocity{};        // wall, inlet or backflow velocity (uniform)
@@ -33,6 +37,10 @@ struct PatchCondition {
   // backflow (VelocityProfile / BackflowProfile as a function); replaces
   // `velocity` when set.
   std::function<Vec3(Vec3)> profile;
+  // MassFlowInlet: kg/s into the domain over the whole patch, and an
+  // optional inward direction (zero: along each face's inward normal).
+  double mass_flow = 0;
+  Vec3 direction{};
 };
 
 // fv/steady_solver.hpp:193-271, the members this solver honours; the others

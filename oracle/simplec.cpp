@@ -103,6 +103,10 @@ struct Condition {
   double pressure = 0;
   int total = 0;  // BackflowPressure::Total at an outlet
   std::function<Vec3(Vec3)> profile;  // inlet or backflow velocity of the face centroid
+  // kind 5, MassFlowInlet (fv/steady_solver.hpp:60-65; a velocity inlet on
+  // the faces): kg/s into the domain and an optional inward direction.
+  double mass_flow = 0;
+  Vec3 direction{};
 };
 
 struct Settings {
@@ -484,7 +488,7 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
   std::vector<bool> patch_is_wall;
   bool outlet = false;
   for (const auto& c : conditions) {
-    ORACLE_REQUIRE(c.kind >= 0 && c.kind <= 4, "SIMPLEC oracle: walls, velocity inlets, pressure outlets, symmetry");
+    ORACLE_REQUIRE(c.kind >= 0 && c.kind <= 5, "SIMPLEC oracle: walls, velocity and mass-flow inlets, pressure outlets, symmetry");
     ORACLE_REQUIRE(std::isfinite(c.velocity.x) && std::isfinite(c.velocity.y) && std::isfinite(c.velocity.z),
                    "FV nonfinite boundary velocity");
     ORACLE_REQUIRE(std::isfinite(c.pressure), "FV nonfinite outlet pressure");
@@ -558,12 +562,34 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
   // Boundary kinds and prescribed values (:314-371).
   std::vector<Index> kinds(nb);
   std::vector<Vec3> velocities(nb);
+  // :305-311: a mass-flow inlet's total and projected areas.
+  std::vector<double> inlet_area(host.patches.size(), 0.0), inlet_projected(host.patches.size(), 0.0);
+  for (std::size_t p = 0; p < host.patches.size(); ++p)
+    if (conditions[p].kind == 5)
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+        inlet_area[p] += norm(geom.face_area[f]);
+        const Vec3 dir = conditions[p].direction;
+        if (norm(dir) > 0) inlet_projected[p] -= dot(dir * (1 / norm(dir)), geom.face_area[f]);
+      }
   double pmin = std::numeric_limits<double>::infinity(), pmax = -pmin;
   for (std::size_t p = 0; p < host.patches.size(); ++p)
     for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
       const auto b = f - mesh.n_internal;
       const auto& c = conditions[p];
-      kinds[b] = c.kind;
+      kinds[b] = c.kind == 5 ? 1 : c.kind;
+      if (c.kind == 5) {  // fv/boundary_physics.cpp:14-26
+        const Vec3 a = geom.face_area[f];
+        ORACLE_REQUIRE(std::isfinite(c.mass_flow) && c.mass_flow >= 0 && inlet_area[p] > 0 && settings.density > 0,
+                       "mass-flow-inlet: nonnegative mass flow and positive area/density required");
+        if (norm(c.direction) > 0) {
+          ORACLE_REQUIRE(inlet_projected[p] > 0, "mass-flow-inlet: nonzero inward direction required");
+          const Vec3 d = c.direction * (1 / norm(c.direction));
+          ORACLE_REQUIRE(dot(d, a) < 0, "mass-flow-inlet: direction must enter every face");
+          velocities[b] = d * (c.mass_flow / (settings.density * inlet_projected[p]));
+        } else {
+          velocities[b] = a * (-c.mass_flow / (settings.density * inlet_area[p] * norm(a)));
+        }
+      }
       if (c.kind == 0 || c.kind == 1 || c.kind == 2)
         velocities[b] = c.profile && c.kind != 0 ? c.profile(geom.face_centroid[f]) : c.velocity;
       ORACLE_REQUIRE(std::isfinite(velocities[b].x) && std::isfinite(velocities[b].y) &&
@@ -1261,6 +1287,13 @@ void* oracle_simplec_create_profiled(int nx, int ny, int nz, const double* exten
     std::vector<Condition> conditions(mesh.patches.size());
     for (std::size_t p = 0; p < conditions.size(); ++p)
       conditions[p] = {kind[p], {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]}, pressure[p], total[p], {}};
+    for (auto& c : conditions)
+      if (c.kind == 5) {  // mass flow in pressure[p], inward direction in velocity
+        c.mass_flow = c.pressure;
+        c.direction = c.velocity;
+        c.velocity = {};
+        c.pressure = 0;
+      }
     for (std::size_t p = 0; p < conditions.size(); ++p)
       if (profiled && profiled[p]) {
         ORACLE_REQUIRE(fn && (kind[p] == 1 || kind[p] == 2), "FV: profiles apply to inlets and backflow");

@@ -204,3 +204,28 @@ def test_symmetry_constrains_normal_velocity_in_a_closed_box():
     initialize(s, (1.0, 0.4, -0.2), 0.0)
     s.iterate(100)
     assert s.report()[18] < 1e-5  # maximum velocity
+
+
+@pytest.mark.parametrize("direction", [None, (1.0, 0.0, 0.0), (2.0, 0.0, 0.0)])
+def test_mass_flow_inlet_gives_the_uniform_stream(direction):
+    """fv/boundary_physics.cpp:14-26: 0.2 kg/s at density 2 through the 1 x 0.1
+    xmin patch enters at 0.2 / (2 x 0.1) = 1 along the inward normal (or the
+    given direction, of any length); with an outlet downstream the uniform
+    stream u = 1 is steady (as tests/fv/test_steady.cpp:22-47)."""
+    from kutta_amd import MassFlowInlet
+    c = through_flow(1)
+    c["xmin"] = MassFlowInlet(0.2, direction)
+    s = make((8, 4, 1), c, SolverSettings(density=2.0))
+    initialize(s, (1.0, 0.0, 0.0), 0.0)
+    s.iterate(6)
+    assert approx(s.field("u"), 1.0, 1e-5)
+    assert np.abs(s.field("v")).max() < 1e-6 and np.abs(s.field("p")).max() < 1e-6
+
+
+def test_mass_flow_inlet_validation():
+    from kutta_amd import MassFlowInlet
+    for bad in (MassFlowInlet(-1.0), MassFlowInlet(0.2, (-1.0, 0.0, 0.0))):
+        c = through_flow(1)
+        c["xmin"] = bad
+        with pytest.raises(oracle.OracleError):
+            make((4, 2, 1), c)
