@@ -324,6 +324,31 @@ def neumann_rhs(n: int, seed: int = 20261202, reference: int = -1) -> np.ndarray
     return b
 
 
+def ref_built(name: str) -> bool:
+    """Whether oracle/_ref/lib<name>.so exists, without loading it (test
+    collection must not map reference builds into a process)."""
+    return (HERE / "_ref" / f"lib{name}.so").exists()
+
+
+class _LazyLibrary:
+    """Loads a reference build on first attribute access."""
+
+    def __init__(self, loader):
+        self._loader, self._lib = loader, None
+
+    def __getattr__(self, name):
+        if self._lib is None:
+            self._lib = self._loader()
+            if self._lib is None:
+                raise OracleError("oracle/_ref is not built")
+        return getattr(self._lib, name)
+
+
+def lazy(loader) -> _LazyLibrary:
+    """A stand-in for loader()'s library that loads it only when first used."""
+    return _LazyLibrary(loader)
+
+
 def ref_mesh():
     """The reference's own mesh code (oracle/_ref/libref_mesh.so), or None."""
     path = HERE / "_ref" / "libref_mesh.so"

@@ -7,7 +7,7 @@ import pytest
 import oracle
 from kutta_amd.fv_specs import SPECS, FvMesh, build_args, same_bits, signature_of
 
-needs_ref = pytest.mark.skipif(oracle.ref_kernels() is None,
+needs_ref = pytest.mark.skipif(not oracle.ref_built("ref_kernels"),
                                reason="oracle/_ref not built (reference tree absent)")
 
 

@@ -8,8 +8,8 @@ import pytest
 
 import oracle
 
-REF = oracle.ref_mesh()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference tree absent)")
+REF = oracle.lazy(oracle.ref_mesh)
+needs_ref = pytest.mark.skipif(not oracle.ref_built("ref_mesh"), reason="oracle/_ref not built (reference tree absent)")
 
 
 def ref_array(h, name, dtype):

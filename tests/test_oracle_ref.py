@@ -9,8 +9,8 @@ import pytest
 import oracle
 from helpers import mixed_magnitude_vector, random_operator, random_vector
 
-REF = oracle.ref_kernels()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference tree absent)")
+REF = oracle.lazy(oracle.ref_kernels)
+needs_ref = pytest.mark.skipif(not oracle.ref_built("ref_kernels"), reason="oracle/_ref not built (reference tree absent)")
 
 
 def p(a):

@@ -152,7 +152,7 @@ UPLOAD_ARRAYS = ("area_x", "area_y", "area_z", "owner_to_face_x", "owner_to_face
                  "wall_normal_distance", "wall_distance")
 
 
-@pytest.mark.skipif(oracle.ref_devmesh() is None, reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.skipif(not oracle.ref_built("ref_devmesh"), reason="oracle/_ref not built (no reference tree)")
 @pytest.mark.parametrize("nx,ny,nz,tet", [(6, 5, 4, False), (4, 3, 5, True), (9, 7, 1, False)])
 def test_upload_mesh_matches_reference_build(nx, ny, nz, tet):
     """fv/device_mesh.cpp:208-256 built from the reference tree: every FP32
