@@ -411,6 +411,20 @@ enum { KH_INTO_SUM = 0, KH_INTO_DOT = 1, KH_INTO_DOT2 = 2 };
 int kh_reduce_into(kh_device* dev, int mode, const float* x, const float* y, const float* u,
                    const float* v, size_t n, float* dst, size_t offset);
 
+/* kh_reduce_into without a device handle and without allocating (no
+ * reference counterpart). For callers that cannot allocate device memory:
+ * code under HIP graph capture, serving loops, runtimes that own their own
+ * streams. Same arithmetic, order and output as kh_reduce_into; the
+ * per-workgroup partials live in a fixed 32 KiB array that each GPU holds for
+ * the life of the process, so the call makes no hipMalloc/hipFree. `stream` is
+ * a hipStream_t of the current device (NULL: its null stream); the work is
+ * asynchronous on it. n above KH_STATIC_REDUCE_MAX_N is an error. All calls
+ * on one GPU share the partials array: calls that can overlap on that GPU
+ * (different streams without ordering between them) race. */
+#define KH_STATIC_REDUCE_MAX_N 2147483647u
+int kh_reduce_into_static(void* stream, int mode, const float* x, const float* y, const float* u,
+                          const float* v, size_t n, float* dst, size_t offset);
+
 /* Graph capture of a fixed launch sequence (no reference counterpart: the
  * reference replays nothing on CUDA). Between begin and end, entry points that
  * enqueue work are recorded instead of executed; blocking calls are errors. */

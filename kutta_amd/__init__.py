@@ -65,6 +65,7 @@ def hip_lib() -> C.CDLL:
         lib.kh_kernel_name.restype = C.c_char_p
         lib.kh_reduce_host.argtypes = [vp, C.POINTER(KhReduction), sz, C.POINTER(C.c_double)]
         lib.kh_reduce_into.argtypes = [vp, i32, vp, vp, vp, vp, sz, vp, sz]
+        lib.kh_reduce_into_static.argtypes = [vp, i32, vp, vp, vp, vp, sz, vp, sz]
         lib.kh_graph_begin.argtypes = [vp]
         lib.kh_graph_end.argtypes = [vp, C.POINTER(vp)]
         lib.kh_graph_launch.argtypes = [vp, vp]

@@ -128,21 +128,56 @@ __global__ void __launch_bounds__(kBlock) k_reduce_finish(const double* __restri
 }
 
 template <int M>
+void reduce_on(hipStream_t stream, double* partials, const float* x, const float* y, const float* u,
+               const float* v, std::size_t n, float* dst, double* result) {
+  const unsigned g = reduce_blocks(n);
+  k_reduce_partials<M><<<dim3(g), dim3(kBlock), 0, stream>>>(x, y, u, v, n, partials);
+  k_reduce_finish<M><<<dim3(1), dim3(kBlock), 0, stream>>>(partials, g, dst, result);
+}
+template <int M>
 void reduce(kh_device* d, const float* x, const float* y, const float* u, const float* v,
             std::size_t n, float* dst, double* result) {
-  const unsigned g = reduce_blocks(n);
-  k_reduce_partials<M><<<dim3(g), dim3(kBlock), 0, d->stream>>>(x, y, u, v, n, d->partials);
-  k_reduce_finish<M><<<dim3(1), dim3(kBlock), 0, d->stream>>>(d->partials, g, dst, result);
+  reduce_on<M>(d->stream, d->partials, x, y, u, v, n, dst, result);
+}
+
+// kh_reduce_into_static keeps its per-workgroup partials here instead of in a
+// device's hipMalloc'd buffer: 2 doubles per workgroup of the largest
+// reduction grid (32 KiB). Each GPU holds its own copy for the life of the
+// process; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[2 * kReduceGrid];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxScratchDevices] = {};
+
+// The current device's copy of `scratch`, resolved on its first use.
+double* device_scratch() {
+  int ordinal = 0;
+  check(hipGetDevice(&ordinal), "hipGetDevice");
+  require(ordinal >= 0 && ordinal < kMaxScratchDevices, "kh_reduce_into_static: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[ordinal]) {
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    require(p != nullptr, "kh_reduce_into_static: null scratch address");
+    g_scratch[ordinal] = static_cast<double*>(p);
+  }
+  return g_scratch[ordinal];
 }
 
 }  // namespace
 
-void launch_fill32(kh_device* d, void* dst, uint32_t word, std::size_t count) {
+namespace {
+void fill32_on(hipStream_t stream, void* dst, uint32_t word, std::size_t count) {
   if (count == 0) return;
   std::size_t blocks = (count + kBlock - 1) / kBlock;
   if (blocks > 65536) blocks = 65536;
-  k_fill32<<<dim3(unsigned(blocks)), dim3(kBlock), 0, d->stream>>>(static_cast<uint32_t*>(dst),
-                                                                    word, count);
+  k_fill32<<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(static_cast<uint32_t*>(dst), word, count);
+}
+}  // namespace
+
+void launch_fill32(kh_device* d, void* dst, uint32_t word, std::size_t count) {
+  fill32_on(d->stream, dst, word, count);
 }
 
 void launch_reduce_into(kh_device* d, int mode, const float* x, const float* y, const float* u,
@@ -409,6 +444,33 @@ int kh_reduce_into(kh_device* d, int mode, const float* x, const float* y, const
                          mode == KH_INTO_DOT2 ? v : x, n, out);
     }
     check(hipGetLastError(), "reduce_into launch");
+  });
+}
+
+int kh_reduce_into_static(void* stream, int mode, const float* x, const float* y, const float* u,
+                          const float* v, size_t n, float* dst, size_t offset) {
+  return guard([&] {
+    require(dst != nullptr, "kh_reduce_into_static: null output");
+    require(mode >= KH_INTO_SUM && mode <= KH_INTO_DOT2, "kh_reduce_into_static: bad mode");
+    require(n <= size_t(KH_STATIC_REDUCE_MAX_N), "kh_reduce_into_static: n exceeds KH_STATIC_REDUCE_MAX_N");
+    require(n == 0 || (x != nullptr && (mode == KH_INTO_SUM || y != nullptr) &&
+                       (mode != KH_INTO_DOT2 || (u != nullptr && v != nullptr))),
+            "kh_reduce_into_static: null input");
+    require(dst != x && dst != y && dst != u && dst != v, "kh_reduce_into_static: output aliases input");
+    const auto s = static_cast<hipStream_t>(stream);
+    float* out = dst + offset;
+    if (n == 0) {
+      fill32_on(s, out, 0u, mode == KH_INTO_DOT2 ? 4 : 2);
+    } else {
+      double* partials = device_scratch();
+      const float* yy = mode == KH_INTO_SUM ? x : y;
+      const float* uu = mode == KH_INTO_DOT2 ? u : x;
+      const float* vv = mode == KH_INTO_DOT2 ? v : x;
+      if (mode == KH_INTO_SUM) reduce_on<kSum>(s, partials, x, yy, uu, vv, n, out, nullptr);
+      else if (mode == KH_INTO_DOT) reduce_on<kDot>(s, partials, x, yy, uu, vv, n, out, nullptr);
+      else reduce_on<kDot2>(s, partials, x, yy, uu, vv, n, out, nullptr);
+    }
+    check(hipGetLastError(), "reduce_into_static launch");
   });
 }
 
