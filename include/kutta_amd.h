@@ -201,6 +201,39 @@ int ka_simplec_create_profiled(void* dev, int nx, int ny, int nz, const double* 
                                const double* velocity, const double* pressure, const int32_t* total,
                                const int32_t* profiled, ka_boundary_profile profile, void* user,
                                const double* settings, void** out);
+/* SteadySolver construction on a general mesh (fv/steady_solver.hpp:313-315,
+ * fv/steady_solver.cpp:203-597), laminar or with k-omega SST. The mesh is
+ * the PolyMesh arrays (mesh/mesh.hpp:36-68): points[3*n_points],
+ * face_offsets[n_faces+1] into face_nodes, owner[n_faces],
+ * neighbour[n_internal] (internal faces first), and n_patches contiguous
+ * patches of boundary faces named in patch_names ('\n' separated) with
+ * patch_start/patch_size. Per patch: kinds[p] (0 wall, 1 velocity inlet, 2
+ * pressure outlet with a backflow velocity, 3 symmetry, 4 pressure outlet
+ * with normal backflow, 5 mass-flow inlet), velocity[3p..3p+2] (a mass-flow
+ * inlet's direction), pressure[p] (a mass-flow inlet's kg/s), total[p]
+ * (BackflowPressure::Total), flags[p] bit 0 (a mass-flow direction is
+ * given), turbulence[p] (-1 none, 0 TurbulenceValues{k, omega}, 1
+ * IntensityLength, 2 IntensityViscosityRatio; fv/steady_solver.hpp:28-35)
+ * with its pair turbulence_ab[2p..2p+1], required at inlets and outlets
+ * under SST. settings[12]: those of ka_simplec_create, then the fixed pseudo
+ * time step (0: automatic). sst: NULL for laminar flow, else SstSettings
+ * (fv/steady_solver.hpp:102-118) as 29 values: the 19 SstOptions of
+ * ka_simplec_sst_update, k second order (0/1: FirstOrderUpwind/SOU), omega
+ * second order, k relaxation, omega relaxation, linear rel_tol, abs_tol,
+ * max_iter, steady_tolerance, compensated_transport (0/1),
+ * pseudo_transient (0/1). */
+int ka_simplec_create_mesh(void* dev, int n_points, const double* points, int n_faces, const int32_t* face_offsets,
+                           const int32_t* face_nodes, const int32_t* owner, int n_internal, const int32_t* neighbour,
+                           int n_cells, int n_patches, const char* patch_names, const int32_t* patch_start,
+                           const int32_t* patch_size, const int32_t* kinds, const double* velocity,
+                           const double* pressure, const int32_t* total, const int32_t* flags,
+                           const int32_t* turbulence, const double* turbulence_ab, const double* settings,
+                           const double* sst, void** out);
+/* SteadySolver::initialize_fields (fv/steady_solver.cpp:1493-1524): u, v, w,
+ * p and, with SST, k and omega (NULL without) per cell in the solver's cell
+ * order; then the reference's three monitor passes. */
+int ka_simplec_initialize_fields(void* solver, const double* u, const double* v, const double* w, const double* p,
+                                 const double* k, const double* omega);
 /* SteadySolver::initialize (fv/steady_solver.cpp:1480-1491): u, v, w, p hold
  * one value per cell in the solver's cell order (that of
  * ka_simplec_centroids); a closed domain's pressure is shifted so that the
@@ -224,10 +257,12 @@ void ka_simplec_destroy(void* solver);
 /* SteadySolver::iterate(count): count SIMPLEC outer iterations. */
 int ka_simplec_iterate(void* solver, int count);
 /* The IterationReport (fv/steady_solver.hpp:276-284), after running monitor()
- * first when run_monitor != 0. out[27]: iteration; u, v, w, continuity as
+ * first when run_monitor != 0. out[36]: iteration; u, v, w, continuity as
  * (raw, scaled, fluent_scaled, fluent_normalization); corrected mass
  * imbalance; maximum velocity; pseudo time step; momentum iterations x3;
- * pressure iterations; pressure final residual; pressure converged. */
+ * pressure iterations; pressure final residual; pressure converged; then
+ * SST's k and omega residuals (4 each, as u) and k, omega iterations (zero
+ * without SST). */
 int ka_simplec_report(void* solver, int run_monitor, double* out);
 /* Downloads device field `name` (u v w p mass u_lo v_lo w_lo p_lo mass_lo ...)
  * into out when non-null; returns its length, or -1 if unknown. */

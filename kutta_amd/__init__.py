@@ -136,6 +136,9 @@ def host_lib() -> C.CDLL:
         lib.ka_simplec_create_open.argtypes = [vp, i32, i32, i32, C.POINTER(dbl), pi32, C.POINTER(dbl),
                                                C.POINTER(dbl), pi32, C.POINTER(dbl), C.POINTER(vp)]
         lib.ka_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
+        lib.ka_simplec_create_mesh.argtypes = [vp, i32, vp, i32, vp, vp, vp, i32, vp, i32, i32, C.c_char_p, vp, vp,
+                                               vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]
+        lib.ka_simplec_initialize_fields.argtypes = [vp] * 7
         lib.ka_simplec_create_profiled.argtypes = [vp, i32, i32, i32, C.POINTER(dbl), pi32, C.POINTER(dbl),
                                                    C.POINTER(dbl), pi32, pi32, BOUNDARY_PROFILE, vp,
                                                    C.POINTER(dbl), C.POINTER(vp)]
@@ -990,18 +993,42 @@ class Symmetry:
 
 
 @dataclass
+class TurbulenceValues:
+    """fv/steady_solver.hpp:28: inflow k and omega."""
+    k: float = 0.0
+    omega: float = 0.0
+
+
+@dataclass
+class IntensityLength:
+    """fv/steady_solver.hpp:29: intensity (a fraction) and length scale."""
+    intensity: float = 0.0
+    length: float = 0.0
+
+
+@dataclass
+class IntensityViscosityRatio:
+    """fv/steady_solver.hpp:30: intensity and turbulent viscosity ratio."""
+    intensity: float = 0.0
+    ratio: float = 0.0
+
+
+@dataclass
 class VelocityInlet:
-    """fv/steady_solver.hpp:38-42 (laminar: no turbulence condition); the
-    velocity is a tuple or a function of the face centroid."""
+    """fv/steady_solver.hpp:38-42; the velocity is a tuple or a function of
+    the face centroid; `turbulence` (TurbulenceValues, IntensityLength or
+    IntensityViscosityRatio) is required under SST."""
     velocity: object = (0.0, 0.0, 0.0)
+    turbulence: object = None
 
 
 @dataclass
 class MassFlowInlet:
-    """fv/steady_solver.hpp:60-65 (laminar): kg/s into the domain over the
-    whole patch, along `direction` or each face's inward normal (None)."""
+    """fv/steady_solver.hpp:60-65: kg/s into the domain over the whole
+    patch, along `direction` or each face's inward normal (None)."""
     mass_flow: float = 0.0
     direction: tuple | None = None
+    turbulence: object = None
 
 
 @dataclass
@@ -1013,6 +1040,7 @@ class PressureOutlet:
     pressure: float = 0.0
     backflow_velocity: object = None
     backflow_total: bool = False
+    turbulence: object = None
 
 
 @dataclass
@@ -1028,12 +1056,14 @@ class SolverSettings:
     paired_state: bool = True
     paired_state_slack_ulps: float = 1.0
     reference_point: tuple = (0.0, 0.0, 0.0)
+    pseudo_time_step: float = 0.0  # fixed pseudo time step; 0 is automatic
 
     def packed(self):
-        return (C.c_double * 11)(self.density, self.viscosity, self.momentum_relaxation,
+        """settings[12] (the box constructors read the first 11)."""
+        return (C.c_double * 12)(self.density, self.viscosity, self.momentum_relaxation,
                                  self.pressure_relaxation, self.pseudo_time_scale, self.monitor_interval,
                                  1.0 if self.paired_state else 0.0, self.paired_state_slack_ulps,
-                                 *self.reference_point)
+                                 *self.reference_point, self.pseudo_time_step)
 
 
 def pack_conditions(conditions: dict):
@@ -1081,6 +1111,84 @@ class SstOptions:
                                  float(self.kato_launder), float(self.vorticity_production),
                                  float(self.wall_production), self.cdkw_floor, float(self.limit_cross_diffusion),
                                  self.cross_diffusion_clip, self.omega_floor, self.k_floor, self.max_viscosity_ratio)
+
+
+@dataclass
+class SstSettings:
+    """fv/steady_solver.hpp:102-118 (no transition): the model options, the
+    k and omega convection schemes (True: second-order upwind), relaxation,
+    the linear solve control (rel_tol, abs_tol, max_iter), the steady
+    tolerance, compensated transport and pseudo-transient scalars."""
+    model: SstOptions = field(default_factory=SstOptions)
+    k_second_order: bool = True
+    omega_second_order: bool = True
+    k_relaxation: float = 0.75
+    omega_relaxation: float = 0.75
+    linear: tuple = (1e-3, 1e-20, 100)
+    steady_tolerance: float = 1e-6
+    compensated_transport: bool = True
+    pseudo_transient: bool = False
+
+    def packed(self):
+        return (C.c_double * 29)(*self.model.packed(), float(self.k_second_order), float(self.omega_second_order),
+                                 self.k_relaxation, self.omega_relaxation, *[float(v) for v in self.linear],
+                                 self.steady_tolerance, float(self.compensated_transport),
+                                 float(self.pseudo_transient))
+
+
+def _turbulence_code(t):
+    if t is None:
+        return -1, 0.0, 0.0
+    if isinstance(t, TurbulenceValues):
+        return 0, float(t.k), float(t.omega)
+    if isinstance(t, IntensityLength):
+        return 1, float(t.intensity), float(t.length)
+    if isinstance(t, IntensityViscosityRatio):
+        return 2, float(t.intensity), float(t.ratio)
+    raise KuttaError(f"FV: unsupported turbulence condition {t!r}")
+
+
+def pack_mesh_conditions(mesh, conditions: dict) -> dict:
+    """Per-patch arrays of ka_simplec_create_mesh for `mesh`'s patches, from
+    a dict patch name -> condition (uniform velocities only)."""
+    names = [p[0] for p in mesh.patches]
+    missing = [n for n in names if n not in conditions]
+    if missing or len(conditions) != len(names):
+        raise KuttaError(f"FV supply exactly one condition per patch (missing {missing})")
+    P = len(names)
+    out = {"kinds": (C.c_int32 * P)(), "velocity": (C.c_double * (3 * P))(), "pressure": (C.c_double * P)(),
+           "total": (C.c_int32 * P)(), "flags": (C.c_int32 * P)(), "turbulence": (C.c_int32 * P)(),
+           "turbulence_ab": (C.c_double * (2 * P))()}
+    for p, name in enumerate(names):
+        c = conditions[name]
+        code, a, b = _turbulence_code(getattr(c, "turbulence", None))
+        out["turbulence"][p] = code
+        out["turbulence_ab"][2 * p:2 * p + 2] = [a, b]
+        vel = None
+        if isinstance(c, Wall):
+            out["kinds"][p], vel = 0, c.velocity
+        elif isinstance(c, VelocityInlet):
+            out["kinds"][p], vel = 1, c.velocity
+        elif isinstance(c, Symmetry):
+            out["kinds"][p] = 3
+        elif isinstance(c, MassFlowInlet):
+            out["kinds"][p] = 5
+            out["pressure"][p] = float(c.mass_flow)
+            if c.direction is not None:
+                vel = c.direction
+                out["flags"][p] = 1
+        elif isinstance(c, PressureOutlet):
+            out["kinds"][p] = 4 if c.backflow_velocity is None else 2
+            vel = c.backflow_velocity
+            out["pressure"][p] = float(c.pressure)
+            out["total"][p] = 1 if c.backflow_total else 0
+        else:
+            raise KuttaError(f"SIMPLEC: unsupported condition for {name}")
+        if vel is not None:
+            if callable(vel):
+                raise KuttaError("SIMPLEC: profiles on general meshes are not supported; give a uniform velocity")
+            out["velocity"][3 * p:3 * p + 3] = [float(v) for v in vel]
+    return out
 
 
 SST_INPUTS = ("u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz", "density",
@@ -1183,7 +1291,9 @@ REPORT_FIELDS = ("iteration", "u_raw", "u_scaled", "u_fluent_scaled", "u_fluent_
                  "continuity_fluent_normalization", "corrected_mass_imbalance", "maximum_velocity",
                  "pseudo_time_step", "momentum_u_iterations", "momentum_v_iterations",
                  "momentum_w_iterations", "pressure_iterations", "pressure_final_residual",
-                 "pressure_converged")
+                 "pressure_converged", "k_raw", "k_scaled", "k_fluent_scaled", "k_fluent_normalization",
+                 "omega_raw", "omega_scaled", "omega_fluent_scaled", "omega_fluent_normalization",
+                 "k_iterations", "omega_iterations")
 
 
 class SteadySolver:
@@ -1206,6 +1316,49 @@ class SteadySolver:
                                                               pressure, total, profiled, callback, None,
                                                               s.packed(), C.byref(h)))
         self.device, self.handle, self.settings = device, h.value, s
+
+    @classmethod
+    def from_mesh(cls, device: Device, mesh, conditions: dict, settings: SolverSettings | None = None,
+                  sst: SstSettings | None = None):
+        """A solver on a general mesh (kutta_amd.meshes.PolyMeshArrays),
+        conditions by patch name; laminar, or k-omega SST with `sst`."""
+        s = settings or SolverSettings()
+        packed = pack_mesh_conditions(mesh, conditions)
+        arrays = [np.ascontiguousarray(mesh.points, np.float64), np.ascontiguousarray(mesh.face_offsets, np.int32),
+                  np.ascontiguousarray(mesh.face_nodes, np.int32), np.ascontiguousarray(mesh.owner, np.int32),
+                  np.ascontiguousarray(mesh.neighbour, np.int32)]
+        starts = np.array([p[1] for p in mesh.patches], np.int32)
+        sizes = np.array([p[2] for p in mesh.patches], np.int32)
+        sst_packed = sst.packed() if sst is not None else None
+        h = C.c_void_p()
+        vp = C.c_void_p
+        _check_host(host_lib().ka_simplec_create_mesh(
+            device.handle, len(mesh.points), arrays[0].ctypes.data, mesh.n_faces, arrays[1].ctypes.data,
+            arrays[2].ctypes.data, arrays[3].ctypes.data, mesh.n_internal,
+            arrays[4].ctypes.data if mesh.n_internal else None, mesh.n_cells, len(mesh.patches),
+            "\n".join(p[0] for p in mesh.patches).encode(), starts.ctypes.data, sizes.ctypes.data,
+            *[C.cast(packed[k], vp) for k in ("kinds", "velocity", "pressure", "total", "flags", "turbulence",
+                                               "turbulence_ab")],
+            C.cast(s.packed(), vp), C.cast(sst_packed, vp) if sst_packed is not None else None, C.byref(h)))
+        self = cls.__new__(cls)
+        self.device, self.handle, self.settings, self.sst = device, h.value, s, sst
+        return self
+
+    def initialize_fields(self, velocity=(0.0, 0.0, 0.0), pressure=0.0, k=None, omega=None) -> None:
+        """SteadySolver::initialize_fields (fv/steady_solver.cpp:1493-1524):
+        constants, functions of the cell centroid or per-cell arrays in the
+        solver's cell order; k and omega are required under SST."""
+        cent = self.centroids()
+        u, v, w, p = initial_fields(cent, velocity, pressure)
+        extra = []
+        for value in (k, omega):
+            if value is None:
+                extra.append(None)
+            else:
+                extra.append(initial_fields(cent, (0.0, 0.0, 0.0), value)[3])
+        _check_host(host_lib().ka_simplec_initialize_fields(
+            self.handle, u.ctypes.data, v.ctypes.data, w.ctypes.data, p.ctypes.data,
+            *[None if e is None else e.ctypes.data for e in extra]))
 
     def initialize(self, velocity=(0.0, 0.0, 0.0), pressure=0.0) -> None:
         """fv/steady_solver.cpp:1480-1491. `velocity` and `pressure` are

@@ -40,6 +40,10 @@ int create(void* dev, int nx, int ny, int nz, const double* extent, const int32_
       if (kinds[p] == PatchCondition::MassFlowInlet) {  // mass flow in pressure[p], direction in velocity
         conditions[p].mass_flow = pressure ? pressure[p] : 0.0;
         conditions[p].direction = conditions[p].velocity;
+        // The zero vector means no direction; anything else (NaN included) is
+        // a given direction that the solver validates.
+        const Vec3 d = conditions[p].direction;
+        conditions[p].has_direction = !(d.x == 0 && d.y == 0 && d.z == 0);
         conditions[p].velocity = {};
         conditions[p].pressure = 0;
       }
@@ -98,6 +102,116 @@ int ka_simplec_create_profiled(void* dev, int nx, int ny, int nz, const double* 
                                const double* settings, void** out) {
   return create(dev, nx, ny, nz, extent, kinds, velocity, pressure, total, profiled, profile, user, settings, out);
 }
+int ka_simplec_create_mesh(void* dev, int n_points, const double* points, int n_faces, const int32_t* face_offsets,
+                           const int32_t* face_nodes, const int32_t* owner, int n_internal, const int32_t* neighbour,
+                           int n_cells, int n_patches, const char* patch_names, const int32_t* patch_start,
+                           const int32_t* patch_size, const int32_t* kinds, const double* velocity,
+                           const double* pressure, const int32_t* total, const int32_t* flags,
+                           const int32_t* turbulence, const double* turbulence_ab, const double* settings,
+                           const double* sst, void** out) {
+  return guard([&] {
+    KA_REQUIRE(dev && points && face_offsets && face_nodes && owner && (n_internal == 0 || neighbour) &&
+                   patch_names && patch_start && patch_size && kinds && velocity && pressure && total && flags &&
+                   turbulence && turbulence_ab && settings && out,
+               "ka_simplec_create_mesh: null argument");
+    KA_REQUIRE(n_points > 0 && n_faces > 0 && n_internal >= 0 && n_internal <= n_faces && n_cells > 0 &&
+                   n_patches > 0,
+               "ka_simplec_create_mesh: empty or inconsistent sizes");
+    KA_REQUIRE(face_offsets[0] == 0, "ka_simplec_create_mesh: face offsets must start at 0");
+    for (int f = 0; f < n_faces; ++f)
+      KA_REQUIRE(face_offsets[f + 1] - face_offsets[f] >= 3, "ka_simplec_create_mesh: a face needs three nodes");
+    PolyMesh mesh;
+    mesh.points.resize(n_points);
+    for (int i = 0; i < n_points; ++i) mesh.points[i] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    mesh.face_offsets.assign(face_offsets, face_offsets + n_faces + 1);
+    mesh.face_nodes.assign(face_nodes, face_nodes + face_offsets[n_faces]);
+    for (auto v : mesh.face_nodes) KA_REQUIRE(v >= 0 && v < n_points, "ka_simplec_create_mesh: node out of range");
+    mesh.owner.assign(owner, owner + n_faces);
+    mesh.neighbour.assign(neighbour, neighbour + n_internal);
+    for (auto c : mesh.owner) KA_REQUIRE(c >= 0 && c < n_cells, "ka_simplec_create_mesh: owner out of range");
+    for (auto c : mesh.neighbour) KA_REQUIRE(c >= 0 && c < n_cells, "ka_simplec_create_mesh: neighbour out of range");
+    mesh.n_cells = n_cells;
+    std::vector<std::string> names(1);
+    for (const char* c = patch_names; *c; ++c) {
+      if (*c == '\n') names.emplace_back();
+      else names.back() += *c;
+    }
+    KA_REQUIRE(int(names.size()) == n_patches, "ka_simplec_create_mesh: one name per patch");
+    Index next = n_internal;
+    for (int p = 0; p < n_patches; ++p) {
+      KA_REQUIRE(patch_start[p] == next && patch_size[p] >= 0, "ka_simplec_create_mesh: patches must tile the boundary faces");
+      mesh.patches.push_back({names[p], patch_start[p], patch_size[p]});
+      next += patch_size[p];
+    }
+    KA_REQUIRE(next == n_faces, "ka_simplec_create_mesh: patches must tile the boundary faces");
+    std::vector<PatchCondition> conditions;
+    for (int p = 0; p < n_patches; ++p) {
+      PatchCondition c;
+      c.patch = names[p];
+      c.kind = kinds[p];
+      c.velocity = {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]};
+      c.pressure = pressure[p];
+      c.total_backflow = total[p] != 0;
+      if (c.kind == PatchCondition::MassFlowInlet) {
+        c.mass_flow = pressure[p];
+        c.direction = c.velocity;
+        c.has_direction = (flags[p] & 1) != 0;
+        c.velocity = {};
+        c.pressure = 0;
+      }
+      c.turbulence = turbulence[p];
+      c.turbulence_a = turbulence_ab[2 * p];
+      c.turbulence_b = turbulence_ab[2 * p + 1];
+      conditions.push_back(std::move(c));
+    }
+    SolverSettings s;
+    s.density = settings[0];
+    s.viscosity = settings[1];
+    s.momentum_relaxation = settings[2];
+    s.pressure_relaxation = settings[3];
+    s.pseudo_time_scale = settings[4];
+    s.monitor_interval = int(settings[5]);
+    s.paired_state = settings[6] != 0;
+    s.paired_state_slack_ulps = settings[7];
+    s.reference_point = {settings[8], settings[9], settings[10]};
+    s.pseudo_time_step = settings[11];
+    if (sst) {
+      s.sst = true;
+      auto& t = s.sst_settings;
+      for (int k = 0; k < 19; ++k) t.model[k] = sst[k];
+      t.k_second_order = sst[19] != 0;
+      t.omega_second_order = sst[20] != 0;
+      t.k_relaxation = sst[21];
+      t.omega_relaxation = sst[22];
+      KA_REQUIRE(std::isfinite(sst[23]) && sst[23] >= 0 && std::isfinite(sst[24]) && sst[24] >= 0 && sst[25] >= 1,
+                 "ka_simplec_create_mesh: invalid SST linear solve control");
+      t.linear = {sst[23], sst[24], int(sst[25]), 8};
+      t.steady_tolerance = sst[26];
+      t.compensated_transport = sst[27] != 0;
+      t.pseudo_transient = sst[28] != 0;
+    }
+    auto* h = new Simplec{static_cast<HipDevice*>(dev), nullptr};
+    try {
+      h->solver = std::make_unique<SteadySolver>(*h->dev, std::move(mesh), std::move(conditions), s);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int ka_simplec_initialize_fields(void* solver, const double* u, const double* v, const double* w, const double* p,
+                                 const double* k, const double* omega) {
+  return guard([&] {
+    auto& s = *handle(solver).solver;
+    const auto n = std::size_t(s.cells());
+    std::vector<Vec3> velocity(n);
+    for (std::size_t c = 0; c < n; ++c) velocity[c] = {u[c], v[c], w[c]};
+    s.initialize_fields(velocity, std::vector<double>(p, p + n),
+                        k ? std::vector<double>(k, k + n) : std::vector<double>{},
+                        omega ? std::vector<double>(omega, omega + n) : std::vector<double>{});
+  });
+}
 int ka_simplec_sst_update(void* solver, const double* options, const float* const* inputs) {
   return guard([&] { handle(solver).solver->sst_update(options, inputs); });
 }
@@ -142,6 +256,14 @@ int ka_simplec_report(void* solver, int run_monitor, double* out) {
     out[k++] = r.pressure.iterations;
     out[k++] = r.pressure.final_residual;
     out[k++] = r.pressure.converged;
+    for (const auto* v : {&r.k, &r.omega}) {
+      out[k++] = v->raw;
+      out[k++] = v->scaled;
+      out[k++] = v->fluent_scaled;
+      out[k++] = v->fluent_normalization;
+    }
+    out[k++] = r.turbulence[0].iterations;
+    out[k++] = r.turbulence[1].iterations;
   });
 }
 long ka_simplec_field(void* solver, const char* name, float* out) {

@@ -186,6 +186,17 @@ struct Equation {
   std::unique_ptr<KrylovSolver> solver;
 };
 
+// A turbulence transport equation (fv/transport.hpp:50-106) with the
+// compensated solve (fv/transport.cpp:51-55: low words of the assembled RHS
+// and diagonal, the delta and its residual) and the steady-stopping work
+// arrays (:64-69). It shares the operator values and scratch of momentum.
+struct ScalarEquation {
+  F rhs, diagonal, consistent_inverse, limiter, rhs_low, diagonal_low, delta, residual;
+  F work_residual, work_absolute, work_scale;
+  Vec3F grad;
+  std::unique_ptr<KrylovSolver> solver;
+};
+
 struct SteadySolver::Impl {
   HipDevice& dev;
   SolverSettings settings;
@@ -217,6 +228,16 @@ struct SteadySolver::Impl {
   I boundary_kind, scalar_kind, total_pressure;
   std::map<std::string, F> sst_fields;
   I sst_wall;
+  // SST k-omega in the step (fv/steady_solver.cpp:111-127, 484-542): the
+  // strain, k and omega fits, the face-jump fit, the transported fields and
+  // the boundary values the model reads.
+  bool sst = false;
+  LsqFit strain_lsq, k_lsq, omega_lsq;
+  Vec3F jump_delta, strain_base[3], strain_gradient[3], transpose_stress;
+  F jump_inverse, jump_forward, jump_reverse;
+  F kinetic, omega, rho_cell, mu_cell, boundary_k, boundary_omega, turbulence_a, turbulence_b;
+  I turbulence_spec, turbulence_kind, zero_mask;
+  ScalarEquation k_eq, omega_eq;
   F pressure_rhs, correction, correction_boundary, coefficient, face_pressure, pressure_jump;
   F residual_u, residual_v, residual_w, continuity, provisional_continuity, scale_cells, speed;
   F wall_diagonal, wall_rhs, physical_diagonal, consistent_inverse;
@@ -256,10 +277,12 @@ struct SteadySolver::Impl {
   // :843-851 (segregated solvers: the maximum cell speed).
   void update_time(double maximum_speed) {
     velocity_scale = std::max({.1 * bc_speed, maximum_speed, pressure_speed, 1e-10});
-    time_step = float(.3 * length * settings.pseudo_time_scale / velocity_scale);
+    time_step = settings.pseudo_time_step > 0 ? float(settings.pseudo_time_step)
+                                              : float(.3 * length * settings.pseudo_time_scale / velocity_scale);
   }
-  // :857-873
+  // :857-873 (a fixed step is not refreshed)
   void refresh_time_step() {
+    if (settings.pseudo_time_step > 0) return;
     dev.launch(KH_FV_SPEED, n, {velocity.x, velocity.y, velocity.z, speed});
     update_time(max_abs(speed, n));
   }
@@ -359,12 +382,14 @@ struct SteadySolver::Impl {
                   pressure_gradient_lo.z, previous_mass, mass, mass_lo, ni, float(settings.density),
                   paired_slack().flux_per_area});
   }
-  // :765-776 (laminar: the coefficient argument is an unused placeholder).
+  // :765-776: the SST wall-function shear coefficient at walls (laminar: the
+  // coefficient argument is an unused placeholder).
   void wall_terms(int k) {
+    const F& coefficient_of_wall = sst ? sst_fields.at("wall_shear_coefficient") : prescribed_pressure;
     dev.launch(KH_FV_BOUNDARY_MOMENTUM, n,
-               {offsets, faces, boundary_kind, area.x, area.y, area.z, prescribed_pressure, viscosity, diffusion,
+               {offsets, faces, boundary_kind, area.x, area.y, area.z, coefficient_of_wall, viscosity, diffusion,
                 velocity.x, velocity.y, velocity.z, boundary_velocity.x, boundary_velocity.y,
-                boundary_velocity.z, wall_diagonal, wall_rhs, ni, k, 0});
+                boundary_velocity.z, wall_diagonal, wall_rhs, ni, k, sst ? 1 : 0});
   }
   // fv/transport.cpp:83-121 with the paired limiter of with_low (:963-976).
   void assemble(int k, float* values = nullptr) {
@@ -412,11 +437,14 @@ struct SteadySolver::Impl {
   bool batch_momentum = false;
   bool pressure_tuned = false;
   F momentum_values[3], momentum_deltas[3];
+  // With SST wall functions the wall faces' terms read the other components
+  // the same way (the tangential projection), so walls must be axis-aligned
+  // too.
   bool batch_eligible() {
     const auto ax = area.x.download(), ay = area.y.download(), az = area.z.download();
     const auto kinds = boundary_kind.download();
     for (Index b = 0; b < nb; ++b) {
-      if (kinds[b] != 3) continue;
+      if (kinds[b] != 3 && !(sst && kinds[b] == 0)) continue;
       const Index f = ni + b;
       const int nonzero = (ax[f] != 0.0f) + (ay[f] != 0.0f) + (az[f] != 0.0f);
       if (nonzero > 1) return false;
@@ -496,6 +524,196 @@ struct SteadySolver::Impl {
   void step();
   IterationReport monitor();
   ForceReport forces(const std::vector<std::string>& patches, double reference_pressure);
+  void initialize_fields(const std::vector<Vec3>& u, const std::vector<double>& p, const std::vector<double>& k,
+                         const std::vector<double>& w);
+
+  // SstModel::update (turbulence/sst.cpp:101-158, no intermittency): in[25]
+  // as ka_simplec_sst_update's inputs; `gradients` runs where the reference
+  // calls reconstruct_scalar_gradients (:126).
+  void sst_kernels(const double* o, const float* const* in, const std::function<void()>& gradients);
+  // :654-680 (the fused vector fit): strain-rate derivatives of each velocity
+  // component, corrected by the face jumps (fv/operators.cpp:254-267).
+  void reconstruct_strain() {
+    const auto& fit = strain_lsq;
+    dev.launch(KH_FV_VECTOR_GRADIENT, n,
+               {offsets, adjacent, lsq_x, lsq_y, lsq_z, fit.bx, fit.by, fit.bz, fit.inverse, velocity.x,
+                velocity.y, velocity.z, boundary_velocity.x, boundary_velocity.y, boundary_velocity.z,
+                strain_base[0].x, strain_base[0].y, strain_base[0].z, strain_base[1].x, strain_base[1].y,
+                strain_base[1].z, strain_base[2].x, strain_base[2].y, strain_base[2].z, n});
+    for (int k = 0; k < 3; ++k) {
+      const auto& base = strain_base[k];
+      auto& out = strain_gradient[k];
+      dev.launch(KH_FV_FACE_JUMP, nf,
+                 {owner, neighbour, owner_to_face.x, owner_to_face.y, owner_to_face.z, neighbour_to_face.x,
+                  neighbour_to_face.y, neighbour_to_face.z, velocity[k], boundary_velocity[k], base.x, base.y,
+                  base.z, jump_forward, jump_reverse, ni});
+      dev.launch(KH_FV_GATHER_JUMP_GRADIENT, n,
+                 {offsets, faces, sign, jump_delta.x, jump_delta.y, jump_delta.z, jump_inverse, jump_forward,
+                  jump_reverse, base.x, base.y, base.z, out.x, out.y, out.z, n});
+    }
+  }
+  // :646-651
+  void turbulence_boundaries() {
+    dev.launch(KH_FV_TURBULENCE_BOUNDARY, nb,
+               {owner, boundary_kind, turbulence_spec, turbulence_a, turbulence_b, mass, boundary_velocity.x,
+                boundary_velocity.y, boundary_velocity.z, kinetic, omega, sst_fields.at("wall_face_omega"),
+                boundary_k, boundary_omega, turbulence_kind, ni, float(settings.density),
+                float(settings.viscosity)});
+  }
+  // :681-712 (SST): the strain fit when the velocity changed, the model with
+  // the turbulence boundary values and the k and omega fits between its wall
+  // and interior passes, then the face effective viscosity.
+  void update_turbulence(bool velocity_changed = true) {
+    if (!sst) return;
+    if (velocity_changed) reconstruct_strain();
+    const float* in[25] = {velocity.x.data(), velocity.y.data(), velocity.z.data(),
+                           strain_gradient[0].x.data(), strain_gradient[0].y.data(), strain_gradient[0].z.data(),
+                           strain_gradient[1].x.data(), strain_gradient[1].y.data(), strain_gradient[1].z.data(),
+                           strain_gradient[2].x.data(), strain_gradient[2].y.data(), strain_gradient[2].z.data(),
+                           rho_cell.data(), mu_cell.data(), kinetic.data(), omega.data(),
+                           k_eq.grad.x.data(), k_eq.grad.y.data(), k_eq.grad.z.data(),
+                           omega_eq.grad.x.data(), omega_eq.grad.y.data(), omega_eq.grad.z.data(),
+                           boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data()};
+    sst_kernels(settings.sst_settings.model, in, [&] {
+      turbulence_boundaries();
+      gradient(k_lsq, kinetic, boundary_k, k_eq.grad);
+      gradient(omega_lsq, omega, boundary_omega, omega_eq.grad);
+    });
+    dev.launch(KH_FV_EFFECTIVE_VISCOSITY, nf,
+               {owner, neighbour, boundary_kind, weight, sst_fields.at("mu_t"), viscosity, ni,
+                float(settings.viscosity)});
+  }
+  // :713-727
+  void add_transpose_stress() {
+    if (!sst) return;
+    const auto &u = momentum[0].grad, &v = momentum[1].grad, &w = momentum[2].grad;
+    dev.launch(KH_FV_TRANSPOSE_STRESS, nf,
+               {owner, neighbour, boundary_kind, weight, viscosity, area.x, area.y, area.z, u.x, u.y, u.z, v.x,
+                v.y, v.z, w.x, w.y, w.z, transpose_stress.x, transpose_stress.y, transpose_stress.z, ni});
+    dev.launch(KH_FV_STRESS_SOURCE, n,
+               {offsets, faces, sign, volume, transpose_stress.x, transpose_stress.y, transpose_stress.z, source.x,
+                source.y, source.z});
+  }
+  // :778-795 with TransportEquation::assemble_reconstructed
+  // (fv/transport.cpp:83-121): Barth-Jespersen limited k or omega with the
+  // model's face diffusivity, sources and Patankar sinks, compensated low
+  // words; omega's wall cells become fixed rows.
+  void assemble_turbulence(bool is_omega) {
+    const auto& opt = settings.sst_settings;
+    auto& eq = is_omega ? omega_eq : k_eq;
+    const F& field = is_omega ? omega : kinetic;
+    const F& bc = is_omega ? boundary_omega : boundary_k;
+    const F& mu = sst_fields.at(is_omega ? "face_diffusivity_omega" : "face_diffusivity_k");
+    const F& src = sst_fields.at(is_omega ? "omega_source" : "k_source");
+    const F& sink = sst_fields.at(is_omega ? "omega_sink" : "k_sink");
+    const bool second_order = is_omega ? opt.omega_second_order : opt.k_second_order;
+    const float dt = opt.pseudo_transient ? time_step : 0.0f;
+    const float relaxation = float(is_omega ? opt.omega_relaxation : opt.k_relaxation);
+    const bool compensated = opt.compensated_transport;
+    dev.launch(KH_FV_LIMITER, n,
+               {offsets, adjacent, faces, sign, owner_to_face.x, owner_to_face.y, owner_to_face.z,
+                neighbour_to_face.x, neighbour_to_face.y, neighbour_to_face.z, field, bc, eq.grad.x, eq.grad.y,
+                eq.grad.z, eq.limiter, -1.0f});
+    dev.launch(KH_FV_TRANSPORT_FACES, nf,
+               {owner, neighbour, weight, owner_to_face.x, owner_to_face.y, owner_to_face.z, neighbour_to_face.x,
+                neighbour_to_face.y, neighbour_to_face.z, nonorth_x, nonorth_y, nonorth_z, eq.grad.x, eq.grad.y,
+                eq.grad.z, eq.limiter, mass, mu, turbulence_kind, face_correction, ni, second_order ? 1 : 0, 0});
+    dev.launch(KH_FV_TRANSPORT_ASSEMBLE, n,
+               {offsets, faces, sign, adjacent, matrix_entry, matrix.row_ptr, matrix.diag, diffusion, mass, mu,
+                turbulence_kind, bc, face_correction, src, volume, field, sink, zero, zero, matrix.val, eq.rhs,
+                compensated ? eq.rhs_low : zero, eq.diagonal, compensated ? eq.diagonal_low : zero,
+                eq.consistent_inverse, ni, float(settings.density), dt, relaxation, compensated ? 1 : 0});
+    if (is_omega)
+      dev.launch(KH_FV_FIXED_ROWS, n,
+                 {matrix.row_ptr, matrix.diag, sst_fields.at("fixed_omega_mask"), sst_fields.at("fixed_omega"),
+                  matrix.val, eq.rhs});
+  }
+  // TransportEquation::physical_residual (fv/transport.cpp:57-62).
+  void physical_residual(ScalarEquation& eq, const F& field, F& result, const F* excluded) {
+    if (!settings.sst_settings.compensated_transport) {
+      dev.launch(KH_CSR_RESIDUAL, n, {matrix.row_ptr, matrix.col, matrix.val, field, eq.rhs, result});
+      return;
+    }
+    dev.launch(KH_FV_COMPENSATED_MOMENTUM_RESIDUAL, n,
+               {matrix.row_ptr, matrix.col, matrix.val, matrix.diag, eq.diagonal, eq.diagonal_low, field, zero, zero,
+                volume, eq.rhs, eq.rhs_low, result});
+    if (excluded) dev.launch(KH_FV_SCALAR_SET_FIXED, n, {*excluded, zero, result});
+  }
+  // TransportEquation::solve (fv/transport.cpp:123-159), Jacobi BiCGStab; the
+  // steady-stopping sums share one host wait. `excluded` holds int32 bits.
+  SolveStats solve_scalar(ScalarEquation& eq, F& field, const F* excluded) {
+    const auto& opt = settings.sst_settings;
+    SolveControl control = opt.linear;
+    const bool compensated = opt.compensated_transport;
+    if (compensated) physical_residual(eq, field, eq.residual, excluded);
+    if (opt.steady_tolerance > 0) {
+      if (!compensated)
+        dev.launch(KH_CSR_RESIDUAL, n, {matrix.row_ptr, matrix.col, matrix.val, field, eq.rhs, eq.work_residual});
+      else
+        dev.copy(eq.work_residual.data(), eq.residual.data(), std::size_t(n) * 4);
+      if (excluded)
+        dev.launch(KH_FV_SCALAR_STEADY_MEASURE, n,
+                   {field, eq.diagonal, eq.work_residual, *excluded, eq.work_absolute, eq.work_scale, 1});
+      else
+        dev.launch(KH_FV_SCALAR_STEADY_MEASURE, n,
+                   {field, eq.diagonal, eq.work_residual, matrix.diag, eq.work_absolute, eq.work_scale, 0});
+      const auto r = dev.reduce_many({HipDevice::sum(eq.work_absolute.data(), n), HipDevice::sum(eq.work_scale.data(), n),
+                                      HipDevice::dot(eq.work_residual.data(), eq.work_residual.data(), n)});
+      if (r[0] <= opt.steady_tolerance * r[1] && std::isfinite(r[2]))
+        control.abs_tol = std::max(control.abs_tol, std::nextafter(std::sqrt(r[2]), std::numeric_limits<double>::infinity()));
+    }
+    eq.solver->refresh(matrix.view());
+    if (!compensated) return eq.solver->solve(matrix.view(), field.data(), eq.rhs.data(), control);
+    dev.fill(eq.delta.data(), 0.0f, n);
+    const auto stats = eq.solver->solve(matrix.view(), eq.delta.data(), eq.residual.data(), control);
+    dev.launch(KH_AXPBYZ, n, {field, field, eq.delta, 1.0f, 1.0f});
+    return stats;
+  }
+  // :796-818 (no transition model).
+  void solve_turbulence() {
+    if (!sst) return;
+    const auto& opt = settings.sst_settings;
+    update_boundaries();
+    update_turbulence();
+    assemble_turbulence(false);
+    report.turbulence[0] = solve_scalar(k_eq, kinetic, nullptr);
+    dev.launch(KH_FV_FLOOR, n, {kinetic, float(opt.model[17])});
+    update_turbulence(false);
+    assemble_turbulence(true);
+    const F& mask = sst_fields.at("fixed_omega_mask");
+    const F& fixed = sst_fields.at("fixed_omega");
+    const bool set_fixed = opt.steady_tolerance > 0 || opt.compensated_transport;
+    if (set_fixed) dev.launch(KH_FV_SCALAR_SET_FIXED, n, {mask, fixed, omega});
+    report.turbulence[1] = solve_scalar(omega_eq, omega, &mask);
+    if (set_fixed) dev.launch(KH_FV_SCALAR_SET_FIXED, n, {mask, fixed, omega});
+    dev.launch(KH_FV_FLOOR, n, {omega, float(opt.model[16])});
+  }
+  // :819-837
+  void monitor_turbulence() {
+    if (!sst) return;
+    const auto measure = [&](bool is_omega) {
+      assemble_turbulence(is_omega);
+      auto& eq = is_omega ? omega_eq : k_eq;
+      const F& q = is_omega ? omega : kinetic;
+      const F& mask = sst_fields.at("fixed_omega_mask");
+      physical_residual(eq, q, residual_u, is_omega ? &mask : nullptr);
+      dev.launch(KH_FV_SCALAR_MONITOR, n,
+                 {q, eq.diagonal, residual_u,
+                  is_omega ? static_cast<const void*>(mask.data()) : static_cast<const void*>(zero_mask.data()),
+                  volume, absolute_residual.x, scale_cells,
+                  fluent_scale.x, float(settings.density),
+                  settings.sst_settings.pseudo_transient ? time_step : 0.0f});
+      const auto r = dev.reduce_many({HipDevice::sum(absolute_residual.x.data(), n), HipDevice::sum(scale_cells.data(), n),
+                                      HipDevice::sum(fluent_scale.x.data(), n)});
+      const double raw = r[0];
+      const auto scaled = [&](double d) {
+        return d > 0 ? raw / d : (raw == 0 ? 0 : std::numeric_limits<double>::infinity());
+      };
+      return ResidualValue{raw, scaled(r[1]), scaled(r[2]), r[2]};
+    };
+    report.k = measure(false);
+    report.omega = measure(true);
+  }
 };
 
 SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition> conditions, SolverSettings s)
@@ -503,7 +721,8 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   KA_REQUIRE(s.density > 0 && std::isfinite(s.density) && s.viscosity > 0 && std::isfinite(s.viscosity) &&
                  s.momentum_relaxation > 0 && s.momentum_relaxation <= 1 && s.pressure_relaxation > 0 &&
                  s.pressure_relaxation <= 1 && s.pseudo_time_scale > 0 && s.monitor_interval > 0 &&
-                 std::isfinite(s.paired_state_slack_ulps) && s.paired_state_slack_ulps > 0,
+                 std::isfinite(s.paired_state_slack_ulps) && s.paired_state_slack_ulps > 0 &&
+                 std::isfinite(s.pseudo_time_step) && s.pseudo_time_step >= 0,
              "FV invalid steady solver settings");
   std::map<std::string, PatchCondition> by_name;
   bool outlet = false;
@@ -694,7 +913,8 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
       const Vec3 dir = patch_condition[p].direction;
       for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
         inlet_area[p] += length_of(geom.face_area[f]);
-        if (length_of(dir) > 0) inlet_projected[p] -= dot(scale(dir, 1 / length_of(dir)), geom.face_area[f]);
+        if (patch_condition[p].has_direction && length_of(dir) > 0)
+          inlet_projected[p] -= dot(scale(dir, 1 / length_of(dir)), geom.face_area[f]);
       }
     }
   double pmin = std::numeric_limits<double>::infinity(), pmax = -pmin;
@@ -711,8 +931,10 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
         const Vec3 a = geom.face_area[f];
         KA_REQUIRE(std::isfinite(c.mass_flow) && c.mass_flow >= 0 && inlet_area[p] > 0 && s.density > 0,
                    "mass-flow-inlet: nonnegative mass flow and positive area/density required");
-        if (length_of(c.direction) > 0) {
-          KA_REQUIRE(inlet_projected[p] > 0, "mass-flow-inlet: nonzero inward direction required");
+        if (c.has_direction) {
+          KA_REQUIRE(std::isfinite(c.direction.x) && std::isfinite(c.direction.y) && std::isfinite(c.direction.z) &&
+                         length_of(c.direction) > 0 && inlet_projected[p] > 0,
+                     "mass-flow-inlet: nonzero inward direction required");
           const Vec3 d = scale(c.direction, 1 / length_of(c.direction));
           KA_REQUIRE(dot(d, a) < 0, "mass-flow-inlet: direction must enter every face");
           v = scale(d, c.mass_flow / (s.density * inlet_projected[p]));
@@ -797,6 +1019,114 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
     KA_REQUIRE(std::all_of(q.begin(), q.end(), [](double v) { return v > .05; }),
                "SIMPLEC: cells at or below the poor-quality threshold are not supported");
   }
+  sst = s.sst;
+  if (sst) {  // :471-542
+    const auto& t = s.sst_settings;
+    const double* o = t.model;
+    KA_REQUIRE(t.k_relaxation > 0 && t.k_relaxation <= 1 && t.omega_relaxation > 0 && t.omega_relaxation <= 1,
+               "FV invalid SST relaxation");
+    KA_REQUIRE(std::isfinite(o[4]) && o[4] > 0 && o[13] > 0 && o[15] > 0 && o[16] > 0 && o[17] > 0 && o[18] > 0 &&
+                   std::isfinite(o[6]) && o[6] >= 0 && !(o[10] != 0 && o[11] != 0),
+               "SST: invalid options");
+    KA_REQUIRE(std::isfinite(t.steady_tolerance) && t.steady_tolerance >= 0, "FV invalid scalar steady tolerance");
+    std::vector<Observation> strain_rows(nb), observe_k(nb), observe_omega(nb);
+    for (Index b = 0; b < nb; ++b) {
+      strain_rows[b] = kinds[b] != 0 ? Observation::FaceValue : Observation::Omitted;
+      observe_k[b] = kinds[b] == 1 ? Observation::FaceValue : Observation::Omitted;
+      observe_omega[b] = (kinds[b] == 0 || kinds[b] == 1) ? Observation::FaceValue : Observation::Omitted;
+      if (kinds[b] == 3) observe_k[b] = observe_omega[b] = Observation::NormalFootValue;
+    }
+    strain_lsq = least_squares(dev, host, geom, cf, strain_rows);
+    {  // face_jump_least_squares (fv/operators.cpp:234-253)
+      std::vector<Vec3> rows;
+      rows.reserve(cf.faces.size());
+      std::vector<float> inverse(6 * std::size_t(n));
+      for (Index c = 0; c < n; ++c) {
+        double a[6]{};
+        for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+          const Vec3 r = sub(geom.face_centroid[cf.faces[j]], geom.cell_centroid[c]);
+          const double w = 1 / dot(r, r);
+          rows.push_back(scale(r, w));
+          a[0] += w * r.x * r.x;
+          a[1] += w * r.x * r.y;
+          a[2] += w * r.x * r.z;
+          a[3] += w * r.y * r.y;
+          a[4] += w * r.y * r.z;
+          a[5] += w * r.z * r.z;
+        }
+        const double det = a[0] * (a[3] * a[5] - a[4] * a[4]) - a[1] * (a[1] * a[5] - a[4] * a[2]) +
+                           a[2] * (a[1] * a[4] - a[3] * a[2]);
+        KA_REQUIRE(det > 1e-14, "FV face-jump fit is rank deficient");
+        const double inv[6]{(a[3] * a[5] - a[4] * a[4]) / det, (a[2] * a[4] - a[1] * a[5]) / det,
+                            (a[1] * a[4] - a[2] * a[3]) / det, (a[0] * a[5] - a[2] * a[2]) / det,
+                            (a[1] * a[2] - a[0] * a[4]) / det, (a[0] * a[3] - a[1] * a[1]) / det};
+        for (int k = 0; k < 6; ++k) inverse[c + k * std::size_t(n)] = float(inv[k]);
+      }
+      upload3(dev, jump_delta, rows.size(), [&](std::size_t j) { return rows[j]; });
+      jump_inverse = to_device(dev, inverse);
+    }
+    jump_forward = zeros(nf);
+    jump_reverse = zeros(std::max<Index>(ni, 1));
+    for (int k = 0; k < 3; ++k) {
+      strain_base[k] = zeros3(n);
+      strain_gradient[k] = zeros3(n);
+    }
+    k_lsq = least_squares(dev, host, geom, cf, observe_k);
+    omega_lsq = least_squares(dev, host, geom, cf, observe_omega);
+    kinetic = F(dev, n);
+    dev.fill(kinetic.data(), float(o[17]), n);
+    omega = F(dev, n);
+    dev.fill(omega.data(), float(o[16]), n);
+    rho_cell = F(dev, n);
+    dev.fill(rho_cell.data(), float(s.density), n);
+    mu_cell = F(dev, n);
+    dev.fill(mu_cell.data(), float(s.viscosity), n);
+    boundary_k = zeros(nb);
+    boundary_omega = zeros(nb);
+    turbulence_kind = zeros_int(nb);
+    zero_mask = zeros_int(n);
+    transpose_stress = zeros3(nf);
+    std::vector<Index> specs(nb, 0), wall(nb, 0);
+    std::vector<float> ta(nb, 0.0f), tb(nb, 0.0f);
+    for (std::size_t p = 0; p < host.patches.size(); ++p) {
+      const auto& c = patch_condition[p];
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f)
+        wall[f - ni] = c.kind == PatchCondition::Wall ? 1 : 0;
+      if (c.kind == PatchCondition::Wall || c.kind == PatchCondition::Symmetry) continue;
+      KA_REQUIRE(c.turbulence >= 0 && c.turbulence <= 2, "FV missing turbulence specification: " + host.patches[p].name);
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+        const Index b = f - ni;
+        specs[b] = c.turbulence;
+        ta[b] = float(c.turbulence_a);
+        tb[b] = float(c.turbulence_b);
+        KA_REQUIRE(ta[b] >= 0 && tb[b] > 0 && std::isfinite(ta[b]) && std::isfinite(tb[b]),
+                   "FV invalid turbulence specification: " + host.patches[p].name);
+      }
+    }
+    turbulence_spec = to_device(dev, specs);
+    turbulence_a = to_device(dev, ta);
+    turbulence_b = to_device(dev, tb);
+    sst_wall = to_device(dev, wall);
+    CsrHost unit = pattern;
+    for (auto j : unit.diag) unit.val[j] = 1;
+    for (auto* eq : {&k_eq, &omega_eq}) {
+      for (F* v : {&eq->rhs, &eq->diagonal, &eq->consistent_inverse, &eq->limiter, &eq->rhs_low, &eq->diagonal_low,
+                   &eq->delta, &eq->residual, &eq->work_residual, &eq->work_absolute, &eq->work_scale})
+        *v = zeros(n);
+      eq->grad = zeros3(n);
+      eq->solver = std::make_unique<KrylovSolver>(dev, SolverKind::BicgstabJacobi, unit, AmgOptions{});
+    }
+    for (const char* name : {"mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity", "f1",
+                             "f2", "cdkw", "diffusivity_k", "diffusivity_omega", "production_k", "fixed_omega",
+                             "wall_production", "fixed_omega_mask", "selected_wall_face"})
+      sst_fields[name] = zeros(n);
+    for (const char* name : {"wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau",
+                             "wall_shear_x", "wall_shear_y", "wall_shear_z", "wall_shear_coefficient",
+                             "wall_face_omega", "wall_cell_omega", "face_production"})
+      sst_fields[name] = zeros(nb);
+    sst_fields["face_diffusivity_k"] = zeros(nf);
+    sst_fields["face_diffusivity_omega"] = zeros(nf);
+  }
   // The initial pressure operator (:553-572): stored-float M-matrix values.
   CsrHost initial = pattern;
   for (Index c = 0; c < n; ++c) {
@@ -820,6 +1150,7 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   }
   AmgOptions amg;
   amg.structure = Structure::SymmetricMMatrix;
+  if (sst) amg.pre_sweeps = amg.post_sweeps = 2;  // :573-578
   pressure_solver = std::make_unique<KrylovSolver>(dev, SolverKind::AmgPcg, initial, amg);
   const double total = std::accumulate(geom.cell_volume.begin(), geom.cell_volume.end(), 0.);
   Vec3 lo = host.points.front(), hi = lo;
@@ -842,8 +1173,10 @@ void SteadySolver::Impl::step() {
   refresh_time_step();
   update_boundaries();
   update_pressure();
+  update_turbulence();
   momentum_source();
   reconstruct_velocity();
+  add_transpose_stress();
   if (batch_momentum) solve_momentum_batched();
   for (int k = 0; k < 3 && !batch_momentum; ++k) {
     wall_terms(k);
@@ -901,6 +1234,7 @@ void SteadySolver::Impl::step() {
                 grad_correction.z, correction, velocity.x, velocity.y, velocity.z, pressure,
                 float(settings.pressure_relaxation)});
   }
+  solve_turbulence();
   // finish_step (:950-954)
   ++report.iteration;
   if (report.iteration <= 5 || report.iteration % 20 == 0) refresh_paired_scales();
@@ -912,8 +1246,10 @@ IterationReport SteadySolver::Impl::monitor() {
   refresh_time_step();
   update_boundaries();
   update_pressure();
+  update_turbulence();
   momentum_source();
   reconstruct_velocity();
+  add_transpose_stress();
   F* residuals[]{&residual_u, &residual_v, &residual_w};
   for (int k = 0; k < 3; ++k) {
     wall_terms(k);
@@ -956,6 +1292,7 @@ IterationReport SteadySolver::Impl::monitor() {
   report.continuity = {raw, scaled(raw, continuity_scale), scaled(raw, continuity_scale), continuity_scale};
   report.maximum_velocity = r[9];
   report.pseudo_time_step = time_step;
+  monitor_turbulence();
   return report;
 }
 
@@ -986,6 +1323,12 @@ ForceReport SteadySolver::Impl::forces(const std::vector<std::string>& patches, 
   dev.launch(KH_FV_WALL_PRESSURE_FORCE, nb,
              {mask, area.x, area.y, area.z, face_pressure, pf.x, pf.y, pf.z, ni, float(reference_pressure)});
   const auto &gu = momentum[0].grad, &gv = momentum[1].grad, &gw = momentum[2].grad;
+  if (sst) {  // viscous_face_forces (:1376-1380): the wall-function shear
+    update_turbulence();
+    dev.launch(KH_FV_SST_FORCE, nb,
+               {mask, area.x, area.y, area.z, sst_fields.at("wall_shear_x"), sst_fields.at("wall_shear_y"),
+                sst_fields.at("wall_shear_z"), vf.x, vf.y, vf.z, ni});
+  } else
   dev.launch(KH_FV_WALL_VISCOUS_FORCE, nb,
              {owner, mask, area.x, area.y, area.z, owner_to_face.x, owner_to_face.y, owner_to_face.z, velocity.x,
               velocity.y, velocity.z, gu.x, gu.y, gu.z, gv.x, gv.y, gv.z, gw.x, gw.y, gw.z, boundary_velocity.x,
@@ -1018,6 +1361,7 @@ void SteadySolver::iterate(int count) {
 }
 void SteadySolver::initialize(const std::vector<Vec3>& u, const std::vector<double>& p) {
   auto& s = *impl_;
+  KA_REQUIRE(!s.sst, "FV SST initialization requires initialize_fields with k and omega");
   const auto n = std::size_t(s.n);
   KA_REQUIRE(u.size() == n && p.size() == n, "FV initialization needs one velocity and pressure per cell");
   const double ref = s.reference >= 0 ? p[std::size_t(s.reference)] : 0;
@@ -1044,66 +1388,121 @@ void SteadySolver::initialize(const std::vector<Vec3>& u, const std::vector<doub
   s.dev.fill(s.consistent_inverse.data(), 0.0f, n);
   s.update_flux();
 }
-void SteadySolver::sst_update(const double* o, const float* const* in) {
-  auto& s = *impl_;
+void SteadySolver::Impl::sst_kernels(const double* o, const float* const* in,
+                                     const std::function<void()>& gradients) {
   KA_REQUIRE(std::isfinite(o[4]) && o[4] > 0 && o[13] > 0 && o[15] > 0 && o[16] > 0 && o[17] > 0 && o[18] > 0 &&
                  std::isfinite(o[6]) && o[6] >= 0 && !(o[10] != 0 && o[11] != 0),
              "SST: invalid options");
   for (int k = 0; k < 25; ++k) KA_REQUIRE(in[k] != nullptr, "SST: missing input array");
-  const Index n = s.n, nf = Index(s.area.x.size()), nb = s.nb, ni = s.ni;
-  if (s.sst_fields.empty()) {  // SstModel constructor (:60-99)
+  if (sst_fields.empty()) {  // SstModel constructor (:60-99) on a laminar solver's walls
     std::vector<Index> wall(nb, 0);
-    for (std::size_t p = 0; p < s.host.patches.size(); ++p)
-      if (s.patch_condition[p].kind == PatchCondition::Wall)
-        for (Index f = s.host.patches[p].start; f < s.host.patches[p].start + s.host.patches[p].size; ++f)
+    for (std::size_t p = 0; p < host.patches.size(); ++p)
+      if (patch_condition[p].kind == PatchCondition::Wall)
+        for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f)
           wall[f - ni] = 1;
-    s.sst_wall = to_device(s.dev, wall);
+    sst_wall = to_device(dev, wall);
     for (const char* name : {"mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity",
                              "f1", "f2", "cdkw", "diffusivity_k", "diffusivity_omega", "production_k",
                              "fixed_omega", "wall_production", "fixed_omega_mask", "selected_wall_face"})
-      s.sst_fields[name] = F(s.dev, n);
+      sst_fields[name] = F(dev, n);
     for (const char* name : {"wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau",
                              "wall_shear_x", "wall_shear_y", "wall_shear_z", "wall_shear_coefficient",
                              "wall_face_omega", "wall_cell_omega", "face_production"})
-      s.sst_fields[name] = F(s.dev, nb);
-    s.sst_fields["face_diffusivity_k"] = F(s.dev, nf);
-    s.sst_fields["face_diffusivity_omega"] = F(s.dev, nf);
+      sst_fields[name] = F(dev, nb);
+    sst_fields["face_diffusivity_k"] = F(dev, nf);
+    sst_fields["face_diffusivity_omega"] = F(dev, nf);
   }
-  auto d = [&](const char* name) { return s.sst_fields.at(name).data(); };
+  auto d = [&](const char* name) { return sst_fields.at(name).data(); };
   const float* const* g = in + 3;
-  const float *rho = in[12], *mu = in[13], *k = in[14], *omega = in[15];
+  const float *rho = in[12], *mu = in[13], *k = in[14], *om = in[15];
   const float* const* gk = in + 16;
   const float* const* go = in + 19;
   const float* const* wu = in + 22;
   const float k_floor = float(o[17]), omega_floor = float(o[16]);
-  auto& dev = s.dev;
   dev.launch(KH_SST_WALL, nb,
-             {s.owner, s.sst_wall, ni, s.area.x, s.area.y, s.area.z, in[0], in[1], in[2], wu[0], wu[1], wu[2],
-              s.wall_normal_distance, rho, mu, k, omega, d("wall_tau"), d("wall_shear_x"), d("wall_shear_y"),
+             {owner, sst_wall, ni, area.x, area.y, area.z, in[0], in[1], in[2], wu[0], wu[1], wu[2],
+              wall_normal_distance, rho, mu, k, om, d("wall_tau"), d("wall_shear_x"), d("wall_shear_y"),
               d("wall_shear_z"), d("wall_u_star"), d("wall_u_tau"), d("wall_y_plus"), d("wall_y_plus_utau"),
               d("face_production"), d("wall_shear_coefficient"), k_floor, omega_floor});
   dev.launch(KH_SST_WALL_OMEGA, nb,
-             {s.owner, s.sst_wall, ni, s.wall_normal_distance, rho, mu, d("wall_u_star"), d("wall_cell_omega"),
+             {owner, sst_wall, ni, wall_normal_distance, rho, mu, d("wall_u_star"), d("wall_cell_omega"),
               d("wall_face_omega")});
   dev.launch(KH_SST_WALL_CELLS, n,
-             {s.offsets, s.faces, s.sst_wall, s.wall_normal_distance, d("wall_cell_omega"), d("face_production"), ni,
+             {offsets, faces, sst_wall, wall_normal_distance, d("wall_cell_omega"), d("face_production"), ni,
               d("fixed_omega_mask"), d("selected_wall_face"), d("fixed_omega"), d("wall_production")});
+  if (gradients) gradients();
   dev.launch(KH_SST_RATES, n, {g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], d("strain"), d("vorticity")});
   dev.launch(KH_SST_BLENDING_VISCOSITY, n,
-             {rho, mu, k, omega, s.wall_distance, gk[0], gk[1], gk[2], go[0], go[1], go[2], d("strain"),
-              d("vorticity"), d("f1"), d("f2"), d("cdkw"), d("mu_t"), d("diffusivity_k"), d("diffusivity_omega"),
-              float(o[13]), k_floor, omega_floor, int(o[7] != 0), int(o[9] != 0), float(o[18])});
+             {rho, mu, k, om, wall_distance, gk[0], gk[1], gk[2], go[0], go[1], go[2], d("strain"), d("vorticity"),
+              d("f1"), d("f2"), d("cdkw"), d("mu_t"), d("diffusivity_k"), d("diffusivity_omega"), float(o[13]),
+              k_floor, omega_floor, int(o[7] != 0), int(o[9] != 0), float(o[18])});
   const int production_mode = o[10] != 0 ? 1 : (o[11] != 0 ? 2 : 0);
   dev.launch(KH_SST_SOURCES, n,
-             {rho, mu, k, omega, d("strain"), d("vorticity"), d("f1"), d("cdkw"), d("mu_t"), d("fixed_omega_mask"),
+             {rho, mu, k, om, d("strain"), d("vorticity"), d("f1"), d("cdkw"), d("mu_t"), d("fixed_omega_mask"),
               d("wall_production"), d("k_source"), d("k_sink"), d("omega_source"), d("omega_sink"),
               d("production_k"), int(o[7] != 0), int(o[8] != 0), production_mode, int(o[3] != 0), int(o[5] != 0),
               int(o[12] != 0), float(o[4]), int(o[14] != 0), float(o[15]), k_floor, omega_floor, float(o[6])});
   if (o[0] != 0)
     dev.launch(KH_SST_SUSTAIN, n, {rho, d("f1"), d("k_source"), d("omega_source"), float(o[1]), float(o[2])});
   dev.launch(KH_SST_FACE_DIFFUSION, nf,
-             {s.owner, s.neighbour, s.weight, s.sst_wall, mu, d("diffusivity_k"), d("diffusivity_omega"), ni,
+             {owner, neighbour, weight, sst_wall, mu, d("diffusivity_k"), d("diffusivity_omega"), ni,
               d("face_diffusivity_k"), d("face_diffusivity_omega")});
+}
+void SteadySolver::sst_update(const double* o, const float* const* in) { impl_->sst_kernels(o, in, {}); }
+
+// SteadySolver::initialize_fields (:1493-1524).
+void SteadySolver::Impl::initialize_fields(const std::vector<Vec3>& u, const std::vector<double>& p,
+                                           const std::vector<double>& k, const std::vector<double>& w) {
+  const auto count = std::size_t(n);
+  if (paired)  // reset_paired (:977-988)
+    for (auto* v : {&velocity_lo.x, &velocity_lo.y, &velocity_lo.z, &pressure_gradient_lo.x,
+                    &pressure_gradient_lo.y, &pressure_gradient_lo.z, &pressure_lo, &mass_lo})
+      dev.fill(v->data(), 0.0f, v->size());
+  KA_REQUIRE(u.size() == count && p.size() == count, "FV initialization field size mismatch");
+  KA_REQUIRE(sst ? (k.size() == count && w.size() == count) : (k.empty() && w.empty()),
+             "FV turbulence initialization size mismatch");
+  const double ref = reference >= 0 ? p[std::size_t(reference)] : 0;
+  std::vector<float> ux(count), uy(count), uz(count), pf(count), kf(sst ? count : 0), wf(sst ? count : 0);
+  for (std::size_t c = 0; c < count; ++c) {
+    ux[c] = float(u[c].x);
+    uy[c] = float(u[c].y);
+    uz[c] = float(u[c].z);
+    pf[c] = float(p[c] - ref);
+    KA_REQUIRE(std::isfinite(pf[c]), "FV nonfinite pressure initialization");
+    if (sst) {
+      KA_REQUIRE(std::isfinite(k[c]) && k[c] >= 0 && std::isfinite(w[c]) && w[c] > 0,
+                 "FV invalid turbulence initialization");
+      kf[c] = float(k[c]);
+      wf[c] = float(w[c]);
+      KA_REQUIRE(std::isfinite(kf[c]) && std::isfinite(wf[c]), "FV turbulence initialization exceeds float storage");
+    }
+  }
+  velocity.x.upload(ux);
+  velocity.y.upload(uy);
+  velocity.z.upload(uz);
+  pressure.upload(pf);
+  if (sst) {
+    kinetic.upload(kf);
+    omega.upload(wf);
+  }
+  dev.fill(physical_diagonal.data(), 1.0f, count);
+  dev.fill(consistent_inverse.data(), 0.0f, count);
+  update_boundaries();
+  update_pressure();
+  update_flux();
+  update_turbulence();
+  for (int pass = 0; pass < 3; ++pass) {
+    monitor();
+    refresh_paired_scales();
+    update_flux();
+    dev.launch(KH_FV_CONTINUITY, n, {offsets, faces, sign, mass, provisional_continuity});
+    update_boundaries();
+    update_pressure();
+  }
+}
+void SteadySolver::initialize_fields(const std::vector<Vec3>& u, const std::vector<double>& p,
+                                     const std::vector<double>& k, const std::vector<double>& w) {
+  impl_->initialize_fields(u, p, k, w);
 }
 IterationReport SteadySolver::monitor() { return impl_->monitor(); }
 ForceReport SteadySolver::forces(const std::vector<std::string>& patches, double reference_pressure) {
@@ -1137,7 +1536,10 @@ const float* SteadySolver::field(const std::string& name, std::size_t* size) con
       {"wall_normal_distance", &s.wall_normal_distance}, {"wall_distance", &s.wall_distance},
       {"velocity_lsq_inverse", &s.velocity_lsq.inverse},
       {"pressure_lsq_inverse", &s.pressure_lsq.inverse}, {"velocity_lsq_boundary_x", &s.velocity_lsq.bx},
-      {"pressure_lsq_boundary_y", &s.pressure_lsq.by}};
+      {"pressure_lsq_boundary_y", &s.pressure_lsq.by}, {"k", &s.kinetic}, {"omega", &s.omega},
+      {"viscosity", &s.viscosity}, {"boundary_k", &s.boundary_k}, {"boundary_omega", &s.boundary_omega},
+      {"source_x", &s.source.x}, {"strain_dudy", &s.strain_gradient[0].y}, {"k_grad_y", &s.k_eq.grad.y},
+      {"omega_rhs", &s.omega_eq.rhs}};
   const auto it = table.find(name);
   const F* array = it != table.end() ? it->second : nullptr;
   if (!array) {

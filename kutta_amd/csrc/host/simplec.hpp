@@ -41,6 +41,28 @@ struct PatchCondition {
   // optional inward direction (zero: along each face's inward normal).
   double mass_flow = 0;
   Vec3 direction{};
+  bool has_direction = false;  // the optional direction is given
+  // SST inflow turbulence at inlets and outlets (fv/steady_solver.hpp:28-35):
+  // -1 none, 0 TurbulenceValues{k, omega}, 1 IntensityLength, 2
+  // IntensityViscosityRatio; a and b are the pair's two values.
+  int turbulence = -1;
+  double turbulence_a = 0, turbulence_b = 0;
+};
+
+// SstSettings (fv/steady_solver.hpp:102-118) with SstOptions
+// (turbulence/sst.hpp:46-78, no intermittency) as model[19]: has_ambient,
+// ambient k, ambient omega, production_limiter, production_clip,
+// limit_omega_production, omega_production_molecular_ratio, low_re,
+// gamma_set Menter2003, viscosity limiter Vorticity, kato_launder,
+// vorticity_production, wall_production, cdkw_floor, limit_cross_diffusion,
+// cross_diffusion_clip, omega_floor, k_floor, max_viscosity_ratio.
+struct SstSettings {
+  double model[19] = {0, 0, 0, 1, 10, 1, 0, 0, 0, 0, 0, 0, 1, 1e-10, 1, 1e6, 1e-20, 1e-14, 1e5};
+  bool k_second_order = true, omega_second_order = true;
+  double k_relaxation = .75, omega_relaxation = .75;
+  SolveControl linear{1e-3, 1e-20, 100, 8};
+  double steady_tolerance = 1e-6;
+  bool compensated_transport = true, pseudo_transient = false;
 };
 
 // fv/steady_solver.hpp:193-271, the members this solver honours; the others
@@ -58,6 +80,9 @@ struct SolverSettings {
   // Solve the three momentum components in one batched launch sequence when
   // that is bitwise the reference's order (no reference counterpart).
   bool batched_momentum = true;
+  double pseudo_time_step = 0;  // a fixed pseudo time step; 0 is automatic
+  bool sst = false;             // k-omega SST with sst_settings
+  SstSettings sst_settings;
 };
 
 // fv/steady_solver.hpp:273-284
@@ -69,6 +94,8 @@ struct IterationReport {
   ResidualValue u, v, w, continuity;
   double corrected_mass_imbalance = 0, maximum_velocity = 0, pseudo_time_step = 0;
   SolveStats momentum[3], pressure;
+  ResidualValue k, omega;
+  SolveStats turbulence[2];
 };
 
 // fv/steady_solver.hpp:286-292 (FP64 sums of the per-face wall forces).
@@ -91,6 +118,11 @@ class SteadySolver {
   // geometry().cell_centroid); a closed domain's pressure is shifted so the
   // reference cell holds zero.
   void initialize(const std::vector<Vec3>& velocity, const std::vector<double>& pressure);
+  // SteadySolver::initialize_fields (fv/steady_solver.cpp:1493-1524) in the
+  // solver's cell order: also the SST k and omega (empty without SST), then
+  // three monitor passes that establish the momentum coefficients.
+  void initialize_fields(const std::vector<Vec3>& velocity, const std::vector<double>& pressure,
+                         const std::vector<double>& k, const std::vector<double>& omega);
   IterationReport monitor();
   const IterationReport& report() const;
   // Pressure and viscous force of the fluid on the named wall patches

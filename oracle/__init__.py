@@ -466,33 +466,40 @@ def ref_fv_each(name: str, n: int, *args) -> None:
         raise OracleError(f"reference build lacks fv kernel {name}")
 
 
+def _simplec_lib():
+    """liboracle with the SIMPLEC prototypes declared."""
+    L = lib()
+    vp, i32 = C.c_void_p, C.c_int
+    L.oracle_simplec_create.restype = vp
+    L.oracle_simplec_create.argtypes = [i32, i32, i32, vp, vp, vp]
+    L.oracle_simplec_last_error.restype = C.c_char_p
+    L.oracle_simplec_free.argtypes = [vp]
+    L.oracle_simplec_iterate.argtypes = [vp, i32]
+    L.oracle_simplec_report.argtypes = [vp, i32, vp]
+    L.oracle_simplec_array.argtypes = [vp, C.c_char_p, vp]
+    L.oracle_simplec_array.restype = C.c_long
+    L.oracle_simplec_reference.argtypes = [vp]
+    L.oracle_simplec_centroids.argtypes = [vp, vp]
+    L.oracle_simplec_create_open.restype = vp
+    L.oracle_simplec_create_open.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
+    L.oracle_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
+    return L
+
+
 class SteadySolver:
     """The serial restatement of the laminar closed-domain SIMPLEC solver
     (fv/steady_solver.cpp; oracle/simplec.cpp) on make_box_hex(nx, ny, nz).
     kinds/wall/settings are packed as for the product (kutta_amd)."""
 
-    REPORT_SIZE = 26
+    REPORT_SIZE = 36
 
     def __init__(self, nx, ny, nz, kinds, wall, settings, pressure=None, total=None, extent=None, profiled=None,
                  profile=None):
         """With `pressure` and `total` (kutta_amd.pack_open_conditions) the
         patches may also be velocity inlets and pressure outlets, and `extent`
         sets the box [0, extent] (the unit cube by default)."""
-        L = lib()
+        L = _simplec_lib()
         vp, i32 = C.c_void_p, C.c_int
-        L.oracle_simplec_create.restype = vp
-        L.oracle_simplec_create.argtypes = [i32, i32, i32, vp, vp, vp]
-        L.oracle_simplec_last_error.restype = C.c_char_p
-        L.oracle_simplec_free.argtypes = [vp]
-        L.oracle_simplec_iterate.argtypes = [vp, i32]
-        L.oracle_simplec_report.argtypes = [vp, i32, vp]
-        L.oracle_simplec_array.argtypes = [vp, C.c_char_p, vp]
-        L.oracle_simplec_array.restype = C.c_long
-        L.oracle_simplec_reference.argtypes = [vp]
-        L.oracle_simplec_centroids.argtypes = [vp, vp]
-        L.oracle_simplec_create_open.restype = vp
-        L.oracle_simplec_create_open.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
-        L.oracle_simplec_initialize.argtypes = [vp, vp, vp, vp, vp]
         box = (C.c_double * 3)(*[float(e) for e in (extent or (1.0, 1.0, 1.0))])
         self._keep = (kinds, wall, settings, pressure, total, box, profiled, profile)
         if profiled is not None:
@@ -514,6 +521,50 @@ class SteadySolver:
             raise OracleError(L.oracle_simplec_last_error().decode())
         self.handle = h
         self.cells = nx * ny * nz
+
+    @classmethod
+    def from_mesh(cls, mesh, packed, settings, sst=None):
+        """On a general mesh (kutta_amd.meshes.PolyMeshArrays) with conditions
+        packed by kutta_amd.pack_mesh_conditions, settings[12] and, for SST,
+        sst[29] (kutta_amd.SstSettings.packed())."""
+        L = _simplec_lib()
+        vp, i32 = C.c_void_p, C.c_int
+        L.oracle_simplec_create_mesh.restype = vp
+        L.oracle_simplec_create_mesh.argtypes = [i32, vp, i32, vp, vp, vp, i32, vp, i32, i32, C.c_char_p, vp, vp,
+                                                 vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_simplec_last_error.restype = C.c_char_p
+        self = cls.__new__(cls)
+        arrays = [np.ascontiguousarray(mesh.points, np.float64), np.ascontiguousarray(mesh.face_offsets, np.int32),
+                  np.ascontiguousarray(mesh.face_nodes, np.int32), np.ascontiguousarray(mesh.owner, np.int32),
+                  np.ascontiguousarray(mesh.neighbour, np.int32)]
+        starts = np.array([p[1] for p in mesh.patches], np.int32)
+        sizes = np.array([p[2] for p in mesh.patches], np.int32)
+        self._keep = (arrays, starts, sizes, packed, settings, sst)
+        h = L.oracle_simplec_create_mesh(
+            len(mesh.points), arrays[0].ctypes.data, mesh.n_faces, arrays[1].ctypes.data, arrays[2].ctypes.data,
+            arrays[3].ctypes.data, mesh.n_internal, arrays[4].ctypes.data if mesh.n_internal else None,
+            mesh.n_cells, len(mesh.patches), "\n".join(p[0] for p in mesh.patches).encode(), starts.ctypes.data,
+            sizes.ctypes.data, *[C.cast(packed[k], vp) for k in ("kinds", "velocity", "pressure", "total", "flags",
+                                                                  "turbulence", "turbulence_ab")],
+            C.cast(settings, vp), C.cast(sst, vp) if sst is not None else None)
+        if not h:
+            raise OracleError(L.oracle_simplec_last_error().decode())
+        self.handle = h
+        self.cells = mesh.n_cells
+        return self
+
+    def initialize_fields(self, u, v, w, p, k=None, omega=None):
+        """SteadySolver::initialize_fields with per-cell float64 arrays in the
+        solver's cell order (k and omega for SST)."""
+        L = lib()
+        L.oracle_simplec_initialize_fields.argtypes = [C.c_void_p] * 7
+        arrays = [np.ascontiguousarray(a, np.float64) for a in (u, v, w, p)]
+        extra = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (k, omega)]
+        if any(a.shape != (self.cells,) for a in arrays + [e for e in extra if e is not None]):
+            raise OracleError("initialize_fields needs one value per cell")
+        if L.oracle_simplec_initialize_fields(self.handle, *[a.ctypes.data for a in arrays],
+                                              *[None if e is None else e.ctypes.data for e in extra]) != 0:
+            raise OracleError(L.oracle_simplec_last_error().decode())
 
     def close(self):
         if self.handle:

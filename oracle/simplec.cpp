@@ -107,6 +107,28 @@ struct Condition {
   // the faces): kg/s into the domain and an optional inward direction.
   double mass_flow = 0;
   Vec3 direction{};
+  bool has_direction = false;  // the optional direction is given
+  // SST inflow turbulence (fv/steady_solver.hpp:28-35) at inlets and
+  // outlets: -1 none, 0 k and omega, 1 intensity and length, 2 intensity and
+  // viscosity ratio; a and b are the pair's values.
+  int turbulence = -1;
+  double turbulence_a = 0, turbulence_b = 0;
+};
+
+// SstSettings (fv/steady_solver.hpp:102-118) with SstOptions
+// (turbulence/sst.hpp:46-78) as model[19]: has_ambient, ambient k, ambient
+// omega, production_limiter, production_clip, limit_omega_production,
+// omega_production_molecular_ratio, low_re, gamma_set Menter2003, viscosity
+// limiter Vorticity, kato_launder, vorticity_production, wall_production,
+// cdkw_floor, limit_cross_diffusion, cross_diffusion_clip, omega_floor,
+// k_floor, max_viscosity_ratio.
+struct SstSettings {
+  double model[19] = {0, 0, 0, 1, 10, 1, 0, 0, 0, 0, 0, 0, 1, 1e-10, 1, 1e6, 1e-20, 1e-14, 1e5};
+  bool k_second_order = true, omega_second_order = true;
+  double k_relaxation = .75, omega_relaxation = .75;
+  SolveControl linear{1e-3, 1e-20, 100, 8};
+  double steady_tolerance = 1e-6;
+  bool compensated_transport = true, pseudo_transient = false;
 };
 
 struct Settings {
@@ -119,6 +141,9 @@ struct Settings {
   Vec3 reference_point{};
   SolveControl momentum_linear{1e-2, 1e-20, 100, 8};
   SolveControl pressure_linear{1e-3, 1e-20, 300, 8};
+  double pseudo_time_step = 0;  // fixed pseudo time step; 0 is automatic
+  bool sst = false;
+  SstSettings sst_settings;
 };
 
 struct ResidualValue {
@@ -135,6 +160,8 @@ struct Report {
   ResidualValue u, v, w, continuity;
   double corrected_mass_imbalance = 0, maximum_velocity = 0, pseudo_time_step = 0;
   SolveStats momentum[3], pressure;
+  ResidualValue k, omega;
+  SolveStats turbulence[2];
 };
 
 namespace {
@@ -390,6 +417,47 @@ struct TransportEquation {
   std::unique_ptr<Solver> solver;
 };
 
+// A turbulence transport equation (fv/transport.hpp:50-106) with the
+// compensated solve (enable_compensated_solve, transport.cpp:51-55) and the
+// steady-stopping work arrays (set_steady_tolerance, :64-69).
+struct ScalarEquation {
+  V rhs, diagonal, consistent_inverse, limiter, rhs_low, diagonal_low, delta, residual;
+  V work_residual, work_absolute, work_scale;
+  Vec3f grad;
+  std::unique_ptr<Solver> solver;
+};
+
+// face_jump_least_squares (fv/operators.cpp:234-253): per cell-face entry the
+// face-centroid row r/|r|^2 and the six-plane inverse of sum r r^T/|r|^2.
+void face_jump_fit(const PolyMesh& m, const Geometry& g, Vec3f& delta, V& inverse) {
+  const auto cf = build_cell_faces(m);
+  std::vector<Vec3> rows;
+  const auto n = std::size_t(m.n_cells);
+  inverse.assign(6 * n, 0.0f);
+  for (Index c = 0; c < m.n_cells; ++c) {
+    double a[6]{};
+    for (Index j = cf.offsets[c]; j < cf.offsets[c + 1]; ++j) {
+      const Vec3 r = g.face_centroid[cf.faces[j]] - g.cell_centroid[c];
+      const double w = 1 / dot(r, r);
+      rows.push_back(r * w);
+      a[0] += w * r.x * r.x;
+      a[1] += w * r.x * r.y;
+      a[2] += w * r.x * r.z;
+      a[3] += w * r.y * r.y;
+      a[4] += w * r.y * r.z;
+      a[5] += w * r.z * r.z;
+    }
+    const double det = a[0] * (a[3] * a[5] - a[4] * a[4]) - a[1] * (a[1] * a[5] - a[4] * a[2]) +
+                       a[2] * (a[1] * a[4] - a[3] * a[2]);
+    ORACLE_REQUIRE(det > 1e-14, "FV face-jump fit is rank deficient");
+    const double inv[6]{(a[3] * a[5] - a[4] * a[4]) / det, (a[2] * a[4] - a[1] * a[5]) / det,
+                        (a[1] * a[4] - a[2] * a[3]) / det, (a[0] * a[5] - a[2] * a[2]) / det,
+                        (a[1] * a[2] - a[0] * a[4]) / det, (a[0] * a[3] - a[1] * a[1]) / det};
+    for (int k = 0; k < 6; ++k) inverse[c + k * n] = float(inv[k]);
+  }
+  to_float(delta, rows);
+}
+
 }  // namespace
 
 class Steady {
@@ -404,6 +472,10 @@ class Steady {
   // SteadySolver::initialize (:1480-1491) from per-cell values in the
   // solver's cell order.
   void initialize(const std::vector<Vec3>& u, const std::vector<double>& p);
+  // SteadySolver::initialize_fields (:1493-1524) in the solver's cell order;
+  // k and omega are required with SST and ignored without.
+  void initialize_fields(const std::vector<Vec3>& u, const std::vector<double>& p, const std::vector<double>& k,
+                         const std::vector<double>& omega);
   const Report& report() const { return report_; }
   ForceReport forces(const std::vector<std::string>& patches, double reference_pressure);
   // Named state arrays, for parity checks.
@@ -435,6 +507,17 @@ class Steady {
   void refresh_paired_scales();
   void finish_step();
   TransportSettings transport() const;
+  // SST k-omega in the step (steady_solver.cpp:638-818).
+  void sst_kernels(const double* options, const float* const* in, const std::function<void()>& gradients);
+  void reconstruct_strain();
+  void turbulence_boundaries();
+  void update_turbulence(bool velocity_changed = true);
+  void add_transpose_stress();
+  void assemble_turbulence(bool is_omega);
+  void physical_residual(ScalarEquation& eq, const V& field, V& result, const VI* excluded);
+  SolveStats solve_scalar(ScalarEquation& eq, V& field, const VI* excluded);
+  void solve_turbulence();
+  void monitor_turbulence();
   struct Slack {
     float velocity, pressure, flux_per_area;
   };
@@ -454,6 +537,14 @@ class Steady {
   Index reference_ = -1;
   std::map<std::string, V> sst_fields_;
   bool paired = false;
+  // SST state (steady_solver.cpp:111-127, 484-542).
+  bool sst = false;
+  LeastSquares strain_lsq, k_lsq, omega_lsq;
+  Vec3f jump_delta, strain_base[3], strain_gradient[3], transpose_stress;
+  V jump_inverse, jump_forward, jump_reverse;
+  V kinetic, omega, rho_cell, mu_cell, boundary_k, boundary_omega, turbulence_a, turbulence_b;
+  VI turbulence_spec, turbulence_kind, zero_mask, sst_wall, fixed_omega_mask, selected_wall_face;
+  ScalarEquation k_eq, omega_eq;
 
   Vec3f velocity, face_velocity, body, source, boundary_velocity, prescribed_velocity;
   Vec3f limited_gradient[3], grad_p, grad_correction, absolute_residual, fluent_scale;
@@ -484,6 +575,7 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
                      s.monitor_interval > 0 && std::isfinite(s.paired_state_slack_ulps) &&
                      s.paired_state_slack_ulps > 0,
                  "FV invalid steady solver settings");
+  ORACLE_REQUIRE(std::isfinite(s.pseudo_time_step) && s.pseudo_time_step >= 0, "FV invalid steady solver settings");
   ORACLE_REQUIRE(conditions.size() == host.patches.size(), "FV supply exactly one condition per patch");
   std::vector<bool> patch_is_wall;
   bool outlet = false;
@@ -569,7 +661,8 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
       for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
         inlet_area[p] += norm(geom.face_area[f]);
         const Vec3 dir = conditions[p].direction;
-        if (norm(dir) > 0) inlet_projected[p] -= dot(dir * (1 / norm(dir)), geom.face_area[f]);
+        if (conditions[p].has_direction && norm(dir) > 0)
+          inlet_projected[p] -= dot(dir * (1 / norm(dir)), geom.face_area[f]);
       }
   double pmin = std::numeric_limits<double>::infinity(), pmax = -pmin;
   for (std::size_t p = 0; p < host.patches.size(); ++p)
@@ -581,8 +674,10 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
         const Vec3 a = geom.face_area[f];
         ORACLE_REQUIRE(std::isfinite(c.mass_flow) && c.mass_flow >= 0 && inlet_area[p] > 0 && settings.density > 0,
                        "mass-flow-inlet: nonnegative mass flow and positive area/density required");
-        if (norm(c.direction) > 0) {
-          ORACLE_REQUIRE(inlet_projected[p] > 0, "mass-flow-inlet: nonzero inward direction required");
+        if (c.has_direction) {
+          ORACLE_REQUIRE(std::isfinite(c.direction.x) && std::isfinite(c.direction.y) &&
+                             std::isfinite(c.direction.z) && norm(c.direction) > 0 && inlet_projected[p] > 0,
+                         "mass-flow-inlet: nonzero inward direction required");
           const Vec3 d = c.direction * (1 / norm(c.direction));
           ORACLE_REQUIRE(dot(d, a) < 0, "mass-flow-inlet: direction must enter every face");
           velocities[b] = d * (c.mass_flow / (settings.density * inlet_projected[p]));
@@ -669,6 +764,86 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
     for (Index c = 0; c < mesh.n_cells; ++c)
       ORACLE_REQUIRE(quality[c] > .05, "SIMPLEC oracle: poor-quality cells are not covered");
   }
+  sst = s.sst;
+  if (sst) {  // :471-542
+    const auto& t = s.sst_settings;
+    const double* o = t.model;
+    ORACLE_REQUIRE(t.k_relaxation > 0 && t.k_relaxation <= 1 && t.omega_relaxation > 0 && t.omega_relaxation <= 1,
+                   "FV invalid SST relaxation");
+    ORACLE_REQUIRE(std::isfinite(o[4]) && o[4] > 0 && o[13] > 0 && o[15] > 0 && o[16] > 0 && o[17] > 0 &&
+                       o[18] > 0 && std::isfinite(o[6]) && o[6] >= 0 && !(o[10] != 0 && o[11] != 0),
+                   "SST: invalid options");
+    ORACLE_REQUIRE(std::isfinite(t.steady_tolerance) && t.steady_tolerance >= 0, "FV invalid scalar steady tolerance");
+    std::vector<Observation> strain_rows(nb), independent_k(nb), independent_omega(nb);
+    for (std::size_t b = 0; b < nb; ++b) {
+      strain_rows[b] = kinds[b] != 0 ? Observation::FaceValue : Observation::Omitted;
+      independent_k[b] = kinds[b] == 1 ? Observation::FaceValue : Observation::Omitted;
+      independent_omega[b] = (kinds[b] == 0 || kinds[b] == 1) ? Observation::FaceValue : Observation::Omitted;
+      if (kinds[b] == 3) independent_k[b] = independent_omega[b] = Observation::NormalFootValue;
+    }
+    strain_lsq = boundary_least_squares(host, geom, strain_rows);
+    face_jump_fit(host, geom, jump_delta, jump_inverse);
+    jump_forward.assign(nf, 0.0f);
+    jump_reverse.assign(mesh.n_internal, 0.0f);
+    for (int k = 0; k < 3; ++k) {
+      strain_base[k].assign(n);
+      strain_gradient[k].assign(n);
+    }
+    k_lsq = boundary_least_squares(host, geom, independent_k);
+    omega_lsq = boundary_least_squares(host, geom, independent_omega);
+    kinetic.assign(n, float(o[17]));
+    omega.assign(n, float(o[16]));
+    rho_cell.assign(n, float(s.density));
+    mu_cell.assign(n, float(s.viscosity));
+    boundary_k.assign(nb, 0.0f);
+    boundary_omega.assign(nb, 0.0f);
+    turbulence_kind.assign(nb, 0);
+    zero_mask.assign(n, 0);
+    transpose_stress.assign(nf);
+    turbulence_spec.assign(nb, 0);
+    turbulence_a.assign(nb, 0.0f);
+    turbulence_b.assign(nb, 0.0f);
+    sst_wall.assign(nb, 0);
+    for (std::size_t p = 0; p < host.patches.size(); ++p) {
+      const auto& c = conditions[p];
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f)
+        sst_wall[f - mesh.n_internal] = c.kind == 0 ? 1 : 0;
+      if (c.kind == 0 || c.kind == 3) continue;
+      ORACLE_REQUIRE(c.turbulence >= 0 && c.turbulence <= 2,
+                     "FV missing turbulence specification: " + host.patches[p].name);
+      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) {
+        const auto b = f - mesh.n_internal;
+        turbulence_spec[b] = c.turbulence;
+        turbulence_a[b] = float(c.turbulence_a);
+        turbulence_b[b] = float(c.turbulence_b);
+        ORACLE_REQUIRE(turbulence_a[b] >= 0 && turbulence_b[b] > 0 && std::isfinite(turbulence_a[b]) &&
+                           std::isfinite(turbulence_b[b]),
+                       "FV invalid turbulence specification: " + host.patches[p].name);
+      }
+    }
+    CsrHost unit = stencil.pattern;
+    unit.val.assign(unit.col.size(), 0.0);
+    for (auto j : unit.diag) unit.val[j] = 1;
+    for (auto* eq : {&k_eq, &omega_eq}) {
+      for (V* v : {&eq->rhs, &eq->diagonal, &eq->consistent_inverse, &eq->limiter, &eq->rhs_low, &eq->diagonal_low,
+                   &eq->delta, &eq->residual, &eq->work_residual, &eq->work_absolute, &eq->work_scale})
+        v->assign(n, 0.0f);
+      eq->grad.assign(n);
+      eq->solver = std::make_unique<Solver>(SolverKind::BicgstabJacobi, unit, AmgOptions{});
+    }
+    for (const char* name : {"mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity",
+                             "f1", "f2", "cdkw", "diffusivity_k", "diffusivity_omega", "production_k",
+                             "fixed_omega", "wall_production"})
+      sst_fields_[name].assign(n, 0.0f);
+    for (const char* name : {"wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau",
+                             "wall_shear_x", "wall_shear_y", "wall_shear_z", "wall_shear_coefficient",
+                             "wall_face_omega", "wall_cell_omega", "face_production"})
+      sst_fields_[name].assign(nb, 0.0f);
+    sst_fields_["face_diffusivity_k"].assign(nf, 0.0f);
+    sst_fields_["face_diffusivity_omega"].assign(nf, 0.0f);
+    fixed_omega_mask.assign(n, 0);
+    selected_wall_face.assign(n, 0);
+  }
   // The initial pressure operator (:553-572).
   CsrHost initial = stencil.pattern;
   initial.val.assign(initial.col.size(), 0.0);
@@ -696,6 +871,7 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
   }
   AmgOptions amg;
   amg.structure = Structure::SymmetricMMatrix;
+  if (sst) amg.pre_sweeps = amg.post_sweeps = 2;  // :573-578
   pressure_solver = std::make_unique<Solver>(SolverKind::AmgPcg, initial, amg);
   const double volume = std::accumulate(geom.cell_volume.begin(), geom.cell_volume.end(), 0.);
   Vec3 lo = host.points.front(), hi = lo;
@@ -713,6 +889,7 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
 }
 
 void Steady::initialize(const std::vector<Vec3>& u, const std::vector<double>& p) {
+  ORACLE_REQUIRE(!sst, "FV SST initialization requires initialize_fields with k and omega");
   const auto n = std::size_t(mesh.n_cells);
   ORACLE_REQUIRE(u.size() == n && p.size() == n, "FV initialization needs one value per cell");
   if (paired) {  // reset_paired (:977-988)
@@ -739,11 +916,13 @@ void Steady::initialize(const std::vector<Vec3>& u, const std::vector<double>& p
 // :843-851 (segregated: the maximum cell speed is the velocity scale).
 void Steady::update_time(double maximum_speed) {
   velocity_scale = std::max({.1 * bc_speed, maximum_speed, pressure_speed, 1e-10});
-  time_step = float(.3 * length * settings.pseudo_time_scale / velocity_scale);
+  time_step = settings.pseudo_time_step > 0 ? float(settings.pseudo_time_step)
+                                            : float(.3 * length * settings.pseudo_time_scale / velocity_scale);
 }
 
-// :857-873, automatic step, no ramp, Barth-Jespersen limiter.
+// :857-873, no ramp, Barth-Jespersen limiter: a fixed step is not refreshed.
 void Steady::refresh_time_step() {
+  if (settings.pseudo_time_step > 0) return;
   fv("fv_speed", mesh.n_cells, velocity.x.data(), velocity.y.data(), velocity.z.data(), speed.data());
   const auto maximum = max_abs(speed.data(), speed.size());
   update_time(maximum);
@@ -873,12 +1052,14 @@ TransportSettings Steady::transport() const {
 }
 
 // :765-776 (laminar: the coefficient argument is an unused placeholder).
+// SST: the wall-function shear coefficient acts at walls.
 void Steady::wall_terms(int k) {
+  const V& coefficient_of_wall = sst ? sst_fields_.at("wall_shear_coefficient") : prescribed_pressure;
   fv("fv_boundary_momentum", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), boundary_kind.data(),
-     mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), prescribed_pressure.data(), viscosity.data(),
+     mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), coefficient_of_wall.data(), viscosity.data(),
      stencil.diffusion.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(),
      boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(), wall_diagonal.data(),
-     wall_rhs.data(), mesh.n_internal, k, 0);
+     wall_rhs.data(), mesh.n_internal, k, sst ? 1 : 0);
 }
 
 // fv/transport.cpp:83-121 with limit_gradient (fv/operators.cpp:345-362) and
@@ -969,10 +1150,12 @@ void Steady::step() {
   refresh_time_step();
   update_boundaries();
   update_pressure();
+  update_turbulence();
   fv("fv_momentum_source", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
      mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), face_pressure.data(), mesh.volume.data(),
      body.x.data(), body.y.data(), body.z.data(), source.x.data(), source.y.data(), source.z.data());
   reconstruct_velocity();
+  add_transpose_stress();
   auto t = transport();
   for (int k = 0; k < 3; ++k) {
     t.limiter_epsilon2 = limiter_epsilon2[k];
@@ -1034,6 +1217,7 @@ void Steady::step() {
        correction.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(), pressure.data(),
        float(settings.pressure_relaxation));
   }
+  solve_turbulence();
   finish_step();
 }
 
@@ -1042,10 +1226,12 @@ Report Steady::monitor() {
   refresh_time_step();
   update_boundaries();
   update_pressure();
+  update_turbulence();
   fv("fv_momentum_source", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
      mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), face_pressure.data(), mesh.volume.data(),
      body.x.data(), body.y.data(), body.z.data(), source.x.data(), source.y.data(), source.z.data());
   reconstruct_velocity();
+  add_transpose_stress();
   auto t = transport();
   V* residuals[]{&residual_u, &residual_v, &residual_w};
   for (int k = 0; k < 3; ++k) {
@@ -1089,6 +1275,7 @@ Report Steady::monitor() {
   report_.continuity = {raw, scaled(raw, continuity_scale), scaled(raw, continuity_scale), continuity_scale};
   report_.maximum_velocity = max_abs(speed.data(), n);
   report_.pseudo_time_step = time_step;
+  monitor_turbulence();
   return report_;
 }
 
@@ -1125,6 +1312,12 @@ ForceReport Steady::forces(const std::vector<std::string>& patches, double refer
      mesh.area.z.data(), face_pressure.data(), pf.x.data(), pf.y.data(), pf.z.data(), mesh.n_internal,
      float(reference_pressure));
   const auto &gu = momentum[0].grad, &gv = momentum[1].grad, &gw = momentum[2].grad;
+  if (sst) {  // :1376-1380
+    update_turbulence();
+    fv("fv_sst_force", int(nb), selected.data(), mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(),
+       sst_fields_.at("wall_shear_x").data(), sst_fields_.at("wall_shear_y").data(),
+       sst_fields_.at("wall_shear_z").data(), vf.x.data(), vf.y.data(), vf.z.data(), mesh.n_internal);
+  } else
   fv("fv_wall_viscous_force", int(nb), mesh.owner.data(), selected.data(), mesh.area.x.data(),
      mesh.area.y.data(), mesh.area.z.data(), mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(),
      mesh.owner_to_face.z.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(), gu.x.data(),
@@ -1153,70 +1346,323 @@ const V* Steady::array(const std::string& name) const {
       {"wall_distance", &mesh.wall_distance},
       {"velocity_lsq_inverse", &velocity_lsq.inverse}, {"pressure_lsq_inverse", &pressure_lsq.inverse},
       {"velocity_lsq_boundary_x", &velocity_lsq.boundary_delta.x},
-      {"pressure_lsq_boundary_y", &pressure_lsq.boundary_delta.y}};
+      {"pressure_lsq_boundary_y", &pressure_lsq.boundary_delta.y}, {"k", &kinetic}, {"omega", &omega},
+      {"viscosity", &viscosity}, {"boundary_k", &boundary_k}, {"boundary_omega", &boundary_omega},
+      {"source_x", &source.x}, {"strain_dudy", &strain_gradient[0].y}, {"k_grad_y", &k_eq.grad.y},
+      {"omega_rhs", &omega_eq.rhs}};
   const auto it = table.find(name);
   if (it != table.end()) return it->second;
   const auto sst = sst_fields_.find(name);
   return sst == sst_fields_.end() ? nullptr : &sst->second;
 }
 
-void Steady::sst_update(const double* o, const float* const* in) {
+// SstModel::update (turbulence/sst.cpp:101-158, no intermittency): in[25] as
+// ka_simplec_sst_update's inputs; `gradients` runs where the reference calls
+// reconstruct_scalar_gradients (:126), between the wall cells and the rates.
+void Steady::sst_kernels(const double* o, const float* const* in, const std::function<void()>& gradients) {
   const int n = mesh.n_cells, nf = mesh.n_faces, nb = mesh.n_boundary(), ni = mesh.n_internal;
   ORACLE_REQUIRE(o[4] > 0 && o[13] > 0 && o[15] > 0 && o[16] > 0 && o[17] > 0 && o[18] > 0 && o[6] >= 0 &&
                      !(o[10] != 0 && o[11] != 0),
                  "SST: invalid options");
-  VI wall(nb, 0);
-  for (std::size_t p = 0; p < host.patches.size(); ++p)
-    if (conditions_[p].kind == 0)
-      for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f) wall[f - ni] = 1;
   auto& F = sst_fields_;
+  if (sst_wall.size() != std::size_t(nb)) {  // the standalone model on a laminar solver's walls
+    sst_wall.assign(nb, 0);
+    for (std::size_t p = 0; p < host.patches.size(); ++p)
+      if (conditions_[p].kind == 0)
+        for (Index f = host.patches[p].start; f < host.patches[p].start + host.patches[p].size; ++f)
+          sst_wall[f - ni] = 1;
+  }
   for (const char* name : {"mu_t", "k_source", "k_sink", "omega_source", "omega_sink", "strain", "vorticity", "f1",
                            "f2", "cdkw", "diffusivity_k", "diffusivity_omega", "production_k", "fixed_omega",
                            "wall_production"})
-    F[name].assign(n, 0.0f);
+    F[name].resize(n, 0.0f);
   for (const char* name : {"wall_tau", "wall_u_star", "wall_u_tau", "wall_y_plus", "wall_y_plus_utau",
                            "wall_shear_x", "wall_shear_y", "wall_shear_z", "wall_shear_coefficient",
                            "wall_face_omega", "wall_cell_omega", "face_production"})
-    F[name].assign(nb, 0.0f);
-  F["face_diffusivity_k"].assign(nf, 0.0f);
-  F["face_diffusivity_omega"].assign(nf, 0.0f);
-  VI fixed_mask(n, 0), selected(n, 0);
+    F[name].resize(nb, 0.0f);
+  F["face_diffusivity_k"].resize(nf, 0.0f);
+  F["face_diffusivity_omega"].resize(nf, 0.0f);
+  fixed_omega_mask.resize(n, 0);
+  selected_wall_face.resize(n, 0);
   const auto ux = in[0], uy = in[1], uz = in[2];
   const float* const* g = in + 3;  // du/dx, du/dy, du/dz, dv/dx, ...
-  const float *rho = in[12], *mu = in[13], *k = in[14], *omega = in[15];
+  const float *rho = in[12], *mu = in[13], *k = in[14], *om = in[15];
   const float* const* gk = in + 16;
   const float* const* go = in + 19;
   const float* const* wu = in + 22;
   const float k_floor = float(o[17]), omega_floor = float(o[16]);
-  auto d = [&](const char* name) { return F[name].data(); };
-  fv("sst_wall", nb, mesh.owner.data(), wall.data(), ni, mesh.area.x.data(), mesh.area.y.data(),
-     mesh.area.z.data(), ux, uy, uz, wu[0], wu[1], wu[2], mesh.wall_normal_distance.data(), rho, mu, k, omega,
+  auto d = [&](const char* name) { return F.at(name).data(); };
+  fv("sst_wall", nb, mesh.owner.data(), sst_wall.data(), ni, mesh.area.x.data(), mesh.area.y.data(),
+     mesh.area.z.data(), ux, uy, uz, wu[0], wu[1], wu[2], mesh.wall_normal_distance.data(), rho, mu, k, om,
      d("wall_tau"), d("wall_shear_x"), d("wall_shear_y"), d("wall_shear_z"), d("wall_u_star"), d("wall_u_tau"),
      d("wall_y_plus"), d("wall_y_plus_utau"), d("face_production"), d("wall_shear_coefficient"), k_floor,
      omega_floor);
-  fv("sst_wall_omega", nb, mesh.owner.data(), wall.data(), ni, mesh.wall_normal_distance.data(), rho, mu,
+  fv("sst_wall_omega", nb, mesh.owner.data(), sst_wall.data(), ni, mesh.wall_normal_distance.data(), rho, mu,
      d("wall_u_star"), d("wall_cell_omega"), d("wall_face_omega"));
-  fv("sst_wall_cells", n, mesh.offsets.data(), mesh.faces.data(), wall.data(), mesh.wall_normal_distance.data(),
-     d("wall_cell_omega"), d("face_production"), ni, fixed_mask.data(), selected.data(), d("fixed_omega"),
-     d("wall_production"));
+  fv("sst_wall_cells", n, mesh.offsets.data(), mesh.faces.data(), sst_wall.data(), mesh.wall_normal_distance.data(),
+     d("wall_cell_omega"), d("face_production"), ni, fixed_omega_mask.data(), selected_wall_face.data(),
+     d("fixed_omega"), d("wall_production"));
+  if (gradients) gradients();
   fv("sst_rates", n, g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], d("strain"), d("vorticity"));
-  fv("sst_blending_viscosity", n, rho, mu, k, omega, mesh.wall_distance.data(), gk[0], gk[1], gk[2], go[0], go[1],
+  fv("sst_blending_viscosity", n, rho, mu, k, om, mesh.wall_distance.data(), gk[0], gk[1], gk[2], go[0], go[1],
      go[2], d("strain"), d("vorticity"), d("f1"), d("f2"), d("cdkw"), d("mu_t"), d("diffusivity_k"),
      d("diffusivity_omega"), float(o[13]), k_floor, omega_floor, int(o[7] != 0), int(o[9] != 0), float(o[18]));
   const int production_mode = o[10] != 0 ? 1 : (o[11] != 0 ? 2 : 0);
-  fv("sst_sources", n, rho, mu, k, omega, d("strain"), d("vorticity"), d("f1"), d("cdkw"), d("mu_t"),
-     fixed_mask.data(), d("wall_production"), d("k_source"), d("k_sink"), d("omega_source"), d("omega_sink"),
+  fv("sst_sources", n, rho, mu, k, om, d("strain"), d("vorticity"), d("f1"), d("cdkw"), d("mu_t"),
+     fixed_omega_mask.data(), d("wall_production"), d("k_source"), d("k_sink"), d("omega_source"), d("omega_sink"),
      d("production_k"), int(o[7] != 0), int(o[8] != 0), production_mode, int(o[3] != 0), int(o[5] != 0),
      int(o[12] != 0), float(o[4]), int(o[14] != 0), float(o[15]), k_floor, omega_floor, float(o[6]));
   if (o[0] != 0)
     fv("sst_sustain", n, rho, d("f1"), d("k_source"), d("omega_source"), float(o[1]), float(o[2]));
-  fv("sst_face_diffusion", nf, mesh.owner.data(), mesh.neighbour.data(), mesh.weight.data(), wall.data(), mu,
+  fv("sst_face_diffusion", nf, mesh.owner.data(), mesh.neighbour.data(), mesh.weight.data(), sst_wall.data(), mu,
      d("diffusivity_k"), d("diffusivity_omega"), ni, d("face_diffusivity_k"), d("face_diffusivity_omega"));
-  for (const auto& [name, v] : {std::pair<const char*, const VI*>{"fixed_omega_mask", &fixed_mask},
-                                {"selected_wall_face", &selected}}) {
+  for (const auto& [name, v] : {std::pair<const char*, const VI*>{"fixed_omega_mask", &fixed_omega_mask},
+                                {"selected_wall_face", &selected_wall_face}}) {
     auto& out = F[name];
     out.resize(v->size());
     std::memcpy(out.data(), v->data(), v->size() * sizeof(int));
+  }
+}
+
+void Steady::sst_update(const double* o, const float* const* in) { sst_kernels(o, in, {}); }
+
+// :654-680, the default (fused vector fit) path: the strain-rate derivative
+// fit of each velocity component plus its face-jump correction.
+void Steady::reconstruct_strain() {
+  const auto& fit = strain_lsq;
+  fv("fv_vector_gradient", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), stencil.lsq_x.data(),
+     stencil.lsq_y.data(), stencil.lsq_z.data(), fit.boundary_delta.x.data(), fit.boundary_delta.y.data(),
+     fit.boundary_delta.z.data(), fit.inverse.data(), velocity.x.data(), velocity.y.data(), velocity.z.data(),
+     boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data(), strain_base[0].x.data(),
+     strain_base[0].y.data(), strain_base[0].z.data(), strain_base[1].x.data(), strain_base[1].y.data(),
+     strain_base[1].z.data(), strain_base[2].x.data(), strain_base[2].y.data(), strain_base[2].z.data(),
+     mesh.n_cells);
+  for (int k = 0; k < 3; ++k) {  // jump_corrected_gradient (fv/operators.cpp:254-267)
+    const auto& base = strain_base[k];
+    auto& out = strain_gradient[k];
+    fv("fv_face_jump", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), mesh.owner_to_face.x.data(),
+       mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(), mesh.neighbour_to_face.x.data(),
+       mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(), velocity[k].data(),
+       boundary_velocity[k].data(), base.x.data(), base.y.data(), base.z.data(), jump_forward.data(),
+       jump_reverse.data(), mesh.n_internal);
+    fv("fv_gather_jump_gradient", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+       jump_delta.x.data(), jump_delta.y.data(), jump_delta.z.data(), jump_inverse.data(), jump_forward.data(),
+       jump_reverse.data(), base.x.data(), base.y.data(), base.z.data(), out.x.data(), out.y.data(), out.z.data(),
+       mesh.n_cells);
+  }
+}
+
+// :646-651
+void Steady::turbulence_boundaries() {
+  fv("fv_turbulence_boundary", mesh.n_boundary(), mesh.owner.data(), boundary_kind.data(), turbulence_spec.data(),
+     turbulence_a.data(), turbulence_b.data(), mass.data(), boundary_velocity.x.data(), boundary_velocity.y.data(),
+     boundary_velocity.z.data(), kinetic.data(), omega.data(), sst_fields_.at("wall_face_omega").data(),
+     boundary_k.data(), boundary_omega.data(), turbulence_kind.data(), mesh.n_internal, float(settings.density),
+     float(settings.viscosity));
+}
+
+// :681-712 (SST): strain fit when the velocity changed, SstModel::update with
+// the turbulence boundary values and k, omega fits between its wall and
+// interior passes, then the face effective viscosity.
+void Steady::update_turbulence(bool velocity_changed) {
+  if (!sst) return;
+  if (velocity_changed) reconstruct_strain();
+  const float* in[25] = {velocity.x.data(), velocity.y.data(), velocity.z.data(),
+                         strain_gradient[0].x.data(), strain_gradient[0].y.data(), strain_gradient[0].z.data(),
+                         strain_gradient[1].x.data(), strain_gradient[1].y.data(), strain_gradient[1].z.data(),
+                         strain_gradient[2].x.data(), strain_gradient[2].y.data(), strain_gradient[2].z.data(),
+                         rho_cell.data(), mu_cell.data(), kinetic.data(), omega.data(),
+                         k_eq.grad.x.data(), k_eq.grad.y.data(), k_eq.grad.z.data(),
+                         omega_eq.grad.x.data(), omega_eq.grad.y.data(), omega_eq.grad.z.data(),
+                         boundary_velocity.x.data(), boundary_velocity.y.data(), boundary_velocity.z.data()};
+  sst_kernels(settings.sst_settings.model, in, [&] {
+    turbulence_boundaries();
+    gradient(k_lsq, kinetic, boundary_k, k_eq.grad);
+    gradient(omega_lsq, omega, boundary_omega, omega_eq.grad);
+  });
+  fv("fv_effective_viscosity", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), boundary_kind.data(),
+     mesh.weight.data(), sst_fields_.at("mu_t").data(), viscosity.data(), mesh.n_internal, float(settings.viscosity));
+}
+
+// :713-727
+void Steady::add_transpose_stress() {
+  if (!sst) return;
+  const auto &u = momentum[0].grad, &v = momentum[1].grad, &w = momentum[2].grad;
+  fv("fv_transpose_stress", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), boundary_kind.data(),
+     mesh.weight.data(), viscosity.data(), mesh.area.x.data(), mesh.area.y.data(), mesh.area.z.data(), u.x.data(),
+     u.y.data(), u.z.data(), v.x.data(), v.y.data(), v.z.data(), w.x.data(), w.y.data(), w.z.data(),
+     transpose_stress.x.data(), transpose_stress.y.data(), transpose_stress.z.data(), mesh.n_internal);
+  fv("fv_stress_source", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(), mesh.volume.data(),
+     transpose_stress.x.data(), transpose_stress.y.data(), transpose_stress.z.data(), source.x.data(),
+     source.y.data(), source.z.data());
+}
+
+// :778-795 with TransportEquation::assemble_reconstructed (fv/transport.cpp:
+// 83-121): Barth-Jespersen limited k or omega, the model's face diffusivity,
+// sources and Patankar sinks, compensated low words; then omega's fixed
+// wall-cell rows.
+void Steady::assemble_turbulence(bool is_omega) {
+  const auto& opt = settings.sst_settings;
+  auto& eq = is_omega ? omega_eq : k_eq;
+  const V& field = is_omega ? omega : kinetic;
+  const V& bc = is_omega ? boundary_omega : boundary_k;
+  const V& mu = sst_fields_.at(is_omega ? "face_diffusivity_omega" : "face_diffusivity_k");
+  const V& src = sst_fields_.at(is_omega ? "omega_source" : "k_source");
+  const V& sink = sst_fields_.at(is_omega ? "omega_sink" : "k_sink");
+  const bool second_order = is_omega ? opt.omega_second_order : opt.k_second_order;
+  const float dt = opt.pseudo_transient ? time_step : 0.0f;
+  const float relaxation = float(is_omega ? opt.omega_relaxation : opt.k_relaxation);
+  const bool compensated = opt.compensated_transport;
+  fv("fv_limiter", mesh.n_cells, mesh.offsets.data(), stencil.adjacent.data(), mesh.faces.data(), mesh.sign.data(),
+     mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+     mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+     field.data(), bc.data(), eq.grad.x.data(), eq.grad.y.data(), eq.grad.z.data(), eq.limiter.data(), -1.0f);
+  fv("fv_transport_faces", mesh.n_faces, mesh.owner.data(), mesh.neighbour.data(), mesh.weight.data(),
+     mesh.owner_to_face.x.data(), mesh.owner_to_face.y.data(), mesh.owner_to_face.z.data(),
+     mesh.neighbour_to_face.x.data(), mesh.neighbour_to_face.y.data(), mesh.neighbour_to_face.z.data(),
+     stencil.nonorth_x.data(), stencil.nonorth_y.data(), stencil.nonorth_z.data(), eq.grad.x.data(),
+     eq.grad.y.data(), eq.grad.z.data(), eq.limiter.data(), mass.data(), mu.data(), turbulence_kind.data(),
+     face_correction_scratch.data(), mesh.n_internal, second_order ? 1 : 0, 0);
+  fv("fv_transport_assemble", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(),
+     stencil.adjacent.data(), stencil.matrix_entry.data(), matrix.row_ptr.data(), matrix.diag.data(),
+     stencil.diffusion.data(), mass.data(), mu.data(), turbulence_kind.data(), bc.data(),
+     face_correction_scratch.data(), src.data(), mesh.volume.data(), field.data(), sink.data(), zero.data(),
+     zero.data(), matrix.val.data(), eq.rhs.data(), compensated ? eq.rhs_low.data() : zero.data(),
+     eq.diagonal.data(), compensated ? eq.diagonal_low.data() : zero.data(), eq.consistent_inverse.data(),
+     mesh.n_internal, float(settings.density), dt, relaxation, compensated ? 1 : 0);
+  if (is_omega)
+    fv("fv_fixed_rows", mesh.n_cells, matrix.row_ptr.data(), matrix.diag.data(), fixed_omega_mask.data(),
+       sst_fields_.at("fixed_omega").data(), matrix.val.data(), eq.rhs.data());
+}
+
+// TransportEquation::physical_residual (fv/transport.cpp:57-62).
+void Steady::physical_residual(ScalarEquation& eq, const V& field, V& result, const VI* excluded) {
+  if (!settings.sst_settings.compensated_transport) {
+    kern::csr_residual(mesh.n_cells, matrix.row_ptr.data(), matrix.col.data(), matrix.val.data(), field.data(),
+                       eq.rhs.data(), result.data());
+    return;
+  }
+  fv("fv_compensated_momentum_residual", mesh.n_cells, matrix.row_ptr.data(), matrix.col.data(),
+     matrix.val.data(), matrix.diag.data(), eq.diagonal.data(), eq.diagonal_low.data(), field.data(), zero.data(),
+     zero.data(), mesh.volume.data(), eq.rhs.data(), eq.rhs_low.data(), result.data());
+  if (excluded) fv("fv_scalar_set_fixed", mesh.n_cells, excluded->data(), zero.data(), result.data());
+}
+
+// TransportEquation::solve (fv/transport.cpp:123-159), Jacobi BiCGStab.
+SolveStats Steady::solve_scalar(ScalarEquation& eq, V& field, const VI* excluded) {
+  const auto& opt = settings.sst_settings;
+  const std::size_t n = mesh.n_cells;
+  SolveControl control = opt.linear;
+  const bool compensated = opt.compensated_transport;
+  if (compensated) physical_residual(eq, field, eq.residual, excluded);
+  if (opt.steady_tolerance > 0) {
+    if (!compensated)
+      kern::csr_residual(mesh.n_cells, matrix.row_ptr.data(), matrix.col.data(), matrix.val.data(), field.data(),
+                         eq.rhs.data(), eq.work_residual.data());
+    else
+      eq.work_residual = eq.residual;
+    fv("fv_scalar_steady_measure", mesh.n_cells, field.data(), eq.diagonal.data(), eq.work_residual.data(),
+       excluded ? excluded->data() : matrix.diag.data(), eq.work_absolute.data(), eq.work_scale.data(),
+       excluded ? 1 : 0);
+    const double defect = sum(eq.work_absolute.data(), n), scale_sum = sum(eq.work_scale.data(), n);
+    const double r2 = ::oracle::dot(eq.work_residual.data(), eq.work_residual.data(), n);
+    if (defect <= opt.steady_tolerance * scale_sum && std::isfinite(r2))
+      control.abs_tol = std::max(control.abs_tol, std::nextafter(std::sqrt(r2), std::numeric_limits<double>::infinity()));
+  }
+  eq.solver->refresh(matrix);
+  if (!compensated) return eq.solver->solve(matrix, field, eq.rhs, control);
+  std::fill(eq.delta.begin(), eq.delta.end(), 0.0f);
+  const auto stats = eq.solver->solve(matrix, eq.delta, eq.residual, control);
+  for (std::size_t i = 0; i < n; ++i) field[i] = 1.0f * field[i] + 1.0f * eq.delta[i];  // axpbyz(1, 1)
+  return stats;
+}
+
+// :796-818 (no transition model).
+void Steady::solve_turbulence() {
+  if (!sst) return;
+  const auto& opt = settings.sst_settings;
+  update_boundaries();
+  update_turbulence();
+  assemble_turbulence(false);
+  report_.turbulence[0] = solve_scalar(k_eq, kinetic, nullptr);
+  fv("fv_floor", mesh.n_cells, kinetic.data(), float(opt.model[17]));
+  update_turbulence(false);
+  assemble_turbulence(true);
+  const auto& fixed = sst_fields_.at("fixed_omega");
+  const bool set_fixed = opt.steady_tolerance > 0 || opt.compensated_transport;
+  if (set_fixed) fv("fv_scalar_set_fixed", mesh.n_cells, fixed_omega_mask.data(), fixed.data(), omega.data());
+  report_.turbulence[1] = solve_scalar(omega_eq, omega, &fixed_omega_mask);
+  if (set_fixed) fv("fv_scalar_set_fixed", mesh.n_cells, fixed_omega_mask.data(), fixed.data(), omega.data());
+  fv("fv_floor", mesh.n_cells, omega.data(), float(opt.model[16]));
+}
+
+// :819-837
+void Steady::monitor_turbulence() {
+  if (!sst) return;
+  const std::size_t n = mesh.n_cells;
+  const auto measure = [&](bool is_omega) {
+    assemble_turbulence(is_omega);
+    auto& eq = is_omega ? omega_eq : k_eq;
+    const V& q = is_omega ? omega : kinetic;
+    physical_residual(eq, q, residual_u, is_omega ? &fixed_omega_mask : nullptr);
+    fv("fv_scalar_monitor", mesh.n_cells, q.data(), eq.diagonal.data(), residual_u.data(),
+       is_omega ? fixed_omega_mask.data() : zero_mask.data(), mesh.volume.data(), absolute_residual.x.data(),
+       scale.data(), fluent_scale.x.data(), float(settings.density),
+       settings.sst_settings.pseudo_transient ? time_step : 0.0f);
+    const double raw = sum(absolute_residual.x.data(), n), den = sum(scale.data(), n);
+    const double fluent_den = sum(fluent_scale.x.data(), n);
+    const auto scaled = [&](double d) {
+      return d > 0 ? raw / d : (raw == 0 ? 0 : std::numeric_limits<double>::infinity());
+    };
+    return ResidualValue{raw, scaled(den), scaled(fluent_den), fluent_den};
+  };
+  report_.k = measure(false);
+  report_.omega = measure(true);
+}
+
+// SteadySolver::initialize_fields (:1493-1524).
+void Steady::initialize_fields(const std::vector<Vec3>& u, const std::vector<double>& p,
+                               const std::vector<double>& k, const std::vector<double>& w) {
+  const auto n = std::size_t(mesh.n_cells);
+  if (paired) {  // reset_paired (:977-988)
+    for (auto* v : {&velocity_lo.x, &velocity_lo.y, &velocity_lo.z, &pressure_gradient_lo.x,
+                    &pressure_gradient_lo.y, &pressure_gradient_lo.z, &pressure_lo, &mass_lo})
+      std::fill(v->begin(), v->end(), 0.0f);
+  }
+  ORACLE_REQUIRE(u.size() == n && p.size() == n, "FV initialization field size mismatch");
+  ORACLE_REQUIRE(sst ? (k.size() == n && w.size() == n) : (k.empty() && w.empty()),
+                 "FV turbulence initialization size mismatch");
+  const double ref = reference_ >= 0 ? p[std::size_t(reference_)] : 0;
+  for (std::size_t c = 0; c < n; ++c) {
+    const float value = float(p[c] - ref);
+    ORACLE_REQUIRE(std::isfinite(value), "FV nonfinite pressure initialization");
+    velocity.x[c] = float(u[c].x);
+    velocity.y[c] = float(u[c].y);
+    velocity.z[c] = float(u[c].z);
+    pressure[c] = value;
+    if (sst) {
+      ORACLE_REQUIRE(std::isfinite(k[c]) && k[c] >= 0 && std::isfinite(w[c]) && w[c] > 0,
+                     "FV invalid turbulence initialization");
+      kinetic[c] = float(k[c]);
+      omega[c] = float(w[c]);
+      ORACLE_REQUIRE(std::isfinite(kinetic[c]) && std::isfinite(omega[c]),
+                     "FV turbulence initialization exceeds float storage");
+    }
+  }
+  std::fill(physical_diagonal.begin(), physical_diagonal.end(), 1.0f);
+  std::fill(consistent_inverse.begin(), consistent_inverse.end(), 0.0f);
+  update_boundaries();
+  update_pressure();
+  update_flux();
+  update_turbulence();
+  for (int pass = 0; pass < 3; ++pass) {
+    monitor();
+    refresh_paired_scales();
+    update_flux();
+    fv("fv_continuity", mesh.n_cells, mesh.offsets.data(), mesh.faces.data(), mesh.sign.data(), mass.data(),
+       provisional_continuity.data());
+    update_boundaries();
+    update_pressure();
   }
 }
 
@@ -1288,9 +1734,10 @@ void* oracle_simplec_create_profiled(int nx, int ny, int nz, const double* exten
     for (std::size_t p = 0; p < conditions.size(); ++p)
       conditions[p] = {kind[p], {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]}, pressure[p], total[p], {}};
     for (auto& c : conditions)
-      if (c.kind == 5) {  // mass flow in pressure[p], inward direction in velocity
+      if (c.kind == 5) {  // mass flow in pressure[p], inward direction in velocity (zero: none)
         c.mass_flow = c.pressure;
         c.direction = c.velocity;
+        c.has_direction = !(c.direction.x == 0 && c.direction.y == 0 && c.direction.z == 0);
         c.velocity = {};
         c.pressure = 0;
       }
@@ -1366,6 +1813,14 @@ int oracle_simplec_report(void* h, int run_monitor, double* out) {
     out[k++] = r.pressure.iterations;
     out[k++] = r.pressure.final_residual;
     out[k++] = r.pressure.converged;
+    for (const auto* v : {&r.k, &r.omega}) {
+      out[k++] = v->raw;
+      out[k++] = v->scaled;
+      out[k++] = v->fluent_scaled;
+      out[k++] = v->fluent_normalization;
+    }
+    out[k++] = r.turbulence[0].iterations;
+    out[k++] = r.turbulence[1].iterations;
     return 0;
   } catch (const std::exception& e) {
     g_simplec_error = e.what();
@@ -1406,6 +1861,115 @@ int oracle_simplec_forces(void* h, const char* names, double reference_pressure,
 int oracle_simplec_sst_update(void* h, const double* options, const float* const* inputs) {
   try {
     static_cast<oracle::simplec::Steady*>(h)->sst_update(options, inputs);
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
+  }
+}
+// A solver on a general mesh (the PolyMesh arrays of mesh/mesh.hpp:36-68;
+// patch names separated by '\n') with per-patch conditions: kind[p] as
+// oracle_simplec_create_open, velocity[3p..3p+2] (a mass-flow inlet's
+// direction), pressure[p] (a mass-flow inlet's kg/s), total[p], flags[p]
+// (bit 0: a mass-flow direction is given), turbulence[p] (-1 none, 0 k and
+// omega, 1 intensity and length, 2 intensity and viscosity ratio) with its
+// pair turbulence_ab[2p..2p+1]. settings[12]: those of oracle_simplec_create
+// and the fixed pseudo time step (0: automatic). sst[29] or null (laminar):
+// the SstOptions model[19], then k second order (0/1), omega second order,
+// k relaxation, omega relaxation, linear rel_tol, abs_tol, max_iter, steady
+// tolerance, compensated transport (0/1), pseudo transient (0/1).
+void* oracle_simplec_create_mesh(int n_points, const double* points, int n_faces, const int* face_offsets,
+                                 const int* face_nodes, const int* owner, int n_internal, const int* neighbour,
+                                 int n_cells, int n_patches, const char* patch_names, const int* patch_start,
+                                 const int* patch_size, const int* kind, const double* velocity,
+                                 const double* pressure, const int* total, const int* flags, const int* turbulence,
+                                 const double* turbulence_ab, const double* settings, const double* sst) {
+  try {
+    using namespace oracle;
+    using namespace oracle::simplec;
+    ORACLE_REQUIRE(n_points > 0 && n_faces > 0 && n_internal >= 0 && n_internal <= n_faces && n_cells > 0 &&
+                       n_patches > 0,
+                   "FV mesh: empty or inconsistent sizes");
+    PolyMesh mesh;
+    mesh.points.resize(n_points);
+    for (int i = 0; i < n_points; ++i) mesh.points[i] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    mesh.face_offsets.assign(face_offsets, face_offsets + n_faces + 1);
+    mesh.face_nodes.assign(face_nodes, face_nodes + face_offsets[n_faces]);
+    mesh.owner.assign(owner, owner + n_faces);
+    mesh.neighbour.assign(neighbour, neighbour + n_internal);
+    mesh.n_cells = n_cells;
+    std::string names(patch_names), item;
+    std::vector<std::string> split;
+    for (char ch : names + "\n") {
+      if (ch == '\n') {
+        split.push_back(item);
+        item.clear();
+      } else {
+        item += ch;
+      }
+    }
+    ORACLE_REQUIRE(int(split.size()) == n_patches, "FV mesh: one name per patch");
+    for (int p = 0; p < n_patches; ++p) mesh.patches.push_back({split[p], patch_start[p], patch_size[p]});
+    std::vector<Condition> conditions(n_patches);
+    for (int p = 0; p < n_patches; ++p) {
+      auto& c = conditions[p];
+      c.kind = kind[p];
+      c.velocity = {velocity[3 * p], velocity[3 * p + 1], velocity[3 * p + 2]};
+      c.pressure = pressure[p];
+      c.total = total[p];
+      if (c.kind == 5) {
+        c.mass_flow = c.pressure;
+        c.direction = c.velocity;
+        c.has_direction = (flags[p] & 1) != 0;
+        c.velocity = {};
+        c.pressure = 0;
+      }
+      c.turbulence = turbulence[p];
+      c.turbulence_a = turbulence_ab[2 * p];
+      c.turbulence_b = turbulence_ab[2 * p + 1];
+    }
+    Settings s;
+    s.density = settings[0];
+    s.viscosity = settings[1];
+    s.momentum_relaxation = settings[2];
+    s.pressure_relaxation = settings[3];
+    s.pseudo_time_scale = settings[4];
+    s.monitor_interval = int(settings[5]);
+    s.paired_state = settings[6] != 0;
+    s.paired_state_slack_ulps = settings[7];
+    s.reference_point = {settings[8], settings[9], settings[10]};
+    s.pseudo_time_step = settings[11];
+    if (sst) {
+      s.sst = true;
+      auto& t = s.sst_settings;
+      for (int k = 0; k < 19; ++k) t.model[k] = sst[k];
+      t.k_second_order = sst[19] != 0;
+      t.omega_second_order = sst[20] != 0;
+      t.k_relaxation = sst[21];
+      t.omega_relaxation = sst[22];
+      t.linear = {sst[23], sst[24], int(sst[25]), 8};
+      t.steady_tolerance = sst[26];
+      t.compensated_transport = sst[27] != 0;
+      t.pseudo_transient = sst[28] != 0;
+    }
+    return new Steady(std::move(mesh), conditions, s);
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return nullptr;
+  }
+}
+// initialize_fields: per-cell values in the solver's cell order; k and omega
+// null without SST.
+int oracle_simplec_initialize_fields(void* h, const double* u, const double* v, const double* w, const double* p,
+                                     const double* k, const double* omega) {
+  try {
+    auto& s = *static_cast<oracle::simplec::Steady*>(h);
+    const auto n = std::size_t(s.cells());
+    std::vector<oracle::Vec3> velocity(n);
+    for (std::size_t c = 0; c < n; ++c) velocity[c] = {u[c], v[c], w[c]};
+    s.initialize_fields(velocity, std::vector<double>(p, p + n),
+                        k ? std::vector<double>(k, k + n) : std::vector<double>{},
+                        omega ? std::vector<double>(omega, omega + n) : std::vector<double>{});
     return 0;
   } catch (const std::exception& e) {
     g_simplec_error = e.what();
