@@ -231,6 +231,95 @@ def simplec_cpu_baseline(size, nz, settings, conditions, warmup, iterations, gpu
             "gpu_speedup": round(ms / gpu_ms, 1)}
 
 
+# The backward-facing step (a k-omega SST channel): air at 20 m/s, 1%
+# intensity with turbulent viscosity ratio 10 at the inlet and at outlet
+# backflow, h = 0.1 m, 5h upstream, 30h downstream, span 4h between symmetry
+# planes, 8h channel above the step, walls graded from a first cell of y+ ~ 1
+# (flat-plate skin friction 0.5 m from the inlet) by 1.15.
+STEP_SPEED, STEP_RHO, STEP_MU = 20.0, 1.225, 1.7894e-5
+
+
+def step_case(nz: int):
+    import kutta_amd
+    from kutta_amd import meshes
+    first = meshes.first_cell_height(STEP_SPEED, STEP_MU / STEP_RHO, 0.5)
+    mesh = meshes.backward_step(meshes.StepGeometry(), first=first, growth=1.15, largest=0.02, nz=nz)
+    turb = kutta_amd.IntensityViscosityRatio(0.01, 10.0)
+    cond = {"inlet": kutta_amd.VelocityInlet((STEP_SPEED, 0.0, 0.0), turb),
+            "outlet": kutta_amd.PressureOutlet(0.0, None, False, turb),
+            "wall": kutta_amd.Wall(), "symmetry": kutta_amd.Symmetry()}
+    settings = kutta_amd.SolverSettings(density=STEP_RHO, viscosity=STEP_MU, monitor_interval=20)
+    k0 = 1.5 * (0.01 * STEP_SPEED) ** 2
+    w0 = STEP_RHO * k0 / (STEP_MU * 10.0)
+    return mesh, cond, settings, k0, w0, first
+
+
+def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, cpu_iterations: int,
+                       graphs: bool = True) -> dict:
+    """ms per steady SST SIMPLEC outer iteration on the backward-facing step
+    (step_case), nz cells across the span: uniform inflow state, `warmup`
+    iterations untimed, then `iterations` timed. The oracle is timed on one
+    host core on the same geometry with cpu_nz spanwise cells (fewer cells:
+    the serial oracle takes about a minute per iteration at full size)."""
+    import kutta_amd
+    mesh, cond, settings, k0, w0, first = step_case(nz)
+    t = time.perf_counter()
+    solver = kutta_amd.SteadySolver.from_mesh(dev, mesh, cond, settings, kutta_amd.SstSettings())
+    setup_s = time.perf_counter() - t
+    solver.set_graphs(graphs)
+    t = time.perf_counter()
+    solver.initialize_fields((STEP_SPEED, 0.0, 0.0), 0.0, k0, w0)
+    solver.iterate(warmup)
+    dev.synchronize()
+    warm_s = time.perf_counter() - t
+    waits0 = solver.info()["host_waits"]
+    t0 = time.perf_counter()
+    solver.iterate(iterations)
+    dev.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(iterations, 1)
+    info = solver.info()
+    report = solver.report(monitor=True)
+    out = {"workload": (f"SST k-omega SIMPLEC, backward-facing step h=0.1 m (5h up, 30h down, 8h channel, "
+                        f"span 4h symmetry), {mesh.n_cells} hex cells, first cell {first:.3g} m growth 1.15, "
+                        f"air 20 m/s, TI 1%, viscosity ratio 10, batched momentum, AMG-PCG V(2,2) pressure"),
+           "cells": info["cells"], "faces": mesh.n_faces, "ms_per_iteration": round(ms, 3),
+           "timed_iterations": iterations, "first_timed_iteration": warmup + 1, "setup_s": round(setup_s, 2),
+           "initialize_and_warmup_s": round(warm_s, 2), "batched_momentum": solver.set_batched_momentum(True),
+           "linear_solver_host_waits_per_iteration": round((info["host_waits"] - waits0) / max(iterations, 1), 2),
+           "last_pressure_iterations": int(report["pressure_iterations"]),
+           "last_k_iterations": int(report["k_iterations"]), "last_omega_iterations": int(report["omega_iterations"]),
+           "residuals": {k: report[k] for k in ("continuity_scaled", "u_scaled", "v_scaled", "k_scaled",
+                                                "omega_scaled")},
+           "maximum_velocity": report["maximum_velocity"]}
+    solver.close()
+    if cpu_iterations > 0:
+        import numpy as np
+
+        import oracle
+        cmesh, ccond, csettings, _, _, _ = step_case(cpu_nz)
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except (AttributeError, OSError):
+            pass
+        cpu = oracle.SteadySolver.from_mesh(cmesh, kutta_amd.pack_mesh_conditions(cmesh, ccond),
+                                            csettings.packed(), kutta_amd.SstSettings().packed())
+        n = cmesh.n_cells
+        cpu.initialize_fields(np.full(n, STEP_SPEED), np.zeros(n), np.zeros(n), np.zeros(n), np.full(n, k0),
+                              np.full(n, w0))
+        cpu.iterate(warmup)
+        t0 = time.perf_counter()
+        cpu.iterate(cpu_iterations)
+        cms = (time.perf_counter() - t0) * 1e3 / cpu_iterations
+        cpu.close()
+        scaled = cms * mesh.n_cells / n
+        out["cpu_baseline"] = {
+            "value": round(cms, 1), "unit": "ms per SIMPLEC iteration", "cores": 1, "kind": "port",
+            "sample": (f"{cpu_iterations} iterations after {warmup} warm-up iterations of the same step with "
+                       f"{cpu_nz} spanwise cells ({n} cells)"),
+            "per_cell_scaled_to_full_mesh_ms": round(scaled, 1), "gpu_speedup_per_cell": round(scaled / ms, 1)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +359,8 @@ def main() -> int:
     ap.add_argument("--learned-max-unexplained", type=float, default=0.5,
                     help="largest unexplained fraction of the training targets for which the "
                          "network is applied (larger: apply it regardless)")
+    ap.add_argument("--simplec-step", type=int, default=0, metavar="NZ",
+                    help="only time SST SIMPLEC on the backward-facing step with NZ spanwise cells (80: 4.06M)")
     ap.add_argument("--simplec-only", action="store_true",
                     help="only time SIMPLEC on the 2-D cavity (--simplec-size, --simplec-iterations)")
     args = ap.parse_args()
@@ -277,6 +368,16 @@ def main() -> int:
         return copy_probe(args.copy_probe, max(args.steps, 1))
     if args.fv_kernels:
         return fv_kernel_bench(args.size, max(args.steps, 1))
+    if args.simplec_step:
+        import kutta_amd
+        dev = kutta_amd.Device(0)
+        line = simplec_step_bench(dev, args.simplec_step, max(args.warmup, 1), max(args.steps, 1), 2,
+                                  0 if args.no_cpu_baseline else 2, not args.no_graphs)
+        line = {"metric": "ms/SIMPLEC iteration", "value": line["ms_per_iteration"], "unit": "ms",
+                "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic", "config": line}
+        print(json.dumps(line), flush=True)
+        dev.close()
+        return 0
     if args.simplec_box or args.simplec_only:
         import kutta_amd
         dev = kutta_amd.Device(0)

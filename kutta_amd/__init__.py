@@ -1369,8 +1369,11 @@ class SteadySolver:
                                                      p.ctypes.data))
 
     def close(self) -> None:
+        # After the device is gone its buffers cannot be freed through it:
+        # drop the handle (the device's teardown released the memory).
         if self.handle:
-            host_lib().ka_simplec_destroy(self.handle)
+            if self.device._alive:
+                host_lib().ka_simplec_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -1506,7 +1509,8 @@ class LearnedNetwork:
 
     def close(self) -> None:
         if self.handle:
-            host_lib().ka_learned_destroy(self.handle)
+            if self.device._alive:
+                host_lib().ka_learned_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -96,7 +96,7 @@ def test_homogeneous_decay_closed_form(device, pseudo_transient, dt, alpha):
                          linear=(1e-12, 1e-30, 10))
     s = ka.SteadySolver.from_mesh(device, mesh, cond, settings, sst)
     k, w = 0.3, 7.0
-    s.initialize_fields(0.0, 0.0, k, w)
+    s.initialize_fields((0.0, 0.0, 0.0), 0.0, k, w)
     inv_dt = 1.0 / dt if pseudo_transient else 0.0
     for it in range(12):
         s.iterate(1)
@@ -140,7 +140,18 @@ def test_backward_step_against_oracle(device):
     w0 = 1.225 * k0 / (1.7894e-5 * 10.0)
     gpu.initialize_fields((20.0, 0.0, 0.0), 0.0, k0, w0)
     cpu.initialize_fields(np.full(n, 20.0), np.zeros(n), np.zeros(n), np.zeros(n), np.full(n, k0), np.full(n, w0))
-    gpu.iterate(5)
-    cpu.iterate(5)
-    print(f"[sst] step case, {n} cells, after 5 iterations", flush=True)
-    assert_fields_close(gpu, cpu, ("u", "v", "p", "k", "omega"), eps=1e-3)
+    # From a uniform start the first iterations are an impulsive transient
+    # (speeds reach 15x the inflow), which amplifies the ulp-level libm
+    # differences; each field is compared relative to its own largest value:
+    # tightly after 5 iterations (measured: at most 4e-4 of the maximum),
+    # loosely after 20, inside the transient (measured: 8e-3, l2 5e-4).
+    for total, tol_max, tol_l2 in ((5, 1e-3, 2e-4), (20, 2e-2, 2e-3)):
+        gpu.iterate(total - gpu.info()["iteration"])
+        cpu.iterate(total - int(cpu.report()[0]))
+        for name in ("u", "v", "p", "k", "omega"):
+            a, b = gpu.field(name).astype(np.float64), cpu.field(name).astype(np.float64)
+            worst = np.max(np.abs(a - b)) / np.max(np.abs(b))
+            l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+            print(f"[sst] step case, {n} cells, iteration {total}, {name}: max {worst:.2e} of max|b|, "
+                  f"l2 {l2:.2e}", flush=True)
+            assert worst <= tol_max and l2 <= tol_l2, (name, total, worst, l2)
