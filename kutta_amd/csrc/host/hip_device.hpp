@@ -74,6 +74,18 @@ class DeviceArray {
   std::size_t n_ = 0;
 };
 
+// Whether a ROCm profiler (rocprofv3, which exports ROCPROF*/ROCPROFILER*
+// variables into the traced process) is attached. A first graph replay under
+// its kernel tracing ended a process with a host SIGSEGV inside
+// hipGraphLaunch, so graph replay is switched off while one is attached:
+// profiled runs launch the same kernels one by one (identical iterates).
+inline bool profiler_attached() {
+  extern char** environ;
+  for (char** e = environ; e && *e; ++e)
+    if (std::strncmp(*e, "ROCPROF", 7) == 0) return true;
+  return false;
+}
+
 // One argument for kh_launch.
 struct Arg {
   kh_arg a{};

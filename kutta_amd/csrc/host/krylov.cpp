@@ -738,7 +738,7 @@ struct KrylovSolver::Impl {
   // past convergence is a no-op, as the launches are. Configuration changes
   // that alter the sequence (tail, dense level, fusion, bfloat16 copies)
   // drop the graph.
-  bool graphs = true;
+  bool graphs = !profiler_attached();
   Graph pcg_graph;
   const Index* graph_rows = nullptr;
   const Index* graph_cols = nullptr;
@@ -1045,7 +1045,7 @@ void KrylovSolver::set_fused(bool fused) {
 }
 void KrylovSolver::set_graphs(bool on) {
   impl_->drop_graphs();
-  impl_->graphs = on;
+  impl_->graphs = on && !profiler_attached();
 }
 
 // Several BiCGStab-Jacobi solves in lockstep: each lane keeps its own entry,
