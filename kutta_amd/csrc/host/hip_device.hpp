@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <unistd.h>  // environ
 #include <initializer_list>
 #include <stdexcept>
 #include <string>
@@ -80,8 +81,7 @@ class DeviceArray {
 // hipGraphLaunch, so graph replay is switched off while one is attached:
 // profiled runs launch the same kernels one by one (identical iterates).
 inline bool profiler_attached() {
-  extern char** environ;
-  for (char** e = environ; e && *e; ++e)
+  for (char** e = ::environ; e && *e; ++e)
     if (std::strncmp(*e, "ROCPROF", 7) == 0) return true;
   return false;
 }
