@@ -24,7 +24,12 @@ on one host core beside it.
 
 The line also carries "simplec": ms per steady SIMPLEC iteration on the
 128x128x1 lid-driven cavity at Re=1000 (500 iterations, iterations 6-500
-timed), with the oracle timed on one core beside it.
+timed), with the oracle timed on one core beside it, and "simplec_sst": ms
+per SST k-omega SIMPLEC iteration on the 4.06M-cell backward-facing step
+(iterations 4-13 timed), with the oracle timed on one core on the same step
+with 2 spanwise cells.
+
+    python bench.py --simplec-step NZ [--steps K]    # SST step with NZ spanwise cells only
 """
 from __future__ import annotations
 
@@ -320,6 +325,49 @@ def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, 
     return out
 
 
+def trace_summary(db: str, line_json: str | None, top: int) -> int:
+    """Markdown summary of a rocprofv3 --kernel-trace database (rocpd SQLite):
+    busy time per kernel and grid size (one row per AMG level for the level
+    kernels), and for the l1-Jacobi smoother the algorithmic bytes 20n + 8m
+    of each level (rows n, nonzeros m from the bench line's "levels"),
+    GB/s and the fraction of the 8 TB/s HBM peak."""
+    import re
+    import sqlite3
+    rows = sqlite3.connect(db).execute("select name, grid_x, duration from kernels").fetchall()
+
+    def short(name):
+        n = name.replace("(anonymous namespace)::", "").replace("void ", "")
+        n = re.sub(r"^kh::", "", n)
+        m = re.match(r"k_each<(?:kh::)?(?:fv::)?([A-Za-z_0-9:]+)", n)
+        return "fv " + m.group(1) if m else re.sub(r"[<(].*", "", n)
+
+    agg = {}
+    for name, grid, ns in rows:
+        a = agg.setdefault((short(name), grid), [0, 0.0])
+        a[0] += 1
+        a[1] += ns
+    total = sum(v[1] for v in agg.values())
+    levels = []
+    if line_json:
+        cfg = json.loads(Path(line_json).read_text())["config"]
+        levels = cfg.get("levels", [])
+    print(f"busy {total / 1e6:.2f} ms over {len(rows)} dispatches\n")
+    print("| kernel | grid | launches | total ms | avg us | % busy | level rows | bytes/launch | GB/s | frac |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
+    for (name, grid), (count, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        avg_us = ns / count / 1e3
+        extra = ["", "", "", ""]
+        if name == "k_amg_smooth":
+            for n_rows, nnz in levels:
+                if grid == -(-n_rows // 256) * 256:
+                    b = 20 * n_rows + 8 * nnz
+                    gbs = b / (avg_us * 1e-6) / 1e9
+                    extra = [str(n_rows), str(b), f"{gbs:.0f}", f"{gbs / HBM_PEAK_GBS:.3f}"]
+        print(f"| {name} | {grid} | {count} | {ns / 1e6:.2f} | {avg_us:.2f} | {100 * ns / total:.1f} | "
+              + " | ".join(extra) + " |")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +407,11 @@ def main() -> int:
     ap.add_argument("--learned-max-unexplained", type=float, default=0.5,
                     help="largest unexplained fraction of the training targets for which the "
                          "network is applied (larger: apply it regardless)")
+    ap.add_argument("--trace-summary", metavar="DB", help="summarize a rocprofv3 kernel-trace database and exit")
+    ap.add_argument("--trace-line", metavar="JSON", help="the bench line of the traced run (AMG level sizes)")
+    ap.add_argument("--trace-top", type=int, default=40)
+    ap.add_argument("--step-nz", type=int, default=80,
+                    help="spanwise cells of the SST step in the default line (0 skips it)")
     ap.add_argument("--simplec-step", type=int, default=0, metavar="NZ",
                     help="only time SST SIMPLEC on the backward-facing step with NZ spanwise cells (80: 4.06M)")
     ap.add_argument("--simplec-only", action="store_true",
@@ -368,6 +421,8 @@ def main() -> int:
         return copy_probe(args.copy_probe, max(args.steps, 1))
     if args.fv_kernels:
         return fv_kernel_bench(args.size, max(args.steps, 1))
+    if args.trace_summary:
+        return trace_summary(args.trace_summary, args.trace_line, args.trace_top)
     if args.simplec_step:
         import kutta_amd
         dev = kutta_amd.Device(0)
@@ -605,6 +660,14 @@ def main() -> int:
                                 max(args.simplec_iterations - 5, 1), 0 if args.no_cpu_baseline else 20,
                                 not args.no_graphs)
 
+    simplec_sst = None
+    if args.step_nz:
+        # Configuration 3: SST on the backward-facing step (4.06M hex cells at
+        # 80 spanwise cells), 3 untimed iterations from the uniform inflow
+        # state, then 10 timed.
+        simplec_sst = simplec_step_bench(dev, args.step_nz, 3, 10, 2, 0 if args.no_cpu_baseline else 2,
+                                         not args.no_graphs)
+
     baseline = None
     x_gpu = x_plain
     if not args.no_cpu_baseline:
@@ -646,6 +709,7 @@ def main() -> int:
         "cpu_baseline": baseline,
         "solver_variants": variants,
         "simplec": simplec,
+        "simplec_sst": simplec_sst,
     }
     print(json.dumps(line), flush=True)
     dev.close()
