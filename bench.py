@@ -295,7 +295,8 @@ def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, 
            "last_k_iterations": int(report["k_iterations"]), "last_omega_iterations": int(report["omega_iterations"]),
            "residuals": {k: report[k] for k in ("continuity_scaled", "u_scaled", "v_scaled", "k_scaled",
                                                 "omega_scaled")},
-           "maximum_velocity": report["maximum_velocity"]}
+           "maximum_velocity": report["maximum_velocity"],
+           "pressure_levels": solver.pressure_levels(), "pressure_plan": solver.pressure_plan()}
     solver.close()
     if cpu_iterations > 0:
         import numpy as np
@@ -695,6 +696,7 @@ def main() -> int:
             "cells": n,
             "nnz": m,
             "hierarchy": solver.describe(),
+            "plan": solver.plan(),
             "levels": solver.levels(),
             "coarse_tail_level": tail_level,
             "tail_autotune_s": round(autotune_s, 3),

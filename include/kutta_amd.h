@@ -104,6 +104,14 @@ int ka_solver_solve(void* solver, void* matrix, float* x, const float* b, double
 int ka_solver_solve_batched(void* const* solvers, void* const* matrices, float* const* x, const float* const* b,
                             int count, double rel_tol, double abs_tol, int max_iter, int interval, double* stats);
 int ka_solver_describe(void* solver, char* out, int capacity);
+/* The V-cycle forms the autotuner (ka_solver_autotune_tail) picked: the
+ * restriction form per level, the coarsest level's sweeps, the tail level.
+ * No reference counterpart; all forms give the same iterates. */
+int ka_solver_plan(void* solver, char* out, int capacity);
+/* Forces the plan: every level's restriction form (0 one kernel, 1 the same
+ * with 128 coarse rows per workgroup, 2 residual then member sums; -1 keeps)
+ * and the coarsest level's one-workgroup sweeps (0/1; -1 keeps). */
+int ka_solver_set_plan(void* solver, int restrict_mode, int coarse_fused_sweeps);
 int ka_solver_levels(void* solver, int32_t* rows, int32_t* nnz, int capacity);
 /* which: 0 aggregate, 1 member_offsets, 2 members, 3 entry_offsets, 4 entries.
  * Returns the length (copies when out is non-null), or -1. */
@@ -289,6 +297,11 @@ int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used);
 /* HIP graph replay of the pressure solve's PCG iterations (default on;
  * identical iterates; ka_solver_set_graphs). */
 int ka_simplec_set_graphs(void* solver, int on);
+/* The pressure solver's hierarchy (rows and nonzeros per level, up to
+ * capacity; returns the level count) and its autotuned V-cycle plan (as
+ * ka_solver_plan). No reference counterpart. */
+int ka_simplec_pressure_levels(void* solver, int32_t* rows, int32_t* nnz, int capacity);
+int ka_simplec_pressure_plan(void* solver, char* out, int capacity);
 int ka_simplec_centroids(void* solver, double* out);
 
 /* ---- Learned coarse correction (no reference counterpart) ----------------

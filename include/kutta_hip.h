@@ -325,8 +325,20 @@ typedef enum kh_kernel {
      lo x y z, hi x y z), triangle count, out (n floats) =
      float(sqrt(min squared FP64 distance)), +inf without triangles. */
   KH_WALL_DISTANCE,
+  /* The coarsest level's smoothing (linalg/amg.cpp:287-293: coarse_sweeps
+     amg_smooth sweeps from zero, the first as KH_FUSED_SMOOTH_FROM_ZERO) in
+     one workgroup, the iterate ping-ponging in LDS, for a level too large
+     for KH_FUSED_COARSE_TAIL (no reference counterpart; bitwise equal to the
+     per-sweep launches): rows,cols,values,b,inverse,x,status,sweeps
+     (n = rows, at most KH_COARSE_SWEEPS_MAX_ROWS; x receives the result). */
+  KH_FUSED_COARSE_SWEEPS,
+  /* KH_FUSED_RESIDUAL_RESTRICT with 128 coarse rows per workgroup instead of
+     64, for aggregates of about two members (no reference counterpart; same
+     arguments and bits). */
+  KH_FUSED_RESIDUAL_RESTRICT_WIDE,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
+#define KH_COARSE_SWEEPS_MAX_ROWS 16384
 
 /* One system of the batched BiCGStab-Jacobi launches: its operator, its
  * vectors (x, r, q, p, z, s, t, zs, shadow; n floats each), its Jacobi

@@ -105,6 +105,10 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_refresh.argtypes = [vp, vp]
         lib.ka_solver_solve.argtypes = [vp, vp, vp, vp, dbl, dbl, i32, i32, C.POINTER(dbl)]
         lib.ka_solver_describe.argtypes = [vp, C.c_char_p, i32]
+        lib.ka_solver_plan.argtypes = [vp, C.c_char_p, i32]
+        lib.ka_solver_set_plan.argtypes = [vp, i32, i32]
+        lib.ka_simplec_pressure_levels.argtypes = [vp, vp, vp, i32]
+        lib.ka_simplec_pressure_plan.argtypes = [vp, C.c_char_p, i32]
         lib.ka_solver_solve_batched.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, i32, i32, C.POINTER(dbl)]
         lib.ka_solver_levels.argtypes = [vp, pi32, pi32, i32]
         lib.ka_solver_transfer.argtypes = [vp, i32, i32, pi32]
@@ -373,6 +377,8 @@ class Kernel(enum.IntEnum):
     bf16_residual_restrict = 240
     bf16_prolong_smooth = 241
     wall_distance = 242
+    fused_coarse_sweeps = 243
+    fused_residual_restrict_wide = 244
 
 
 class Buffer:
@@ -772,6 +778,16 @@ class LinearSolver:
         host_lib().ka_solver_describe(self.handle, buf, 512)
         return buf.value.decode()
 
+    def set_plan(self, restrict_mode: int = -1, coarse_fused_sweeps: int = -1) -> None:
+        """Forces the V-cycle forms (see plan()); -1 keeps a setting."""
+        _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_fused_sweeps)))
+
+    def plan(self) -> str:
+        """The V-cycle forms the autotuner picked (same iterates for all)."""
+        buf = C.create_string_buffer(512)
+        host_lib().ka_solver_plan(self.handle, buf, 512)
+        return buf.value.decode()
+
     def levels(self) -> list[tuple[int, int]]:
         rows, nnz = (C.c_int32 * 64)(), (C.c_int32 * 64)()
         k = host_lib().ka_solver_levels(self.handle, rows, nnz, 64)
@@ -832,6 +848,9 @@ def _named_array(fn, handle, name: str) -> np.ndarray:
 
 
 C2_SEED = 20261202
+# Largest coarsest level smoothed in one launch (KH_COARSE_SWEEPS_MAX_ROWS).
+COARSE_SWEEPS_MAX_ROWS = 16384
+
 # Limits of the one-workgroup coarse-grid tail (KH_TAIL_* in kutta_hip.h).
 TAIL_MAX_ROWS = 8192
 TAIL_MAX_LEVELS = 16
@@ -1444,6 +1463,19 @@ class SteadySolver:
         used = C.c_int32()
         _check_host(host_lib().ka_simplec_set_batched_momentum(self.handle, int(bool(on)), C.byref(used)))
         return bool(used.value)
+
+    def pressure_levels(self) -> list:
+        rows, nnz = (C.c_int32 * 64)(), (C.c_int32 * 64)()
+        count = host_lib().ka_simplec_pressure_levels(self.handle, rows, nnz, 64)
+        if count < 0:
+            raise KuttaError(host_lib().ka_last_error().decode())
+        return [(rows[i], nnz[i]) for i in range(min(count, 64))]
+
+    def pressure_plan(self) -> str:
+        buf = C.create_string_buffer(512)
+        if host_lib().ka_simplec_pressure_plan(self.handle, buf, 512) < 0:
+            raise KuttaError(host_lib().ka_last_error().decode())
+        return buf.value.decode()
 
     def centroids(self) -> np.ndarray:
         n = self.info()["cells"]

@@ -174,6 +174,17 @@ int ka_solver_describe(void* s, char* out, int capacity) {
   }
   return int(text.size());
 }
+int ka_solver_set_plan(void* s, int restrict_mode, int coarse_fused_sweeps) {
+  return guard([&] { static_cast<Solver*>(s)->solver.set_plan(restrict_mode, coarse_fused_sweeps); });
+}
+int ka_solver_plan(void* s, char* out, int capacity) {
+  const auto text = static_cast<Solver*>(s)->solver.plan();
+  if (capacity > 0) {
+    std::strncpy(out, text.c_str(), std::size_t(capacity));
+    out[capacity - 1] = 0;
+  }
+  return int(text.size());
+}
 int ka_solver_levels(void* s, int32_t* rows, int32_t* nnz, int capacity) {
   const auto levels = static_cast<Solver*>(s)->solver.levels();
   for (std::size_t i = 0; i < levels.size() && int(i) < capacity; ++i) {

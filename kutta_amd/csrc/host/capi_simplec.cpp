@@ -228,6 +228,28 @@ void ka_simplec_destroy(void* solver) { delete static_cast<Simplec*>(solver); }
 int ka_simplec_set_graphs(void* solver, int on) {
   return guard([&] { handle(solver).solver->set_graphs(on != 0); });
 }
+int ka_simplec_pressure_levels(void* solver, int32_t* rows, int32_t* nnz, int capacity) {
+  int count = -1;
+  if (guard([&] {
+        const auto levels = handle(solver).solver->pressure_levels();
+        for (std::size_t i = 0; i < levels.size() && int(i) < capacity; ++i) {
+          rows[i] = levels[i].first;
+          nnz[i] = levels[i].second;
+        }
+        count = int(levels.size());
+      }) != 0)
+    return -1;
+  return count;
+}
+int ka_simplec_pressure_plan(void* solver, char* out, int capacity) {
+  std::string text;
+  if (guard([&] { text = handle(solver).solver->pressure_plan(); }) != 0) return -1;
+  if (capacity > 0) {
+    std::strncpy(out, text.c_str(), std::size_t(capacity));
+    out[capacity - 1] = 0;
+  }
+  return int(text.size());
+}
 int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used) {
   return guard([&] {
     const bool u = handle(solver).solver->set_batched_momentum(on != 0);

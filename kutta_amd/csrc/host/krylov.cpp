@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -47,6 +48,11 @@ struct Level {
   bool has_transfer = false;
   DeviceArray<Index> aggregate, member_offsets, members, entry_offsets, entries;
   Index coarse_rows = 0;
+  // How the fused path restricts this level's residual (all bitwise equal,
+  // picked by the autotuner): 0 one kernel, 64 coarse rows per workgroup;
+  // 1 the same with 128; 2 a row-parallel residual then the member sums,
+  // which reads the fine rows coalesced when aggregates are scattered.
+  int restrict_mode = 0;
 };
 
 // What the caller wants fused into the last sweep of the fine level.
@@ -559,6 +565,27 @@ struct KrylovSolver::Impl {
     bool sum_ready = false;  // the fine-level projection sum is already in `projection`
   };
 
+  // The fused path's restriction of level l's residual into coarse_b, in one
+  // of the restrict_mode forms; `spare` is an iterate buffer free to hold the
+  // fine residual (mode 2).
+  void restrict_level(Level& l, const DeviceCsr& a, const DeviceCsr& ca, const float* current, const float* b,
+                      float* spare, float* coarse_b, int mode) {
+    const int* st = status.data();
+    if (mode == 2) {
+      dev.launch(KH_CSR_RESIDUAL_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, current, b, spare, st});
+      dev.launch(KH_AMG_RESTRICT_GUARDED, ca.n_rows, {l.member_offsets, l.members, spare, coarse_b, st});
+      return;
+    }
+    dev.launch(mode == 1 ? KH_FUSED_RESIDUAL_RESTRICT_WIDE : KH_FUSED_RESIDUAL_RESTRICT, ca.n_rows,
+               {a.row_ptr, a.col, a.val, current, b, l.member_offsets, l.members, coarse_b, st});
+  }
+  bool coarse_fused_sweeps = false;
+  bool coarse_sweeps_fit(std::size_t index) const {
+    const Level& l = levels[index];
+    return index > 0 && index + 1 == levels.size() && fused && !nullspace && !bf16_level(l) && l.rows > 24 &&
+           l.rows <= KH_COARSE_SWEEPS_MAX_ROWS && !(direct_level > 0 && int(index) == direct_level);
+  }
+
   // linalg/amg.cpp:283-310. Sweep outputs alternate between x and the level's
   // scratch so that the last sweep writes x.
   CycleResult cycle(std::size_t index, const DeviceCsr& a, const float* b, float* x, Tail tail) {
@@ -586,6 +613,14 @@ struct KrylovSolver::Impl {
       dev.launch(KH_AMG_COARSE_SOLVE, 1,
                  {a.row_ptr, a.col, a.val, x, b, l.l1, st, int(a.n_rows), options.coarse_sweeps});
       if (nullspace && index != 0) project(x, a.n_rows, sum);
+      return result;
+    }
+    // A coarsest level too large for the coarse tail (aggregation stalls on
+    // anisotropic meshes) may run all its sweeps in one workgroup, bitwise the
+    // sweep-by-sweep sequence below, when the autotuner finds that faster.
+    if (coarse_fused_sweeps && coarse_sweeps_fit(index)) {
+      dev.launch(KH_FUSED_COARSE_SWEEPS, a.n_rows,
+                 {a.row_ptr, a.col, a.val, b, l.l1, x, st, options.coarse_sweeps});
       return result;
     }
     const int pre = coarsest ? options.coarse_sweeps : options.pre_sweeps;
@@ -618,8 +653,7 @@ struct KrylovSolver::Impl {
         dev.launch(KH_BF16_RESIDUAL_RESTRICT, ca.n_rows,
                    {a.row_ptr, a.col, l.val16, current, b, l.member_offsets, l.members, c.b, st});
       } else if (fused) {
-        dev.launch(KH_FUSED_RESIDUAL_RESTRICT, ca.n_rows,
-                   {a.row_ptr, a.col, a.val, current, b, l.member_offsets, l.members, c.b, st});
+        restrict_level(l, a, ca, current, b, current == x ? w : x, c.b.data(), l.restrict_mode);
       } else {
         float* rbuf = current == x ? w : x;
         dev.launch(KH_CSR_RESIDUAL_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, current, b, rbuf, st});
@@ -1164,7 +1198,51 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     if (I.tail_fits(i)) candidates.push_back(int(i));
   DeviceArray<float> v(I.dev, I.n), out(I.dev, I.n);
   I.dev.fill(v.data(), 1.0f, I.n);
-  const int previous = I.tail_request;
+  const auto seconds = [&](const std::function<void()>& body) {
+    body();
+    I.dev.synchronize();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < repeats; ++k) body();
+    I.dev.synchronize();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  const int requested = I.tail_request;
+  // Restriction form per level: time each form on the level's current data
+  // (left by one V-cycle) and keep the fastest. Every form gives the same bits.
+  I.tail_request = 0;
+  I.plan_tail();
+  I.reset(0);
+  I.apply(a, v.data(), out.data(), Tail::None);
+  for (std::size_t i = 0; i + 1 < I.levels.size(); ++i) {
+    Level& l = I.levels[i];
+    if (I.bf16_level(l)) continue;
+    const DeviceCsr li = i == 0 ? a : l.matrix.view();
+    const DeviceCsr ci = I.levels[i + 1].matrix.view();
+    const float* xin = i == 0 ? out.data() : l.x.data();
+    const float* bin = i == 0 ? v.data() : l.b.data();
+    float* spare = l.r.data();
+    DeviceArray<float> coarse(I.dev, ci.n_rows);
+    double best_form = 0;
+    for (int mode = 0; mode < 3; ++mode) {
+      const double t = seconds([&] { I.restrict_level(l, li, ci, xin, bin, spare, coarse.data(), mode); });
+      if (mode == 0 || t < best_form) {
+        best_form = t;
+        l.restrict_mode = mode;
+      }
+    }
+  }
+  // The coarsest level's sweeps: launch by launch or in one workgroup.
+  if (I.coarse_sweeps_fit(I.levels.size() - 1)) {
+    double by_launch = 0;
+    for (int fused_sweeps = 0; fused_sweeps < 2; ++fused_sweeps) {
+      I.coarse_fused_sweeps = fused_sweeps != 0;
+      I.reset(0);
+      const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
+      if (fused_sweeps == 0) by_launch = t;
+      else I.coarse_fused_sweeps = t < by_launch;
+    }
+  }
+  const int previous = requested;
   int best = previous;
   double best_s = 0;
   for (int first : candidates) {
@@ -1255,6 +1333,25 @@ std::string KrylovSolver::describe() const {
   if (impl_->learned && impl_->learned_mode == 2)
     out << ", learned correction on level " << impl_->learned_level << " ("
         << impl_->levels[impl_->learned_level].rows << " rows)";
+  return out.str();
+}
+void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps) {
+  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_fused_sweeps >= -1 && coarse_fused_sweeps <= 1,
+             "solver plan: restriction form -1..2, coarsest sweeps -1..1");
+  impl_->drop_graphs();
+  if (restrict_mode >= 0)
+    for (auto& l : impl_->levels) l.restrict_mode = restrict_mode;
+  if (coarse_fused_sweeps >= 0) impl_->coarse_fused_sweeps = coarse_fused_sweeps != 0;
+}
+std::string KrylovSolver::plan() const {
+  std::ostringstream out;
+  const auto& L = impl_->levels;
+  out << "restriction per level";
+  for (std::size_t i = 0; i + 1 < L.size(); ++i)
+    out << (i ? "," : " ") << (L[i].restrict_mode == 2 ? "split" : (L[i].restrict_mode == 1 ? "wide" : "fused"));
+  if (!L.empty() && impl_->coarse_sweeps_fit(L.size() - 1))
+    out << "; coarsest level " << (impl_->coarse_fused_sweeps ? "one-workgroup sweeps" : "sweep launches");
+  out << "; tail from level " << impl_->tail_first;
   return out.str();
 }
 std::vector<std::pair<Index, Index>> KrylovSolver::levels() const {

@@ -68,6 +68,12 @@ class KrylovSolver {
   // x and b are device vectors of n_rows floats (linalg/solver.cpp:97-176).
   SolveStats solve(const DeviceCsr& a, float* x, const float* b, const SolveControl& control);
   std::string describe() const;
+  // The V-cycle forms the autotuner picked (restriction per level, coarsest
+  // sweeps, tail level); no reference counterpart.
+  std::string plan() const;
+  // Sets every level's restriction form (0 fused, 1 wide, 2 split; -1
+  // keeps) and the coarsest level's one-workgroup sweeps (0/1; -1 keeps).
+  void set_plan(int restrict_mode, int coarse_fused_sweeps);
   std::vector<std::pair<Index, Index>> levels() const;  // (rows, nnz) per level
   // Host copies for parity checks: transfer arrays and coarse values.
   std::vector<Index> transfer(std::size_t level, int which) const;

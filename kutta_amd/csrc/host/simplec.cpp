@@ -1514,6 +1514,10 @@ const MeshGeometry& SteadySolver::geometry() const { return impl_->geom; }
 Index SteadySolver::cells() const { return impl_->n; }
 bool SteadySolver::set_batched_momentum(bool on) { return impl_->set_batched(on); }
 void SteadySolver::set_graphs(bool on) { impl_->pressure_solver->set_graphs(on); }
+std::vector<std::pair<Index, Index>> SteadySolver::pressure_levels() const {
+  return impl_->pressure_solver->levels();
+}
+std::string SteadySolver::pressure_plan() const { return impl_->pressure_solver->plan(); }
 bool SteadySolver::batched_momentum() const { return impl_->batch_momentum; }
 int SteadySolver::host_waits() const {
   int waits = impl_->pressure_solver->host_waits();

@@ -749,19 +749,20 @@ __global__ void __launch_bounds__(kBlock) k_batch_finish2(const kh_bicg_lane* __
 // adds its members' residuals in member order (Kahan), carrying the sum from
 // tile to tile. Every member row gets a lane of its own, as in a sweep.
 constexpr int kRestrictRows = 64;
-template <class V>
+template <class V, int kGroupRows = kRestrictRows>
 __global__ void __launch_bounds__(kBlock) k_fused_residual_restrict(int n,
     const int* __restrict__ rows, const int* __restrict__ cols, const V* __restrict__ values,
     const float* __restrict__ x, const float* __restrict__ b, const int* __restrict__ offsets,
     const int* __restrict__ members, float* __restrict__ coarse, const int* __restrict__ status) {
   if (status[0] != 0) return;
   __shared__ float residual[kBlock];
-  const int g0 = int(xcd_block()) * kRestrictRows;
+  static_assert(kGroupRows <= kBlock, "one owner lane per coarse row");
+  const int g0 = int(xcd_block()) * kGroupRows;
   if (g0 >= n) return;
-  const int g1 = min(g0 + kRestrictRows, n);
+  const int g1 = min(g0 + kGroupRows, n);
   const int first = offsets[g0], last = offsets[g1];
   const int g = g0 + int(threadIdx.x);
-  const bool owner = threadIdx.x < kRestrictRows && g < g1;
+  const bool owner = threadIdx.x < kGroupRows && g < g1;
   const int mine = owner ? offsets[g] : 0, mine_end = owner ? offsets[g + 1] : 0;
   Kahan k;
   for (int tile = first; tile < last; tile += kBlock) {
@@ -813,6 +814,48 @@ __global__ void __launch_bounds__(kBlock) k_fused_smooth_from_zero(int n,
     return;
   }
   next[i] = 0.0f + (2.0f / 3.0f) * inverse[i] * (bi + 0.0f);
+}
+
+// The coarsest level's coarse_sweeps l1-Jacobi sweeps (linalg/amg.cpp:287-293)
+// in one workgroup: the first sweep is k_fused_smooth_from_zero's row update,
+// the others k_amg_smooth's, each row's arithmetic unchanged; the iterate
+// ping-pongs between two LDS arrays (a workgroup barrier between sweeps) and
+// only the last sweep's iterate is stored, into x. The operator stays in
+// global memory, where it is L2-resident at these sizes. One launch replaces
+// coarse_sweeps launches of a few microseconds each on a level with a few
+// thousand rows.
+constexpr int kCoarseSweepThreads = 1024;
+__global__ void __launch_bounds__(kCoarseSweepThreads) k_fused_coarse_sweeps(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ x,
+    const int* __restrict__ status, int sweeps) {
+  if (status[0] != 0) return;
+  __shared__ float iterate[2][KH_COARSE_SWEEPS_MAX_ROWS];
+  for (int i = threadIdx.x; i < n; i += kCoarseSweepThreads) {
+    const float bi = b[i];
+    float next;
+    if (bi - bi != 0.0f) {
+      RowResidual acc(bi);
+      for (int j = rows[i]; j < rows[i + 1]; ++j) acc.add(values[j], 0.0f);
+      next = 0.0f + (2.0f / 3.0f) * inverse[i] * acc.value();
+    } else {
+      next = 0.0f + (2.0f / 3.0f) * inverse[i] * (bi + 0.0f);
+    }
+    iterate[0][i] = next;
+  }
+  int cur = 0;
+  for (int t = 1; t < sweeps; ++t) {
+    __syncthreads();
+    const float* in = iterate[cur];
+    float* out = iterate[cur ^ 1];
+    for (int i = threadIdx.x; i < n; i += kCoarseSweepThreads) {
+      const float r = row_residual(b[i], rows[i], rows[i + 1], cols, values, Plain{in});
+      out[i] = in[i] + (2.0f / 3.0f) * inverse[i] * r;
+    }
+    cur ^= 1;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kCoarseSweepThreads) x[i] = iterate[cur][i];
 }
 
 // FP32 -> bfloat16, round to nearest even (NaN stays NaN): the mixed path's
@@ -1223,6 +1266,18 @@ void l_fused_residual_restrict(kh_device* d, int n, const kh_arg* a) {
   k_fused_residual_restrict<float><<<dim3(groups), dim3(kBlock), 0, d->stream>>>(
       n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6), KF(7), KB(8));
 }
+void l_fused_residual_restrict_wide(kh_device* d, int n, const kh_arg* a) {
+  const unsigned groups = unsigned((n + 127) / 128);
+  k_fused_residual_restrict<float, 128><<<dim3(groups), dim3(kBlock), 0, d->stream>>>(
+      n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KB(5), KB(6), KF(7), KB(8));
+}
+void l_fused_coarse_sweeps(kh_device* d, int n, const kh_arg* a) {
+  if (n > KH_COARSE_SWEEPS_MAX_ROWS)
+    throw std::runtime_error("fused_coarse_sweeps: more rows than KH_COARSE_SWEEPS_MAX_ROWS");
+  if (i32(a, 7) < 1) throw std::runtime_error("fused_coarse_sweeps: at least one sweep");
+  k_fused_coarse_sweeps<<<dim3(1), dim3(kCoarseSweepThreads), 0, d->stream>>>(
+      n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6), i32(a, 7));
+}
 void l_fused_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
   k_fused_prolong_smooth<float><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
                                              KB(7), KCF(8), KB(9));
@@ -1435,6 +1490,9 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_FUSED_PROJECT_DOT_BETA, "fused_project_dot_beta", "bbffbbb", l_fused_project_dot_beta},
       {KH_FUSED_SMOOTH_FROM_ZERO, "fused_smooth_from_zero", "bbbbbbb", l_fused_smooth_from_zero},
       {KH_FUSED_COARSE_TAIL, "fused_coarse_tail", "biiiiibbb", l_fused_coarse_tail},
+      {KH_FUSED_COARSE_SWEEPS, "fused_coarse_sweeps", "bbbbbbbi", l_fused_coarse_sweeps},
+      {KH_FUSED_RESIDUAL_RESTRICT_WIDE, "fused_residual_restrict_wide", "bbbbbbbbb",
+       l_fused_residual_restrict_wide},
   };
   return table;
 }

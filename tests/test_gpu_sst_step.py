@@ -127,7 +127,14 @@ def test_batched_momentum_with_wall_functions_is_bitwise_sequential(device):
 
 def test_backward_step_against_oracle(device):
     """A coarse backward-facing step (the C3 geometry, h = 0.1 m, first cell
-    4 mm, growth 1.3) at 20 m/s, air, 1% intensity with viscosity ratio 10."""
+    4 mm, growth 1.3) at 20 m/s, air, 1% intensity with viscosity ratio 10.
+
+    From the uniform start the first iterations are an impulsive transient
+    (speeds reach 15x the inflow) that amplifies any rounding difference, so
+    the bound is calibrated on the oracle itself: the GPU's distance from the
+    oracle, per field and relative to the field's largest value, must not
+    exceed the oracle's own distance from a run whose initial k is perturbed
+    by one part in a million (about ten float ulps)."""
     g = meshes.StepGeometry(h=0.1, upstream=2.0, downstream=8.0, channel=3.0, span=1.0)
     mesh = meshes.backward_step(g, first=0.004, growth=1.3, largest=0.04, nz=3)
     turb = ka.IntensityViscosityRatio(0.01, 10.0)
@@ -135,23 +142,24 @@ def test_backward_step_against_oracle(device):
             "wall": ka.Wall(), "symmetry": ka.Symmetry()}
     settings = ka.SolverSettings(density=1.225, viscosity=1.7894e-5, monitor_interval=10)
     gpu, cpu = pair(device, mesh, cond, settings, ka.SstSettings())
+    nudged = oracle.SteadySolver.from_mesh(mesh, ka.pack_mesh_conditions(mesh, cond), settings.packed(),
+                                           ka.SstSettings().packed())
     n = mesh.n_cells
     k0 = 1.5 * (0.01 * 20.0) ** 2
     w0 = 1.225 * k0 / (1.7894e-5 * 10.0)
     gpu.initialize_fields((20.0, 0.0, 0.0), 0.0, k0, w0)
-    cpu.initialize_fields(np.full(n, 20.0), np.zeros(n), np.zeros(n), np.zeros(n), np.full(n, k0), np.full(n, w0))
-    # From a uniform start the first iterations are an impulsive transient
-    # (speeds reach 15x the inflow), which amplifies the ulp-level libm
-    # differences; each field is compared relative to its own largest value:
-    # tightly after 5 iterations (measured: at most 4e-4 of the maximum),
-    # loosely after 20, inside the transient (measured: 8e-3, l2 5e-4).
-    for total, tol_max, tol_l2 in ((5, 1e-3, 2e-4), (20, 2e-2, 2e-3)):
+    for solver, k in ((cpu, k0), (nudged, k0 * (1 + 1e-6))):
+        solver.initialize_fields(np.full(n, 20.0), np.zeros(n), np.zeros(n), np.zeros(n), np.full(n, k),
+                                 np.full(n, w0))
+    for total in (5, 20):
         gpu.iterate(total - gpu.info()["iteration"])
-        cpu.iterate(total - int(cpu.report()[0]))
+        for solver in (cpu, nudged):
+            solver.iterate(total - int(solver.report()[0]))
         for name in ("u", "v", "p", "k", "omega"):
             a, b = gpu.field(name).astype(np.float64), cpu.field(name).astype(np.float64)
-            worst = np.max(np.abs(a - b)) / np.max(np.abs(b))
-            l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
-            print(f"[sst] step case, {n} cells, iteration {total}, {name}: max {worst:.2e} of max|b|, "
-                  f"l2 {l2:.2e}", flush=True)
-            assert worst <= tol_max and l2 <= tol_l2, (name, total, worst, l2)
+            c = nudged.field(name).astype(np.float64)
+            scale = np.max(np.abs(b))
+            worst, sensitivity = np.max(np.abs(a - b)) / scale, np.max(np.abs(c - b)) / scale
+            print(f"[sst] step case, {n} cells, iteration {total}, {name}: gpu {worst:.2e}, "
+                  f"oracle perturbed 1e-6: {sensitivity:.2e} (of max|b|)", flush=True)
+            assert worst <= max(sensitivity, 1e-6), (name, total, worst, sensitivity)
