@@ -64,3 +64,31 @@ def test_pinned_pressure_solve_matches_oracle(device, sweeps):
     assert st.iterations == sto["iterations"]
     assert solver.describe() == so.describe()
     assert np.array_equal(bits(x.download()), bits(xo))
+
+
+def test_full_size_pinned_solve_bitwise_vs_oracle(device):
+    """The benchmark solve itself: the 998,250-row C2 operator assembled on the
+    GPU, its seeded zero-mean Gaussian right-hand side (pinned cell zero),
+    aggregation AMG with 2 passes, V(5,5), PCG to 1e-6 from x = 0: the same
+    iteration count and x bit for bit as the oracle on the same bytes."""
+    p = kutta_amd.PressureProblem(device, 55, 55, 55, reference=-2)
+    rng = np.random.default_rng(20261202)
+    b64 = rng.standard_normal(p.n_cells)
+    b64 -= b64.mean()
+    b = b64.astype(np.float32)
+    b[p.reference] = 0
+    options = AmgOptions(pre_sweeps=5, post_sweeps=5, aggregation_passes=2,
+                         structure=OperatorStructure.SymmetricMMatrix)
+    solver = kutta_amd.make_amg_solver(SolverKind.AmgPcg, device, p.host, options)
+    solver.refresh(p)
+    solver.autotune_tail(p, 2)  # whichever plan: the same iterates
+    x = device.alloc(p.n_cells)
+    x.fill(np.float32(0))
+    st = solver.solve(p, x, device.upload(b), SolveControl(1e-6, 0.0, 2000))
+    print(f"[c2 full] {p.n_cells} rows, {st.iterations} iterations, plan: {solver.plan()}", flush=True)
+    a = oracle.Csr.from_arrays(p["row_ptr"], p["col"], p["diag"], p["values"].astype(np.float64))
+    so = oracle.Solver("AmgPcg", a, pre=5, post=5, passes=2, m_matrix=True)
+    xo, sto = so.solve(b, rel_tol=1e-6, abs_tol=0.0, max_iter=2000)
+    assert st.converged and sto["converged"]
+    assert st.iterations == sto["iterations"] == 26
+    assert np.array_equal(bits(x.download()), bits(xo))

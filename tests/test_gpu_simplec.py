@@ -83,7 +83,8 @@ def test_simplec_matches_oracle_bitwise(device, nx, ny, nz, re, paired, iteratio
 
 def test_c1_cavity_first_iterations_match_oracle(device):
     """The 128x128x1 Re=1000 cavity (momentum and pressure relaxation 1,
-    monitor every 100) for 6 iterations, bit for bit."""
+    monitor every 100) for 6 iterations, then to iteration 120, past the
+    first 100-iteration monitor, bit for bit."""
     print("[simplec] C1 128x128x1 Re=1000", flush=True)
     settings = kutta_amd.SolverSettings(viscosity=1e-3, momentum_relaxation=1.0, pressure_relaxation=1.0,
                                         monitor_interval=100)
@@ -93,6 +94,11 @@ def test_c1_cavity_first_iterations_match_oracle(device):
     assert_same_state(gpu, cpu, True)
     assert_same_report(gpu, cpu)
     assert np.all(gpu.field("w") == 0)  # symmetry front and back keep the flow planar
+    gpu.iterate(114)
+    cpu.iterate(114)
+    print("[simplec] C1 after 120 iterations", flush=True)
+    assert_same_state(gpu, cpu, True)
+    assert_same_report(gpu, cpu)
 
 
 @pytest.mark.parametrize("nx,ny,nz,paired", [(24, 20, 1, True), (24, 20, 1, False), (7, 6, 5, True)])
