@@ -162,15 +162,47 @@ def copy_probe(nbytes: int, repeats: int) -> int:
     return 0
 
 
+def fv_touched_bytes(mesh, domain: str, spec: str, args) -> int:
+    """Compulsory bytes of one FV kernel launch: each array once, over the
+    elements the launch can touch. Face arrays touch the launch's face range
+    (the boundary slice for boundary-face kernels, the internal faces for
+    internal-face kernels); a boundary-face kernel reaches only the cells
+    that own boundary faces; everything else is touched whole."""
+    import numpy as np
+    owners = int(np.unique(mesh.owner[mesh.internal:]).size)
+    face_like = {"F", "F+", "oF", "owner"}
+    cell_like = {"C", "C+", "oC", "Cu", "mask", "oCi", "C6", "oC6"}
+    total = 0
+    for role, a in zip(spec.split(), args):
+        if not isinstance(a, np.ndarray):
+            continue
+        per = a.nbytes // max(a.size, 1)
+        width = 6 if role in ("C6", "oC6") else 1
+        if role in face_like and domain == "boundary":
+            count = mesh.boundary
+        elif role in face_like and domain == "internal":
+            count = mesh.internal
+        elif role in cell_like and domain == "boundary":
+            count = owners * width
+        else:
+            count = a.size
+        total += count * per
+    return total
+
+
 def fv_kernel_bench(size: int, repeats: int) -> int:
     """Times every finite-volume kernel on the perturbed size^3 tetrahedral box
     (998,250 cells at size 55) with seeded inputs (kutta_amd.fv_specs) and
     prints one JSON line. Each figure is the wall time of `repeats`
     back-to-back launches on the device stream divided by `repeats`, so a
     kernel shorter than the launch interval reports the interval; the
-    kernel-trace profile gives the execution time alone.
-    Algorithmic bytes count every array argument once at its full size (an
-    array updated in place counts once); the roofline is HBM bandwidth."""
+    kernel-trace profile gives the execution time alone (--trace-summary with
+    this line as --trace-line joins the two).
+    Algorithmic bytes count each array argument once, over the elements the
+    launch touches (fv_touched_bytes: a boundary-face kernel touches the
+    boundary slice of face arrays and only the cells that own boundary
+    faces); an array updated in place counts once. The roofline is HBM
+    bandwidth."""
     import numpy as np
 
     import kutta_amd
@@ -191,7 +223,7 @@ def fv_kernel_bench(size: int, repeats: int) -> int:
             dev.launch(kernel, n, *on_device)
         dev.synchronize()
         seconds = time.perf_counter() - t0
-        nbytes = sum(a.nbytes for a in args if isinstance(a, np.ndarray))
+        nbytes = fv_touched_bytes(mesh, domain, spec, args)
         us = seconds * 1e6 / repeats
         gbs = nbytes / (us * 1e-6) / 1e9
         rows.append({"kernel": name, "n": n, "bytes": nbytes, "us": round(us, 3),
@@ -361,7 +393,8 @@ def trace_summary(db: str, line_json: str | None, top: int) -> int:
     busy time per kernel and grid size (one row per AMG level for the level
     kernels), and for the l1-Jacobi smoother the algorithmic bytes 20n + 8m
     of each level (rows n, nonzeros m from the bench line's "levels"),
-    GB/s and the fraction of the 8 TB/s HBM peak."""
+    GB/s and the fraction of the 8 TB/s HBM peak. With an --fv-kernels line
+    the finite-volume kernels get their algorithmic bytes from it."""
     import re
     import sqlite3
     rows = sqlite3.connect(db).execute("select name, grid_x, duration from kernels").fetchall()
@@ -378,10 +411,14 @@ def trace_summary(db: str, line_json: str | None, top: int) -> int:
         a[0] += 1
         a[1] += ns
     total = sum(v[1] for v in agg.values())
-    levels = []
+    levels, fv_bytes = [], {}
     if line_json:
-        cfg = json.loads(Path(line_json).read_text())["config"]
-        levels = cfg.get("levels", [])
+        line = json.loads(Path(line_json).read_text())
+        levels = line["config"].get("levels", [])
+        # An --fv-kernels line: algorithmic bytes per kernel, keyed by the
+        # functor's name (fv_paired_limiter -> FvPairedLimiter).
+        for k in line.get("kernels", []):
+            fv_bytes["fv " + "".join(w.capitalize() for w in k["kernel"].split("_"))] = k["bytes"]
     print(f"busy {total / 1e6:.2f} ms over {len(rows)} dispatches\n")
     print("| kernel | grid | launches | total ms | avg us | % busy | level rows | bytes/launch | GB/s | frac |")
     print("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
@@ -394,6 +431,10 @@ def trace_summary(db: str, line_json: str | None, top: int) -> int:
                     b = 20 * n_rows + 8 * nnz
                     gbs = b / (avg_us * 1e-6) / 1e9
                     extra = [str(n_rows), str(b), f"{gbs:.0f}", f"{gbs / HBM_PEAK_GBS:.3f}"]
+        elif name in fv_bytes:
+            b = fv_bytes[name]
+            gbs = b / (avg_us * 1e-6) / 1e9
+            extra = ["", str(b), f"{gbs:.0f}", f"{gbs / HBM_PEAK_GBS:.3f}"]
         print(f"| {name} | {grid} | {count} | {ns / 1e6:.2f} | {avg_us:.2f} | {100 * ns / total:.1f} | "
               + " | ".join(extra) + " |")
     return 0
