@@ -576,22 +576,19 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     ms = elapsed * 1e3 / args.steps
 
-    # Fine-level smoothing sweeps are timed on the device stream by HIP events
-    # attached to their own dispatch packets (hipExtLaunchKernelGGL), so the
-    # average is the kernel's execution time, as a kernel trace reports it.
-    # Graph-replayed launches carry no events, so these solves launch kernel
-    # by kernel (the same kernels and iterates).
-    solver.set_graphs(False)
-    dev.profile_reset()
-    dev.profile(Kernel.amg_smooth, True)
-    dev.profile_filter(Kernel.amg_smooth, n)
-    for _ in range(max(args.steps // 2, 2)):
-        x.fill(np.float32(0))
-        solver.solve(problem, x, b, control)
-    dev.synchronize()
-    launches, kernel_ms = dev.profile_read(Kernel.amg_smooth)
-    dev.profile(Kernel.amg_smooth, False)
-    solver.set_graphs(not args.no_graphs)
+    # The fine-level smoothing sweep timed on its own: `repeats` back-to-back
+    # amg_smooth launches over the fine operator (the l1-Jacobi inverse from
+    # krylov_diagonal, x the solution, b the right-hand side), bracketed by one
+    # pair of HIP events on the device stream (kh_time_launches), so the
+    # average is the kernel's execution time plus the gap between launches.
+    repeats = 50
+    inverse, invalid, swept = dev.alloc(n), dev.alloc(n), dev.alloc(n)
+    guard = dev.upload(np.zeros(3, np.int32))
+    dev.launch(Kernel.krylov_diagonal, n, problem.row_ptr, problem.diag, problem.val, inverse, invalid, 1)
+    smooth_args = (problem.row_ptr, problem.col, problem.val, x, b, inverse, swept, guard)
+    dev.launch(Kernel.amg_smooth, n, *smooth_args)
+    kernel_ms = dev.time_launches(Kernel.amg_smooth, n, *smooth_args, repeats=repeats)
+    launches = repeats
     st = stats[-1]
     if not st.converged:
         print(f"[bench] warning: solve did not converge ({st})", file=sys.stderr)

@@ -336,6 +336,14 @@ typedef enum kh_kernel {
      64, for aggregates of about two members (no reference counterpart; same
      arguments and bits). */
   KH_FUSED_RESIDUAL_RESTRICT_WIDE,
+  /* kernels/scalars.kern.h:157-186: the optional K-cycle's two inner Krylov
+     corrections of a coarse equation (linalg/amg.cpp:315-336), on the
+     level's 12 pair scalars and 1-int status. */
+  KH_AMG_INNER_ALPHA,            /* :157-166 scalars,status,parent_status,first */
+  KH_AMG_INNER_FIRST,            /* :167-172 b,p,q,x,t,scalars,status (unguarded) */
+  KH_AMG_INNER_CHECK,            /* :173-176 scalars,status */
+  KH_AMG_INNER_BETA,             /* :177-181 scalars,status */
+  KH_AMG_INNER_STEP,             /* :182-186 x,p,scalars,status */
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 #define KH_COARSE_SWEEPS_MAX_ROWS 16384
@@ -454,6 +462,12 @@ int kh_profile_enable(kh_device* dev, uint32_t kernel, int enable);
 int kh_profile_filter(kh_device* dev, uint32_t kernel, size_t n);
 int kh_profile_read(kh_device* dev, uint32_t kernel, long* launches, double* total_ms);
 int kh_profile_reset(kh_device* dev);
+/* Times `repeats` back-to-back launches of one kernel (no reference
+ * counterpart): one HIP event before the first and one after the last on the
+ * device stream, then a synchronization; *total_ms is the elapsed time of the
+ * whole batch (kernel execution plus the gaps between launches). */
+int kh_time_launches(kh_device* dev, uint32_t kernel, size_t n, const kh_arg* args, size_t nargs,
+                     int repeats, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,8 @@ def hip_lib() -> C.CDLL:
         lib.kh_profile_enable.argtypes = [vp, C.c_uint32, i32]
         lib.kh_profile_read.argtypes = [vp, C.c_uint32, C.POINTER(C.c_long), C.POINTER(C.c_double)]
         lib.kh_profile_reset.argtypes = [vp]
+        lib.kh_time_launches.argtypes = [vp, C.c_uint32, sz, C.POINTER(KhArg), sz, C.c_int,
+                                         C.POINTER(C.c_double)]
         _hip = lib
     return _hip
 
@@ -379,6 +381,11 @@ class Kernel(enum.IntEnum):
     wall_distance = 242
     fused_coarse_sweeps = 243
     fused_residual_restrict_wide = 244
+    amg_inner_alpha = 245
+    amg_inner_first = 246
+    amg_inner_check = 247
+    amg_inner_beta = 248
+    amg_inner_step = 249
 
 
 class Buffer:
@@ -535,6 +542,15 @@ class Device:
         lib.kh_profile_filter.argtypes = [C.c_void_p, C.c_uint32, C.c_size_t]
         _check_hip(lib.kh_profile_filter(self.kh, int(kernel), int(n)))
 
+    def time_launches(self, kernel, n: int, *args, repeats: int = 20) -> float:
+        """Milliseconds of `repeats` back-to-back launches, bracketed by one
+        pair of HIP events on the device stream."""
+        arr = (KhArg * max(len(args), 1))(*[_arg(a) for a in args])
+        ms = C.c_double()
+        _check_hip(hip_lib().kh_time_launches(self.kh, int(kernel), int(n), arr, len(args), int(repeats),
+                                              C.byref(ms)))
+        return ms.value
+
     def profile_read(self, kernel) -> tuple[int, float]:
         n, ms = C.c_long(), C.c_double()
         _check_hip(hip_lib().kh_profile_read(self.kh, int(kernel), C.byref(n), C.byref(ms)))
@@ -566,9 +582,17 @@ class OperatorStructure(enum.IntEnum):
     SymmetricMMatrix = 1
 
 
+class AmgCycle(enum.IntEnum):
+    """linalg/amg.hpp:6"""
+    V = 0
+    K = 1
+
+
 @dataclass
 class AmgOptions:
-    """linalg/amg.hpp:8-20 (V-cycle)."""
+    """linalg/amg.hpp:8-20. k_depth (no reference counterpart) stops the
+    K-cycle's inner corrections below that coarse level; 0 keeps them
+    wherever the reference uses them."""
     pre_sweeps: int = 5
     post_sweeps: int = 5
     coarse_sweeps: int = 32
@@ -578,11 +602,13 @@ class AmgOptions:
     strength_threshold: float = 0.25
     nullspace: NullSpace = NullSpace.NONE
     structure: OperatorStructure = OperatorStructure.General
+    cycle: AmgCycle = AmgCycle.V
+    k_depth: int = 0
 
     def packed(self):
-        return (C.c_int32 * 8)(self.pre_sweeps, self.post_sweeps, self.coarse_sweeps,
-                               self.coarse_rows, self.max_levels, self.aggregation_passes,
-                               int(self.nullspace), int(self.structure))
+        return (C.c_int32 * 10)(self.pre_sweeps, self.post_sweeps, self.coarse_sweeps,
+                                self.coarse_rows, self.max_levels, self.aggregation_passes,
+                                int(self.nullspace), int(self.structure), int(self.cycle), self.k_depth)
 
 
 @dataclass
@@ -917,13 +943,28 @@ class PressureProblem:
     def assemble(self, reference: int = -2) -> None:
         _check_host(host_lib().ka_pressure_problem_assemble(self.handle, reference))
 
+    def _view(self, fn, n, dtype):
+        b = Buffer.__new__(Buffer)
+        b.device, b.dtype, b.n = self.device, np.dtype(dtype), n
+        b.ptr = getattr(host_lib(), fn)(self.handle)
+        b.free = lambda: None  # owned by the problem
+        return b
+
     @property
     def val(self) -> Buffer:
-        b = Buffer.__new__(Buffer)
-        b.device, b.dtype, b.n = self.device, np.dtype(np.float32), self.nnz
-        b.ptr = host_lib().ka_matrix_values(self.handle)
-        b.free = lambda: None
-        return b
+        return self._view("ka_matrix_values", self.nnz, np.float32)
+
+    @property
+    def row_ptr(self) -> Buffer:
+        return self._view("ka_matrix_row_ptr", self.n_rows + 1, np.int32)
+
+    @property
+    def col(self) -> Buffer:
+        return self._view("ka_matrix_col", self.nnz, np.int32)
+
+    @property
+    def diag(self) -> Buffer:
+        return self._view("ka_matrix_diag", self.n_rows, np.int32)
 
     def __del__(self):
         try:

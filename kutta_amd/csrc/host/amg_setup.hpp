@@ -32,6 +32,9 @@ enum class NullSpace : std::uint8_t { None, Constant };
 enum class Structure : std::uint8_t { General, SymmetricMMatrix };
 
 // linalg/amg.hpp:8-20 (V-cycle options).
+// linalg/amg.hpp:6 — V: one recursive coarse correction; K: two inner
+// preconditioned Krylov corrections per coarse level.
+enum class AmgCycle : std::uint8_t { V, K };
 struct AmgOptions {
   int pre_sweeps = 5, post_sweeps = 5, coarse_sweeps = 32;
   Index coarse_rows = 24;
@@ -39,6 +42,11 @@ struct AmgOptions {
   double strength_threshold = 0.25;
   NullSpace nullspace = NullSpace::None;
   Structure structure = Structure::General;
+  AmgCycle cycle = AmgCycle::V;
+  // K-cycle only (no reference counterpart): the inner corrections stop at
+  // this coarse level, plain recursion below; 0 uses them wherever the
+  // reference does (every coarse level with two levels below it).
+  int k_depth = 0;
 };
 
 struct Coarsening {

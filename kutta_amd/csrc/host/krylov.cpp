@@ -53,6 +53,10 @@ struct Level {
   // 1 the same with 128; 2 a row-parallel residual then the member sums,
   // which reads the fine rows coalesced when aggregates are scattered.
   int restrict_mode = 0;
+  // K-cycle workspaces of a coarse level (linalg/amg.cpp:185-193): the inner
+  // corrections' vectors, 12 pair scalars and 1-int status.
+  DeviceArray<float> kp, kq, kz, kt, kscalars;
+  DeviceArray<Index> kstatus;
 };
 
 // What the caller wants fused into the last sweep of the fine level.
@@ -409,6 +413,14 @@ struct KrylovSolver::Impl {
       coarse.b = DeviceArray<float>(dev, c.matrix.n_rows);
       coarse.r = DeviceArray<float>(dev, c.matrix.n_rows);
       if (nullspace) coarse.projection = DeviceArray<float>(dev, 2);
+      if (options.cycle == AmgCycle::K) {
+        for (auto* v : {&coarse.kp, &coarse.kq, &coarse.kz, &coarse.kt})
+          *v = DeviceArray<float>(dev, c.matrix.n_rows);
+        coarse.kscalars = DeviceArray<float>(dev, 12);
+        coarse.kstatus = DeviceArray<Index>(dev, 1);
+        dev.fill(coarse.kscalars.data(), 0.0f, 12);
+        dev.fill_int(coarse.kstatus.data(), 0, 1);
+      }
       coarse.matrix = OwnedCsr(dev, c.matrix);
       coarse.host = c.matrix;
       owned = std::move(c.matrix);
@@ -422,7 +434,7 @@ struct KrylovSolver::Impl {
   // iterates and right-hand sides, one sweep's new values and the largest
   // level's operator copy in LDS at once.
   bool tail_fits(std::size_t first) const {
-    if (first < 1 || first >= levels.size() || levels.size() - first > KH_TAIL_MAX_LEVELS) return false;
+    if (first < tail_floor() || first >= levels.size() || levels.size() - first > KH_TAIL_MAX_LEVELS) return false;
     std::size_t vectors = 0, widest = 0, largest = 0;
     for (std::size_t i = first; i < levels.size(); ++i) {
       const std::size_t rows = std::size_t(levels[i].rows), nnz = std::size_t(levels[i].nnz);
@@ -539,8 +551,7 @@ struct KrylovSolver::Impl {
   // One l1-Jacobi sweep (linalg/amg.cpp:272-282). `first` marks the sweep that
   // starts from the zero iterate the cycle's fill would have stored.
   void sweep(Level& l, const DeviceCsr& a, const float* b, const float* in, float* out, bool first,
-             float* zero_x) {
-    const int* st = status.data();
+             float* zero_x, const int* st) {
     if (bf16_level(l)) {
       if (first)
         dev.launch(KH_BF16_SMOOTH_FROM_ZERO, a.n_rows, {a.row_ptr, a.col, l.val16, b, l.l1, out, st});
@@ -569,8 +580,7 @@ struct KrylovSolver::Impl {
   // of the restrict_mode forms; `spare` is an iterate buffer free to hold the
   // fine residual (mode 2).
   void restrict_level(Level& l, const DeviceCsr& a, const DeviceCsr& ca, const float* current, const float* b,
-                      float* spare, float* coarse_b, int mode) {
-    const int* st = status.data();
+                      float* spare, float* coarse_b, int mode, const int* st) {
     if (mode == 2) {
       dev.launch(KH_CSR_RESIDUAL_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, current, b, spare, st});
       dev.launch(KH_AMG_RESTRICT_GUARDED, ca.n_rows, {l.member_offsets, l.members, spare, coarse_b, st});
@@ -587,11 +597,12 @@ struct KrylovSolver::Impl {
   }
 
   // linalg/amg.cpp:283-310. Sweep outputs alternate between x and the level's
-  // scratch so that the last sweep writes x.
-  CycleResult cycle(std::size_t index, const DeviceCsr& a, const float* b, float* x, Tail tail) {
+  // scratch so that the last sweep writes x. `st` guards every launch: the
+  // solver's status, or that of the K-cycle correction the cycle belongs to.
+  CycleResult cycle(std::size_t index, const DeviceCsr& a, const float* b, float* x, Tail tail,
+                    const int* st) {
     Level& l = levels[index];
     float* w = l.r.data();
-    const int* st = status.data();
     CycleResult result;
     if (fused && tail_first > 0 && int(index) == tail_first) {
       dev.launch(KH_FUSED_COARSE_TAIL, 1,
@@ -636,12 +647,12 @@ struct KrylovSolver::Impl {
     auto run_sweep = [&](int t, float* out) {
       if (t == total && fuse_dot) {
         dev.launch(KH_FUSED_SMOOTH_DOT_BETA, a.n_rows,
-                   {a.row_ptr, a.col, a.val, current, b, l.l1, out, status.data(), scalars.data()});
+                   {a.row_ptr, a.col, a.val, current, b, l.l1, out, st, scalars.data()});
         result.dot_beta = true;
       } else if (t == total && fuse_sum) {
         dev.launch(KH_FUSED_SMOOTH_SUM, a.n_rows, {a.row_ptr, a.col, a.val, current, b, l.l1, out, st, sum});
       } else {
-        sweep(l, a, b, current, out, t == 1, x);
+        sweep(l, a, b, current, out, t == 1, x, st);
       }
       current = out;
     };
@@ -653,14 +664,15 @@ struct KrylovSolver::Impl {
         dev.launch(KH_BF16_RESIDUAL_RESTRICT, ca.n_rows,
                    {a.row_ptr, a.col, l.val16, current, b, l.member_offsets, l.members, c.b, st});
       } else if (fused) {
-        restrict_level(l, a, ca, current, b, current == x ? w : x, c.b.data(), l.restrict_mode);
+        restrict_level(l, a, ca, current, b, current == x ? w : x, c.b.data(), l.restrict_mode, st);
       } else {
         float* rbuf = current == x ? w : x;
         dev.launch(KH_CSR_RESIDUAL_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, current, b, rbuf, st});
         dev.launch(KH_AMG_RESTRICT_GUARDED, ca.n_rows, {l.member_offsets, l.members, rbuf, c.b, st});
       }
       if (nullspace) project(c.b.data(), ca.n_rows, c.projection.data());
-      cycle(index + 1, ca, c.b.data(), c.x.data(), Tail::None);
+      if (k_inner(index + 1)) inner(index + 1, st);
+      else cycle(index + 1, ca, c.b.data(), c.x.data(), Tail::None, st);
       for (int t = pre + 1; t <= total; ++t) {
         float* out = target(t);
         if (t == pre + 1) {
@@ -698,14 +710,14 @@ struct KrylovSolver::Impl {
   void apply(const DeviceCsr& a, const float* rin, float* zout, Tail tail) {
     CycleResult done;
     if (amg) {
-      done = cycle(0, a, rin, zout, tail);
+      done = cycle(0, a, rin, zout, variable() ? Tail::None : tail, status.data());
     } else {
       dev.launch(KH_KRYLOV_MULTIPLY_GUARDED, n, {rin, jacobi_inverse, zout, status});
     }
     if (done.dot_beta) return;
     if (nullspace) {
       if (!done.sum_ready) dev.reduce_into(KH_INTO_SUM, zout, zout, zout, zout, n, projection.data(), 0);
-      if (tail == Tail::DotBeta && fused) {
+      if (tail == Tail::DotBeta && fused && !variable()) {
         const float c = static_cast<float>(n);
         dev.launch(KH_FUSED_PROJECT_DOT_BETA, n,
                    {zout, projection, c, static_cast<float>(double(n) - double(c)), r, scalars, status});
@@ -714,9 +726,63 @@ struct KrylovSolver::Impl {
       project_with(zout, n, projection.data());
     }
     if (tail == Tail::DotBeta) {
-      dot_into(r.data(), zout, 4);
-      dev.launch(KH_SOLVER_PCG_BETA, 1, {scalars, status, 0});
+      // linalg/solver.cpp:295-297: a variable preconditioner (the K-cycle)
+      // takes the flexible beta, -alpha z.Ap / rho.
+      if (variable()) dev.reduce_into(KH_INTO_DOT2, r.data(), zout, zout, q.data(), n, scalars.data(), 4);
+      else dot_into(r.data(), zout, 4);
+      dev.launch(KH_SOLVER_PCG_BETA, 1, {scalars, status, variable() ? 1 : 0});
     }
+  }
+
+  // linalg/amg.cpp:262: the K-cycle is a variable preconditioner.
+  bool variable() const { return amg && options.cycle == AmgCycle::K; }
+  // Whether coarse level `c` is reached through the K-cycle's inner
+  // corrections (linalg/amg.cpp:301-302): every level with two below it,
+  // down to k_depth when set, and never at or below a dense-solved level.
+  bool k_inner(std::size_t c) const {
+    return variable() && c + 1 < levels.size() && (options.k_depth == 0 || c <= std::size_t(options.k_depth)) &&
+           !(direct_level > 0 && c >= std::size_t(direct_level));
+  }
+  // The first level the one-workgroup tail (a V-cycle) may start at.
+  std::size_t tail_floor() const {
+    std::size_t c = 1;
+    while (c < levels.size() && k_inner(c)) ++c;
+    return c;
+  }
+
+  // linalg/amg.cpp:315-336: two preconditioned Krylov corrections of the
+  // coarse equation A x = b at `index` (A-conjugate directions for SPD, GCR
+  // otherwise), each cycle guarded by `parent` then by the level's own status,
+  // which also skips the second correction once |t| <= |b| / 4.
+  void inner(std::size_t index, const int* parent) {
+    Level& l = levels[index];
+    const DeviceCsr a = l.matrix.view();
+    const std::size_t m = std::size_t(a.n_rows);
+    const bool spd = kind == SolverKind::AmgPcg;
+    float* s = l.kscalars.data();
+    Index* st = l.kstatus.data();
+    float *b = l.b.data(), *x = l.x.data(), *p = l.kp.data(), *q = l.kq.data(), *z = l.kz.data(),
+          *t = l.kt.data(), *w = l.r.data();
+    auto spmv = [&](const float* in, float* out, const int* guard) {
+      dev.launch(KH_CSR_SPMV_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, in, out, guard});
+    };
+    cycle(index, a, b, z, Tail::None, parent);
+    dev.copy(p, z, sizeof(float) * m);
+    spmv(p, q, parent);
+    dev.reduce_into(KH_INTO_DOT2, spd ? p : q, q, b, spd ? p : q, m, s, 0);
+    dev.launch(KH_AMG_INNER_ALPHA, 1, {s, st, parent, 1});
+    dev.launch(KH_AMG_INNER_FIRST, a.n_rows, {b, p, q, x, t, s, st});
+    dev.reduce_into(KH_INTO_DOT2, t, t, b, b, m, s, 4);
+    dev.launch(KH_AMG_INNER_CHECK, 1, {s, st});
+    cycle(index, a, t, z, Tail::None, st);
+    if (!spd) spmv(z, w, st);
+    dev.reduce_into(KH_INTO_DOT, spd ? z : w, q, spd ? z : w, q, m, s, 4);
+    dev.launch(KH_AMG_INNER_BETA, 1, {s, st});
+    dev.launch(KH_SOLVER_DIRECTION, a.n_rows, {z, p, s, st});
+    spmv(p, q, st);
+    dev.reduce_into(KH_INTO_DOT2, spd ? p : q, q, t, spd ? p : q, m, s, 0);
+    dev.launch(KH_AMG_INNER_ALPHA, 1, {s, st, parent, 0});
+    dev.launch(KH_AMG_INNER_STEP, a.n_rows, {x, p, s, st});
   }
 
   void residual(const DeviceCsr& a, const float* x, const float* b, float* out) {
@@ -1224,7 +1290,8 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     DeviceArray<float> coarse(I.dev, ci.n_rows);
     double best_form = 0;
     for (int mode = 0; mode < 3; ++mode) {
-      const double t = seconds([&] { I.restrict_level(l, li, ci, xin, bin, spare, coarse.data(), mode); });
+      const double t =
+          seconds([&] { I.restrict_level(l, li, ci, xin, bin, spare, coarse.data(), mode, I.status.data()); });
       if (mode == 0 || t < best_form) {
         best_form = t;
         l.restrict_mode = mode;
@@ -1322,7 +1389,8 @@ std::string KrylovSolver::describe() const {
                                             : "BiCGStab")
       << ", ";
   if (impl_->amg)
-    out << "aggregation AMG V-cycle, " << impl_->levels.size() << " levels, op complexity "
+    out << "aggregation AMG " << (impl_->options.cycle == AmgCycle::K ? 'K' : 'V') << "-cycle, "
+        << impl_->levels.size() << " levels, op complexity "
         << std::fixed << std::setprecision(3) << impl_->complexity;
   else out << "Jacobi";
   if (impl_->nullspace) out << ", constant-nullspace projection";

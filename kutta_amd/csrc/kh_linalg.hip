@@ -337,6 +337,50 @@ __global__ void k_solver_direction(int n, const float* __restrict__ z, float* __
   const int i = row_index();
   if (i < n) p[i] = z[i] + pfloat(pload(s, 10)) * p[i];
 }
+// kernels/scalars.kern.h:157-186: the optional K-cycle's inner corrections.
+// Slots of a level's 12 scalars: 0 the step's denominator (p.Ap for SPD,
+// |Ap|^2 otherwise), 2 its numerator, 4 t.t then the conjugation product,
+// 6 b.b, 8 alpha, 10 beta. `first` makes the level's status follow the
+// parent's; a failed first step is phase 2 (keep the first correction
+// unscaled), a failed second one phase 1 (keep the first step).
+__global__ void k_amg_inner_alpha(float* s, int* status, const int* parent, int first) {
+  if (threadIdx.x != 0) return;
+  if (first != 0) status[0] = parent[0] == 0 ? 0 : 2;
+  if (status[0] != 0) return;
+  const Pair denom = pload(s, 0), alpha = pdiv(pload(s, 2), denom);
+  if (!ppositive(denom) || !pcoefficient(alpha) || (pzero(alpha) && !pzero(pload(s, 2))))
+    status[0] = first != 0 ? 2 : 1;
+  else pstore(s, 8, alpha);
+}
+// Unguarded: x = alpha p and t = b - alpha q while running, else x = p, t = 0.
+__global__ void k_amg_inner_first(int n, const float* __restrict__ b, const float* __restrict__ p,
+    const float* __restrict__ q, float* __restrict__ x, float* __restrict__ t,
+    const float* __restrict__ s, const int* __restrict__ status) {
+  const int i = row_index();
+  if (i >= n) return;
+  const bool running = status[0] == 0;
+  const float alpha = pfloat(pload(s, 8));
+  x[i] = running ? alpha * p[i] : p[i];
+  t[i] = running ? b[i] - alpha * q[i] : 0.0f;
+}
+// Adaptive exit: the second correction is skipped once t.t <= b.b / 16.
+__global__ void k_amg_inner_check(const float* s, int* status) {
+  if (threadIdx.x != 0 || status[0] != 0) return;
+  if (ple(pload(s, 4), pmul({0.0625f, 0.0f}, pload(s, 6)))) status[0] = 1;
+}
+__global__ void k_amg_inner_beta(float* s, int* status) {
+  if (threadIdx.x != 0 || status[0] != 0) return;
+  const Pair beta = pdiv(pneg(pload(s, 4)), pload(s, 0));
+  if (!pcoefficient(beta) || (pzero(beta) && !pzero(pload(s, 4)))) status[0] = 1;
+  else pstore(s, 10, beta);
+}
+__global__ void k_amg_inner_step(int n, float* __restrict__ x, const float* __restrict__ p,
+    const float* __restrict__ s, const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  if (i < n) x[i] += pfloat(pload(s, 8)) * p[i];
+}
+
 // kernels/scalars.kern.h:132-137
 __global__ void k_solver_bicg_direction(int n, const float* __restrict__ r,
     const float* __restrict__ v, float* __restrict__ p, const float* __restrict__ s,
@@ -1229,6 +1273,21 @@ void l_solver_step(kh_device* d, int n, const kh_arg* a) {
 void l_solver_direction(kh_device* d, int n, const kh_arg* a) {
   k_solver_direction<<<GRID(n)>>>(n, KCF(0), KF(1), KCF(2), KB(3));
 }
+void l_amg_inner_alpha(kh_device* d, int, const kh_arg* a) {
+  k_amg_inner_alpha<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)), KB(2), i32(a, 3));
+}
+void l_amg_inner_first(kh_device* d, int n, const kh_arg* a) {
+  k_amg_inner_first<<<GRID(n)>>>(n, KCF(0), KCF(1), KCF(2), KF(3), KF(4), KCF(5), KB(6));
+}
+void l_amg_inner_check(kh_device* d, int, const kh_arg* a) {
+  k_amg_inner_check<<<SINGLE>>>(KCF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_amg_inner_beta(kh_device* d, int, const kh_arg* a) {
+  k_amg_inner_beta<<<SINGLE>>>(KF(0), static_cast<int*>(buf(a, 1)));
+}
+void l_amg_inner_step(kh_device* d, int n, const kh_arg* a) {
+  k_amg_inner_step<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2), KB(3));
+}
 void l_solver_bicg_direction(kh_device* d, int n, const kh_arg* a) {
   k_solver_bicg_direction<<<GRID(n)>>>(n, KCF(0), KCF(1), KF(2), KCF(3), KB(4));
 }
@@ -1493,6 +1552,11 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_FUSED_COARSE_SWEEPS, "fused_coarse_sweeps", "bbbbbbbi", l_fused_coarse_sweeps},
       {KH_FUSED_RESIDUAL_RESTRICT_WIDE, "fused_residual_restrict_wide", "bbbbbbbbb",
        l_fused_residual_restrict_wide},
+      {KH_AMG_INNER_ALPHA, "amg_inner_alpha", "bbbi", l_amg_inner_alpha},
+      {KH_AMG_INNER_FIRST, "amg_inner_first", "bbbbbbb", l_amg_inner_first},
+      {KH_AMG_INNER_CHECK, "amg_inner_check", "bb", l_amg_inner_check},
+      {KH_AMG_INNER_BETA, "amg_inner_beta", "bb", l_amg_inner_beta},
+      {KH_AMG_INNER_STEP, "amg_inner_step", "bbbb", l_amg_inner_step},
   };
   return table;
 }

@@ -405,6 +405,35 @@ int kh_launch(kh_device* d, uint32_t kernel, size_t n, const kh_arg* args, size_
   });
 }
 
+int kh_time_launches(kh_device* d, uint32_t kernel, size_t n, const kh_arg* args, size_t nargs, int repeats,
+                     double* total_ms) {
+  return guard([&] {
+    require(d != nullptr && total_ms != nullptr, "kh_time_launches: null argument");
+    require(repeats >= 1, "kh_time_launches: repeats must be positive");
+    no_capture(d, "kh_time_launches");
+    hipEvent_t start = nullptr, stop = nullptr;
+    check(hipEventCreate(&start), "hipEventCreate");
+    check(hipEventCreate(&stop), "hipEventCreate");
+    int status = 0;
+    try {
+      check(hipEventRecord(start, d->stream), "hipEventRecord");
+      for (int k = 0; k < repeats && status == 0; ++k) status = kh_launch(d, kernel, n, args, nargs);
+      check(hipEventRecord(stop, d->stream), "hipEventRecord");
+      check(hipEventSynchronize(stop), "hipEventSynchronize");
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
+      *total_ms = ms;
+    } catch (...) {
+      (void)hipEventDestroy(start);
+      (void)hipEventDestroy(stop);
+      throw;
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    if (status != 0) throw std::runtime_error(kh_last_error());
+  });
+}
+
 int kh_reduce_host(kh_device* d, const kh_reduction* req, size_t count, double* out) {
   return guard([&] {
     require(d != nullptr, "kh_reduce_host: null device");

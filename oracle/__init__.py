@@ -69,6 +69,9 @@ def lib() -> C.CDLL:
         L.oracle_solver_transfer.restype = lng
         L.oracle_solver_level_values.argtypes = [vp, i32, vp]
         L.oracle_solver_level_values.restype = lng
+        L.oracle_solver_level_array.argtypes = [vp, i32, i32, vp]
+        L.oracle_solver_level_array.restype = lng
+        L.oracle_solver_apply.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -160,9 +163,12 @@ KINDS ={"Jacobi": 0, "PcgJacobi": 1, "BicgstabJacobi": 2, "AmgPcg": 3, "AmgBicgs
 
 class Solver:
     def __init__(self, kind: str, a: Csr, pre=5, post=5, coarse_sweeps=32, coarse_rows=24,
-                 max_levels=24, passes=3, nullspace=False, m_matrix=False, threshold=0.25):
+                 max_levels=24, passes=3, nullspace=False, m_matrix=False, threshold=0.25, cycle="V",
+                 k_depth=0):
+        """cycle "V" or "K" (linalg/amg.hpp:6); k_depth > 0 stops the K-cycle's
+        inner corrections below that coarse level."""
         opts = np.array([pre, post, coarse_sweeps, coarse_rows, max_levels, passes,
-                         int(nullspace), int(m_matrix)], dtype=np.int32)
+                         int(nullspace), int(m_matrix), "VK".index(cycle), k_depth], dtype=np.int32)
         h = C.c_void_p()
         check(lib().oracle_solver_create(KINDS[kind], a.handle, _p(opts), threshold, C.byref(h)))
         self.handle = h.value
@@ -199,6 +205,23 @@ class Solver:
         out = np.empty(n, np.int32)
         lib().oracle_solver_transfer(self.handle, level, w, _p(out))
         return out
+
+    def level_array(self, level: int, which: str):
+        """row_ptr, col, diag (coarse levels) or l1 (the l1-Jacobi inverse)."""
+        w = ["row_ptr", "col", "diag", "l1"].index(which)
+        n = lib().oracle_solver_level_array(self.handle, level, w, None)
+        if n < 0:
+            return None
+        out = np.empty(n, np.float32 if which == "l1" else np.int32)
+        lib().oracle_solver_level_array(self.handle, level, w, _p(out))
+        return out
+
+    def apply(self, r) -> np.ndarray:
+        """One preconditioner application M^-1 r."""
+        rr = _f32(r)
+        z = np.zeros(self.n, np.float32)
+        check(lib().oracle_solver_apply(self.handle, _p(rr), _p(z)))
+        return z
 
     def level_values(self, level: int):
         n = lib().oracle_solver_level_values(self.handle, level, None)

@@ -126,6 +126,18 @@ void oracle_amg_m_matrix_diagonal(int n, const int* offsets, const int* members,
                                   const int* rows, const int* diag, float* coarse) {
   kern::amg_m_matrix_diagonal(n, offsets, members, fine_rows, fine_diag, fine, rows, diag, coarse);
 }
+// The K-cycle's inner kernels (kernels/scalars.kern.h:157-186): which 0 alpha
+// (flag = first), 1 first (n entries), 2 check, 3 beta, 4 step (n entries).
+void oracle_amg_inner(int which, int n, const float* b, const float* p, const float* q, float* x,
+                      float* t, float* scalars, int* status, const int* parent, int flag) {
+  switch (which) {
+    case 0: kern::amg_inner_alpha(scalars, status, parent, flag); break;
+    case 1: kern::amg_inner_first(n, b, p, q, x, t, scalars, status); break;
+    case 2: kern::amg_inner_check(scalars, status); break;
+    case 3: kern::amg_inner_beta(scalars, status); break;
+    case 4: kern::amg_inner_step(n, x, p, scalars, status); break;
+  }
+}
 void oracle_solver_scalar(int which, float* scalars, int* status, int flag) {
   switch (which) {
     case 0: kern::solver_reset(scalars, status, flag); break;
@@ -202,7 +214,8 @@ long oracle_coarsening_values(void* c, double* out) {
 void oracle_coarsening_free(void* c) { delete static_cast<Coarsening*>(c); }
 
 // ---- solvers (linalg/amg.cpp:345-382, linalg/solver.cpp:75-176)
-// options: pre, post, coarse_sweeps, coarse_rows, max_levels, passes, nullspace, structure
+// options[10]: pre, post, coarse_sweeps, coarse_rows, max_levels, passes, nullspace,
+// structure, cycle (0 V, 1 K), K-cycle depth (0: every level)
 int oracle_solver_create(int kind, void* csr, const int* options, double threshold, void** out) {
   return guard([&] {
     AmgOptions o;
@@ -210,6 +223,9 @@ int oracle_solver_create(int kind, void* csr, const int* options, double thresho
     o.coarse_rows = options[3]; o.max_levels = options[4]; o.aggregation_passes = options[5];
     o.nullspace = options[6] ? NullSpace::Constant : NullSpace::None;
     o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
+    ORACLE_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0, "AMG: invalid options");
+    o.cycle = options[8] ? AmgCycle::K : AmgCycle::V;
+    o.k_depth = options[9];
     o.strength_threshold = threshold;
     *out = new SolverHandle(static_cast<SolverKind>(kind), *static_cast<CsrHost*>(csr), o);
   });
@@ -260,6 +276,33 @@ long oracle_solver_transfer(void* h, int level, int which, int* out) {
       : which == 2 ? &t->members : which == 3 ? &t->entry_offsets : &t->entries;
   if (out) std::memcpy(out, v->data(), v->size() * sizeof(int));
   return long(v->size());
+}
+// One preconditioner application to r (n floats) into z.
+int oracle_solver_apply(void* h, const float* r, float* z) {
+  return guard([&] {
+    auto* s = static_cast<SolverHandle*>(h);
+    const std::size_t n = std::size_t(s->matrix.n_rows);
+    std::vector<float> rv(r, r + n), zv(n, 0.0f);
+    s->solver.apply(s->matrix, rv, zv);
+    std::memcpy(z, zv.data(), n * sizeof(float));
+  });
+}
+// A level's arrays: which 0 row_ptr, 1 col, 2 diag (level >= 1), 3 the
+// l1-Jacobi inverse (any level). Returns the length; copies when out is
+// non-null; -1 when absent.
+long oracle_solver_level_array(void* h, int level, int which, void* out) {
+  const auto& solver = static_cast<SolverHandle*>(h)->solver;
+  if (which == 3) {
+    const auto* l1 = solver.level_l1(std::size_t(level));
+    if (!l1) return -1;
+    if (out) std::memcpy(out, l1->data(), l1->size() * sizeof(float));
+    return long(l1->size());
+  }
+  const auto* m = solver.level_matrix(std::size_t(level));
+  if (!m || which < 0 || which > 2) return -1;
+  const std::vector<Index>& v = which == 0 ? m->row_ptr : which == 1 ? m->col : m->diag;
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(Index));
+  return long(v.size());
 }
 // Coarse operator values after the latest refresh (level >= 1).
 long oracle_solver_level_values(void* h, int level, float* out) {

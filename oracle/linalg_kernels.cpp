@@ -392,6 +392,46 @@ void solver_bicg_step(int n, float* x, float* r, const float* z, const float* zs
     else { x[i] += alpha * z[i] + omega * zs[i]; r[i] = sv[i] - omega * t[i]; }
   }
 }
+// kernels/scalars.kern.h:157-166 — the K-cycle's inner step length. `first`
+// makes the level's status follow the parent's (a frozen parent freezes the
+// level with phase 2); a failed first step is phase 2 (keep the unscaled
+// first correction), a failed second one phase 1 (keep the first step).
+void amg_inner_alpha(float* s, int* status, const int* parent, int first) {
+  if (first != 0) status[0] = parent[0] == 0 ? 0 : 2;
+  if (status[0] != 0) return;
+  Pair denom = load(s, 0), alpha = div(load(s, 2), denom);
+  if (!positive(denom) || !coefficient(alpha) || (zero(alpha) && !zero(load(s, 2))))
+    status[0] = first != 0 ? 2 : 1;
+  else store(s, 8, alpha);
+}
+// kernels/scalars.kern.h:167-172 — unguarded: x = alpha p and t = b - alpha q
+// while running, else x = p and t = 0.
+void amg_inner_first(int n, const float* b, const float* p, const float* q, float* x, float* t,
+                     const float* s, const int* status) {
+  const float alpha = to_float(load(s, 8));
+  for (int i = 0; i < n; ++i) {
+    x[i] = status[0] == 0 ? alpha * p[i] : p[i];
+    t[i] = status[0] == 0 ? b[i] - alpha * q[i] : 0.0f;
+  }
+}
+// kernels/scalars.kern.h:173-176 — adaptive exit: t.t <= b.b / 16.
+void amg_inner_check(const float* s, int* status) {
+  if (status[0] != 0) return;
+  if (le(load(s, 4), mul(kp(0.0625f, 0.0f), load(s, 6)))) status[0] = 1;
+}
+// kernels/scalars.kern.h:177-181
+void amg_inner_beta(float* s, int* status) {
+  if (status[0] != 0) return;
+  Pair beta = div(neg(load(s, 4)), load(s, 0));
+  if (!coefficient(beta) || (zero(beta) && !zero(load(s, 4)))) status[0] = 1;
+  else store(s, 10, beta);
+}
+// kernels/scalars.kern.h:182-186
+void amg_inner_step(int n, float* x, const float* p, const float* s, const int* status) {
+  if (status[0] != 0) return;
+  const float alpha = to_float(load(s, 8));
+  for (int i = 0; i < n; ++i) x[i] += alpha * p[i];
+}
 // kernels/scalars.kern.h:150-152
 void solver_project(int n, float* x, const float* sum, float count_hi, float count_lo) {
   for (int i = 0; i < n; ++i) x[i] -= to_float(div(load(sum, 0), kp(count_hi, count_lo)));

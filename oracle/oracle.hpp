@@ -114,6 +114,13 @@ void solver_bicg_step(int n, float* x, float* r, const float* z, const float* zs
                       const float* s, const float* t, const float* scalars,
                       const int* status);
 void solver_project(int n, float* x, const float* sum, float count_hi, float count_lo);
+// The optional K-cycle's inner corrections (kernels/scalars.kern.h:157-186).
+void amg_inner_alpha(float* scalars, int* status, const int* parent, int first);
+void amg_inner_first(int n, const float* b, const float* p, const float* q, float* x, float* t,
+                     const float* scalars, const int* status);
+void amg_inner_check(const float* scalars, int* status);
+void amg_inner_beta(float* scalars, int* status);
+void amg_inner_step(int n, float* x, const float* p, const float* scalars, const int* status);
 }  // namespace kern
 
 // ------------------------------------------------------------------------- CSR
@@ -143,6 +150,9 @@ struct SolveStats {
 enum class SolverKind : std::uint8_t { Jacobi, PcgJacobi, BicgstabJacobi, AmgPcg, AmgBicgstab };
 enum class NullSpace : std::uint8_t { None, Constant };
 enum class Structure : std::uint8_t { General, SymmetricMMatrix };
+// linalg/amg.hpp:6 — V: one recursive coarse correction; K: two inner
+// preconditioned Krylov corrections per coarse level.
+enum class AmgCycle : std::uint8_t { V, K };
 struct AmgOptions {
   int pre_sweeps = 5, post_sweeps = 5, coarse_sweeps = 32;
   Index coarse_rows = 24;
@@ -150,6 +160,11 @@ struct AmgOptions {
   double strength_threshold = 0.25;
   NullSpace nullspace = NullSpace::None;
   Structure structure = Structure::General;
+  AmgCycle cycle = AmgCycle::V;
+  // K-cycle only: the inner corrections stop at this coarse level (plain
+  // recursion below it); 0 uses them wherever the reference does (every
+  // level with two levels below it).
+  int k_depth = 0;
 };
 
 // The aggregation of one level (linalg/amg.cpp:17-20).
@@ -190,6 +205,10 @@ class Solver {
   // The hierarchy's transfer arrays and coarse values, for parity checks.
   const Coarsening* transfer(std::size_t level) const;
   const CsrDevice* level_matrix(std::size_t level) const;
+  const std::vector<float>* level_l1(std::size_t level) const;
+  // One preconditioner application z = M^-1 r with a running status (the
+  // V- or K-cycle, or Jacobi), for checks of the cycle itself.
+  void apply(const CsrDevice& a, const std::vector<float>& r, std::vector<float>& z);
 
  private:
   struct Impl;

@@ -155,6 +155,16 @@ void ref_solver_scalar(int which, float* s, int* st, int flag) {
     case 7: refk::solver_check(0, s, st, flag); break;
   }
 }
+void ref_amg_inner(int which, int n, const float* b, const float* p, const float* q, float* x,
+                   float* t, float* s, int* st, const int* parent, int flag) {
+  switch (which) {
+    case 0: refk::amg_inner_alpha(0, s, st, parent, flag); break;
+    case 1: LOOP(amg_inner_first, b, p, q, x, t, s, st); break;
+    case 2: refk::amg_inner_check(0, s, st); break;
+    case 3: refk::amg_inner_beta(0, s, st); break;
+    case 4: LOOP(amg_inner_step, x, p, s, st); break;
+  }
+}
 void ref_solver_project(int n, float* x, const float* sum, float hi, float lo) {
   LOOP(solver_project, x, sum, hi, lo);
 }

@@ -214,6 +214,9 @@ using V = std::vector<float>;
 struct Level {
   CsrDevice matrix;
   V l1, x, b, r, projection;
+  // K-cycle workspaces of a coarse level (linalg/amg.cpp:185-193).
+  V p, q, z, t, scalars;
+  std::vector<int> status;
   Coarsening transfer;  // valid when a coarser level exists
   bool has_transfer = false;
 };
@@ -308,6 +311,11 @@ struct Solver::Impl {
       l.b.assign(c.matrix.n_rows, 0.0f);
       l.r.assign(c.matrix.n_rows, 0.0f);
       if (nullspace) l.projection.assign(2, 0.0f);
+      if (options.cycle == AmgCycle::K) {
+        for (V* v : {&l.p, &l.q, &l.z, &l.t}) v->assign(c.matrix.n_rows, 0.0f);
+        l.scalars.assign(12, 0.0f);
+        l.status.assign(1, 0);
+      }
       l.matrix = upload(c.matrix);
       owned = c.matrix;
       levels.back().transfer = std::move(c);
@@ -346,46 +354,89 @@ struct Solver::Impl {
     }
   }
 
-  // linalg/amg.cpp:272-282 — ping-pong between x and the level's r.
-  void smooth(Level& l, const CsrDevice& a, const V& b, V& x, int sweeps, bool zero_start) {
+  // linalg/amg.cpp:272-282 — ping-pong between x and the level's r. `st` is
+  // the status that guards the sweeps: the solver's, or an inner correction's.
+  void smooth(Level& l, const CsrDevice& a, const V& b, V& x, int sweeps, bool zero_start,
+              const int* st) {
     for (int sweep = 0; sweep < sweeps; ++sweep) {
       if (zero_start && sweep == 0)
         kern::amg_smooth_zero(a.n_rows, a.row_ptr.data(), a.col.data(), a.val.data(), x.data(),
-                              b.data(), l.l1.data(), l.r.data(), status.data());
+                              b.data(), l.l1.data(), l.r.data(), st);
       else
         kern::amg_smooth(a.n_rows, a.row_ptr.data(), a.col.data(), a.val.data(), x.data(),
-                         b.data(), l.l1.data(), l.r.data(), status.data());
+                         b.data(), l.l1.data(), l.r.data(), st);
       std::swap(x, l.r);
     }
   }
-  // linalg/amg.cpp:283-310 — V-cycle.
-  void cycle(std::size_t index, const CsrDevice& a, const V& b, V& x) {
+  bool k_inner(std::size_t coarse) const {
+    return options.cycle == AmgCycle::K && coarse + 1 < levels.size() &&
+           (options.k_depth == 0 || coarse <= std::size_t(options.k_depth));
+  }
+  // linalg/amg.cpp:283-310 — V-cycle, or K-cycle where k_inner holds.
+  void cycle(std::size_t index, const CsrDevice& a, const V& b, V& x, const int* st) {
     auto& l = levels[index];
     std::fill(x.begin(), x.end(), 0.0f);
     if (index + 1 == levels.size()) {
       if (a.n_rows <= 24)
         kern::amg_coarse_solve(1, a.row_ptr.data(), a.col.data(), a.val.data(), x.data(), b.data(),
-                               l.l1.data(), status.data(), a.n_rows, options.coarse_sweeps);
-      else smooth(l, a, b, x, options.coarse_sweeps, true);
+                               l.l1.data(), st, a.n_rows, options.coarse_sweeps);
+      else smooth(l, a, b, x, options.coarse_sweeps, true, st);
     } else {
-      smooth(l, a, b, x, options.pre_sweeps, true);
+      smooth(l, a, b, x, options.pre_sweeps, true, st);
       kern::csr_residual_guarded(a.n_rows, a.row_ptr.data(), a.col.data(), a.val.data(), x.data(),
-                                 b.data(), l.r.data(), status.data());
+                                 b.data(), l.r.data(), st);
       auto& c = levels[index + 1];
       kern::amg_restrict_guarded(c.matrix.n_rows, l.transfer.member_offsets.data(),
-                                 l.transfer.members.data(), l.r.data(), c.b.data(), status.data());
+                                 l.transfer.members.data(), l.r.data(), c.b.data(), st);
       if (nullspace) project(c.b, c.projection);
-      cycle(index + 1, c.matrix, c.b, c.x);
-      kern::amg_prolong_guarded(a.n_rows, l.transfer.aggregate.data(), c.x.data(), x.data(),
-                                status.data());
-      smooth(l, a, b, x, options.post_sweeps, false);
+      if (k_inner(index + 1)) inner(index + 1, st);
+      else cycle(index + 1, c.matrix, c.b, c.x, st);
+      kern::amg_prolong_guarded(a.n_rows, l.transfer.aggregate.data(), c.x.data(), x.data(), st);
+      smooth(l, a, b, x, options.post_sweeps, false, st);
     }
     if (nullspace && index != 0) project(x, l.projection);
   }
 
+  // linalg/amg.cpp:315-336 — two preconditioned Krylov corrections of the
+  // coarse equation A x = b at `index` (A-conjugate for SPD, GCR otherwise),
+  // the second skipped on the device once the first leaves |t| <= |b| / 4.
+  void inner(std::size_t index, const int* parent) {
+    auto& l = levels[index];
+    const auto& a = l.matrix;
+    const int n = a.n_rows;
+    const bool spd = kind == SolverKind::AmgPcg;
+    float* s = l.scalars.data();
+    int* st = l.status.data();
+    auto spmv = [&](const V& in, V& out, const int* guard) {
+      kern::csr_spmv_guarded(n, a.row_ptr.data(), a.col.data(), a.val.data(), in.data(), out.data(), guard);
+    };
+    auto dot2 = [&](const V& x, const V& y, const V& u, const V& v, int offset) {
+      reduce_into(x.data(), y.data(), u.data(), v.data(), n, s + offset, Reduction::Dot2);
+    };
+    cycle(index, a, l.b, l.z, parent);
+    l.p = l.z;
+    spmv(l.p, l.q, parent);
+    dot2(spd ? l.p : l.q, l.q, l.b, spd ? l.p : l.q, 0);
+    kern::amg_inner_alpha(s, st, parent, 1);
+    kern::amg_inner_first(n, l.b.data(), l.p.data(), l.q.data(), l.x.data(), l.t.data(), s, st);
+    dot2(l.t, l.t, l.b, l.b, 4);
+    kern::amg_inner_check(s, st);
+    cycle(index, a, l.t, l.z, st);
+    if (!spd) spmv(l.z, l.r, st);
+    const V& w = spd ? l.z : l.r;
+    reduce_into(w.data(), l.q.data(), w.data(), l.q.data(), n, s + 4, Reduction::Dot);
+    kern::amg_inner_beta(s, st);
+    kern::solver_direction(n, l.z.data(), l.p.data(), s, st);
+    spmv(l.p, l.q, st);
+    dot2(spd ? l.p : l.q, l.q, l.t, spd ? l.p : l.q, 0);
+    kern::amg_inner_alpha(s, st, parent, 0);
+    kern::amg_inner_step(n, l.x.data(), l.p.data(), s, st);
+  }
+  bool variable() const { return amg && options.cycle == AmgCycle::K; }
+
   // linalg/solver.cpp:197-200
   void apply(const CsrDevice& a, const V& rin, V& zout) {
-    if (amg) cycle(0, a, rin, zout);
+    if (amg) cycle(0, a, rin, zout, status.data());
     else kern::krylov_multiply_guarded(int(rin.size()), rin.data(), jacobi_inverse.data(),
                                        zout.data(), status.data());
     if (nullspace) project(zout, projection);
@@ -451,8 +502,10 @@ struct Solver::Impl {
         kern::solver_check(scalars.data(), status.data(), 1);
         if (st.iterations + j + 1 == c.max_iter) continue;
         apply(a, r, z);
-        dot_into(r, z, 4);
-        kern::solver_pcg_beta(scalars.data(), status.data(), 0);
+        if (variable())
+          reduce_into(r.data(), z.data(), z.data(), q.data(), n, scalars.data() + 4, Reduction::Dot2);
+        else dot_into(r, z, 4);
+        kern::solver_pcg_beta(scalars.data(), status.data(), variable() ? 1 : 0);
         kern::solver_direction(n, z.data(), p.data(), scalars.data(), status.data());
       }
       st.iterations = status[1];
@@ -587,7 +640,8 @@ std::string Solver::describe() const {
   const bool pcg = impl_->kind == SolverKind::PcgJacobi || impl_->kind == SolverKind::AmgPcg;
   out << (pcg ? "PCG" : "BiCGStab") << ", ";
   if (impl_->amg)
-    out << "aggregation AMG V-cycle, " << impl_->levels.size() << " levels, op complexity "
+    out << "aggregation AMG " << (impl_->options.cycle == AmgCycle::K ? 'K' : 'V') << "-cycle, "
+        << impl_->levels.size() << " levels, op complexity "
         << std::fixed << std::setprecision(3) << impl_->complexity;
   else out << "Jacobi";
   if (impl_->nullspace) out << ", constant-nullspace projection";
@@ -608,6 +662,16 @@ const Coarsening* Solver::transfer(std::size_t level) const {
 const CsrDevice* Solver::level_matrix(std::size_t level) const {
   if (level == 0 || level >= impl_->levels.size()) return nullptr;
   return &impl_->levels[level].matrix;
+}
+const std::vector<float>* Solver::level_l1(std::size_t level) const {
+  if (level >= impl_->levels.size()) return nullptr;
+  return &impl_->levels[level].l1;
+}
+void Solver::apply(const CsrDevice& a, const std::vector<float>& r, std::vector<float>& z) {
+  ORACLE_REQUIRE(a.n_rows == impl_->n && r.size() == std::size_t(impl_->n) && z.size() == r.size(),
+                 "apply: size mismatch");
+  impl_->status.assign(3, 0);
+  impl_->apply(a, r, z);
 }
 
 // tests/linalg/matrices.hpp:13-15
