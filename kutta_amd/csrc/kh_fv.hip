@@ -730,6 +730,64 @@ struct FvTransportFaces {
     correction[i] = mu[i] * diffusion - (second_order != 0 ? convection : 0.0f);
   }
 };
+// The transport row's storage during assembly: the matrix itself, or, for
+// rows of at most kSlots entries, registers (a slot is read from memory only
+// if no face of the cell writes it, and every slot is stored once at the
+// end). Either way each entry sees the same additions in the same order.
+struct MemoryRow {
+  float* values;
+  int begin, end, di;
+  __device__ void clear(int e) { values[e] = 0.0f; }
+  __device__ void fill_untouched() {}
+  __device__ void add(int e, float v) { values[e] += v; }
+  template <class F>
+  __device__ void each_off_diagonal(F f) const {
+    for (int j = begin; j < end; ++j)
+      if (j != di) f(values[j]);
+  }
+  __device__ void set_diagonal(float v) { values[di] = v; }
+  __device__ void store() {}
+};
+struct RegisterRow {
+  static constexpr int kSlots = 16;
+  float* values;
+  int begin, len, dk;
+  unsigned touched = 0;
+  float slot[kSlots] = {};
+  __device__ void clear(int e) {
+    const int k = e - begin;
+    if (k >= 0 && k < kSlots) touched |= 1u << k;
+  }
+  __device__ void fill_untouched() {
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+      if (s < len && !((touched >> s) & 1u)) slot[s] = values[begin + s];
+  }
+  // Every slot takes an addition, -0.0 (the exact identity of IEEE addition)
+  // for all but the entry's own: a compare-and-branch chain is folded back
+  // into a dynamically indexed store, which would put the row in scratch.
+  __device__ void add(int e, float v) {
+    const int k = e - begin;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) slot[s] += s == k ? v : -0.0f;
+  }
+  template <class F>
+  __device__ void each_off_diagonal(F f) const {
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+      if (s < len && s != dk) f(slot[s]);
+  }
+  __device__ void set_diagonal(float v) {
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) slot[s] = s == dk ? v : slot[s];
+  }
+  __device__ void store() {
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+      if (s < len) values[begin + s] = slot[s];
+  }
+};
+
 // :37-112. One transport-matrix row per lane, written into the row's own
 // entries only: off-diagonals -D + min(F, 0); the diagonal as the upward-
 // rounded sum of |off-diagonals|, net outflow, boundary anchors and sinks;
@@ -742,6 +800,25 @@ struct FvTransportAssemble {
                              CF extra_diagonal, CF extra_rhs, WF values, WF rhs, WF rhs_low,
                              WF diagonal, WF diagonal_low, WF consistent_inverse, int internal,
                              float rho, float dt, float relax, int compensated) const {
+    const int begin = rows[i], end = rows[i + 1], di = diag[i];
+    if (end - begin <= RegisterRow::kSlots) {
+      RegisterRow row{values, begin, end - begin, di - begin};
+      assemble(row, i, offsets, faces, sign, adjacent, entry, diffusion, flux, mu, kind, boundary, correction,
+               source, volume, old, sink, extra_diagonal, extra_rhs, rhs, rhs_low, diagonal, diagonal_low,
+               consistent_inverse, internal, rho, dt, relax, compensated);
+    } else {
+      MemoryRow row{values, begin, end, di};
+      assemble(row, i, offsets, faces, sign, adjacent, entry, diffusion, flux, mu, kind, boundary, correction,
+               source, volume, old, sink, extra_diagonal, extra_rhs, rhs, rhs_low, diagonal, diagonal_low,
+               consistent_inverse, internal, rho, dt, relax, compensated);
+    }
+  }
+  template <class Row>
+  __device__ __forceinline__ static void assemble(Row& row, int i, CI offsets, CI faces, CF sign, CI adjacent, CI entry,
+                                  CF diffusion, CF flux, CF mu, CI kind, CF boundary, CF correction, CF source,
+                                  CF volume, CF old, CF sink, CF extra_diagonal, CF extra_rhs, WF rhs,
+                                  WF rhs_low, WF diagonal, WF diagonal_low, WF consistent_inverse, int internal,
+                                  float rho, float dt, float relax, int compensated) {
     const float vol = volume[i], sink_volume = sink[i] * vol;
     UpperSum ap = upper_add({sink_volume, 0.0f}, extra_diagonal[i]);
     float b = source[i] * vol + extra_rhs[i], carry = 0.0f;
@@ -751,16 +828,17 @@ struct FvTransportAssemble {
       b = start.hi;
       carry = start.lo + __builtin_fmaf(source[i], vol, -product);
     }
-    const int begin = offsets[i], end = offsets[i + 1];
-    for (int j = begin; j < end; ++j)
-      if (adjacent[j] >= 0) values[entry[j]] = 0.0f;
-    for (int j = begin; j < end; ++j) {
+    const int first = offsets[i], last = offsets[i + 1];
+    for (int j = first; j < last; ++j)
+      if (adjacent[j] >= 0) row.clear(entry[j]);
+    row.fill_untouched();
+    for (int j = first; j < last; ++j) {
       const int f = faces[j], o = adjacent[j];
       const float mass = sign[j] * flux[f], inflow = kmin(mass, 0.0f);
       float d = mu[f] * diffusion[f];
       ap = upper_add(ap, mass);
       if (o >= 0) {
-        values[entry[j]] += -d + inflow;
+        row.add(entry[j], -d + inflow);
       } else {
         const int bf = f - internal;
         if (kind[bf] != 0) d = 0.0f;
@@ -786,9 +864,7 @@ struct FvTransportAssemble {
         b += deferred;
       }
     }
-    const int di = diag[i], row_begin = rows[i], row_end = rows[i + 1];
-    for (int j = row_begin; j < row_end; ++j)
-      if (j != di) ap = upper_add(ap, -values[j]);
+    row.each_off_diagonal([&](float v) { ap = upper_add(ap, -v); });
     const float physical = upper_value(ap);
     diagonal[i] = physical;
     if (compensated) {
@@ -807,12 +883,12 @@ struct FvTransportAssemble {
     } else {
       b += (relaxed - physical) * previous;
     }
-    values[di] = relaxed;
+    row.set_diagonal(relaxed);
     rhs[i] = b;
     if (compensated) rhs_low[i] = carry;
     UpperSum denominator{relaxed, 0.0f};
-    for (int j = row_begin; j < row_end; ++j)
-      if (j != di) denominator = upper_add(denominator, values[j]);
+    row.each_off_diagonal([&](float v) { denominator = upper_add(denominator, v); });
+    row.store();
     const float consistent = upper_value(denominator);
     consistent_inverse[i] = 1.0f / kmax(consistent > 0.0f ? consistent : relaxed, 1.0e-20f);
   }

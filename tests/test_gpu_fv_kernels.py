@@ -55,3 +55,58 @@ def test_fv_kernel_bitwise_vs_oracle(device, mesh, name):
         for k, a in enumerate(args):
             if isinstance(a, np.ndarray) and k not in outputs:
                 assert np.array_equal(on_device[k].download().view(np.uint8), a.view(np.uint8)), (name, k)
+
+
+def ring_mesh(cells=300, reach=10):
+    """A synthetic finite-volume connectivity whose matrix rows are longer than
+    the transport assembly's register row (16 entries): cell i shares a face
+    with cells i +- 1 ... i +- reach (mod cells), 2 reach + 1 = 21 entries per
+    row, and owns one boundary face."""
+    owner, neighbour = [], []
+    for k in range(1, reach + 1):
+        for i in range(cells):
+            owner.append(i)
+            neighbour.append((i + k) % cells)
+    internal = len(owner)
+    owner += list(range(cells))
+    cell_faces = [[] for _ in range(cells)]
+    for f in range(internal):
+        cell_faces[owner[f]].append((f, 1.0, neighbour[f]))
+        cell_faces[neighbour[f]].append((f, -1.0, owner[f]))
+    for i in range(cells):
+        cell_faces[i].append((internal + i, 1.0, -1))
+    rows, cols, diag = [0], [], []
+    offsets, faces, sign, adjacent, entry = [0], [], [], [], []
+    for i in range(cells):
+        row = sorted({i} | {o for _, _, o in cell_faces[i] if o >= 0})
+        start = len(cols)
+        cols += row
+        diag.append(start + row.index(i))
+        rows.append(len(cols))
+        for f, s, o in cell_faces[i]:
+            faces.append(f)
+            sign.append(s)
+            adjacent.append(o)
+            entry.append(start + row.index(o) if o >= 0 else -1)
+        offsets.append(len(faces))
+    i32 = lambda v: np.asarray(v, np.int32)  # noqa: E731
+    return {"owner": i32(owner), "neighbour": i32(neighbour), "cell_face_offsets": i32(offsets),
+            "cell_faces": i32(faces), "cell_face_sign": np.asarray(sign, np.float32), "adjacent": i32(adjacent),
+            "matrix_entry": i32(entry), "row_ptr": i32(rows), "diag": i32(diag), "col": i32(cols)}
+
+
+def test_transport_assemble_long_rows_bitwise(device):
+    """Rows of 21 entries take the assembly's in-memory row form; shorter rows
+    (every box mesh above) take the register form."""
+    mesh = FvMesh(ring_mesh())
+    assert np.diff(mesh.rows).min() > 16
+    name = "fv_transport_assemble"
+    domain, spec = SPECS[name]
+    for seed in (5, 6):
+        args, outputs = build_args(mesh, spec, seed=seed)
+        on_device = [device.upload(a) if isinstance(a, np.ndarray) else a for a in args]
+        expected = [a.copy() if isinstance(a, np.ndarray) else a for a in args]
+        device.launch(kutta_amd.Kernel[name], mesh.cells, *on_device)
+        oracle.fv_each(name, mesh.cells, *expected)
+        for k in outputs:
+            assert same_bits(on_device[k].download(), expected[k]), (seed, k)
