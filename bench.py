@@ -22,7 +22,8 @@ on one host core beside it.
     python bench.py --fv-kernels [--steps K]         # each FV kernel at C2 size
     python bench.py --simplec-box N [--steps K]       # SIMPLEC on the N^3 lid-driven box
 
-The line also carries "simplec": ms per steady SIMPLEC iteration on the
+The line also carries "large_mesh": the plain path and the K-cycle variant
+timed on the 110^3 box (7,986,000 cells), and "simplec": ms per steady SIMPLEC iteration on the
 128x128x1 lid-driven cavity at Re=1000 (500 iterations, iterations 6-500
 timed), with the oracle timed on one core beside it, and "simplec_sst": ms
 per SST k-omega SIMPLEC iteration on the 4.06M-cell backward-facing step
@@ -88,6 +89,93 @@ def fp64_error(x, x64) -> dict:
     d = x.astype(np.float64) - x64
     return {"l2": float(np.linalg.norm(d) / np.linalg.norm(x64)),
             "max_over_max": float(np.max(np.abs(d)) / np.max(np.abs(x64)))}
+
+
+# The fast pressure variant (no reference counterpart as a configuration; the
+# cycle is the reference's AmgCycle::K): flexible AMG-PCG whose coarse levels 1
+# and 2 take the K-cycle's two inner Krylov corrections, plain V-cycle
+# recursion below, one pre- and one post-smoothing sweep, 8 sweeps of the
+# coarsest level. Picked by timing the 8M-cell solve over sweeps 1-5, depths
+# 1-4 and 2-32 coarsest sweeps.
+KCYCLE = dict(pre_sweeps=1, post_sweeps=1, coarse_sweeps=8, k_depth=2)
+
+
+def time_solver(dev, problem, options, kind, b, x, control, steps, warmup, graphs, direct=0, bf16=False,
+                tail=-1):
+    """Builds a solver on the problem's operator, refreshes it (timed), lets
+    the autotuner plan its V-cycle (unless a dense level replaces the tail),
+    and times `steps` solves of b from x = 0 after `warmup`. Returns the
+    solver, its last stats and the figures."""
+    import numpy as np
+
+    import kutta_amd
+    solver = kutta_amd.make_amg_solver(kind, dev, problem.host, options)
+    level = solver.set_direct(direct) if direct else 0
+    solver.set_coarse_bf16(bf16)
+    solver.set_graphs(graphs)
+    solver.set_tail(tail)
+    t = time.perf_counter()
+    solver.refresh(problem)
+    dev.synchronize()
+    refresh_ms = (time.perf_counter() - t) * 1e3
+    if tail < 0 and not level:
+        solver.autotune_tail(problem, 5)
+    for _ in range(max(warmup, 1)):
+        x.fill(np.float32(0))
+        solver.solve(problem, x, b, control)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x.fill(np.float32(0))
+        st = solver.solve(problem, x, b, control)
+    dev.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return solver, st, {"ms": round(ms, 4), "iterations": st.iterations,
+                        "relative_residual": st.final_residual / st.initial_residual, "converged": st.converged,
+                        "refresh_ms": round(refresh_ms, 2), "refresh_plus_solve_ms": round(ms + refresh_ms, 4),
+                        "direct_level": level, "hierarchy": solver.describe(), "plan": solver.plan()}
+
+
+def large_mesh_comparison(dev, size, steps, warmup, graphs, args) -> dict:
+    """The plain FP32 AMG-PCG path (V(5,5), the reference's configuration) and
+    the K-cycle variant on the size^3 box (7,986,000 cells at 110): ms to
+    1e-6, with and without the refresh each solve needs when the operator
+    changes (every SIMPLEC outer iteration), and each solution's error
+    against the FP64 solution of the same operator."""
+    import numpy as np
+
+    import kutta_amd
+    from kutta_amd import AmgCycle, AmgOptions, OperatorStructure, PressureProblem, SolveControl, SolverKind
+    t = time.perf_counter()
+    problem = PressureProblem(dev, size, size, size, tet=True, amplitude=0.2, seed=SEED, reference=-2)
+    setup_s = time.perf_counter() - t
+    b_host = rhs_for(problem.n_cells, problem.reference)
+    b, x = dev.upload(b_host), dev.alloc(problem.n_cells)
+    control = SolveControl(rel_tol=1e-6, abs_tol=0.0, max_iter=2000)
+    base = dict(aggregation_passes=2, structure=OperatorStructure.SymmetricMMatrix)
+    plain, _, out_plain = time_solver(dev, problem, AmgOptions(pre_sweeps=5, post_sweeps=5, **base),
+                                      SolverKind.AmgPcg, b, x, control, steps, warmup, graphs)
+    x_plain = x.download()
+
+    def gpu_solve(rhs):
+        x.fill(np.float32(0))
+        plain.solve(problem, x, dev.upload(rhs), control)
+        return x.download()
+
+    x64, x64_residual, _ = fp64_reference(problem, b_host, x_plain, gpu_solve)
+    out_plain["fp64_error"] = fp64_error(x_plain, x64)
+    del plain
+    _, _, out_k = time_solver(dev, problem, AmgOptions(cycle=AmgCycle.K, **KCYCLE, **base), SolverKind.AmgPcg,
+                              b, x, control, steps, warmup, graphs)
+    out_k["fp64_error"] = fp64_error(x.download(), x64)
+    out_k["speedup_vs_plain_pcg"] = round(out_plain["ms"] / out_k["ms"], 3)
+    out_k["speedup_refresh_plus_solve"] = round(out_plain["refresh_plus_solve_ms"] / out_k["refresh_plus_solve_ms"], 3)
+    print(f"[bench] {problem.n_cells} cells: plain {out_plain['ms']:.2f} ms ({out_plain['iterations']} it), "
+          f"K-cycle {out_k['ms']:.2f} ms ({out_k['iterations']} it), {out_k['speedup_vs_plain_pcg']}x",
+          file=sys.stderr, flush=True)
+    return {"workload": f"the same pressure solve on the {size}^3 box ({problem.n_cells} cells, {problem.nnz} nnz)",
+            "setup_s": round(setup_s, 2), "fp64_relative_residual": x64_residual,
+            "plain_pcg": out_plain, "pcg_kcycle": out_k}
 
 
 def cpu_baseline(problem, b, sweeps, gpu_stats):
@@ -482,6 +570,9 @@ def main() -> int:
     ap.add_argument("--trace-summary", metavar="DB", help="summarize a rocprofv3 kernel-trace database and exit")
     ap.add_argument("--trace-line", metavar="JSON", help="the bench line of the traced run (AMG level sizes)")
     ap.add_argument("--trace-top", type=int, default=40)
+    ap.add_argument("--large-size", type=int, default=110,
+                    help="cells per side of the box on which the plain path and the K-cycle variant are also "
+                         "timed (110: 7,986,000 cells; 0 skips it)")
     ap.add_argument("--step-nz", type=int, default=80,
                     help="spanwise cells of the SST step in the default line (0 skips it)")
     ap.add_argument("--simplec-step", type=int, default=0, metavar="NZ",
@@ -531,7 +622,7 @@ def main() -> int:
     import numpy as np
 
     import kutta_amd
-    from kutta_amd import (AmgOptions, Kernel, OperatorStructure, PressureProblem, SolveControl,
+    from kutta_amd import (AmgCycle, AmgOptions, Kernel, OperatorStructure, PressureProblem, SolveControl,
                            SolverKind)
 
     dev = kutta_amd.Device(0)
@@ -546,7 +637,10 @@ def main() -> int:
     setup_amg_s = time.perf_counter() - t
     tail_level = solver.set_tail(args.tail)
     solver.set_graphs(not args.no_graphs)
+    t = time.perf_counter()
     solver.refresh(problem)
+    dev.synchronize()
+    plain_refresh_ms = (time.perf_counter() - t) * 1e3
     if args.tail < 0:  # automatic: the autotuner times each admissible tail level (identical iterates)
         t = time.perf_counter()
         tail_level = solver.autotune_tail(problem, 5)
@@ -576,19 +670,32 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     ms = elapsed * 1e3 / args.steps
 
-    # The fine-level smoothing sweep timed on its own: `repeats` back-to-back
-    # amg_smooth launches over the fine operator (the l1-Jacobi inverse from
-    # krylov_diagonal, x the solution, b the right-hand side), bracketed by one
-    # pair of HIP events on the device stream (kh_time_launches), so the
-    # average is the kernel's execution time plus the gap between launches.
+    # Fine-level smoothing sweeps are timed on the device stream by HIP events
+    # attached to their own dispatch packets (hipExtLaunchKernelGGL), so the
+    # average is the kernel's execution time, as a kernel trace reports it.
+    # Graph-replayed launches carry no events, so these solves launch kernel
+    # by kernel (the same kernels and iterates).
+    solver.set_graphs(False)
+    dev.profile_reset()
+    dev.profile(Kernel.amg_smooth, True)
+    dev.profile_filter(Kernel.amg_smooth, n)
+    for _ in range(max(args.steps // 2, 2)):
+        x.fill(np.float32(0))
+        solver.solve(problem, x, b, control)
+    dev.synchronize()
+    launches, kernel_ms = dev.profile_read(Kernel.amg_smooth)
+    dev.profile(Kernel.amg_smooth, False)
+    solver.set_graphs(not args.no_graphs)
+    # The same sweep as a batch of back-to-back launches with one pair of HIP
+    # events around the batch (kh_time_launches): execution plus the gap
+    # between launches, the rate a sequence of sweeps sustains.
     repeats = 50
     inverse, invalid, swept = dev.alloc(n), dev.alloc(n), dev.alloc(n)
     guard = dev.upload(np.zeros(3, np.int32))
     dev.launch(Kernel.krylov_diagonal, n, problem.row_ptr, problem.diag, problem.val, inverse, invalid, 1)
     smooth_args = (problem.row_ptr, problem.col, problem.val, x, b, inverse, swept, guard)
     dev.launch(Kernel.amg_smooth, n, *smooth_args)
-    kernel_ms = dev.time_launches(Kernel.amg_smooth, n, *smooth_args, repeats=repeats)
-    launches = repeats
+    batch_us = dev.time_launches(Kernel.amg_smooth, n, *smooth_args, repeats=repeats) * 1e3 / repeats
     st = stats[-1]
     if not st.converged:
         print(f"[bench] warning: solve did not converge ({st})", file=sys.stderr)
@@ -610,14 +717,19 @@ def main() -> int:
         "bytes_per_launch": bytes_per_launch,
         "avg_launch_us": round(avg_us, 3),
         "launches": launches,
+        "batch_avg_us": round(batch_us, 3),
+        "batch_frac": round(bytes_per_launch / (batch_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
     }
 
     # The same solve with the additive solver variants (no reference
     # counterpart): the mixed-precision V-cycle (first coarse level of at most
     # --direct-rows rows solved by an FP64 dense inverse; with pcg_mixed_bf16
-    # also the coarse levels' values in bfloat16) under PCG, and flexible
-    # GMRES(30) with the plain and the mixed V-cycle. x is compared with the
-    # plain FP32 solution, both within the 1e-6 relative residual.
+    # also the coarse levels' values in bfloat16) under PCG, flexible
+    # GMRES(30) with the plain and the mixed V-cycle, and the K-cycle variant
+    # (KCYCLE) alone and with the dense level. x is compared with the plain
+    # FP32 solution and with the FP64 solution, all within the 1e-6 relative
+    # residual. Each variant's refresh (the operator update every SIMPLEC
+    # outer iteration pays) is timed beside its solve.
     variants = {}
     x_plain = x.download()
 
@@ -634,40 +746,26 @@ def main() -> int:
     wanted = (None if args.variants == "all" else
               set() if args.variants == "none" else set(args.variants.split(",")))
     dr = args.direct_rows
-    for label, kind, direct, bf16 in (("pcg_mixed", SolverKind.AmgPcg, dr, False),
-                                      ("pcg_mixed_bf16", SolverKind.AmgPcg, dr, True),
-                                      ("fgmres_plain", SolverKind.AmgFgmres, 0, False),
-                                      ("fgmres_mixed", SolverKind.AmgFgmres, dr, False)):
+    k_options = AmgOptions(cycle=AmgCycle.K, aggregation_passes=2, structure=OperatorStructure.SymmetricMMatrix,
+                           **KCYCLE)
+    for label, kind, opts, direct, bf16 in (("pcg_mixed", SolverKind.AmgPcg, options, dr, False),
+                                            ("pcg_mixed_bf16", SolverKind.AmgPcg, options, dr, True),
+                                            ("fgmres_plain", SolverKind.AmgFgmres, options, 0, False),
+                                            ("fgmres_mixed", SolverKind.AmgFgmres, options, dr, False),
+                                            ("pcg_kcycle", SolverKind.AmgPcg, k_options, 0, False),
+                                            ("pcg_kcycle_mixed", SolverKind.AmgPcg, k_options, dr, False)):
         if wanted is not None and label not in wanted:
             continue
-        other = kutta_amd.make_amg_solver(kind, dev, problem.host, options)
-        level = other.set_direct(direct) if direct else 0
-        other.set_coarse_bf16(bf16)
-        other.set_graphs(not args.no_graphs)
-        t = time.perf_counter()
-        other.refresh(problem)
-        dev.synchronize()
-        refresh_ms = (time.perf_counter() - t) * 1e3
-        for _ in range(max(args.warmup, 1)):
-            x.fill(np.float32(0))
-            other.solve(problem, x, b, control)
-        dev.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            x.fill(np.float32(0))
-            vst = other.solve(problem, x, b, control)
-        dev.synchronize()
-        vms = (time.perf_counter() - t0) * 1e3 / args.steps
+        _, vst, out = time_solver(dev, problem, opts, kind, b, x, control, args.steps, args.warmup,
+                                  not args.no_graphs, direct, bf16, tail=-1 if opts is k_options else args.tail)
         xv = x.download()
-        variants[label] = {"ms": round(vms, 4), "iterations": vst.iterations,
-                           "relative_residual": vst.final_residual / vst.initial_residual,
-                           "converged": vst.converged, "speedup_vs_plain_pcg": round(ms / vms, 3),
-                           "refresh_ms": round(refresh_ms, 2),
-                           "direct_level": level,
-                           "fp64_error": fp64_error(xv, x64),
-                           "max_abs_diff_vs_plain_over_max_abs_x":
-                               float(np.abs(xv.astype(np.float64) - x_plain).max() / np.abs(x_plain).max())}
-        print(f"[bench] {label}: {vms:.3f} ms, {vst.iterations} iterations", file=sys.stderr, flush=True)
+        out["speedup_vs_plain_pcg"] = round(ms / out["ms"], 3)
+        out["speedup_refresh_plus_solve"] = round((ms + plain_refresh_ms) / out["refresh_plus_solve_ms"], 3)
+        out["fp64_error"] = fp64_error(xv, x64)
+        out["max_abs_diff_vs_plain_over_max_abs_x"] = float(np.abs(xv.astype(np.float64) - x_plain).max() /
+                                                            np.abs(x_plain).max())
+        variants[label] = out
+        print(f"[bench] {label}: {out['ms']:.3f} ms, {vst.iterations} iterations", file=sys.stderr, flush=True)
 
     # The learned coarse correction inside FGMRES(30), over the plain and the
     # mixed V-cycle: the solver harvests restricted residuals from plain
@@ -750,6 +848,10 @@ def main() -> int:
         simplec_sst = simplec_step_bench(dev, args.step_nz, 3, 10, 2, 0 if args.no_cpu_baseline else 2,
                                          not args.no_graphs)
 
+    large = None
+    if args.large_size:
+        large = large_mesh_comparison(dev, args.large_size, max(min(args.steps, 5), 1), 1, not args.no_graphs, args)
+
     baseline = None
     x_gpu = x_plain
     if not args.no_cpu_baseline:
@@ -784,6 +886,7 @@ def main() -> int:
             "iterations": st.iterations,
             "relative_residual": st.final_residual / st.initial_residual,
             "converged": st.converged,
+            "refresh_ms": round(plain_refresh_ms, 2),
             "setup_s": {"mesh_and_assembly": round(setup_mesh_s, 3), "amg": round(setup_amg_s, 3),
                         **{k: round(v, 3) for k, v in problem.setup_seconds.items()}},
             "device": dev.name,
@@ -792,6 +895,7 @@ def main() -> int:
         "cpu_baseline": baseline,
         "fp64_reference": fp64,
         "solver_variants": variants,
+        "large_mesh": large,
         "simplec": simplec,
         "simplec_sst": simplec_sst,
     }
