@@ -344,9 +344,14 @@ typedef enum kh_kernel {
   KH_AMG_INNER_CHECK,            /* :173-176 scalars,status */
   KH_AMG_INNER_BETA,             /* :177-181 scalars,status */
   KH_AMG_INNER_STEP,             /* :182-186 x,p,scalars,status */
+  /* KH_AMG_SMOOTH for levels with rows longer than KH_WIDE_ROW entries: the
+     row's entries loaded in chunks of up to 64 instead of 16 (no reference
+     counterpart; same arguments, same bits). */
+  KH_AMG_SMOOTH_WIDE,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 #define KH_COARSE_SWEEPS_MAX_ROWS 16384
+#define KH_WIDE_ROW 16
 
 /* One system of the batched BiCGStab-Jacobi launches: its operator, its
  * vectors (x, r, q, p, z, s, t, zs, shadow; n floats each), its Jacobi

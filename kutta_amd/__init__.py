@@ -108,7 +108,7 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_solve.argtypes = [vp, vp, vp, vp, dbl, dbl, i32, i32, C.POINTER(dbl)]
         lib.ka_solver_describe.argtypes = [vp, C.c_char_p, i32]
         lib.ka_solver_plan.argtypes = [vp, C.c_char_p, i32]
-        lib.ka_solver_set_plan.argtypes = [vp, i32, i32]
+        lib.ka_solver_set_plan.argtypes = [vp, i32, i32, i32]
         lib.ka_simplec_pressure_levels.argtypes = [vp, vp, vp, i32]
         lib.ka_simplec_pressure_plan.argtypes = [vp, C.c_char_p, i32]
         lib.ka_solver_solve_batched.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, i32, i32, C.POINTER(dbl)]
@@ -386,6 +386,7 @@ class Kernel(enum.IntEnum):
     amg_inner_check = 247
     amg_inner_beta = 248
     amg_inner_step = 249
+    amg_smooth_wide = 250
 
 
 class Buffer:
@@ -804,9 +805,10 @@ class LinearSolver:
         host_lib().ka_solver_describe(self.handle, buf, 512)
         return buf.value.decode()
 
-    def set_plan(self, restrict_mode: int = -1, coarse_fused_sweeps: int = -1) -> None:
+    def set_plan(self, restrict_mode: int = -1, coarse_fused_sweeps: int = -1, prolong_mode: int = -1) -> None:
         """Forces the V-cycle forms (see plan()); -1 keeps a setting."""
-        _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_fused_sweeps)))
+        _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_fused_sweeps),
+                                                  int(prolong_mode)))
 
     def plan(self) -> str:
         """The V-cycle forms the autotuner picked (same iterates for all)."""

@@ -41,6 +41,7 @@ std::array<float, 2> pair_of(double value) {
 
 struct Level {
   Index rows = 0, nnz = 0;
+  Index max_row = 0;  // longest row (entries)
   OwnedCsr matrix;  // coarse levels only; level 0 uses the caller's operator
   CsrHost host;     // coarse levels: the operator as built (pattern and setup values)
   DeviceArray<std::uint16_t> val16;  // bfloat16 copy of the values (mixed path)
@@ -53,11 +54,22 @@ struct Level {
   // 1 the same with 128; 2 a row-parallel residual then the member sums,
   // which reads the fine rows coalesced when aggregates are scattered.
   int restrict_mode = 0;
+  // How the fused path prolongs into this level (bitwise equal, picked by the
+  // autotuner): 0 the prolongation folded into the first post-smoothing
+  // sweep's gathers; 1 the in-place prolongation, then a plain sweep, which
+  // keeps the sweep's gathers to one dependent load per entry.
+  int prolong_mode = 0;
   // K-cycle workspaces of a coarse level (linalg/amg.cpp:185-193): the inner
   // corrections' vectors, 12 pair scalars and 1-int status.
   DeviceArray<float> kp, kq, kz, kt, kscalars;
   DeviceArray<Index> kstatus;
 };
+
+Index longest_row(const CsrHost& a) {
+  Index longest = 0;
+  for (Index i = 0; i < a.n_rows; ++i) longest = std::max(longest, a.row_ptr[i + 1] - a.row_ptr[i]);
+  return longest;
+}
 
 // What the caller wants fused into the last sweep of the fine level.
 enum class Tail { None, DotBeta };
@@ -386,6 +398,7 @@ struct KrylovSolver::Impl {
     levels.emplace_back();
     levels.back().rows = a.n_rows;
     levels.back().nnz = a.nnz();
+    levels.back().max_row = longest_row(a);
     levels.back().l1 = to_device(dev, diagonal_values(a, true, nullspace));
     levels.back().r = DeviceArray<float>(dev, a.n_rows);
     if (nullspace) levels.back().projection = DeviceArray<float>(dev, 2);
@@ -408,6 +421,7 @@ struct KrylovSolver::Impl {
       Level coarse;
       coarse.rows = c.matrix.n_rows;
       coarse.nnz = c.matrix.nnz();
+      coarse.max_row = longest_row(c.matrix);
       coarse.l1 = to_device(dev, diagonal_values(c.matrix, true, nullspace));
       coarse.x = DeviceArray<float>(dev, c.matrix.n_rows);
       coarse.b = DeviceArray<float>(dev, c.matrix.n_rows);
@@ -566,8 +580,13 @@ struct KrylovSolver::Impl {
         dev.launch(KH_AMG_SMOOTH_ZERO, a.n_rows, {a.row_ptr, a.col, a.val, zero_x, b, l.l1, out, st});
       }
     } else {
-      dev.launch(KH_AMG_SMOOTH, a.n_rows, {a.row_ptr, a.col, a.val, in, b, l.l1, out, st});
+      dev.launch(smooth_kernel(l), a.n_rows, {a.row_ptr, a.col, a.val, in, b, l.l1, out, st});
     }
+  }
+  // The plain sweep's kernel: the wide-chunk instantiation on the fused path
+  // for levels with rows longer than KH_WIDE_ROW (same bits).
+  kh_kernel smooth_kernel(const Level& l) const {
+    return fused && l.max_row > KH_WIDE_ROW ? KH_AMG_SMOOTH_WIDE : KH_AMG_SMOOTH;
   }
 
   // What a V-cycle completed beyond z = M^-1 b.
@@ -679,13 +698,13 @@ struct KrylovSolver::Impl {
           if (bf16_level(l)) {
             dev.launch(KH_BF16_PROLONG_SMOOTH, a.n_rows,
                        {a.row_ptr, a.col, l.val16, current, b, l.l1, out, l.aggregate, c.x, st});
-          } else if (fused) {
+          } else if (fused && l.prolong_mode == 0) {
             dev.launch(KH_FUSED_PROLONG_SMOOTH, a.n_rows,
                        {a.row_ptr, a.col, a.val, current, b, l.l1, out, l.aggregate, c.x, st});
           } else {
             float* xin = const_cast<float*>(current);
             dev.launch(KH_AMG_PROLONG_GUARDED, a.n_rows, {l.aggregate, c.x, xin, st});
-            dev.launch(KH_AMG_SMOOTH, a.n_rows, {a.row_ptr, a.col, a.val, xin, b, l.l1, out, st});
+            dev.launch(smooth_kernel(l), a.n_rows, {a.row_ptr, a.col, a.val, xin, b, l.l1, out, st});
           }
           current = out;
         } else {
@@ -1298,6 +1317,21 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
       }
     }
   }
+  // Prolongation form per level: time whole V-cycles with each level's two
+  // forms in turn (the in-place form changes the level's iterate, so it is
+  // not timed in isolation) and keep the faster. Every form gives the same bits.
+  for (std::size_t i = 0; i + 1 < I.levels.size(); ++i) {
+    Level& l = I.levels[i];
+    if (I.bf16_level(l)) continue;
+    double fused_form = 0;
+    for (int mode = 0; mode < 2; ++mode) {
+      l.prolong_mode = mode;
+      I.reset(0);
+      const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
+      if (mode == 0) fused_form = t;
+      else l.prolong_mode = t < fused_form ? 1 : 0;
+    }
+  }
   // The coarsest level's sweeps: launch by launch or in one workgroup.
   if (I.coarse_sweeps_fit(I.levels.size() - 1)) {
     double by_launch = 0;
@@ -1403,12 +1437,15 @@ std::string KrylovSolver::describe() const {
         << impl_->levels[impl_->learned_level].rows << " rows)";
   return out.str();
 }
-void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps) {
-  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_fused_sweeps >= -1 && coarse_fused_sweeps <= 1,
-             "solver plan: restriction form -1..2, coarsest sweeps -1..1");
+void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps, int prolong_mode) {
+  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_fused_sweeps >= -1 && coarse_fused_sweeps <= 1 &&
+                 prolong_mode >= -1 && prolong_mode <= 1,
+             "solver plan: restriction form -1..2, coarsest sweeps -1..1, prolongation form -1..1");
   impl_->drop_graphs();
   if (restrict_mode >= 0)
     for (auto& l : impl_->levels) l.restrict_mode = restrict_mode;
+  if (prolong_mode >= 0)
+    for (auto& l : impl_->levels) l.prolong_mode = prolong_mode;
   if (coarse_fused_sweeps >= 0) impl_->coarse_fused_sweeps = coarse_fused_sweeps != 0;
 }
 std::string KrylovSolver::plan() const {
@@ -1417,6 +1454,8 @@ std::string KrylovSolver::plan() const {
   out << "restriction per level";
   for (std::size_t i = 0; i + 1 < L.size(); ++i)
     out << (i ? "," : " ") << (L[i].restrict_mode == 2 ? "split" : (L[i].restrict_mode == 1 ? "wide" : "fused"));
+  out << "; prolongation per level";
+  for (std::size_t i = 0; i + 1 < L.size(); ++i) out << (i ? "," : " ") << (L[i].prolong_mode ? "split" : "fused");
   if (!L.empty() && impl_->coarse_sweeps_fit(L.size() - 1))
     out << "; coarsest level " << (impl_->coarse_fused_sweeps ? "one-workgroup sweeps" : "sweep launches");
   out << "; tail from level " << impl_->tail_first;

@@ -330,6 +330,18 @@ __device__ __forceinline__ float row_residual(float b, int begin, int end,
     return row_residual_chunked<2 * kChunk>(b, begin, end, cols, values, gather);
   return row_residual_chunked<kChunk>(b, begin, end, cols, values, gather);
 }
+// The same with a third width, four chunks, for levels whose rows run past two
+// chunks (the last levels where pairwise aggregation stalls have rows of up
+// to 70 entries): a separate instantiation, so the registers the wide chunk
+// needs do not lower the occupancy of the kernels that never take it.
+template <class X, class V>
+__device__ __forceinline__ float row_residual_wide(float b, int begin, int end,
+                                                   const int* __restrict__ cols,
+                                                   const V* __restrict__ values, X gather) {
+  if (__any(end - begin > 2 * kChunk))
+    return row_residual_chunked<4 * kChunk>(b, begin, end, cols, values, gather);
+  return row_residual(b, begin, end, cols, values, gather);
+}
 
 // Maps workgroups so that each of the 8 XCDs (which receive workgroups
 // round-robin) processes one contiguous range of the index space, keeping

@@ -143,8 +143,9 @@ __global__ void __launch_bounds__(64) k_amg_coarse_solve(const int* __restrict__
   }
   if (row < count) x[row] = cur[row];
 }
-// kernels/linalg.kern.h:150-156
-template <class V>
+// kernels/linalg.kern.h:150-156. Wide: the instantiation for long rows
+// (row_residual_wide), same arithmetic.
+template <class V, bool Wide = false>
 __global__ void __launch_bounds__(kBlock) k_amg_smooth(int n, const int* __restrict__ rows,
     const int* __restrict__ cols, const V* __restrict__ values, const float* __restrict__ x,
     const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
@@ -152,7 +153,8 @@ __global__ void __launch_bounds__(kBlock) k_amg_smooth(int n, const int* __restr
   if (status[0] != 0) return;
   const int i = row_index();
   if (i >= n) return;
-  const float r = residual_row(i, rows, cols, values, x, b);
+  const float r = Wide ? row_residual_wide(b[i], rows[i], rows[i + 1], cols, values, Plain{x})
+                       : residual_row(i, rows, cols, values, x, b);
   next[i] = x[i] + (2.0f / 3.0f) * inverse[i] * r;
 }
 // kernels/linalg.kern.h:162-170
@@ -1240,6 +1242,9 @@ void l_scale_copy(kh_device* d, int n, const kh_arg* a) {
 }
 void l_mul(kh_device* d, int n, const kh_arg* a) { k_mul<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2)); }
 #define SINGLE dim3(1), dim3(64), 0, d->stream
+void l_amg_smooth_wide(kh_device* d, int n, const kh_arg* a) {
+  k_amg_smooth<float, true><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
+}
 void l_solver_limits(kh_device* d, int n, const kh_arg* a) {
   k_solver_limits<<<GRID(n)>>>(n, KF(0), f32(a, 1), f32(a, 2), f32(a, 3), f32(a, 4));
 }
@@ -1557,6 +1562,7 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_AMG_INNER_CHECK, "amg_inner_check", "bb", l_amg_inner_check},
       {KH_AMG_INNER_BETA, "amg_inner_beta", "bb", l_amg_inner_beta},
       {KH_AMG_INNER_STEP, "amg_inner_step", "bbbb", l_amg_inner_step},
+      {KH_AMG_SMOOTH_WIDE, "amg_smooth_wide", "bbbbbbbb", l_amg_smooth_wide},
   };
   return table;
 }
