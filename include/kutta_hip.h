@@ -348,6 +348,23 @@ typedef enum kh_kernel {
      row's entries loaded in chunks of up to 64 instead of 16 (no reference
      counterpart; same arguments, same bits). */
   KH_AMG_SMOOTH_WIDE,
+  /* The K-cycle's inner correction in four fused steps (no reference
+     counterpart; amg.cpp:320-336 with each dot product in its producer and
+     each scalar kernel in the reduction's finisher; n = the level's rows;
+     spd 1 for A-conjugate steps, 0 for GCR):
+     FIRST_STEP   p = z (unguarded), q = A z, (d.q, b.d) -> scalars 0, 2,
+                  amg_inner_alpha(first): rows,cols,values,z,p,q,b,scalars,
+                  status,parent_status,spd
+     FIRST_UPDATE amg_inner_first, (t.t, b.b) -> scalars 4, 6,
+                  amg_inner_check: b,p,q,x,t,scalars,status
+     BETA_STEP    w = z or A z, w.q -> scalars 4, amg_inner_beta:
+                  rows,cols,values,z,w,q,scalars,status,spd
+     SECOND_STEP  q = A p, (d.q, t.d) -> scalars 0, 2, amg_inner_alpha:
+                  rows,cols,values,p,q,t,scalars,status,parent_status,spd */
+  KH_INNER_FIRST_STEP,
+  KH_INNER_FIRST_UPDATE,
+  KH_INNER_BETA_STEP,
+  KH_INNER_SECOND_STEP,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 #define KH_COARSE_SWEEPS_MAX_ROWS 16384

@@ -387,6 +387,10 @@ class Kernel(enum.IntEnum):
     amg_inner_beta = 248
     amg_inner_step = 249
     amg_smooth_wide = 250
+    inner_first_step = 251
+    inner_first_update = 252
+    inner_beta_step = 253
+    inner_second_step = 254
 
 
 class Buffer:

@@ -785,6 +785,20 @@ struct KrylovSolver::Impl {
     auto spmv = [&](const float* in, float* out, const int* guard) {
       dev.launch(KH_CSR_SPMV_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, in, out, guard});
     };
+    if (fused) {
+      // The same steps, each dot product in the kernel producing its operand
+      // and each scalar update in the reduction's finisher.
+      const int sp = spd ? 1 : 0;
+      cycle(index, a, b, z, Tail::None, parent);
+      dev.launch(KH_INNER_FIRST_STEP, a.n_rows, {a.row_ptr, a.col, a.val, z, p, q, b, s, st, parent, sp});
+      dev.launch(KH_INNER_FIRST_UPDATE, a.n_rows, {b, p, q, x, t, s, st});
+      cycle(index, a, t, z, Tail::None, st);
+      dev.launch(KH_INNER_BETA_STEP, a.n_rows, {a.row_ptr, a.col, a.val, z, w, q, s, st, sp});
+      dev.launch(KH_SOLVER_DIRECTION, a.n_rows, {z, p, s, st});
+      dev.launch(KH_INNER_SECOND_STEP, a.n_rows, {a.row_ptr, a.col, a.val, p, q, t, s, st, parent, sp});
+      dev.launch(KH_AMG_INNER_STEP, a.n_rows, {x, p, s, st});
+      return;
+    }
     cycle(index, a, b, z, Tail::None, parent);
     dev.copy(p, z, sizeof(float) * m);
     spmv(p, q, parent);

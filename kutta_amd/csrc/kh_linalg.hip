@@ -345,14 +345,26 @@ __global__ void k_solver_direction(int n, const float* __restrict__ z, float* __
 // 6 b.b, 8 alpha, 10 beta. `first` makes the level's status follow the
 // parent's; a failed first step is phase 2 (keep the first correction
 // unscaled), a failed second one phase 1 (keep the first step).
-__global__ void k_amg_inner_alpha(float* s, int* status, const int* parent, int first) {
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ void inner_alpha_body(float* s, int* status, const int* parent, int first) {
   if (first != 0) status[0] = parent[0] == 0 ? 0 : 2;
   if (status[0] != 0) return;
   const Pair denom = pload(s, 0), alpha = pdiv(pload(s, 2), denom);
   if (!ppositive(denom) || !pcoefficient(alpha) || (pzero(alpha) && !pzero(pload(s, 2))))
     status[0] = first != 0 ? 2 : 1;
   else pstore(s, 8, alpha);
+}
+__device__ __forceinline__ void inner_check_body(const float* s, int* status) {
+  if (status[0] != 0) return;
+  if (ple(pload(s, 4), pmul({0.0625f, 0.0f}, pload(s, 6)))) status[0] = 1;
+}
+__device__ __forceinline__ void inner_beta_body(float* s, int* status) {
+  if (status[0] != 0) return;
+  const Pair beta = pdiv(pneg(pload(s, 4)), pload(s, 0));
+  if (!pcoefficient(beta) || (pzero(beta) && !pzero(pload(s, 4)))) status[0] = 1;
+  else pstore(s, 10, beta);
+}
+__global__ void k_amg_inner_alpha(float* s, int* status, const int* parent, int first) {
+  if (threadIdx.x == 0) inner_alpha_body(s, status, parent, first);
 }
 // Unguarded: x = alpha p and t = b - alpha q while running, else x = p, t = 0.
 __global__ void k_amg_inner_first(int n, const float* __restrict__ b, const float* __restrict__ p,
@@ -367,14 +379,10 @@ __global__ void k_amg_inner_first(int n, const float* __restrict__ b, const floa
 }
 // Adaptive exit: the second correction is skipped once t.t <= b.b / 16.
 __global__ void k_amg_inner_check(const float* s, int* status) {
-  if (threadIdx.x != 0 || status[0] != 0) return;
-  if (ple(pload(s, 4), pmul({0.0625f, 0.0f}, pload(s, 6)))) status[0] = 1;
+  if (threadIdx.x == 0) inner_check_body(s, status);
 }
 __global__ void k_amg_inner_beta(float* s, int* status) {
-  if (threadIdx.x != 0 || status[0] != 0) return;
-  const Pair beta = pdiv(pneg(pload(s, 4)), pload(s, 0));
-  if (!pcoefficient(beta) || (pzero(beta) && !pzero(pload(s, 4)))) status[0] = 1;
-  else pstore(s, 10, beta);
+  if (threadIdx.x == 0) inner_beta_body(s, status);
 }
 __global__ void k_amg_inner_step(int n, float* __restrict__ x, const float* __restrict__ p,
     const float* __restrict__ s, const int* __restrict__ status) {
@@ -703,6 +711,115 @@ __global__ void __launch_bounds__(kBlock) k_batch_bicg_step(int n, const kh_bicg
                                                             int mask, double* partials, unsigned stride) {
   LANE_OR_RETURN
   fused_bicg_step_body(n, L.x, L.r, L.z, L.zs, L.s, L.t, L.shadow, L.scalars, L.status, part);
+}
+
+// The K-cycle's inner corrections in four fused steps (no reference
+// counterpart: linalg/amg.cpp:320-336 with each dot product folded into the
+// kernel that produces its operand and each scalar kernel into the
+// reduction's finisher; the scalars and status go through the reference's
+// transitions, and the reductions are the same FP64 partial sums as the
+// fused PCG kernels').
+// q = A src and the partials of (d.q, rhs.d), d = src (SPD) or q (GCR),
+// guarded by `guard`; with `copy` set, copy = src first, unguarded (the
+// reference's dev.copy(p, z)).
+__global__ void __launch_bounds__(kBlock) k_inner_spmv_dot2(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ src,
+    float* __restrict__ copy, float* __restrict__ q, const float* __restrict__ rhs,
+    const int* __restrict__ guard, int spd, double* __restrict__ partials) {
+  const int i = row_index();
+  if (copy != nullptr && i < n) copy[i] = src[i];
+  if (guard[0] != 0) return;
+  double mine[2] = {0.0, 0.0};
+  if (i < n) {
+    const float qi = spmv_row(i, rows, cols, values, src);
+    q[i] = qi;
+    const float d = spd != 0 ? src[i] : qi;
+    mine[0] = double(d) * double(qi);
+    mine[1] = double(rhs[i]) * double(d);
+  }
+  __shared__ double lds[2][kBlock / 64];
+  mine[0] = block_sum(mine[0], lds[0]);
+  mine[1] = block_sum(mine[1], lds[1]);
+  store_partials<2>(mine, partials);
+}
+// The first correction (amg_inner_first, unguarded) and the partials of
+// (t.t, b.b).
+__global__ void __launch_bounds__(kBlock) k_inner_first_dot2(int n, const float* __restrict__ b,
+    const float* __restrict__ p, const float* __restrict__ q, float* __restrict__ x, float* __restrict__ t,
+    const float* __restrict__ s, const int* __restrict__ status, double* __restrict__ partials) {
+  const int i = row_index();
+  double mine[2] = {0.0, 0.0};
+  if (i < n) {
+    const bool running = status[0] == 0;
+    const float alpha = pfloat(pload(s, 8));
+    const float ti = running ? b[i] - alpha * q[i] : 0.0f;
+    x[i] = running ? alpha * p[i] : p[i];
+    t[i] = ti;
+    mine[0] = double(ti) * double(ti);
+    mine[1] = double(b[i]) * double(b[i]);
+  }
+  __shared__ double lds[2][kBlock / 64];
+  mine[0] = block_sum(mine[0], lds[0]);
+  mine[1] = block_sum(mine[1], lds[1]);
+  store_partials<2>(mine, partials);
+}
+// The partials of w.q, w = z (SPD) or w = A z stored in `w` (GCR), guarded.
+__global__ void __launch_bounds__(kBlock) k_inner_beta_dot(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ z,
+    float* __restrict__ w, const float* __restrict__ q, const int* __restrict__ status, int spd,
+    double* __restrict__ partials) {
+  if (status[0] != 0) return;
+  const int i = row_index();
+  double mine[1] = {0.0};
+  if (i < n) {
+    float v = z[i];
+    if (spd == 0) {
+      v = spmv_row(i, rows, cols, values, z);
+      w[i] = v;
+    }
+    mine[0] = double(v) * double(q[i]);
+  }
+  __shared__ double lds[kBlock / 64];
+  mine[0] = block_sum(mine[0], lds);
+  store_partials<1>(mine, partials);
+}
+enum class FinishInner { First, Check, Beta, Second };
+template <FinishInner F>
+__global__ void __launch_bounds__(kBlock) k_inner_finish(const double* __restrict__ partials, unsigned count,
+                                                         float* s, int* status, const int* parent) {
+  if constexpr (F == FinishInner::First) {
+    if (parent[0] == 0) {
+      double total[2];
+      sum_partials<2>(partials, count, total);
+      if (threadIdx.x == 0) {
+        pstore(s, 0, to_pair(total[0]));
+        pstore(s, 2, to_pair(total[1]));
+      }
+    }
+    if (threadIdx.x == 0) inner_alpha_body(s, status, parent, 1);
+  } else if constexpr (F == FinishInner::Check) {
+    double total[2];
+    sum_partials<2>(partials, count, total);
+    if (threadIdx.x != 0) return;
+    pstore(s, 4, to_pair(total[0]));
+    pstore(s, 6, to_pair(total[1]));
+    inner_check_body(s, status);
+  } else if constexpr (F == FinishInner::Beta) {
+    if (status[0] != 0) return;
+    double total[1];
+    sum_partials<1>(partials, count, total);
+    if (threadIdx.x != 0) return;
+    pstore(s, 4, to_pair(total[0]));
+    inner_beta_body(s, status);
+  } else {
+    if (status[0] != 0) return;
+    double total[2];
+    sum_partials<2>(partials, count, total);
+    if (threadIdx.x != 0) return;
+    pstore(s, 0, to_pair(total[0]));
+    pstore(s, 2, to_pair(total[1]));
+    inner_alpha_body(s, status, parent, 0);
+  }
 }
 
 // Finishers: the FP64 total of `count` partials as a normalized pair, then the
@@ -1319,6 +1436,32 @@ void l_fused_spmv_dot_alpha(kh_device* d, int n, const kh_arg* a) {
   k_fused_spmv_dot<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), st, d->partials);
   k_finish<Finish::Alpha><<<FINISH>>>(d->partials, g, KF(5), st, nullptr);
 }
+void l_inner_first_step(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 8));
+  k_inner_spmv_dot2<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KF(5), KCF(6), KB(9), i32(a, 10),
+                                 d->partials);
+  k_inner_finish<FinishInner::First><<<FINISH>>>(d->partials, g, KF(7), st, KB(9));
+}
+void l_inner_first_update(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 6));
+  k_inner_first_dot2<<<GRID(n)>>>(n, KCF(0), KCF(1), KCF(2), KF(3), KF(4), KCF(5), st, d->partials);
+  k_inner_finish<FinishInner::Check><<<FINISH>>>(d->partials, g, KF(5), st, nullptr);
+}
+void l_inner_beta_step(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 7));
+  k_inner_beta_dot<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KF(4), KCF(5), st, i32(a, 8), d->partials);
+  k_inner_finish<FinishInner::Beta><<<FINISH>>>(d->partials, g, KF(6), st, nullptr);
+}
+void l_inner_second_step(kh_device* d, int n, const kh_arg* a) {
+  const unsigned g = reduce_grid(n);
+  int* st = static_cast<int*>(buf(a, 7));
+  k_inner_spmv_dot2<<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), nullptr, KF(4), KCF(5), st, i32(a, 9),
+                                 d->partials);
+  k_inner_finish<FinishInner::Second><<<FINISH>>>(d->partials, g, KF(6), st, KB(8));
+}
 void l_fused_step_dot_check(kh_device* d, int n, const kh_arg* a) {
   const unsigned g = reduce_grid(n);
   int* st = static_cast<int*>(buf(a, 5));
@@ -1563,6 +1706,10 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_AMG_INNER_BETA, "amg_inner_beta", "bb", l_amg_inner_beta},
       {KH_AMG_INNER_STEP, "amg_inner_step", "bbbb", l_amg_inner_step},
       {KH_AMG_SMOOTH_WIDE, "amg_smooth_wide", "bbbbbbbb", l_amg_smooth_wide},
+      {KH_INNER_FIRST_STEP, "inner_first_step", "bbbbbbbbbbi", l_inner_first_step},
+      {KH_INNER_FIRST_UPDATE, "inner_first_update", "bbbbbbb", l_inner_first_update},
+      {KH_INNER_BETA_STEP, "inner_beta_step", "bbbbbbbbi", l_inner_beta_step},
+      {KH_INNER_SECOND_STEP, "inner_second_step", "bbbbbbbbbi", l_inner_second_step},
   };
   return table;
 }
