@@ -484,6 +484,14 @@ int kh_profile_enable(kh_device* dev, uint32_t kernel, int enable);
 int kh_profile_filter(kh_device* dev, uint32_t kernel, size_t n);
 int kh_profile_read(kh_device* dev, uint32_t kernel, long* launches, double* total_ms);
 int kh_profile_reset(kh_device* dev);
+/* The reference's logical-bytes model of device traffic (no reference
+ * counterpart as an entry point; bench/bench_fv.cpp:68-93 counts the same way):
+ * while enabled, each kernel launch adds the full size of the allocation behind
+ * every buffer argument, reduce_into adds 4n / 8n / 16n bytes (sum / dot /
+ * dot2) and a host reduction 8n (dot) or 4n. Returns the totals since the last
+ * call, resets them and sets counting on or off. Launches recorded into a
+ * graph are not counted. */
+int kh_logical_bytes(kh_device* dev, int enable, double* bytes, long* launches);
 /* Times `repeats` back-to-back launches of one kernel (no reference
  * counterpart): one HIP event before the first and one after the last on the
  * device stream, then a synchronization; *total_ms is the elapsed time of the

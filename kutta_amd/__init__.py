@@ -75,6 +75,7 @@ def hip_lib() -> C.CDLL:
         lib.kh_profile_reset.argtypes = [vp]
         lib.kh_time_launches.argtypes = [vp, C.c_uint32, sz, C.POINTER(KhArg), sz, C.c_int,
                                          C.POINTER(C.c_double)]
+        lib.kh_logical_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
         _hip = lib
     return _hip
 
@@ -546,6 +547,13 @@ class Device:
         lib = hip_lib()
         lib.kh_profile_filter.argtypes = [C.c_void_p, C.c_uint32, C.c_size_t]
         _check_hip(lib.kh_profile_filter(self.kh, int(kernel), int(n)))
+
+    def logical_bytes(self, enable: bool) -> tuple[float, int]:
+        """(bytes, counted launches) of the logical-bytes model since the last
+        call (kh_logical_bytes); resets them and turns counting on or off."""
+        b, n = C.c_double(), C.c_long()
+        _check_hip(hip_lib().kh_logical_bytes(self.kh, int(enable), C.byref(b), C.byref(n)))
+        return b.value, n.value
 
     def time_launches(self, kernel, n: int, *args, repeats: int = 20) -> float:
         """Milliseconds of `repeats` back-to-back launches, bracketed by one

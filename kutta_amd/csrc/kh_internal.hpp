@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,12 @@ struct kh_device {
   hipEvent_t packet_start = nullptr;
   hipEvent_t packet_stop = nullptr;
   std::vector<hipEvent_t> event_pool;
+  // Logical-bytes accounting (kh_logical_bytes): the size of every live
+  // allocation, and while counting, the running totals.
+  std::map<std::uintptr_t, std::size_t> allocations;
+  bool count_bytes = false;
+  double logical_bytes = 0;
+  long logical_launches = 0;
 };
 
 namespace kh {

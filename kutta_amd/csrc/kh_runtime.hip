@@ -296,7 +296,35 @@ int kh_alloc(kh_device* d, size_t bytes, void** out) {
     void* p = nullptr;
     // Zero-byte buffers still get a distinct handle (device.hpp:40-45).
     check(hipMalloc(&p, bytes == 0 ? 4 : bytes), "hipMalloc");
+    d->allocations[reinterpret_cast<std::uintptr_t>(p)] = bytes == 0 ? 4 : bytes;
     *out = p;
+  });
+}
+
+namespace {
+// The size of the allocation holding p (0 for memory not from kh_alloc).
+std::size_t allocation_bytes(const kh_device* d, const void* p) {
+  const auto key = reinterpret_cast<std::uintptr_t>(p);
+  auto it = d->allocations.upper_bound(key);
+  if (it == d->allocations.begin()) return 0;
+  --it;
+  return key < it->first + it->second ? it->second : 0;
+}
+void count_logical(kh_device* d, double bytes) {
+  if (!d->count_bytes || d->capturing) return;
+  d->logical_bytes += bytes;
+  ++d->logical_launches;
+}
+}  // namespace
+
+int kh_logical_bytes(kh_device* d, int enable, double* bytes, long* launches) {
+  return guard([&] {
+    require(d != nullptr, "kh_logical_bytes: null device");
+    if (bytes) *bytes = d->logical_bytes;
+    if (launches) *launches = d->logical_launches;
+    d->logical_bytes = 0;
+    d->logical_launches = 0;
+    d->count_bytes = enable != 0;
   });
 }
 
@@ -308,6 +336,7 @@ int kh_free(kh_device* d, void* ptr) {
     // Queued work may still read the buffer: finish it first (cuda_device.cu:21-23).
     check(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
     check(hipFree(ptr), "hipFree");
+    d->allocations.erase(reinterpret_cast<std::uintptr_t>(ptr));
   });
 }
 
@@ -381,6 +410,12 @@ int kh_launch(kh_device* d, uint32_t kernel, size_t n, const kh_arg* args, size_
                                  std::to_string(k) + " is a null buffer");
     }
     if (n == 0) return;
+    if (d->count_bytes && !d->capturing) {
+      double bytes = 0;
+      for (std::size_t k = 0; k < nargs; ++k)
+        if (args[k].kind == KH_ARG_BUFFER) bytes += double(allocation_bytes(d, args[k].value.ptr));
+      count_logical(d, bytes);
+    }
     auto& slot = d->profile[kernel < d->profile.size() ? kernel : 0];
     const bool timed = kernel < d->profile.size() && slot.enabled && !d->capturing &&
                        (slot.only_n == 0 || slot.only_n == n);
@@ -443,6 +478,7 @@ int kh_reduce_host(kh_device* d, const kh_reduction* req, size_t count, double* 
       for (size_t k = 0; k < chunk; ++k) {
         const auto& r = req[base + k];
         require(r.kind <= KH_REDUCE_MAX_ABS, "kh_reduce_host: bad kind");
+        count_logical(d, double(r.n) * (r.kind == KH_REDUCE_DOT ? 8 : 4));
         if (r.n == 0) {
           launch_fill32(d, d->results + k, 0u, 2);  // +0.0 as a double
         } else {
@@ -466,6 +502,7 @@ int kh_reduce_into(kh_device* d, int mode, const float* x, const float* y, const
     require(mode >= KH_INTO_SUM && mode <= KH_INTO_DOT2, "kh_reduce_into: bad mode");
     require(dst != x && dst != y && dst != u && dst != v, "kh_reduce_into: output aliases input");
     float* out = dst + offset;
+    count_logical(d, double(n) * (mode == KH_INTO_SUM ? 4 : mode == KH_INTO_DOT ? 8 : 16));
     if (n == 0) {
       launch_fill32(d, out, 0u, mode == KH_INTO_DOT2 ? 4 : 2);
     } else {
