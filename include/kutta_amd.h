@@ -151,8 +151,10 @@ int ka_solver_set_restart(void* solver, int m);
  * mean square unexplained, and dropped for the rest of a solve when the
  * residual estimate falls by less than `reduction` per step on average; 0.5
  * and 0.5 by default). targets: exact level corrections of the harvested
- * residuals by AMG-PCG to rel_tol. train: Adam steps over the pairs,
- * losses[steps] and the unexplained fractions relative[steps] (null skipped).
+ * residuals by AMG-PCG to rel_tol. train: Adam steps cycling over the first
+ * `pairs` pairs (0: all), losses[steps] and the unexplained fractions
+ * relative[steps] (null skipped). evaluate: the mean unexplained fraction on
+ * pairs [first, first + count) without training on them (held-out pairs).
  * report out[11]: level, rows, mode, harvested, targets, applications,
  * fallbacks, solves in mode 2, last loss, last epoch's unexplained fraction
  * (-1 before training), solves not applied by that gate.
@@ -160,7 +162,8 @@ int ka_solver_set_restart(void* solver, int m);
 int ka_solver_set_learned(void* solver, int level, int layers, uint64_t seed, int capacity);
 int ka_solver_learned_mode(void* solver, int mode, double reduction, double max_relative);
 int ka_solver_learned_targets(void* solver, double rel_tol, int32_t* pairs);
-int ka_solver_learned_train(void* solver, int steps, float lr, double* losses, double* relative);
+int ka_solver_learned_train(void* solver, int steps, float lr, int pairs, double* losses, double* relative);
+int ka_solver_learned_evaluate(void* solver, int first, int count, double* relative);
 int ka_solver_learned_report(void* solver, double* out);
 int ka_solver_learned_parameters(void* solver, float* theta, int64_t count, int set);
 /* Mixed-precision AMG (no reference counterpart): coarse levels (1 and

@@ -129,7 +129,8 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_set_learned.argtypes = [vp, i32, i32, C.c_uint64, i32]
         lib.ka_solver_learned_mode.argtypes = [vp, i32, dbl, dbl]
         lib.ka_solver_learned_targets.argtypes = [vp, dbl, pi32]
-        lib.ka_solver_learned_train.argtypes = [vp, i32, C.c_float, vp, vp]
+        lib.ka_solver_learned_train.argtypes = [vp, i32, C.c_float, i32, vp, vp]
+        lib.ka_solver_learned_evaluate.argtypes = [vp, i32, i32, C.POINTER(C.c_double)]
         lib.ka_solver_learned_report.argtypes = [vp, C.POINTER(dbl)]
         lib.ka_solver_learned_parameters.argtypes = [vp, vp, C.c_int64, i32]
         lib.ka_aggregate.argtypes = [vp, pi32, dbl, C.POINTER(vp)]
@@ -763,13 +764,20 @@ class LinearSolver:
         _check_host(host_lib().ka_solver_learned_targets(self.handle, float(rel_tol), C.byref(k)))
         return k.value
 
-    def learned_train(self, steps: int, lr: float = 1e-3):
-        """(loss, unexplained fraction) before each Adam step."""
+    def learned_train(self, steps: int, lr: float = 1e-3, pairs: int = 0):
+        """(loss, unexplained fraction) before each Adam step, cycling over the
+        first `pairs` harvested pairs (0: all)."""
         out = np.empty(max(int(steps), 1), np.float64)
         rel = np.empty(max(int(steps), 1), np.float64)
-        _check_host(host_lib().ka_solver_learned_train(self.handle, int(steps), float(lr), out.ctypes.data,
-                                                       rel.ctypes.data))
+        _check_host(host_lib().ka_solver_learned_train(self.handle, int(steps), float(lr), int(pairs),
+                                                       out.ctypes.data, rel.ctypes.data))
         return out[:int(steps)], rel[:int(steps)]
+
+    def learned_evaluate(self, first: int, count: int) -> float:
+        """Mean unexplained fraction on pairs [first, first + count), untrained."""
+        v = C.c_double()
+        _check_host(host_lib().ka_solver_learned_evaluate(self.handle, int(first), int(count), C.byref(v)))
+        return v.value
 
     def learned_report(self) -> dict:
         out = (C.c_double * 11)()

@@ -240,12 +240,15 @@ int ka_solver_learned_targets(void* solver, double rel_tol, int32_t* pairs) {
     if (pairs) *pairs = k;
   });
 }
-int ka_solver_learned_train(void* solver, int steps, float lr, double* losses, double* relative) {
+int ka_solver_learned_train(void* solver, int steps, float lr, int pairs, double* losses, double* relative) {
   return guard([&] {
-    const auto l = static_cast<Solver*>(solver)->solver.learned_train(steps, lr);
+    const auto l = static_cast<Solver*>(solver)->solver.learned_train(steps, lr, pairs);
     if (losses) std::copy(l.loss.begin(), l.loss.end(), losses);
     if (relative) std::copy(l.relative.begin(), l.relative.end(), relative);
   });
+}
+int ka_solver_learned_evaluate(void* solver, int first, int count, double* relative) {
+  return guard([&] { *relative = static_cast<Solver*>(solver)->solver.learned_evaluate(first, count); });
 }
 int ka_solver_learned_report(void* solver, double* out) {
   return guard([&] {

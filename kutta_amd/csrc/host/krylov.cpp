@@ -298,19 +298,36 @@ struct KrylovSolver::Impl {
     targeted = harvested;
     return targeted;
   }
-  TrainResult learned_train(int steps, float lr) {
+  TrainResult learned_train(int steps, float lr, int pairs) {
     KA_REQUIRE(learned != nullptr, "learned stage: not configured");
     KA_REQUIRE(targeted > 0, "learned stage: no harvested pairs with targets");
-    auto result = learned->train(harvest.data(), targets.data(), targeted, steps, lr);
+    KA_REQUIRE(pairs >= 0, "learned stage: negative training pair count");
+    const int used = pairs > 0 ? std::min(pairs, targeted) : targeted;
+    auto result = learned->train(harvest.data(), targets.data(), used, steps, lr);
     if (!result.loss.empty()) {
       learned_loss = result.loss.back();
       // The last epoch: one step per pair (or every step, if fewer).
-      const std::size_t k = std::min<std::size_t>(std::size_t(targeted), result.relative.size());
+      const std::size_t k = std::min<std::size_t>(std::size_t(used), result.relative.size());
       double sum = 0;
       for (std::size_t i = result.relative.size() - k; i < result.relative.size(); ++i) sum += result.relative[i];
       learned_relative = sum / double(k);
     }
     return result;
+  }
+  // The mean unexplained fraction of the network on harvested pairs
+  // [first, first + count), without training on them (held-out pairs).
+  double learned_evaluate(int first, int count) {
+    KA_REQUIRE(learned != nullptr, "learned stage: not configured");
+    KA_REQUIRE(first >= 0 && count >= 1 && first + count <= targeted, "learned stage: pairs out of range");
+    const std::size_t rows = std::size_t(levels[learned_level].rows);
+    double sum = 0;
+    for (int k = first; k < first + count; ++k) {
+      const double loss = learned->gradient(harvest.data() + std::size_t(k) * rows,
+                                            targets.data() + std::size_t(k) * rows);
+      const double energy = learned->target_energy();
+      sum += energy > 0 ? loss / energy : 0.0;
+    }
+    return sum / count;
   }
   void learned_stage(const DeviceCsr& a, const float* v, float* zj) {
     const Index rows = levels[learned_level].rows;
@@ -1397,7 +1414,10 @@ void KrylovSolver::set_learned_mode(int mode, double reduction, double max_relat
   impl_->learned_max_relative = max_relative;
 }
 int KrylovSolver::learned_targets(double rel_tol) { return impl_->learned_targets(rel_tol); }
-TrainResult KrylovSolver::learned_train(int steps, float lr) { return impl_->learned_train(steps, lr); }
+TrainResult KrylovSolver::learned_train(int steps, float lr, int pairs) {
+  return impl_->learned_train(steps, lr, pairs);
+}
+double KrylovSolver::learned_evaluate(int first, int count) { return impl_->learned_evaluate(first, count); }
 LearnedReport KrylovSolver::learned_report() const {
   const auto& i = *impl_;
   LearnedReport r;

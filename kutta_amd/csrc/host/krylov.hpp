@@ -121,7 +121,10 @@ class KrylovSolver {
   // Exact level corrections of the harvested residuals; returns the pairs.
   int learned_targets(double rel_tol);
   // Adam steps over the pairs: the loss and unexplained fraction per step.
-  TrainResult learned_train(int steps, float lr);
+  // Adam steps over the first `pairs` harvested pairs (0: all of them).
+  TrainResult learned_train(int steps, float lr, int pairs = 0);
+  // Mean unexplained fraction on pairs [first, first + count), not trained.
+  double learned_evaluate(int first, int count);
   LearnedReport learned_report() const;
   std::vector<float> learned_parameters() const;
   void set_learned_parameters(const std::vector<float>& theta);

@@ -43,11 +43,13 @@ def trained(device):
         assert st.converged
     harvested = solver.learned_report()["harvested"]
     pairs = solver.learned_targets(1e-6)
-    losses, relative = solver.learned_train(400, 1e-3)
+    train_pairs = (4 * pairs) // 5  # the rest are held out
+    losses, relative = solver.learned_train(400, 1e-3, train_pairs)
+    held_out = solver.learned_evaluate(train_pairs, pairs - train_pairs)
     solver.set_learned_mode(0)
     A = sp.csr_matrix((a.val, a.col, a.row_ptr), shape=(a.n_rows, a.n_rows))
     return dict(a=a, A=A, matrix=matrix, solver=solver, harvested=harvested, pairs=pairs, losses=losses,
-                relative=relative)
+                relative=relative, train_pairs=train_pairs, held_out=held_out)
 
 
 def run(device, t, b, mode, reduction=0.5, max_relative=np.inf):
@@ -78,16 +80,31 @@ def test_harvest_targets_and_training(trained):
           f"unexplained {relative[:20].mean():.3f} -> {relative[-20:].mean():.3f}", flush=True)
     assert losses[-20:].mean() < losses[:20].mean()
     assert relative[-20:].mean() < relative[:20].mean()
-    # The report's figure is the mean over the last epoch (one step per pair).
-    k = trained["pairs"]
+    # The report's figure is the mean over the last epoch (one step per
+    # training pair).
+    k = trained["train_pairs"]
     assert rep["relative"] == pytest.approx(relative[-k:].mean(), rel=1e-12)
+
+
+def test_held_out_pairs(trained):
+    """Pairs the network never trained on: its unexplained fraction there is
+    below one (it generalizes beyond its training residuals) and no better
+    than a few times its training figure."""
+    held, train = trained["held_out"], trained["solver"].learned_report()["relative"]
+    print(f"[learned solver] unexplained: training {train:.3f}, held out {held:.3f} "
+          f"({trained['pairs'] - trained['train_pairs']} pairs)", flush=True)
+    assert 0 < held < 1
+    assert held > 0.5 * train
 
 
 def test_learned_solves_converge_and_compare_with_plain(device, trained):
     """A/B on three unseen right-hand sides: the plain V-cycle and the learned
     stage (never dropped: reduction 0, no gate) both reach 1e-6 by the solver's own
-    true residual and by an FP64 host residual. The iteration counts and
-    times are printed; what the network gains is reported, not assumed."""
+    true residual and by an FP64 host residual. On this operator (level 2:
+    5k nodes) the network explains most of its targets, and the stage takes
+    no more FGMRES steps than the plain V-cycle on any right-hand side and
+    fewer in total; the times are printed (the stage's launches cost more
+    than the steps it saves)."""
     A = trained["A"]
     before = trained["solver"].learned_report()
     plain_it, learned_it = [], []
@@ -103,6 +120,7 @@ def test_learned_solves_converge_and_compare_with_plain(device, trained):
         learned_it.append(st1.iterations)
         print(f"[learned A/B] seed {seed}: plain {st0.iterations} it {ms0:.2f} ms, "
               f"learned {st1.iterations} it {ms1:.2f} ms", flush=True)
+    assert all(l <= p for l, p in zip(learned_it, plain_it)) and sum(learned_it) < sum(plain_it)
     after = trained["solver"].learned_report()
     assert after["solves"] == before["solves"] + 3
     assert after["fallbacks"] == before["fallbacks"]
