@@ -92,12 +92,14 @@ def fp64_error(x, x64) -> dict:
 
 
 # The fast pressure variant (no reference counterpart as a configuration; the
-# cycle is the reference's AmgCycle::K): flexible AMG-PCG whose coarse levels 1
-# and 2 take the K-cycle's two inner Krylov corrections, plain V-cycle
-# recursion below, one pre- and one post-smoothing sweep, 8 sweeps of the
-# coarsest level. Picked by timing the 8M-cell solve over sweeps 1-5, depths
-# 1-4 and 2-32 coarsest sweeps.
-KCYCLE = dict(pre_sweeps=1, post_sweeps=1, coarse_sweeps=8, k_depth=2)
+# cycle is the reference's AmgCycle::K): flexible AMG-PCG whose coarse levels
+# of at least 50,000 rows take the K-cycle's two inner Krylov corrections,
+# plain V-cycle recursion below, one pre- and one post-smoothing sweep, a
+# six-level hierarchy whose coarsest level gets 4 sweeps. Picked by timing the
+# 1M- and 8M-cell solves over sweeps 1-5, K depths 1-4, 2-32 coarsest sweeps
+# and 5-8 levels: the levels past the sixth barely coarsen, and a level below
+# ~50k rows is latency-bound, so a second visit costs more than it saves.
+KCYCLE = dict(pre_sweeps=1, post_sweeps=1, coarse_sweeps=4, max_levels=6, k_min_rows=50000)
 
 
 def time_solver(dev, problem, options, kind, b, x, control, steps, warmup, graphs, direct=0, bf16=False,
@@ -138,7 +140,7 @@ def time_solver(dev, problem, options, kind, b, x, control, steps, warmup, graph
 
 def large_mesh_comparison(dev, size, steps, warmup, graphs, args) -> dict:
     """The plain FP32 AMG-PCG path (V(5,5), the reference's configuration) and
-    the K-cycle variant on the size^3 box (7,986,000 cells at 110): ms to
+    the K-cycle variant (KCYCLE) on the size^3 box (7,986,000 cells at 110): ms to
     1e-6, with and without the refresh each solve needs when the operator
     changes (every SIMPLEC outer iteration), and each solution's error
     against the FP64 solution of the same operator."""

@@ -86,12 +86,13 @@ long ka_mesh_setup_array(void* setup, const char* name, int kind, void* out);
  * kind: 0 Jacobi, 1 PcgJacobi, 2 BicgstabJacobi, 3 AmgPcg, 4 AmgBicgstab;
  * additive, no reference counterpart: 5 FgmresJacobi, 6 AmgFgmres (flexible
  * GMRES(m), right preconditioned, restart length 30 unless set).
- * options[10]: pre_sweeps, post_sweeps, coarse_sweeps, coarse_rows, max_levels,
+ * options[11]: pre_sweeps, post_sweeps, coarse_sweeps, coarse_rows, max_levels,
  *             aggregation_passes, nullspace (0 none, 1 constant),
  *             structure (0 general, 1 symmetric M-matrix),
- *             cycle (0 V, 1 K: linalg/amg.hpp:6), K-cycle depth (no reference
- *             counterpart: the deepest coarse level with inner corrections,
- *             0 every level the reference gives them). */
+ *             cycle (0 V, 1 K: linalg/amg.hpp:6), and two bounds on where
+ *             the K-cycle's inner corrections run (no reference counterpart):
+ *             the deepest coarse level (0 every level the reference gives
+ *             them) and the fewest rows of such a level (0 no bound). */
 int ka_solver_create(void* dev, int kind, void* csr, const int32_t* options, double threshold,
                      void** out);
 void ka_solver_destroy(void* solver);

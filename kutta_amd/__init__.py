@@ -604,9 +604,10 @@ class AmgCycle(enum.IntEnum):
 
 @dataclass
 class AmgOptions:
-    """linalg/amg.hpp:8-20. k_depth (no reference counterpart) stops the
-    K-cycle's inner corrections below that coarse level; 0 keeps them
-    wherever the reference uses them."""
+    """linalg/amg.hpp:8-20. k_depth and k_min_rows (no reference
+    counterpart) keep the K-cycle's inner corrections to coarse levels down
+    to k_depth and of at least k_min_rows rows; 0 keeps them wherever the
+    reference uses them."""
     pre_sweeps: int = 5
     post_sweeps: int = 5
     coarse_sweeps: int = 32
@@ -618,11 +619,13 @@ class AmgOptions:
     structure: OperatorStructure = OperatorStructure.General
     cycle: AmgCycle = AmgCycle.V
     k_depth: int = 0
+    k_min_rows: int = 0
 
     def packed(self):
-        return (C.c_int32 * 10)(self.pre_sweeps, self.post_sweeps, self.coarse_sweeps,
+        return (C.c_int32 * 11)(self.pre_sweeps, self.post_sweeps, self.coarse_sweeps,
                                 self.coarse_rows, self.max_levels, self.aggregation_passes,
-                                int(self.nullspace), int(self.structure), int(self.cycle), self.k_depth)
+                                int(self.nullspace), int(self.structure), int(self.cycle), self.k_depth,
+                                self.k_min_rows)
 
 
 @dataclass

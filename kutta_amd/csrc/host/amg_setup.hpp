@@ -47,6 +47,10 @@ struct AmgOptions {
   // this coarse level, plain recursion below; 0 uses them wherever the
   // reference does (every coarse level with two levels below it).
   int k_depth = 0;
+  // K-cycle only (no reference counterpart): inner corrections only on coarse
+  // levels of at least this many rows (0: no bound), where a level's
+  // launches move enough data to be worth a second visit.
+  Index k_min_rows = 0;
 };
 
 struct Coarsening {

@@ -122,9 +122,10 @@ int ka_solver_create(void* dev, int kind, void* csr, const int32_t* options, dou
     o.aggregation_passes = options[5];
     o.nullspace = options[6] ? NullSpace::Constant : NullSpace::None;
     o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
-    KA_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0, "AMG: invalid options");
+    KA_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0 && options[10] >= 0, "AMG: invalid options");
     o.cycle = options[8] ? AmgCycle::K : AmgCycle::V;
     o.k_depth = options[9];
+    o.k_min_rows = options[10];
     o.strength_threshold = threshold;
     auto* d = static_cast<HipDevice*>(dev);
     *out = new Solver{d, KrylovSolver(*d, static_cast<SolverKind>(kind), *static_cast<CsrHost*>(csr), o)};

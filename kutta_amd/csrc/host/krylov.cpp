@@ -774,10 +774,11 @@ struct KrylovSolver::Impl {
   bool variable() const { return amg && options.cycle == AmgCycle::K; }
   // Whether coarse level `c` is reached through the K-cycle's inner
   // corrections (linalg/amg.cpp:301-302): every level with two below it,
-  // down to k_depth when set, and never at or below a dense-solved level.
+  // down to k_depth and of at least k_min_rows rows when set, and never at or
+  // below a dense-solved level.
   bool k_inner(std::size_t c) const {
     return variable() && c + 1 < levels.size() && (options.k_depth == 0 || c <= std::size_t(options.k_depth)) &&
-           !(direct_level > 0 && c >= std::size_t(direct_level));
+           levels[c].rows >= options.k_min_rows && !(direct_level > 0 && c >= std::size_t(direct_level));
   }
   // The first level the one-workgroup tail (a V-cycle) may start at.
   std::size_t tail_floor() const {

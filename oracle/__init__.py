@@ -164,11 +164,12 @@ KINDS ={"Jacobi": 0, "PcgJacobi": 1, "BicgstabJacobi": 2, "AmgPcg": 3, "AmgBicgs
 class Solver:
     def __init__(self, kind: str, a: Csr, pre=5, post=5, coarse_sweeps=32, coarse_rows=24,
                  max_levels=24, passes=3, nullspace=False, m_matrix=False, threshold=0.25, cycle="V",
-                 k_depth=0):
+                 k_depth=0, k_min_rows=0):
         """cycle "V" or "K" (linalg/amg.hpp:6); k_depth > 0 stops the K-cycle's
-        inner corrections below that coarse level."""
+        inner corrections below that coarse level, k_min_rows > 0 keeps them
+        to levels of at least that many rows."""
         opts = np.array([pre, post, coarse_sweeps, coarse_rows, max_levels, passes,
-                         int(nullspace), int(m_matrix), "VK".index(cycle), k_depth], dtype=np.int32)
+                         int(nullspace), int(m_matrix), "VK".index(cycle), k_depth, k_min_rows], dtype=np.int32)
         h = C.c_void_p()
         check(lib().oracle_solver_create(KINDS[kind], a.handle, _p(opts), threshold, C.byref(h)))
         self.handle = h.value

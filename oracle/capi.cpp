@@ -214,8 +214,9 @@ long oracle_coarsening_values(void* c, double* out) {
 void oracle_coarsening_free(void* c) { delete static_cast<Coarsening*>(c); }
 
 // ---- solvers (linalg/amg.cpp:345-382, linalg/solver.cpp:75-176)
-// options[10]: pre, post, coarse_sweeps, coarse_rows, max_levels, passes, nullspace,
-// structure, cycle (0 V, 1 K), K-cycle depth (0: every level)
+// options[11]: pre, post, coarse_sweeps, coarse_rows, max_levels, passes, nullspace,
+// structure, cycle (0 V, 1 K), K-cycle depth (0: every level), fewest rows of a
+// K-cycle level (0: no bound)
 int oracle_solver_create(int kind, void* csr, const int* options, double threshold, void** out) {
   return guard([&] {
     AmgOptions o;
@@ -223,9 +224,11 @@ int oracle_solver_create(int kind, void* csr, const int* options, double thresho
     o.coarse_rows = options[3]; o.max_levels = options[4]; o.aggregation_passes = options[5];
     o.nullspace = options[6] ? NullSpace::Constant : NullSpace::None;
     o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
-    ORACLE_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0, "AMG: invalid options");
+    ORACLE_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0 && options[10] >= 0,
+                   "AMG: invalid options");
     o.cycle = options[8] ? AmgCycle::K : AmgCycle::V;
     o.k_depth = options[9];
+    o.k_min_rows = options[10];
     o.strength_threshold = threshold;
     *out = new SolverHandle(static_cast<SolverKind>(kind), *static_cast<CsrHost*>(csr), o);
   });

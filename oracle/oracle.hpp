@@ -165,6 +165,9 @@ struct AmgOptions {
   // recursion below it); 0 uses them wherever the reference does (every
   // level with two levels below it).
   int k_depth = 0;
+  // K-cycle only: inner corrections only on coarse levels of at least this
+  // many rows (0: no bound).
+  Index k_min_rows = 0;
 };
 
 // The aggregation of one level (linalg/amg.cpp:17-20).

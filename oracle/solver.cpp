@@ -370,7 +370,8 @@ struct Solver::Impl {
   }
   bool k_inner(std::size_t coarse) const {
     return options.cycle == AmgCycle::K && coarse + 1 < levels.size() &&
-           (options.k_depth == 0 || coarse <= std::size_t(options.k_depth));
+           (options.k_depth == 0 || coarse <= std::size_t(options.k_depth)) &&
+           levels[coarse].matrix.n_rows >= options.k_min_rows;
   }
   // linalg/amg.cpp:283-310 — V-cycle, or K-cycle where k_inner holds.
   void cycle(std::size_t index, const CsrDevice& a, const V& b, V& x, const int* st) {

@@ -34,14 +34,15 @@ def solve_both(device, a, b, kind, options, control, fused=True, graphs=True, au
                         coarse_sweeps=options.coarse_sweeps, max_levels=options.max_levels,
                         passes=options.aggregation_passes, nullspace=options.nullspace == NullSpace.Constant,
                         m_matrix=options.structure == OperatorStructure.SymmetricMMatrix,
-                        cycle=options.cycle.name, k_depth=options.k_depth)
+                        cycle=options.cycle.name, k_depth=options.k_depth, k_min_rows=options.k_min_rows)
     xo, sto = ref.solve(b, rel_tol=control[0], abs_tol=control[1], max_iter=control[2])
     assert solver.describe() == ref.describe()
     return st, x.download(), sto, xo, solver
 
 
-CASES = {  # poisson(...), kind, sweeps, passes, k_depth, max_levels
+CASES = {  # poisson(...), kind, sweeps, passes, k_depth, max_levels[, k_min_rows]
     "poisson3d": ((24, 22, 12, 1.0), SolverKind.AmgPcg, 2, 1, 0, 24),
+    "poisson3d-rows": ((24, 22, 12, 1.0), SolverKind.AmgPcg, 1, 1, 0, 6, 1000),
     "poisson3d-depth1": ((24, 22, 12, 1.0), SolverKind.AmgPcg, 1, 1, 1, 24),
     "anisotropic-random": ((16, 15, 9, 1000.0, True), SolverKind.AmgPcg, 1, 1, 0, 24),
     "three-passes": ((30, 28, 20, 1.0), SolverKind.AmgPcg, 5, 3, 0, 24),
@@ -53,12 +54,12 @@ CASES = {  # poisson(...), kind, sweeps, passes, k_depth, max_levels
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("form", ["fused+graphs", "fused", "reference-kernels", "autotuned"])
 def test_k_cycle_bitwise_vs_oracle(device, name, form):
-    problem, kind, sweeps, passes, k_depth, levels = CASES[name]
+    problem, kind, sweeps, passes, k_depth, levels, *rows = CASES[name]
     print(f"[k-cycle] {name} {form}", flush=True)
     a = oracle.poisson(*problem)
     b = np.random.default_rng(3).standard_normal(a.n_rows).astype(np.float32)
     options = AmgOptions(pre_sweeps=sweeps, post_sweeps=sweeps, aggregation_passes=passes, max_levels=levels,
-                         cycle=AmgCycle.K, k_depth=k_depth)
+                         cycle=AmgCycle.K, k_depth=k_depth, k_min_rows=rows[0] if rows else 0)
     st, x, sto, xo, _ = solve_both(device, a, b, kind, options, (1e-6, 1e-12, 400), fused=form != "reference-kernels",
                                    graphs=form == "fused+graphs", autotune=form == "autotuned")
     assert st.converged and sto["converged"]
@@ -112,3 +113,24 @@ def test_k_cycle_resolve_after_refresh_is_bitwise(device):
         outs.append((st.iterations, x.download()))
     assert outs[0][0] == outs[1][0]
     assert np.array_equal(bits(outs[0][1]), bits(outs[1][1]))
+
+
+def test_full_size_pressure_variant_bitwise(device):
+    """The benchmark's K-cycle variant on the 998,250-row C2 operator:
+    V(1,1), 4 coarsest sweeps, 6 levels, inner corrections on the coarse
+    levels of at least 50,000 rows (levels 1 and 2): the same iteration count
+    and x bit for bit as the oracle."""
+    p = kutta_amd.PressureProblem(device, 55, 55, 55, reference=-2)
+    rng = np.random.default_rng(20261202)
+    b64 = rng.standard_normal(p.n_cells)
+    b64 -= b64.mean()
+    b = b64.astype(np.float32)
+    b[p.reference] = 0
+    a = oracle.Csr.from_arrays(p["row_ptr"], p["col"], p["diag"], p["values"].astype(np.float64))
+    options = AmgOptions(pre_sweeps=1, post_sweeps=1, coarse_sweeps=4, max_levels=6, aggregation_passes=2,
+                         structure=OperatorStructure.SymmetricMMatrix, cycle=AmgCycle.K, k_min_rows=50000)
+    st, x, sto, xo, solver = solve_both(device, a, b, SolverKind.AmgPcg, options, (1e-6, 0.0, 2000),
+                                        autotune=True)
+    print(f"[k-cycle] C2 variant: {st.iterations} iterations, plan {solver.plan()}", flush=True)
+    assert st.converged and st.iterations == sto["iterations"]
+    assert np.array_equal(bits(x), bits(xo))

@@ -101,7 +101,7 @@ class Hierarchy:
         return np.bincount(agg[mask], weights=fine[mask], minlength=nc)
 
 
-def textbook_kcycle(h, b, pre, post, coarse_sweeps, spd, k_depth, ratios):
+def textbook_kcycle(h, b, pre, post, coarse_sweeps, spd, k_depth, ratios, k_min_rows=0):
     levels = h.levels
     last = len(levels) - 1
 
@@ -114,7 +114,7 @@ def textbook_kcycle(h, b, pre, post, coarse_sweeps, spd, k_depth, ratios):
     def inner_level(c):
         """K-cycle corrections at every coarse level with two levels below it
         (down to k_depth when set)."""
-        return c + 1 < len(levels) and (k_depth == 0 or c <= k_depth)
+        return c + 1 < len(levels) and (k_depth == 0 or c <= k_depth) and levels[c][0].shape[0] >= k_min_rows
 
     def cycle(i, rhs):
         x = np.zeros_like(rhs)
@@ -221,6 +221,20 @@ def test_two_levels_k_applies_as_v():
     assert "K-cycle" in k.describe() and "V-cycle" in v.describe()
     b = rhs_vector(a.n_rows, 4)
     assert np.array_equal(v.apply(b).view(np.uint32), k.apply(b).view(np.uint32))
+
+
+def test_row_bound_matches_textbook():
+    """Inner corrections only on coarse levels of at least k_min_rows rows
+    (here levels 1 and 2 of 1,456 / 728 / 364 / 182 / 91 rows)."""
+    a = oracle.poisson(14, 13, 8)
+    s = oracle.Solver("AmgPcg", a, pre=2, post=2, max_levels=5, passes=1, cycle="K", k_min_rows=300)
+    h = Hierarchy(s, a)
+    b = rhs_vector(a.n_rows, 11)
+    ratios = []
+    want = textbook_kcycle(h, b.astype(np.float64), 2, 2, 32, True, 0, ratios, k_min_rows=300)
+    assert len(ratios) in (2, 3)  # one level-1 correction, one level-2 correction per level-1 cycle
+    z = s.apply(b).astype(np.float64)
+    assert np.linalg.norm(z - want) / np.linalg.norm(want) < 2e-5
 
 
 def test_k_depth_at_or_below_last_inner_level_is_the_full_k_cycle():
