@@ -1290,6 +1290,34 @@ SST_INT_FIELDS = ("fixed_omega_mask", "selected_wall_face")
 BOUNDARY_PROFILE = C.CFUNCTYPE(None, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
 
 
+class _ProfileCallback:
+    """The C callback around the patches' velocity functions. A function that
+    raises, or returns anything but three numbers, writes NaN, which the
+    solver's finiteness check rejects; the first such exception is kept and
+    re-raised once the constructor returns (an exception cannot cross the C
+    frames)."""
+
+    def __init__(self, functions):
+        self.functions = functions
+        self.error = None
+        self.c_function = BOUNDARY_PROFILE(self._call)
+
+    def _call(self, patch, point, velocity, _user):
+        try:
+            v = [float(x) for x in self.functions[patch]((point[0], point[1], point[2]))]
+            if len(v) != 3:
+                raise ValueError(f"boundary profile of patch {PATCHES[patch]} returned {len(v)} components, not 3")
+            velocity[0], velocity[1], velocity[2] = v
+        except Exception as exc:  # noqa: BLE001 - reported after the C call returns
+            if self.error is None:
+                self.error = exc
+            velocity[0] = velocity[1] = velocity[2] = float("nan")
+
+    def reraise(self):
+        if self.error is not None:
+            raise self.error
+
+
 def profile_callback(conditions: dict):
     """(profiled[6], callback) for the patches whose inlet or backflow
     velocity is a function of the face centroid, else (None, None)."""
@@ -1303,12 +1331,7 @@ def profile_callback(conditions: dict):
     if not functions:
         return None, None
     profiled = (C.c_int32 * 6)(*[1 if p in functions else 0 for p in range(6)])
-
-    def call(patch, point, velocity, _user):
-        v = functions[patch]((point[0], point[1], point[2]))
-        velocity[0], velocity[1], velocity[2] = (float(x) for x in v)
-
-    return profiled, BOUNDARY_PROFILE(call)
+    return profiled, _ProfileCallback(functions)
 
 
 def pack_open_conditions(conditions: dict):
@@ -1397,9 +1420,11 @@ class SteadySolver:
             _check_host(host_lib().ka_simplec_create_open(device.handle, nx, ny, nz, box, kinds, velocity, pressure,
                                                           total, s.packed(), C.byref(h)))
         else:  # the callback runs inside the constructor only
-            _check_host(host_lib().ka_simplec_create_profiled(device.handle, nx, ny, nz, box, kinds, velocity,
-                                                              pressure, total, profiled, callback, None,
-                                                              s.packed(), C.byref(h)))
+            status = host_lib().ka_simplec_create_profiled(device.handle, nx, ny, nz, box, kinds, velocity,
+                                                           pressure, total, profiled, callback.c_function, None,
+                                                           s.packed(), C.byref(h))
+            callback.reraise()
+            _check_host(status)
         self.device, self.handle, self.settings = device, h.value, s
 
     @classmethod

@@ -527,12 +527,16 @@ class SteadySolver:
         box = (C.c_double * 3)(*[float(e) for e in (extent or (1.0, 1.0, 1.0))])
         self._keep = (kinds, wall, settings, pressure, total, box, profiled, profile)
         if profiled is not None:
+            # kutta_amd.profile_callback's wrapper, or a bare C function.
+            function = getattr(profile, "c_function", profile)
             L.oracle_simplec_create_profiled.restype = vp
-            L.oracle_simplec_create_profiled.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, type(profile), vp,
+            L.oracle_simplec_create_profiled.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, type(function), vp,
                                                          vp]
             h = L.oracle_simplec_create_profiled(nx, ny, nz, C.cast(box, vp), C.cast(kinds, vp), C.cast(wall, vp),
                                                  C.cast(pressure, vp), C.cast(total, vp), C.cast(profiled, vp),
-                                                 profile, None, C.cast(settings, vp))
+                                                 function, None, C.cast(settings, vp))
+            if hasattr(profile, "reraise"):
+                profile.reraise()
         elif pressure is None and extent is None:
             h = L.oracle_simplec_create(nx, ny, nz, C.cast(kinds, vp), C.cast(wall, vp), C.cast(settings, vp))
         else:

@@ -105,3 +105,39 @@ def test_rcm_is_deterministic_and_a_permutation():
     assert np.all(owner[:len(nb)] < nb)
     key = owner[:len(nb)].astype(np.int64) * (1 << 32) + nb
     assert np.all(np.diff(key) > 0)
+
+
+@pytest.mark.parametrize("shape,amplitude,seed", [((9, 8, 7), 0.2, 20261202), ((12, 5, 6), 0.35, 7)])
+def test_perturbation_and_reference_cell_properties(shape, amplitude, seed):
+    """The seeded perturbation of the benchmark box (oracle/mesh.cpp, the
+    workload definition): every node moves by at most amplitude x the cell
+    size along each axis, nodes on a face of the box stay on its plane
+    (corners and edges stay put), every cell keeps a positive volume, the
+    same seed gives the same points and another seed others; the pinned
+    reference cell (-2: the cell holding the origin) has the origin among its
+    nodes."""
+    b = oracle.pressure_problem(*shape, tet=True, amplitude=amplitude, seed=seed)
+    p = b["points"].reshape(-1, 3)
+    # The generator's lattice, node v = i + (nx+1) (j + (ny+1) k), before the noise.
+    nx, ny, nz = shape
+    v = np.arange(p.shape[0])
+    g = np.stack([(v % (nx + 1)) / nx, (v // (nx + 1) % (ny + 1)) / ny, (v // ((nx + 1) * (ny + 1))) / nz], 1)
+    assert np.array_equal(oracle.pressure_problem(*shape, tet=True, amplitude=0.0)["points"].reshape(-1, 3), g)
+    h = 1.0 / np.asarray(shape, np.float64)
+    assert np.all(np.abs(p - g) <= amplitude * h * (1 + 1e-12))
+    assert np.any(np.abs(p - g) > 0.5 * amplitude * h)  # the noise is not degenerate
+    for axis in range(3):
+        for plane in (0.0, 1.0):
+            on = g[:, axis] == plane
+            assert on.any() and np.array_equal(p[on, axis], g[on, axis])
+    assert np.all(b["cell_volume"] > 0)
+    assert b["cell_volume"].sum() == pytest.approx(1.0, rel=1e-12)
+    again = oracle.pressure_problem(*shape, tet=True, amplitude=amplitude, seed=seed)
+    assert np.array_equal(again["points"].view(np.uint64), b["points"].view(np.uint64))
+    other = oracle.pressure_problem(*shape, tet=True, amplitude=amplitude, seed=seed + 1)
+    assert not np.array_equal(other["points"], b["points"])
+    ref = int(b["reference"][0])
+    faces = b["cell_faces"][b["cell_face_offsets"][ref]:b["cell_face_offsets"][ref + 1]]
+    offsets, face_nodes = b["face_offsets"], b["face_nodes"]
+    nodes = np.concatenate([face_nodes[offsets[f]:offsets[f + 1]] for f in faces])
+    assert np.any(np.all(p[nodes] == 0.0, axis=1))

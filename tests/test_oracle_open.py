@@ -229,3 +229,14 @@ def test_mass_flow_inlet_validation():
         c["xmin"] = bad
         with pytest.raises(oracle.OracleError):
             make((4, 2, 1), c)
+
+
+def test_profile_errors_reach_the_caller():
+    """A boundary profile that raises, or returns the wrong number of
+    components, is reported to the caller with its own exception; the face it
+    was evaluated for gets NaN, which the solver's finiteness check rejects."""
+    for bad, kind in ((lambda x: 1 / 0, ZeroDivisionError), (lambda x: (1.0, 0.0), ValueError)):
+        c = couette()
+        c["xmin"] = VelocityInlet(bad)
+        with pytest.raises(kind):
+            make_profiled((4, 12, 1), c, SolverSettings(viscosity=1.0))
