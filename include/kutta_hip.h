@@ -365,6 +365,11 @@ typedef enum kh_kernel {
   KH_INNER_FIRST_UPDATE,
   KH_INNER_BETA_STEP,
   KH_INNER_SECOND_STEP,
+  /* KH_AMG_SMOOTH with one wavefront per row: the lanes gather the row and
+     form its products and their errors in parallel, one lane sums them in
+     entry order (no reference counterpart; same arguments, same bits). For
+     levels with long rows. */
+  KH_AMG_SMOOTH_ROWWAVE,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 #define KH_COARSE_SWEEPS_MAX_ROWS 16384

@@ -109,7 +109,7 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_solve.argtypes = [vp, vp, vp, vp, dbl, dbl, i32, i32, C.POINTER(dbl)]
         lib.ka_solver_describe.argtypes = [vp, C.c_char_p, i32]
         lib.ka_solver_plan.argtypes = [vp, C.c_char_p, i32]
-        lib.ka_solver_set_plan.argtypes = [vp, i32, i32, i32]
+        lib.ka_solver_set_plan.argtypes = [vp, i32, i32, i32, i32]
         lib.ka_simplec_pressure_levels.argtypes = [vp, vp, vp, i32]
         lib.ka_simplec_pressure_plan.argtypes = [vp, C.c_char_p, i32]
         lib.ka_solver_solve_batched.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, i32, i32, C.POINTER(dbl)]
@@ -393,6 +393,7 @@ class Kernel(enum.IntEnum):
     inner_first_update = 252
     inner_beta_step = 253
     inner_second_step = 254
+    amg_smooth_rowwave = 255
 
 
 class Buffer:
@@ -828,15 +829,16 @@ class LinearSolver:
         host_lib().ka_solver_describe(self.handle, buf, 512)
         return buf.value.decode()
 
-    def set_plan(self, restrict_mode: int = -1, coarse_fused_sweeps: int = -1, prolong_mode: int = -1) -> None:
+    def set_plan(self, restrict_mode: int = -1, coarse_fused_sweeps: int = -1, prolong_mode: int = -1,
+                 sweep_mode: int = -1) -> None:
         """Forces the V-cycle forms (see plan()); -1 keeps a setting."""
         _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_fused_sweeps),
-                                                  int(prolong_mode)))
+                                                  int(prolong_mode), int(sweep_mode)))
 
     def plan(self) -> str:
         """The V-cycle forms the autotuner picked (same iterates for all)."""
-        buf = C.create_string_buffer(512)
-        host_lib().ka_solver_plan(self.handle, buf, 512)
+        buf = C.create_string_buffer(2048)
+        host_lib().ka_solver_plan(self.handle, buf, 2048)
         return buf.value.decode()
 
     def levels(self) -> list[tuple[int, int]]:
@@ -1563,8 +1565,8 @@ class SteadySolver:
         return [(rows[i], nnz[i]) for i in range(min(count, 64))]
 
     def pressure_plan(self) -> str:
-        buf = C.create_string_buffer(512)
-        if host_lib().ka_simplec_pressure_plan(self.handle, buf, 512) < 0:
+        buf = C.create_string_buffer(2048)
+        if host_lib().ka_simplec_pressure_plan(self.handle, buf, 2048) < 0:
             raise KuttaError(host_lib().ka_last_error().decode())
         return buf.value.decode()
 

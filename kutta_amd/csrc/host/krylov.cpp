@@ -59,6 +59,10 @@ struct Level {
   // sweep's gathers; 1 the in-place prolongation, then a plain sweep, which
   // keeps the sweep's gathers to one dependent load per entry.
   int prolong_mode = 0;
+  // The plain sweep's form (bitwise equal, picked by the autotuner for levels
+  // with rows longer than KH_WIDE_ROW): 0 one thread per row, 1 one
+  // wavefront per row.
+  int sweep_mode = 0;
   // K-cycle workspaces of a coarse level (linalg/amg.cpp:185-193): the inner
   // corrections' vectors, 12 pair scalars and 1-int status.
   DeviceArray<float> kp, kq, kz, kt, kscalars;
@@ -603,7 +607,8 @@ struct KrylovSolver::Impl {
   // The plain sweep's kernel: the wide-chunk instantiation on the fused path
   // for levels with rows longer than KH_WIDE_ROW (same bits).
   kh_kernel smooth_kernel(const Level& l) const {
-    return fused && l.max_row > KH_WIDE_ROW ? KH_AMG_SMOOTH_WIDE : KH_AMG_SMOOTH;
+    if (!fused || l.max_row <= KH_WIDE_ROW) return KH_AMG_SMOOTH;
+    return l.sweep_mode == 1 ? KH_AMG_SMOOTH_ROWWAVE : KH_AMG_SMOOTH_WIDE;
   }
 
   // What a V-cycle completed beyond z = M^-1 b.
@@ -1349,6 +1354,25 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
       }
     }
   }
+  // Sweep form per level with long rows: time plain sweeps of each form on
+  // the level's current data. Every form gives the same bits.
+  for (std::size_t i = 0; i < I.levels.size(); ++i) {
+    Level& l = I.levels[i];
+    if (I.bf16_level(l) || l.max_row <= KH_WIDE_ROW) continue;
+    const DeviceCsr li = i == 0 ? a : l.matrix.view();
+    const float* xin = i == 0 ? out.data() : l.x.data();
+    const float* bin = i == 0 ? v.data() : l.b.data();
+    double thread_form = 0;
+    for (int mode = 0; mode < 2; ++mode) {
+      l.sweep_mode = mode;
+      const kh_kernel k = I.smooth_kernel(l);
+      const double t = seconds([&] {
+        I.dev.launch(k, li.n_rows, {li.row_ptr, li.col, li.val, xin, bin, l.l1, l.r, I.status.data()});
+      });
+      if (mode == 0) thread_form = t;
+      else l.sweep_mode = t < thread_form ? 1 : 0;
+    }
+  }
   // Prolongation form per level: time whole V-cycles with each level's two
   // forms in turn (the in-place form changes the level's iterate, so it is
   // not timed in isolation) and keep the faster. Every form gives the same bits.
@@ -1472,15 +1496,17 @@ std::string KrylovSolver::describe() const {
         << impl_->levels[impl_->learned_level].rows << " rows)";
   return out.str();
 }
-void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps, int prolong_mode) {
+void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps, int prolong_mode, int sweep_mode) {
   KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_fused_sweeps >= -1 && coarse_fused_sweeps <= 1 &&
-                 prolong_mode >= -1 && prolong_mode <= 1,
-             "solver plan: restriction form -1..2, coarsest sweeps -1..1, prolongation form -1..1");
+                 prolong_mode >= -1 && prolong_mode <= 1 && sweep_mode >= -1 && sweep_mode <= 1,
+             "solver plan: restriction form -1..2, coarsest sweeps -1..1, prolongation and sweep forms -1..1");
   impl_->drop_graphs();
   if (restrict_mode >= 0)
     for (auto& l : impl_->levels) l.restrict_mode = restrict_mode;
   if (prolong_mode >= 0)
     for (auto& l : impl_->levels) l.prolong_mode = prolong_mode;
+  if (sweep_mode >= 0)
+    for (auto& l : impl_->levels) l.sweep_mode = sweep_mode;
   if (coarse_fused_sweeps >= 0) impl_->coarse_fused_sweeps = coarse_fused_sweeps != 0;
 }
 std::string KrylovSolver::plan() const {
@@ -1494,6 +1520,14 @@ std::string KrylovSolver::plan() const {
   if (!L.empty() && impl_->coarse_sweeps_fit(L.size() - 1))
     out << "; coarsest level " << (impl_->coarse_fused_sweeps ? "one-workgroup sweeps" : "sweep launches");
   out << "; tail from level " << impl_->tail_first;
+  out << "; longest row per level";
+  for (std::size_t i = 0; i < L.size(); ++i) out << (i ? "," : " ") << L[i].max_row;
+  out << "; sweeps per level";
+  for (std::size_t i = 0; i < L.size(); ++i)
+    out << (i ? "," : " ")
+        << (impl_->smooth_kernel(L[i]) == KH_AMG_SMOOTH_ROWWAVE ? "wave"
+            : impl_->smooth_kernel(L[i]) == KH_AMG_SMOOTH_WIDE  ? "wide"
+                                                                 : "thread");
   return out.str();
 }
 std::vector<std::pair<Index, Index>> KrylovSolver::levels() const {

@@ -157,6 +157,66 @@ __global__ void __launch_bounds__(kBlock) k_amg_smooth(int n, const int* __restr
                        : residual_row(i, rows, cols, values, x, b);
   next[i] = x[i] + (2.0f / 3.0f) * inverse[i] * r;
 }
+// The same sweep with one wavefront per row, for levels whose rows are long
+// (the last levels where pairwise aggregation stalls have rows of up to ~100
+// entries): the 64 lanes load the row's entries and gathered x values and
+// form each product and its error (independent of the running sum) in
+// parallel, up to kRowWaveEntries per row at once; lane 0 then runs the
+// compensated sum over them in entry order, the arithmetic of RowResidual.
+constexpr int kRowWaveSlots = 4;                       // entries per lane per pass
+constexpr int kRowWaveEntries = 64 * kRowWaveSlots;    // entries per pass
+__global__ void __launch_bounds__(kBlock) k_amg_smooth_rowwave(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+    const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  __shared__ float term[kBlock / 64][kRowWaveEntries];
+  __shared__ float error[kBlock / 64][kRowWaveEntries];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = int(xcd_block()) * (kBlock / 64) + wave;
+  if (i >= n) return;  // uniform per wavefront; no block-wide barrier follows
+  const int begin = rows[i], end = rows[i + 1];
+  float s = b[i], e = 0.0f;
+  for (int base = begin; base < end; base += kRowWaveEntries) {
+    float a[kRowWaveSlots], xv[kRowWaveSlots];
+    int c[kRowWaveSlots];
+#pragma unroll
+    for (int k = 0; k < kRowWaveSlots; ++k) {
+      const int j = base + k * 64 + lane;
+      if (j < end) {
+        c[k] = cols[j];
+        a[k] = values[j];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRowWaveSlots; ++k)
+      if (base + k * 64 + lane < end) xv[k] = x[c[k]];
+#pragma unroll
+    for (int k = 0; k < kRowWaveSlots; ++k) {
+      if (base + k * 64 + lane < end) {
+        const float prod = a[k] * xv[k];
+        term[wave][k * 64 + lane] = -prod;
+        error[wave][k * 64 + lane] = product_error(a[k], xv[k], prod);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane == 0) {
+      const int count = min(end - base, kRowWaveEntries);
+      for (int k = 0; k < count; ++k) {
+        const float t = term[wave][k];
+        const float nx = s + t;
+        e += fabsf(s) >= fabsf(t) ? (s - nx) + t : (t - nx) + s;
+        e -= error[wave][k];
+        s = nx;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (lane == 0) next[i] = x[i] + (2.0f / 3.0f) * inverse[i] * (s + e);
+}
+
 // kernels/linalg.kern.h:162-170
 __global__ void __launch_bounds__(kBlock) k_amg_smooth_zero(int n, const int* __restrict__ rows,
     const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
@@ -1359,6 +1419,11 @@ void l_scale_copy(kh_device* d, int n, const kh_arg* a) {
 }
 void l_mul(kh_device* d, int n, const kh_arg* a) { k_mul<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2)); }
 #define SINGLE dim3(1), dim3(64), 0, d->stream
+void l_amg_smooth_rowwave(kh_device* d, int n, const kh_arg* a) {
+  const unsigned blocks = unsigned((n + kBlock / 64 - 1) / (kBlock / 64));
+  k_amg_smooth_rowwave<<<dim3(blocks), dim3(kBlock), 0, d->stream>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4),
+                                                                     KCF(5), KF(6), KB(7));
+}
 void l_amg_smooth_wide(kh_device* d, int n, const kh_arg* a) {
   k_amg_smooth<float, true><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
 }
@@ -1706,6 +1771,7 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_AMG_INNER_BETA, "amg_inner_beta", "bb", l_amg_inner_beta},
       {KH_AMG_INNER_STEP, "amg_inner_step", "bbbb", l_amg_inner_step},
       {KH_AMG_SMOOTH_WIDE, "amg_smooth_wide", "bbbbbbbb", l_amg_smooth_wide},
+      {KH_AMG_SMOOTH_ROWWAVE, "amg_smooth_rowwave", "bbbbbbbb", l_amg_smooth_rowwave},
       {KH_INNER_FIRST_STEP, "inner_first_step", "bbbbbbbbbbi", l_inner_first_step},
       {KH_INNER_FIRST_UPDATE, "inner_first_update", "bbbbbbb", l_inner_first_update},
       {KH_INNER_BETA_STEP, "inner_beta_step", "bbbbbbbbi", l_inner_beta_step},

@@ -2,7 +2,8 @@
 (anisotropic meshes, where pairwise aggregation stalls): a coarsest level too
 large for the one-workgroup coarse tail is smoothed by one launch of all its
 sweeps, aggregates of about two members are restricted 128 coarse rows per
-workgroup, and the prolongation runs folded into the next sweep or in place. Both must leave the iterates bitwise equal to the oracle's
+workgroup, the prolongation runs folded into the next sweep or in place,
+and levels with long rows are swept one thread or one wavefront per row. Both must leave the iterates bitwise equal to the oracle's
 reference sequence (linalg/amg.cpp:283-310)."""
 import numpy as np
 import pytest
@@ -31,11 +32,11 @@ def test_large_coarsest_level_and_pair_aggregates_bitwise(device, size, aniso, l
     solver.set_tail(0)
     solver.refresh(matrix)
     xb = device.upload(b)
-    plans = [(True, mode, sweeps, prolong) for mode in (0, 1, 2) for sweeps in (0, 1) for prolong in (0, 1)]
-    plans += [(False, 0, 0, 0)]
-    for fused, mode, sweeps, prolong in plans:
+    plans = [(True, mode, sweeps, prolong, 0) for mode in (0, 1, 2) for sweeps in (0, 1) for prolong in (0, 1)]
+    plans += [(True, 0, 0, prolong, 1) for prolong in (0, 1)] + [(False, 0, 0, 0, 0)]
+    for fused, mode, sweeps, prolong, sweep in plans:
         solver.set_fused(fused)
-        solver.set_plan(mode, sweeps, prolong)
+        solver.set_plan(mode, sweeps, prolong, sweep)
         x = device.alloc(a.n_rows)
         x.fill(np.float32(0))
         st = solver.solve(matrix, x, xb, SolveControl(1e-6, 1e-12, 300))

@@ -366,3 +366,33 @@ def test_fused_kernels_equal_reference_sequences(device, op, status0):
     device.launch(Kernel.solver_pcg_beta, 1, sb, stb, 0)
     same(o1.download(), o2.download())
     same(sa.download()[[0, 4, 10]], sb.download()[[0, 4, 10]])
+
+
+@pytest.mark.parametrize("lengths", ["short", "long"])
+def test_long_row_sweep_forms(device, lengths, status0, status1):
+    """The wide-chunk and wavefront-per-row sweeps equal the reference's
+    amg_smooth bit for bit, on rows of 1 to 600 entries (crossing the
+    wavefront form's 256-entry passes) with mixed-magnitude values; a frozen
+    status leaves the output untouched."""
+    rng = np.random.default_rng(21 if lengths == "short" else 22)
+    n = 700
+    top = 20 if lengths == "short" else 600
+    row_len = rng.integers(1, top + 1, n)
+    row_len[:3] = [1, 256, min(top, 257)]
+    rows = np.concatenate([[0], np.cumsum(row_len)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(n, size=k, replace=False)) for k in row_len]).astype(np.int32)
+    vals = mixed_magnitude_vector(int(rows[-1]), 23)
+    x = mixed_magnitude_vector(n, 24)
+    b = mixed_magnitude_vector(n, 25)
+    inv = (np.abs(random_vector(n, 26)) + 0.1).astype(np.float32)
+    drows, dcols, dvals = device.upload(rows), device.upload(cols), device.upload(vals)
+    dx, db, di, dn = device.upload(x), device.upload(b), device.upload(inv), device.alloc(n)
+    want = np.zeros(n, np.float32)
+    oracle.kernel("amg_smooth", n, rows, cols, vals, x, b, inv, want, np.zeros(3, np.int32))
+    for kernel in (Kernel.amg_smooth, Kernel.amg_smooth_wide, Kernel.amg_smooth_rowwave):
+        dn.fill(np.float32(0))
+        device.launch(kernel, n, drows, dcols, dvals, dx, db, di, dn, status0)
+        same(dn.download(), want)
+        dn.fill(np.float32(3))
+        device.launch(kernel, n, drows, dcols, dvals, dx, db, di, dn, status1)
+        assert np.all(dn.download() == 3)
