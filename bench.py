@@ -536,12 +536,14 @@ def step_case(nz: int):
 
 
 def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, cpu_iterations: int,
-                       graphs: bool = True) -> dict:
+                       graphs: bool = True, kcycle: bool = True) -> dict:
     """ms per steady SST SIMPLEC outer iteration on the backward-facing step
     (step_case), nz cells across the span: uniform inflow state, `warmup`
     iterations untimed, then `iterations` timed. The oracle is timed on one
     host core on the same geometry with cpu_nz spanwise cells (fewer cells:
-    the serial oracle takes about a minute per iteration at full size)."""
+    the serial oracle takes about a minute per iteration at full size).
+    With kcycle the same iterations are timed again with the pressure AMG
+    rebuilt as the K-cycle (set_pressure_options; no reference counterpart)."""
     import kutta_amd
     mesh, cond, settings, k0, w0, first = step_case(nz)
     t = time.perf_counter()
@@ -574,6 +576,33 @@ def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, 
            "maximum_velocity": report["maximum_velocity"],
            "pressure_levels": solver.pressure_levels(), "pressure_plan": solver.pressure_plan()}
     out["roofline"] = logical_traffic(dev, solver, ms, 2)
+    if kcycle:
+        # The same iterations with the pressure AMG rebuilt as the K-cycle
+        # (KCYCLE, 3 aggregation passes as the default hierarchy): fields
+        # re-initialised to the uniform inflow state, same warm-up.
+        from kutta_amd import AmgCycle, AmgOptions, OperatorStructure
+        solver.set_graphs(graphs)
+        solver.initialize_fields((STEP_SPEED, 0.0, 0.0), 0.0, k0, w0)
+        solver.set_pressure_options(AmgOptions(structure=OperatorStructure.SymmetricMMatrix, cycle=AmgCycle.K,
+                                               **KCYCLE))
+        solver.iterate(warmup)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        solver.iterate(iterations)
+        dev.synchronize()
+        kms = (time.perf_counter() - t0) * 1e3 / max(iterations, 1)
+        kreport = solver.report(monitor=True)
+        out["pressure_kcycle"] = {
+            "options": f"K-cycle V(1,1), 4 coarsest sweeps, 6 levels, inner corrections on levels of "
+                       f"{KCYCLE['k_min_rows']} rows or more",
+            "ms_per_iteration": round(kms, 3), "speedup": round(ms / kms, 3),
+            "last_pressure_iterations": int(kreport["pressure_iterations"]),
+            "residuals": {k: kreport[k] for k in ("continuity_scaled", "u_scaled", "v_scaled", "k_scaled",
+                                                  "omega_scaled")},
+            "pressure_plan": solver.pressure_plan()}
+        print(f"[bench] SST step with K-cycle pressure: {kms:.3f} ms per iteration "
+              f"({out['pressure_kcycle']['last_pressure_iterations']} pressure iterations last)",
+              file=sys.stderr, flush=True)
     solver.close()
     if cpu_iterations > 0:
         import numpy as np

@@ -308,6 +308,12 @@ int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used);
 /* HIP graph replay of the pressure solve's PCG iterations (default on;
  * identical iterates; ka_solver_set_graphs). */
 int ka_simplec_set_graphs(void* solver, int on);
+/* Rebuilds the pressure solve's AMG with other options (options[11] and the
+ * strength threshold as for ka_solver_create; symmetric M-matrix structure,
+ * no nullspace), e.g. the K-cycle. No reference counterpart: the reference
+ * fixes them (fv/steady_solver.cpp:573-578). The next step refreshes the new
+ * hierarchy with the current operator. */
+int ka_simplec_set_pressure_options(void* solver, const int32_t* options, double threshold);
 /* The pressure solver's hierarchy (rows and nonzeros per level, up to
  * capacity; returns the level count) and its autotuned V-cycle plan (as
  * ka_solver_plan). No reference counterpart. */

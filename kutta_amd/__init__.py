@@ -163,6 +163,7 @@ def host_lib() -> C.CDLL:
         lib.ka_simplec_centroids.argtypes = [vp, vp]
         lib.ka_simplec_set_batched_momentum.argtypes = [vp, i32, pi32]
         lib.ka_simplec_set_graphs.argtypes = [vp, i32]
+        lib.ka_simplec_set_pressure_options.argtypes = [vp, vp, C.c_double]
         lib.ka_learned_parameter_count.argtypes = [i32]
         lib.ka_learned_parameter_count.restype = C.c_int64
         lib.ka_learned_create.argtypes = [vp, i32, C.c_uint64, C.POINTER(vp)]
@@ -1549,6 +1550,13 @@ class SteadySolver:
     def set_graphs(self, on: bool) -> None:
         """HIP graph replay of the pressure PCG iterations (default on)."""
         _check_host(host_lib().ka_simplec_set_graphs(self.handle, int(bool(on))))
+
+    def set_pressure_options(self, options: "AmgOptions") -> None:
+        """Rebuilds the pressure AMG with other options (no reference
+        counterpart), e.g. the K-cycle; structure SymmetricMMatrix."""
+        packed = options.packed()
+        _check_host(host_lib().ka_simplec_set_pressure_options(self.handle, C.cast(packed, C.c_void_p),
+                                                               float(options.strength_threshold)))
 
     def set_batched_momentum(self, on: bool) -> bool:
         """Solve u, v, w in one batched BiCGStab (on by default where it is

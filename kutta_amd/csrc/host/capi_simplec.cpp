@@ -250,6 +250,27 @@ int ka_simplec_pressure_plan(void* solver, char* out, int capacity) {
   }
   return int(text.size());
 }
+int ka_simplec_set_pressure_options(void* solver, const int32_t* options, double threshold) {
+  return guard([&] {
+    KA_REQUIRE(options != nullptr, "SIMPLEC: null pressure options");
+    KA_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0 && options[10] >= 0,
+               "AMG: invalid options");
+    AmgOptions o;
+    o.pre_sweeps = options[0];
+    o.post_sweeps = options[1];
+    o.coarse_sweeps = options[2];
+    o.coarse_rows = options[3];
+    o.max_levels = options[4];
+    o.aggregation_passes = options[5];
+    o.nullspace = options[6] ? NullSpace::Constant : NullSpace::None;
+    o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
+    o.cycle = options[8] ? AmgCycle::K : AmgCycle::V;
+    o.k_depth = options[9];
+    o.k_min_rows = options[10];
+    o.strength_threshold = threshold;
+    handle(solver).solver->set_pressure_options(o);
+  });
+}
 int ka_simplec_set_batched_momentum(void* solver, int on, int32_t* used) {
   return guard([&] {
     const bool u = handle(solver).solver->set_batched_momentum(on != 0);

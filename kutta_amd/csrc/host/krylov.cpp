@@ -1203,6 +1203,7 @@ void KrylovSolver::set_graphs(bool on) {
   impl_->drop_graphs();
   impl_->graphs = on && !profiler_attached();
 }
+bool KrylovSolver::graphs_enabled() const { return impl_->graphs; }
 
 // Several BiCGStab-Jacobi solves in lockstep: each lane keeps its own entry,
 // batches, polls, replacements and exit, and its iterations run as lanes of

@@ -83,6 +83,7 @@ class KrylovSolver {
   // Replay each fused PCG iteration from a captured HIP graph (default on;
   // identical iterates). No reference counterpart.
   void set_graphs(bool on);
+  bool graphs_enabled() const;
   // Solves up to KH_BATCH_MAX_LANES independent systems with fused
   // BiCGStab-Jacobi solvers at once (one launch per step for all of them):
   // each system's iterates, statistics and x are bitwise those of

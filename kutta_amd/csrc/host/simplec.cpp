@@ -219,6 +219,7 @@ struct SteadySolver::Impl {
   LsqFit velocity_lsq, pressure_lsq;
   OwnedCsr matrix;
   std::unique_ptr<KrylovSolver> pressure_solver;
+  CsrHost pressure_initial;  // the operator the pressure hierarchy was built from
   Equation momentum[3];
   F face_correction, zero;
 
@@ -1152,6 +1153,7 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
   amg.structure = Structure::SymmetricMMatrix;
   if (sst) amg.pre_sweeps = amg.post_sweeps = 2;  // :573-578
   pressure_solver = std::make_unique<KrylovSolver>(dev, SolverKind::AmgPcg, initial, amg);
+  pressure_initial = std::move(initial);
   const double total = std::accumulate(geom.cell_volume.begin(), geom.cell_volume.end(), 0.);
   Vec3 lo = host.points.front(), hi = lo;
   for (auto p : host.points) {
@@ -1514,6 +1516,18 @@ const MeshGeometry& SteadySolver::geometry() const { return impl_->geom; }
 Index SteadySolver::cells() const { return impl_->n; }
 bool SteadySolver::set_batched_momentum(bool on) { return impl_->set_batched(on); }
 void SteadySolver::set_graphs(bool on) { impl_->pressure_solver->set_graphs(on); }
+// No reference counterpart: the pressure AMG options are fixed there
+// (steady_solver.cpp:573-578). The hierarchy is rebuilt from the operator it
+// was first built from; the next step refreshes it with the current values.
+void SteadySolver::set_pressure_options(const AmgOptions& options) {
+  Impl& I = *impl_;
+  KA_REQUIRE(options.structure == Structure::SymmetricMMatrix && options.nullspace == NullSpace::None,
+             "SIMPLEC: the pressure operator is a pinned symmetric M-matrix");
+  auto next = std::make_unique<KrylovSolver>(I.dev, SolverKind::AmgPcg, I.pressure_initial, options);
+  next->set_graphs(I.pressure_solver->graphs_enabled());
+  I.pressure_solver = std::move(next);
+  I.pressure_tuned = false;
+}
 std::vector<std::pair<Index, Index>> SteadySolver::pressure_levels() const {
   return impl_->pressure_solver->levels();
 }

@@ -157,6 +157,9 @@ class SteadySolver {
   // HIP graph replay of the pressure PCG iterations (default on; identical
   // iterates).
   void set_graphs(bool on);
+  // Rebuilds the pressure AMG with other options (no reference counterpart;
+  // e.g. the K-cycle). Symmetric M-matrix structure, no nullspace.
+  void set_pressure_options(const AmgOptions& options);
   // The pressure solver's hierarchy (rows, nonzeros per level) and plan.
   std::vector<std::pair<Index, Index>> pressure_levels() const;
   std::string pressure_plan() const;

@@ -499,6 +499,7 @@ def _simplec_lib():
     L.oracle_simplec_last_error.restype = C.c_char_p
     L.oracle_simplec_free.argtypes = [vp]
     L.oracle_simplec_iterate.argtypes = [vp, i32]
+    L.oracle_simplec_set_pressure_options.argtypes = [vp, vp, C.c_double]
     L.oracle_simplec_report.argtypes = [vp, i32, vp]
     L.oracle_simplec_array.argtypes = [vp, C.c_char_p, vp]
     L.oracle_simplec_array.restype = C.c_long
@@ -622,6 +623,12 @@ class SteadySolver:
         ptrs = (C.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
         if L.oracle_simplec_sst_update(self.handle, C.cast(options, C.c_void_p), ptrs) != 0:
             raise OracleError(L.oracle_simplec_last_error().decode())
+
+    def set_pressure_options(self, packed, threshold=0.25):
+        """Rebuilds the pressure AMG from options[11] (kutta_amd.AmgOptions.packed())."""
+        if _simplec_lib().oracle_simplec_set_pressure_options(self.handle, C.cast(packed, C.c_void_p),
+                                                              float(threshold)) != 0:
+            raise OracleError(lib().oracle_simplec_last_error().decode())
 
     def iterate(self, count=1):
         if lib().oracle_simplec_iterate(self.handle, count) != 0:

@@ -487,6 +487,13 @@ class Steady {
   void sst_update(const double* options, const float* const* in);
   Index reference() const { return reference_; }
   int cells() const { return mesh.n_cells; }
+  // Rebuilds the pressure AMG with other options from the operator it was
+  // first built from (no reference counterpart; e.g. the K-cycle).
+  void set_pressure_options(const AmgOptions& options) {
+    ORACLE_REQUIRE(options.structure == Structure::SymmetricMMatrix && options.nullspace == NullSpace::None,
+                   "SIMPLEC: the pressure operator is a pinned symmetric M-matrix");
+    pressure_solver = std::make_unique<Solver>(SolverKind::AmgPcg, pressure_initial, options);
+  }
   const Geometry& geometry() const { return geom; }
 
  private:
@@ -532,6 +539,7 @@ class Steady {
   LeastSquares velocity_lsq, pressure_lsq;
   CsrDevice matrix;
   std::unique_ptr<Solver> pressure_solver;
+  CsrHost pressure_initial;
   TransportEquation momentum[3];
   V face_correction_scratch, zero;
   Index reference_ = -1;
@@ -873,6 +881,7 @@ Steady::Steady(PolyMesh input, const std::vector<Condition>& conditions, const S
   amg.structure = Structure::SymmetricMMatrix;
   if (sst) amg.pre_sweeps = amg.post_sweeps = 2;  // :573-578
   pressure_solver = std::make_unique<Solver>(SolverKind::AmgPcg, initial, amg);
+  pressure_initial = std::move(initial);
   const double volume = std::accumulate(geom.cell_volume.begin(), geom.cell_volume.end(), 0.);
   Vec3 lo = host.points.front(), hi = lo;
   for (auto p : host.points) {
@@ -1781,6 +1790,31 @@ int oracle_simplec_initialize(void* h, const double* u, const double* v, const d
   }
 }
 void oracle_simplec_free(void* h) { delete static_cast<oracle::simplec::Steady*>(h); }
+// options[11] and the threshold as oracle_solver_create.
+int oracle_simplec_set_pressure_options(void* h, const int* options, double threshold) {
+  using oracle::AmgCycle;
+  using oracle::AmgOptions;
+  using oracle::NullSpace;
+  using oracle::Structure;
+  try {
+    AmgOptions o;
+    o.pre_sweeps = options[0]; o.post_sweeps = options[1]; o.coarse_sweeps = options[2];
+    o.coarse_rows = options[3]; o.max_levels = options[4]; o.aggregation_passes = options[5];
+    o.nullspace = options[6] ? NullSpace::Constant : NullSpace::None;
+    o.structure = options[7] ? Structure::SymmetricMMatrix : Structure::General;
+    ORACLE_REQUIRE((options[8] == 0 || options[8] == 1) && options[9] >= 0 && options[10] >= 0,
+                   "AMG: invalid options");
+    o.cycle = options[8] ? AmgCycle::K : AmgCycle::V;
+    o.k_depth = options[9];
+    o.k_min_rows = options[10];
+    o.strength_threshold = threshold;
+    static_cast<oracle::simplec::Steady*>(h)->set_pressure_options(o);
+    return 0;
+  } catch (const std::exception& e) {
+    g_simplec_error = e.what();
+    return 1;
+  }
+}
 int oracle_simplec_iterate(void* h, int count) {
   try {
     static_cast<oracle::simplec::Steady*>(h)->iterate(count);

@@ -210,3 +210,37 @@ def test_steady_cavity_walls_balance_the_lid():
     assert abs(fx) < 3e-3 * abs(fx_lid)
     assert total["area_vector"][:2] == pytest.approx([0.0, 0.0], abs=1e-15)
     assert lid["area_vector"] == pytest.approx([0.0, 1.0, 0.0], rel=1e-12)  # the unit lid
+
+
+def test_k_cycle_pressure_reaches_the_same_steady_cavity():
+    """The pressure AMG rebuilt as the K-cycle (no reference counterpart)
+    changes each outer iteration's pressure solve but not the fixed point:
+    both solvers reach the same steady 24x24 Re=100 cavity."""
+    states, early = [], []
+    for k_cycle in (False, True):
+        s = cavity(24, 24, 100, relaxation=0.7)
+        if k_cycle:
+            packed = (C.c_int32 * 11)(1, 1, 4, 24, 6, 3, 0, 1, 1, 0, 0)  # V(1,1), 4 coarsest sweeps, 6 levels
+            s.set_pressure_options(packed, 0.25)
+        s.iterate(10)
+        early.append(s.field("p"))
+
+        def report():
+            r = s.report()
+            return {"u_scaled": r[2], "v_scaled": r[6], "corrected_mass_imbalance": r[17]}
+
+        r, u, v, steady = run_to_steady(s, s.iterate, report, lambda: (s.field("u"), s.field("v")), 1e-6, 3000)
+        assert steady
+        s.iterate(200)
+        u, v = s.field("u"), s.field("v")
+        states.append((u, v, s.field("p")))
+    assert not np.array_equal(early[0], early[1])  # the rebuilt hierarchy is the one in use
+    for a, b in zip(*states):
+        assert np.allclose(a, b, rtol=0, atol=2e-5 * np.abs(b).max())
+
+
+def test_pressure_options_keep_the_m_matrix_structure():
+    s = cavity(8, 8, 100)
+    with pytest.raises(oracle.OracleError, match="M-matrix"):
+        s.set_pressure_options((C.c_int32 * 11)(1, 1, 4, 24, 6, 3, 0, 0, 1, 0, 0), 0.25)
+    s.iterate(2)
