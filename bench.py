@@ -739,6 +739,8 @@ def main() -> int:
                     help="spanwise cells of the SST step in the default line (0 skips it)")
     ap.add_argument("--simplec-step", type=int, default=0, metavar="NZ",
                     help="only time SST SIMPLEC on the backward-facing step with NZ spanwise cells (80: 4.06M)")
+    ap.add_argument("--no-step-kcycle", action="store_true",
+                    help="skip the SST step's second timing with the K-cycle pressure AMG")
     ap.add_argument("--simplec-only", action="store_true",
                     help="only time SIMPLEC on the 2-D cavity (--simplec-size, --simplec-iterations)")
     args = ap.parse_args()
@@ -752,7 +754,7 @@ def main() -> int:
         import kutta_amd
         dev = kutta_amd.Device(0)
         line = simplec_step_bench(dev, args.simplec_step, max(args.warmup, 1), max(args.steps, 1), 2,
-                                  0 if args.no_cpu_baseline else 2, not args.no_graphs)
+                                  0 if args.no_cpu_baseline else 2, not args.no_graphs, not args.no_step_kcycle)
         line = {"metric": "ms/SIMPLEC iteration", "value": line["ms_per_iteration"], "unit": "ms",
                 "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic", "config": line}
         print(json.dumps(line), flush=True)
@@ -959,7 +961,7 @@ def main() -> int:
         # 80 spanwise cells), 3 untimed iterations from the uniform inflow
         # state, then 10 timed.
         simplec_sst = simplec_step_bench(dev, args.step_nz, 3, 10, 2, 0 if args.no_cpu_baseline else 2,
-                                         not args.no_graphs)
+                                         not args.no_graphs, not args.no_step_kcycle)
 
     large = None
     if args.large_size:

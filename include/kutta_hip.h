@@ -370,10 +370,22 @@ typedef enum kh_kernel {
      entry order (no reference counterpart; same arguments, same bits). For
      levels with long rows. */
   KH_AMG_SMOOTH_ROWWAVE,
+  /* KH_FUSED_COARSE_SWEEPS with the operator read once per launch (no
+     reference counterpart; same bits): each sweep forms every entry's term
+     and product error in parallel (`cap` entries per thread held in
+     registers: 8 or 16), then each thread sums its rows in entry order
+     (table: first row and end row per thread, built by
+     kh_resident_partition). rows,cols,values,b,inverse,x,status,sweeps,
+     table,cap (n = rows, at most KH_RESIDENT_MAX_ROWS; at most 16 x
+     KH_RESIDENT_THREADS entries). */
+  KH_COARSE_SWEEPS_RESIDENT,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 #define KH_COARSE_SWEEPS_MAX_ROWS 16384
 #define KH_WIDE_ROW 16
+#define KH_RESIDENT_THREADS 512
+#define KH_RESIDENT_MAX_ROWS 4096
+#define KH_RESIDENT_TABLE_INTS (2 * KH_RESIDENT_THREADS)
 
 /* One system of the batched BiCGStab-Jacobi launches: its operator, its
  * vectors (x, r, q, p, z, s, t, zs, shadow; n floats each), its Jacobi
@@ -503,6 +515,14 @@ int kh_logical_bytes(kh_device* dev, int enable, double* bytes, long* launches);
  * whole batch (kernel execution plus the gaps between launches). */
 int kh_time_launches(kh_device* dev, uint32_t kernel, size_t n, const kh_arg* args, size_t nargs,
                      int repeats, double* total_ms);
+/* The row partition KH_COARSE_SWEEPS_RESIDENT reads (no reference
+ * counterpart; host only, no device call): from a CSR pattern of `rows` rows
+ * (row_ptr[0] = 0, every row non-empty, rows <= KH_RESIDENT_MAX_ROWS, at most
+ * 16 x KH_RESIDENT_THREADS entries; cols is not read), fills table
+ * (KH_RESIDENT_TABLE_INTS ints) and *cap (8 or 16). Returns 0, or 1 when the
+ * level does not fit (then nothing is written). */
+int kh_resident_partition(const int32_t* row_ptr, const int32_t* cols, int32_t rows, int32_t* table,
+                          int32_t* cap);
 
 #ifdef __cplusplus
 }

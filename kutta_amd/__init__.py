@@ -76,6 +76,7 @@ def hip_lib() -> C.CDLL:
         lib.kh_time_launches.argtypes = [vp, C.c_uint32, sz, C.POINTER(KhArg), sz, C.c_int,
                                          C.POINTER(C.c_double)]
         lib.kh_logical_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        lib.kh_resident_partition.argtypes = [vp, vp, C.c_int32, vp, C.POINTER(C.c_int32)]
         _hip = lib
     return _hip
 
@@ -395,6 +396,7 @@ class Kernel(enum.IntEnum):
     inner_beta_step = 253
     inner_second_step = 254
     amg_smooth_rowwave = 255
+    coarse_sweeps_resident = 256
 
 
 class Buffer:
@@ -830,10 +832,12 @@ class LinearSolver:
         host_lib().ka_solver_describe(self.handle, buf, 512)
         return buf.value.decode()
 
-    def set_plan(self, restrict_mode: int = -1, coarse_fused_sweeps: int = -1, prolong_mode: int = -1,
+    def set_plan(self, restrict_mode: int = -1, coarse_sweeps: int = -1, prolong_mode: int = -1,
                  sweep_mode: int = -1) -> None:
-        """Forces the V-cycle forms (see plan()); -1 keeps a setting."""
-        _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_fused_sweeps),
+        """Forces the V-cycle forms (see plan()); -1 keeps a setting.
+        coarse_sweeps: 0 a launch per coarsest-level sweep, 1 one workgroup,
+        2 one workgroup with the operator in registers."""
+        _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_sweeps),
                                                   int(prolong_mode), int(sweep_mode)))
 
     def plan(self) -> str:
@@ -904,6 +908,27 @@ def _named_array(fn, handle, name: str) -> np.ndarray:
 C2_SEED = 20261202
 # Largest coarsest level smoothed in one launch (KH_COARSE_SWEEPS_MAX_ROWS).
 COARSE_SWEEPS_MAX_ROWS = 16384
+
+# Limits of the register-resident coarsest-level sweeps (KH_RESIDENT_* in
+# kutta_hip.h).
+RESIDENT_THREADS = 512
+RESIDENT_MAX_ROWS = 4096
+
+
+def resident_partition(row_ptr, cols=None):
+    """kh_resident_partition: the row partition of KH_COARSE_SWEEPS_RESIDENT
+    for a CSR pattern, as (table int32[2 * RESIDENT_THREADS]: first row and
+    end row per thread, cap: entries per thread of the parallel phase), or
+    None when the level does not fit. Host only."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    cl = np.ascontiguousarray(cols if cols is not None else np.zeros(1), np.int32)
+    table = np.zeros(2 * RESIDENT_THREADS, np.int32)
+    cap = C.c_int32(0)
+    if hip_lib().kh_resident_partition(rp.ctypes.data, cl.ctypes.data, len(rp) - 1, table.ctypes.data,
+                                       C.byref(cap)) != 0:
+        return None
+    return table, cap.value
+
 
 # Limits of the one-workgroup coarse-grid tail (KH_TAIL_* in kutta_hip.h).
 TAIL_MAX_ROWS = 8192

@@ -178,9 +178,9 @@ int ka_solver_describe(void* s, char* out, int capacity) {
   }
   return int(text.size());
 }
-int ka_solver_set_plan(void* s, int restrict_mode, int coarse_fused_sweeps, int prolong_mode, int sweep_mode) {
+int ka_solver_set_plan(void* s, int restrict_mode, int coarse_sweeps, int prolong_mode, int sweep_mode) {
   return guard([&] {
-    static_cast<Solver*>(s)->solver.set_plan(restrict_mode, coarse_fused_sweeps, prolong_mode, sweep_mode);
+    static_cast<Solver*>(s)->solver.set_plan(restrict_mode, coarse_sweeps, prolong_mode, sweep_mode);
   });
 }
 int ka_solver_plan(void* s, char* out, int capacity) {

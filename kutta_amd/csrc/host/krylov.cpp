@@ -63,6 +63,10 @@ struct Level {
   // with rows longer than KH_WIDE_ROW): 0 one thread per row, 1 one
   // wavefront per row.
   int sweep_mode = 0;
+  // The coarsest level's row partition for KH_COARSE_SWEEPS_RESIDENT and its
+  // register capacity (0: the level does not fit).
+  DeviceArray<Index> resident;
+  int resident_cap = 0;
   // K-cycle workspaces of a coarse level (linalg/amg.cpp:185-193): the inner
   // corrections' vectors, 12 pair scalars and 1-int status.
   DeviceArray<float> kp, kq, kz, kt, kscalars;
@@ -463,6 +467,15 @@ struct KrylovSolver::Impl {
       current = &owned;
     }
     complexity = a.nnz() ? double(total) / a.nnz() : 0;
+    Level& last = levels.back();
+    if (levels.size() > 1 && last.rows <= KH_RESIDENT_MAX_ROWS) {
+      std::vector<Index> table(KH_RESIDENT_TABLE_INTS);
+      int32_t cap = 0;
+      if (kh_resident_partition(last.host.row_ptr.data(), last.host.col.data(), last.rows, table.data(), &cap) == 0) {
+        last.resident = to_device(dev, table);
+        last.resident_cap = cap;
+      }
+    }
   }
 
   // Whether levels [first, end) fit one KH_FUSED_COARSE_TAIL workgroup: all
@@ -630,7 +643,11 @@ struct KrylovSolver::Impl {
     dev.launch(mode == 1 ? KH_FUSED_RESIDUAL_RESTRICT_WIDE : KH_FUSED_RESIDUAL_RESTRICT, ca.n_rows,
                {a.row_ptr, a.col, a.val, current, b, l.member_offsets, l.members, coarse_b, st});
   }
-  bool coarse_fused_sweeps = false;
+  // The coarsest level's sweeps (bitwise equal, picked by the autotuner): 0 a
+  // launch per sweep, 1 all in one workgroup reading the operator from
+  // global memory, 2 all in one workgroup with the operator in registers
+  // (when the level has a resident partition).
+  int coarse_sweeps_mode = 0;
   bool coarse_sweeps_fit(std::size_t index) const {
     const Level& l = levels[index];
     return index > 0 && index + 1 == levels.size() && fused && !nullspace && !bf16_level(l) && l.rows > 24 &&
@@ -670,7 +687,12 @@ struct KrylovSolver::Impl {
     // A coarsest level too large for the coarse tail (aggregation stalls on
     // anisotropic meshes) may run all its sweeps in one workgroup, bitwise the
     // sweep-by-sweep sequence below, when the autotuner finds that faster.
-    if (coarse_fused_sweeps && coarse_sweeps_fit(index)) {
+    if (coarse_sweeps_mode == 2 && l.resident_cap > 0 && coarse_sweeps_fit(index)) {
+      dev.launch(KH_COARSE_SWEEPS_RESIDENT, a.n_rows,
+                 {a.row_ptr, a.col, a.val, b, l.l1, x, st, options.coarse_sweeps, l.resident, l.resident_cap});
+      return result;
+    }
+    if (coarse_sweeps_mode != 0 && coarse_sweeps_fit(index)) {
       dev.launch(KH_FUSED_COARSE_SWEEPS, a.n_rows,
                  {a.row_ptr, a.col, a.val, b, l.l1, x, st, options.coarse_sweeps});
       return result;
@@ -1389,16 +1411,22 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
       else l.prolong_mode = t < fused_form ? 1 : 0;
     }
   }
-  // The coarsest level's sweeps: launch by launch or in one workgroup.
+  // The coarsest level's sweeps: launch by launch, in one workgroup, or in
+  // one workgroup with the operator in registers.
   if (I.coarse_sweeps_fit(I.levels.size() - 1)) {
-    double by_launch = 0;
-    for (int fused_sweeps = 0; fused_sweeps < 2; ++fused_sweeps) {
-      I.coarse_fused_sweeps = fused_sweeps != 0;
+    const int forms = I.levels.back().resident_cap > 0 ? 3 : 2;
+    double fastest = 0;
+    int pick = 0;
+    for (int mode = 0; mode < forms; ++mode) {
+      I.coarse_sweeps_mode = mode;
       I.reset(0);
       const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
-      if (fused_sweeps == 0) by_launch = t;
-      else I.coarse_fused_sweeps = t < by_launch;
+      if (mode == 0 || t < fastest) {
+        fastest = t;
+        pick = mode;
+      }
     }
+    I.coarse_sweeps_mode = pick;
   }
   const int previous = requested;
   int best = previous;
@@ -1497,10 +1525,10 @@ std::string KrylovSolver::describe() const {
         << impl_->levels[impl_->learned_level].rows << " rows)";
   return out.str();
 }
-void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps, int prolong_mode, int sweep_mode) {
-  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_fused_sweeps >= -1 && coarse_fused_sweeps <= 1 &&
+void KrylovSolver::set_plan(int restrict_mode, int coarse_sweeps, int prolong_mode, int sweep_mode) {
+  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_sweeps >= -1 && coarse_sweeps <= 2 &&
                  prolong_mode >= -1 && prolong_mode <= 1 && sweep_mode >= -1 && sweep_mode <= 1,
-             "solver plan: restriction form -1..2, coarsest sweeps -1..1, prolongation and sweep forms -1..1");
+             "solver plan: restriction form -1..2, coarsest sweeps -1..2, prolongation and sweep forms -1..1");
   impl_->drop_graphs();
   if (restrict_mode >= 0)
     for (auto& l : impl_->levels) l.restrict_mode = restrict_mode;
@@ -1508,7 +1536,9 @@ void KrylovSolver::set_plan(int restrict_mode, int coarse_fused_sweeps, int prol
     for (auto& l : impl_->levels) l.prolong_mode = prolong_mode;
   if (sweep_mode >= 0)
     for (auto& l : impl_->levels) l.sweep_mode = sweep_mode;
-  if (coarse_fused_sweeps >= 0) impl_->coarse_fused_sweeps = coarse_fused_sweeps != 0;
+  // Form 2 needs the level's resident partition; without one the level
+  // runs form 1, the same bits.
+  if (coarse_sweeps >= 0) impl_->coarse_sweeps_mode = coarse_sweeps;
 }
 std::string KrylovSolver::plan() const {
   std::ostringstream out;
@@ -1519,7 +1549,10 @@ std::string KrylovSolver::plan() const {
   out << "; prolongation per level";
   for (std::size_t i = 0; i + 1 < L.size(); ++i) out << (i ? "," : " ") << (L[i].prolong_mode ? "split" : "fused");
   if (!L.empty() && impl_->coarse_sweeps_fit(L.size() - 1))
-    out << "; coarsest level " << (impl_->coarse_fused_sweeps ? "one-workgroup sweeps" : "sweep launches");
+    out << "; coarsest level "
+        << (impl_->coarse_sweeps_mode == 0                                      ? "sweep launches"
+            : impl_->coarse_sweeps_mode == 2 && L.back().resident_cap > 0 ? "register-resident sweeps"
+                                                                            : "one-workgroup sweeps");
   out << "; tail from level " << impl_->tail_first;
   out << "; longest row per level";
   for (std::size_t i = 0; i < L.size(); ++i) out << (i ? "," : " ") << L[i].max_row;

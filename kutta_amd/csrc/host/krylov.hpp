@@ -72,8 +72,9 @@ class KrylovSolver {
   // sweeps, tail level); no reference counterpart.
   std::string plan() const;
   // Sets every level's restriction form (0 fused, 1 wide, 2 split; -1
-  // keeps) and the coarsest level's one-workgroup sweeps (0/1; -1 keeps).
-  void set_plan(int restrict_mode, int coarse_fused_sweeps, int prolong_mode = -1, int sweep_mode = -1);
+  // keeps) and the coarsest level's sweeps (0 a launch each, 1 one
+  // workgroup, 2 one workgroup with the operator in registers; -1 keeps).
+  void set_plan(int restrict_mode, int coarse_sweeps, int prolong_mode = -1, int sweep_mode = -1);
   std::vector<std::pair<Index, Index>> levels() const;  // (rows, nnz) per level
   // Host copies for parity checks: transfer arrays and coarse values.
   std::vector<Index> transfer(std::size_t level, int which) const;

@@ -8,6 +8,10 @@
 // reference wherever no FP64 reduction order is involved.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "kh_device_math.hpp"
 
 namespace kh {
@@ -1081,6 +1085,95 @@ __global__ void __launch_bounds__(kCoarseSweepThreads) k_fused_coarse_sweeps(int
   for (int i = threadIdx.x; i < n; i += kCoarseSweepThreads) x[i] = iterate[cur][i];
 }
 
+// The same sweeps with the operator read once per launch and each sweep in
+// two phases. Phase 1 is entry-parallel: thread t holds entries t,
+// t + threads, ... (Cap of them) in registers and forms each entry's term
+// -(a x) and product error into LDS. Phase 2 is row-serial: each thread owns consecutive rows
+// (table: first row, end row; rows longer than 16 entries alone, in the
+// first wavefronts) and runs the compensated sum of each row over the
+// precomputed terms, in entry order, then updates the row in place (no other
+// thread reads x in phase 2). The arithmetic is RowResidual::add split in
+// two, so the iterates are k_fused_coarse_sweeps' bits; a long row's serial
+// part is three additions per entry instead of the whole product.
+constexpr int kResidentThreads = KH_RESIDENT_THREADS;
+template <int Cap>
+__global__ void __launch_bounds__(kResidentThreads) k_coarse_sweeps_resident(int n,
+    const int* __restrict__ rows, const int* __restrict__ cols, const float* __restrict__ values,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ x,
+    const int* __restrict__ status, int sweeps, const int* __restrict__ table) {
+  static_assert(Cap == 8 || Cap == 16, "Cap is 8 or 16");
+  if (status[0] != 0) return;
+  __shared__ float iterate[KH_RESIDENT_MAX_ROWS];
+  __shared__ float2 bw[KH_RESIDENT_MAX_ROWS];  // b and (2/3) l1 inverse, as the sweeps form it
+  __shared__ int row_start[KH_RESIDENT_MAX_ROWS + 1];
+  __shared__ float term[KH_RESIDENT_THREADS * 16];
+  __shared__ float error[KH_RESIDENT_THREADS * 16];
+  const int t = threadIdx.x;
+  const int nnz = rows[n];
+  if (nnz > kResidentThreads * Cap) return;  // not this partition's operator (uniform, before any barrier)
+  for (int i = t; i < n; i += kResidentThreads) {
+    bw[i] = make_float2(b[i], (2.0f / 3.0f) * inverse[i]);
+    iterate[i] = 0.0f;
+  }
+  for (int i = t; i <= n; i += kResidentThreads) row_start[i] = rows[i];
+  // Entry t + k*threads is this thread's k-th: neighbouring lanes touch
+  // neighbouring words of the term and error arrays (no LDS bank conflicts).
+  float val[Cap];
+  int col[Cap];
+#pragma unroll
+  for (int k = 0; k < Cap; ++k) {
+    const int j = t + k * kResidentThreads;
+    val[k] = j < nnz ? values[j] : 0.0f;
+    col[k] = j < nnz ? cols[j] : 0;
+  }
+  const int r0 = table[2 * t], r1 = table[2 * t + 1];
+  for (int s = 0; s < sweeps; ++s) {
+    __syncthreads();
+    // Phase 1: terms and product errors of this thread's entries.
+#pragma unroll
+    for (int c = 0; c < Cap; c += 8) {
+      float xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xv[u] = iterate[col[c + u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = t + (c + u) * kResidentThreads;
+        if (j < nnz) {
+          const float prod = val[c + u] * xv[u];
+          term[j] = -prod;
+          error[j] = product_error(val[c + u], xv[u], prod);
+        }
+      }
+    }
+    __syncthreads();
+    // Phase 2: each row's compensated sum in entry order, then its update.
+    for (int r = r0; r < r1; ++r) {
+      const float2 q = bw[r];
+      if (s == 0 && q.x - q.x == 0.0f) {
+        iterate[r] = 0.0f + q.y * (q.x + 0.0f);
+        continue;
+      }
+      RowResidual acc(q.x);
+      const int e1 = row_start[r + 1];
+      for (int j = row_start[r]; j < e1; j += 8) {
+        float tv[8], ev[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (j + u < e1) {
+            tv[u] = term[j + u];
+            ev[u] = error[j + u];
+          }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (j + u < e1) acc.add_split(tv[u], ev[u]);
+      }
+      iterate[r] = iterate[r] + q.y * acc.value();
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < n; i += kResidentThreads) x[i] = iterate[i];
+}
+
 // FP32 -> bfloat16, round to nearest even (NaN stays NaN): the mixed path's
 // coarse-level value copies.
 __global__ void __launch_bounds__(kBlock) k_to_bf16(int n, const float* __restrict__ src, bf16v* __restrict__ dst) {
@@ -1550,6 +1643,24 @@ void l_fused_coarse_sweeps(kh_device* d, int n, const kh_arg* a) {
   k_fused_coarse_sweeps<<<dim3(1), dim3(kCoarseSweepThreads), 0, d->stream>>>(
       n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5), KB(6), i32(a, 7));
 }
+void l_coarse_sweeps_resident(kh_device* d, int n, const kh_arg* a) {
+  if (n < 1 || n > KH_RESIDENT_MAX_ROWS)
+    throw std::runtime_error("coarse_sweeps_resident: rows outside 1..KH_RESIDENT_MAX_ROWS");
+  if (i32(a, 7) < 1) throw std::runtime_error("coarse_sweeps_resident: at least one sweep");
+  const dim3 grid(1), block(kResidentThreads);
+  switch (i32(a, 9)) {
+    case 8:
+      k_coarse_sweeps_resident<8><<<grid, block, 0, d->stream>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5),
+                                                                  KB(6), i32(a, 7), KB(8));
+      break;
+    case 16:
+      k_coarse_sweeps_resident<16><<<grid, block, 0, d->stream>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KF(5),
+                                                                   KB(6), i32(a, 7), KB(8));
+      break;
+    default:
+      throw std::runtime_error("coarse_sweeps_resident: cap must be 8 or 16");
+  }
+}
 void l_fused_prolong_smooth(kh_device* d, int n, const kh_arg* a) {
   k_fused_prolong_smooth<float><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6),
                                              KB(7), KCF(8), KB(9));
@@ -1772,6 +1883,7 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_AMG_INNER_STEP, "amg_inner_step", "bbbb", l_amg_inner_step},
       {KH_AMG_SMOOTH_WIDE, "amg_smooth_wide", "bbbbbbbb", l_amg_smooth_wide},
       {KH_AMG_SMOOTH_ROWWAVE, "amg_smooth_rowwave", "bbbbbbbb", l_amg_smooth_rowwave},
+      {KH_COARSE_SWEEPS_RESIDENT, "coarse_sweeps_resident", "bbbbbbbibi", l_coarse_sweeps_resident},
       {KH_INNER_FIRST_STEP, "inner_first_step", "bbbbbbbbbbi", l_inner_first_step},
       {KH_INNER_FIRST_UPDATE, "inner_first_update", "bbbbbbb", l_inner_first_update},
       {KH_INNER_BETA_STEP, "inner_beta_step", "bbbbbbbbi", l_inner_beta_step},
@@ -1781,3 +1893,63 @@ const std::vector<KernelEntry>& linalg_kernels() {
 }
 
 }  // namespace kh
+
+// Row partition of KH_COARSE_SWEEPS_RESIDENT: the entries per thread of
+// phase 1 (8, or 16 above 8 x KH_RESIDENT_THREADS entries) and the rows of
+// each thread in phase 2. Rows longer than 16 entries take a thread each,
+// the first threads, in row order; the other rows are cut greedily, in order,
+// into chunks of at most `target` entries (a longer row alone; the smallest
+// target from their entries per remaining thread up that needs no more
+// threads), starting at the next wavefront.
+extern "C" int kh_resident_partition(const int32_t* row_ptr, const int32_t* cols, int32_t rows, int32_t* table,
+                                     int32_t* cap_out) {
+  constexpr int T = KH_RESIDENT_THREADS;
+  constexpr int kLong = 16;
+  (void)cols;
+  if (!row_ptr || !table || !cap_out || rows < 1 || rows > KH_RESIDENT_MAX_ROWS || row_ptr[0] != 0) return 1;
+  for (int32_t r = 0; r < rows; ++r)
+    if (row_ptr[r + 1] <= row_ptr[r]) return 1;
+  const long nnz = row_ptr[rows];
+  if (nnz > 16L * T) return 1;
+  struct Chunk {
+    int32_t first, last;
+  };
+  std::vector<Chunk> chunks;
+  long short_entries = 0;
+  for (int32_t r = 0; r < rows; ++r) {
+    const int32_t len = row_ptr[r + 1] - row_ptr[r];
+    if (len > kLong) chunks.push_back({r, r + 1});
+    else short_entries += len;
+  }
+  const std::size_t lead = (chunks.size() + 63) / 64 * 64;
+  if (lead > std::size_t(T) || (lead == std::size_t(T) && short_entries > 0)) return 1;
+  const long room = T - long(lead);
+  chunks.resize(lead, Chunk{rows, rows});
+  std::vector<Chunk> shorts;
+  bool fits = short_entries == 0;
+  for (long target = std::max(1L, (short_entries + room - 1) / std::max(1L, room)); !fits; ++target) {
+    shorts.clear();
+    long open = 0;
+    for (int32_t r = 0; r < rows; ++r) {
+      const int32_t len = row_ptr[r + 1] - row_ptr[r];
+      if (len > kLong) {
+        open = 0;
+      } else if (open == 0 || open + len > target) {
+        shorts.push_back({r, r + 1});
+        open = len;
+      } else {
+        shorts.back().last = r + 1;
+        open += len;
+      }
+    }
+    fits = long(shorts.size()) <= room;
+  }
+  chunks.insert(chunks.end(), shorts.begin(), shorts.end());
+  for (int t = 0; t < T; ++t) {
+    const Chunk c = t < int(chunks.size()) ? chunks[t] : Chunk{rows, rows};
+    table[2 * t] = c.first;
+    table[2 * t + 1] = c.last;
+  }
+  *cap_out = nnz <= 8L * T ? 8 : 16;
+  return 0;
+}

@@ -396,3 +396,69 @@ def test_long_row_sweep_forms(device, lengths, status0, status1):
         dn.fill(np.float32(3))
         device.launch(kernel, n, drows, dcols, dvals, dx, db, di, dn, status1)
         assert np.all(dn.download() == 3)
+
+
+RESIDENT_CASES = {  # rows, longest ordinary row, rows of 17-105 entries, non-finite b entries
+    "tiny": (25, 5, 0, 0),
+    "c2-coarsest": (1067, 5, 70, 0),
+    "long-rows": (2000, 3, 12, 2),
+    "most-rows": (4096, 1, 0, 0),
+    "cap16": (3000, 3, 3, 1),
+    "short-runs": (600, 12, 0, 1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RESIDENT_CASES))
+def test_register_resident_coarse_sweeps(device, case, status0, status1):
+    """The two-phase coarsest-level sweeps equal the reference's sequence
+    (amg_smooth_zero, then 31 amg_smooth sweeps) bit for bit, as the
+    one-workgroup global-memory form does, for both entry capacities, with
+    long rows and non-finite right-hand sides; a frozen status leaves x
+    untouched."""
+    import kutta_amd
+    n, top, long_rows, bad = RESIDENT_CASES[case]
+    rng = np.random.default_rng(sum(map(ord, case)))
+    lengths = rng.integers(1, top + 1, n)
+    if long_rows:
+        lengths[rng.choice(n, long_rows, replace=False)] = rng.integers(17, 106, long_rows)
+    # Rows with a non-finite right-hand side couple only to themselves, so
+    # the non-finite values stay in those rows.
+    isolated = rng.choice(n, bad, replace=False)
+    lengths[isolated] = 1
+    others = np.setdiff1d(np.arange(n), isolated)
+    rows = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    cols = np.concatenate([[i] if i in isolated else
+                           np.sort(np.concatenate([[i], rng.choice(others[others != i], size=k - 1, replace=False)]))
+                           for i, k in enumerate(lengths)]).astype(np.int32)
+    vals = (random_vector(int(rows[-1]), 31) * 0.1).astype(np.float32)
+    b = mixed_magnitude_vector(n, 32)
+    b[isolated] = [np.inf, np.nan][:bad]
+    inv = (np.abs(random_vector(n, 33)) * 0.5 + 0.1).astype(np.float32)
+    table, cap = kutta_amd.resident_partition(rows, cols)
+    print(f"[resident] {case}: {n} rows, {rows[-1]} entries, cap {cap}", flush=True)
+    assert np.all((table[0::2] <= table[1::2]) & (table[1::2] <= n)) and rows[-1] <= cap * len(table) // 2
+    sweeps = 32
+    cur = np.zeros(n, np.float32)
+    nxt = np.zeros(n, np.float32)
+    oracle.kernel("amg_smooth_zero", n, rows, cols, vals, np.zeros(n, np.float32), b, inv, cur, np.zeros(3, np.int32))
+    for _ in range(sweeps - 1):
+        oracle.kernel("amg_smooth", n, rows, cols, vals, cur, b, inv, nxt, np.zeros(3, np.int32))
+        cur, nxt = nxt, cur
+    drows, dcols, dvals = device.upload(rows), device.upload(cols), device.upload(vals)
+    db, di, dx, dt = device.upload(b), device.upload(inv), device.alloc(n), device.upload(table)
+    got = []
+    for kernel, extra in ((Kernel.coarse_sweeps_resident, (dt, cap)), (Kernel.fused_coarse_sweeps, ())):
+        dx.fill(np.float32(7))
+        device.launch(kernel, n, drows, dcols, dvals, db, di, dx, status0, sweeps, *extra)
+        got.append(dx.download())
+    # A NaN's sign and payload follow the processor and the operand order
+    # the compiler picks for a commutative operation, so for NaNs only their
+    # place must agree; every other value bit for bit.
+    nan = np.isnan(cur)
+    assert nan.sum() <= bad
+    for g in got:
+        assert np.array_equal(np.isnan(g), nan)
+        same(g[~nan], cur[~nan])
+    dx.fill(np.float32(3))
+    device.launch(Kernel.coarse_sweeps_resident, n, drows, dcols, dvals, db, di, dx, status1, sweeps, dt, cap)
+    assert np.all(dx.download() == 3)
