@@ -635,7 +635,8 @@ def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, 
 def trace_summary(db: str, line_json: str | None, top: int) -> int:
     """Markdown summary of a rocprofv3 --kernel-trace database (rocpd SQLite):
     busy time per kernel and grid size (one row per AMG level for the level
-    kernels), and for the l1-Jacobi smoother the algorithmic bytes 20n + 8m
+    kernels), and for the l1-Jacobi smoother (thread- or wavefront-per-row
+    form) the algorithmic bytes 20n + 8m
     of each level (rows n, nonzeros m from the bench line's "levels"),
     GB/s and the fraction of the 8 TB/s HBM peak. With an --fv-kernels line
     the finite-volume kernels get their algorithmic bytes from it."""
@@ -658,7 +659,7 @@ def trace_summary(db: str, line_json: str | None, top: int) -> int:
     levels, fv_bytes = [], {}
     if line_json:
         line = json.loads(Path(line_json).read_text())
-        levels = line["config"].get("levels", [])
+        levels = line["config"].get("levels") or line["config"].get("pressure_levels", [])
         # An --fv-kernels line: algorithmic bytes per kernel, keyed by the
         # functor's name (fv_paired_limiter -> FvPairedLimiter).
         for k in line.get("kernels", []):
@@ -669,12 +670,18 @@ def trace_summary(db: str, line_json: str | None, top: int) -> int:
     for (name, grid), (count, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
         avg_us = ns / count / 1e3
         extra = ["", "", "", ""]
-        if name == "k_amg_smooth":
-            for n_rows, nnz in levels:
-                if grid == -(-n_rows // 256) * 256:
-                    b = 20 * n_rows + 8 * nnz
-                    gbs = b / (avg_us * 1e-6) / 1e9
-                    extra = [str(n_rows), str(b), f"{gbs:.0f}", f"{gbs / HBM_PEAK_GBS:.3f}"]
+        if name in ("k_amg_smooth", "k_amg_smooth_rowwave"):
+            # thread per row: 256 rows per workgroup; wavefront per row: 4 rows
+            # per 256-thread workgroup. Levels sharing a grid size stay unnamed.
+            per_group = 256 if name == "k_amg_smooth" else 4
+            match = [(n_rows, nnz) for n_rows, nnz in levels if grid == -(-n_rows // per_group) * 256]
+            if len(match) == 1:
+                n_rows, nnz = match[0]
+                b = 20 * n_rows + 8 * nnz
+                gbs = b / (avg_us * 1e-6) / 1e9
+                extra = [str(n_rows), str(b), f"{gbs:.0f}", f"{gbs / HBM_PEAK_GBS:.3f}"]
+            elif match:
+                extra = ["/".join(str(r) for r, _ in match), "", "", ""]
         elif name in fv_bytes:
             b = fv_bytes[name]
             gbs = b / (avg_us * 1e-6) / 1e9

@@ -1343,11 +1343,20 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     if (I.tail_fits(i)) candidates.push_back(int(i));
   DeviceArray<float> v(I.dev, I.n), out(I.dev, I.n);
   I.dev.fill(v.data(), 1.0f, I.n);
+  // Where PCG iterations are replayed from graphs, each candidate is timed
+  // the same way (captured once, replayed), so launch-bound forms are priced
+  // as the solve will run them.
+  const bool replayed = I.graphs && I.kind == SolverKind::AmgPcg;
   const auto seconds = [&](const std::function<void()>& body) {
     body();
     I.dev.synchronize();
+    Graph graph;
+    if (replayed) graph = Graph(I.dev.capture(body));
     const auto t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < repeats; ++k) body();
+    for (int k = 0; k < repeats; ++k) {
+      if (replayed) I.dev.replay(graph.get());
+      else body();
+    }
     I.dev.synchronize();
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
