@@ -113,13 +113,14 @@ int ka_solver_describe(void* solver, char* out, int capacity);
  * No reference counterpart; all forms give the same iterates. */
 int ka_solver_plan(void* solver, char* out, int capacity);
 /* Forces the plan: every level's restriction form (0 one kernel, 1 the same
- * with 128 coarse rows per workgroup, 2 residual then member sums; -1 keeps),
- * the coarsest level's sweeps (0 a launch per sweep, 1 one workgroup, 2 one
- * workgroup with the operator in registers, where the level has a resident
- * partition, else as 1; -1 keeps), every level's
- * prolongation form (0 folded into the next sweep, 1 in place then a plain
- * sweep; -1 keeps) and the sweep form of levels with rows longer than
- * KH_WIDE_ROW (0 a thread per row, 1 a wavefront per row; -1 keeps). */
+ * with 128 coarse rows per workgroup, 2 residual then member sums, 3 the same
+ * with a wavefront per residual row; -1 keeps), the coarsest level's sweeps
+ * (0 a launch per sweep, 1 one workgroup, 2 one workgroup reading the
+ * operator once, where the level has a resident partition, else as 1; -1
+ * keeps), every level's prolongation form (0 folded into the next sweep, 1
+ * in place then a plain sweep, 2 folded, a wavefront per row; -1 keeps) and
+ * the sweep form of levels with rows longer than KH_WIDE_ROW (0 a thread per
+ * row, 1 a wavefront per row; -1 keeps). */
 int ka_solver_set_plan(void* solver, int restrict_mode, int coarse_sweeps, int prolong_mode,
                        int sweep_mode);
 int ka_solver_levels(void* solver, int32_t* rows, int32_t* nnz, int capacity);

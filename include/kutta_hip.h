@@ -379,6 +379,11 @@ typedef enum kh_kernel {
      table,cap (n = rows, at most KH_RESIDENT_MAX_ROWS; at most 16 x
      KH_RESIDENT_THREADS entries). */
   KH_COARSE_SWEEPS_RESIDENT,
+  /* KH_CSR_RESIDUAL_GUARDED and KH_FUSED_PROLONG_SMOOTH with one wavefront
+     per row, as KH_AMG_SMOOTH_ROWWAVE (no reference counterpart; same
+     arguments, same bits), for levels with long rows. */
+  KH_CSR_RESIDUAL_ROWWAVE,
+  KH_PROLONG_SMOOTH_ROWWAVE,
   KH_KERNEL_COUNT_MARK
 } kh_kernel;
 #define KH_COARSE_SWEEPS_MAX_ROWS 16384

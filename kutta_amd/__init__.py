@@ -397,6 +397,8 @@ class Kernel(enum.IntEnum):
     inner_second_step = 254
     amg_smooth_rowwave = 255
     coarse_sweeps_resident = 256
+    csr_residual_rowwave = 257
+    prolong_smooth_rowwave = 258
 
 
 class Buffer:
@@ -835,8 +837,11 @@ class LinearSolver:
     def set_plan(self, restrict_mode: int = -1, coarse_sweeps: int = -1, prolong_mode: int = -1,
                  sweep_mode: int = -1) -> None:
         """Forces the V-cycle forms (see plan()); -1 keeps a setting.
-        coarse_sweeps: 0 a launch per coarsest-level sweep, 1 one workgroup,
-        2 one workgroup with the operator in registers."""
+        restrict_mode: 0 fused, 1 wide, 2 split, 3 split with a wavefront per
+        residual row; coarse_sweeps: 0 a launch per coarsest-level sweep, 1
+        one workgroup, 2 one workgroup reading the operator once;
+        prolong_mode: 0 folded, 1 split, 2 folded with a wavefront per row;
+        sweep_mode: 0 thread per row, 1 wavefront per row."""
         _check_host(host_lib().ka_solver_set_plan(self.handle, int(restrict_mode), int(coarse_sweeps),
                                                   int(prolong_mode), int(sweep_mode)))
 

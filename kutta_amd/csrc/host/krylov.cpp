@@ -52,12 +52,14 @@ struct Level {
   // How the fused path restricts this level's residual (all bitwise equal,
   // picked by the autotuner): 0 one kernel, 64 coarse rows per workgroup;
   // 1 the same with 128; 2 a row-parallel residual then the member sums,
-  // which reads the fine rows coalesced when aggregates are scattered.
+  // which reads the fine rows coalesced when aggregates are scattered; 3 the
+  // same with the residual one wavefront per row (levels with long rows).
   int restrict_mode = 0;
   // How the fused path prolongs into this level (bitwise equal, picked by the
   // autotuner): 0 the prolongation folded into the first post-smoothing
   // sweep's gathers; 1 the in-place prolongation, then a plain sweep, which
-  // keeps the sweep's gathers to one dependent load per entry.
+  // keeps the sweep's gathers to one dependent load per entry; 2 the folded
+  // form one wavefront per row (levels with long rows).
   int prolong_mode = 0;
   // The plain sweep's form (bitwise equal, picked by the autotuner for levels
   // with rows longer than KH_WIDE_ROW): 0 one thread per row, 1 one
@@ -635,8 +637,9 @@ struct KrylovSolver::Impl {
   // fine residual (mode 2).
   void restrict_level(Level& l, const DeviceCsr& a, const DeviceCsr& ca, const float* current, const float* b,
                       float* spare, float* coarse_b, int mode, const int* st) {
-    if (mode == 2) {
-      dev.launch(KH_CSR_RESIDUAL_GUARDED, a.n_rows, {a.row_ptr, a.col, a.val, current, b, spare, st});
+    if (mode >= 2) {
+      dev.launch(mode == 3 ? KH_CSR_RESIDUAL_ROWWAVE : KH_CSR_RESIDUAL_GUARDED, a.n_rows,
+                 {a.row_ptr, a.col, a.val, current, b, spare, st});
       dev.launch(KH_AMG_RESTRICT_GUARDED, ca.n_rows, {l.member_offsets, l.members, spare, coarse_b, st});
       return;
     }
@@ -742,8 +745,8 @@ struct KrylovSolver::Impl {
           if (bf16_level(l)) {
             dev.launch(KH_BF16_PROLONG_SMOOTH, a.n_rows,
                        {a.row_ptr, a.col, l.val16, current, b, l.l1, out, l.aggregate, c.x, st});
-          } else if (fused && l.prolong_mode == 0) {
-            dev.launch(KH_FUSED_PROLONG_SMOOTH, a.n_rows,
+          } else if (fused && (l.prolong_mode == 0 || l.prolong_mode == 2)) {
+            dev.launch(l.prolong_mode == 2 ? KH_PROLONG_SMOOTH_ROWWAVE : KH_FUSED_PROLONG_SMOOTH, a.n_rows,
                        {a.row_ptr, a.col, a.val, current, b, l.l1, out, l.aggregate, c.x, st});
           } else {
             float* xin = const_cast<float*>(current);
@@ -1347,18 +1350,25 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
   // the same way (captured once, replayed), so launch-bound forms are priced
   // as the solve will run them.
   const bool replayed = I.graphs && I.kind == SolverKind::AmgPcg;
+  // The fastest of three timed batches of `repeats` runs, so that one
+  // disturbed batch does not decide a form.
   const auto seconds = [&](const std::function<void()>& body) {
     body();
     I.dev.synchronize();
     Graph graph;
     if (replayed) graph = Graph(I.dev.capture(body));
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < repeats; ++k) {
-      if (replayed) I.dev.replay(graph.get());
-      else body();
+    double best = 0;
+    for (int trial = 0; trial < 3; ++trial) {
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < repeats; ++k) {
+        if (replayed) I.dev.replay(graph.get());
+        else body();
+      }
+      I.dev.synchronize();
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      best = trial == 0 ? t : std::min(best, t);
     }
-    I.dev.synchronize();
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return best;
   };
   const int requested = I.tail_request;
   // Restriction form per level: time each form on the level's current data
@@ -1377,7 +1387,8 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     float* spare = l.r.data();
     DeviceArray<float> coarse(I.dev, ci.n_rows);
     double best_form = 0;
-    for (int mode = 0; mode < 3; ++mode) {
+    const int forms = l.max_row > KH_WIDE_ROW ? 4 : 3;
+    for (int mode = 0; mode < forms; ++mode) {
       const double t =
           seconds([&] { I.restrict_level(l, li, ci, xin, bin, spare, coarse.data(), mode, I.status.data()); });
       if (mode == 0 || t < best_form) {
@@ -1405,20 +1416,40 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
       else l.sweep_mode = t < thread_form ? 1 : 0;
     }
   }
-  // Prolongation form per level: time whole V-cycles with each level's two
-  // forms in turn (the in-place form changes the level's iterate, so it is
-  // not timed in isolation) and keep the faster. Every form gives the same bits.
+  // Prolongation form per level: time each form alone on the level's data
+  // (the in-place form works on a copy of the iterate, whose values drift
+  // over the repeats without changing the work) and keep the fastest. Every
+  // form gives the same bits.
   for (std::size_t i = 0; i + 1 < I.levels.size(); ++i) {
     Level& l = I.levels[i];
     if (I.bf16_level(l)) continue;
-    double fused_form = 0;
-    for (int mode = 0; mode < 2; ++mode) {
-      l.prolong_mode = mode;
-      I.reset(0);
-      const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
-      if (mode == 0) fused_form = t;
-      else l.prolong_mode = t < fused_form ? 1 : 0;
+    const DeviceCsr li = i == 0 ? a : l.matrix.view();
+    const float* xin = i == 0 ? out.data() : l.x.data();
+    const float* bin = i == 0 ? v.data() : l.b.data();
+    const float* correction = I.levels[i + 1].x.data();
+    DeviceArray<float> copy(I.dev, li.n_rows), result(I.dev, li.n_rows);
+    I.dev.copy(copy.data(), xin, sizeof(float) * li.n_rows);
+    double fastest = 0;
+    int pick = 0;
+    const int forms = l.max_row > KH_WIDE_ROW ? 3 : 2;
+    for (int mode = 0; mode < forms; ++mode) {
+      const double t = seconds([&] {
+        if (mode == 1) {
+          I.dev.launch(KH_AMG_PROLONG_GUARDED, li.n_rows, {l.aggregate, correction, copy, I.status.data()});
+          I.dev.launch(I.smooth_kernel(l), li.n_rows,
+                       {li.row_ptr, li.col, li.val, copy, bin, l.l1, result, I.status.data()});
+        } else {
+          I.dev.launch(mode == 2 ? KH_PROLONG_SMOOTH_ROWWAVE : KH_FUSED_PROLONG_SMOOTH, li.n_rows,
+                       {li.row_ptr, li.col, li.val, xin, bin, l.l1, result, l.aggregate, correction,
+                        I.status.data()});
+        }
+      });
+      if (mode == 0 || t < fastest) {
+        fastest = t;
+        pick = mode;
+      }
     }
+    l.prolong_mode = pick;
   }
   // The coarsest level's sweeps: launch by launch, in one workgroup, or in
   // one workgroup with the operator in registers.
@@ -1535,9 +1566,10 @@ std::string KrylovSolver::describe() const {
   return out.str();
 }
 void KrylovSolver::set_plan(int restrict_mode, int coarse_sweeps, int prolong_mode, int sweep_mode) {
-  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 2 && coarse_sweeps >= -1 && coarse_sweeps <= 2 &&
-                 prolong_mode >= -1 && prolong_mode <= 1 && sweep_mode >= -1 && sweep_mode <= 1,
-             "solver plan: restriction form -1..2, coarsest sweeps -1..2, prolongation and sweep forms -1..1");
+  KA_REQUIRE(restrict_mode >= -1 && restrict_mode <= 3 && coarse_sweeps >= -1 && coarse_sweeps <= 2 &&
+                 prolong_mode >= -1 && prolong_mode <= 2 && sweep_mode >= -1 && sweep_mode <= 1,
+             "solver plan: restriction form -1..3, coarsest sweeps -1..2, prolongation form -1..2, sweep form "
+             "-1..1");
   impl_->drop_graphs();
   if (restrict_mode >= 0)
     for (auto& l : impl_->levels) l.restrict_mode = restrict_mode;
@@ -1554,9 +1586,15 @@ std::string KrylovSolver::plan() const {
   const auto& L = impl_->levels;
   out << "restriction per level";
   for (std::size_t i = 0; i + 1 < L.size(); ++i)
-    out << (i ? "," : " ") << (L[i].restrict_mode == 2 ? "split" : (L[i].restrict_mode == 1 ? "wide" : "fused"));
+    out << (i ? "," : " ")
+        << (L[i].restrict_mode == 3   ? "split-wave"
+            : L[i].restrict_mode == 2 ? "split"
+            : L[i].restrict_mode == 1 ? "wide"
+                                      : "fused");
   out << "; prolongation per level";
-  for (std::size_t i = 0; i + 1 < L.size(); ++i) out << (i ? "," : " ") << (L[i].prolong_mode ? "split" : "fused");
+  for (std::size_t i = 0; i + 1 < L.size(); ++i)
+    out << (i ? "," : " ")
+        << (L[i].prolong_mode == 2 ? "fused-wave" : L[i].prolong_mode == 1 ? "split" : "fused");
   if (!L.empty() && impl_->coarse_sweeps_fit(L.size() - 1))
     out << "; coarsest level "
         << (impl_->coarse_sweeps_mode == 0                                      ? "sweep launches"

@@ -71,9 +71,11 @@ class KrylovSolver {
   // The V-cycle forms the autotuner picked (restriction per level, coarsest
   // sweeps, tail level); no reference counterpart.
   std::string plan() const;
-  // Sets every level's restriction form (0 fused, 1 wide, 2 split; -1
-  // keeps) and the coarsest level's sweeps (0 a launch each, 1 one
-  // workgroup, 2 one workgroup with the operator in registers; -1 keeps).
+  // Sets every level's restriction form (0 fused, 1 wide, 2 split, 3 split
+  // with a wavefront per residual row; -1 keeps), the coarsest level's sweeps
+  // (0 a launch each, 1 one workgroup, 2 one workgroup reading the operator
+  // once; -1 keeps), the prolongation form (0 folded, 1 split, 2 folded with
+  // a wavefront per row) and the long-row sweep form (0 thread, 1 wavefront).
   void set_plan(int restrict_mode, int coarse_sweeps, int prolong_mode = -1, int sweep_mode = -1);
   std::vector<std::pair<Index, Index>> levels() const;  // (rows, nnz) per level
   // Host copies for parity checks: transfer arrays and coarse values.

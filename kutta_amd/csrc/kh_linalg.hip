@@ -169,9 +169,15 @@ __global__ void __launch_bounds__(kBlock) k_amg_smooth(int n, const int* __restr
 // compensated sum over them in entry order, the arithmetic of RowResidual.
 constexpr int kRowWaveSlots = 4;                       // entries per lane per pass
 constexpr int kRowWaveEntries = 64 * kRowWaveSlots;    // entries per pass
-__global__ void __launch_bounds__(kBlock) k_amg_smooth_rowwave(int n, const int* __restrict__ rows,
-    const int* __restrict__ cols, const float* __restrict__ values, const float* __restrict__ x,
-    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ next,
+// What a wavefront-per-row launch writes for row i: the sweep's new value
+// gather(i) + (2/3) l1_i r_i (amg_smooth, or the prolongation folded into the
+// sweep when the gather adds the coarse correction), or the residual r_i
+// alone (csr_residual).
+enum class RowWaveOut { Smooth, Residual };
+template <RowWaveOut Out, class X>
+__global__ void __launch_bounds__(kBlock) k_rowwave(int n, const int* __restrict__ rows,
+    const int* __restrict__ cols, const float* __restrict__ values, X gather,
+    const float* __restrict__ b, const float* __restrict__ inverse, float* __restrict__ out,
     const int* __restrict__ status) {
   if (status[0] != 0) return;
   __shared__ float term[kBlock / 64][kRowWaveEntries];
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(kBlock) k_amg_smooth_rowwave(int n, const int*
     }
 #pragma unroll
     for (int k = 0; k < kRowWaveSlots; ++k)
-      if (base + k * 64 + lane < end) xv[k] = x[c[k]];
+      if (base + k * 64 + lane < end) xv[k] = gather(c[k]);
 #pragma unroll
     for (int k = 0; k < kRowWaveSlots; ++k) {
       if (base + k * 64 + lane < end) {
@@ -218,7 +224,10 @@ __global__ void __launch_bounds__(kBlock) k_amg_smooth_rowwave(int n, const int*
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
-  if (lane == 0) next[i] = x[i] + (2.0f / 3.0f) * inverse[i] * (s + e);
+  if (lane == 0) {
+    if constexpr (Out == RowWaveOut::Smooth) out[i] = gather(i) + (2.0f / 3.0f) * inverse[i] * (s + e);
+    else out[i] = s + e;
+  }
 }
 
 // kernels/linalg.kern.h:162-170
@@ -1512,10 +1521,20 @@ void l_scale_copy(kh_device* d, int n, const kh_arg* a) {
 }
 void l_mul(kh_device* d, int n, const kh_arg* a) { k_mul<<<GRID(n)>>>(n, KF(0), KCF(1), KCF(2)); }
 #define SINGLE dim3(1), dim3(64), 0, d->stream
+#define ROWWAVE_GRID dim3(unsigned((n + kBlock / 64 - 1) / (kBlock / 64))), dim3(kBlock), 0, d->stream
 void l_amg_smooth_rowwave(kh_device* d, int n, const kh_arg* a) {
-  const unsigned blocks = unsigned((n + kBlock / 64 - 1) / (kBlock / 64));
-  k_amg_smooth_rowwave<<<dim3(blocks), dim3(kBlock), 0, d->stream>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4),
-                                                                     KCF(5), KF(6), KB(7));
+  k_rowwave<RowWaveOut::Smooth><<<ROWWAVE_GRID>>>(n, KB(0), KB(1), KCF(2), Plain{KCF(3)}, KCF(4), KCF(5), KF(6),
+                                                  KB(7));
+}
+// rows, cols, values, x, b, r, status
+void l_csr_residual_rowwave(kh_device* d, int n, const kh_arg* a) {
+  k_rowwave<RowWaveOut::Residual><<<ROWWAVE_GRID>>>(n, KB(0), KB(1), KCF(2), Plain{KCF(3)}, KCF(4), nullptr,
+                                                    KF(5), KB(6));
+}
+// rows, cols, values, x, b, inverse, next, aggregate, coarse, status
+void l_prolong_smooth_rowwave(kh_device* d, int n, const kh_arg* a) {
+  k_rowwave<RowWaveOut::Smooth><<<ROWWAVE_GRID>>>(n, KB(0), KB(1), KCF(2), Prolonged{KCF(3), KB(7), KCF(8)},
+                                                  KCF(4), KCF(5), KF(6), KB(9));
 }
 void l_amg_smooth_wide(kh_device* d, int n, const kh_arg* a) {
   k_amg_smooth<float, true><<<GRID(n)>>>(n, KB(0), KB(1), KCF(2), KCF(3), KCF(4), KCF(5), KF(6), KB(7));
@@ -1884,6 +1903,8 @@ const std::vector<KernelEntry>& linalg_kernels() {
       {KH_AMG_SMOOTH_WIDE, "amg_smooth_wide", "bbbbbbbb", l_amg_smooth_wide},
       {KH_AMG_SMOOTH_ROWWAVE, "amg_smooth_rowwave", "bbbbbbbb", l_amg_smooth_rowwave},
       {KH_COARSE_SWEEPS_RESIDENT, "coarse_sweeps_resident", "bbbbbbbibi", l_coarse_sweeps_resident},
+      {KH_CSR_RESIDUAL_ROWWAVE, "csr_residual_rowwave", "bbbbbbb", l_csr_residual_rowwave},
+      {KH_PROLONG_SMOOTH_ROWWAVE, "prolong_smooth_rowwave", "bbbbbbbbbb", l_prolong_smooth_rowwave},
       {KH_INNER_FIRST_STEP, "inner_first_step", "bbbbbbbbbbi", l_inner_first_step},
       {KH_INNER_FIRST_UPDATE, "inner_first_update", "bbbbbbb", l_inner_first_update},
       {KH_INNER_BETA_STEP, "inner_beta_step", "bbbbbbbbi", l_inner_beta_step},

@@ -3,8 +3,10 @@
 large for the one-workgroup coarse tail is smoothed by one launch of all its
 sweeps (the operator read from global memory, or held in registers where the
 level has at most 4,096 rows and 8,192 entries), aggregates of about two members are restricted 128 coarse rows per
-workgroup, the prolongation runs folded into the next sweep or in place,
-and levels with long rows are swept one thread or one wavefront per row. All
+workgroup (or as a residual, one thread or one wavefront per row, then the
+member sums), the prolongation runs folded into the next sweep (one thread or
+one wavefront per row) or in place, and levels with long rows are swept one
+thread or one wavefront per row. All
 must leave the iterates bitwise equal to the oracle's
 reference sequence (linalg/amg.cpp:283-310)."""
 import numpy as np
@@ -36,8 +38,8 @@ def test_large_coarsest_level_and_pair_aggregates_bitwise(device, size, aniso, l
     solver.set_tail(0)
     solver.refresh(matrix)
     xb = device.upload(b)
-    plans = [(True, mode, sweeps, prolong, 0) for mode in (0, 1, 2) for sweeps in (0, 1, 2) for prolong in (0, 1)]
-    plans += [(True, 0, 0, prolong, 1) for prolong in (0, 1)] + [(False, 0, 0, 0, 0)]
+    plans = [(True, mode, sweeps, prolong, 0) for mode in (0, 1, 2, 3) for sweeps in (0, 1, 2) for prolong in (0, 1, 2)]
+    plans += [(True, 0, 0, prolong, 1) for prolong in (0, 1, 2)] + [(False, 0, 0, 0, 0)]
     for fused, mode, sweeps, prolong, sweep in plans:
         solver.set_fused(fused)
         solver.set_plan(mode, sweeps, prolong, sweep)

@@ -371,9 +371,11 @@ def test_fused_kernels_equal_reference_sequences(device, op, status0):
 @pytest.mark.parametrize("lengths", ["short", "long"])
 def test_long_row_sweep_forms(device, lengths, status0, status1):
     """The wide-chunk and wavefront-per-row sweeps equal the reference's
-    amg_smooth bit for bit, on rows of 1 to 600 entries (crossing the
-    wavefront form's 256-entry passes) with mixed-magnitude values; a frozen
-    status leaves the output untouched."""
+    amg_smooth bit for bit, and the wavefront-per-row residual and folded
+    prolongation sweep equal csr_residual and amg_prolong + amg_smooth, on
+    rows of 1 to 600 entries (crossing the wavefront form's 256-entry passes)
+    with mixed-magnitude values; a frozen status leaves the output
+    untouched."""
     rng = np.random.default_rng(21 if lengths == "short" else 22)
     n = 700
     top = 20 if lengths == "short" else 600
@@ -396,6 +398,32 @@ def test_long_row_sweep_forms(device, lengths, status0, status1):
         dn.fill(np.float32(3))
         device.launch(kernel, n, drows, dcols, dvals, dx, db, di, dn, status1)
         assert np.all(dn.download() == 3)
+    # The residual one wavefront per row (csr_residual).
+    want_r = np.zeros(n, np.float32)
+    oracle.kernel("csr_residual", n, rows, cols, vals, x, b, want_r)
+    dn.fill(np.float32(0))
+    device.launch(Kernel.csr_residual_rowwave, n, drows, dcols, dvals, dx, db, dn, status0)
+    same(dn.download(), want_r)
+    dn.fill(np.float32(3))
+    device.launch(Kernel.csr_residual_rowwave, n, drows, dcols, dvals, dx, db, dn, status1)
+    assert np.all(dn.download() == 3)
+    # The prolongation folded into a sweep one wavefront per row: amg_prolong
+    # (x + coarse[aggregate], unaggregated rows unchanged), then amg_smooth.
+    nc = 97
+    agg = rng.integers(-1, nc, n).astype(np.int32)
+    coarse = mixed_magnitude_vector(nc, 27)
+    xp = x.copy()
+    oracle.kernel("amg_prolong", n, agg, coarse, xp)
+    want_p = np.zeros(n, np.float32)
+    oracle.kernel("amg_smooth", n, rows, cols, vals, xp, b, inv, want_p, np.zeros(3, np.int32))
+    dagg, dc = device.upload(agg), device.upload(coarse)
+    for kernel in (Kernel.fused_prolong_smooth, Kernel.prolong_smooth_rowwave):
+        dn.fill(np.float32(0))
+        device.launch(kernel, n, drows, dcols, dvals, dx, db, di, dn, dagg, dc, status0)
+        same(dn.download(), want_p)
+    dn.fill(np.float32(3))
+    device.launch(Kernel.prolong_smooth_rowwave, n, drows, dcols, dvals, dx, db, di, dn, dagg, dc, status1)
+    assert np.all(dn.download() == 3)
 
 
 RESIDENT_CASES = {  # rows, longest ordinary row, rows of 17-105 entries, non-finite b entries
