@@ -379,6 +379,31 @@ __device__ __forceinline__ float pair_delta(F2 a, F2 b) {
 __device__ __forceinline__ float across(int other, CF cell, CF boundary) {
   return other < 0 ? boundary[-other - 1] : cell[other];
 }
+// The same value read through one selected address (one load, no branch).
+__device__ __forceinline__ float across_load(int other, CF cell, CF boundary) {
+  return *(other < 0 ? boundary + (-other - 1) : cell + other);
+}
+// A face-indexed value or, for a boundary neighbour o < 0, boundary slot -o-1.
+__device__ __forceinline__ float face_or_boundary(int other, int j, CF per_face, CF per_boundary) {
+  return *(other < 0 ? per_boundary + (-other - 1) : per_face + j);
+}
+// Runs use(j, v) for the faces j of [first, last) in order, where v = load(j)
+// holds the values a face reads. The loads of kFaceChunk faces are issued
+// before any is used (a chunk's slots past `last` load the last face again
+// and are not used), so a cell pays about one memory round trip per chunk
+// instead of one per face; the arithmetic is the face-by-face loop's.
+constexpr int kFaceChunk = 8;
+template <int K = 6, class Load, class Use>
+__device__ __forceinline__ void each_face(int first, int last, Load load, Use use) {
+  for (int j0 = first; j0 < last; j0 += K) {
+    decltype(load(first)) v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = load(min(j0 + k, last - 1));
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (j0 + k < last) use(j0 + k, v[k]);
+  }
+}
 // The symmetric 3x3 least-squares inverse of cell i: planes xx xy xz yy yz zz.
 struct Sym3 {
   float xx, xy, xz, yy, yz, zz;
@@ -394,14 +419,20 @@ struct FvGradient {
                              int n) const {
     const float own = cell[i];
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int o = adjacent[j];
-      const float d = across(o, cell, boundary) - own;
-      const float wx = o < 0 ? fx[-o - 1] : dx[j];
-      const float wy = o < 0 ? fy[-o - 1] : dy[j];
-      const float wz = o < 0 ? fz[-o - 1] : dz[j];
-      bx += wx * d; by += wy * d; bz += wz * d;
-    }
+    struct Face {
+      float v, wx, wy, wz;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int o = adjacent[j];
+          return Face{across_load(o, cell, boundary), face_or_boundary(o, j, dx, fx), face_or_boundary(o, j, dy, fy),
+                      face_or_boundary(o, j, dz, fz)};
+        },
+        [&](int, const Face& q) {
+          const float d = q.v - own;
+          bx += q.wx * d; by += q.wy * d; bz += q.wz * d;
+        });
     const Sym3 m(inverse, i, n);
     gx[i] = m.xx * bx + m.xy * by + m.xz * bz;
     gy[i] = m.xy * bx + m.yy * by + m.yz * bz;
@@ -454,13 +485,21 @@ __device__ __forceinline__ float limit_cell(int i, float gx, float gy, float gz,
                                             float down, CI offsets, CI faces, CF sign, CF rx,
                                             CF ry, CF rz, CF nx, CF ny, CF nz, float eps2) {
   float psi = 1.0f;
-  for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-    const int f = faces[j];
-    const bool own = sign[j] > 0.0f;
-    const float d = gx * (own ? rx[f] : nx[f]) + gy * (own ? ry[f] : ny[f]) + gz * (own ? rz[f] : nz[f]);
-    if (d > 0.0f) psi = kmin(psi, limit_face(d, up, eps2));
-    if (d < 0.0f) psi = kmin(psi, limit_face(d, down, eps2));
-  }
+  struct Face {
+    float x, y, z;
+  };
+  each_face(
+      offsets[i], offsets[i + 1],
+      [&](int j) {
+        const int f = faces[j];
+        const bool own = sign[j] > 0.0f;
+        return Face{*((own ? rx : nx) + f), *((own ? ry : ny) + f), *((own ? rz : nz) + f)};
+      },
+      [&](int, const Face& q) {
+        const float d = gx * q.x + gy * q.y + gz * q.z;
+        if (d > 0.0f) psi = kmin(psi, limit_face(d, up, eps2));
+        if (d < 0.0f) psi = kmin(psi, limit_face(d, down, eps2));
+      });
   return kmax(0.0f, kmin(1.0f, psi));
 }
 // :76-94
@@ -470,10 +509,11 @@ struct FvLimiter {
                              CF gz, WF limiter, float eps2) const {
     const float own = cell[i];
     float lo = own, hi = own;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const float v = across(adjacent[j], cell, boundary);
-      lo = kmin(lo, v); hi = kmax(hi, v);
-    }
+    each_face(
+        offsets[i], offsets[i + 1], [&](int j) { return across_load(adjacent[j], cell, boundary); },
+        [&](int, float v) {
+          lo = kmin(lo, v); hi = kmax(hi, v);
+        });
     limiter[i] = limit_cell(i, gx[i], gy[i], gz[i], hi - own, lo - own, offsets, faces, sign, rx,
                             ry, rz, nx, ny, nz, eps2);
   }
@@ -485,13 +525,19 @@ struct FvPairedLimiter {
                              CF gx, CF gy, CF gz, WF limiter, float eps2) const {
     const float own = cell[i], own_low = low[i];
     float lo = 0.0f, hi = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int o = adjacent[j];
-      const float v = across(o, cell, boundary);
-      const float l = o < 0 ? (kind[-o - 1] == 2 ? own_low : 0.0f) : low[o];
-      const float d = (v - own) + (l - own_low);
-      lo = kmin(lo, d); hi = kmax(hi, d);
-    }
+    struct Face {
+      float v, l;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int o = adjacent[j];
+          return Face{across_load(o, cell, boundary), o < 0 ? (kind[-o - 1] == 2 ? own_low : 0.0f) : low[o]};
+        },
+        [&](int, const Face& q) {
+          const float d = (q.v - own) + (q.l - own_low);
+          lo = kmin(lo, d); hi = kmax(hi, d);
+        });
     limiter[i] = limit_cell(i, gx[i], gy[i], gz[i], hi, lo, offsets, faces, sign, rx, ry, rz, nx,
                             ny, nz, eps2);
   }
@@ -522,16 +568,22 @@ struct FvVectorGradient {
                              int n) const {
     const float u0 = u[i], v0 = v[i], w0 = w[i];
     float m[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int o = adjacent[j];
-      const float du = across(o, u, bu) - u0, dv = across(o, v, bv) - v0, dw = across(o, w, bw) - w0;
-      const float rx = o < 0 ? fx[-o - 1] : dx[j];
-      const float ry = o < 0 ? fy[-o - 1] : dy[j];
-      const float rz = o < 0 ? fz[-o - 1] : dz[j];
-      m[0] += rx * du; m[1] += ry * du; m[2] += rz * du;
-      m[3] += rx * dv; m[4] += ry * dv; m[5] += rz * dv;
-      m[6] += rx * dw; m[7] += ry * dw; m[8] += rz * dw;
-    }
+    struct Face {
+      float u, v, w, rx, ry, rz;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int o = adjacent[j];
+          return Face{across_load(o, u, bu), across_load(o, v, bv), across_load(o, w, bw),
+                      face_or_boundary(o, j, dx, fx), face_or_boundary(o, j, dy, fy), face_or_boundary(o, j, dz, fz)};
+        },
+        [&](int, const Face& q) {
+          const float du = q.u - u0, dv = q.v - v0, dw = q.w - w0;
+          m[0] += q.rx * du; m[1] += q.ry * du; m[2] += q.rz * du;
+          m[3] += q.rx * dv; m[4] += q.ry * dv; m[5] += q.rz * dv;
+          m[6] += q.rx * dw; m[7] += q.ry * dw; m[8] += q.rz * dw;
+        });
     const Sym3 s(inverse, i, n);
     ux[i] = s.xx * m[0] + s.xy * m[1] + s.xz * m[2];
     uy[i] = s.xy * m[0] + s.yy * m[1] + s.yz * m[2];
@@ -599,17 +651,28 @@ struct FvJumpGradient {
                              int n) const {
     const float c0 = cell[i], g0x = gx[i], g0y = gy[i], g0z = gz[i];
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j], o = adjacent[j];
-      const bool side = sign[j] > 0.0f;
-      const float own = c0 + g0x * (side ? px[f] : nx[f]) + g0y * (side ? py[f] : ny[f]) +
-                        g0z * (side ? pz[f] : nz[f]);
-      const float other = o < 0 ? boundary[-o - 1]
-                                : cell[o] + gx[o] * (side ? nx[f] : px[f]) +
-                                      gy[o] * (side ? ny[f] : py[f]) + gz[o] * (side ? nz[f] : pz[f]);
-      const float jump = other - own;
-      bx += dx[j] * jump; by += dy[j] * jump; bz += dz[j] * jump;
-    }
+    // A boundary face reads the boundary value through the neighbour slot and
+    // this cell's gradient as a stand-in (unused) for the neighbour's.
+    struct Face {
+      int o;
+      float v, ogx, ogy, ogz, ax, ay, az, bx, by, bz, dx, dy, dz;
+    };
+    each_face<4>(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j], o = adjacent[j], oc = o < 0 ? i : o;
+          const bool side = sign[j] > 0.0f;
+          return Face{o, across_load(o, cell, boundary), gx[oc], gy[oc], gz[oc],
+                      *((side ? px : nx) + f), *((side ? py : ny) + f), *((side ? pz : nz) + f),
+                      *((side ? nx : px) + f), *((side ? ny : py) + f), *((side ? nz : pz) + f),
+                      dx[j], dy[j], dz[j]};
+        },
+        [&](int, const Face& q) {
+          const float own = c0 + g0x * q.ax + g0y * q.ay + g0z * q.az;
+          const float other = q.o < 0 ? q.v : q.v + q.ogx * q.bx + q.ogy * q.by + q.ogz * q.bz;
+          const float jump = other - own;
+          bx += q.dx * jump; by += q.dy * jump; bz += q.dz * jump;
+        });
     half_jump_update(i, n, inverse, gx, gy, gz, bx, by, bz, x, y, z);
   }
 };
@@ -637,11 +700,15 @@ struct FvGatherJumpGradient {
                              CF inverse, CF forward, CF reverse, CF gx, CF gy, CF gz, WF x, WF y,
                              WF z, int n) const {
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j];
-      const float jump = sign[j] > 0.0f ? forward[f] : reverse[f];
-      bx += dx[j] * jump; by += dy[j] * jump; bz += dz[j] * jump;
-    }
+    struct Face {
+      float jump, dx, dy, dz;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) { return Face{*((sign[j] > 0.0f ? forward : reverse) + faces[j]), dx[j], dy[j], dz[j]}; },
+        [&](int, const Face& q) {
+          bx += q.dx * q.jump; by += q.dy * q.jump; bz += q.dz * q.jump;
+        });
     half_jump_update(i, n, inverse, gx, gy, gz, bx, by, bz, x, y, z);
   }
 };
@@ -652,22 +719,35 @@ struct FvPairedJumpGradient {
                              CF cell, CF low, CF boundary, CF boundary_low, CF gx, CF gy, CF gz,
                              CF lx, CF ly, CF lz, WF x, WF y, WF z, int n) const {
     F2 bx{0.0f, 0.0f}, by{0.0f, 0.0f}, bz{0.0f, 0.0f};
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j], o = adjacent[j];
-      const bool side = sign[j] > 0.0f;
-      const F2 own = pair_trace(cell[i], low[i], gx[i], gy[i], gz[i], lx[i], ly[i], lz[i],
-                                side ? px[f] : nx[f], side ? py[f] : ny[f], side ? pz[f] : nz[f]);
-      F2 other{0.0f, 0.0f};
-      if (o < 0) other = {boundary[-o - 1], boundary_low[-o - 1]};
-      else
-        other = pair_trace(cell[o], low[o], gx[o], gy[o], gz[o], lx[o], ly[o], lz[o],
-                           side ? nx[f] : px[f], side ? ny[f] : py[f], side ? nz[f] : pz[f]);
-      F2 jump = two_sum(other.hi, -own.hi);
-      jump = product_add(jump, other.lo, 1.0f);
-      jump = product_add(jump, -own.lo, 1.0f);
-      bx = add_weighted(bx, dx[j], jump); by = add_weighted(by, dy[j], jump);
-      bz = add_weighted(bz, dz[j], jump);
-    }
+    const float c0 = cell[i], l0 = low[i], g0x = gx[i], g0y = gy[i], g0z = gz[i];
+    const float h0x = lx[i], h0y = ly[i], h0z = lz[i];
+    // As FvJumpGradient: a boundary face reads the boundary pair through the
+    // neighbour slots and this cell's gradients as unused stand-ins.
+    struct Face {
+      int o;
+      float v, vl, ogx, ogy, ogz, olx, oly, olz, ax, ay, az, bx, by, bz, dx, dy, dz;
+    };
+    each_face<4>(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j], o = adjacent[j], oc = o < 0 ? i : o;
+          const bool side = sign[j] > 0.0f;
+          return Face{o, across_load(o, cell, boundary), across_load(o, low, boundary_low), gx[oc], gy[oc], gz[oc],
+                      lx[oc], ly[oc], lz[oc], *((side ? px : nx) + f), *((side ? py : ny) + f),
+                      *((side ? pz : nz) + f), *((side ? nx : px) + f), *((side ? ny : py) + f),
+                      *((side ? nz : pz) + f), dx[j], dy[j], dz[j]};
+        },
+        [&](int, const Face& q) {
+          const F2 own = pair_trace(c0, l0, g0x, g0y, g0z, h0x, h0y, h0z, q.ax, q.ay, q.az);
+          F2 other{0.0f, 0.0f};
+          if (q.o < 0) other = {q.v, q.vl};
+          else other = pair_trace(q.v, q.vl, q.ogx, q.ogy, q.ogz, q.olx, q.oly, q.olz, q.bx, q.by, q.bz);
+          F2 jump = two_sum(other.hi, -own.hi);
+          jump = product_add(jump, other.lo, 1.0f);
+          jump = product_add(jump, -own.lo, 1.0f);
+          bx = add_weighted(bx, q.dx, jump); by = add_weighted(by, q.dy, jump);
+          bz = add_weighted(bz, q.dz, jump);
+        });
     const Sym3 m(inverse, i, n);
     const float xx = .5f * m.xx, xy = .5f * m.xy, xz = .5f * m.xz;
     const float yy = .5f * m.yy, yz = .5f * m.yz, zz = .5f * m.zz;
@@ -832,36 +912,61 @@ struct FvTransportAssemble {
     for (int j = first; j < last; ++j)
       if (adjacent[j] >= 0) row.clear(entry[j]);
     row.fill_untouched();
-    for (int j = first; j < last; ++j) {
-      const int f = faces[j], o = adjacent[j];
-      const float mass = sign[j] * flux[f], inflow = kmin(mass, 0.0f);
-      float d = mu[f] * diffusion[f];
-      ap = upper_add(ap, mass);
-      if (o >= 0) {
-        row.add(entry[j], -d + inflow);
-      } else {
-        const int bf = f - internal;
-        if (kind[bf] != 0) d = 0.0f;
-        ap = upper_add(ap, d);
-        ap = upper_add(ap, -inflow);
-        const float value = boundary[bf];
+    // Faces a chunk at a time: the chunk's face ids, neighbours, signs and
+    // entries are loaded first, then the gathered face values, then the
+    // faces are summed in order, the arithmetic of the face-by-face loop.
+    for (int j0 = first; j0 < last; j0 += kFaceChunk) {
+      int fc[kFaceChunk], oc[kFaceChunk], ec[kFaceChunk];
+      float sc[kFaceChunk], flc[kFaceChunk], mc[kFaceChunk], dc[kFaceChunk], cc[kFaceChunk];
+#pragma unroll
+      for (int k = 0; k < kFaceChunk; ++k)
+        if (j0 + k < last) {
+          fc[k] = faces[j0 + k];
+          oc[k] = adjacent[j0 + k];
+          sc[k] = sign[j0 + k];
+          ec[k] = entry[j0 + k];
+        }
+#pragma unroll
+      for (int k = 0; k < kFaceChunk; ++k)
+        if (j0 + k < last) {
+          flc[k] = flux[fc[k]];
+          mc[k] = mu[fc[k]];
+          dc[k] = diffusion[fc[k]];
+          cc[k] = correction[fc[k]];
+        }
+#pragma unroll
+      for (int k = 0; k < kFaceChunk; ++k) {
+        if (j0 + k >= last) break;
+        const int f = fc[k], o = oc[k];
+        const float mass = sc[k] * flc[k], inflow = kmin(mass, 0.0f);
+        float d = mc[k] * dc[k];
+        ap = upper_add(ap, mass);
+        if (o >= 0) {
+          row.add(ec[k], -d + inflow);
+        } else {
+          const int bf = f - internal;
+          if (kind[bf] != 0) d = 0.0f;
+          ap = upper_add(ap, d);
+          ap = upper_add(ap, -inflow);
+          const float value = boundary[bf];
+          if (compensated) {
+            const F2 factor = two_sum(d, -inflow);
+            const float product = factor.hi * value;
+            const F2 sum = two_sum(b, product);
+            carry += sum.lo + __builtin_fmaf(factor.hi, value, -product) + factor.lo * value;
+            b = sum.hi;
+          } else {
+            b += (d - inflow) * value;
+          }
+        }
+        const float deferred = sc[k] * cc[k];
         if (compensated) {
-          const F2 factor = two_sum(d, -inflow);
-          const float product = factor.hi * value;
-          const F2 sum = two_sum(b, product);
-          carry += sum.lo + __builtin_fmaf(factor.hi, value, -product) + factor.lo * value;
+          const F2 sum = two_sum(b, deferred);
+          carry += sum.lo;
           b = sum.hi;
         } else {
-          b += (d - inflow) * value;
+          b += deferred;
         }
-      }
-      const float deferred = sign[j] * correction[f];
-      if (compensated) {
-        const F2 sum = two_sum(b, deferred);
-        carry += sum.lo;
-        b = sum.hi;
-      } else {
-        b += deferred;
       }
     }
     row.each_off_diagonal([&](float v) { ap = upper_add(ap, -v); });
