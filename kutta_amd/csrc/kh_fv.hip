@@ -53,6 +53,48 @@ __global__ void __launch_bounds__(kBlock) k_fv_pressure_assemble(int n,
 using CI = const int* __restrict__;
 using CF = const float* __restrict__;
 using WF = float* __restrict__;
+
+// The same value read through one selected address (one load, no branch).
+__device__ __forceinline__ float across_load(int other, CF cell, CF boundary) {
+  return *(other < 0 ? boundary + (-other - 1) : cell + other);
+}
+// A face-indexed value or, for a boundary neighbour o < 0, boundary slot -o-1.
+__device__ __forceinline__ float face_or_boundary(int other, int j, CF per_face, CF per_boundary) {
+  return *(other < 0 ? per_boundary + (-other - 1) : per_face + j);
+}
+// Runs use(j, v) for the faces (or row entries) j of [first, last) in order,
+// where v = load(j) holds the values a face reads. The loads of up to K faces
+// are issued before any is used, so a cell pays about one memory round trip
+// per chunk instead of one per face; the arithmetic is the face-by-face
+// loop's. A full chunk loads without conditions (one straight run of loads;
+// six covers a hexahedron's faces), a partial one (a tetrahedron's four)
+// loads only its faces.
+constexpr int kFaceChunk = 8;  // the transport assembly's chunk
+template <int K = 6, bool Exact = true, class Load, class Use>
+__device__ __forceinline__ void each_face(int first, int last, Load load, Use use) {
+  for (int j0 = first; j0 < last; j0 += K) {
+    decltype(load(first)) v[K];
+    if (!Exact) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = load(min(j0 + k, last - 1));
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (j0 + k < last) use(j0 + k, v[k]);
+    } else if (last - j0 >= K) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = load(j0 + k);
+#pragma unroll
+      for (int k = 0; k < K; ++k) use(j0 + k, v[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (j0 + k < last) v[k] = load(j0 + k);
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (j0 + k < last) use(j0 + k, v[k]);
+    }
+  }
+}
 using WI = int* __restrict__;
 
 // kernels/fv_flow.kern.h:5-32. Boundary kinds: 0 wall, 1 velocity inlet,
@@ -131,11 +173,19 @@ struct FvMomentumSource {
                              CF face_pressure, CF volume, CF body_u, CF body_v, CF body_w, WF su,
                              WF sv, WF sw) const {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j];
-      const float pf = sign[j] * face_pressure[f];
-      fx += pf * sx[f]; fy += pf * sy[f]; fz += pf * sz[f];
-    }
+    struct Face {
+      float s, p, x, y, z;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j];
+          return Face{sign[j], face_pressure[f], sx[f], sy[f], sz[f]};
+        },
+        [&](int, const Face& q) {
+          const float pf = q.s * q.p;
+          fx += pf * q.x; fy += pf * q.y; fz += pf * q.z;
+        });
     const float vol = volume[i];
     su[i] = body_u[i] - fx / vol; sv[i] = body_v[i] - fy / vol; sw[i] = body_w[i] - fz / vol;
   }
@@ -379,31 +429,6 @@ __device__ __forceinline__ float pair_delta(F2 a, F2 b) {
 __device__ __forceinline__ float across(int other, CF cell, CF boundary) {
   return other < 0 ? boundary[-other - 1] : cell[other];
 }
-// The same value read through one selected address (one load, no branch).
-__device__ __forceinline__ float across_load(int other, CF cell, CF boundary) {
-  return *(other < 0 ? boundary + (-other - 1) : cell + other);
-}
-// A face-indexed value or, for a boundary neighbour o < 0, boundary slot -o-1.
-__device__ __forceinline__ float face_or_boundary(int other, int j, CF per_face, CF per_boundary) {
-  return *(other < 0 ? per_boundary + (-other - 1) : per_face + j);
-}
-// Runs use(j, v) for the faces j of [first, last) in order, where v = load(j)
-// holds the values a face reads. The loads of kFaceChunk faces are issued
-// before any is used (a chunk's slots past `last` load the last face again
-// and are not used), so a cell pays about one memory round trip per chunk
-// instead of one per face; the arithmetic is the face-by-face loop's.
-constexpr int kFaceChunk = 8;
-template <int K = 6, class Load, class Use>
-__device__ __forceinline__ void each_face(int first, int last, Load load, Use use) {
-  for (int j0 = first; j0 < last; j0 += K) {
-    decltype(load(first)) v[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = load(min(j0 + k, last - 1));
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (j0 + k < last) use(j0 + k, v[k]);
-  }
-}
 // The symmetric 3x3 least-squares inverse of cell i: planes xx xy xz yy yz zz.
 struct Sym3 {
   float xx, xy, xz, yy, yz, zz;
@@ -488,7 +513,7 @@ __device__ __forceinline__ float limit_cell(int i, float gx, float gy, float gz,
   struct Face {
     float x, y, z;
   };
-  each_face(
+  each_face<6, false>(
       offsets[i], offsets[i + 1],
       [&](int j) {
         const int f = faces[j];
@@ -509,7 +534,7 @@ struct FvLimiter {
                              CF gz, WF limiter, float eps2) const {
     const float own = cell[i];
     float lo = own, hi = own;
-    each_face(
+    each_face<6, false>(
         offsets[i], offsets[i + 1], [&](int j) { return across_load(adjacent[j], cell, boundary); },
         [&](int, float v) {
           lo = kmin(lo, v); hi = kmax(hi, v);
@@ -528,7 +553,7 @@ struct FvPairedLimiter {
     struct Face {
       float v, l;
     };
-    each_face(
+    each_face<6, false>(
         offsets[i], offsets[i + 1],
         [&](int j) {
           const int o = adjacent[j];
@@ -1246,13 +1271,21 @@ struct FvPairedMomentumResidual {
                              CF physical_low, CF hi, CF lo, CF pressure_gradient_lo, CF volume,
                              CF rhs, CF rhs_low, WF r) const {
     float sum = rhs[i], error = rhs_low[i];
-    for (int j = rows[i]; j < rows[i + 1]; ++j) {
-      const int col = cols[j];
-      const float a = values[j], x = hi[col], product = a * x;
-      const F2 s = two_sum(sum, -product);
-      sum = s.hi;
-      error += s.lo - __builtin_fmaf(a, x, -product) - a * lo[col];
-    }
+    struct Entry {
+      float a, x, l;
+    };
+    each_face(
+        rows[i], rows[i + 1],
+        [&](int j) {
+          const int col = cols[j];
+          return Entry{values[j], hi[col], lo[col]};
+        },
+        [&](int, const Entry& q) {
+          const float product = q.a * q.x;
+          const F2 s = two_sum(sum, -product);
+          sum = s.hi;
+          error += s.lo - __builtin_fmaf(q.a, q.x, -product) - q.a * q.l;
+        });
     const float pl = physical_low[i], li = lo[i];
     error += (values[diag[i]] - physical[i]) * li - volume[i] * pressure_gradient_lo[i];
     error -= pl * hi[i] + pl * li;
@@ -1266,12 +1299,21 @@ struct FvCompensatedMomentumResidual {
                              CF rhs, CF rhs_low, WF r) const {
     F2 value = two_sum(rhs[i], rhs_low[i]);
     const int di = diag[i];
-    for (int j = rows[i]; j < rows[i + 1]; ++j) {
-      const int col = cols[j];
-      value = product_add(value, -values[j], hi[col]);
-      const float low_coefficient = j == di ? physical[i] : values[j];
-      value = product_add(value, -low_coefficient, lo[col]);
-    }
+    const float own_physical = physical[i];
+    struct Entry {
+      float a, x, l;
+    };
+    each_face(
+        rows[i], rows[i + 1],
+        [&](int j) {
+          const int col = cols[j];
+          return Entry{values[j], hi[col], lo[col]};
+        },
+        [&](int j, const Entry& q) {
+          value = product_add(value, -q.a, q.x);
+          const float low_coefficient = j == di ? own_physical : q.a;
+          value = product_add(value, -low_coefficient, q.l);
+        });
     value = product_add(value, -physical_low[i], hi[i]);
     value = product_add(value, -physical_low[i], lo[i]);
     value = product_add(value, -volume[i], pressure_gradient_lo[i]);
@@ -1291,15 +1333,23 @@ struct FvCompensatedPressureForce {
                              CF pressure, CF low, WF x, WF y, WF z) const {
     float a = 0.0f, b = 0.0f, c = 0.0f;
     F2 ax{0.0f, 0.0f}, ay{0.0f, 0.0f}, az{0.0f, 0.0f};
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j];
-      const float s = sign[j], hp = s * pressure[f], lp = s * low[f];
-      const float fx = sx[f], fy = sy[f], fz = sz[f];
-      a += hp * fx; b += hp * fy; c += hp * fz;
-      ax = product_add(product_add(ax, hp, fx), lp, fx);
-      ay = product_add(product_add(ay, hp, fy), lp, fy);
-      az = product_add(product_add(az, hp, fz), lp, fz);
-    }
+    struct Face {
+      float s, p, l, x, y, z;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j];
+          return Face{sign[j], pressure[f], low[f], sx[f], sy[f], sz[f]};
+        },
+        [&](int, const Face& q) {
+          const float hp = q.s * q.p, lp = q.s * q.l;
+          const float fx = q.x, fy = q.y, fz = q.z;
+          a += hp * fx; b += hp * fy; c += hp * fz;
+          ax = product_add(product_add(ax, hp, fx), lp, fx);
+          ay = product_add(product_add(ay, hp, fy), lp, fy);
+          az = product_add(product_add(az, hp, fz), lp, fz);
+        });
     const float vol = volume[i];
     ax = product_add(ax, -a / vol, vol);
     ay = product_add(ay, -b / vol, vol);
@@ -1363,13 +1413,21 @@ struct FvPairedFaceResidual {
   __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF secondary, CF advection,
                              CF stress, CF boundary_rhs, WF residual, int turbulent) const {
     float sum = residual[i], error = boundary_rhs[i];
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j];
-      const float value = sign[j] * (secondary[f] - advection[f] + (turbulent ? stress[f] : 0.0f));
-      const F2 s = two_sum(sum, value);
-      sum = s.hi;
-      error += s.lo;
-    }
+    struct Face {
+      float s, a, b, c;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j];
+          return Face{sign[j], secondary[f], advection[f], turbulent ? stress[f] : 0.0f};
+        },
+        [&](int, const Face& q) {
+          const float value = q.s * (q.a - q.b + q.c);
+          const F2 t = two_sum(sum, value);
+          sum = t.hi;
+          error += t.lo;
+        });
     residual[i] = sum + error;
   }
 };
@@ -1431,13 +1489,20 @@ struct FvPairedContinuity {
   __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF flux, CF flux_lo,
                              WF imbalance, WF magnitude) const {
     float sum = 0.0f, error = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j];
-      const float s = sign[j];
-      const F2 t = two_sum(sum, s * flux[f]);
-      sum = t.hi;
-      error += t.lo + s * flux_lo[f];
-    }
+    struct Face {
+      float s, hi, lo;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j];
+          return Face{sign[j], flux[f], flux_lo[f]};
+        },
+        [&](int, const Face& q) {
+          const F2 t = two_sum(sum, q.s * q.hi);
+          sum = t.hi;
+          error += t.lo + q.s * q.lo;
+        });
     const float mass = sum + error;
     imbalance[i] = mass;
     magnitude[i] = fabsf(mass);
@@ -1574,11 +1639,18 @@ struct FvStressSource {
   __device__ void operator()(int i, CI offsets, CI faces, CF sign, CF volume, CF x, CF y, CF z,
                              WF su, WF sv, WF sw) const {
     float a = 0.0f, b = 0.0f, c = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int f = faces[j];
-      const float s = sign[j];
-      a += s * x[f]; b += s * y[f]; c += s * z[f];
-    }
+    struct Face {
+      float s, x, y, z;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int f = faces[j];
+          return Face{sign[j], x[f], y[f], z[f]};
+        },
+        [&](int, const Face& q) {
+          a += q.s * q.x; b += q.s * q.y; c += q.s * q.z;
+        });
     const float vol = volume[i];
     su[i] += a / vol; sv[i] += b / vol; sw[i] += c / vol;
   }
