@@ -647,6 +647,11 @@ def trace_summary(db: str, line_json: str | None, top: int) -> int:
     def short(name):
         n = name.replace("(anonymous namespace)::", "").replace("void ", "")
         n = re.sub(r"^kh::", "", n)
+        # The wavefront-per-row kernels share one template: name them by form.
+        if n.startswith("k_rowwave<"):
+            if "Prolonged" in n:
+                return "k_prolong_smooth_rowwave"
+            return "k_csr_residual_rowwave" if "RowWaveOut)1" in n else "k_amg_smooth_rowwave"
         m = re.match(r"k_each<(?:kh::)?(?:fv::)?([A-Za-z_0-9:]+)", n)
         return "fv " + m.group(1) if m else re.sub(r"[<(].*", "", n)
 
