@@ -138,6 +138,59 @@ def time_solver(dev, problem, options, kind, b, x, control, steps, warmup, graph
                         "direct_level": level, "hierarchy": solver.describe(), "plan": solver.plan()}
 
 
+def lazy_dense_sequence(dev, problem, options, b, x, control, direct, graphs, steps=12, drift=1e-3,
+                        tolerance=1e-2) -> dict:
+    """The mixed path over a drifting operator, as SIMPLEC refreshes it every
+    outer iteration: the operator's values scaled by 1 + drift*k at step k
+    (untimed upload), then refresh + solve timed, for the plain FP32 path, the
+    mixed path with the dense inverse recomputed at every refresh, and the
+    mixed path with the lazy refresh (recomputed once the level moved more
+    than `tolerance` from the values of its inverse). A uniform scaling is a
+    simple model of the slow drift of a converging SIMPLEC run."""
+    import numpy as np
+
+    import kutta_amd
+    from kutta_amd import SolverKind
+    m = kutta_amd.upload(dev, problem.host)
+    vals = m.val.download()
+    out = {"model": f"values x (1 + {drift:g} k), k = 1..{steps}; refresh + solve per step",
+           "tolerance": tolerance}
+    for label, dr, tol in (("plain", 0, 0.0), ("mixed_eager", direct, 0.0), ("mixed_lazy", direct, tolerance)):
+        m.val.upload(vals)
+        solver = kutta_amd.make_amg_solver(SolverKind.AmgPcg, dev, problem.host, options)
+        if dr:
+            solver.set_direct(dr)
+            solver.set_dense_tolerance(tol)
+        solver.set_graphs(graphs)
+        solver.refresh(m)
+        if not dr:
+            solver.autotune_tail(m, 5)
+        x.fill(np.float32(0))
+        solver.solve(m, x, b, control)
+        total, iterations = 0.0, []
+        for k in range(1, steps + 1):
+            m.val.upload((vals * np.float32(1.0 + drift * k)).astype(np.float32))
+            dev.synchronize()
+            t0 = time.perf_counter()
+            solver.refresh(m)
+            x.fill(np.float32(0))
+            st = solver.solve(m, x, b, control)
+            dev.synchronize()
+            total += time.perf_counter() - t0
+            iterations.append(st.iterations)
+        entry = {"refresh_plus_solve_ms": round(total * 1e3 / steps, 3), "iterations": iterations}
+        if dr:
+            inv, kept, _ = solver.dense_refreshes()
+            entry.update({"dense_inversions": inv, "dense_kept": kept})
+        out[label] = entry
+        del solver
+    out["lazy_vs_plain"] = round(out["plain"]["refresh_plus_solve_ms"] / out["mixed_lazy"]["refresh_plus_solve_ms"], 3)
+    out["lazy_vs_eager"] = round(out["mixed_eager"]["refresh_plus_solve_ms"] /
+                                 out["mixed_lazy"]["refresh_plus_solve_ms"], 3)
+    m.val.upload(vals)
+    return out
+
+
 def large_mesh_comparison(dev, size, steps, warmup, graphs, args) -> dict:
     """The plain FP32 AMG-PCG path (V(5,5), the reference's configuration) and
     the K-cycle variant (KCYCLE) on the size^3 box (7,986,000 cells at 110): ms to
@@ -724,7 +777,7 @@ def main() -> int:
                          "its FP64 dense inverse")
     ap.add_argument("--variants", default="all",
                     help="comma-separated solver variants to time beside the plain solve "
-                         "(pcg_mixed, pcg_mixed_bf16, fgmres_plain, fgmres_mixed, fgmres_learned, "
+                         "(pcg_mixed, pcg_mixed_bf16, pcg_mixed_lazy, fgmres_plain, fgmres_mixed, fgmres_learned, "
                          "fgmres_mixed_learned), "
                          "'all' or 'none'")
     ap.add_argument("--learned-train", type=int, default=None, metavar="STEPS",
@@ -942,6 +995,16 @@ def main() -> int:
                                                             np.abs(x_plain).max())
         variants[label] = out
         print(f"[bench] {label}: {out['ms']:.3f} ms, {vst.iterations} iterations", file=sys.stderr, flush=True)
+
+    # The mixed path's lazy dense refresh over a drifting operator.
+    if wanted is None or "pcg_mixed_lazy" in wanted:
+        variants["pcg_mixed_lazy"] = lazy_dense_sequence(dev, problem, options, b, x, control, dr,
+                                                         not args.no_graphs)
+        v = variants["pcg_mixed_lazy"]
+        print(f"[bench] pcg_mixed_lazy: {v['mixed_lazy']['refresh_plus_solve_ms']:.3f} ms refresh + solve "
+              f"({v['mixed_lazy']['dense_inversions']} inversions, {v['mixed_lazy']['dense_kept']} kept); "
+              f"eager {v['mixed_eager']['refresh_plus_solve_ms']:.3f}, plain {v['plain']['refresh_plus_solve_ms']:.3f}",
+              file=sys.stderr, flush=True)
 
     # The learned coarse correction inside FGMRES(30) (learned_variant):
     # harvest, targets and training are setup, reported beside the timed ms.

@@ -183,6 +183,14 @@ int ka_solver_set_coarse_bf16(void* solver, int on);
  * rows = 0 turns it off. *level receives the level (0: none). Symmetric
  * M-matrix operators without nullspace only. */
 int ka_solver_set_direct(void* solver, int rows, int32_t* level);
+/* Lazy refresh of that dense inverse (no reference counterpart): with
+ * tolerance > 0, ka_solver_refresh recomputes it only when the level's values
+ * have moved more than `tolerance` in relative l2 distance from the values it
+ * was computed from (an FP64 |a - s|/|s| in the refresh's existing host wait);
+ * 0 (default) recomputes at every refresh. */
+int ka_solver_set_dense_tolerance(void* solver, double tolerance);
+/* Inversions done, inverses kept, and the last relative distance. */
+int ka_solver_dense_refreshes(void* solver, int64_t* inversions, int64_t* skips, double* staleness);
 
 /* Steady laminar SIMPLEC (fv/steady_solver.hpp:286-360: SteadySolver
  * constructor, iterate, monitor, report, field access) on a box mesh

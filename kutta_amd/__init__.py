@@ -124,6 +124,9 @@ def host_lib() -> C.CDLL:
         lib.ka_solver_set_tail.argtypes = [vp, i32, pi32]
         lib.ka_solver_set_restart.argtypes = [vp, i32]
         lib.ka_solver_set_direct.argtypes = [vp, i32, pi32]
+        lib.ka_solver_set_dense_tolerance.argtypes = [vp, dbl]
+        lib.ka_solver_dense_refreshes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                                  C.POINTER(dbl)]
         lib.ka_solver_set_coarse_bf16.argtypes = [vp, i32]
         lib.ka_solver_set_graphs.argtypes = [vp, i32]
         lib.ka_solver_autotune_tail.argtypes = [vp, vp, i32, pi32]
@@ -753,6 +756,18 @@ class LinearSolver:
         level = C.c_int32()
         _check_host(host_lib().ka_solver_set_direct(self.handle, int(rows), C.byref(level)))
         return level.value
+
+    def set_dense_tolerance(self, tolerance: float) -> None:
+        """Lazy refresh of the dense level's inverse: recompute it only when
+        the level's values moved more than `tolerance` (relative l2) from
+        those it was computed from; 0 recomputes at every refresh."""
+        _check_host(host_lib().ka_solver_set_dense_tolerance(self.handle, float(tolerance)))
+
+    def dense_refreshes(self) -> tuple[int, int, float]:
+        """(inversions done, inverses kept, last relative distance)."""
+        i, k, d = C.c_int64(), C.c_int64(), C.c_double()
+        _check_host(host_lib().ka_solver_dense_refreshes(self.handle, C.byref(i), C.byref(k), C.byref(d)))
+        return i.value, k.value, d.value
 
     def set_learned(self, level: int = 2, layers: int = 3, seed: int = 0, capacity: int = 256) -> None:
         """Learned coarse correction stage (AMG FGMRES only): a network on

@@ -231,6 +231,19 @@ int ka_solver_set_direct(void* solver, int rows, int32_t* level) {
     if (level) *level = l;
   });
 }
+int ka_solver_set_dense_tolerance(void* solver, double tolerance) {
+  return guard([&] { static_cast<Solver*>(solver)->solver.set_dense_tolerance(tolerance); });
+}
+int ka_solver_dense_refreshes(void* solver, int64_t* inversions, int64_t* skips, double* staleness) {
+  return guard([&] {
+    long i = 0, k = 0;
+    double d = 0;
+    static_cast<Solver*>(solver)->solver.dense_refreshes(&i, &k, &d);
+    if (inversions) *inversions = i;
+    if (skips) *skips = k;
+    if (staleness) *staleness = d;
+  });
+}
 int ka_solver_set_learned(void* solver, int level, int layers, uint64_t seed, int capacity) {
   return guard([&] { static_cast<Solver*>(solver)->solver.set_learned(level, layers, seed, capacity); });
 }

@@ -180,10 +180,18 @@ struct KrylovSolver::Impl {
       for (auto& l : levels) l.val16 = DeviceArray<std::uint16_t>();
   }
   DeviceArray<double> dense_inverse, dense_scratch;
+  // Lazy dense refresh (no reference counterpart): with a tolerance > 0 a
+  // refresh keeps the inverse while the level's values stay within that
+  // relative l2 distance of the values it was computed from (`dense_basis`);
+  // 0 recomputes it at every refresh.
+  double dense_tolerance = 0.0, dense_staleness = 0.0;
+  DeviceArray<float> dense_basis;
+  long dense_inversions = 0, dense_skips = 0;
   int set_direct(Index rows) {
     drop_graphs();
     direct_level = 0;
     direct_ready = false;
+    dense_basis = DeviceArray<float>();
     dense_inverse = DeviceArray<double>();
     dense_scratch = DeviceArray<double>();
     if (rows <= 0) {
@@ -562,15 +570,41 @@ struct KrylovSolver::Impl {
                      {l.member_offsets, l.members, m.row_ptr, m.diag, m.val, c.row_ptr, c.diag, c.val});
       }
     }
+    // The dense level's distance from the values its inverse was computed
+    // from: |a|^2, |s|^2 and a.s in FP64, in the same host wait.
+    const bool lazy = direct_level > 0 && direct_ready && dense_tolerance > 0 && !dense_basis.empty();
+    if (lazy) {
+      const DeviceCsr c = levels[direct_level].matrix.view();
+      checks.push_back(HipDevice::dot(c.val, c.val, c.nnz));
+      checks.push_back(HipDevice::dot(dense_basis.data(), dense_basis.data(), c.nnz));
+      checks.push_back(HipDevice::dot(c.val, dense_basis.data(), c.nnz));
+    }
     const auto invalid = dev.reduce_many(checks);
     ++waits;
-    for (std::size_t i = 0; i < invalid.size(); ++i) {
+    for (std::size_t i = 0; i < allowed.size(); ++i) {
       KA_REQUIRE(invalid[i] != 3, "AMG: requires positive diagonals");
       KA_REQUIRE(invalid[i] <= allowed[i], "AMG: invalid values or zero diagonal");
     }
     if (direct_level > 0) {
       const DeviceCsr c = levels[direct_level].matrix.view();
-      dev.launch(KH_DENSE_INVERT, c.n_rows, {c.row_ptr, c.col, c.val, dense_inverse, dense_scratch});
+      bool invert = true;
+      if (lazy) {
+        const std::size_t k = allowed.size();
+        const double aa = invalid[k], ss = invalid[k + 1], as = invalid[k + 2];
+        dense_staleness = ss > 0 ? std::sqrt(std::max(0.0, aa + ss - 2 * as) / ss) : 1.0;
+        invert = !(dense_staleness <= dense_tolerance);
+      }
+      if (invert) {
+        dev.launch(KH_DENSE_INVERT, c.n_rows, {c.row_ptr, c.col, c.val, dense_inverse, dense_scratch});
+        if (dense_tolerance > 0) {
+          if (dense_basis.size() != std::size_t(c.nnz)) dense_basis = DeviceArray<float>(dev, c.nnz);
+          dev.copy(dense_basis.data(), c.val, sizeof(float) * std::size_t(c.nnz));
+        }
+        dense_staleness = 0.0;
+        ++dense_inversions;
+      } else {
+        ++dense_skips;
+      }
       direct_ready = true;
     }
     if (coarse_bf16)
@@ -1492,6 +1526,16 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
 }
 int KrylovSolver::host_waits() const { return impl_->waits; }
 int KrylovSolver::set_direct(Index rows) { return impl_->set_direct(rows); }
+void KrylovSolver::set_dense_tolerance(double tolerance) {
+  KA_REQUIRE(std::isfinite(tolerance) && tolerance >= 0, "AMG: the dense refresh tolerance must be finite and >= 0");
+  impl_->dense_tolerance = tolerance;
+  if (tolerance == 0) impl_->dense_basis = DeviceArray<float>();
+}
+void KrylovSolver::dense_refreshes(long* inversions, long* skips, double* staleness) const {
+  *inversions = impl_->dense_inversions;
+  *skips = impl_->dense_skips;
+  *staleness = impl_->dense_staleness;
+}
 void KrylovSolver::set_coarse_bf16(bool on) { impl_->set_coarse_bf16(on); }
 void KrylovSolver::set_learned(int level, int layers, std::uint64_t seed, int capacity) {
   impl_->set_learned(level, layers, seed, capacity);

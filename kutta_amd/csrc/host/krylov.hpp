@@ -108,6 +108,13 @@ class KrylovSolver {
   // solved by an FP64 dense inverse, computed at the next refresh(); 0 turns
   // it off. Returns that level (0: none). No reference counterpart.
   int set_direct(Index rows);
+  // Lazy dense refresh (no reference counterpart): a refresh recomputes the
+  // dense inverse only when the level's values have moved more than
+  // `tolerance` (relative l2) from those it was computed from; 0 (default)
+  // recomputes at every refresh. The counts of inversions and kept inverses
+  // and the last relative distance are reported by dense_refreshes().
+  void set_dense_tolerance(double tolerance);
+  void dense_refreshes(long* inversions, long* skips, double* staleness) const;
   // Mixed-precision AMG: coarse levels smoothed, restricted and prolonged
   // from bfloat16 copies of their values (FP32 arithmetic), made at the next
   // refresh(). No reference counterpart.

@@ -181,3 +181,56 @@ def test_blocked_gauss_jordan_inverse(device, nx, ny, nz):
     cond = np.linalg.cond(A)
     assert np.abs(got - ref).max() <= 1e-13 * cond * np.abs(ref).max()
     assert np.abs(A @ got - np.eye(n)).max() <= 1e-12 * cond
+
+
+def test_lazy_dense_refresh(device):
+    """Lazy refresh of the dense level's inverse: an unchanged operator keeps
+    it (the solve bit for bit as before); values 0.1% away keep it too and
+    the solve still converges, close to the eagerly refreshed solve; values
+    5% away recompute it, and the solve then equals the eager solver's bit for
+    bit (the same inverse of the same values)."""
+    p = kutta_amd.PressureProblem(device, 24, 23, 22)
+    b = device.upload(oracle.neumann_rhs(p.n_cells, reference=p.reference))
+    opts = AmgOptions(aggregation_passes=2, structure=OperatorStructure.SymmetricMMatrix)
+    control = SolveControl(1e-6, 0.0, 400)
+    m = kutta_amd.upload(device, p.host)
+    vals = m.val.download()
+    lazy = kutta_amd.make_amg_solver(SolverKind.AmgPcg, device, p.host, opts)
+    eager = kutta_amd.make_amg_solver(SolverKind.AmgPcg, device, p.host, opts)
+    assert lazy.set_direct(8192) == eager.set_direct(8192) > 0
+    with pytest.raises(kutta_amd.KuttaError):
+        lazy.set_dense_tolerance(-1.0)
+    lazy.set_dense_tolerance(1e-2)
+
+    def solve(solver):
+        x = device.alloc(p.n_cells)
+        x.fill(np.float32(0))
+        st = solver.solve(m, x, b, control)
+        assert st.converged
+        return st, x.download()
+
+    lazy.refresh(m)
+    assert lazy.dense_refreshes()[:2] == (1, 0)
+    st0, x0 = solve(lazy)
+    lazy.refresh(m)
+    inv, kept, dist = lazy.dense_refreshes()
+    assert (inv, kept) == (1, 1) and dist == 0.0
+    st1, x1 = solve(lazy)
+    assert st1.iterations == st0.iterations and np.array_equal(x1.view(np.uint32), x0.view(np.uint32))
+    m.val.upload((vals * np.float32(1.001)).astype(np.float32))
+    lazy.refresh(m)
+    inv, kept, dist = lazy.dense_refreshes()
+    assert (inv, kept) == (1, 2) and 5e-4 < dist < 2e-3
+    st2, x2 = solve(lazy)
+    eager.refresh(m)
+    ste, xe = solve(eager)
+    assert abs(st2.iterations - ste.iterations) <= 2
+    assert np.abs(x2.astype(np.float64) - xe).max() <= 1e-4 * np.abs(xe).max()
+    m.val.upload((vals * np.float32(1.05)).astype(np.float32))
+    lazy.refresh(m)
+    inv, kept, dist = lazy.dense_refreshes()
+    assert (inv, kept) == (2, 2) and dist == 0.0
+    st3, x3 = solve(lazy)
+    eager.refresh(m)
+    ste, xe = solve(eager)
+    assert st3.iterations == ste.iterations and np.array_equal(x3.view(np.uint32), xe.view(np.uint32))
