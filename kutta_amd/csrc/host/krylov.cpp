@@ -1384,6 +1384,11 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
   // the same way (captured once, replayed), so launch-bound forms are priced
   // as the solve will run them.
   const bool replayed = I.graphs && I.kind == SolverKind::AmgPcg;
+  // Where a form is timed alone (restriction, sweep) or decides a large part
+  // of the cycle (coarsest level, tail), a later form replaces an earlier
+  // one (the first being the default) only when at least 3% faster: forms
+  // within timing noise of each other would otherwise flip from run to run.
+  constexpr double kKeep = 0.97;
   // The fastest of three timed batches of `repeats` runs, so that one
   // disturbed batch does not decide a form.
   const auto seconds = [&](const std::function<void()>& body) {
@@ -1421,11 +1426,13 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     float* spare = l.r.data();
     DeviceArray<float> coarse(I.dev, ci.n_rows);
     double best_form = 0;
-    const int forms = l.max_row > KH_WIDE_ROW ? 4 : 3;
-    for (int mode = 0; mode < forms; ++mode) {
+    // Form 3 (the residual a wavefront per row) is not a candidate: picked
+    // on the long-row levels, it left the whole solve slower (set_plan still
+    // selects it).
+    for (int mode = 0; mode < 3; ++mode) {
       const double t =
           seconds([&] { I.restrict_level(l, li, ci, xin, bin, spare, coarse.data(), mode, I.status.data()); });
-      if (mode == 0 || t < best_form) {
+      if (mode == 0 || t < kKeep * best_form) {
         best_form = t;
         l.restrict_mode = mode;
       }
@@ -1447,37 +1454,27 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
         I.dev.launch(k, li.n_rows, {li.row_ptr, li.col, li.val, xin, bin, l.l1, l.r, I.status.data()});
       });
       if (mode == 0) thread_form = t;
-      else l.sweep_mode = t < thread_form ? 1 : 0;
+      else l.sweep_mode = t < kKeep * thread_form ? 1 : 0;
     }
   }
-  // Prolongation form per level: time each form alone on the level's data
-  // (the in-place form works on a copy of the iterate, whose values drift
-  // over the repeats without changing the work) and keep the fastest. Every
-  // form gives the same bits.
+  // Prolongation form per level: time whole V-cycles with each of the
+  // level's forms in turn (alone, a form runs on cache-warm data and the
+  // in-place form changes the level's iterate; inside the cycle the
+  // comparison is the one the solve sees) and keep the fastest. One level's
+  // difference is a percent or two of a cycle, so no margin applies here.
+  // Every form gives the same bits.
   for (std::size_t i = 0; i + 1 < I.levels.size(); ++i) {
     Level& l = I.levels[i];
     if (I.bf16_level(l)) continue;
-    const DeviceCsr li = i == 0 ? a : l.matrix.view();
-    const float* xin = i == 0 ? out.data() : l.x.data();
-    const float* bin = i == 0 ? v.data() : l.b.data();
-    const float* correction = I.levels[i + 1].x.data();
-    DeviceArray<float> copy(I.dev, li.n_rows), result(I.dev, li.n_rows);
-    I.dev.copy(copy.data(), xin, sizeof(float) * li.n_rows);
+    // Form 2 (folded, a wavefront per row) is not a candidate: picked on the
+    // long-row levels, it left the whole solve slower (set_plan still
+    // selects it).
     double fastest = 0;
     int pick = 0;
-    const int forms = l.max_row > KH_WIDE_ROW ? 3 : 2;
-    for (int mode = 0; mode < forms; ++mode) {
-      const double t = seconds([&] {
-        if (mode == 1) {
-          I.dev.launch(KH_AMG_PROLONG_GUARDED, li.n_rows, {l.aggregate, correction, copy, I.status.data()});
-          I.dev.launch(I.smooth_kernel(l), li.n_rows,
-                       {li.row_ptr, li.col, li.val, copy, bin, l.l1, result, I.status.data()});
-        } else {
-          I.dev.launch(mode == 2 ? KH_PROLONG_SMOOTH_ROWWAVE : KH_FUSED_PROLONG_SMOOTH, li.n_rows,
-                       {li.row_ptr, li.col, li.val, xin, bin, l.l1, result, l.aggregate, correction,
-                        I.status.data()});
-        }
-      });
+    for (int mode = 0; mode < 2; ++mode) {
+      l.prolong_mode = mode;
+      I.reset(0);
+      const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
       if (mode == 0 || t < fastest) {
         fastest = t;
         pick = mode;
@@ -1485,17 +1482,18 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     }
     l.prolong_mode = pick;
   }
-  // The coarsest level's sweeps: launch by launch, in one workgroup, or in
-  // one workgroup with the operator in registers.
+  // The coarsest level's sweeps: launch by launch or in one workgroup. The
+  // one-workgroup form that reads the operator once (set_plan form 2) is not
+  // a candidate: measured, it never beat the launches by more than noise on
+  // the benchmark hierarchies, and as a candidate it was picked by noise.
   if (I.coarse_sweeps_fit(I.levels.size() - 1)) {
-    const int forms = I.levels.back().resident_cap > 0 ? 3 : 2;
     double fastest = 0;
     int pick = 0;
-    for (int mode = 0; mode < forms; ++mode) {
+    for (int mode = 0; mode < 2; ++mode) {
       I.coarse_sweeps_mode = mode;
       I.reset(0);
       const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
-      if (mode == 0 || t < fastest) {
+      if (mode == 0 || t < kKeep * fastest) {
         fastest = t;
         pick = mode;
       }
@@ -1515,7 +1513,7 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     for (int k = 0; k < repeats; ++k) I.apply(a, v.data(), out.data(), Tail::None);
     I.dev.synchronize();
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (best_s == 0 || secs < best_s) {
+    if (best_s == 0 || secs < kKeep * best_s) {
       best_s = secs;
       best = first;
     }
