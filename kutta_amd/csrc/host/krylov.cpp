@@ -1391,17 +1391,24 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
   constexpr double kKeep = 0.97;
   // The fastest of three timed batches of `repeats` runs, so that one
   // disturbed batch does not decide a form.
+  // Under replay the `repeats` runs are captured into one graph, so each run
+  // is priced by its launches inside a graph (one replay's own overhead is
+  // shared by all of them).
   const auto seconds = [&](const std::function<void()>& body) {
     body();
     I.dev.synchronize();
     Graph graph;
-    if (replayed) graph = Graph(I.dev.capture(body));
+    if (replayed)
+      graph = Graph(I.dev.capture([&] {
+        for (int k = 0; k < repeats; ++k) body();
+      }));
     double best = 0;
     for (int trial = 0; trial < 3; ++trial) {
       const auto t0 = std::chrono::steady_clock::now();
-      for (int k = 0; k < repeats; ++k) {
-        if (replayed) I.dev.replay(graph.get());
-        else body();
+      if (replayed) {
+        I.dev.replay(graph.get());
+      } else {
+        for (int k = 0; k < repeats; ++k) body();
       }
       I.dev.synchronize();
       const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1507,12 +1514,7 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     I.tail_request = first;
     I.plan_tail();
     I.reset(0);
-    I.apply(a, v.data(), out.data(), Tail::None);
-    I.dev.synchronize();
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < repeats; ++k) I.apply(a, v.data(), out.data(), Tail::None);
-    I.dev.synchronize();
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double secs = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
     if (best_s == 0 || secs < kKeep * best_s) {
       best_s = secs;
       best = first;
