@@ -1433,10 +1433,8 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
     float* spare = l.r.data();
     DeviceArray<float> coarse(I.dev, ci.n_rows);
     double best_form = 0;
-    // Form 3 (the residual a wavefront per row) is not a candidate: picked
-    // on the long-row levels, it left the whole solve slower (set_plan still
-    // selects it).
-    for (int mode = 0; mode < 3; ++mode) {
+    const int forms = l.max_row > KH_WIDE_ROW ? 4 : 3;
+    for (int mode = 0; mode < forms; ++mode) {
       const double t =
           seconds([&] { I.restrict_level(l, li, ci, xin, bin, spare, coarse.data(), mode, I.status.data()); });
       if (mode == 0 || t < kKeep * best_form) {
@@ -1473,12 +1471,10 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
   for (std::size_t i = 0; i + 1 < I.levels.size(); ++i) {
     Level& l = I.levels[i];
     if (I.bf16_level(l)) continue;
-    // Form 2 (folded, a wavefront per row) is not a candidate: picked on the
-    // long-row levels, it left the whole solve slower (set_plan still
-    // selects it).
     double fastest = 0;
     int pick = 0;
-    for (int mode = 0; mode < 2; ++mode) {
+    const int forms = l.max_row > KH_WIDE_ROW ? 3 : 2;
+    for (int mode = 0; mode < forms; ++mode) {
       l.prolong_mode = mode;
       I.reset(0);
       const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
@@ -1496,7 +1492,8 @@ int KrylovSolver::autotune_tail(const DeviceCsr& a, int repeats) {
   if (I.coarse_sweeps_fit(I.levels.size() - 1)) {
     double fastest = 0;
     int pick = 0;
-    for (int mode = 0; mode < 2; ++mode) {
+    const int forms = I.levels.back().resident_cap > 0 ? 3 : 2;
+    for (int mode = 0; mode < forms; ++mode) {
       I.coarse_sweeps_mode = mode;
       I.reset(0);
       const double t = seconds([&] { I.apply(a, v.data(), out.data(), Tail::None); });
