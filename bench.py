@@ -95,11 +95,12 @@ def fp64_error(x, x64) -> dict:
 # cycle is the reference's AmgCycle::K): flexible AMG-PCG whose coarse levels
 # of at least 50,000 rows take the K-cycle's two inner Krylov corrections,
 # plain V-cycle recursion below, one pre- and one post-smoothing sweep, a
-# six-level hierarchy whose coarsest level gets 4 sweeps. Picked by timing the
-# 1M- and 8M-cell solves over sweeps 1-5, K depths 1-4, 2-32 coarsest sweeps
-# and 5-8 levels: the levels past the sixth barely coarsen, and a level below
-# ~50k rows is latency-bound, so a second visit costs more than it saves.
-KCYCLE = dict(pre_sweeps=1, post_sweeps=1, coarse_sweeps=4, max_levels=6, k_min_rows=50000)
+# five-level hierarchy whose coarsest level gets 4 sweeps. Picked by timing the
+# 1M- and 8M-cell solves over sweeps 1-5, K depths 1-4, 2-32 coarsest sweeps,
+# 5-8 levels and row bounds of 20k-120k: the levels past the fifth barely
+# coarsen and are latency-bound, and a level below ~50k rows costs more on a
+# second visit than it saves.
+KCYCLE = dict(pre_sweeps=1, post_sweeps=1, coarse_sweeps=4, max_levels=5, k_min_rows=50000)
 
 
 def time_solver(dev, problem, options, kind, b, x, control, steps, warmup, graphs, direct=0, bf16=False,
@@ -646,7 +647,7 @@ def simplec_step_bench(dev, nz: int, warmup: int, iterations: int, cpu_nz: int, 
         kms = (time.perf_counter() - t0) * 1e3 / max(iterations, 1)
         kreport = solver.report(monitor=True)
         out["pressure_kcycle"] = {
-            "options": f"K-cycle V(1,1), 4 coarsest sweeps, 6 levels, inner corrections on levels of "
+            "options": f"K-cycle V(1,1), 4 coarsest sweeps, 5 levels, inner corrections on levels of "
                        f"{KCYCLE['k_min_rows']} rows or more",
             "ms_per_iteration": round(kms, 3), "speedup": round(ms / kms, 3),
             "last_pressure_iterations": int(kreport["pressure_iterations"]),

@@ -117,7 +117,7 @@ def test_k_cycle_resolve_after_refresh_is_bitwise(device):
 
 def test_full_size_pressure_variant_bitwise(device):
     """The benchmark's K-cycle variant on the 998,250-row C2 operator:
-    V(1,1), 4 coarsest sweeps, 6 levels, inner corrections on the coarse
+    V(1,1), 4 coarsest sweeps, 5 levels, inner corrections on the coarse
     levels of at least 50,000 rows (levels 1 and 2): the same iteration count
     and x bit for bit as the oracle."""
     p = kutta_amd.PressureProblem(device, 55, 55, 55, reference=-2)
@@ -127,7 +127,7 @@ def test_full_size_pressure_variant_bitwise(device):
     b = b64.astype(np.float32)
     b[p.reference] = 0
     a = oracle.Csr.from_arrays(p["row_ptr"], p["col"], p["diag"], p["values"].astype(np.float64))
-    options = AmgOptions(pre_sweeps=1, post_sweeps=1, coarse_sweeps=4, max_levels=6, aggregation_passes=2,
+    options = AmgOptions(pre_sweeps=1, post_sweeps=1, coarse_sweeps=4, max_levels=5, aggregation_passes=2,
                          structure=OperatorStructure.SymmetricMMatrix, cycle=AmgCycle.K, k_min_rows=50000)
     st, x, sto, xo, solver = solve_both(device, a, b, SolverKind.AmgPcg, options, (1e-6, 0.0, 2000),
                                         autotune=True)
