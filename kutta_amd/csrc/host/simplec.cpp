@@ -829,6 +829,44 @@ SteadySolver::Impl::Impl(HipDevice& d, PolyMesh mesh, std::vector<PatchCondition
     }
     const auto count = tri.size() / 9;
     KA_REQUIRE(count < std::size_t(1) << 31, "FV: too many wall triangles");
+    // Triangles in Morton order of their centroids, so that each group of 32
+    // is spatially compact and its box prunes well (the patches' face order
+    // makes long strips). The nearest distance is a minimum over the same set
+    // of exact FP64 distances, so the order does not change it.
+    if (count > 1) {
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      std::vector<double> mid(3 * count);
+      for (std::size_t t = 0; t < count; ++t)
+        for (int c = 0; c < 3; ++c) {
+          mid[3 * t + c] = (tri[9 * t + c] + tri[9 * t + 3 + c] + tri[9 * t + 6 + c]) / 3;
+          lo[c] = std::min(lo[c], mid[3 * t + c]);
+          hi[c] = std::max(hi[c], mid[3 * t + c]);
+        }
+      const auto spread = [](std::uint64_t v) {  // 21 bits to every third bit
+        v &= 0x1fffffULL;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+      };
+      std::vector<std::pair<std::uint64_t, std::uint32_t>> order(count);
+      for (std::size_t t = 0; t < count; ++t) {
+        std::uint64_t key = 0;
+        for (int c = 0; c < 3; ++c) {
+          const double extent = hi[c] - lo[c];
+          const double u = extent > 0 ? (mid[3 * t + c] - lo[c]) / extent : 0.0;
+          key |= spread(std::uint64_t(std::min(u, 1.0) * 2097151.0)) << c;
+        }
+        order[t] = {key, std::uint32_t(t)};
+      }
+      std::sort(order.begin(), order.end());
+      std::vector<double> sorted(tri.size());
+      for (std::size_t t = 0; t < count; ++t)
+        std::copy_n(tri.begin() + 9 * std::size_t(order[t].second), 9, sorted.begin() + 9 * t);
+      tri.swap(sorted);
+    }
     // Bounding box of each group of 32 consecutive triangles (the kernel's
     // pruning unit; the reference's Bounds::extend over the vertices).
     constexpr std::size_t kGroup = 32;
