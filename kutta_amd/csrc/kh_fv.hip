@@ -14,38 +14,6 @@ namespace {
 
 __device__ __forceinline__ int cell_index() { return int(xcd_block() * kBlock + threadIdx.x); }
 
-// kernels/fv_flow.kern.h:164-183 — pressure-correction M-matrix row: clear the
-// row's neighbour entries, subtract each face coefficient from its (possibly
-// shared) entry, eliminate the reference cell's row and column, and build the
-// diagonal as the upward-rounded sum of the stored off-diagonals plus boundary
-// anchors; b = -sum sign * flux.
-__global__ void __launch_bounds__(kBlock) k_fv_pressure_assemble(int n,
-    const int* __restrict__ offsets, const int* __restrict__ faces,
-    const float* __restrict__ sign, const int* __restrict__ adjacent,
-    const int* __restrict__ entry, const int* __restrict__ rows, const int* __restrict__ diag,
-    const float* __restrict__ coefficient, const float* __restrict__ flux,
-    float* __restrict__ values, float* __restrict__ rhs, int reference) {
-  const int i = cell_index();
-  if (i >= n) return;
-  const int begin = offsets[i], end = offsets[i + 1];
-  for (int j = begin; j < end; ++j)
-    if (adjacent[j] >= 0) values[entry[j]] = 0.0f;
-  UpperSum ap{0.0f, 0.0f};
-  float b = 0.0f;
-  for (int j = begin; j < end; ++j) {
-    const int f = faces[j], nb = adjacent[j];
-    const float d = coefficient[f];
-    b -= sign[j] * flux[f];
-    if (nb < 0 || nb == reference) ap = upper_add(ap, d);
-    if (nb >= 0 && nb != reference && i != reference) values[entry[j]] -= d;
-  }
-  const int di = diag[i];
-  for (int j = rows[i]; j < rows[i + 1]; ++j)
-    if (j != di) ap = upper_add(ap, -values[j]);
-  values[di] = i == reference ? 1.0f : upper_value(ap);
-  rhs[i] = i == reference ? 0.0f : b;
-}
-
 // ------------------------------------------------- one lane per face or cell
 // Each body below is a functor whose operator()(i, ...) takes the reference
 // kernel's parameters in order; k_each runs it for i in [0, n) and the table
@@ -892,6 +860,64 @@ struct RegisterRow {
       if (s < len) values[begin + s] = slot[s];
   }
 };
+
+// kernels/fv_flow.kern.h:164-183 — pressure-correction M-matrix row: clear the
+// row's neighbour entries, subtract each face coefficient from its (possibly
+// shared) entry, eliminate the reference cell's row and column, and build the
+// diagonal as the upward-rounded sum of the stored off-diagonals plus boundary
+// anchors; b = -sum sign * flux. As the transport assembly, the row is held in
+// registers for rows of at most RegisterRow::kSlots entries (every entry sees
+// the same operations in the same order: a cleared entry is 0, and x - d is
+// x + (-d)) and the faces' values are loaded a chunk at a time.
+struct PressureAssembly {
+  template <class Row>
+  __device__ __forceinline__ static void assemble(Row& row, int i, int begin, int end, CI faces, CF sign,
+                                                  CI adjacent, CI entry, CF coefficient, CF flux,
+                                                  WF rhs, int reference) {
+    for (int j = begin; j < end; ++j)
+      if (adjacent[j] >= 0) row.clear(entry[j]);
+    row.fill_untouched();
+    UpperSum ap{0.0f, 0.0f};
+    float b = 0.0f;
+    struct Face {
+      int nb, e;
+      float s, d, m;
+    };
+    each_face(
+        begin, end,
+        [&](int j) {
+          const int f = faces[j];
+          return Face{adjacent[j], entry[j], sign[j], coefficient[f], flux[f]};
+        },
+        [&](int, const Face& q) {
+          b -= q.s * q.m;
+          if (q.nb < 0 || q.nb == reference) ap = upper_add(ap, q.d);
+          if (q.nb >= 0 && q.nb != reference && i != reference) row.add(q.e, -q.d);
+        });
+    row.each_off_diagonal([&](float v) { ap = upper_add(ap, -v); });
+    row.set_diagonal(i == reference ? 1.0f : upper_value(ap));
+    row.store();
+    rhs[i] = i == reference ? 0.0f : b;
+  }
+};
+__global__ void __launch_bounds__(kBlock) k_fv_pressure_assemble(int n,
+    const int* __restrict__ offsets, const int* __restrict__ faces,
+    const float* __restrict__ sign, const int* __restrict__ adjacent,
+    const int* __restrict__ entry, const int* __restrict__ rows, const int* __restrict__ diag,
+    const float* __restrict__ coefficient, const float* __restrict__ flux,
+    float* __restrict__ values, float* __restrict__ rhs, int reference) {
+  const int i = cell_index();
+  if (i >= n) return;
+  const int begin = offsets[i], end = offsets[i + 1];
+  const int r0 = rows[i], r1 = rows[i + 1], di = diag[i];
+  if (r1 - r0 <= RegisterRow::kSlots) {
+    RegisterRow row{values, r0, r1 - r0, di - r0};
+    PressureAssembly::assemble(row, i, begin, end, faces, sign, adjacent, entry, coefficient, flux, rhs, reference);
+  } else {
+    MemoryRow row{values, r0, r1, di};
+    PressureAssembly::assemble(row, i, begin, end, faces, sign, adjacent, entry, coefficient, flux, rhs, reference);
+  }
+}
 
 // :37-112. One transport-matrix row per lane, written into the row's own
 // entries only: off-diagonals -D + min(F, 0); the diagonal as the upward-
