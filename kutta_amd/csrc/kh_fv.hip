@@ -439,17 +439,22 @@ struct FvPairedGradient {
                              WF gx, WF gy, WF gz, WF lx, WF ly, WF lz, int n) const {
     const float own = cell[i], own_low = low[i];
     F2 bx{0.0f, 0.0f}, by{0.0f, 0.0f}, bz{0.0f, 0.0f};
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int o = adjacent[j];
-      const float hi = across(o, cell, boundary), lo = o < 0 ? boundary_low[-o - 1] : low[o];
-      F2 d = two_sum(hi, -own);
-      d = product_add(d, lo, 1.0f);
-      d = product_add(d, -own_low, 1.0f);
-      const float wx = o < 0 ? fx[-o - 1] : dx[j];
-      const float wy = o < 0 ? fy[-o - 1] : dy[j];
-      const float wz = o < 0 ? fz[-o - 1] : dz[j];
-      bx = add_weighted(bx, wx, d); by = add_weighted(by, wy, d); bz = add_weighted(bz, wz, d);
-    }
+    struct Face {
+      float hi, lo, wx, wy, wz;
+    };
+    each_face(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int o = adjacent[j];
+          return Face{across_load(o, cell, boundary), across_load(o, low, boundary_low),
+                      face_or_boundary(o, j, dx, fx), face_or_boundary(o, j, dy, fy), face_or_boundary(o, j, dz, fz)};
+        },
+        [&](int, const Face& q) {
+          F2 d = two_sum(q.hi, -own);
+          d = product_add(d, q.lo, 1.0f);
+          d = product_add(d, -own_low, 1.0f);
+          bx = add_weighted(bx, q.wx, d); by = add_weighted(by, q.wy, d); bz = add_weighted(bz, q.wz, d);
+        });
     const Sym3 m(inverse, i, n);
     const F2 zero{0.0f, 0.0f};
     const F2 x = add_weighted(add_weighted(add_weighted(zero, m.xx, bx), m.xy, by), m.xz, bz);
@@ -543,12 +548,16 @@ struct FvPairedGradientLimiter {
                              WF limiter, float eps2) const {
     const F2 own{cell[i], low[i]};
     float lo = 0.0f, hi = 0.0f;
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-      const int o = adjacent[j];
-      const F2 v{across(o, cell, boundary), o < 0 ? boundary_low[-o - 1] : low[o]};
-      const float d = pair_delta(v, own);
-      lo = kmin(lo, d); hi = kmax(hi, d);
-    }
+    each_face<6, false>(
+        offsets[i], offsets[i + 1],
+        [&](int j) {
+          const int o = adjacent[j];
+          return F2{across_load(o, cell, boundary), across_load(o, low, boundary_low)};
+        },
+        [&](int, const F2& v) {
+          const float d = pair_delta(v, own);
+          lo = kmin(lo, d); hi = kmax(hi, d);
+        });
     limiter[i] = limit_cell(i, gx[i] + lx[i], gy[i] + ly[i], gz[i] + lz[i], hi, lo, offsets,
                             faces, sign, rx, ry, rz, nx, ny, nz, eps2);
   }
